@@ -1,0 +1,241 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/qutree_oracle.cpp).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline load it, and only as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_ROOT, "oracle", "_build", "libqutree_oracle.so")
+_lib = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(_ROOT, "oracle")])
+    L = C.CDLL(_LIB_PATH)
+    L.orc_last_error.restype = C.c_char_p
+    L.orc_matvec.argtypes = [dp, dp, dp, C.c_long, C.c_long, C.c_long, C.c_int]
+    L.orc_tmatvec.argtypes = L.orc_matvec.argtypes
+    L.orc_ctmatvec.argtypes = L.orc_matvec.argtypes
+    L.orc_rhomat.argtypes = [dp, dp, dp, C.c_long, C.c_long, C.c_long]
+    L.orc_mode_product.argtypes = [dp, dp, dp, C.c_long, C.c_long, C.c_long, C.c_long, C.c_int]
+    L.orc_hole_contraction.argtypes = [dp, dp, dp, C.c_long, C.c_long, C.c_long, C.c_long, C.c_int]
+    L.orc_microbench_batch.argtypes = [C.c_long, lp, lp, lp, lp, dp, dp, dp, dp, dp]
+    L.orc_gram_schmidt.argtypes = [dp, C.c_long, C.c_long]
+    L.orc_heev.argtypes = [dp, C.c_long, dp, dp]
+    L.orc_reg_inverse.argtypes = [dp, C.c_long, C.c_double, dp]
+    L.orc_project_out.argtypes = [dp, dp, C.c_long, C.c_long, dp]
+    L.orc_balanced_tree_text.argtypes = [C.c_long] * 4 + [C.c_double] * 4 + [C.c_char_p, C.c_long]
+    L.orc_balanced_tree_text.restype = C.c_long
+    L.orc_model_create.argtypes = [C.c_char_p]
+    L.orc_model_create.restype = C.c_void_p
+    L.orc_model_destroy.argtypes = [C.c_void_p]
+    L.orc_model_nnodes.argtypes = [C.c_void_p]
+    L.orc_model_nnodes.restype = C.c_long
+    L.orc_model_psi_size.argtypes = [C.c_void_p]
+    L.orc_model_psi_size.restype = C.c_long
+    L.orc_model_node_info.argtypes = [C.c_void_p, C.c_long, lp, C.c_long]
+    L.orc_model_node_info.restype = C.c_long
+    L.orc_model_leaf.argtypes = [C.c_void_p, C.c_long, dp, dp, dp]
+    L.orc_model_set_hamiltonian.argtypes = [C.c_void_p, C.c_long, dp, lp, lp, lp, lp, dp]
+    L.orc_fill_random.argtypes = [C.c_void_p, C.c_ulong, C.c_int, dp]
+    L.orc_orthogonal.argtypes = [C.c_void_p, dp]
+    L.orc_sparse_tree.argtypes = [C.c_void_p, C.c_long, lp]
+    L.orc_sparse_tree.restype = C.c_long
+    L.orc_sparse_tree_modes.argtypes = [C.c_void_p, lp, C.c_long, C.c_int, lp]
+    L.orc_sparse_tree_modes.restype = C.c_long
+    L.orc_build.argtypes = [C.c_void_p, dp, C.c_double]
+    L.orc_get_matrix.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_long, dp]
+    L.orc_derivative.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double]
+    L.orc_apply_all.argtypes = [C.c_void_p, dp, dp]
+    L.orc_energy.argtypes = [C.c_void_p, dp]
+    L.orc_energy.restype = C.c_double
+    L.orc_dot_product_top.argtypes = [C.c_void_p, dp, dp, dp]
+    L.orc_cmf.argtypes = ([C.c_void_p, dp] + [C.c_double] * 8 + [C.c_long, C.c_long, dp, dp, dp, lp])
+    L.orc_cmf.restype = C.c_long
+    _lib = L
+    return L
+
+
+def cview(z: np.ndarray) -> np.ndarray:
+    """complex128 array -> interleaved float64 view (C-contiguous copy if needed)."""
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    return z.view(np.float64)
+
+
+def as_complex(x: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(x).view(np.complex128)
+
+
+def matvec(M, B, a, b, c, C=None, add=False):
+    """LA_lib.f matvec on Fortran-ordered flat arrays."""
+    out = np.zeros(a * b * c, np.complex128) if C is None else np.array(C, np.complex128)
+    o = cview(out)
+    lib().orc_matvec(o, cview(B.ravel(order="F") if B.ndim > 1 else B),
+                     cview(M.ravel(order="F") if M.ndim > 1 else M), a, b, c, int(add))
+    return as_complex(o)
+
+
+def rhomat(bra, ket, a, b, c):
+    out = np.zeros(a * a, np.complex128)
+    o = cview(out)
+    lib().orc_rhomat(cview(bra), cview(ket), o, a, b, c)
+    return as_complex(o)
+
+
+def balanced_tree_text(num_leaves, dim_leaves=2, dim_nodes=2, dim_inc=0, omega=1.0, r0=0.0,
+                       wfr0=1.0, wfomega=0.0) -> str:
+    """TreeFactory::balancedTree with the C++ defaults of TreeFactory.h:12-14."""
+    L = lib()
+    n = L.orc_balanced_tree_text(num_leaves, dim_leaves, dim_nodes, dim_inc, omega, r0, wfr0, wfomega,
+                                 None, 0)
+    buf = C.create_string_buffer(n)
+    L.orc_balanced_tree_text(num_leaves, dim_leaves, dim_nodes, dim_inc, omega, r0, wfr0, wfomega, buf, n)
+    return buf.value.decode()
+
+
+class OracleModel:
+    """Tree + Hamiltonian + state, FP64 CPU."""
+
+    def __init__(self, tree_text: str, H=None):
+        L = lib()
+        self.h = L.orc_model_create(tree_text.encode())
+        if not self.h:
+            raise RuntimeError(L.orc_last_error().decode())
+        self.nnodes = L.orc_model_nnodes(self.h)
+        self.psi_size = L.orc_model_psi_size(self.h)
+        self.nodes = [self._info(a) for a in range(self.nnodes)]
+        off = 0
+        for n in self.nodes:
+            n["off"] = off
+            off += int(np.prod(n["dims"]))
+        if H is not None:
+            self.set_hamiltonian(H)
+
+    def _info(self, a):
+        buf = np.zeros(64, np.int64)
+        k = lib().orc_model_node_info(self.h, a, buf, 64)
+        v = buf[:k]
+        order, nch = int(v[4]), int(v[5])
+        return dict(address=a, parent=int(v[0]), childIdx=int(v[1]), bottom=bool(v[2]), layer=int(v[3]),
+                    mode=int(v[6]), dims=[int(x) for x in v[7:7 + order]],
+                    children=[int(x) for x in v[7 + order:7 + order + nch]])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_model_destroy(self.h)
+            self.h = None
+
+    def set_hamiltonian(self, H):
+        c, toff, fm, fk, fp, pool = H.arrays()
+        r = lib().orc_model_set_hamiltonian(self.h, H.nterms, c, toff, fm, fk, fp, pool)
+        if r:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self.nterms = H.nterms
+
+    def leaf(self, mode):
+        n = next(n for n in self.nodes if n["bottom"] and n["mode"] == mode)
+        d = n["dims"][0]
+        x = np.zeros(d)
+        kin = np.zeros(d * d)
+        tr = np.zeros(d * d)
+        lib().orc_model_leaf(self.h, mode, x, kin, tr)
+        return x, kin.reshape(d, d, order="F"), tr.reshape(d, d, order="F")
+
+    def fill_random(self, seed, delta_lowest=True):
+        psi = np.zeros(2 * self.psi_size)
+        lib().orc_fill_random(self.h, seed, int(delta_lowest), psi)
+        return as_complex(psi).copy()
+
+    def orthogonal(self, psi):
+        p = cview(psi).copy()
+        lib().orc_orthogonal(self.h, p)
+        return as_complex(p).copy()
+
+    def node_tensor(self, psi, a):
+        n = self.nodes[a]
+        size = int(np.prod(n["dims"]))
+        return psi[n["off"]:n["off"] + size].reshape(n["dims"], order="F")
+
+    def sparse_tree(self, l):
+        buf = np.zeros(self.nnodes, np.int64)
+        k = lib().orc_sparse_tree(self.h, l, buf)
+        return buf[:k].tolist()
+
+    def sparse_tree_modes(self, modes, tail=True):
+        buf = np.zeros(self.nnodes, np.int64)
+        m = np.asarray(modes, np.int64)
+        k = lib().orc_sparse_tree_modes(self.h, m, len(m), int(tail), buf)
+        return buf[:k].tolist()
+
+    def build(self, psi, eps=1e-10):
+        if lib().orc_build(self.h, cview(psi), eps):
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def matrix(self, which, l, a):
+        n = self.nodes[a]["dims"][-1]
+        out = np.zeros(2 * n * n)
+        r = lib().orc_get_matrix(self.h, which, l, a, out)
+        if r:
+            return None
+        return as_complex(out).reshape(n, n, order="F").copy()
+
+    def rho(self, a):
+        return self.matrix(0, 0, a)
+
+    def rho_inv(self, a):
+        return self.matrix(1, 0, a)
+
+    def hmat(self, l, a):
+        return self.matrix(2, l, a)
+
+    def hole(self, l, a):
+        return self.matrix(3, l, a)
+
+    def derivative(self, psi, phase=1.0, eps=1e-10):
+        out = np.zeros(2 * self.psi_size)
+        if lib().orc_derivative(self.h, cview(psi), out, complex(phase).real, complex(phase).imag, eps):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return as_complex(out).copy()
+
+    def apply_all(self, psi):
+        out = np.zeros(2 * self.psi_size)
+        if lib().orc_apply_all(self.h, cview(psi), out):
+            raise RuntimeError("build first")
+        return as_complex(out).copy()
+
+    def energy(self, psi):
+        return lib().orc_energy(self.h, cview(psi))
+
+    def overlap(self, bra, ket):
+        out = np.zeros(2)
+        lib().orc_dot_product_top(self.h, cview(bra), cview(ket), out)
+        return complex(out[0], out[1])
+
+    def cmf(self, psi, t0, tend, dt, out, acc_cmf=1e-4, acc_bs=1e-5, phase=1.0, max_steps=10 ** 9,
+            max_out=100000):
+        p = cview(psi).copy()
+        times = np.zeros(max_out)
+        en = np.zeros(max_out)
+        ac = np.zeros(2 * max_out)
+        stats = np.zeros(4, np.int64)
+        n = lib().orc_cmf(self.h, p, t0, tend, dt, out, acc_cmf, acc_bs, complex(phase).real,
+                          complex(phase).imag, max_steps, max_out, times, en, ac, stats)
+        if n < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        n = min(n, max_out)
+        return dict(psi=as_complex(p).copy(), times=times[:n], energies=en[:n],
+                    autocorr=as_complex(ac)[:n].copy(), steps=int(stats[0]), rejected=int(stats[1]),
+                    builds=int(stats[2]), derivs=int(stats[3]))
