@@ -1,0 +1,186 @@
+/*
+ * qutree_amd.h -- C ABI of the MI355X (gfx950) tree-tensor contraction engine.
+ *
+ * Conventions
+ *  - Complex data is interleaved (re, im): complex128 as double[2], complex64
+ *    as float[2].  Suffix _z = complex128, _c = complex64.
+ *  - Tensors are column-major with the first index fastest; an order-f tensor
+ *    seen on mode k is the 3-index array (before, n_k, after) with
+ *    before = prod_{i<k} d_i and after = prod_{i>k} d_i
+ *    (src/Core/TensorShape.cpp:3-40 of QuTree).  Matrices are column-major,
+ *    (i, j) -> j * dim1 + i (include/Core/Matrix_Implementation.h:84-96).
+ *  - Node-level entry points take DEVICE pointers.  The caller owns every
+ *    buffer; nothing is allocated by a node-level call (grouped plans own
+ *    their descriptor copies and split-K workspace, allocated at creation).
+ *  - Every call returns a qtg_status; nothing calls exit().  qtg_last_error()
+ *    returns a description of the most recent failure on the calling thread.
+ *  - stream == NULL means the library's default stream of the current device.
+ */
+#ifndef QUTREE_AMD_H
+#define QUTREE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  QTG_OK = 0,
+  QTG_ESHAPE = 1,   /* dimensions inconsistent or unsupported */
+  QTG_EDEVICE = 2,  /* HIP runtime error or no device */
+  QTG_EARG = 3,     /* bad pointer / flag */
+  QTG_ENOMEM = 4,   /* device allocation failed */
+  QTG_ESTATE = 5    /* object not in the required state (e.g. not built) */
+} qtg_status;
+
+typedef void* qtg_stream;
+
+/* ---------------------------------------------------------------- runtime */
+int qtg_init(int device);
+const char* qtg_last_error(void);
+int qtg_device_alloc(void** ptr, size_t bytes);
+int qtg_device_free(void* ptr);
+int qtg_memcpy_h2d(void* dst, const void* src, size_t bytes, qtg_stream stream);
+int qtg_memcpy_d2h(void* dst, const void* src, size_t bytes, qtg_stream stream);
+int qtg_memset(void* dst, int value, size_t bytes, qtg_stream stream);
+int qtg_stream_create(qtg_stream* stream);
+int qtg_stream_destroy(qtg_stream stream);
+int qtg_synchronize(qtg_stream stream);
+/* Elapsed milliseconds between two points recorded on `stream` around a
+ * callback-free region: qtg_timer_start / qtg_timer_stop bracket the work. */
+int qtg_timer_start(qtg_stream stream);
+int qtg_timer_stop(qtg_stream stream, float* ms);
+
+/* ------------------------------------------------------------- node level */
+/* Mode-k product, replaces LA_lib.f matvec_ (src/Core/LA_lib.f:34-81) as
+ * bound in include/Core/Tensor_Implementation.h:450-461 and called from
+ * matrixTensor (Tensor_Implementation.h:554-577):
+ *   C(b,k,c) (+)= sum_l M(k,l) B(b,l,c),  M a x a,  B and C (b, a, c).
+ * add != 0 accumulates into C (the Fortran logical `add`). */
+int qtg_matvec_z(double* C, const double* B, const double* M, int64_t a, int64_t b, int64_t c,
+                 int add, qtg_stream stream);
+/* Transposed product, replaces LA_lib.f tmatvec_ (LA_lib.f:104-151):
+ *   C(b,k,c) (+)= sum_l M(l,k) B(b,l,c). */
+int qtg_tmatvec_z(double* C, const double* B, const double* M, int64_t a, int64_t b, int64_t c,
+                  int add, qtg_stream stream);
+/* Adjoint product, replaces LA_lib.f ctmatvec_ (LA_lib.f:244-291) behind
+ * tMatrixTensor (Tensor_Implementation.h:580-644):
+ *   C(b,k,c) (+)= sum_l conj(M(l,k)) B(b,l,c). */
+int qtg_ctmatvec_z(double* C, const double* B, const double* M, int64_t a, int64_t b, int64_t c,
+                   int add, qtg_stream stream);
+/* Hole contraction, replaces LA_lib.f rhomat_ (LA_lib.f:718-762) behind
+ * contraction() (Tensor_Implementation.h:482-531):
+ *   M(j,l) = sum_{n,m} conj(bra(n,j,m)) ket(n,l,m); always overwrites M. */
+int qtg_rhomat_z(const double* bra, const double* ket, double* M, int64_t a, int64_t b, int64_t c,
+                 qtg_stream stream);
+
+/* Rectangular mode product with matrixTensorBLAS semantics
+ * (src/Core/TensorBLAS.cpp:124-173,232-271): M is nC x nB, B (before,nB,after),
+ * C (before,nC,after).  op: 0 = M, 1 = M^T, 2 = M^H (M then nB x nC). */
+int qtg_mode_product_z(double* C, const double* M, const double* B, int64_t before, int64_t nB,
+                       int64_t nC, int64_t after, int op, int add, qtg_stream stream);
+int qtg_mode_product_c(float* C, const float* M, const float* B, int64_t before, int64_t nB,
+                       int64_t nC, int64_t after, int op, int add, qtg_stream stream);
+/* Hole contraction with contractionBLAS semantics (TensorBLAS.cpp:200-225,
+ * 279-326): S (nA x nB) = (accumulate ? S : 0) + sum conj(A(b,j,a)) B(b,l,a). */
+int qtg_hole_contraction_z(double* S, const double* A, const double* B, int64_t before, int64_t nA,
+                           int64_t nB, int64_t after, int accumulate, qtg_stream stream);
+int qtg_hole_contraction_c(float* S, const float* A, const float* B, int64_t before, int64_t nA,
+                           int64_t nB, int64_t after, int accumulate, qtg_stream stream);
+
+/* Grouped variants: `count` independent problems of uneven shape launched
+ * together.  A plan copies the descriptors to the device once and sizes the
+ * split-K workspace of the grouped hole contraction; executing a plan
+ * allocates nothing.  Offsets are in complex elements from the base pointers
+ * passed at execution.  No reference counterpart: these batch the per-term,
+ * per-node calls of TreeFunctions
+ * (include/TreeClasses/SparseMatrixTreeFunctions_Implementation.h:18-276). */
+typedef struct {
+  int64_t offC, offM, offB;       /* complex-element offsets */
+  int64_t before, nB, nC, after;  /* view of B on the contracted mode */
+  int32_t op, add;                /* op as in qtg_mode_product_z */
+} qtg_mode_product_desc;
+typedef struct {
+  int64_t offS, offA, offB;
+  int64_t before, nA, nB, after;
+  int32_t accumulate, pad;
+} qtg_hole_contraction_desc;
+typedef void* qtg_plan;
+int qtg_mode_product_plan_create(qtg_plan* plan, const qtg_mode_product_desc* descs, int64_t count,
+                                 int complex64);
+int qtg_hole_contraction_plan_create(qtg_plan* plan, const qtg_hole_contraction_desc* descs,
+                                     int64_t count, int complex64);
+/* Cbase/Sbase, Mbase/Abase, Bbase: double* for complex128 plans, float* for complex64. */
+int qtg_plan_execute(qtg_plan plan, void* out_base, const void* first_base, const void* second_base,
+                     qtg_stream stream);
+int qtg_plan_destroy(qtg_plan plan);
+/* Total algorithmic FLOPs (8 per complex multiply-add) and minimum HBM bytes
+ * (each operand read once, each result written once) of a plan. */
+int qtg_plan_counts(qtg_plan plan, double* flops, double* bytes);
+
+/* ------------------------------------------------------------- tree engine */
+/* A device-resident model of one tree and one sum-of-products Hamiltonian.
+ * The tree is QuTree's compact text (Tree(istream), src/TreeShape/Node.cpp:
+ * 101-165, followed by one "omega r0 wfr0 wfomega" line per mode; HO-DVR
+ * leaves).  The wave function is the concatenation of the node tensors in
+ * node-address order (post-order, last child first, Tree.cpp:92-115), complex128
+ * interleaved.  precision: 0 = complex128 throughout, 1 = complex64 bulk
+ * contractions with FP64 density inverse, projector and derivative output. */
+typedef void* qtg_engine;
+int qtg_engine_create(const char* tree_text, int precision, qtg_engine* engine);
+int qtg_engine_destroy(qtg_engine engine);
+/* SOP H = sum_l coeff[l] prod_f h_f (SumOfProductsOperator / MultiLeafOperator,
+ * src/TreeOperators/MultiLeafOperator.cpp:11-64).  Term l owns factors
+ * term_off[l] .. term_off[l+1]-1; factor f acts on leaf mode fmode[f] with kind
+ * fkind[f]: 0 x, 1 x^2, 2 kinetic, 3 p (DVRBasis.cpp:8-58), 4 identity,
+ * 5 explicit d x d complex matrix at pool + 2*fpool[f] (LeafMatrix.cpp:37-41),
+ * 6 LeafMatrix a0 T + a1 x + a2 x^2 + a3 x^4 with (a0..a3) at pool + 2*fpool[f]. */
+int qtg_engine_set_hamiltonian(qtg_engine engine, int64_t nterms, const double* coeff,
+                               const int64_t* term_off, const int64_t* fmode, const int64_t* fkind,
+                               const int64_t* fpool, const double* pool);
+int64_t qtg_engine_psi_size(qtg_engine engine);  /* complex elements */
+int64_t qtg_engine_num_nodes(qtg_engine engine);
+/* info: parent, childIdx, bottom, depth, order, nchildren, mode (-1), offset,
+ * dims[order], children[nchildren] */
+int qtg_engine_node_info(qtg_engine engine, int64_t addr, int64_t* info, int64_t len);
+/* TensorTree(mt19937(seed), tree, delta_lowest) (TensorTree_Implementation.h:39-76) */
+int qtg_engine_fill_random(qtg_engine engine, uint64_t seed, int delta_lowest, double* psi_host);
+int qtg_engine_set_psi(qtg_engine engine, const double* psi_host, qtg_stream stream);
+int qtg_engine_set_psi_device(qtg_engine engine, const double* psi_device, qtg_stream stream);
+/* HamiltonianRepresentation::build (include/TreeClasses/HamiltonianRepresentation.h:62-87):
+ * density tree, regularised inverse, h-matrices and hole matrices of all terms. */
+int qtg_engine_build(qtg_engine engine, qtg_stream stream);
+/* LayerDerivative (src/TreeClasses/HamiltonianRepresentation.cpp:45-72) at every
+ * node with the current representation. */
+int qtg_engine_layer_derivative_all(qtg_engine engine, double phase_re, double phase_im,
+                                    qtg_stream stream);
+/* Derivative (HamiltonianRepresentation.cpp:74-81) = build + LayerDerivative
+ * at every node: one H|Psi> sweep. */
+int qtg_engine_derivative(qtg_engine engine, double phase_re, double phase_im, qtg_stream stream);
+/* which: 0 = derivative, 1 = Apply() output (before projector), 2 = Psi */
+int qtg_engine_get_dpsi(qtg_engine engine, double* host, int which, qtg_stream stream);
+/* Expectation / S_top (HamiltonianRepresentation.cpp:45-51, CMFIntegrator.cpp:235-241)
+ * of the last derivative evaluation (orthonormal lower nodes). */
+int qtg_engine_energy(qtg_engine engine, double* energy, qtg_stream stream);
+/* which: 0 rho[node], 1 rho^-1[node], 2 h-matrix[term][node], 3 hole[term][node] */
+int qtg_engine_get_matrix(qtg_engine engine, int which, int64_t term, int64_t node, double* out,
+                          qtg_stream stream);
+/* FLOPs of one H|Psi> sweep (8 per complex multiply-add) and item counts
+ * (items[0] sandwich items, items[1] apply chunks, items[2] apply terms). */
+int qtg_engine_counts(qtg_engine engine, double* sweep_flops, int64_t* items);
+/* Host-only variants (no device is touched): plan statistics, state size and
+ * the random initial state of a tree. */
+int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* coeff,
+                         const int64_t* term_off, const int64_t* fmode, const int64_t* fkind,
+                         const int64_t* fpool, const double* pool, double* sweep_flops, int64_t* items);
+int64_t qtg_host_psi_size(const char* tree_text);
+int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest, double* psi_host);
+/* "rho_eps": regularisation of the density inverse (default 1e-10) */
+int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QUTREE_AMD_H */

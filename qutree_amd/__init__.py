@@ -1,0 +1,283 @@
+"""MI355X (gfx950) tree-tensor contraction engine -- Python binding.
+
+Thin ctypes layer over libqutree_amd.so (C ABI in include/qutree_amd.h).
+The shared library must have been built (``python -m qutree_amd.build`` or
+``__graft_entry__.build()``); there is no CPU fallback: every entry point
+raises if the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqutree_amd.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i64 = C.c_int64
+
+
+class QtgError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libqutree_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QtgError(f"{LIB_PATH} is missing: build it with python -m qutree_amd.build")
+    L = C.CDLL(LIB_PATH)
+    L.qtg_last_error.restype = C.c_char_p
+    L.qtg_init.argtypes = [C.c_int]
+    L.qtg_device_alloc.argtypes = [C.POINTER(_vp), C.c_size_t]
+    L.qtg_device_free.argtypes = [_vp]
+    L.qtg_memcpy_h2d.argtypes = [_vp, _vp, C.c_size_t, _vp]
+    L.qtg_memcpy_d2h.argtypes = [_vp, _vp, C.c_size_t, _vp]
+    L.qtg_memset.argtypes = [_vp, C.c_int, C.c_size_t, _vp]
+    L.qtg_synchronize.argtypes = [_vp]
+    L.qtg_timer_start.argtypes = [_vp]
+    L.qtg_timer_stop.argtypes = [_vp, C.POINTER(C.c_float)]
+    for name in ("qtg_matvec_z", "qtg_tmatvec_z", "qtg_ctmatvec_z"):
+        getattr(L, name).argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, C.c_int, _vp]
+    L.qtg_rhomat_z.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _vp]
+    for name in ("qtg_mode_product_z", "qtg_mode_product_c"):
+        getattr(L, name).argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _i64, C.c_int, C.c_int, _vp]
+    for name in ("qtg_hole_contraction_z", "qtg_hole_contraction_c"):
+        getattr(L, name).argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _i64, C.c_int, _vp]
+    L.qtg_mode_product_plan_create.argtypes = [C.POINTER(_vp), _vp, _i64, C.c_int]
+    L.qtg_hole_contraction_plan_create.argtypes = [C.POINTER(_vp), _vp, _i64, C.c_int]
+    L.qtg_plan_execute.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.qtg_plan_destroy.argtypes = [_vp]
+    L.qtg_plan_counts.argtypes = [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.qtg_engine_create.argtypes = [C.c_char_p, C.c_int, C.POINTER(_vp)]
+    L.qtg_engine_destroy.argtypes = [_vp]
+    L.qtg_engine_set_hamiltonian.argtypes = [_vp, _i64, _dp, _lp, _lp, _lp, _lp, _dp]
+    L.qtg_engine_psi_size.argtypes = [_vp]
+    L.qtg_engine_psi_size.restype = _i64
+    L.qtg_engine_num_nodes.argtypes = [_vp]
+    L.qtg_engine_num_nodes.restype = _i64
+    L.qtg_engine_node_info.argtypes = [_vp, _i64, _lp, _i64]
+    L.qtg_engine_fill_random.argtypes = [_vp, C.c_uint64, C.c_int, _dp]
+    L.qtg_engine_set_psi.argtypes = [_vp, _dp, _vp]
+    L.qtg_engine_set_psi_device.argtypes = [_vp, _vp, _vp]
+    L.qtg_engine_build.argtypes = [_vp, _vp]
+    L.qtg_engine_derivative.argtypes = [_vp, C.c_double, C.c_double, _vp]
+    L.qtg_engine_layer_derivative_all.argtypes = [_vp, C.c_double, C.c_double, _vp]
+    L.qtg_engine_get_dpsi.argtypes = [_vp, _dp, C.c_int, _vp]
+    L.qtg_engine_energy.argtypes = [_vp, C.POINTER(C.c_double), _vp]
+    L.qtg_engine_get_matrix.argtypes = [_vp, C.c_int, _i64, _i64, _dp, _vp]
+    L.qtg_engine_counts.argtypes = [_vp, C.POINTER(C.c_double), _lp]
+    L.qtg_engine_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise QtgError(f"qtg status {rc}: {lib().qtg_last_error().decode()}")
+
+
+_initialised = False
+
+
+def init(device: int = 0) -> None:
+    global _initialised
+    if not _initialised:
+        check(lib().qtg_init(device))
+        _initialised = True
+
+
+class DeviceBuffer:
+    """Device allocation owned by the library's allocator."""
+
+    def __init__(self, nbytes: int):
+        init()
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(lib().qtg_device_alloc(C.byref(p), max(self.nbytes, 16)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().qtg_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes, None))
+
+    def download(self, dtype, count: Optional[int] = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(n, dtype)
+        check(lib().qtg_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes, None))
+        return out
+
+    def zero(self) -> None:
+        check(lib().qtg_memset(self.ptr, 0, self.nbytes, None))
+        synchronize()
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().qtg_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    check(lib().qtg_synchronize(None))
+
+
+class Timer:
+    """HIP events recorded on the library's default stream."""
+
+    def start(self) -> None:
+        check(lib().qtg_timer_start(None))
+
+    def stop(self) -> float:
+        ms = C.c_float()
+        check(lib().qtg_timer_stop(None, C.byref(ms)))
+        return float(ms.value)
+
+
+MP_DESC = np.dtype([("offC", "i8"), ("offM", "i8"), ("offB", "i8"), ("before", "i8"), ("nB", "i8"),
+                    ("nC", "i8"), ("after", "i8"), ("op", "i4"), ("add", "i4")])
+HC_DESC = np.dtype([("offS", "i8"), ("offA", "i8"), ("offB", "i8"), ("before", "i8"), ("nA", "i8"),
+                    ("nB", "i8"), ("after", "i8"), ("accumulate", "i4"), ("pad", "i4")])
+
+
+class Plan:
+    """Grouped node-level contraction plan (descriptors resident on the device)."""
+
+    def __init__(self, kind: str, descs: np.ndarray, complex64: bool = False):
+        init()
+        descs = np.ascontiguousarray(descs)
+        p = _vp()
+        if kind == "mode_product":
+            assert descs.dtype == MP_DESC
+            check(lib().qtg_mode_product_plan_create(C.byref(p), descs.ctypes.data, len(descs), int(complex64)))
+        else:
+            assert descs.dtype == HC_DESC
+            check(lib().qtg_hole_contraction_plan_create(C.byref(p), descs.ctypes.data, len(descs),
+                                                          int(complex64)))
+        self.ptr = p.value
+
+    def execute(self, out: DeviceBuffer, first: DeviceBuffer, second: DeviceBuffer) -> None:
+        check(lib().qtg_plan_execute(self.ptr, out.ptr, first.ptr, second.ptr, None))
+
+    def counts(self):
+        f, b = C.c_double(), C.c_double()
+        check(lib().qtg_plan_counts(self.ptr, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qtg_plan_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def _ci(z: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(z, dtype=np.complex128).view(np.float64)
+
+
+class Engine:
+    """Device-resident tree + SOP Hamiltonian (qtg_engine_*)."""
+
+    def __init__(self, tree_text: str, H=None, precision: int = 0):
+        init()
+        p = _vp()
+        check(lib().qtg_engine_create(tree_text.encode(), precision, C.byref(p)))
+        self.ptr = p.value
+        self.precision = precision
+        self.psi_size = lib().qtg_engine_psi_size(self.ptr)
+        self.nnodes = lib().qtg_engine_num_nodes(self.ptr)
+        self.nodes = []
+        for a in range(self.nnodes):
+            buf = np.zeros(64, np.int64)
+            check(lib().qtg_engine_node_info(self.ptr, a, buf, 64))
+            order, nch = int(buf[4]), int(buf[5])
+            self.nodes.append(dict(parent=int(buf[0]), childIdx=int(buf[1]), bottom=bool(buf[2]),
+                                   depth=int(buf[3]), mode=int(buf[6]), off=int(buf[7]),
+                                   dims=[int(x) for x in buf[8:8 + order]],
+                                   children=[int(x) for x in buf[8 + order:8 + order + nch]]))
+        if H is not None:
+            self.set_hamiltonian(H)
+
+    def set_hamiltonian(self, H) -> None:
+        c, toff, fm, fk, fp, pool = H.arrays()
+        check(lib().qtg_engine_set_hamiltonian(self.ptr, H.nterms, c, toff, fm, fk, fp, pool))
+
+    def fill_random(self, seed: int, delta_lowest: bool = True) -> np.ndarray:
+        out = np.zeros(2 * self.psi_size)
+        check(lib().qtg_engine_fill_random(self.ptr, seed, int(delta_lowest), out))
+        return out.view(np.complex128).copy()
+
+    def set_psi(self, psi: np.ndarray) -> None:
+        check(lib().qtg_engine_set_psi(self.ptr, _ci(psi), None))
+
+    def build(self) -> None:
+        check(lib().qtg_engine_build(self.ptr, None))
+
+    def derivative(self, phase: complex = 1.0, sync: bool = True) -> None:
+        check(lib().qtg_engine_derivative(self.ptr, complex(phase).real, complex(phase).imag, None))
+        if sync:
+            synchronize()
+
+    def layer_derivative_all(self, phase: complex = 1.0) -> None:
+        check(lib().qtg_engine_layer_derivative_all(self.ptr, complex(phase).real, complex(phase).imag, None))
+
+    def get(self, which: int = 0) -> np.ndarray:
+        out = np.zeros(2 * self.psi_size)
+        check(lib().qtg_engine_get_dpsi(self.ptr, out, which, None))
+        return out.view(np.complex128).copy()
+
+    def energy(self) -> float:
+        e = C.c_double()
+        check(lib().qtg_engine_energy(self.ptr, C.byref(e), None))
+        return e.value
+
+    def matrix(self, which: int, term: int, node: int) -> np.ndarray:
+        n = self.nodes[node]["dims"][-1]
+        out = np.zeros(2 * n * n)
+        check(lib().qtg_engine_get_matrix(self.ptr, which, term, node, out, None))
+        return out.view(np.complex128).reshape(n, n, order="F").copy()
+
+    def counts(self):
+        f = C.c_double()
+        items = np.zeros(2, np.int64)
+        check(lib().qtg_engine_counts(self.ptr, C.byref(f), items))
+        return f.value, items
+
+    def set_option(self, key: str, value: float) -> None:
+        check(lib().qtg_engine_set_option(self.ptr, key.encode(), value))
+
+    def node_tensor(self, psi: np.ndarray, a: int) -> np.ndarray:
+        n = self.nodes[a]
+        size = int(np.prod(n["dims"]))
+        return psi[n["off"]:n["off"] + size].reshape(n["dims"], order="F")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qtg_engine_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass

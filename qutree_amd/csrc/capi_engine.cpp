@@ -1,0 +1,166 @@
+// C ABI of the tree engine (declared in include/qutree_amd.h).
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace qtg;
+
+extern "C" {
+
+int qtg_engine_create(const char* tree_text, int precision, qtg_engine* out) {
+  if (!tree_text || !out) {
+    set_error("engine_create: null argument");
+    return QTG_EARG;
+  }
+  auto* e = new Engine();
+  int rc = e->create(tree_text, precision);
+  if (rc) {
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return QTG_OK;
+}
+
+int qtg_engine_destroy(qtg_engine e) {
+  delete static_cast<Engine*>(e);
+  return QTG_OK;
+}
+
+int qtg_engine_set_hamiltonian(qtg_engine e, int64_t nterms, const double* coeff, const int64_t* term_off,
+                               const int64_t* fmode, const int64_t* fkind, const int64_t* fpool,
+                               const double* pool) {
+  if (!e || nterms < 0 || (nterms > 0 && (!coeff || !term_off || !fmode || !fkind || !fpool || !pool))) {
+    set_error("engine_set_hamiltonian: bad argument");
+    return QTG_EARG;
+  }
+  return static_cast<Engine*>(e)->set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool);
+}
+
+int64_t qtg_engine_psi_size(qtg_engine e) { return e ? static_cast<Engine*>(e)->tree().total : -1; }
+int64_t qtg_engine_num_nodes(qtg_engine e) {
+  return e ? (int64_t) static_cast<Engine*>(e)->tree().nodes.size() : -1;
+}
+
+int qtg_engine_node_info(qtg_engine e, int64_t addr, int64_t* info, int64_t len) {
+  if (!e || !info) return QTG_EARG;
+  const HTree& t = static_cast<Engine*>(e)->tree();
+  if (addr < 0 || addr >= (int64_t)t.nodes.size()) return QTG_EARG;
+  const HNode& n = t.nodes[addr];
+  std::vector<int64_t> v = {n.parent, n.childIdx, n.bottom ? 1 : 0, n.depth, n.order(), (int64_t)n.children.size(),
+                            n.bottom ? n.leaf.mode : -1, n.off};
+  for (int d : n.dims) v.push_back(d);
+  for (int c : n.children) v.push_back(c);
+  if (len < (int64_t)v.size()) return QTG_EARG;
+  std::copy(v.begin(), v.end(), info);
+  return QTG_OK;
+}
+
+int qtg_engine_fill_random(qtg_engine e, uint64_t seed, int delta_lowest, double* psi_host) {
+  if (!e || !psi_host) return QTG_EARG;
+  std::vector<cd> psi;
+  fill_random(static_cast<Engine*>(e)->tree(), seed, delta_lowest != 0, psi);
+  std::memcpy(psi_host, psi.data(), sizeof(cd) * psi.size());
+  return QTG_OK;
+}
+
+int qtg_engine_set_psi(qtg_engine e, const double* psi_host, qtg_stream s) {
+  if (!e || !psi_host) return QTG_EARG;
+  return static_cast<Engine*>(e)->set_psi(psi_host, S(s));
+}
+
+int qtg_engine_set_psi_device(qtg_engine e, const double* psi_dev, qtg_stream s) {
+  if (!e || !psi_dev) return QTG_EARG;
+  return static_cast<Engine*>(e)->set_psi_device(psi_dev, S(s));
+}
+
+int qtg_engine_build(qtg_engine e, qtg_stream s) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->build(S(s));
+}
+
+int qtg_engine_derivative(qtg_engine e, double phase_re, double phase_im, qtg_stream s) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->derivative(S(s), phase_re, phase_im);
+}
+
+int qtg_engine_layer_derivative_all(qtg_engine e, double phase_re, double phase_im, qtg_stream s) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->layer_derivative_all(S(s), phase_re, phase_im);
+}
+
+int qtg_engine_get_dpsi(qtg_engine e, double* host, int which, qtg_stream s) {
+  if (!e || !host) return QTG_EARG;
+  auto* en = static_cast<Engine*>(e);
+  const double* src = which == 0 ? en->dpsi64() : (which == 1 ? en->apply64() : en->psi64());
+  QTG_HIP_TRY(hipMemcpyAsync(host, src, sizeof(double) * 2 * en->tree().total, hipMemcpyDeviceToHost, S(s)));
+  QTG_HIP_TRY(hipStreamSynchronize(S(s)));
+  return QTG_OK;
+}
+
+int qtg_engine_energy(qtg_engine e, double* energy, qtg_stream s) {
+  if (!e || !energy) return QTG_EARG;
+  return static_cast<Engine*>(e)->energy(S(s), energy);
+}
+
+int qtg_engine_get_matrix(qtg_engine e, int which, int64_t term, int64_t node, double* out, qtg_stream s) {
+  if (!e || !out) return QTG_EARG;
+  return static_cast<Engine*>(e)->get_matrix(which, (int)term, (int)node, out, S(s));
+}
+
+int qtg_engine_counts(qtg_engine e, double* sweep_flops, int64_t* items) {
+  if (!e) return QTG_EARG;
+  auto* en = static_cast<Engine*>(e);
+  if (sweep_flops) *sweep_flops = en->sweep_flops();
+  if (items) {
+    items[0] = en->sandwich_items();
+    items[1] = en->apply_terms_chunks();
+    items[2] = en->apply_terms();
+  }
+  return QTG_OK;
+}
+
+// Host-only planning (no device is touched): counts of the work-item plan.
+int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* coeff, const int64_t* term_off,
+                         const int64_t* fmode, const int64_t* fkind, const int64_t* fpool, const double* pool,
+                         double* sweep_flops, int64_t* items) {
+  if (!tree_text) return QTG_EARG;
+  Engine en;
+  int rc = en.create(tree_text, 0, true);
+  if (rc) return rc;
+  rc = en.set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool);
+  if (rc) return rc;
+  return qtg_engine_counts(&en, sweep_flops, items);
+}
+
+int64_t qtg_host_psi_size(const char* tree_text) {
+  if (!tree_text) return -1;
+  Engine en;
+  if (en.create(tree_text, 0, true)) return -1;
+  return en.tree().total;
+}
+
+int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest, double* psi_host) {
+  if (!tree_text || !psi_host) return QTG_EARG;
+  Engine en;
+  int rc = en.create(tree_text, 0, true);
+  if (rc) return rc;
+  std::vector<cd> psi;
+  fill_random(en.tree(), seed, delta_lowest != 0, psi);
+  std::memcpy(psi_host, psi.data(), sizeof(cd) * psi.size());
+  return QTG_OK;
+}
+
+int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
+  if (!e || !key) return QTG_EARG;
+  auto* en = static_cast<Engine*>(e);
+  if (!std::strcmp(key, "rho_eps")) {
+    en->eps_ = value;
+    return QTG_OK;
+  }
+  set_error(std::string("unknown option ") + key);
+  return QTG_EARG;
+}
+
+}  // extern "C"

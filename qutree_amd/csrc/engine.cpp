@@ -1,0 +1,461 @@
+// Device-resident engine for one tree and one SOP Hamiltonian: builds the
+// work-item plans of HamiltonianRepresentation::build (density tree,
+// represent, hole contraction, regularised rho inverse) and of
+// LayerDerivative over all nodes, and launches them layer by layer.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace qtg {
+
+namespace {
+
+template <typename X>
+int up(X** dst, const std::vector<X>& v) {
+  *dst = nullptr;
+  if (v.empty()) return QTG_OK;
+  QTG_HIP_TRY(hipMalloc((void**)dst, sizeof(X) * v.size()));
+  QTG_HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
+  return QTG_OK;
+}
+
+__global__ void to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+}  // namespace
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+  for (void* p : device_buffers_)
+    if (p) (void)hipFree(p);
+  device_buffers_.clear();
+}
+
+int Engine::alloc(void** p, size_t bytes) {
+  if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) {
+    set_error("engine: device allocation failed");
+    return QTG_ENOMEM;
+  }
+  device_buffers_.push_back(*p);
+  return QTG_OK;
+}
+
+int Engine::create(const std::string& tree_text, int precision, bool host_only) {
+  host_only_ = host_only;
+  try {
+    tree_ = parse_tree(tree_text);
+  } catch (std::exception& e) {
+    set_error(std::string("tree: ") + e.what());
+    return QTG_ESHAPE;
+  }
+  if (precision != 0 && precision != 1) {
+    set_error("precision must be 0 (complex128) or 1 (complex64 bulk)");
+    return QTG_EARG;
+  }
+  c64_ = precision == 1;
+  nmax_ = 256;
+  nspf_ = 1;
+  for (auto& n : tree_.nodes) {
+    nmax_ = std::max(nmax_, n.N);
+    nspf_ = std::max(nspf_, n.n);
+  }
+  const size_t es = c64_ ? 8 : 16;
+  if (work_lds_bytes(nmax_, es) > kLdsBudget || finish_lds_bytes(nmax_, nspf_) > kLdsBudget || nspf_ > 64) {
+    set_error("engine: node tensors of " + std::to_string(nmax_) + " elements / " + std::to_string(nspf_) +
+              " SPFs exceed the LDS-resident kernels of this build");
+    return QTG_ESHAPE;
+  }
+  return QTG_OK;
+}
+
+int Engine::materialize() {
+  std::vector<DNode> dn;
+  for (auto& n : tree_.nodes) {
+    DNode d{};
+    d.order = n.order();
+    d.n = n.n;
+    d.N = n.N;
+    for (int i = 0; i < n.order(); ++i) d.dims[i] = n.dims[i];
+    d.off = n.off;
+    dn.push_back(d);
+  }
+  int rc = up(&d_nodes_, dn);
+  if (rc) return rc;
+  device_buffers_.push_back(d_nodes_);
+  if ((rc = alloc((void**)&d_psi64_, sizeof(double) * 2 * tree_.total))) return rc;
+  if ((rc = alloc((void**)&d_dpsi64_, sizeof(double) * 2 * tree_.total))) return rc;
+  if ((rc = alloc((void**)&d_apply64_, sizeof(double) * 2 * tree_.total))) return rc;
+  if (c64_ && (rc = alloc((void**)&d_psi32_, sizeof(float) * 2 * tree_.total))) return rc;
+  if ((rc = alloc((void**)&d_energy_, sizeof(double) * 4))) return rc;
+  return QTG_OK;
+}
+
+int Engine::set_hamiltonian(int64_t nterms, const double* coeff, const int64_t* term_off, const int64_t* fmode,
+                            const int64_t* fkind, const int64_t* fpool, const double* pool) {
+  std::string err;
+  if (build_sop(tree_, nterms, coeff, term_off, fmode, fkind, fpool, pool, sop_, err)) {
+    set_error("hamiltonian: " + err);
+    return QTG_EARG;
+  }
+  return plan();
+}
+
+// Active node sets: union of leaf-to-root paths of each term (SparseTree with tail).
+void Engine::active_sets() {
+  const int L = (int)sop_.terms.size();
+  active_.assign(L, std::vector<char>(tree_.nodes.size(), 0));
+  for (int l = 0; l < L; ++l)
+    for (auto& lm : sop_.terms[l].leafmats) {
+      int a = tree_.leafNode[lm.first];
+      while (a >= 0) {
+        active_[l][a] = 1;
+        a = tree_.nodes[a].parent;
+      }
+    }
+}
+
+int Engine::plan() {
+  active_sets();
+  const int L = (int)sop_.terms.size();
+  const int NN = (int)tree_.nodes.size();
+  // ---- pool layout (complex elements)
+  int64_t off = 0;
+  leafOff_.clear();
+  for (auto& m : sop_.leafMatrices) {
+    leafOff_.push_back(off);
+    off += (int64_t)m.size();
+  }
+  hslot_.assign(L, std::vector<int64_t>(NN, -1));
+  holeslot_.assign(L, std::vector<int64_t>(NN, -1));
+  for (int l = 0; l < L; ++l)
+    for (int a = 0; a < NN; ++a)
+      if (active_[l][a] && !tree_.nodes[a].top()) {
+        const int64_t nn = (int64_t)tree_.nodes[a].n * tree_.nodes[a].n;
+        hslot_[l][a] = off;
+        off += nn;
+        holeslot_[l][a] = off;
+        off += nn;
+      }
+  rhoslot_.assign(NN, -1);
+  for (int a = 0; a < NN; ++a) {
+    rhoslot_[a] = off;
+    off += (int64_t)tree_.nodes[a].n * tree_.nodes[a].n;
+  }
+  poolSize_ = off;
+  int64_t off64 = 0;
+  rhoinv_.assign(NN, -1);
+  for (int a = 0; a < NN; ++a) {
+    rhoinv_[a] = off64;
+    off64 += (int64_t)tree_.nodes[a].n * tree_.nodes[a].n;
+  }
+  pool64Size_ = off64;
+
+  auto leafmat = [&](int l, int mode) -> int64_t {
+    for (auto& lm : sop_.terms[l].leafmats)
+      if (lm.first == mode) return leafOff_[lm.second];
+    return -1;
+  };
+  flops_ = 0.0;
+  // ---- represent: bottom-up by height
+  std::vector<SwItem> items;
+  repRange_.assign(tree_.maxHeight, {0, 0});
+  for (int h = 0; h < tree_.maxHeight; ++h) {
+    repRange_[h].first = (int64_t)items.size();
+    for (int l = 0; l < L; ++l)
+      for (auto& v : tree_.nodes) {
+        if (v.height != h || v.top() || !active_[l][v.addr]) continue;
+        SwItem it{};
+        it.node = v.addr;
+        it.k = v.order() - 1;
+        it.bra = it.ket = v.off;
+        it.out = hslot_[l][v.addr];
+        if (v.bottom) {
+          it.fmode[it.nf] = 0;
+          it.fmat[it.nf++] = leafmat(l, v.leaf.mode);
+          flops_ += 8.0 * v.N * v.dims[0];
+        } else {
+          for (int c : v.children)
+            if (active_[l][c]) {
+              it.fmode[it.nf] = tree_.nodes[c].childIdx;
+              it.fmat[it.nf++] = hslot_[l][c];
+              flops_ += 8.0 * v.N * tree_.nodes[c].n;
+            }
+        }
+        flops_ += 8.0 * v.N * v.n;
+        items.push_back(it);
+      }
+    repRange_[h].second = (int64_t)items.size();
+  }
+  // ---- holes and density matrices: top-down by depth
+  holeRange_.assign(tree_.maxDepth + 1, {0, 0});
+  for (int dd = 1; dd <= tree_.maxDepth; ++dd) {
+    holeRange_[dd].first = (int64_t)items.size();
+    for (auto& v : tree_.nodes) {
+      if (v.depth != dd) continue;
+      const HNode& p = tree_.nodes[v.parent];
+      SwItem it{};
+      it.node = p.addr;
+      it.k = v.childIdx;
+      it.bra = it.ket = p.off;
+      it.out = rhoslot_[v.addr];
+      if (!p.top()) {
+        it.fmode[it.nf] = p.order() - 1;
+        it.fmat[it.nf++] = rhoslot_[p.addr];
+        flops_ += 8.0 * p.N * p.n;
+      }
+      flops_ += 8.0 * p.N * v.n;
+      items.push_back(it);
+    }
+    for (int l = 0; l < L; ++l)
+      for (auto& v : tree_.nodes) {
+        if (v.depth != dd || !active_[l][v.addr]) continue;
+        const HNode& p = tree_.nodes[v.parent];
+        SwItem it{};
+        it.node = p.addr;
+        it.k = v.childIdx;
+        it.bra = it.ket = p.off;
+        it.out = holeslot_[l][v.addr];
+        for (int s : p.children)
+          if (s != v.addr && active_[l][s]) {
+            it.fmode[it.nf] = tree_.nodes[s].childIdx;
+            it.fmat[it.nf++] = hslot_[l][s];
+            flops_ += 8.0 * p.N * tree_.nodes[s].n;
+          }
+        if (!p.top()) {
+          it.fmode[it.nf] = p.order() - 1;
+          it.fmat[it.nf++] = holeslot_[l][p.addr];
+          flops_ += 8.0 * p.N * p.n;
+        }
+        flops_ += 8.0 * p.N * v.n;
+        items.push_back(it);
+      }
+    holeRange_[dd].second = (int64_t)items.size();
+  }
+  nSw_ = (int64_t)items.size();
+  // ---- density eigen / inverse
+  std::vector<EigItem> eig;
+  for (auto& v : tree_.nodes) {
+    if (v.top()) continue;
+    if (v.n > 64) {
+      set_error("engine: more than 64 SPFs at a node is not supported");
+      return QTG_ESHAPE;
+    }
+    EigItem e{};
+    e.n = v.n;
+    e.in = rhoslot_[v.addr];
+    e.out = rhoinv_[v.addr];
+    e.ev = -1;
+    eig.push_back(e);
+  }
+  // ---- Apply at every node, in term chunks
+  std::vector<ApTerm> terms;
+  std::vector<ApChunk> chunks;
+  std::vector<FinNode> fin;
+  int64_t part = 0;
+  for (auto& v : tree_.nodes) {
+    FinNode fn{};
+    fn.node = v.addr;
+    fn.top = v.top() ? 1 : 0;
+    fn.part0 = part;
+    fn.rhoinv = rhoinv_[v.addr];
+    std::vector<ApTerm> vt;
+    for (int l = 0; l < L; ++l) {
+      if (!active_[l][v.addr]) continue;
+      ApTerm t{};
+      t.cre = sop_.terms[l].coeff.real();
+      t.cim = sop_.terms[l].coeff.imag();
+      if (v.bottom) {
+        t.fmode[t.nf] = 0;
+        t.fmat[t.nf++] = leafmat(l, v.leaf.mode);
+        flops_ += 8.0 * v.N * v.dims[0];
+      } else {
+        for (int c : v.children)
+          if (active_[l][c]) {
+            t.fmode[t.nf] = tree_.nodes[c].childIdx;
+            t.fmat[t.nf++] = hslot_[l][c];
+            flops_ += 8.0 * v.N * tree_.nodes[c].n;
+          }
+      }
+      if (!v.top()) {
+        t.fmode[t.nf] = v.order() - 1;
+        t.fmat[t.nf++] = holeslot_[l][v.addr];
+        flops_ += 8.0 * v.N * v.n;
+      }
+      vt.push_back(t);
+    }
+    if (!v.top()) flops_ += 4 * 8.0 * v.N * v.n;  // projector and rho^-1
+    const int per = chunkTerms_;
+    int nch = 0;
+    for (size_t t0 = 0; t0 < vt.size() || (t0 == 0 && vt.empty()); t0 += per) {
+      ApChunk c{};
+      c.node = v.addr;
+      c.t0 = (int32_t)(terms.size() + t0);
+      c.t1 = (int32_t)(terms.size() + std::min(vt.size(), t0 + per));
+      c.out = part;
+      part += v.N;
+      chunks.push_back(c);
+      ++nch;
+      if (vt.empty()) break;
+    }
+    terms.insert(terms.end(), vt.begin(), vt.end());
+    fn.nchunk = nch;
+    fin.push_back(fn);
+  }
+  nChunks_ = (int64_t)chunks.size();
+  nFin_ = (int64_t)fin.size();
+  nEig_ = (int64_t)eig.size();
+  partialSize_ = part;
+  h_items_ = items;
+  h_eig_ = eig;
+  h_terms_ = terms;
+  h_chunks_ = chunks;
+  h_fin_ = fin;
+  if (host_only_) {
+    planned_ = true;
+    return QTG_OK;
+  }
+  release();
+  int rc = materialize();
+  if (rc) return rc;
+  if ((rc = up(&d_sw_, items))) return rc;
+  device_buffers_.push_back(d_sw_);
+  if ((rc = up(&d_eig_, eig))) return rc;
+  device_buffers_.push_back(d_eig_);
+  if ((rc = up(&d_terms_, terms))) return rc;
+  device_buffers_.push_back(d_terms_);
+  if ((rc = up(&d_chunks_, chunks))) return rc;
+  device_buffers_.push_back(d_chunks_);
+  if ((rc = up(&d_fin_, fin))) return rc;
+  device_buffers_.push_back(d_fin_);
+  const size_t es = c64_ ? 8 : 16;
+  if ((rc = alloc(&d_pool_, es * poolSize_))) return rc;
+  if ((rc = alloc((void**)&d_pool64_, 16 * pool64Size_))) return rc;
+  if ((rc = alloc(&d_partial_, es * partialSize_))) return rc;
+  // leaf matrices (constant)
+  std::vector<double> lm;
+  for (auto& m : sop_.leafMatrices)
+    for (auto& z : m) {
+      lm.push_back(z.real());
+      lm.push_back(z.imag());
+    }
+  if (!lm.empty()) {
+    if (c64_) {
+      std::vector<float> lf(lm.begin(), lm.end());
+      QTG_HIP_TRY(hipMemcpy(d_pool_, lf.data(), sizeof(float) * lf.size(), hipMemcpyHostToDevice));
+    } else {
+      QTG_HIP_TRY(hipMemcpy(d_pool_, lm.data(), sizeof(double) * lm.size(), hipMemcpyHostToDevice));
+    }
+  }
+  planned_ = true;
+  return QTG_OK;
+}
+
+int Engine::set_psi(const double* host, hipStream_t s) {
+  QTG_HIP_TRY(hipMemcpyAsync(d_psi64_, host, sizeof(double) * 2 * tree_.total, hipMemcpyHostToDevice, s));
+  QTG_HIP_TRY(hipStreamSynchronize(s));
+  built_ = false;
+  return QTG_OK;
+}
+
+int Engine::set_psi_device(const double* dev, hipStream_t s) {
+  QTG_HIP_TRY(hipMemcpyAsync(d_psi64_, dev, sizeof(double) * 2 * tree_.total, hipMemcpyDeviceToDevice, s));
+  built_ = false;
+  return QTG_OK;
+}
+
+template <typename T>
+int Engine::build_t(hipStream_t s) {
+  const void* psi = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
+  if (c64_) {
+    const int64_t n = 2 * tree_.total;
+    hipLaunchKernelGGL(to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_psi64_, d_psi32_, n);
+    QTG_LAUNCH_CHECK();
+  }
+  int rc;
+  for (auto& r : repRange_)
+    if ((rc = launch_sandwich<T>(d_sw_ + r.first, r.second - r.first, d_nodes_, psi, psi, d_pool_, nmax_, s)))
+      return rc;
+  for (size_t dd = 1; dd < holeRange_.size(); ++dd) {
+    auto& r = holeRange_[dd];
+    if ((rc = launch_sandwich<T>(d_sw_ + r.first, r.second - r.first, d_nodes_, psi, psi, d_pool_, nmax_, s)))
+      return rc;
+  }
+  if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, s))) return rc;
+  built_ = true;
+  return QTG_OK;
+}
+
+int Engine::build(hipStream_t s) {
+  if (!planned_) {
+    set_error("engine: no Hamiltonian");
+    return QTG_ESTATE;
+  }
+  return c64_ ? build_t<float>(s) : build_t<double>(s);
+}
+
+template <typename T>
+int Engine::apply_t(hipStream_t s, double ph_re, double ph_im) {
+  const void* psi = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
+  int rc = launch_apply<T>(d_chunks_, nChunks_, d_terms_, d_nodes_, psi, d_pool_, d_partial_, nmax_, s);
+  if (rc) return rc;
+  return launch_finish<T>(d_fin_, nFin_, d_nodes_, d_psi64_, d_partial_, d_pool64_, d_dpsi64_, d_apply64_, ph_re,
+                          ph_im, d_energy_, nmax_, nspf_, s);
+}
+
+int Engine::layer_derivative_all(hipStream_t s, double ph_re, double ph_im) {
+  if (!built_) {
+    set_error("engine: build() first");
+    return QTG_ESTATE;
+  }
+  return c64_ ? apply_t<float>(s, ph_re, ph_im) : apply_t<double>(s, ph_re, ph_im);
+}
+
+int Engine::derivative(hipStream_t s, double ph_re, double ph_im) {
+  int rc = build(s);
+  if (rc) return rc;
+  return layer_derivative_all(s, ph_re, ph_im);
+}
+
+int Engine::energy(hipStream_t s, double* out) {
+  double e[4];
+  QTG_HIP_TRY(hipMemcpyAsync(e, d_energy_, sizeof(e), hipMemcpyDeviceToHost, s));
+  QTG_HIP_TRY(hipStreamSynchronize(s));
+  // <A_top|Apply(A_top)> / <A_top|A_top>
+  const cd num(e[0], e[1]), den(e[2], e[3]);
+  *out = (num / den).real();
+  return QTG_OK;
+}
+
+int Engine::get_matrix(int which, int term, int node, double* out, hipStream_t s) {
+  if (node < 0 || node >= (int)tree_.nodes.size()) return QTG_EARG;
+  const int n = tree_.nodes[node].n;
+  int64_t off = -1;
+  bool from64 = false;
+  if (which == 0) off = rhoslot_[node];
+  else if (which == 1) {
+    off = rhoinv_[node];
+    from64 = true;
+  } else if (term >= 0 && term < (int)hslot_.size()) {
+    off = (which == 2) ? hslot_[term][node] : holeslot_[term][node];
+  }
+  if (off < 0) return QTG_EARG;
+  if (from64 || !c64_) {
+    const void* base = from64 ? (const void*)d_pool64_ : d_pool_;
+    QTG_HIP_TRY(hipMemcpyAsync(out, (const char*)base + 16 * off, 16 * (size_t)n * n, hipMemcpyDeviceToHost, s));
+    QTG_HIP_TRY(hipStreamSynchronize(s));
+  } else {
+    std::vector<float> tmp(2 * (size_t)n * n);
+    QTG_HIP_TRY(hipMemcpyAsync(tmp.data(), (const char*)d_pool_ + 8 * off, 8 * (size_t)n * n,
+                               hipMemcpyDeviceToHost, s));
+    QTG_HIP_TRY(hipStreamSynchronize(s));
+    for (size_t i = 0; i < tmp.size(); ++i) out[i] = tmp[i];
+  }
+  return QTG_OK;
+}
+
+}  // namespace qtg

@@ -1,0 +1,88 @@
+// Device-resident tree engine (see engine.cpp).
+#pragma once
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+#include "model.hpp"
+#include "tree_ops.hpp"
+
+namespace qtg {
+
+class Engine {
+ public:
+  ~Engine();
+  int create(const std::string& tree_text, int precision, bool host_only = false);
+  int set_hamiltonian(int64_t nterms, const double* coeff, const int64_t* term_off, const int64_t* fmode,
+                      const int64_t* fkind, const int64_t* fpool, const double* pool);
+  int set_psi(const double* host, hipStream_t s);
+  int set_psi_device(const double* dev, hipStream_t s);
+  // HamiltonianRepresentation::build: rho tree, rho^-1, h-matrices, holes
+  int build(hipStream_t s);
+  // LayerDerivative at every node with the current representation
+  int layer_derivative_all(hipStream_t s, double ph_re, double ph_im);
+  // Derivative (build + LayerDerivative everywhere)
+  int derivative(hipStream_t s, double ph_re, double ph_im);
+  int energy(hipStream_t s, double* out);
+  int get_matrix(int which, int term, int node, double* out, hipStream_t s);
+
+  const HTree& tree() const { return tree_; }
+  const HSop& sop() const { return sop_; }
+  double sweep_flops() const { return flops_; }
+  double* psi64() { return d_psi64_; }
+  double* dpsi64() { return d_dpsi64_; }
+  double* apply64() { return d_apply64_; }
+  bool complex64() const { return c64_; }
+  int64_t sandwich_items() const { return nSw_; }
+  int64_t apply_terms_chunks() const { return nChunks_; }
+  int64_t apply_terms() const { return (int64_t)h_terms_.size(); }
+  double eps_ = 1e-10;
+  int chunkTerms_ = 32;
+
+ private:
+  template <typename T>
+  int build_t(hipStream_t s);
+  template <typename T>
+  int apply_t(hipStream_t s, double ph_re, double ph_im);
+  int plan();
+  int materialize();
+  void active_sets();
+  int alloc(void** p, size_t bytes);
+  void release();
+
+  HTree tree_;
+  HSop sop_;
+  bool c64_ = false, planned_ = false, built_ = false, host_only_ = false;
+  std::vector<SwItem> h_items_;
+  std::vector<EigItem> h_eig_;
+  std::vector<ApTerm> h_terms_;
+  std::vector<ApChunk> h_chunks_;
+  std::vector<FinNode> h_fin_;
+  int nmax_ = 0, nspf_ = 1;
+  std::vector<std::vector<char>> active_;
+  std::vector<int64_t> leafOff_, rhoslot_, rhoinv_;
+  std::vector<std::vector<int64_t>> hslot_, holeslot_;
+  std::vector<std::pair<int64_t, int64_t>> repRange_, holeRange_;
+  int64_t poolSize_ = 0, pool64Size_ = 0, partialSize_ = 0;
+  int64_t nSw_ = 0, nChunks_ = 0, nFin_ = 0, nEig_ = 0;
+  double flops_ = 0.0;
+
+  DNode* d_nodes_ = nullptr;
+  SwItem* d_sw_ = nullptr;
+  EigItem* d_eig_ = nullptr;
+  ApTerm* d_terms_ = nullptr;
+  ApChunk* d_chunks_ = nullptr;
+  FinNode* d_fin_ = nullptr;
+  double* d_psi64_ = nullptr;
+  double* d_dpsi64_ = nullptr;
+  double* d_apply64_ = nullptr;
+  float* d_psi32_ = nullptr;
+  void* d_pool_ = nullptr;
+  double* d_pool64_ = nullptr;
+  void* d_partial_ = nullptr;
+  double* d_energy_ = nullptr;
+  std::vector<void*> device_buffers_;
+};
+
+}  // namespace qtg

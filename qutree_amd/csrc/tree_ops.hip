@@ -1,0 +1,588 @@
+// Grouped tree kernels of the gfx950 engine.  One launch processes every
+// (term, node) work item of one tree layer; items have uneven shapes and are
+// described by small device records.
+//
+//  sandwich_kernel   out(j,l) = sum_{all indices but k} conj(Bra(..j..)) [F Ket](..l..)
+//                    with F a product of <= 3 mode factors on modes != k.  This
+//                    one form covers representLayer (k = last, F = children's
+//                    h-matrices or the leaf operator), contractionLayer (k =
+//                    child index, F = siblings' h-matrices and the parent's
+//                    hole), the density tree (F = rho of the parent) and the
+//                    overlap tree (F = children's overlaps).
+//  apply_kernel      partial(v, chunk) = sum_{terms in chunk} c_t F_t A_v: the
+//                    node-local SOP application of Apply(); factors are the
+//                    children's h-matrices (or the leaf operator) and the
+//                    node's hole matrix on the last mode.
+//  finish_kernel     FP64: sums the partials of a node, applies the (1-P)
+//                    projector with the overlap inverse, rho^-1 and -i*phase
+//                    (LayerDerivative), and forms <A_top|Apply(A_top)> for the
+//                    energy.
+//  heev_reginv_kernel FP64 batched parallel-order (tournament) Jacobi
+//                    eigensolver of the density matrices and the regularised
+//                    inverse U diag(1/(l + eps e^{-l/eps})) U^H.
+//
+// Node tensors live in one arena (post-order by node address); matrices live
+// in one pool; all offsets are in complex elements.  Work buffers are staged in
+// LDS: the node tensor, and up to two intermediate products.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "tree_ops.hpp"
+
+namespace qtg {
+
+constexpr int TB = 256;
+
+__device__ inline void node_view(const DNode& nd, int k, int& bef, int& nk, int& aft) {
+  bef = 1;
+  aft = 1;
+  for (int i = 0; i < k; ++i) bef *= nd.dims[i];
+  nk = nd.dims[k];
+  for (int i = k + 1; i < nd.order; ++i) aft *= nd.dims[i];
+}
+
+// dst = M x_f src on an LDS-resident node tensor (M is n_f x n_f).
+template <typename T>
+__device__ void mode_apply(cplx<T>* dst, const cplx<T>* src, const cplx<T>* __restrict__ M, const DNode& nd,
+                           int f) {
+  int bef, nf, aft;
+  node_view(nd, f, bef, nf, aft);
+  const int N = nd.N;
+  for (int e = threadIdx.x; e < N; e += TB) {
+    const int b = e % bef;
+    const int q = e / bef;
+    const int k = q % nf;
+    const int a = q / nf;
+    const cplx<T>* s = src + b + a * bef * nf;
+    cplx<T> acc = czero<T>();
+    for (int m = 0; m < nf; ++m) cfma(acc, M[k + m * nf], s[m * bef]);
+    dst[e] = acc;
+  }
+}
+
+// out(j,l) = sum_r conj(X(r,j)) Y(r,l) over all indices but k.  scratch needs TB entries.
+template <typename T>
+__device__ void contract_k(cplx<T>* __restrict__ out, const cplx<T>* X, const cplx<T>* Y, const DNode& nd, int k,
+                           cplx<T>* scratch, bool accumulate) {
+  int bef, nk, aft;
+  node_view(nd, k, bef, nk, aft);
+  const int outs = nk * nk;
+  const int R = bef * aft;
+  if (outs >= TB) {
+    for (int o = threadIdx.x; o < outs; o += TB) {
+      const int j = o % nk, l = o / nk;
+      cplx<T> acc = czero<T>();
+      for (int r = 0; r < R; ++r) {
+        const int b = r % bef, a = r / bef;
+        const int base = b + a * bef * nk;
+        cfmac(acc, X[base + j * bef], Y[base + l * bef]);
+      }
+      if (accumulate) acc = cadd(acc, out[o]);
+      out[o] = acc;
+    }
+    return;
+  }
+  const int G = TB / outs;
+  const int o = threadIdx.x % outs, g = threadIdx.x / outs;
+  const int j = o % nk, l = o / nk;
+  cplx<T> acc = czero<T>();
+  if (g < G) {
+    for (int r = g; r < R; r += G) {
+      const int b = r % bef, a = r / bef;
+      const int base = b + a * bef * nk;
+      cfmac(acc, X[base + j * bef], Y[base + l * bef]);
+    }
+  }
+  scratch[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < outs) {
+    cplx<T> s = scratch[threadIdx.x];
+    for (int gg = 1; gg < G; ++gg) s = cadd(s, scratch[gg * outs + threadIdx.x]);
+    if (accumulate) s = cadd(s, out[threadIdx.x]);
+    out[threadIdx.x] = s;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB)
+    sandwich_kernel(const SwItem* __restrict__ items, const DNode* __restrict__ nodes, const cplx<T>* bra_base,
+                    const cplx<T>* ket_base, cplx<T>* pool, int nmax) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* buf0 = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf1 = buf0 + nmax;  // nmax >= TB (scratch of contract_k)
+  cplx<T>* buf2 = buf1 + nmax;
+  const SwItem it = items[blockIdx.x];
+  const DNode nd = nodes[it.node];
+  const int N = nd.N;
+  // ket into buf0; the bra stays in global memory unless it is the ket
+  for (int e = threadIdx.x; e < N; e += TB) buf0[e] = ket_base[it.ket + e];
+  const bool same = (bra_base == ket_base) && (it.bra == it.ket);
+  __syncthreads();
+  cplx<T>* cur = buf0;
+  cplx<T>* bufs[2] = {buf1, buf2};
+  int nb = 0;
+  for (int f = 0; f < it.nf; ++f) {
+    cplx<T>* dst = bufs[nb];
+    mode_apply<T>(dst, cur, pool + it.fmat[f], nd, it.fmode[f]);
+    __syncthreads();
+    cur = dst;
+    nb ^= 1;
+  }
+  // bra: buf0 still holds the ket tensor when bra == ket; otherwise stage the
+  // bra into a buffer that is not the product buffer.
+  const cplx<T>* bra = buf0;
+  cplx<T>* scratch = (cur == buf1) ? buf2 : buf1;
+  if (!same) {
+    cplx<T>* b = (cur == buf0) ? buf1 : buf0;
+    scratch = (cur == buf0) ? buf2 : ((cur == buf1) ? buf2 : buf1);
+    __syncthreads();
+    for (int e = threadIdx.x; e < N; e += TB) b[e] = bra_base[it.bra + e];
+    __syncthreads();
+    bra = b;
+  }
+  contract_k<T>(pool + it.out, bra, cur, nd, it.k, scratch, false);
+}
+
+// ---------------------------------------------------------------------- apply
+template <typename T, int EPT>
+__global__ void __launch_bounds__(TB)
+    apply_kernel(const ApChunk* __restrict__ chunks, const ApTerm* __restrict__ terms,
+                 const DNode* __restrict__ nodes, const cplx<T>* psi, const cplx<T>* __restrict__ pool,
+                 cplx<T>* partial, int nmax) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* buf0 = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf1 = buf0 + nmax;
+  cplx<T>* buf2 = buf1 + nmax;
+  const ApChunk ch = chunks[blockIdx.x];
+  const DNode nd = nodes[ch.node];
+  const int N = nd.N;
+  for (int e = threadIdx.x; e < N; e += TB) buf0[e] = psi[nd.off + e];
+  cplx<T> acc[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) acc[i] = czero<T>();
+  __syncthreads();
+  for (int t = ch.t0; t < ch.t1; ++t) {
+    const ApTerm tm = terms[t];
+    const cplx<T> c = mk<T>((T)tm.cre, (T)tm.cim);
+    const cplx<T>* cur = buf0;
+    int nb = 0;
+    cplx<T>* bufs[2] = {buf1, buf2};
+    for (int f = 0; f + 1 < tm.nf; ++f) {
+      mode_apply<T>(bufs[nb], cur, pool + tm.fmat[f], nd, tm.fmode[f]);
+      __syncthreads();
+      cur = bufs[nb];
+      nb ^= 1;
+    }
+    if (tm.nf == 0) {
+#pragma unroll
+      for (int i = 0; i < EPT; ++i) {
+        const int e = threadIdx.x + i * TB;
+        if (e < N) cfma(acc[i], c, cur[e]);
+      }
+    } else {
+      const cplx<T>* M = pool + tm.fmat[tm.nf - 1];
+      int bef, nf, aft;
+      node_view(nd, tm.fmode[tm.nf - 1], bef, nf, aft);
+#pragma unroll
+      for (int i = 0; i < EPT; ++i) {
+        const int e = threadIdx.x + i * TB;
+        if (e < N) {
+          const int b = e % bef, q = e / bef, k = q % nf, a = q / nf;
+          const cplx<T>* s = cur + b + a * bef * nf;
+          cplx<T> v = czero<T>();
+          for (int m = 0; m < nf; ++m) cfma(v, M[k + m * nf], s[m * bef]);
+          cfma(acc[i], c, v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int e = threadIdx.x + i * TB;
+    if (e < N) partial[ch.out + e] = acc[i];
+  }
+}
+
+// --------------------------------------------------------------------- finish
+// Block-wide FP64 complex sum over TB threads.
+__device__ inline cplx<double> block_sum(cplx<double> v, cplx<double>* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = TB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  cplx<double> r = red[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB)
+    finish_kernel(const FinNode* __restrict__ fin, const DNode* __restrict__ nodes,
+                  const cplx<double>* __restrict__ psi64, const cplx<T>* __restrict__ partial,
+                  const cplx<double>* __restrict__ pool64, cplx<double>* dpsi, cplx<double>* apply_out,
+                  double ph_re, double ph_im, cplx<double>* energy, int nmax) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<double>* sY = reinterpret_cast<cplx<double>*>(smem_raw);
+  cplx<double>* sA = sY + nmax;
+  cplx<double>* sm = sA + nmax;  // two n x n matrices (n = SPFs of the node)
+  __shared__ cplx<double> red[TB];
+  const FinNode fn = fin[blockIdx.x];
+  const DNode nd = nodes[fn.node];
+  const int N = nd.N, n = nd.dims[nd.order - 1], L = N / n;
+  for (int e = threadIdx.x; e < N; e += TB) {
+    cplx<double> s = czero<double>();
+    for (int c = 0; c < fn.nchunk; ++c) {
+      const cplx<T> p = partial[fn.part0 + (int64_t)c * N + e];
+      s.re += (double)p.re;
+      s.im += (double)p.im;
+    }
+    sY[e] = s;
+    sA[e] = psi64[nd.off + e];
+  }
+  __syncthreads();
+  if (apply_out)
+    for (int e = threadIdx.x; e < N; e += TB) apply_out[nd.off + e] = sY[e];
+  // -i * phase
+  const cplx<double> f = mk<double>(ph_im, -ph_re);
+  if (fn.top) {
+    cplx<double> ev = czero<double>(), nv = czero<double>();
+    for (int e = threadIdx.x; e < N; e += TB) {
+      cfmac(ev, sA[e], sY[e]);
+      cfmac(nv, sA[e], sA[e]);
+    }
+    ev = block_sum(ev, red);
+    nv = block_sum(nv, red);
+    if (threadIdx.x == 0 && energy) {
+      energy[0] = ev;
+      energy[1] = nv;
+    }
+    for (int e = threadIdx.x; e < N; e += TB) dpsi[nd.off + e] = cmul(f, sY[e]);
+    return;
+  }
+  cplx<double>* S = sm;          // n x n overlap A^H A
+  cplx<double>* X = sm + n * n;  // A^H Y, then solution
+  // overlaps
+  for (int o = threadIdx.x; o < 2 * n * n; o += TB) {
+    const int which = o / (n * n), oo = o % (n * n);
+    const int j = oo % n, l = oo / n;
+    const cplx<double>* R = which ? sY : sA;
+    cplx<double> acc = czero<double>();
+    for (int i = 0; i < L; ++i) cfmac(acc, sA[i + j * L], R[i + l * L]);
+    (which ? X : S)[oo] = acc;
+  }
+  __syncthreads();
+  // Gauss-Jordan with partial pivoting on [S | X] (column-major, n x n each)
+  __shared__ int piv;
+  __shared__ cplx<double> facs[64];
+  for (int col = 0; col < n; ++col) {
+    if (threadIdx.x == 0) {
+      int p = col;
+      double best = -1.0;
+      for (int r = col; r < n; ++r) {
+        const cplx<double> v = S[r + col * n];
+        const double a = v.re * v.re + v.im * v.im;
+        if (a > best) {
+          best = a;
+          p = r;
+        }
+      }
+      piv = p;
+    }
+    __syncthreads();
+    const int p = piv;
+    if (p != col) {
+      for (int k = threadIdx.x; k < 2 * n; k += TB) {
+        cplx<double>* Mk = (k < n) ? S + k * n : X + (k - n) * n;
+        const cplx<double> t = Mk[col];
+        Mk[col] = Mk[p];
+        Mk[p] = t;
+      }
+      __syncthreads();
+    }
+    const cplx<double> d = S[col + col * n];
+    const double dd = d.re * d.re + d.im * d.im;
+    const cplx<double> dinv = mk<double>(d.re / dd, -d.im / dd);
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * n; k += TB) {
+      cplx<double>* Mk = (k < n) ? S + k * n : X + (k - n) * n;
+      Mk[col] = cmul(Mk[col], dinv);
+    }
+    __syncthreads();
+    // eliminate: every row r != col: row_r -= S(r,col) * row_col.  Read the
+    // factors first so that the column col itself is updated last.
+    for (int r = threadIdx.x; r < n; r += TB) facs[r] = S[r + col * n];
+    __syncthreads();
+    for (int o = threadIdx.x; o < n * 2 * n; o += TB) {
+      const int r = o % n, k = o / n;
+      if (r == col) continue;
+      cplx<double>* Mk = (k < n) ? S + k * n : X + (k - n) * n;
+      const cplx<double> fr = facs[r];
+      const cplx<double> v = Mk[col];
+      cplx<double> w = Mk[r];
+      w.re -= fr.re * v.re - fr.im * v.im;
+      w.im -= fr.re * v.im + fr.im * v.re;
+      Mk[r] = w;
+    }
+    __syncthreads();
+  }
+  // Y <- Y - A X (each lane updates only its own elements; A is read-only)
+  for (int e = threadIdx.x; e < N; e += TB) {
+    const int i = e % L, k = e / L;
+    cplx<double> acc = sY[e];
+    for (int m = 0; m < n; ++m) {
+      const cplx<double> a = sA[i + m * L], x = X[m + k * n];
+      acc.re -= a.re * x.re - a.im * x.im;
+      acc.im -= a.re * x.im + a.im * x.re;
+    }
+    sY[e] = acc;
+  }
+  __syncthreads();
+  // dpsi(i,k) = f * sum_l rhoinv(k,l) Y(i,l)
+  const cplx<double>* Ri = pool64 + fn.rhoinv;
+  for (int e = threadIdx.x; e < N; e += TB) {
+    const int i = e % L, k = e / L;
+    cplx<double> acc = czero<double>();
+    for (int l = 0; l < n; ++l) cfma(acc, Ri[k + l * n], sY[i + l * L]);
+    dpsi[nd.off + e] = cmul(f, acc);
+  }
+}
+
+// --------------------------------------------------------- Jacobi eigensolver
+// Hermitian n x n (n <= 64), column-major in LDS.  Tournament ordering: each
+// step rotates n/2 disjoint (p,q) pairs whose parameters are computed from the
+// same matrix, columns first, then rows.
+template <typename T>
+__global__ void __launch_bounds__(TB)
+    heev_reginv_kernel(const EigItem* __restrict__ items, const cplx<T>* __restrict__ pool,
+                       cplx<double>* pool64, double eps) {
+  __shared__ cplx<double> A[64 * 65];
+  __shared__ cplx<double> U[64 * 65];
+  __shared__ int perm[64];
+  __shared__ double pc[32], ps[32];
+  __shared__ cplx<double> pph[32];
+  __shared__ int pp[32], pq[32];
+  __shared__ double red[TB];
+  __shared__ double lam[64];
+  const EigItem it = items[blockIdx.x];
+  const int n = it.n;
+  const int ld = 65;
+  for (int o = threadIdx.x; o < n * n; o += TB) {
+    const int i = o % n, j = o / n;
+    const cplx<T> v = pool[it.in + o];
+    A[i + j * ld] = mk<double>((double)v.re, (double)v.im);
+    U[i + j * ld] = mk<double>(i == j ? 1.0 : 0.0, 0.0);
+  }
+  // Hermitise (the input is Hermitian up to rounding)
+  __syncthreads();
+  for (int o = threadIdx.x; o < n * n; o += TB) {
+    const int i = o % n, j = o / n;
+    if (i < j) {
+      const cplx<double> a = A[i + j * ld], b = A[j + i * ld];
+      const cplx<double> m = mk<double>(0.5 * (a.re + b.re), 0.5 * (a.im - b.im));
+      A[i + j * ld] = m;
+      A[j + i * ld] = cconj(m);
+    } else if (i == j) {
+      A[i + i * ld].im = 0.0;
+    }
+  }
+  const int m = (n + 1) & ~1;  // even player count (a dummy when n is odd)
+  for (int i = threadIdx.x; i < m; i += TB) perm[i] = i;
+  __syncthreads();
+  if (n > 1) {
+    for (int sweep = 0; sweep < 30; ++sweep) {
+      // convergence test: off-diagonal vs diagonal Frobenius norms
+      double off = 0.0, dia = 0.0;
+      for (int o = threadIdx.x; o < n * n; o += TB) {
+        const int i = o % n, j = o / n;
+        const cplx<double> v = A[i + j * ld];
+        const double a = v.re * v.re + v.im * v.im;
+        if (i == j) dia += a;
+        else off += a;
+      }
+      red[threadIdx.x] = off;
+      __syncthreads();
+      for (int s = TB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      off = red[0];
+      __syncthreads();
+      red[threadIdx.x] = dia;
+      __syncthreads();
+      for (int s = TB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      dia = red[0];
+      __syncthreads();
+      if (off <= 1e-32 * dia || off == 0.0) break;
+      for (int step = 0; step < m - 1; ++step) {
+        const int np = m / 2;
+        // pair i: perm[i] vs perm[m-1-i]
+        for (int i = threadIdx.x; i < np; i += TB) {
+          int p = perm[i], q = perm[m - 1 - i];
+          if (p > q) {
+            const int t = p;
+            p = q;
+            q = t;
+          }
+          pp[i] = p;
+          pq[i] = q;
+          pc[i] = 1.0;
+          ps[i] = 0.0;
+          pph[i] = mk<double>(1.0, 0.0);
+          if (q < n) {
+            const cplx<double> apq = A[p + q * ld];
+            const double aa = sqrt(apq.re * apq.re + apq.im * apq.im);
+            if (aa > 0.0) {
+              const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
+              const double theta = (aqq - app) / (2.0 * aa);
+              const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+              const double c = 1.0 / sqrt(t * t + 1.0);
+              pc[i] = c;
+              ps[i] = t * c;
+              pph[i] = mk<double>(apq.re / aa, apq.im / aa);
+            }
+          }
+        }
+        __syncthreads();
+        // columns: A <- A G, U <- U G  (G = [[c, s ph], [-s conj(ph), c]] on (p,q))
+        for (int o = threadIdx.x; o < np * n; o += TB) {
+          const int i = o / n, r = o % n;
+          const int p = pp[i], q = pq[i];
+          if (q >= n || ps[i] == 0.0) continue;
+          const double c = pc[i], s = ps[i];
+          const cplx<double> ph = pph[i];
+          const cplx<double> arp = A[r + p * ld], arq = A[r + q * ld];
+          // new p = c*arp - s*conj(ph)*arq ; new q = s*ph*arp + c*arq
+          A[r + p * ld] = mk<double>(c * arp.re - s * (ph.re * arq.re + ph.im * arq.im),
+                                     c * arp.im - s * (ph.re * arq.im - ph.im * arq.re));
+          A[r + q * ld] = mk<double>(s * (ph.re * arp.re - ph.im * arp.im) + c * arq.re,
+                                     s * (ph.re * arp.im + ph.im * arp.re) + c * arq.im);
+          const cplx<double> urp = U[r + p * ld], urq = U[r + q * ld];
+          U[r + p * ld] = mk<double>(c * urp.re - s * (ph.re * urq.re + ph.im * urq.im),
+                                     c * urp.im - s * (ph.re * urq.im - ph.im * urq.re));
+          U[r + q * ld] = mk<double>(s * (ph.re * urp.re - ph.im * urp.im) + c * urq.re,
+                                     s * (ph.re * urp.im + ph.im * urp.re) + c * urq.im);
+        }
+        __syncthreads();
+        // rows: A <- G^H A  (new p = c*apk - s*ph*aqk ; new q = s*conj(ph)*apk + c*aqk)
+        for (int o = threadIdx.x; o < np * n; o += TB) {
+          const int i = o / n, k = o % n;
+          const int p = pp[i], q = pq[i];
+          if (q >= n || ps[i] == 0.0) continue;
+          const double c = pc[i], s = ps[i];
+          const cplx<double> ph = pph[i];
+          const cplx<double> apk = A[p + k * ld], aqk = A[q + k * ld];
+          A[p + k * ld] = mk<double>(c * apk.re - s * (ph.re * aqk.re - ph.im * aqk.im),
+                                     c * apk.im - s * (ph.re * aqk.im + ph.im * aqk.re));
+          A[q + k * ld] = mk<double>(s * (ph.re * apk.re + ph.im * apk.im) + c * aqk.re,
+                                     s * (ph.re * apk.im - ph.im * apk.re) + c * aqk.im);
+        }
+        __syncthreads();
+        // rotate the tournament: keep perm[0], shift the rest
+        if (threadIdx.x == 0) {
+          const int last = perm[m - 1];
+          for (int i = m - 1; i > 1; --i) perm[i] = perm[i - 1];
+          perm[1] = last;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += TB) lam[i] = A[i + i * ld].re;
+  __syncthreads();
+  if (it.ev >= 0)
+    for (int i = threadIdx.x; i < n; i += TB) pool64[it.ev + i] = mk<double>(lam[i], 0.0);
+  // regularised inverse  R(i,j) = sum_k U(i,k) w_k conj(U(j,k))
+  for (int o = threadIdx.x; o < n * n; o += TB) {
+    const int i = o % n, j = o / n;
+    cplx<double> acc = czero<double>();
+    for (int k = 0; k < n; ++k) {
+      const double l = lam[k];
+      const double w = 1.0 / (l + eps * exp(-l / eps));
+      const cplx<double> u = U[i + k * ld], v = U[j + k * ld];
+      acc.re += w * (u.re * v.re + u.im * v.im);
+      acc.im += w * (u.im * v.re - u.re * v.im);
+    }
+    pool64[it.out + o] = acc;
+  }
+}
+
+// ------------------------------------------------------------------- launches
+template <typename T>
+int launch_sandwich(const SwItem* items, int64_t count, const DNode* nodes, const void* bra, const void* ket,
+                    void* pool, int nmax, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  const size_t lds = 3 * (size_t)nmax * sizeof(cplx<T>);
+  hipLaunchKernelGGL(sandwich_kernel<T>, dim3((unsigned)count), dim3(TB), lds, s, items, nodes,
+                     (const cplx<T>*)bra, (const cplx<T>*)ket, (cplx<T>*)pool, nmax);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+template <typename T>
+int launch_apply(const ApChunk* chunks, int64_t nchunks, const ApTerm* terms, const DNode* nodes, const void* psi,
+                 const void* pool, void* partial, int nmax, hipStream_t s) {
+  if (nchunks == 0) return QTG_OK;
+  const size_t lds = 3 * (size_t)nmax * sizeof(cplx<T>);
+  const int ept = (nmax + TB - 1) / TB;
+#define QTG_APPLY(E)                                                                                     \
+  hipLaunchKernelGGL((apply_kernel<T, E>), dim3((unsigned)nchunks), dim3(TB), lds, s, chunks, terms, nodes, \
+                     (const cplx<T>*)psi, (const cplx<T>*)pool, (cplx<T>*)partial, nmax)
+  if (ept <= 1) QTG_APPLY(1);
+  else if (ept <= 2) QTG_APPLY(2);
+  else if (ept <= 4) QTG_APPLY(4);
+  else if (ept <= 8) QTG_APPLY(8);
+  else if (ept <= 16) QTG_APPLY(16);
+  else {
+    set_error("apply: node tensor larger than 4096 elements is not supported by the fused kernel");
+    return QTG_ESHAPE;
+  }
+#undef QTG_APPLY
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+template <typename T>
+int launch_finish(const FinNode* fin, int64_t count, const DNode* nodes, const void* psi64, const void* partial,
+                  const void* pool64, void* dpsi, void* apply_out, double ph_re, double ph_im, void* energy,
+                  int nmax, int nspf, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  const size_t lds = finish_lds_bytes(nmax, nspf);
+  hipLaunchKernelGGL(finish_kernel<T>, dim3((unsigned)count), dim3(TB), lds, s, fin, nodes,
+                     (const cplx<double>*)psi64, (const cplx<T>*)partial, (const cplx<double>*)pool64,
+                     (cplx<double>*)dpsi, (cplx<double>*)apply_out, ph_re, ph_im, (cplx<double>*)energy, nmax);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+template <typename T>
+int launch_heev(const EigItem* items, int64_t count, const void* pool, void* pool64, double eps, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  hipLaunchKernelGGL(heev_reginv_kernel<T>, dim3((unsigned)count), dim3(TB), 0, s, items, (const cplx<T>*)pool,
+                     (cplx<double>*)pool64, eps);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+template int launch_sandwich<double>(const SwItem*, int64_t, const DNode*, const void*, const void*, void*, int,
+                                     hipStream_t);
+template int launch_sandwich<float>(const SwItem*, int64_t, const DNode*, const void*, const void*, void*, int,
+                                    hipStream_t);
+template int launch_apply<double>(const ApChunk*, int64_t, const ApTerm*, const DNode*, const void*, const void*,
+                                  void*, int, hipStream_t);
+template int launch_apply<float>(const ApChunk*, int64_t, const ApTerm*, const DNode*, const void*, const void*,
+                                 void*, int, hipStream_t);
+template int launch_finish<double>(const FinNode*, int64_t, const DNode*, const void*, const void*, const void*,
+                                   void*, void*, double, double, void*, int, int, hipStream_t);
+template int launch_finish<float>(const FinNode*, int64_t, const DNode*, const void*, const void*, const void*,
+                                  void*, void*, double, double, void*, int, int, hipStream_t);
+template int launch_heev<double>(const EigItem*, int64_t, const void*, void*, double, hipStream_t);
+template int launch_heev<float>(const EigItem*, int64_t, const void*, void*, double, hipStream_t);
+
+}  // namespace qtg

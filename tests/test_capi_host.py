@@ -1,0 +1,130 @@
+"""CPU checks of the product library: every symbol of include/qutree_amd.h is
+exported, and the host-side logic (tree parsing, random states, work-item
+planning) agrees with the oracle and with an independent count."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import qutree_amd as Q
+from qutree_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "qutree_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s+(qtg_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert len(names) > 30
+    L = Q.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _host():
+    L = Q.lib()
+    L.qtg_host_psi_size.argtypes = [C.c_char_p]
+    L.qtg_host_psi_size.restype = C.c_int64
+    L.qtg_host_fill_random.argtypes = [C.c_char_p, C.c_uint64, C.c_int, np.ctypeslib.ndpointer(np.float64)]
+    L.qtg_host_plan_counts.argtypes = [C.c_char_p, C.c_int64] + [np.ctypeslib.ndpointer(np.float64)] + \
+        [np.ctypeslib.ndpointer(np.int64)] * 4 + [np.ctypeslib.ndpointer(np.float64), C.POINTER(C.c_double),
+                                                  np.ctypeslib.ndpointer(np.int64)]
+    return L
+
+
+@pytest.mark.parametrize("model", ["cmf8", "sweep64", "ref8"])
+def test_random_state_matches_oracle(model):
+    if model == "cmf8":
+        text, H = W.cmf8_model()
+    elif model == "sweep64":
+        text, H = W.sweep64_model()
+    else:
+        text = O.balanced_tree_text(8, 2, 2)
+    L = _host()
+    n = L.qtg_host_psi_size(text.encode())
+    m = O.OracleModel(text)
+    assert n == m.psi_size
+    for delta in (True, False):
+        out = np.zeros(2 * n)
+        assert L.qtg_host_fill_random(text.encode(), 77, int(delta), out) == 0
+        ref = m.fill_random(77, delta_lowest=delta)
+        # same mt19937 stream; Gram-Schmidt rounding differs between the two builds
+        assert np.max(np.abs(out.view(np.complex128) - ref)) < 1e-11
+
+
+def test_bad_tree_text_is_an_error():
+    L = _host()
+    assert L.qtg_host_psi_size(b"4 -2\n 3 0 0\n") == -1
+    assert L.qtg_host_psi_size(b"garbage") == -1
+
+
+@pytest.mark.parametrize("model", ["cmf8", "sweep64", "small"])
+def test_plan_counts_against_independent_count(model):
+    if model == "cmf8":
+        text, H = W.cmf8_model()
+    elif model == "sweep64":
+        text, H = W.sweep64_model()
+    else:
+        text, H = W.balanced_tree_text(4, 3, [2, 3]), W.coupled_ho(4, 1)
+    m = O.OracleModel(text, H)
+    L = _host()
+    c, toff, fm, fk, fp, pool = H.arrays()
+    flops = C.c_double()
+    items = np.zeros(3, np.int64)
+    assert L.qtg_host_plan_counts(text.encode(), H.nterms, c, toff, fm, fk, fp, pool, C.byref(flops), items) == 0
+    # independent: one represent + one hole item per (term, active non-top node),
+    # one density item per non-top node, one apply term per (term, active node)
+    nodes = m.nodes
+    pairs = 0
+    apply_terms = 0
+    fl = 0.0
+    for l in range(H.nterms):
+        act = set(m.sparse_tree(l))
+        for a in act:
+            n = nodes[a]
+            N = int(np.prod(n["dims"]))
+            nv = n["dims"][-1]
+            apply_terms += 1
+            if n["parent"] < 0:
+                fl += sum(8.0 * N * nodes[c]["dims"][-1] for c in n["children"] if c in act)
+                continue
+            pairs += 1
+            # represent
+            if n["bottom"]:
+                fl += 8.0 * N * n["dims"][0]
+            else:
+                fl += sum(8.0 * N * nodes[c]["dims"][-1] for c in n["children"] if c in act)
+            fl += 8.0 * N * nv
+            # hole
+            p = nodes[n["parent"]]
+            Np = int(np.prod(p["dims"]))
+            fl += sum(8.0 * Np * nodes[s]["dims"][-1] for s in p["children"] if s != a and s in act)
+            if p["parent"] >= 0:
+                fl += 8.0 * Np * p["dims"][-1]
+            fl += 8.0 * Np * nv
+            # apply
+            if n["bottom"]:
+                fl += 8.0 * N * n["dims"][0]
+            else:
+                fl += sum(8.0 * N * nodes[c]["dims"][-1] for c in n["children"] if c in act)
+            fl += 8.0 * N * nv
+    nontop = [n for n in nodes if n["parent"] >= 0]
+    for n in nontop:
+        p = nodes[n["parent"]]
+        Np = int(np.prod(p["dims"]))
+        fl += 8.0 * Np * n["dims"][-1] + (8.0 * Np * p["dims"][-1] if p["parent"] >= 0 else 0.0)
+        fl += 4 * 8.0 * int(np.prod(n["dims"])) * n["dims"][-1]
+    assert items[0] == 2 * pairs + len(nontop)
+    assert items[2] == apply_terms
+    assert abs(flops.value - fl) < 1e-9 * fl
+    if model == "sweep64":
+        assert pairs == 22752  # 64 + 2016 terms over the 64-mode tree

@@ -1,0 +1,154 @@
+"""GPU parity of the tree engine (HamiltonianRepresentation::build, Apply,
+LayerDerivative, Derivative, energy) against the FP64 oracle on the same
+seeded trees, Hamiltonians and states.
+
+complex128: 1e-10 relative per node (random full-rank states keep rho^-1
+well conditioned); complex64 bulk: energy within 1e-6 relative of FP64."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import qutree_amd as Q
+from qutree_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _models():
+    return {
+        "f4": (W.balanced_tree_text(4, 3, [2, 3]), W.coupled_ho(4, 11, c_sigma=0.1)),
+        "f8quartic": (W.balanced_tree_text(8, 2, [2, 3, 3]), W.coupled_quartic(8, 12, c_sigma=0.1)),
+        "cmf8": W.cmf8_model(),
+        "f16": (W.balanced_tree_text(16, 16, [8, 10, 11, 13]), W.coupled_ho(16, 16, c_sigma=0.01,
+                                                                             one_mode_matrix=True)),
+        "sweep64": W.sweep64_model(),
+    }
+
+
+def node_rel(eng, a_vec, b_vec):
+    """Worst per-node relative difference.  A node whose SPFs span its whole
+    configuration space has (1-P) = 0, so its derivative is rounding noise on
+    both sides; the scale floor (1e-6 of the largest entry) covers those."""
+    floor = 1e-6 * np.max(np.abs(b_vec))
+    worst = 0.0
+    for a in range(eng.nnodes):
+        x = eng.node_tensor(a_vec, a)
+        y = eng.node_tensor(b_vec, a)
+        worst = max(worst, np.max(np.abs(x - y)) / max(floor, np.max(np.abs(y))))
+    return worst
+
+
+def derivative_err(eng, om, got, ref, app_ref):
+    """Per-node derivative difference in units of the projector input:
+    |d_gpu - d_ref| / (max|Apply_v| * ||rho_v^-1||_2).  (1-P) subtracts the
+    component of Apply inside span(A_v), so both implementations lose the
+    same leading digits; this scale is what the rounding of either side is
+    proportional to."""
+    worst = 0.0
+    for a in range(eng.nnodes):
+        d = np.max(np.abs(eng.node_tensor(got, a) - om.node_tensor(ref, a)))
+        scale = np.max(np.abs(om.node_tensor(app_ref, a)))
+        if eng.nodes[a]["parent"] >= 0:
+            scale *= np.linalg.norm(om.rho_inv(a), 2)
+        worst = max(worst, d / max(scale, 1e-300))
+    return worst
+
+
+@pytest.mark.parametrize("name", ["f4", "f8quartic", "cmf8", "f16", "sweep64"])
+def test_derivative_matches_oracle(name):
+    text, H = _models()[name]
+    print(f"case engine derivative {name}", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(3, delta_lowest=False)
+    ref = om.derivative(psi)
+    e_ref = om.energy(psi)
+    app_ref = om.apply_all(psi)
+    eng = Q.Engine(text, H)
+    eng.set_psi(psi)
+    eng.derivative()
+    got = eng.get(0)
+    assert node_rel(eng, eng.get(1), app_ref) < 1e-11
+    err = derivative_err(eng, om, got, ref, app_ref)
+    print(f"  derivative err {err:.3e}", flush=True)
+    assert err < 1e-13
+    assert abs(eng.energy() - e_ref) < 1e-12 * abs(e_ref)
+
+
+@pytest.mark.parametrize("name", ["f4", "cmf8", "f16"])
+def test_representation_matrices_match_oracle(name):
+    text, H = _models()[name]
+    print(f"case engine matrices {name}", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(4, delta_lowest=False)
+    om.build(psi)
+    eng = Q.Engine(text, H)
+    eng.set_psi(psi)
+    eng.build()
+    Q.synchronize()
+    rng = np.random.default_rng(0)
+    for a in range(eng.nnodes):
+        if eng.nodes[a]["parent"] < 0:
+            continue
+        r = om.rho(a)
+        assert np.max(np.abs(eng.matrix(0, 0, a) - r)) < 1e-12 * max(1, np.max(np.abs(r)))
+        ri = om.rho_inv(a)
+        assert np.max(np.abs(eng.matrix(1, 0, a) - ri)) < 1e-9 * np.max(np.abs(ri))
+    for l in rng.choice(H.nterms, size=min(H.nterms, 12), replace=False):
+        for a in om.sparse_tree(int(l)):
+            if eng.nodes[a]["parent"] < 0:
+                continue
+            h = om.hmat(int(l), a)
+            g = om.hole(int(l), a)
+            assert np.max(np.abs(eng.matrix(2, int(l), a) - h)) < 1e-11 * max(1, np.max(np.abs(h)))
+            assert np.max(np.abs(eng.matrix(3, int(l), a) - g)) < 1e-11 * max(1, np.max(np.abs(g)))
+
+
+def test_hartree_start_energy_and_top_derivative():
+    # Hartree product start: rho = diag(1,0,..) so rho^-1 carries 1e10 in the
+    # empty directions; the gauge-invariant energy and the top-node derivative
+    # stay comparable.
+    text, H = W.cmf8_model()
+    print("case engine hartree cmf8", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(2024, delta_lowest=True)
+    ref = om.derivative(psi)
+    eng = Q.Engine(text, H)
+    eng.set_psi(psi)
+    eng.derivative()
+    top = eng.nnodes - 1
+    got = eng.get(0)
+    x, y = eng.node_tensor(got, top), om.node_tensor(ref, top)
+    assert np.max(np.abs(x - y)) < 1e-12 * np.max(np.abs(y))
+    assert abs(eng.energy() - om.energy(psi)) < 1e-12 * abs(om.energy(psi))
+
+
+@pytest.mark.parametrize("name", ["cmf8", "sweep64"])
+def test_complex64_bulk_error_against_fp64(name):
+    text, H = _models()[name]
+    print(f"case engine c64 {name}", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(5, delta_lowest=False)
+    ref = om.derivative(psi)
+    e64 = om.energy(psi)
+    eng = Q.Engine(text, H, precision=1)
+    eng.set_psi(psi)
+    eng.derivative()
+    e32 = eng.energy()
+    app_ref = om.apply_all(psi)
+    err = derivative_err(eng, om, eng.get(0), ref, app_ref)
+    print(f"  energy rel err {abs(e32 - e64) / abs(e64):.3e}  derivative err {err:.3e}", flush=True)
+    assert abs(e32 - e64) < 1e-6 * abs(e64)
+    assert err < 1e-5
+
+
+def test_repeated_derivative_is_deterministic():
+    text, H = W.sweep64_model()
+    print("case engine determinism sweep64", flush=True)
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(9, delta_lowest=False)
+    eng.set_psi(psi)
+    eng.derivative()
+    a = eng.get(0)
+    eng.derivative()
+    b = eng.get(0)
+    assert np.array_equal(a, b)
