@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Benchmark of the tree-tensor contraction path on one MI355X.
+
+Workload (one "step"): one H|Psi> sweep -- QuTree's Derivative(), i.e.
+HamiltonianRepresentation::build (density tree, regularised rho^-1, h-matrices
+and hole matrices of every SOP term) followed by LayerDerivative at every node
+-- on the 64-mode balanced binary tree: 16 HO-DVR points per leaf, SPF counts
+8,10,11,13,14,16 from the bottom layer to the top's children, 64 one-mode terms
+T_i + w_i^2 q_i^2 / 2 plus 2016 pair terms c_ij q_i q_j (c_ij ~ N(0, 0.01^2)),
+random orthonormal tree state.  Inputs are resident in HBM before timing.
+
+Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H|Psi> sweeps/s & CMF steps/s, 64-mode tree, at ≤1e-8 rel. energy error"
+PEAK = {"complex128": 78.6, "complex64": 157.3}  # TFLOP/s, dense FP64 / FP32
+
+
+def cpu_baseline(text, H, psi, gpu_energy, budget_s=12.0, max_sweeps=5):
+    """FP64 CPU restatement (oracle), single thread, timed on a bounded number
+    of sweeps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    om = O.OracleModel(text, H)
+    om.derivative(psi)  # warm-up: page in, build plans
+    n, t0 = 0, time.perf_counter()
+    while n < max_sweeps:
+        om.derivative(psi)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    e_ref = om.energy(psi)
+    return {"value": n / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} full sweeps of the same 64-mode workload (FP64 oracle, -O3 -ffast-math, one thread)",
+            }, e_ref
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["complex128", "complex64"], default="complex128")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding over SOP terms is design-only", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import qutree_amd as Q
+    from qutree_amd import workloads as W
+
+    text, H = W.sweep64_model()
+    prec = 0 if args.precision == "complex128" else 1
+    eng = Q.Engine(text, H, precision=prec)
+    psi = eng.fill_random(64, delta_lowest=False)
+    eng.set_psi(psi)
+    for _ in range(args.warmup):
+        eng.derivative(sync=False)
+    Q.synchronize()
+
+    eng.set_timing(True)
+    Q.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.derivative(sync=False)
+    Q.synchronize()
+    t1 = time.perf_counter()
+    timing = eng.read_timing()
+    eng.set_timing(False)
+    sweep_s = (t1 - t0) / args.steps
+    value = 1.0 / sweep_s
+    energy = eng.energy()
+    flops, items = eng.counts()
+
+    # dominant kernel = largest summed event time
+    dom = max(timing, key=lambda k: timing[k]["ms"])
+    d = timing[dom]
+    avg_ms = d["ms"] / max(1, d["launches"])
+    flops_per_launch = d["flops"] / max(1, d["launches"])
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    peak = PEAK[args.precision]
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 5), "traffic": None, "kernel": dom,
+                "avg_launch_ms": round(avg_ms, 5), "launches_per_step": d["launches"] / args.steps,
+                "flops_per_launch": flops_per_launch}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sweep_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": "H|Psi> sweep (Derivative: build + LayerDerivative at all nodes), 64-mode "
+                               "balanced binary tree, 16 HO-DVR, SPF 8,10,11,13,14,16, 64 one-mode + 2016 "
+                               "pair terms",
+                   "modes": 64, "nodes": eng.nnodes, "terms": H.nterms, "psi_elements": int(eng.psi_size),
+                   "parallelism": "single GPU"},
+        "sweep_flops": flops,
+        "tflops_per_s": round(flops * value / 1e12, 3),
+        "kernels": {k: {"ms_per_step": round(v["ms"] / args.steps, 4),
+                        "launches_per_step": v["launches"] / args.steps} for k, v in timing.items()},
+        "energy": energy,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        base, e_ref = cpu_baseline(text, H, psi, energy)
+        out["cpu_baseline"] = base
+        out["energy_rel_err_vs_fp64_oracle"] = abs(energy - e_ref) / abs(e_ref)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -177,6 +177,13 @@ int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* co
                          const int64_t* fpool, const double* pool, double* sweep_flops, int64_t* items);
 int64_t qtg_host_psi_size(const char* tree_text);
 int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest, double* psi_host);
+/* Kernel timing: while on, HIP events are recorded on the launch stream
+ * around every launch group; read_timing returns, per kernel group
+ * [0 sandwich (represent/holes/density), 1 eigensolver, 2 apply, 3 finish,
+ * 4 precision conversion], the summed milliseconds, launch counts and
+ * algorithmic FLOPs since set_timing(1).  Arrays hold 5 entries. */
+int qtg_engine_set_timing(qtg_engine engine, int on);
+int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, double* flops);
 /* "rho_eps": regularisation of the density inverse (default 1e-10) */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
 

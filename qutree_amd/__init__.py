@@ -76,6 +76,8 @@ def lib():
     L.qtg_engine_get_matrix.argtypes = [_vp, C.c_int, _i64, _i64, _dp, _vp]
     L.qtg_engine_counts.argtypes = [_vp, C.POINTER(C.c_double), _lp]
     L.qtg_engine_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
+    L.qtg_engine_set_timing.argtypes = [_vp, C.c_int]
+    L.qtg_engine_read_timing.argtypes = [_vp, _dp, _lp, _dp]
     _lib = L
     return L
 
@@ -265,6 +267,19 @@ class Engine:
         items = np.zeros(2, np.int64)
         check(lib().qtg_engine_counts(self.ptr, C.byref(f), items))
         return f.value, items
+
+    KERNELS = ("sandwich_kernel", "heev_reginv_kernel", "apply_kernel", "finish_kernel", "to_f32")
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().qtg_engine_set_timing(self.ptr, int(on)))
+
+    def read_timing(self):
+        ms = np.zeros(5)
+        n = np.zeros(5, np.int64)
+        fl = np.zeros(5)
+        check(lib().qtg_engine_read_timing(self.ptr, ms, n, fl))
+        return {k: dict(ms=float(ms[i]), launches=int(n[i]), flops=float(fl[i]))
+                for i, k in enumerate(self.KERNELS)}
 
     def set_option(self, key: str, value: float) -> None:
         check(lib().qtg_engine_set_option(self.ptr, key.encode(), value))

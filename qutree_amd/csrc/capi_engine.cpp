@@ -152,6 +152,16 @@ int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest,
   return QTG_OK;
 }
 
+int qtg_engine_set_timing(qtg_engine e, int on) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->set_timing(on != 0);
+}
+
+int qtg_engine_read_timing(qtg_engine e, double* ms, int64_t* launches, double* flops) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->read_timing(ms, launches, flops);
+}
+
 int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   if (!e || !key) return QTG_EARG;
   auto* en = static_cast<Engine*>(e);

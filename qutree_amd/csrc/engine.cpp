@@ -28,7 +28,10 @@ __global__ void to_f32(const double* __restrict__ in, float* __restrict__ out, i
 
 }  // namespace
 
-Engine::~Engine() { release(); }
+Engine::~Engine() {
+  release();
+  for (auto e : evpool_) (void)hipEventDestroy(e);
+}
 
 void Engine::release() {
   for (void* p : device_buffers_)
@@ -161,6 +164,9 @@ int Engine::plan() {
     return -1;
   };
   flops_ = 0.0;
+  layerFlops_.clear();
+  for (auto& x : phaseFlops_) x = 0.0;
+  double mark = 0.0;
   // ---- represent: bottom-up by height
   std::vector<SwItem> items;
   repRange_.assign(tree_.maxHeight, {0, 0});
@@ -190,6 +196,8 @@ int Engine::plan() {
         items.push_back(it);
       }
     repRange_[h].second = (int64_t)items.size();
+    layerFlops_.push_back(flops_ - mark);
+    mark = flops_;
   }
   // ---- holes and density matrices: top-down by depth
   holeRange_.assign(tree_.maxDepth + 1, {0, 0});
@@ -235,8 +243,11 @@ int Engine::plan() {
         items.push_back(it);
       }
     holeRange_[dd].second = (int64_t)items.size();
+    layerFlops_.push_back(flops_ - mark);
+    mark = flops_;
   }
   nSw_ = (int64_t)items.size();
+  phaseFlops_[0] = flops_;
   // ---- density eigen / inverse
   std::vector<EigItem> eig;
   for (auto& v : tree_.nodes) {
@@ -308,6 +319,15 @@ int Engine::plan() {
   }
   nChunks_ = (int64_t)chunks.size();
   nFin_ = (int64_t)fin.size();
+  {
+    // apply: everything counted after the sandwich phase except the
+    // projector / rho^-1 work of the finish kernel
+    double proj = 0.0;
+    for (auto& v : tree_.nodes)
+      if (!v.top()) proj += 4 * 8.0 * v.N * v.n;
+    phaseFlops_[2] = flops_ - phaseFlops_[0] - proj;
+    phaseFlops_[3] = proj;
+  }
   nEig_ = (int64_t)eig.size();
   partialSize_ = part;
   h_items_ = items;
@@ -368,24 +388,90 @@ int Engine::set_psi_device(const double* dev, hipStream_t s) {
   return QTG_OK;
 }
 
+int Engine::set_timing(bool on) {
+  timing_ = on;
+  marks_.clear();
+  evnext_ = 0;
+  for (int i = 0; i < kPhases; ++i) tms_[i] = tfl_[i] = 0.0, tn_[i] = 0;
+  return QTG_OK;
+}
+
+int Engine::begin_mark(int phase, hipStream_t s) {
+  if (!timing_) return QTG_OK;
+  while (evpool_.size() < evnext_ + 2) {
+    hipEvent_t e;
+    QTG_HIP_TRY(hipEventCreate(&e));
+    evpool_.push_back(e);
+  }
+  Mark m{evpool_[evnext_], evpool_[evnext_ + 1], phase};
+  evnext_ += 2;
+  QTG_HIP_TRY(hipEventRecord(m.a, s));
+  marks_.push_back(m);
+  return QTG_OK;
+}
+
+int Engine::end_mark(hipStream_t s) {
+  if (!timing_) return QTG_OK;
+  QTG_HIP_TRY(hipEventRecord(marks_.back().b, s));
+  // keep the event pool bounded: fold finished marks into the totals
+  if (marks_.size() >= 512) return collect_marks();
+  return QTG_OK;
+}
+
+int Engine::collect_marks() {
+  for (auto& m : marks_) {
+    QTG_HIP_TRY(hipEventSynchronize(m.b));
+    float ms = 0.f;
+    QTG_HIP_TRY(hipEventElapsedTime(&ms, m.a, m.b));
+    tms_[m.phase] += ms;
+    tn_[m.phase] += 1;
+  }
+  marks_.clear();
+  evnext_ = 0;
+  return QTG_OK;
+}
+
+int Engine::read_timing(double* ms, int64_t* launches, double* flops) {
+  int rc = collect_marks();
+  if (rc) return rc;
+  for (int i = 0; i < kPhases; ++i) {
+    if (ms) ms[i] = tms_[i];
+    if (launches) launches[i] = tn_[i];
+    if (flops) flops[i] = tfl_[i];
+  }
+  return QTG_OK;
+}
+
 template <typename T>
 int Engine::build_t(hipStream_t s) {
   const void* psi = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
+  int rc;
   if (c64_) {
     const int64_t n = 2 * tree_.total;
+    if ((rc = begin_mark(4, s))) return rc;
     hipLaunchKernelGGL(to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_psi64_, d_psi32_, n);
     QTG_LAUNCH_CHECK();
+    if ((rc = end_mark(s))) return rc;
   }
-  int rc;
+  size_t li = 0;
+  auto sw = [&](const std::pair<int64_t, int64_t>& r) -> int {
+    const double fl = li < layerFlops_.size() ? layerFlops_[li] : 0.0;
+    ++li;
+    if (r.second == r.first) return QTG_OK;
+    int e = begin_mark(0, s);
+    if (e) return e;
+    if ((e = launch_sandwich<T>(d_sw_ + r.first, r.second - r.first, d_nodes_, psi, psi, d_pool_, nmax_, s)))
+      return e;
+    if (timing_) tfl_[0] += fl;
+    return end_mark(s);
+  };
   for (auto& r : repRange_)
-    if ((rc = launch_sandwich<T>(d_sw_ + r.first, r.second - r.first, d_nodes_, psi, psi, d_pool_, nmax_, s)))
-      return rc;
-  for (size_t dd = 1; dd < holeRange_.size(); ++dd) {
-    auto& r = holeRange_[dd];
-    if ((rc = launch_sandwich<T>(d_sw_ + r.first, r.second - r.first, d_nodes_, psi, psi, d_pool_, nmax_, s)))
-      return rc;
-  }
+    if ((rc = sw(r))) return rc;
+  for (size_t dd = 1; dd < holeRange_.size(); ++dd)
+    if ((rc = sw(holeRange_[dd]))) return rc;
+  if ((rc = begin_mark(1, s))) return rc;
   if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, s))) return rc;
+  if ((rc = end_mark(s))) return rc;
   built_ = true;
   return QTG_OK;
 }
@@ -401,10 +487,17 @@ int Engine::build(hipStream_t s) {
 template <typename T>
 int Engine::apply_t(hipStream_t s, double ph_re, double ph_im) {
   const void* psi = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
-  int rc = launch_apply<T>(d_chunks_, nChunks_, d_terms_, d_nodes_, psi, d_pool_, d_partial_, nmax_, s);
+  int rc = begin_mark(2, s);
   if (rc) return rc;
-  return launch_finish<T>(d_fin_, nFin_, d_nodes_, d_psi64_, d_partial_, d_pool64_, d_dpsi64_, d_apply64_, ph_re,
-                          ph_im, d_energy_, nmax_, nspf_, s);
+  if ((rc = launch_apply<T>(d_chunks_, nChunks_, d_terms_, d_nodes_, psi, d_pool_, d_partial_, nmax_, s))) return rc;
+  if ((rc = end_mark(s))) return rc;
+  if (timing_) tfl_[2] += phaseFlops_[2];
+  if ((rc = begin_mark(3, s))) return rc;
+  if ((rc = launch_finish<T>(d_fin_, nFin_, d_nodes_, d_psi64_, d_partial_, d_pool64_, d_dpsi64_, d_apply64_, ph_re,
+                             ph_im, d_energy_, nmax_, nspf_, s)))
+    return rc;
+  if (timing_) tfl_[3] += phaseFlops_[3];
+  return end_mark(s);
 }
 
 int Engine::layer_derivative_all(hipStream_t s, double ph_re, double ph_im) {

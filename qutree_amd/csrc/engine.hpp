@@ -39,6 +39,11 @@ class Engine {
   int64_t apply_terms() const { return (int64_t)h_terms_.size(); }
   double eps_ = 1e-10;
   int chunkTerms_ = 32;
+  // Per-kernel timing with HIP events recorded on the launch stream around
+  // every launch group: 0 sandwich, 1 heev, 2 apply, 3 finish, 4 convert.
+  static constexpr int kPhases = 5;
+  int set_timing(bool on);
+  int read_timing(double* ms, int64_t* launches, double* flops);
 
  private:
   template <typename T>
@@ -83,6 +88,22 @@ class Engine {
   void* d_partial_ = nullptr;
   double* d_energy_ = nullptr;
   std::vector<void*> device_buffers_;
+  // timing state
+  bool timing_ = false;
+  struct Mark {
+    hipEvent_t a, b;
+    int phase;
+  };
+  std::vector<Mark> marks_;
+  std::vector<hipEvent_t> evpool_;
+  size_t evnext_ = 0;
+  double tms_[kPhases] = {0}, tfl_[kPhases] = {0};
+  int64_t tn_[kPhases] = {0};
+  double phaseFlops_[kPhases] = {0};
+  std::vector<double> layerFlops_;  // flops of each sandwich launch, in launch order
+  int begin_mark(int phase, hipStream_t s);
+  int end_mark(hipStream_t s);
+  int collect_marks();
 };
 
 }  // namespace qtg
