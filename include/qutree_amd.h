@@ -174,17 +174,24 @@ int qtg_engine_counts(qtg_engine engine, double* sweep_flops, int64_t* items);
  * the random initial state of a tree. */
 int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* coeff,
                          const int64_t* term_off, const int64_t* fmode, const int64_t* fkind,
-                         const int64_t* fpool, const double* pool, double* sweep_flops, int64_t* items);
+                         const int64_t* fpool, const double* pool, int compress, double* sweep_flops,
+                         int64_t* items);
 int64_t qtg_host_psi_size(const char* tree_text);
 int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest, double* psi_host);
 /* Kernel timing: while on, HIP events are recorded on the launch stream
  * around every launch group; read_timing returns, per kernel group
  * [0 sandwich (represent/holes/density), 1 eigensolver, 2 apply, 3 finish,
- * 4 precision conversion], the summed milliseconds, launch counts and
- * algorithmic FLOPs since set_timing(1).  Arrays hold 5 entries. */
+ * 4 precision conversion, 5 matrix combination], the summed milliseconds,
+ * launch counts and algorithmic FLOPs since set_timing(1).  Arrays hold 6
+ * entries. */
 int qtg_engine_set_timing(qtg_engine engine, int on);
 int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, double* flops);
-/* "rho_eps": regularisation of the density inverse (default 1e-10) */
+/* "rho_eps": regularisation of the density inverse (default 1e-10).
+ * "compress" (set before qtg_engine_set_hamiltonian; default 1): compute each
+ * distinct h-matrix / hole matrix once (they depend only on the term's
+ * factors inside / outside the node's subtree) and merge Apply summands that
+ * share all slots but one; 0 keeps one work item per (term, node) as the
+ * reference loops do. */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
 
 #ifdef __cplusplus

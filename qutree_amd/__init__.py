@@ -203,11 +203,12 @@ def _ci(z: np.ndarray) -> np.ndarray:
 class Engine:
     """Device-resident tree + SOP Hamiltonian (qtg_engine_*)."""
 
-    def __init__(self, tree_text: str, H=None, precision: int = 0):
+    def __init__(self, tree_text: str, H=None, precision: int = 0, compress: bool = True):
         init()
         p = _vp()
         check(lib().qtg_engine_create(tree_text.encode(), precision, C.byref(p)))
         self.ptr = p.value
+        check(lib().qtg_engine_set_option(self.ptr, b"compress", float(compress)))
         self.precision = precision
         self.psi_size = lib().qtg_engine_psi_size(self.ptr)
         self.nnodes = lib().qtg_engine_num_nodes(self.ptr)
@@ -264,19 +265,20 @@ class Engine:
 
     def counts(self):
         f = C.c_double()
-        items = np.zeros(2, np.int64)
+        items = np.zeros(3, np.int64)
         check(lib().qtg_engine_counts(self.ptr, C.byref(f), items))
         return f.value, items
 
-    KERNELS = ("sandwich_kernel", "heev_reginv_kernel", "apply_kernel", "finish_kernel", "to_f32")
+    KERNELS = ("sandwich_kernel", "heev_reginv_kernel", "apply_kernel", "finish_kernel", "to_f32",
+               "combine_kernel")
 
     def set_timing(self, on: bool) -> None:
         check(lib().qtg_engine_set_timing(self.ptr, int(on)))
 
     def read_timing(self):
-        ms = np.zeros(5)
-        n = np.zeros(5, np.int64)
-        fl = np.zeros(5)
+        ms = np.zeros(6)
+        n = np.zeros(6, np.int64)
+        fl = np.zeros(6)
         check(lib().qtg_engine_read_timing(self.ptr, ms, n, fl))
         return {k: dict(ms=float(ms[i]), launches=int(n[i]), flops=float(fl[i]))
                 for i, k in enumerate(self.KERNELS)}

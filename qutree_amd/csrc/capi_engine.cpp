@@ -124,11 +124,12 @@ int qtg_engine_counts(qtg_engine e, double* sweep_flops, int64_t* items) {
 // Host-only planning (no device is touched): counts of the work-item plan.
 int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* coeff, const int64_t* term_off,
                          const int64_t* fmode, const int64_t* fkind, const int64_t* fpool, const double* pool,
-                         double* sweep_flops, int64_t* items) {
+                         int compress, double* sweep_flops, int64_t* items) {
   if (!tree_text) return QTG_EARG;
   Engine en;
   int rc = en.create(tree_text, 0, true);
   if (rc) return rc;
+  en.compress_ = compress != 0;
   rc = en.set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool);
   if (rc) return rc;
   return qtg_engine_counts(&en, sweep_flops, items);
@@ -167,6 +168,10 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   auto* en = static_cast<Engine*>(e);
   if (!std::strcmp(key, "rho_eps")) {
     en->eps_ = value;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "compress")) {
+    en->compress_ = value != 0.0;
     return QTG_OK;
   }
   set_error(std::string("unknown option ") + key);

@@ -122,11 +122,43 @@ void Engine::active_sets() {
     }
 }
 
+// Term restrictions: the factors of term l on the leaves inside (or outside)
+// the subtree of node a.  An h-matrix depends only on the inside restriction,
+// a hole matrix only on the outside one, so equal restrictions share a slot.
+std::vector<std::pair<int, int>> Engine::restrict_in(int l, int a) const {
+  std::vector<std::pair<int, int>> r;
+  for (auto& lm : sop_.terms[l].leafmats)
+    if (inSub_[a][lm.first]) r.push_back(lm);
+  return r;
+}
+
+std::vector<std::pair<int, int>> Engine::restrict_out(int l, int a) const {
+  std::vector<std::pair<int, int>> r;
+  for (auto& lm : sop_.terms[l].leafmats)
+    if (!inSub_[a][lm.first]) r.push_back(lm);
+  return r;
+}
+
 int Engine::plan() {
   active_sets();
   const int L = (int)sop_.terms.size();
   const int NN = (int)tree_.nodes.size();
-  // ---- pool layout (complex elements)
+  const int F = (int)tree_.leafNode.size();
+  inSub_.assign(NN, std::vector<char>(F, 0));
+  for (auto& v : tree_.nodes) {
+    if (v.bottom) inSub_[v.addr][v.leaf.mode] = 1;
+    else
+      for (int c : v.children)
+        for (int m = 0; m < F; ++m)
+          if (inSub_[c][m]) inSub_[v.addr][m] = 1;
+  }
+  for (auto& v : tree_.nodes)
+    if ((v.bottom ? 1 : (int)v.children.size()) + (v.top() ? 0 : 1) > kMaxFactors) {
+      set_error("engine: nodes with more than two children are not supported");
+      return QTG_ESHAPE;
+    }
+  auto nn = [&](int a) { return (int64_t)tree_.nodes[a].n * tree_.nodes[a].n; };
+  // ---- pool layout (complex elements): leaf matrices, h slots, hole slots
   int64_t off = 0;
   leafOff_.clear();
   for (auto& m : sop_.leafMatrices) {
@@ -135,26 +167,56 @@ int Engine::plan() {
   }
   hslot_.assign(L, std::vector<int64_t>(NN, -1));
   holeslot_.assign(L, std::vector<int64_t>(NN, -1));
-  for (int l = 0; l < L; ++l)
-    for (int a = 0; a < NN; ++a)
-      if (active_[l][a] && !tree_.nodes[a].top()) {
-        const int64_t nn = (int64_t)tree_.nodes[a].n * tree_.nodes[a].n;
-        hslot_[l][a] = off;
-        off += nn;
-        holeslot_[l][a] = off;
-        off += nn;
-      }
   rhoslot_.assign(NN, -1);
+  // per node: (slot, representative term) of every distinct h / hole matrix
+  std::vector<std::vector<std::pair<int64_t, int>>> hent(NN), qent(NN);
+  typedef std::vector<std::pair<int, int>> Key;
+  std::vector<std::map<Key, int64_t>> hkey(NN), qkey(NN);
   for (int a = 0; a < NN; ++a) {
+    const HNode& v = tree_.nodes[a];
+    if (v.top()) continue;
+    // density matrix = hole of the empty outside restriction
     rhoslot_[a] = off;
-    off += (int64_t)tree_.nodes[a].n * tree_.nodes[a].n;
+    if (compress_) qkey[a][Key()] = off;
+    qent[a].push_back({off, -1});
+    off += nn(a);
+    for (int l = 0; l < L; ++l) {
+      if (!active_[l][a]) continue;
+      if (compress_) {
+        Key ki = restrict_in(l, a), ko = restrict_out(l, a);
+        auto hi = hkey[a].find(ki);
+        if (hi == hkey[a].end()) {
+          hkey[a][ki] = off;
+          hent[a].push_back({off, l});
+          hslot_[l][a] = off;
+          off += nn(a);
+        } else {
+          hslot_[l][a] = hi->second;
+        }
+        auto qi = qkey[a].find(ko);
+        if (qi == qkey[a].end()) {
+          qkey[a][ko] = off;
+          qent[a].push_back({off, l});
+          holeslot_[l][a] = off;
+          off += nn(a);
+        } else {
+          holeslot_[l][a] = qi->second;
+        }
+      } else {
+        hslot_[l][a] = off;
+        hent[a].push_back({off, l});
+        off += nn(a);
+        holeslot_[l][a] = off;
+        qent[a].push_back({off, l});
+        off += nn(a);
+      }
+    }
   }
-  poolSize_ = off;
   int64_t off64 = 0;
   rhoinv_.assign(NN, -1);
   for (int a = 0; a < NN; ++a) {
     rhoinv_[a] = off64;
-    off64 += (int64_t)tree_.nodes[a].n * tree_.nodes[a].n;
+    off64 += nn(a);
   }
   pool64Size_ = off64;
 
@@ -165,6 +227,7 @@ int Engine::plan() {
   };
   flops_ = 0.0;
   layerFlops_.clear();
+  layerNmax_.clear();
   for (auto& x : phaseFlops_) x = 0.0;
   double mark = 0.0;
   // ---- represent: bottom-up by height
@@ -172,14 +235,15 @@ int Engine::plan() {
   repRange_.assign(tree_.maxHeight, {0, 0});
   for (int h = 0; h < tree_.maxHeight; ++h) {
     repRange_[h].first = (int64_t)items.size();
-    for (int l = 0; l < L; ++l)
-      for (auto& v : tree_.nodes) {
-        if (v.height != h || v.top() || !active_[l][v.addr]) continue;
+    for (auto& v : tree_.nodes) {
+      if (v.height != h || v.top()) continue;
+      for (auto& e : hent[v.addr]) {
+        const int l = e.second;
         SwItem it{};
         it.node = v.addr;
         it.k = v.order() - 1;
         it.bra = it.ket = v.off;
-        it.out = hslot_[l][v.addr];
+        it.out = e.first;
         if (v.bottom) {
           it.fmode[it.nf] = 0;
           it.fmat[it.nf++] = leafmat(l, v.leaf.mode);
@@ -195,7 +259,11 @@ int Engine::plan() {
         flops_ += 8.0 * v.N * v.n;
         items.push_back(it);
       }
+    }
     repRange_[h].second = (int64_t)items.size();
+    layerNmax_.push_back(256);
+    for (int64_t i = repRange_[h].first; i < repRange_[h].second; ++i)
+      layerNmax_.back() = std::max(layerNmax_.back(), tree_.nodes[items[i].node].N);
     layerFlops_.push_back(flops_ - mark);
     mark = flops_;
   }
@@ -206,43 +274,33 @@ int Engine::plan() {
     for (auto& v : tree_.nodes) {
       if (v.depth != dd) continue;
       const HNode& p = tree_.nodes[v.parent];
-      SwItem it{};
-      it.node = p.addr;
-      it.k = v.childIdx;
-      it.bra = it.ket = p.off;
-      it.out = rhoslot_[v.addr];
-      if (!p.top()) {
-        it.fmode[it.nf] = p.order() - 1;
-        it.fmat[it.nf++] = rhoslot_[p.addr];
-        flops_ += 8.0 * p.N * p.n;
-      }
-      flops_ += 8.0 * p.N * v.n;
-      items.push_back(it);
-    }
-    for (int l = 0; l < L; ++l)
-      for (auto& v : tree_.nodes) {
-        if (v.depth != dd || !active_[l][v.addr]) continue;
-        const HNode& p = tree_.nodes[v.parent];
+      for (auto& e : qent[v.addr]) {
+        const int l = e.second;  // -1: density matrix
         SwItem it{};
         it.node = p.addr;
         it.k = v.childIdx;
         it.bra = it.ket = p.off;
-        it.out = holeslot_[l][v.addr];
-        for (int s : p.children)
-          if (s != v.addr && active_[l][s]) {
-            it.fmode[it.nf] = tree_.nodes[s].childIdx;
-            it.fmat[it.nf++] = hslot_[l][s];
-            flops_ += 8.0 * p.N * tree_.nodes[s].n;
-          }
+        it.out = e.first;
+        if (l >= 0)
+          for (int sb : p.children)
+            if (sb != v.addr && active_[l][sb]) {
+              it.fmode[it.nf] = tree_.nodes[sb].childIdx;
+              it.fmat[it.nf++] = hslot_[l][sb];
+              flops_ += 8.0 * p.N * tree_.nodes[sb].n;
+            }
         if (!p.top()) {
           it.fmode[it.nf] = p.order() - 1;
-          it.fmat[it.nf++] = holeslot_[l][p.addr];
+          it.fmat[it.nf++] = (l >= 0) ? holeslot_[l][p.addr] : rhoslot_[p.addr];
           flops_ += 8.0 * p.N * p.n;
         }
         flops_ += 8.0 * p.N * v.n;
         items.push_back(it);
       }
+    }
     holeRange_[dd].second = (int64_t)items.size();
+    layerNmax_.push_back(256);
+    for (int64_t i = holeRange_[dd].first; i < holeRange_[dd].second; ++i)
+      layerNmax_.back() = std::max(layerNmax_.back(), tree_.nodes[items[i].node].N);
     layerFlops_.push_back(flops_ - mark);
     mark = flops_;
   }
@@ -263,10 +321,17 @@ int Engine::plan() {
     e.ev = -1;
     eig.push_back(e);
   }
-  // ---- Apply at every node, in term chunks
+  // ---- Apply at every node.  Each active term is a Kronecker product over
+  // the node's slots (children or leaf operator, then the hole on the last
+  // mode).  With compression, terms that agree on all slots but one are
+  // merged into one summand whose free slot is the coefficient-weighted sum
+  // of their matrices (greedy, largest group first).
   std::vector<ApTerm> terms;
   std::vector<ApChunk> chunks;
   std::vector<FinNode> fin;
+  std::vector<CombOut> combOut;
+  std::vector<CombTerm> combTerm;
+  double combFlops = 0.0;
   int64_t part = 0;
   for (auto& v : tree_.nodes) {
     FinNode fn{};
@@ -274,33 +339,106 @@ int Engine::plan() {
     fn.top = v.top() ? 1 : 0;
     fn.part0 = part;
     fn.rhoinv = rhoinv_[v.addr];
-    std::vector<ApTerm> vt;
+    // slots of node v
+    std::vector<int> smode, sdim;
+    if (v.bottom) {
+      smode.push_back(0);
+      sdim.push_back(v.dims[0]);
+    } else {
+      for (int c : v.children) {
+        smode.push_back(tree_.nodes[c].childIdx);
+        sdim.push_back(tree_.nodes[c].n);
+      }
+    }
+    if (!v.top()) {
+      smode.push_back(v.order() - 1);
+      sdim.push_back(v.n);
+    }
+    const int S = (int)smode.size();
+    struct G {
+      std::vector<int64_t> ids;
+      cd coef;
+    };
+    std::vector<G> tl;
     for (int l = 0; l < L; ++l) {
       if (!active_[l][v.addr]) continue;
-      ApTerm t{};
-      t.cre = sop_.terms[l].coeff.real();
-      t.cim = sop_.terms[l].coeff.imag();
+      G g;
+      g.coef = sop_.terms[l].coeff;
       if (v.bottom) {
-        t.fmode[t.nf] = 0;
-        t.fmat[t.nf++] = leafmat(l, v.leaf.mode);
-        flops_ += 8.0 * v.N * v.dims[0];
+        g.ids.push_back(leafmat(l, v.leaf.mode));
       } else {
-        for (int c : v.children)
-          if (active_[l][c]) {
-            t.fmode[t.nf] = tree_.nodes[c].childIdx;
-            t.fmat[t.nf++] = hslot_[l][c];
-            flops_ += 8.0 * v.N * tree_.nodes[c].n;
-          }
+        for (int c : v.children) g.ids.push_back(active_[l][c] ? hslot_[l][c] : -1);
       }
-      if (!v.top()) {
-        t.fmode[t.nf] = v.order() - 1;
-        t.fmat[t.nf++] = holeslot_[l][v.addr];
-        flops_ += 8.0 * v.N * v.n;
+      if (!v.top()) g.ids.push_back(holeslot_[l][v.addr]);
+      tl.push_back(g);
+    }
+    std::vector<G> groups;
+    if (compress_) {
+      std::vector<char> alive(tl.size(), 1);
+      size_t nalive = tl.size();
+      while (nalive > 1) {
+        int best_s = -1;
+        std::vector<int> best;
+        for (int sl = 0; sl < S; ++sl) {
+          std::map<std::vector<int64_t>, std::vector<int>> gm;
+          for (size_t t = 0; t < tl.size(); ++t) {
+            if (!alive[t]) continue;
+            std::vector<int64_t> k = tl[t].ids;
+            k.erase(k.begin() + sl);
+            gm[k].push_back((int)t);
+          }
+          for (auto& kv : gm)
+            if (kv.second.size() > std::max<size_t>(1, best.size())) {
+              best = kv.second;
+              best_s = sl;
+            }
+        }
+        if (best_s < 0) break;
+        // new aggregated matrix for slot best_s
+        const int64_t d = sdim[best_s];
+        CombOut co{};
+        co.out = off;
+        co.n = (int32_t)d;
+        co.t0 = (int32_t)combTerm.size();
+        for (int t : best) {
+          CombTerm ct{};
+          ct.cre = tl[t].coef.real();
+          ct.cim = tl[t].coef.imag();
+          ct.src = tl[t].ids[best_s];
+          combTerm.push_back(ct);
+          alive[t] = 0;
+          --nalive;
+        }
+        co.t1 = (int32_t)combTerm.size();
+        combOut.push_back(co);
+        combFlops += 8.0 * d * d * best.size();
+        G g;
+        g.ids = tl[best[0]].ids;
+        g.ids[best_s] = off;
+        g.coef = cd(1.0, 0.0);
+        off += d * d;
+        groups.push_back(g);
+      }
+      for (size_t t = 0; t < tl.size(); ++t)
+        if (alive[t]) groups.push_back(tl[t]);
+    } else {
+      groups = tl;
+    }
+    std::vector<ApTerm> vt;
+    for (auto& g : groups) {
+      ApTerm t{};
+      t.cre = g.coef.real();
+      t.cim = g.coef.imag();
+      for (int sl = 0; sl < S; ++sl) {
+        if (g.ids[sl] < 0) continue;
+        t.fmode[t.nf] = smode[sl];
+        t.fmat[t.nf++] = g.ids[sl];
+        flops_ += 8.0 * v.N * sdim[sl];
       }
       vt.push_back(t);
     }
     if (!v.top()) flops_ += 4 * 8.0 * v.N * v.n;  // projector and rho^-1
-    const int per = chunkTerms_;
+    const int per = compress_ ? chunkTermsCompressed_ : chunkTerms_;
     int nch = 0;
     for (size_t t0 = 0; t0 < vt.size() || (t0 == 0 && vt.empty()); t0 += per) {
       ApChunk c{};
@@ -317,16 +455,18 @@ int Engine::plan() {
     fn.nchunk = nch;
     fin.push_back(fn);
   }
+  poolSize_ = off;
   nChunks_ = (int64_t)chunks.size();
   nFin_ = (int64_t)fin.size();
+  nComb_ = (int64_t)combOut.size();
   {
-    // apply: everything counted after the sandwich phase except the
-    // projector / rho^-1 work of the finish kernel
     double proj = 0.0;
     for (auto& v : tree_.nodes)
       if (!v.top()) proj += 4 * 8.0 * v.N * v.n;
     phaseFlops_[2] = flops_ - phaseFlops_[0] - proj;
     phaseFlops_[3] = proj;
+    phaseFlops_[5] = combFlops;
+    flops_ += combFlops;
   }
   nEig_ = (int64_t)eig.size();
   partialSize_ = part;
@@ -352,6 +492,10 @@ int Engine::plan() {
   device_buffers_.push_back(d_chunks_);
   if ((rc = up(&d_fin_, fin))) return rc;
   device_buffers_.push_back(d_fin_);
+  if ((rc = up(&d_combOut_, combOut))) return rc;
+  device_buffers_.push_back(d_combOut_);
+  if ((rc = up(&d_combTerm_, combTerm))) return rc;
+  device_buffers_.push_back(d_combTerm_);
   const size_t es = c64_ ? 8 : 16;
   if ((rc = alloc(&d_pool_, es * poolSize_))) return rc;
   if ((rc = alloc((void**)&d_pool64_, 16 * pool64Size_))) return rc;
@@ -456,11 +600,12 @@ int Engine::build_t(hipStream_t s) {
   size_t li = 0;
   auto sw = [&](const std::pair<int64_t, int64_t>& r) -> int {
     const double fl = li < layerFlops_.size() ? layerFlops_[li] : 0.0;
+    const int lnmax = li < layerNmax_.size() ? layerNmax_[li] : nmax_;
     ++li;
     if (r.second == r.first) return QTG_OK;
     int e = begin_mark(0, s);
     if (e) return e;
-    if ((e = launch_sandwich<T>(d_sw_ + r.first, r.second - r.first, d_nodes_, psi, psi, d_pool_, nmax_, s)))
+    if ((e = launch_sandwich<T>(d_sw_ + r.first, r.second - r.first, d_nodes_, psi, psi, d_pool_, lnmax, s)))
       return e;
     if (timing_) tfl_[0] += fl;
     return end_mark(s);
@@ -470,7 +615,7 @@ int Engine::build_t(hipStream_t s) {
   for (size_t dd = 1; dd < holeRange_.size(); ++dd)
     if ((rc = sw(holeRange_[dd]))) return rc;
   if ((rc = begin_mark(1, s))) return rc;
-  if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, s))) return rc;
+  if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, s))) return rc;
   if ((rc = end_mark(s))) return rc;
   built_ = true;
   return QTG_OK;
@@ -487,7 +632,14 @@ int Engine::build(hipStream_t s) {
 template <typename T>
 int Engine::apply_t(hipStream_t s, double ph_re, double ph_im) {
   const void* psi = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
-  int rc = begin_mark(2, s);
+  int rc;
+  if (nComb_ > 0) {
+    if ((rc = begin_mark(5, s))) return rc;
+    if ((rc = launch_combine<T>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
+    if ((rc = end_mark(s))) return rc;
+    if (timing_) tfl_[5] += phaseFlops_[5];
+  }
+  rc = begin_mark(2, s);
   if (rc) return rc;
   if ((rc = launch_apply<T>(d_chunks_, nChunks_, d_terms_, d_nodes_, psi, d_pool_, d_partial_, nmax_, s))) return rc;
   if ((rc = end_mark(s))) return rc;

@@ -1,5 +1,6 @@
 // Device-resident tree engine (see engine.cpp).
 #pragma once
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -39,9 +40,11 @@ class Engine {
   int64_t apply_terms() const { return (int64_t)h_terms_.size(); }
   double eps_ = 1e-10;
   int chunkTerms_ = 32;
+  int chunkTermsCompressed_ = 4;
+  bool compress_ = true;  // share equal h/hole matrices, aggregate Apply summands
   // Per-kernel timing with HIP events recorded on the launch stream around
-  // every launch group: 0 sandwich, 1 heev, 2 apply, 3 finish, 4 convert.
-  static constexpr int kPhases = 5;
+  // every launch group: 0 sandwich, 1 heev, 2 apply, 3 finish, 4 convert, 5 combine.
+  static constexpr int kPhases = 6;
   int set_timing(bool on);
   int read_timing(double* ms, int64_t* launches, double* flops);
 
@@ -52,6 +55,9 @@ class Engine {
   int apply_t(hipStream_t s, double ph_re, double ph_im);
   int plan();
   int materialize();
+  std::vector<std::pair<int, int>> restrict_in(int l, int a) const;
+  std::vector<std::pair<int, int>> restrict_out(int l, int a) const;
+  std::vector<std::vector<char>> inSub_;  // node x mode: leaf below the node
   void active_sets();
   int alloc(void** p, size_t bytes);
   void release();
@@ -70,7 +76,7 @@ class Engine {
   std::vector<std::vector<int64_t>> hslot_, holeslot_;
   std::vector<std::pair<int64_t, int64_t>> repRange_, holeRange_;
   int64_t poolSize_ = 0, pool64Size_ = 0, partialSize_ = 0;
-  int64_t nSw_ = 0, nChunks_ = 0, nFin_ = 0, nEig_ = 0;
+  int64_t nSw_ = 0, nChunks_ = 0, nFin_ = 0, nEig_ = 0, nComb_ = 0;
   double flops_ = 0.0;
 
   DNode* d_nodes_ = nullptr;
@@ -79,6 +85,8 @@ class Engine {
   ApTerm* d_terms_ = nullptr;
   ApChunk* d_chunks_ = nullptr;
   FinNode* d_fin_ = nullptr;
+  CombOut* d_combOut_ = nullptr;
+  CombTerm* d_combTerm_ = nullptr;
   double* d_psi64_ = nullptr;
   double* d_dpsi64_ = nullptr;
   double* d_apply64_ = nullptr;
@@ -101,6 +109,7 @@ class Engine {
   int64_t tn_[kPhases] = {0};
   double phaseFlops_[kPhases] = {0};
   std::vector<double> layerFlops_;  // flops of each sandwich launch, in launch order
+  std::vector<int> layerNmax_;      // largest node tensor of each sandwich launch
   int begin_mark(int phase, hipStream_t s);
   int end_mark(hipStream_t s);
   int collect_marks();

@@ -73,11 +73,12 @@ __device__ void contract_k(cplx<T>* __restrict__ out, const cplx<T>* X, const cp
     for (int o = threadIdx.x; o < outs; o += TB) {
       const int j = o % nk, l = o / nk;
       cplx<T> acc = czero<T>();
-      for (int r = 0; r < R; ++r) {
-        const int b = r % bef, a = r / bef;
-        const int base = b + a * bef * nk;
-        cfmac(acc, X[base + j * bef], Y[base + l * bef]);
-      }
+      const int jo = j * bef, lo = l * bef, abn = bef * nk;
+      for (int a = 0; a < aft; ++a)
+        for (int b = 0; b < bef; ++b) {
+          const int base = b + a * abn;
+          cfmac(acc, X[base + jo], Y[base + lo]);
+        }
       if (accumulate) acc = cadd(acc, out[o]);
       out[o] = acc;
     }
@@ -87,11 +88,20 @@ __device__ void contract_k(cplx<T>* __restrict__ out, const cplx<T>* X, const cp
   const int o = threadIdx.x % outs, g = threadIdx.x / outs;
   const int j = o % nk, l = o / nk;
   cplx<T> acc = czero<T>();
-  if (g < G) {
+  if (g < G && g < R) {
+    // rows r = g, g+G, ...; (b, a) = (r mod bef, r / bef) advanced incrementally
+    int b = g % bef, a = g / bef;
+    const int db = G % bef, da = G / bef;
+    const int jo = j * bef, lo = l * bef, abn = bef * nk;
     for (int r = g; r < R; r += G) {
-      const int b = r % bef, a = r / bef;
-      const int base = b + a * bef * nk;
-      cfmac(acc, X[base + j * bef], Y[base + l * bef]);
+      const int base = b + a * abn;
+      cfmac(acc, X[base + jo], Y[base + lo]);
+      b += db;
+      a += da;
+      if (b >= bef) {
+        b -= bef;
+        ++a;
+      }
     }
   }
   scratch[threadIdx.x] = acc;
@@ -352,33 +362,44 @@ __global__ void __launch_bounds__(TB)
 }
 
 // --------------------------------------------------------- Jacobi eigensolver
-// Hermitian n x n (n <= 64), column-major in LDS.  Tournament ordering: each
-// step rotates n/2 disjoint (p,q) pairs whose parameters are computed from the
-// same matrix, columns first, then rows.
+// Hermitian n x n (n <= 64), column-major in LDS (leading dimension n+1), one
+// wavefront per matrix.  Tournament (round-robin) ordering: each step rotates
+// up to 32 disjoint (p,q) pairs whose parameters come from the same matrix,
+// columns first, then rows.  Player at position j of step t:
+// j == 0 -> 0, else 1 + (j - 1 + t) mod (m - 1).  A pair is skipped when
+// |a_pq| <= eps_mach sqrt(|a_pp a_qq|) (relative accuracy of Demmel-Veselic;
+// eps_mach = 2.2e-16); sweeps stop when no pair rotates or after 20 sweeps.
+constexpr int EW = 64;
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, EW);
+  return v;
+}
+
 template <typename T>
-__global__ void __launch_bounds__(TB)
+__global__ void __launch_bounds__(EW)
     heev_reginv_kernel(const EigItem* __restrict__ items, const cplx<T>* __restrict__ pool,
-                       cplx<double>* pool64, double eps) {
-  __shared__ cplx<double> A[64 * 65];
-  __shared__ cplx<double> U[64 * 65];
-  __shared__ int perm[64];
+                       cplx<double>* pool64, double eps, int nmaxspf) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ldm = nmaxspf + 1;
+  cplx<double>* A = reinterpret_cast<cplx<double>*>(smem_raw);
+  cplx<double>* U = A + nmaxspf * ldm;
   __shared__ double pc[32], ps[32];
   __shared__ cplx<double> pph[32];
   __shared__ int pp[32], pq[32];
-  __shared__ double red[TB];
   __shared__ double lam[64];
   const EigItem it = items[blockIdx.x];
   const int n = it.n;
-  const int ld = 65;
-  for (int o = threadIdx.x; o < n * n; o += TB) {
+  const int ld = n + 1;
+  for (int o = threadIdx.x; o < n * n; o += EW) {
     const int i = o % n, j = o / n;
     const cplx<T> v = pool[it.in + o];
     A[i + j * ld] = mk<double>((double)v.re, (double)v.im);
     U[i + j * ld] = mk<double>(i == j ? 1.0 : 0.0, 0.0);
   }
-  // Hermitise (the input is Hermitian up to rounding)
   __syncthreads();
-  for (int o = threadIdx.x; o < n * n; o += TB) {
+  // Hermitise (the input is Hermitian up to rounding)
+  for (int o = threadIdx.x; o < n * n; o += EW) {
     const int i = o % n, j = o / n;
     if (i < j) {
       const cplx<double> a = A[i + j * ld], b = A[j + i * ld];
@@ -389,42 +410,17 @@ __global__ void __launch_bounds__(TB)
       A[i + i * ld].im = 0.0;
     }
   }
-  const int m = (n + 1) & ~1;  // even player count (a dummy when n is odd)
-  for (int i = threadIdx.x; i < m; i += TB) perm[i] = i;
   __syncthreads();
+  const int m = (n + 1) & ~1;  // even player count (a dummy when n is odd)
+  const int np = m / 2;
   if (n > 1) {
-    for (int sweep = 0; sweep < 30; ++sweep) {
-      // convergence test: off-diagonal vs diagonal Frobenius norms
-      double off = 0.0, dia = 0.0;
-      for (int o = threadIdx.x; o < n * n; o += TB) {
-        const int i = o % n, j = o / n;
-        const cplx<double> v = A[i + j * ld];
-        const double a = v.re * v.re + v.im * v.im;
-        if (i == j) dia += a;
-        else off += a;
-      }
-      red[threadIdx.x] = off;
-      __syncthreads();
-      for (int s = TB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-      }
-      off = red[0];
-      __syncthreads();
-      red[threadIdx.x] = dia;
-      __syncthreads();
-      for (int s = TB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-      }
-      dia = red[0];
-      __syncthreads();
-      if (off <= 1e-32 * dia || off == 0.0) break;
+    for (int sweep = 0; sweep < 20; ++sweep) {
+      int rotated = 0;
       for (int step = 0; step < m - 1; ++step) {
-        const int np = m / 2;
-        // pair i: perm[i] vs perm[m-1-i]
-        for (int i = threadIdx.x; i < np; i += TB) {
-          int p = perm[i], q = perm[m - 1 - i];
+        for (int i = threadIdx.x; i < np; i += EW) {
+          const int j0 = i, j1 = m - 1 - i;
+          int p = (j0 == 0) ? 0 : 1 + (j0 - 1 + step) % (m - 1);
+          int q = (j1 == 0) ? 0 : 1 + (j1 - 1 + step) % (m - 1);
           if (p > q) {
             const int t = p;
             p = q;
@@ -432,14 +428,12 @@ __global__ void __launch_bounds__(TB)
           }
           pp[i] = p;
           pq[i] = q;
-          pc[i] = 1.0;
           ps[i] = 0.0;
-          pph[i] = mk<double>(1.0, 0.0);
           if (q < n) {
             const cplx<double> apq = A[p + q * ld];
             const double aa = sqrt(apq.re * apq.re + apq.im * apq.im);
-            if (aa > 0.0) {
-              const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
+            const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
+            if (aa > 2.2e-16 * sqrt(fabs(app * aqq)) && aa > 1e-300) {
               const double theta = (aqq - app) / (2.0 * aa);
               const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
               const double c = 1.0 / sqrt(t * t + 1.0);
@@ -450,56 +444,56 @@ __global__ void __launch_bounds__(TB)
           }
         }
         __syncthreads();
-        // columns: A <- A G, U <- U G  (G = [[c, s ph], [-s conj(ph), c]] on (p,q))
-        for (int o = threadIdx.x; o < np * n; o += TB) {
-          const int i = o / n, r = o % n;
-          const int p = pp[i], q = pq[i];
-          if (q >= n || ps[i] == 0.0) continue;
-          const double c = pc[i], s = ps[i];
-          const cplx<double> ph = pph[i];
-          const cplx<double> arp = A[r + p * ld], arq = A[r + q * ld];
-          // new p = c*arp - s*conj(ph)*arq ; new q = s*ph*arp + c*arq
-          A[r + p * ld] = mk<double>(c * arp.re - s * (ph.re * arq.re + ph.im * arq.im),
-                                     c * arp.im - s * (ph.re * arq.im - ph.im * arq.re));
-          A[r + q * ld] = mk<double>(s * (ph.re * arp.re - ph.im * arp.im) + c * arq.re,
-                                     s * (ph.re * arp.im + ph.im * arp.re) + c * arq.im);
-          const cplx<double> urp = U[r + p * ld], urq = U[r + q * ld];
-          U[r + p * ld] = mk<double>(c * urp.re - s * (ph.re * urq.re + ph.im * urq.im),
-                                     c * urp.im - s * (ph.re * urq.im - ph.im * urq.re));
-          U[r + q * ld] = mk<double>(s * (ph.re * urp.re - ph.im * urp.im) + c * urq.re,
-                                     s * (ph.re * urp.im + ph.im * urp.re) + c * urq.im);
-        }
-        __syncthreads();
-        // rows: A <- G^H A  (new p = c*apk - s*ph*aqk ; new q = s*conj(ph)*apk + c*aqk)
-        for (int o = threadIdx.x; o < np * n; o += TB) {
-          const int i = o / n, k = o % n;
-          const int p = pp[i], q = pq[i];
-          if (q >= n || ps[i] == 0.0) continue;
-          const double c = pc[i], s = ps[i];
-          const cplx<double> ph = pph[i];
-          const cplx<double> apk = A[p + k * ld], aqk = A[q + k * ld];
-          A[p + k * ld] = mk<double>(c * apk.re - s * (ph.re * aqk.re - ph.im * aqk.im),
-                                     c * apk.im - s * (ph.re * aqk.im + ph.im * aqk.re));
-          A[q + k * ld] = mk<double>(s * (ph.re * apk.re + ph.im * apk.im) + c * aqk.re,
-                                     s * (ph.re * apk.im - ph.im * apk.re) + c * aqk.im);
-        }
-        __syncthreads();
-        // rotate the tournament: keep perm[0], shift the rest
-        if (threadIdx.x == 0) {
-          const int last = perm[m - 1];
-          for (int i = m - 1; i > 1; --i) perm[i] = perm[i - 1];
-          perm[1] = last;
+        int any = 0;
+        for (int i = 0; i < np; ++i) any |= (ps[i] != 0.0);
+        if (any) {
+          rotated = 1;
+          // columns: A <- A G, U <- U G  (G = [[c, s ph], [-s conj(ph), c]] on (p,q))
+          for (int o = threadIdx.x; o < np * n; o += EW) {
+            const int i = o / n, r = o % n;
+            const double s_ = ps[i];
+            if (s_ == 0.0) continue;
+            const int p = pp[i], q = pq[i];
+            const double c = pc[i];
+            const cplx<double> ph = pph[i];
+            const cplx<double> arp = A[r + p * ld], arq = A[r + q * ld];
+            A[r + p * ld] = mk<double>(c * arp.re - s_ * (ph.re * arq.re + ph.im * arq.im),
+                                       c * arp.im - s_ * (ph.re * arq.im - ph.im * arq.re));
+            A[r + q * ld] = mk<double>(s_ * (ph.re * arp.re - ph.im * arp.im) + c * arq.re,
+                                       s_ * (ph.re * arp.im + ph.im * arp.re) + c * arq.im);
+            const cplx<double> urp = U[r + p * ld], urq = U[r + q * ld];
+            U[r + p * ld] = mk<double>(c * urp.re - s_ * (ph.re * urq.re + ph.im * urq.im),
+                                       c * urp.im - s_ * (ph.re * urq.im - ph.im * urq.re));
+            U[r + q * ld] = mk<double>(s_ * (ph.re * urp.re - ph.im * urp.im) + c * urq.re,
+                                       s_ * (ph.re * urp.im + ph.im * urp.re) + c * urq.im);
+          }
+          __syncthreads();
+          // rows: A <- G^H A  (new p = c*apk - s*ph*aqk ; new q = s*conj(ph)*apk + c*aqk)
+          for (int o = threadIdx.x; o < np * n; o += EW) {
+            const int i = o / n, k = o % n;
+            const double s_ = ps[i];
+            if (s_ == 0.0) continue;
+            const int p = pp[i], q = pq[i];
+            const double c = pc[i];
+            const cplx<double> ph = pph[i];
+            const cplx<double> apk = A[p + k * ld], aqk = A[q + k * ld];
+            A[p + k * ld] = mk<double>(c * apk.re - s_ * (ph.re * aqk.re - ph.im * aqk.im),
+                                       c * apk.im - s_ * (ph.re * aqk.im + ph.im * aqk.re));
+            A[q + k * ld] = mk<double>(s_ * (ph.re * apk.re + ph.im * apk.im) + c * aqk.re,
+                                       s_ * (ph.re * apk.im - ph.im * apk.re) + c * aqk.im);
+          }
         }
         __syncthreads();
       }
+      if (!rotated) break;
     }
   }
-  for (int i = threadIdx.x; i < n; i += TB) lam[i] = A[i + i * ld].re;
+  for (int i = threadIdx.x; i < n; i += EW) lam[i] = A[i + i * ld].re;
   __syncthreads();
   if (it.ev >= 0)
-    for (int i = threadIdx.x; i < n; i += TB) pool64[it.ev + i] = mk<double>(lam[i], 0.0);
+    for (int i = threadIdx.x; i < n; i += EW) pool64[it.ev + i] = mk<double>(lam[i], 0.0);
   // regularised inverse  R(i,j) = sum_k U(i,k) w_k conj(U(j,k))
-  for (int o = threadIdx.x; o < n * n; o += TB) {
+  for (int o = threadIdx.x; o < n * n; o += EW) {
     const int i = o % n, j = o / n;
     cplx<double> acc = czero<double>();
     for (int k = 0; k < n; ++k) {
@@ -513,7 +507,44 @@ __global__ void __launch_bounds__(TB)
   }
 }
 
+// ------------------------------------------------------------ combinations
+template <typename T>
+__global__ void __launch_bounds__(TB)
+    combine_kernel(const CombOut* __restrict__ outs, const CombTerm* __restrict__ terms, cplx<T>* pool) {
+  const CombOut co = outs[blockIdx.x];
+  const int n = co.n;
+  for (int e = threadIdx.x; e < n * n; e += TB) {
+    const int i = e % n, j = e / n;
+    double re = 0.0, im = 0.0;
+    for (int t = co.t0; t < co.t1; ++t) {
+      const CombTerm ct = terms[t];
+      double mr, mi;
+      if (ct.src < 0) {
+        mr = (i == j) ? 1.0 : 0.0;
+        mi = 0.0;
+      } else {
+        const cplx<T> m = pool[ct.src + e];
+        mr = (double)m.re;
+        mi = (double)m.im;
+      }
+      re += ct.cre * mr - ct.cim * mi;
+      im += ct.cre * mi + ct.cim * mr;
+    }
+    pool[co.out + e] = mk<T>((T)re, (T)im);
+  }
+}
+
 // ------------------------------------------------------------------- launches
+template <typename T>
+int launch_combine(const CombOut* outs, int64_t count, const CombTerm* terms, void* pool, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  hipLaunchKernelGGL(combine_kernel<T>, dim3((unsigned)count), dim3(TB), 0, s, outs, terms, (cplx<T>*)pool);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+template int launch_combine<double>(const CombOut*, int64_t, const CombTerm*, void*, hipStream_t);
+template int launch_combine<float>(const CombOut*, int64_t, const CombTerm*, void*, hipStream_t);
+
 template <typename T>
 int launch_sandwich(const SwItem* items, int64_t count, const DNode* nodes, const void* bra, const void* ket,
                     void* pool, int nmax, hipStream_t s) {
@@ -562,10 +593,12 @@ int launch_finish(const FinNode* fin, int64_t count, const DNode* nodes, const v
 }
 
 template <typename T>
-int launch_heev(const EigItem* items, int64_t count, const void* pool, void* pool64, double eps, hipStream_t s) {
+int launch_heev(const EigItem* items, int64_t count, const void* pool, void* pool64, double eps, int nspf,
+                hipStream_t s) {
   if (count == 0) return QTG_OK;
-  hipLaunchKernelGGL(heev_reginv_kernel<T>, dim3((unsigned)count), dim3(TB), 0, s, items, (const cplx<T>*)pool,
-                     (cplx<double>*)pool64, eps);
+  const size_t lds = 2 * (size_t)nspf * (nspf + 1) * sizeof(cplx<double>);
+  hipLaunchKernelGGL(heev_reginv_kernel<T>, dim3((unsigned)count), dim3(EW), lds, s, items, (const cplx<T>*)pool,
+                     (cplx<double>*)pool64, eps, nspf);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -582,7 +615,7 @@ template int launch_finish<double>(const FinNode*, int64_t, const DNode*, const 
                                    void*, void*, double, double, void*, int, int, hipStream_t);
 template int launch_finish<float>(const FinNode*, int64_t, const DNode*, const void*, const void*, const void*,
                                   void*, void*, double, double, void*, int, int, hipStream_t);
-template int launch_heev<double>(const EigItem*, int64_t, const void*, void*, double, hipStream_t);
-template int launch_heev<float>(const EigItem*, int64_t, const void*, void*, double, hipStream_t);
+template int launch_heev<double>(const EigItem*, int64_t, const void*, void*, double, int, hipStream_t);
+template int launch_heev<float>(const EigItem*, int64_t, const void*, void*, double, int, hipStream_t);
 
 }  // namespace qtg

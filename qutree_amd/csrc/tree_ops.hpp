@@ -47,6 +47,16 @@ struct FinNode {
   int64_t rhoinv;  // offset of rho^-1 in the FP64 pool
 };
 
+// out (n x n) = sum_t coef_t M_t, M_t = pool matrix at src (identity if src < 0)
+struct CombOut {
+  int64_t out;
+  int32_t n, t0, t1, pad;
+};
+struct CombTerm {
+  double cre, cim;
+  int64_t src;
+};
+
 struct EigItem {
   int32_t n, pad;
   int64_t in;   // density matrix (bulk pool)
@@ -72,6 +82,9 @@ inline size_t finish_lds_bytes(int nmax, int nspf) {
 inline size_t work_lds_bytes(int nmax, size_t elem) { return 3 * (size_t)nmax * elem; }
 constexpr size_t kLdsBudget = 160 * 1024 - 8 * 1024;  // leave room for static shared arrays
 template <typename T>
-int launch_heev(const EigItem* items, int64_t count, const void* pool, void* pool64, double eps, hipStream_t s);
+int launch_combine(const CombOut* outs, int64_t count, const CombTerm* terms, void* pool, hipStream_t s);
+template <typename T>
+int launch_heev(const EigItem* items, int64_t count, const void* pool, void* pool64, double eps, int nspf,
+                hipStream_t s);
 
 }  // namespace qtg

@@ -36,8 +36,8 @@ def _host():
     L.qtg_host_psi_size.restype = C.c_int64
     L.qtg_host_fill_random.argtypes = [C.c_char_p, C.c_uint64, C.c_int, np.ctypeslib.ndpointer(np.float64)]
     L.qtg_host_plan_counts.argtypes = [C.c_char_p, C.c_int64] + [np.ctypeslib.ndpointer(np.float64)] + \
-        [np.ctypeslib.ndpointer(np.int64)] * 4 + [np.ctypeslib.ndpointer(np.float64), C.POINTER(C.c_double),
-                                                  np.ctypeslib.ndpointer(np.int64)]
+        [np.ctypeslib.ndpointer(np.int64)] * 4 + [np.ctypeslib.ndpointer(np.float64), C.c_int,
+                                                  C.POINTER(C.c_double), np.ctypeslib.ndpointer(np.int64)]
     return L
 
 
@@ -80,7 +80,8 @@ def test_plan_counts_against_independent_count(model):
     c, toff, fm, fk, fp, pool = H.arrays()
     flops = C.c_double()
     items = np.zeros(3, np.int64)
-    assert L.qtg_host_plan_counts(text.encode(), H.nterms, c, toff, fm, fk, fp, pool, C.byref(flops), items) == 0
+    assert L.qtg_host_plan_counts(text.encode(), H.nterms, c, toff, fm, fk, fp, pool, 0, C.byref(flops),
+                                  items) == 0
     # independent: one represent + one hole item per (term, active non-top node),
     # one density item per non-top node, one apply term per (term, active node)
     nodes = m.nodes
@@ -128,3 +129,58 @@ def test_plan_counts_against_independent_count(model):
     assert abs(flops.value - fl) < 1e-9 * fl
     if model == "sweep64":
         assert pairs == 22752  # 64 + 2016 terms over the 64-mode tree
+
+
+def _subtree_modes(nodes):
+    sub = {}
+    for n in nodes:  # children precede parents
+        if n["bottom"]:
+            sub[n["address"]] = {n["mode"]}
+        else:
+            sub[n["address"]] = set().union(*(sub[c] for c in n["children"]))
+    return sub
+
+
+@pytest.mark.parametrize("model", ["cmf8", "sweep64", "quartic16"])
+def test_compressed_plan_counts_distinct_restrictions(model):
+    """With compression one represent item exists per distinct restriction of
+    a term to a node's subtree, and one hole item per distinct restriction to
+    its complement (plus the density matrix, the empty restriction)."""
+    if model == "cmf8":
+        text, H = W.cmf8_model()
+    elif model == "sweep64":
+        text, H = W.sweep64_model()
+    else:
+        text, H = W.balanced_tree_text(16, 8, [4, 6, 6, 8]), W.coupled_quartic(16, 3)
+    m = O.OracleModel(text, H)
+    sub = _subtree_modes(m.nodes)
+    # term factors as (mode, kind, pool values) -> hashable per mode, merged per mode in order
+    terms = []
+    for l in range(H.nterms):
+        per = {}
+        for f in range(H.term_off[l], H.term_off[l + 1]):
+            vals = tuple(H.pool[2 * H.fpool[f]: 2 * H.fpool[f] + 4]) if H.fkind[f] == W.FACTOR_POLY else ()
+            per.setdefault(H.fmode[f], []).append((H.fkind[f], vals))
+        terms.append({k: tuple(v) for k, v in per.items()})
+    nrep = nhole = 0
+    for n in m.nodes:
+        if n["parent"] < 0:
+            continue
+        a = n["address"]
+        ins, outs = set(), {()}
+        for l, t in enumerate(terms):
+            if not (set(t) & sub[a]):
+                continue
+            ins.add(tuple(sorted((k, v) for k, v in t.items() if k in sub[a])))
+            outs.add(tuple(sorted((k, v) for k, v in t.items() if k not in sub[a])))
+        nrep += len(ins)
+        nhole += len(outs)
+    L = _host()
+    c, toff, fm, fk, fp, pool = H.arrays()
+    flops = C.c_double()
+    items = np.zeros(3, np.int64)
+    assert L.qtg_host_plan_counts(text.encode(), H.nterms, c, toff, fm, fk, fp, pool, 1, C.byref(flops),
+                                  items) == 0
+    assert items[0] == nrep + nhole
+    if model == "sweep64":
+        assert nrep + nhole < 45630 // 4

@@ -54,16 +54,17 @@ def derivative_err(eng, om, got, ref, app_ref):
     return worst
 
 
+@pytest.mark.parametrize("compress", [True, False])
 @pytest.mark.parametrize("name", ["f4", "f8quartic", "cmf8", "f16", "sweep64"])
-def test_derivative_matches_oracle(name):
+def test_derivative_matches_oracle(name, compress):
     text, H = _models()[name]
-    print(f"case engine derivative {name}", flush=True)
+    print(f"case engine derivative {name} compress={compress}", flush=True)
     om = O.OracleModel(text, H)
     psi = om.fill_random(3, delta_lowest=False)
     ref = om.derivative(psi)
     e_ref = om.energy(psi)
     app_ref = om.apply_all(psi)
-    eng = Q.Engine(text, H)
+    eng = Q.Engine(text, H, compress=compress)
     eng.set_psi(psi)
     eng.derivative()
     got = eng.get(0)
@@ -74,14 +75,15 @@ def test_derivative_matches_oracle(name):
     assert abs(eng.energy() - e_ref) < 1e-12 * abs(e_ref)
 
 
+@pytest.mark.parametrize("compress", [True, False])
 @pytest.mark.parametrize("name", ["f4", "cmf8", "f16"])
-def test_representation_matrices_match_oracle(name):
+def test_representation_matrices_match_oracle(name, compress):
     text, H = _models()[name]
-    print(f"case engine matrices {name}", flush=True)
+    print(f"case engine matrices {name} compress={compress}", flush=True)
     om = O.OracleModel(text, H)
     psi = om.fill_random(4, delta_lowest=False)
     om.build(psi)
-    eng = Q.Engine(text, H)
+    eng = Q.Engine(text, H, compress=compress)
     eng.set_psi(psi)
     eng.build()
     Q.synchronize()
