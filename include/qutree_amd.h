@@ -14,7 +14,9 @@
  *    their descriptor copies and split-K workspace, allocated at creation).
  *  - Every call returns a qtg_status; nothing calls exit().  qtg_last_error()
  *    returns a description of the most recent failure on the calling thread.
- *  - stream == NULL means the library's default stream of the current device.
+ *  - stream == NULL means the legacy default stream for node-level calls and
+ *    the engine's own non-blocking stream for qtg_engine_* calls;
+ *    qtg_synchronize(NULL) waits for the whole device.
  */
 #ifndef QUTREE_AMD_H
 #define QUTREE_AMD_H
@@ -185,13 +187,17 @@ int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest,
  * launch counts and algorithmic FLOPs since set_timing(1).  Arrays hold 6
  * entries. */
 int qtg_engine_set_timing(qtg_engine engine, int on);
+/* waits for the engine's own stream */
+int qtg_engine_synchronize(qtg_engine engine);
 int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, double* flops);
 /* "rho_eps": regularisation of the density inverse (default 1e-10).
  * "compress" (set before qtg_engine_set_hamiltonian; default 1): compute each
  * distinct h-matrix / hole matrix once (they depend only on the term's
  * factors inside / outside the node's subtree) and merge Apply summands that
  * share all slots but one; 0 keeps one work item per (term, node) as the
- * reference loops do. */
+ * reference loops do.
+ * "graph" (default 1): Derivative() on the engine's stream is captured once
+ * into a HIP graph and replayed; 0 launches eagerly. */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
 
 #ifdef __cplusplus

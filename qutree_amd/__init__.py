@@ -77,6 +77,7 @@ def lib():
     L.qtg_engine_counts.argtypes = [_vp, C.POINTER(C.c_double), _lp]
     L.qtg_engine_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
     L.qtg_engine_set_timing.argtypes = [_vp, C.c_int]
+    L.qtg_engine_synchronize.argtypes = [_vp]
     L.qtg_engine_read_timing.argtypes = [_vp, _dp, _lp, _dp]
     _lib = L
     return L
@@ -203,12 +204,13 @@ def _ci(z: np.ndarray) -> np.ndarray:
 class Engine:
     """Device-resident tree + SOP Hamiltonian (qtg_engine_*)."""
 
-    def __init__(self, tree_text: str, H=None, precision: int = 0, compress: bool = True):
+    def __init__(self, tree_text: str, H=None, precision: int = 0, compress: bool = True, graph: bool = True):
         init()
         p = _vp()
         check(lib().qtg_engine_create(tree_text.encode(), precision, C.byref(p)))
         self.ptr = p.value
         check(lib().qtg_engine_set_option(self.ptr, b"compress", float(compress)))
+        check(lib().qtg_engine_set_option(self.ptr, b"graph", float(graph)))
         self.precision = precision
         self.psi_size = lib().qtg_engine_psi_size(self.ptr)
         self.nnodes = lib().qtg_engine_num_nodes(self.ptr)
@@ -242,7 +244,10 @@ class Engine:
     def derivative(self, phase: complex = 1.0, sync: bool = True) -> None:
         check(lib().qtg_engine_derivative(self.ptr, complex(phase).real, complex(phase).imag, None))
         if sync:
-            synchronize()
+            self.synchronize()
+
+    def synchronize(self) -> None:
+        check(lib().qtg_engine_synchronize(self.ptr))
 
     def layer_derivative_all(self, phase: complex = 1.0) -> None:
         check(lib().qtg_engine_layer_derivative_all(self.ptr, complex(phase).real, complex(phase).imag, None))

@@ -6,6 +6,13 @@
 
 using namespace qtg;
 
+namespace {
+// a NULL stream means the engine's own (non-blocking) stream
+inline hipStream_t ES(qtg_engine e, qtg_stream s) {
+  return s ? S(s) : static_cast<Engine*>(e)->stream();
+}
+}  // namespace
+
 extern "C" {
 
 int qtg_engine_create(const char* tree_text, int precision, qtg_engine* out) {
@@ -67,46 +74,46 @@ int qtg_engine_fill_random(qtg_engine e, uint64_t seed, int delta_lowest, double
 
 int qtg_engine_set_psi(qtg_engine e, const double* psi_host, qtg_stream s) {
   if (!e || !psi_host) return QTG_EARG;
-  return static_cast<Engine*>(e)->set_psi(psi_host, S(s));
+  return static_cast<Engine*>(e)->set_psi(psi_host, ES(e, s));
 }
 
 int qtg_engine_set_psi_device(qtg_engine e, const double* psi_dev, qtg_stream s) {
   if (!e || !psi_dev) return QTG_EARG;
-  return static_cast<Engine*>(e)->set_psi_device(psi_dev, S(s));
+  return static_cast<Engine*>(e)->set_psi_device(psi_dev, ES(e, s));
 }
 
 int qtg_engine_build(qtg_engine e, qtg_stream s) {
   if (!e) return QTG_EARG;
-  return static_cast<Engine*>(e)->build(S(s));
+  return static_cast<Engine*>(e)->build(ES(e, s));
 }
 
 int qtg_engine_derivative(qtg_engine e, double phase_re, double phase_im, qtg_stream s) {
   if (!e) return QTG_EARG;
-  return static_cast<Engine*>(e)->derivative(S(s), phase_re, phase_im);
+  return static_cast<Engine*>(e)->derivative(ES(e, s), phase_re, phase_im);
 }
 
 int qtg_engine_layer_derivative_all(qtg_engine e, double phase_re, double phase_im, qtg_stream s) {
   if (!e) return QTG_EARG;
-  return static_cast<Engine*>(e)->layer_derivative_all(S(s), phase_re, phase_im);
+  return static_cast<Engine*>(e)->layer_derivative_all(ES(e, s), phase_re, phase_im);
 }
 
 int qtg_engine_get_dpsi(qtg_engine e, double* host, int which, qtg_stream s) {
   if (!e || !host) return QTG_EARG;
   auto* en = static_cast<Engine*>(e);
   const double* src = which == 0 ? en->dpsi64() : (which == 1 ? en->apply64() : en->psi64());
-  QTG_HIP_TRY(hipMemcpyAsync(host, src, sizeof(double) * 2 * en->tree().total, hipMemcpyDeviceToHost, S(s)));
-  QTG_HIP_TRY(hipStreamSynchronize(S(s)));
+  QTG_HIP_TRY(hipMemcpyAsync(host, src, sizeof(double) * 2 * en->tree().total, hipMemcpyDeviceToHost, ES(e, s)));
+  QTG_HIP_TRY(hipStreamSynchronize(ES(e, s)));
   return QTG_OK;
 }
 
 int qtg_engine_energy(qtg_engine e, double* energy, qtg_stream s) {
   if (!e || !energy) return QTG_EARG;
-  return static_cast<Engine*>(e)->energy(S(s), energy);
+  return static_cast<Engine*>(e)->energy(ES(e, s), energy);
 }
 
 int qtg_engine_get_matrix(qtg_engine e, int which, int64_t term, int64_t node, double* out, qtg_stream s) {
   if (!e || !out) return QTG_EARG;
-  return static_cast<Engine*>(e)->get_matrix(which, (int)term, (int)node, out, S(s));
+  return static_cast<Engine*>(e)->get_matrix(which, (int)term, (int)node, out, ES(e, s));
 }
 
 int qtg_engine_counts(qtg_engine e, double* sweep_flops, int64_t* items) {
@@ -153,6 +160,12 @@ int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest,
   return QTG_OK;
 }
 
+int qtg_engine_synchronize(qtg_engine e) {
+  if (!e) return QTG_EARG;
+  QTG_HIP_TRY(hipStreamSynchronize(static_cast<Engine*>(e)->stream()));
+  return QTG_OK;
+}
+
 int qtg_engine_set_timing(qtg_engine e, int on) {
   if (!e) return QTG_EARG;
   return static_cast<Engine*>(e)->set_timing(on != 0);
@@ -172,6 +185,11 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   }
   if (!std::strcmp(key, "compress")) {
     en->compress_ = value != 0.0;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "graph")) {
+    en->graph_ = value != 0.0;
+    en->invalidate_graph();
     return QTG_OK;
   }
   set_error(std::string("unknown option ") + key);

@@ -29,8 +29,17 @@ __global__ void to_f32(const double* __restrict__ in, float* __restrict__ out, i
 }  // namespace
 
 Engine::~Engine() {
+  invalidate_graph();
   release();
   for (auto e : evpool_) (void)hipEventDestroy(e);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::invalidate_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_h_) (void)hipGraphDestroy(graph_h_);
+  graph_exec_ = nullptr;
+  graph_h_ = nullptr;
 }
 
 void Engine::release() {
@@ -77,6 +86,8 @@ int Engine::create(const std::string& tree_text, int precision, bool host_only) 
 }
 
 int Engine::materialize() {
+  invalidate_graph();
+  if (!stream_) QTG_HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   std::vector<DNode> dn;
   for (auto& n : tree_.nodes) {
     DNode d{};
@@ -329,8 +340,8 @@ int Engine::plan() {
   std::vector<ApTerm> terms;
   std::vector<ApChunk> chunks;
   std::vector<FinNode> fin;
-  std::vector<CombOut> combOut;
-  std::vector<CombTerm> combTerm;
+  std::vector<CombOut> combOut, combOut2;
+  std::vector<CombTerm> combTerm, combTerm2;
   double combFlops = 0.0;
   int64_t part = 0;
   for (auto& v : tree_.nodes) {
@@ -395,28 +406,56 @@ int Engine::plan() {
         }
         if (best_s < 0) break;
         // new aggregated matrix for slot best_s
+        // Long sums are split into partial sums of <= kCombChunk terms
+        // (first launch) that a second launch adds, so no workgroup walks a
+        // long serial chain of loads.
         const int64_t d = sdim[best_s];
-        CombOut co{};
-        co.out = off;
-        co.n = (int32_t)d;
-        co.t0 = (int32_t)combTerm.size();
-        for (int t : best) {
-          CombTerm ct{};
-          ct.cre = tl[t].coef.real();
-          ct.cim = tl[t].coef.imag();
-          ct.src = tl[t].ids[best_s];
-          combTerm.push_back(ct);
-          alive[t] = 0;
-          --nalive;
+        const size_t kCombChunk = 32;
+        std::vector<int64_t> parts;
+        for (size_t c0 = 0; c0 < best.size(); c0 += kCombChunk) {
+          CombOut co{};
+          co.out = off;
+          co.n = (int32_t)d;
+          co.t0 = (int32_t)combTerm.size();
+          for (size_t q = c0; q < std::min(best.size(), c0 + kCombChunk); ++q) {
+            const int t = best[q];
+            CombTerm ct{};
+            ct.cre = tl[t].coef.real();
+            ct.cim = tl[t].coef.imag();
+            ct.src = tl[t].ids[best_s];
+            combTerm.push_back(ct);
+            alive[t] = 0;
+            --nalive;
+          }
+          co.t1 = (int32_t)combTerm.size();
+          combOut.push_back(co);
+          parts.push_back(off);
+          off += d * d;
         }
-        co.t1 = (int32_t)combTerm.size();
-        combOut.push_back(co);
         combFlops += 8.0 * d * d * best.size();
+        int64_t result = parts[0];
+        if (parts.size() > 1) {
+          CombOut co{};
+          co.out = off;
+          co.n = (int32_t)d;
+          co.t0 = (int32_t)combTerm2.size();
+          for (int64_t pidx : parts) {
+            CombTerm ct{};
+            ct.cre = 1.0;
+            ct.cim = 0.0;
+            ct.src = pidx;
+            combTerm2.push_back(ct);
+          }
+          co.t1 = (int32_t)combTerm2.size();
+          combOut2.push_back(co);
+          combFlops += 8.0 * d * d * parts.size();
+          result = off;
+          off += d * d;
+        }
         G g;
         g.ids = tl[best[0]].ids;
-        g.ids[best_s] = off;
+        g.ids[best_s] = result;
         g.coef = cd(1.0, 0.0);
-        off += d * d;
         groups.push_back(g);
       }
       for (size_t t = 0; t < tl.size(); ++t)
@@ -459,6 +498,7 @@ int Engine::plan() {
   nChunks_ = (int64_t)chunks.size();
   nFin_ = (int64_t)fin.size();
   nComb_ = (int64_t)combOut.size();
+  nComb2_ = (int64_t)combOut2.size();
   {
     double proj = 0.0;
     for (auto& v : tree_.nodes)
@@ -496,6 +536,10 @@ int Engine::plan() {
   device_buffers_.push_back(d_combOut_);
   if ((rc = up(&d_combTerm_, combTerm))) return rc;
   device_buffers_.push_back(d_combTerm_);
+  if ((rc = up(&d_combOut2_, combOut2))) return rc;
+  device_buffers_.push_back(d_combOut2_);
+  if ((rc = up(&d_combTerm2_, combTerm2))) return rc;
+  device_buffers_.push_back(d_combTerm2_);
   const size_t es = c64_ ? 8 : 16;
   if ((rc = alloc(&d_pool_, es * poolSize_))) return rc;
   if ((rc = alloc((void**)&d_pool64_, 16 * pool64Size_))) return rc;
@@ -636,6 +680,7 @@ int Engine::apply_t(hipStream_t s, double ph_re, double ph_im) {
   if (nComb_ > 0) {
     if ((rc = begin_mark(5, s))) return rc;
     if ((rc = launch_combine<T>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
+    if ((rc = launch_combine<T>(d_combOut2_, nComb2_, d_combTerm2_, d_pool_, s))) return rc;
     if ((rc = end_mark(s))) return rc;
     if (timing_) tfl_[5] += phaseFlops_[5];
   }
@@ -660,10 +705,44 @@ int Engine::layer_derivative_all(hipStream_t s, double ph_re, double ph_im) {
   return c64_ ? apply_t<float>(s, ph_re, ph_im) : apply_t<double>(s, ph_re, ph_im);
 }
 
-int Engine::derivative(hipStream_t s, double ph_re, double ph_im) {
+int Engine::derivative_eager(hipStream_t s, double ph_re, double ph_im) {
   int rc = build(s);
   if (rc) return rc;
   return layer_derivative_all(s, ph_re, ph_im);
+}
+
+// One sweep = a fixed sequence of ~20 launches on fixed buffers, so it is
+// captured once into a HIP graph and replayed (the phase is baked into the
+// finish launch; a new phase re-captures).  Eager launches are used while
+// kernel timing is on.
+int Engine::derivative(hipStream_t s, double ph_re, double ph_im) {
+  if (!planned_) {
+    set_error("engine: no Hamiltonian");
+    return QTG_ESTATE;
+  }
+  if (!graph_ || timing_ || s == nullptr) return derivative_eager(s, ph_re, ph_im);
+  if (graph_exec_ && (ph_re != graph_ph_re_ || ph_im != graph_ph_im_)) invalidate_graph();
+  if (!graph_exec_) {
+    QTG_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = derivative_eager(s, ph_re, ph_im);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error(std::string("graph capture: ") + hipGetErrorString(e));
+      return QTG_EDEVICE;
+    }
+    graph_h_ = g;
+    QTG_HIP_TRY(hipGraphInstantiate(&graph_exec_, graph_h_, nullptr, nullptr, 0));
+    graph_ph_re_ = ph_re;
+    graph_ph_im_ = ph_im;
+  }
+  QTG_HIP_TRY(hipGraphLaunch(graph_exec_, s));
+  built_ = true;
+  return QTG_OK;
 }
 
 int Engine::energy(hipStream_t s, double* out) {

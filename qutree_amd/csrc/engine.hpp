@@ -42,6 +42,9 @@ class Engine {
   int chunkTerms_ = 32;
   int chunkTermsCompressed_ = 4;
   bool compress_ = true;  // share equal h/hole matrices, aggregate Apply summands
+  bool graph_ = true;     // replay Derivative() from a captured HIP graph
+  hipStream_t stream() const { return stream_; }
+  void invalidate_graph();
   // Per-kernel timing with HIP events recorded on the launch stream around
   // every launch group: 0 sandwich, 1 heev, 2 apply, 3 finish, 4 convert, 5 combine.
   static constexpr int kPhases = 6;
@@ -76,7 +79,7 @@ class Engine {
   std::vector<std::vector<int64_t>> hslot_, holeslot_;
   std::vector<std::pair<int64_t, int64_t>> repRange_, holeRange_;
   int64_t poolSize_ = 0, pool64Size_ = 0, partialSize_ = 0;
-  int64_t nSw_ = 0, nChunks_ = 0, nFin_ = 0, nEig_ = 0, nComb_ = 0;
+  int64_t nSw_ = 0, nChunks_ = 0, nFin_ = 0, nEig_ = 0, nComb_ = 0, nComb2_ = 0;
   double flops_ = 0.0;
 
   DNode* d_nodes_ = nullptr;
@@ -87,6 +90,8 @@ class Engine {
   FinNode* d_fin_ = nullptr;
   CombOut* d_combOut_ = nullptr;
   CombTerm* d_combTerm_ = nullptr;
+  CombOut* d_combOut2_ = nullptr;
+  CombTerm* d_combTerm2_ = nullptr;
   double* d_psi64_ = nullptr;
   double* d_dpsi64_ = nullptr;
   double* d_apply64_ = nullptr;
@@ -96,6 +101,11 @@ class Engine {
   void* d_partial_ = nullptr;
   double* d_energy_ = nullptr;
   std::vector<void*> device_buffers_;
+  hipStream_t stream_ = nullptr;
+  hipGraph_t graph_h_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  double graph_ph_re_ = 0.0, graph_ph_im_ = 0.0;
+  int derivative_eager(hipStream_t s, double ph_re, double ph_im);
   // timing state
   bool timing_ = false;
   struct Mark {

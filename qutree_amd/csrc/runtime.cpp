@@ -74,7 +74,9 @@ int qtg_stream_destroy(qtg_stream stream) {
 }
 
 int qtg_synchronize(qtg_stream stream) {
-  QTG_HIP_TRY(hipStreamSynchronize(S(stream)));
+  // NULL: wait for all work on the device, including engine-owned streams
+  if (stream) QTG_HIP_TRY(hipStreamSynchronize(S(stream)));
+  else QTG_HIP_TRY(hipDeviceSynchronize());
   return QTG_OK;
 }
 

@@ -515,22 +515,36 @@ __global__ void __launch_bounds__(TB)
   const int n = co.n;
   for (int e = threadIdx.x; e < n * n; e += TB) {
     const int i = e % n, j = e / n;
-    double re = 0.0, im = 0.0;
-    for (int t = co.t0; t < co.t1; ++t) {
+    const double id = (i == j) ? 1.0 : 0.0;
+    double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+    int t = co.t0;
+    // four independent partial sums keep four loads in flight
+    for (; t + 4 <= co.t1; t += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const CombTerm ct = terms[t + u];
+        double mr = id, mi = 0.0;
+        if (ct.src >= 0) {
+          const cplx<T> m = pool[ct.src + e];
+          mr = (double)m.re;
+          mi = (double)m.im;
+        }
+        re[u] += ct.cre * mr - ct.cim * mi;
+        im[u] += ct.cre * mi + ct.cim * mr;
+      }
+    }
+    for (; t < co.t1; ++t) {
       const CombTerm ct = terms[t];
-      double mr, mi;
-      if (ct.src < 0) {
-        mr = (i == j) ? 1.0 : 0.0;
-        mi = 0.0;
-      } else {
+      double mr = id, mi = 0.0;
+      if (ct.src >= 0) {
         const cplx<T> m = pool[ct.src + e];
         mr = (double)m.re;
         mi = (double)m.im;
       }
-      re += ct.cre * mr - ct.cim * mi;
-      im += ct.cre * mi + ct.cim * mr;
+      re[0] += ct.cre * mr - ct.cim * mi;
+      im[0] += ct.cre * mi + ct.cim * mr;
     }
-    pool[co.out + e] = mk<T>((T)re, (T)im);
+    pool[co.out + e] = mk<T>((T)((re[0] + re[1]) + (re[2] + re[3])), (T)((im[0] + im[1]) + (im[2] + im[3])));
   }
 }
 

@@ -154,3 +154,24 @@ def test_repeated_derivative_is_deterministic():
     eng.derivative()
     b = eng.get(0)
     assert np.array_equal(a, b)
+
+
+def test_graph_replays_match_eager():
+    """The captured sweep is replayed several times with new states and eager
+    work in between; every replay must equal the eager result bit for bit."""
+    text, H = W.sweep64_model()
+    print("case engine graph replays sweep64", flush=True)
+    g = Q.Engine(text, H, graph=True)
+    e = Q.Engine(text, H, graph=False)
+    for k, (seed, phase) in enumerate([(1, 1.0), (2, 1.0), (3, 1.0), (3, -1j), (4, -1j)]):
+        psi = e.fill_random(seed, delta_lowest=False)
+        g.set_psi(psi)
+        e.set_psi(psi)
+        g.derivative(phase)
+        e.derivative(phase)
+        a, b = g.get(0), e.get(0)
+        assert np.array_equal(a, b), k
+        assert g.energy() == e.energy()
+        # eager work on the graph engine between replays
+        g.build()
+        g.synchronize()
