@@ -54,6 +54,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", choices=["complex128", "complex64"], default="complex128")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cmf-steps", type=int, default=6, help="CMF steps timed on the 64-mode tree (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -135,6 +136,24 @@ def main() -> int:
         "energy": energy,
         "roofline": roofline,
     }
+    if args.cmf_steps > 0 and args.precision == "complex128":
+        # CMF steps (CMFIntegrator::Integrate iterations: 5 half/quarter
+        # substeps of per-node Bulirsch-Stoer, 2-3 builds, error control,
+        # orthogonal) on the same tree from a displaced Hartree product
+        text_c, H_c = W.cmf64_model()
+        engc = Q.Engine(text_c, H_c, graph=False)
+        psi_c = engc.fill_random(64, delta_lowest=True)
+        cmf = Q.CMF(engc)
+        cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=1)  # warm-up
+        Q.synchronize()
+        c0 = time.perf_counter()
+        r = cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=args.cmf_steps)
+        Q.synchronize()
+        c1 = time.perf_counter()
+        out["cmf"] = {"steps_per_s": round(r["steps"] / (c1 - c0), 3), "steps": r["steps"],
+                      "rejected": r["rejected"], "builds": r["builds"], "node_derivatives": r["derivs"],
+                      "ticks": r["ticks"], "ms_per_step": round((c1 - c0) / r["steps"] * 1e3, 3),
+                      "workload": "64-mode tree, displaced Hartree start (q0 = 1), dt = 0.1, cmf 1e-4, bs 1e-5"}
     if not args.no_cpu:
         base, e_ref = cpu_baseline(text, H, psi, energy)
         out["cpu_baseline"] = base

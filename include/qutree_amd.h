@@ -200,6 +200,27 @@ int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, dou
  * into a HIP graph and replayed; 0 launches eagerly. */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
 
+/* ----------------------------------------------------------- CMF integrator */
+/* CMFIntegrator(H, tree, phase).Integrate(ivars) (src/Applications/
+ * CMFIntegrator.cpp:25-249) on a complex128 engine: constant-mean-field
+ * steps with the reference's step-size control, per-node Bulirsch-Stoer
+ * integration (include/Util/BS_integrator.h:31-256) of LayerDerivative with
+ * the matrices of the last build, CMF error and orthogonal() after accepted
+ * steps.  psi_host (complex128 interleaved, engine layout) is the initial
+ * state on entry and the final state on return.  At t0 and after every
+ * accepted step that reaches the next output time: times, energies
+ * (Expectation / S_top) and autocorr (<Psi(t0)|Psi(t)>, interleaved).
+ * stats (5 entries): steps, rejected steps, builds, node derivative
+ * evaluations, lockstep ticks.  Returns the number of outputs, or a negative
+ * qtg_status. */
+typedef void* qtg_cmf;
+int qtg_cmf_create(qtg_engine engine, qtg_cmf* cmf);
+int qtg_cmf_destroy(qtg_cmf cmf);
+int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend, double dt, double out,
+                          double acc_cmf, double acc_bs, double phase_re, double phase_im,
+                          int64_t max_steps, int64_t max_out, double* times, double* energies,
+                          double* autocorr, int64_t* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,10 @@ def lib():
     L.qtg_engine_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
     L.qtg_engine_set_timing.argtypes = [_vp, C.c_int]
     L.qtg_engine_synchronize.argtypes = [_vp]
+    L.qtg_cmf_create.argtypes = [_vp, C.POINTER(_vp)]
+    L.qtg_cmf_destroy.argtypes = [_vp]
+    L.qtg_cmf_integrate.argtypes = [_vp, _dp] + [C.c_double] * 8 + [_i64, _i64, _dp, _dp, _dp, _lp]
+    L.qtg_cmf_integrate.restype = _i64
     L.qtg_engine_read_timing.argtypes = [_vp, _dp, _lp, _dp]
     _lib = L
     return L
@@ -300,6 +304,40 @@ class Engine:
         try:
             if self.ptr:
                 lib().qtg_engine_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class CMF:
+    """Device CMF integrator bound to a complex128 Engine (qtg_cmf_*)."""
+
+    def __init__(self, engine: Engine):
+        p = _vp()
+        check(lib().qtg_cmf_create(engine.ptr, C.byref(p)))
+        self.ptr = p.value
+        self.engine = engine
+
+    def integrate(self, psi, t0, tend, dt, out, acc_cmf=1e-4, acc_bs=1e-5, phase=1.0, max_steps=10 ** 9,
+                  max_out=100000):
+        p = _ci(psi).copy()
+        times = np.zeros(max_out)
+        en = np.zeros(max_out)
+        ac = np.zeros(2 * max_out)
+        stats = np.zeros(5, np.int64)
+        n = lib().qtg_cmf_integrate(self.ptr, p, t0, tend, dt, out, acc_cmf, acc_bs, complex(phase).real,
+                                    complex(phase).imag, max_steps, max_out, times, en, ac, stats)
+        if n < 0:
+            check(-n)
+        n = min(n, max_out)
+        return dict(psi=p.view(np.complex128).copy(), times=times[:n], energies=en[:n],
+                    autocorr=ac.view(np.complex128)[:n].copy(), steps=int(stats[0]), rejected=int(stats[1]),
+                    builds=int(stats[2]), derivs=int(stats[3]), ticks=int(stats[4]))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qtg_cmf_destroy(self.ptr)
                 self.ptr = None
         except Exception:
             pass

@@ -2,6 +2,7 @@
 #include <cstring>
 #include <vector>
 
+#include "cmf.hpp"
 #include "engine.hpp"
 
 using namespace qtg;
@@ -194,6 +195,67 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   }
   set_error(std::string("unknown option ") + key);
   return QTG_EARG;
+}
+
+// ------------------------------------------------------------------------ CMF
+int qtg_cmf_create(qtg_engine e, qtg_cmf* out) {
+  if (!e || !out) return QTG_EARG;
+  auto* c = new Cmf(*static_cast<Engine*>(e));
+  int rc = c->init();
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return QTG_OK;
+}
+
+int qtg_cmf_destroy(qtg_cmf c) {
+  delete static_cast<Cmf*>(c);
+  return QTG_OK;
+}
+
+int64_t qtg_cmf_integrate(qtg_cmf c, double* psi_host, double t0, double tend, double dt, double out,
+                          double acc_cmf, double acc_bs, double phase_re, double phase_im, int64_t max_steps,
+                          int64_t max_out, double* times, double* energies, double* autocorr, int64_t* stats) {
+  if (!c || !psi_host) {
+    set_error("cmf_integrate: null argument");
+    return -QTG_EARG;
+  }
+  auto* cm = static_cast<Cmf*>(c);
+  const size_t bytes = sizeof(double) * 2 * cm->size();
+  if (hipMemcpyAsync(cm->state(), psi_host, bytes, hipMemcpyHostToDevice, cm->stream()) != hipSuccess ||
+      hipStreamSynchronize(cm->stream()) != hipSuccess) {
+    set_error("cmf_integrate: upload failed");
+    return -QTG_EDEVICE;
+  }
+  cm->ph_re_ = phase_re;
+  cm->ph_im_ = phase_im;
+  CmfResult r;
+  int rc = cm->integrate(t0, tend, dt, out, acc_cmf, acc_bs, max_steps, r);
+  if (rc) return -rc;
+  if (hipMemcpyAsync(psi_host, cm->state(), bytes, hipMemcpyDeviceToHost, cm->stream()) != hipSuccess ||
+      hipStreamSynchronize(cm->stream()) != hipSuccess) {
+    set_error("cmf_integrate: download failed");
+    return -QTG_EDEVICE;
+  }
+  const int64_t n = std::min<int64_t>((int64_t)r.times.size(), max_out);
+  for (int64_t i = 0; i < n; ++i) {
+    if (times) times[i] = r.times[i];
+    if (energies) energies[i] = r.energies[i];
+    if (autocorr) {
+      autocorr[2 * i] = r.autocorr[i].real();
+      autocorr[2 * i + 1] = r.autocorr[i].imag();
+    }
+  }
+  if (stats) {
+    stats[0] = (int64_t)r.steps;
+    stats[1] = (int64_t)r.rejected;
+    stats[2] = (int64_t)r.builds;
+    stats[3] = (int64_t)r.derivs;
+    stats[4] = (int64_t)r.ticks;
+  }
+  return (int64_t)r.times.size();
 }
 
 }  // extern "C"

@@ -1,0 +1,497 @@
+// Constant-mean-field integrator on the device (CMFIntegrator.cpp:25-199 and
+// BS_integrator.h:31-256 of QuTree).
+//
+// Host: step-size control of CMF and of every node's Bulirsch-Stoer
+// integrator in FP64, exactly as the reference decides them.  Device: all
+// tensor arithmetic.  The nodes' BS integrators run in lockstep "ticks": each
+// tick every node that is still integrating takes exactly one derivative
+// evaluation (LayerDerivative with the matrices of the last build), preceded
+// and followed by its own vector program (midpoint updates, rational
+// extrapolation, error norm).  A tick is four launches (program, Apply,
+// finish, program) and one read of the per-node errors.
+#include "cmf.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace qtg {
+
+namespace {
+constexpr int kMaxUse = 7, kMax = 11;
+constexpr double kShrink = 0.95, kGrow = 1.2;
+const int kSeq[kMax] = {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96};
+constexpr int kOpsPerNode = 16;
+}  // namespace
+
+Cmf::~Cmf() {
+  for (void* p : dev_)
+    if (p) (void)hipFree(p);
+  if (host_stage_) (void)hipHostFree(host_stage_);
+}
+
+int Cmf::dalloc(void** p, size_t bytes) {
+  if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) {
+    set_error("cmf: device allocation failed");
+    return QTG_ENOMEM;
+  }
+  dev_.push_back(*p);
+  return QTG_OK;
+}
+
+int Cmf::init() {
+  if (E.c64_) {
+    set_error("cmf: the integrator runs on a complex128 engine");
+    return QTG_ESTATE;
+  }
+  if (!E.planned_) {
+    set_error("cmf: engine has no Hamiltonian");
+    return QTG_ESTATE;
+  }
+  const HTree& t = E.tree_;
+  NN_ = (int)t.nodes.size();
+  total_ = t.total;
+  s_ = E.stream_;
+  const size_t arena = sizeof(cplx<double>) * total_;
+  int rc;
+  for (cplx<double>** a : {&P_, &P0_, &P2_, &PS_, &X_, &D_})
+    if ((rc = dalloc((void**)a, arena))) return rc;
+  for (int k = 0; k < 5; ++k)
+    if ((rc = dalloc((void**)&Y_[k], arena))) return rc;
+  for (int k = 0; k < 7; ++k)
+    if ((rc = dalloc((void**)&YT_[k], arena))) return rc;
+  if ((rc = dalloc((void**)&err_, sizeof(double) * NN_))) return rc;
+  // rho slot per node (-1 at the top) for the node error
+  std::vector<int64_t> rs(NN_, -1);
+  for (auto& v : t.nodes)
+    if (!v.top()) rs[v.addr] = E.rhoslot_[v.addr];
+  if ((rc = dalloc((void**)&rhoslot_, sizeof(int64_t) * NN_))) return rc;
+  QTG_HIP_TRY(hipMemcpy(rhoslot_, rs.data(), sizeof(int64_t) * NN_, hipMemcpyHostToDevice));
+  // staging for per-tick programs and the active mask
+  stage_bytes_ = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode) + NN_ + 64;
+  if (hipHostMalloc(&host_stage_, stage_bytes_) != hipSuccess) {
+    set_error("cmf: pinned allocation failed");
+    return QTG_ENOMEM;
+  }
+  if ((rc = dalloc(&dev_stage_, stage_bytes_))) return rc;
+  QTG_HIP_TRY(hipHostMalloc((void**)&err_h_, sizeof(double) * NN_));
+  // orthogonal(): per height, the nodes to orthonormalise and the parents
+  // receiving their square roots (children in node-address order)
+  int64_t swoff = 0;
+  std::vector<int64_t> swslot(NN_, -1);
+  for (auto& v : t.nodes)
+    if (!v.top()) {
+      swslot[v.addr] = swoff;
+      swoff += (int64_t)v.n * v.n;
+    }
+  if ((rc = dalloc((void**)&sw_, sizeof(cplx<double>) * std::max<int64_t>(swoff, 1)))) return rc;
+  std::vector<OrthoItem> oi;
+  std::vector<SwApply> sa;
+  orthoRange_.clear();
+  for (int h = 0; h < t.maxHeight; ++h) {
+    std::pair<int64_t, int64_t> r1{(int64_t)oi.size(), 0}, r2{(int64_t)sa.size(), 0};
+    for (auto& v : t.nodes)
+      if (v.height == h && !v.top()) oi.push_back(OrthoItem{v.addr, 0, swslot[v.addr]});
+    r1.second = (int64_t)oi.size();
+    for (auto& p : t.nodes) {
+      if (p.bottom) continue;
+      SwApply it{};
+      it.node = p.addr;
+      std::vector<int> kids = p.children;
+      std::sort(kids.begin(), kids.end());  // address order
+      for (int c : kids)
+        if (t.nodes[c].height == h) {
+          it.fmode[it.nf] = t.nodes[c].childIdx;
+          it.fmat[it.nf++] = swslot[c];
+        }
+      if (it.nf > 0) sa.push_back(it);
+    }
+    r2.second = (int64_t)sa.size();
+    orthoRange_.push_back({r1, r2});
+  }
+  if ((rc = dalloc((void**)&d_oi_, sizeof(OrthoItem) * std::max<size_t>(oi.size(), 1)))) return rc;
+  if ((rc = dalloc((void**)&d_sa_, sizeof(SwApply) * std::max<size_t>(sa.size(), 1)))) return rc;
+  if (!oi.empty()) QTG_HIP_TRY(hipMemcpy(d_oi_, oi.data(), sizeof(OrthoItem) * oi.size(), hipMemcpyHostToDevice));
+  if (!sa.empty()) QTG_HIP_TRY(hipMemcpy(d_sa_, sa.data(), sizeof(SwApply) * sa.size(), hipMemcpyHostToDevice));
+  // overlap tree (TreeFunctions::dotProduct, MatrixTreeFunctions_Implementation.h:16-39)
+  int64_t soff = 0;
+  sslot_.assign(NN_, -1);
+  for (auto& v : t.nodes) {
+    sslot_[v.addr] = soff;
+    soff += (int64_t)v.n * v.n;
+  }
+  if ((rc = dalloc((void**)&spool_, sizeof(cplx<double>) * soff))) return rc;
+  std::vector<SwItem> ov;
+  ovRange_.clear();
+  ovNmax_.clear();
+  for (int h = 0; h <= t.maxHeight; ++h) {
+    std::pair<int64_t, int64_t> r{(int64_t)ov.size(), 0};
+    int nm = 256;
+    for (auto& v : t.nodes) {
+      if (v.height != h) continue;
+      SwItem it{};
+      it.node = v.addr;
+      it.k = v.order() - 1;
+      it.bra = it.ket = v.off;
+      it.out = sslot_[v.addr];
+      for (int c : v.children) {
+        it.fmode[it.nf] = t.nodes[c].childIdx;
+        it.fmat[it.nf++] = sslot_[c];
+      }
+      ov.push_back(it);
+      nm = std::max(nm, v.N);
+    }
+    r.second = (int64_t)ov.size();
+    ovRange_.push_back(r);
+    ovNmax_.push_back(nm);
+  }
+  if ((rc = dalloc((void**)&d_ov_, sizeof(SwItem) * ov.size()))) return rc;
+  QTG_HIP_TRY(hipMemcpy(d_ov_, ov.data(), sizeof(SwItem) * ov.size(), hipMemcpyHostToDevice));
+  if ((rc = dalloc((void**)&energy_, sizeof(double) * 4))) return rc;
+  return QTG_OK;
+}
+
+CmfBufs Cmf::bufs(cplx<double>* P) const {
+  CmfBufs b{};
+  b.b[B_P] = P;
+  for (int k = 0; k < 5; ++k) b.b[B_Y0 + k] = Y_[k];
+  for (int k = 0; k < 7; ++k) b.b[B_YT0 + k] = YT_[k];
+  b.b[B_X] = X_;
+  b.b[B_D] = D_;
+  b.b[B_E1] = P_;
+  b.b[B_E2] = P2_;
+  return b;
+}
+
+int Cmf::copy(cplx<double>* dst, const cplx<double>* src) {
+  QTG_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(cplx<double>) * total_, hipMemcpyDeviceToDevice, s_));
+  return QTG_OK;
+}
+
+int Cmf::build(const cplx<double>* P) {
+  int rc = copy((cplx<double>*)E.d_psi64_, P);
+  if (rc) return rc;
+  ++nbuild_;
+  return E.build(s_);
+}
+
+// ---------------------------------------------------------------- BS ticks
+static VecOp op0(int code, int a = 0, int b = 0, int c = 0, double h = 0.0) {
+  VecOp o{};
+  o.code = code;
+  o.a = a;
+  o.b = b;
+  o.c = c;
+  o.h = h;
+  return o;
+}
+
+// Emits the node's ops up to and around its next derivative evaluation.
+// Returns false when the node has finished its interval.
+bool Cmf::advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, double eps) {
+  NodeBS& b = bs_[v];
+  double& dt = dtbs_[v];
+  while (true) {
+    switch (b.st) {
+      case NodeBS::AWAIT_ERR: {
+        const double e = err_h_[v];
+        if (e < eps) {
+          b.x += b.h;
+          if (b.i == kMaxUse - 1) dt = b.h * kShrink;
+          else if (b.i == kMaxUse - 2) dt = b.h * kGrow;
+          else dt = (b.h * kSeq[kMaxUse - 2]) / (1.0 * kSeq[b.i]);
+          pre.push_back(op0(V_COPY, B_P, B_Y0));
+          b.st = NodeBS::NEED_START;
+        } else {
+          ++b.i;
+          if (b.i >= kMax) {
+            b.h = b.h / 4.0;
+            for (int k = 0; k < (kMax - kMaxUse) / 2.0; ++k) b.h /= 2.0;
+            if (dt + b.h == dt) {
+              failed_ = true;
+              return false;
+            }
+            b.i = 0;
+          }
+          b.st = NodeBS::START_MMID;
+        }
+        continue;
+      }
+      case NodeBS::NEED_START: {
+        // BS_integrator::Integrate loop condition
+        if (!(b.xend - b.x > dt * 1e-6)) {
+          b.st = NodeBS::DONE;
+          return false;
+        }
+        if (b.x + dt > b.xend) dt = b.xend - b.x + 1e-10;
+        // bsstep: clearmemory, y0 = y3 = y, y4 = f(y)
+        std::fill(b.xtab, b.xtab + kMax, 0.0);
+        pre.push_back(op0(V_COPY, B_Y0, B_P));
+        pre.push_back(op0(V_COPY, B_Y0 + 3, B_P));
+        for (int k = 1; k < 7; ++k) pre.push_back(op0(V_ZERO, B_YT0 + k));
+        pre.push_back(op0(V_COPY, B_X, B_P));
+        post.push_back(op0(V_COPY, B_Y0 + 4, B_D));
+        b.h = dt;
+        b.i = 0;
+        b.st = NodeBS::START_MMID;
+        return true;
+      }
+      case NodeBS::START_MMID: {
+        b.step = kSeq[b.i];
+        b.hh = b.h / (1.0 * b.step);
+        pre.push_back(op0(V_COPY, B_Y0, B_Y0 + 3));
+        pre.push_back(op0(V_AXPY, B_Y0 + 1, B_Y0 + 3, B_Y0 + 4, b.hh));
+        pre.push_back(op0(V_COPY, B_X, B_Y0 + 1));
+        post.push_back(op0(V_COPY, B_Y0 + 2, B_D));
+        b.nsub = 1;
+        b.st = NodeBS::IN_MMID;
+        if (b.nsub == b.step) finish_mmid(v, post);
+        return true;
+      }
+      case NodeBS::IN_MMID: {
+        pre.push_back(op0(V_MMID_STEP, 0, 0, 0, b.hh));
+        pre.push_back(op0(V_COPY, B_X, B_Y0 + 1));
+        post.push_back(op0(V_COPY, B_Y0 + 2, B_D));
+        ++b.nsub;
+        if (b.nsub == b.step) finish_mmid(v, post);
+        return true;
+      }
+      default:
+        return false;
+    }
+  }
+}
+
+void Cmf::finish_mmid(int v, std::vector<VecOp>& post) {
+  NodeBS& b = bs_[v];
+  post.push_back(op0(V_MMID_FINAL, 0, 0, 0, b.hh));
+  const double xest = std::pow(b.h / (1.0 * b.step), 2);
+  b.xtab[b.i] = xest;
+  VecOp r = op0(V_RZEXTR);
+  r.i = b.i;
+  r.mi = (b.i < kMaxUse) ? b.i + 1 : kMaxUse;
+  for (int k = 1; k < r.mi; ++k) r.fx[k] = b.xtab[b.i - k] / xest;
+  post.push_back(r);
+  post.push_back(op0(V_POST_EXTR));
+  post.push_back(op0(V_ERROR, B_Y0, B_Y0 + 1));
+  b.st = NodeBS::AWAIT_ERR;
+}
+
+int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
+  for (int v = 0; v < NN_; ++v) {
+    bs_[v].st = NodeBS::NEED_START;
+    bs_[v].x = t0;
+    bs_[v].xend = t1;
+  }
+  const CmfBufs bb = bufs(P);
+  unsigned char* hs = (unsigned char*)host_stage_;
+  unsigned char* ds = (unsigned char*)dev_stage_;
+  std::vector<NodeProg> ppre, ppost;
+  std::vector<VecOp> opre, opost, pre, post;
+  std::vector<uint8_t> act(NN_);
+  auto align = [](size_t x) { return (x + 63) & ~size_t(63); };
+  while (true) {
+    ppre.clear();
+    ppost.clear();
+    opre.clear();
+    opost.clear();
+    int nact = 0;
+    bool need_err = false;
+    for (int v = 0; v < NN_; ++v) {
+      act[v] = 0;
+      if (bs_[v].st == NodeBS::DONE) continue;
+      pre.clear();
+      post.clear();
+      const bool d = advance(v, pre, post, eps);
+      if (failed_) {
+        set_error("BS integrator: step size underflow");
+        return QTG_ESTATE;
+      }
+      if (!pre.empty()) {
+        ppre.push_back(NodeProg{v, (int)opre.size(), (int)(opre.size() + pre.size()), 0});
+        opre.insert(opre.end(), pre.begin(), pre.end());
+      }
+      if (!d) continue;
+      act[v] = 1;
+      ++nact;
+      ++nderiv_;
+      ppost.push_back(NodeProg{v, (int)opost.size(), (int)(opost.size() + post.size()), 0});
+      for (auto& o : post)
+        if (o.code == V_ERROR) need_err = true;
+      opost.insert(opost.end(), post.begin(), post.end());
+    }
+    // one compact upload: [pre progs][pre ops][post progs][post ops][active]
+    size_t o1 = 0, o2 = align(o1 + sizeof(NodeProg) * ppre.size());
+    size_t o3 = align(o2 + sizeof(VecOp) * opre.size()), o4 = align(o3 + sizeof(NodeProg) * ppost.size());
+    size_t o5 = align(o4 + sizeof(VecOp) * opost.size()), oend = o5 + NN_;
+    if (oend > stage_bytes_) {
+      set_error("cmf: staging overflow");
+      return QTG_ESTATE;
+    }
+    std::memcpy(hs + o1, ppre.data(), sizeof(NodeProg) * ppre.size());
+    std::memcpy(hs + o2, opre.data(), sizeof(VecOp) * opre.size());
+    std::memcpy(hs + o3, ppost.data(), sizeof(NodeProg) * ppost.size());
+    std::memcpy(hs + o4, opost.data(), sizeof(VecOp) * opost.size());
+    std::memcpy(hs + o5, act.data(), NN_);
+    QTG_HIP_TRY(hipMemcpyAsync(ds, hs, oend, hipMemcpyHostToDevice, s_));
+    int rc = launch_programs((NodeProg*)(ds + o1), (int64_t)ppre.size(), (VecOp*)(ds + o2), bb, E.d_nodes_,
+                             (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
+    if (rc) return rc;
+    if (nact == 0) break;  // only trailing accept copies were left
+    ++nticks_;
+    const uint8_t* dact = (const uint8_t*)(ds + o5);
+    if ((rc = launch_apply<double>(E.d_chunks_, E.nChunks_, E.d_terms_, E.d_nodes_, X_, E.d_pool_, E.d_partial_,
+                                   E.nmax_, s_, dact)))
+      return rc;
+    if ((rc = launch_finish<double>(E.d_fin_, E.nFin_, E.d_nodes_, X_, E.d_partial_, E.d_pool64_, D_, nullptr,
+                                    ph_re_, ph_im_, nullptr, E.nmax_, E.nspf_, s_, dact)))
+      return rc;
+    if ((rc = launch_programs((NodeProg*)(ds + o3), (int64_t)ppost.size(), (VecOp*)(ds + o4), bb, E.d_nodes_,
+                              (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_)))
+      return rc;
+    if (need_err) QTG_HIP_TRY(hipMemcpyAsync(err_h_, err_, sizeof(double) * NN_, hipMemcpyDeviceToHost, s_));
+    // the pinned staging is rewritten next tick, and the errors are read
+    QTG_HIP_TRY(hipStreamSynchronize(s_));
+  }
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  return QTG_OK;
+}
+
+// CMFIntegrator::Error: sqrt(sum_v LayerInterface::Error(Psi_v, Chi_v)^2)
+int Cmf::error(double* out) {
+  unsigned char* hs = (unsigned char*)host_stage_;
+  unsigned char* ds = (unsigned char*)dev_stage_;
+  const size_t progB = sizeof(NodeProg) * NN_;
+  NodeProg* hp = (NodeProg*)hs;
+  VecOp* ho = (VecOp*)(hs + progB);
+  for (int v = 0; v < NN_; ++v) {
+    hp[v] = NodeProg{v, v, v + 1, 0};
+    ho[v] = op0(V_ERROR, B_E1, B_E2);
+  }
+  QTG_HIP_TRY(hipMemcpyAsync(ds, hs, stage_bytes_, hipMemcpyHostToDevice, s_));
+  int rc = launch_programs((NodeProg*)ds, NN_, (VecOp*)(ds + progB), bufs(P_), E.d_nodes_,
+                           (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
+  if (rc) return rc;
+  QTG_HIP_TRY(hipMemcpyAsync(err_h_, err_, sizeof(double) * NN_, hipMemcpyDeviceToHost, s_));
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  double e = 0.0;
+  for (int v = 0; v < NN_; ++v) e += err_h_[v] * err_h_[v];
+  *out = std::sqrt(e);
+  return QTG_OK;
+}
+
+int Cmf::orthogonal(cplx<double>* P) {
+  int rc;
+  for (auto& r : orthoRange_) {
+    if ((rc = launch_ortho(d_oi_ + r.first.first, r.first.second - r.first.first, E.d_nodes_, P, sw_, E.nmax_,
+                           E.nspf_, s_)))
+      return rc;
+    if ((rc = launch_apply_sw(d_sa_ + r.second.first, r.second.second - r.second.first, E.d_nodes_, P, sw_,
+                              E.nmax_, s_)))
+      return rc;
+  }
+  return QTG_OK;
+}
+
+int Cmf::overlap_top(const cplx<double>* bra, const cplx<double>* ket, cplx<double>* out) {
+  for (size_t h = 0; h < ovRange_.size(); ++h) {
+    auto& r = ovRange_[h];
+    int rc = launch_sandwich<double>(d_ov_ + r.first, r.second - r.first, E.d_nodes_, bra, ket, spool_, ovNmax_[h],
+                                     s_);
+    if (rc) return rc;
+  }
+  QTG_HIP_TRY(hipMemcpyAsync(out, spool_ + sslot_[NN_ - 1], sizeof(cplx<double>), hipMemcpyDeviceToHost, s_));
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  return QTG_OK;
+}
+
+// CMFIntegrator::Output: Expectation with the current matrices / S_top, and
+// the autocorrelation <Psi(0)|Psi(t)>.
+int Cmf::output(double time, CmfResult& res) {
+  cplx<double> stop, ac;
+  int rc = overlap_top(P_, P_, &stop);
+  if (rc) return rc;
+  if ((rc = overlap_top(PS_, P_, &ac))) return rc;
+  // Apply at the top only
+  std::vector<uint8_t> act(NN_, 0);
+  act[NN_ - 1] = 1;
+  unsigned char* ds = (unsigned char*)dev_stage_;
+  const size_t off = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode);
+  QTG_HIP_TRY(hipMemcpyAsync(ds + off, act.data(), NN_, hipMemcpyHostToDevice, s_));
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  if ((rc = launch_apply<double>(E.d_chunks_, E.nChunks_, E.d_terms_, E.d_nodes_, P_, E.d_pool_, E.d_partial_,
+                                 E.nmax_, s_, ds + off)))
+    return rc;
+  if ((rc = launch_finish<double>(E.d_fin_, E.nFin_, E.d_nodes_, P_, E.d_partial_, E.d_pool64_, D_, nullptr, 1.0,
+                                  0.0, energy_, E.nmax_, E.nspf_, s_, ds + off)))
+    return rc;
+  double e[4];
+  QTG_HIP_TRY(hipMemcpyAsync(e, energy_, sizeof(e), hipMemcpyDeviceToHost, s_));
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  const cd num(e[0], e[1]), den(stop.re, stop.im);
+  res.times.push_back(time);
+  res.energies.push_back((num / den).real());
+  res.autocorr.push_back(cd(ac.re, ac.im));
+  return QTG_OK;
+}
+
+int Cmf::integrate(double time, double timeend, double dt, double out, double acc_cmf, double acc_bs,
+                   int64_t max_steps, CmfResult& res) {
+  const double tconst = 0.25, max_increase = 2.25;
+  bs_.assign(NN_, NodeBS());
+  dtbs_.assign(NN_, dt / 2.0);
+  failed_ = false;
+  nbuild_ = nderiv_ = nticks_ = 0;
+  double t_next = time + out;
+  bool calc_mat = true;
+  int rc;
+  if ((rc = copy(PS_, P_))) return rc;
+  if ((rc = build(P_))) return rc;
+  calc_mat = false;
+  if ((rc = output(time, res))) return rc;
+  double err = 0.0;
+  while (time + 1e-6 < timeend && res.steps < (size_t)max_steps) {
+    const double dtmax = std::min(timeend - time, t_next - time);
+    if (dt > dtmax) dt = dtmax;
+    else if (2 * dt > dtmax) dt = std::min(dt, dtmax * 0.6);
+    if ((rc = copy(P0_, P_))) return rc;
+    if (calc_mat) {
+      if ((rc = build(P_))) return rc;
+      calc_mat = false;
+    }
+    const double ts = dt * tconst;
+    if ((rc = cmfstep(P_, time, time + ts, acc_bs))) return rc;
+    if ((rc = copy(P2_, P_))) return rc;
+    if ((rc = cmfstep(P2_, time + ts, time + 2 * ts, acc_bs))) return rc;
+    if ((rc = build(P2_))) return rc;
+    if ((rc = cmfstep(P_, time + dt * tconst, time + dt * (1 - tconst), acc_bs))) return rc;
+    if ((rc = copy(P2_, P0_))) return rc;
+    if ((rc = cmfstep(P2_, time, time + dt, acc_bs))) return rc;
+    if ((rc = build(P2_))) return rc;
+    if ((rc = cmfstep(P_, time + dt * (1 - tconst), time + dt, acc_bs))) return rc;
+    if ((rc = error(&err))) return rc;
+    err *= 0.25;
+    ++res.steps;
+    if (err > acc_cmf) {
+      if ((rc = copy(P_, P0_))) return rc;
+      calc_mat = true;
+      dt *= std::pow(acc_cmf / err, 1.0 / 3.0) * 0.8;
+      ++res.rejected;
+    } else {
+      time += dt;
+      if ((rc = orthogonal(P_))) return rc;
+      if (time + 1e-10 >= t_next) {
+        if ((rc = output(time, res))) return rc;
+        t_next = time + out;
+      }
+      dt = dt * std::min(std::pow(acc_cmf / err, 1.0 / 3.0), max_increase);
+    }
+  }
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  res.builds = nbuild_;
+  res.derivs = nderiv_;
+  res.ticks = nticks_;
+  return QTG_OK;
+}
+
+}  // namespace qtg

@@ -1,0 +1,82 @@
+// Device CMF integrator (see cmf.cpp).
+#pragma once
+#include <complex>
+#include <utility>
+#include <vector>
+
+#include "cmf_ops.hpp"
+#include "engine.hpp"
+
+namespace qtg {
+
+struct CmfResult {
+  std::vector<double> times, energies;
+  std::vector<cd> autocorr;
+  size_t steps = 0, rejected = 0, builds = 0, derivs = 0, ticks = 0;
+};
+
+struct NodeBS {
+  enum State { NEED_START, START_MMID, IN_MMID, AWAIT_ERR, DONE };
+  State st = NEED_START;
+  double x = 0.0, xend = 0.0, h = 0.0, hh = 0.0;
+  int i = 0, step = 0, nsub = 0;
+  double xtab[11] = {0};
+};
+
+class Cmf {
+ public:
+  explicit Cmf(Engine& e) : E(e) {}
+  ~Cmf();
+  int init();
+  // state in/out: P_ (device arena)
+  cplx<double>* state() { return P_; }
+  int64_t size() const { return total_; }
+  hipStream_t stream() const { return s_; }
+  int integrate(double time, double timeend, double dt, double out, double acc_cmf, double acc_bs,
+                int64_t max_steps, CmfResult& res);
+  double ph_re_ = 1.0, ph_im_ = 0.0;  // propagation phase (1 = real time)
+
+ private:
+  int dalloc(void** p, size_t bytes);
+  CmfBufs bufs(cplx<double>* P) const;
+  int copy(cplx<double>* dst, const cplx<double>* src);
+  int build(const cplx<double>* P);
+  bool advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, double eps);
+  void finish_mmid(int v, std::vector<VecOp>& post);
+  int cmfstep(cplx<double>* P, double t0, double t1, double eps);
+  int error(double* out);
+  int orthogonal(cplx<double>* P);
+  int overlap_top(const cplx<double>* bra, const cplx<double>* ket, cplx<double>* out);
+  int output(double time, CmfResult& res);
+
+  Engine& E;
+  hipStream_t s_ = nullptr;
+  int NN_ = 0;
+  int64_t total_ = 0;
+  cplx<double>*P_ = nullptr, *P0_ = nullptr, *P2_ = nullptr, *PS_ = nullptr, *X_ = nullptr, *D_ = nullptr;
+  cplx<double>* Y_[5] = {nullptr};
+  cplx<double>* YT_[7] = {nullptr};
+  double* err_ = nullptr;
+  double* err_h_ = nullptr;
+  int64_t* rhoslot_ = nullptr;
+  void* host_stage_ = nullptr;
+  void* dev_stage_ = nullptr;
+  size_t stage_bytes_ = 0;
+  cplx<double>* sw_ = nullptr;
+  OrthoItem* d_oi_ = nullptr;
+  SwApply* d_sa_ = nullptr;
+  std::vector<std::pair<std::pair<int64_t, int64_t>, std::pair<int64_t, int64_t>>> orthoRange_;
+  cplx<double>* spool_ = nullptr;
+  std::vector<int64_t> sslot_;
+  SwItem* d_ov_ = nullptr;
+  std::vector<std::pair<int64_t, int64_t>> ovRange_;
+  std::vector<int> ovNmax_;
+  double* energy_ = nullptr;
+  std::vector<NodeBS> bs_;
+  std::vector<double> dtbs_;
+  bool failed_ = false;
+  size_t nbuild_ = 0, nderiv_ = 0, nticks_ = 0;
+  std::vector<void*> dev_;
+};
+
+}  // namespace qtg

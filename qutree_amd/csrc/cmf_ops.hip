@@ -1,0 +1,310 @@
+// Kernels of the CMF integrator (CMFIntegrator.cpp / BS_integrator.h of
+// QuTree): the vector arithmetic of every node's Bulirsch-Stoer state machine,
+// the rho-weighted node error (LayerInterface::Error) and orthogonal()
+// (TensorTree_Implementation.h:143-171).  One workgroup per node.
+#include "cmf_ops.hpp"
+#include "jacobi.hpp"
+#include "node_tile.hpp"
+
+namespace qtg {
+
+using cd2 = cplx<double>;
+
+__device__ inline cd2 cdiv(const cd2& a, const cd2& b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return mk<double>((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+
+__device__ inline double block_sum_d(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = TB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(TB)
+    program_kernel(const NodeProg* __restrict__ progs, const VecOp* __restrict__ ops, CmfBufs bufs,
+                   const DNode* __restrict__ nodes, const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot,
+                   double* err) {
+  __shared__ double red[TB];
+  const NodeProg pg = progs[blockIdx.x];
+  const DNode nd = nodes[pg.node];
+  const int N = nd.N;
+  const int64_t off = nd.off;
+  for (int k = pg.op0; k < pg.op1; ++k) {
+    const VecOp op = ops[k];
+    switch (op.code) {
+      case V_COPY: {
+        cd2* A = bufs.b[op.a] + off;
+        const cd2* B = bufs.b[op.b] + off;
+        for (int e = threadIdx.x; e < N; e += TB) A[e] = B[e];
+        break;
+      }
+      case V_AXPY: {
+        cd2* A = bufs.b[op.a] + off;
+        const cd2* B = bufs.b[op.b] + off;
+        const cd2* Cc = bufs.b[op.c] + off;
+        for (int e = threadIdx.x; e < N; e += TB)
+          A[e] = mk<double>(B[e].re + op.h * Cc[e].re, B[e].im + op.h * Cc[e].im);
+        break;
+      }
+      case V_ZERO: {
+        cd2* A = bufs.b[op.a] + off;
+        for (int e = threadIdx.x; e < N; e += TB) A[e] = czero<double>();
+        break;
+      }
+      case V_MMID_STEP: {
+        cd2* y0 = bufs.b[B_Y0] + off;
+        cd2* y1 = bufs.b[B_Y0 + 1] + off;
+        cd2* y2 = bufs.b[B_Y0 + 2] + off;
+        for (int e = threadIdx.x; e < N; e += TB) {
+          const cd2 t = mk<double>(y0[e].re + 2.0 * op.h * y2[e].re, y0[e].im + 2.0 * op.h * y2[e].im);
+          y0[e] = y1[e];
+          y1[e] = t;
+          y2[e] = t;
+        }
+        break;
+      }
+      case V_MMID_FINAL: {
+        cd2* y0 = bufs.b[B_Y0] + off;
+        const cd2* y1 = bufs.b[B_Y0 + 1] + off;
+        const cd2* y2 = bufs.b[B_Y0 + 2] + off;
+        for (int e = threadIdx.x; e < N; e += TB)
+          y0[e] = mk<double>((y0[e].re + y1[e].re + op.h * y2[e].re) / 2.0,
+                             (y0[e].im + y1[e].im + op.h * y2[e].im) / 2.0);
+        break;
+      }
+      case V_RZEXTR: {
+        // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2
+        cd2* yest = bufs.b[B_Y0] + off;
+        cd2* ysav = bufs.b[B_Y0 + 1] + off;
+        cd2* dy = bufs.b[B_Y0 + 2] + off;
+        cd2* yt0 = bufs.b[B_YT0] + off;
+        for (int e = threadIdx.x; e < N; e += TB) {
+          if (op.i == 0) {
+            const cd2 v = yest[e];
+            ysav[e] = v;
+            dy[e] = v;
+            yt0[e] = v;
+            continue;
+          }
+          cd2 ddy = czero<double>();
+          cd2 yy = yest[e];
+          cd2 v = yt0[e];
+          cd2 c = yy;
+          yt0[e] = yy;
+          for (int k2 = 1; k2 < op.mi; ++k2) {
+            const cd2 bi = mk<double>(op.fx[k2] * v.re, op.fx[k2] * v.im);
+            cd2 b = mk<double>(bi.re - c.re, bi.im - c.im);
+            if (b.re != 0.0 || b.im != 0.0) {
+              b = cdiv(mk<double>(c.re - v.re, c.im - v.im), b);
+              ddy = cmul(c, b);
+              c = cmul(bi, b);
+            } else {
+              ddy = v;
+            }
+            cd2* ytk = bufs.b[B_YT0 + k2] + off;
+            v = ytk[e];
+            ytk[e] = ddy;
+            yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
+          }
+          dy[e] = ddy;
+          ysav[e] = yy;
+        }
+        break;
+      }
+      case V_POST_EXTR: {
+        cd2* y0 = bufs.b[B_Y0] + off;
+        cd2* y1 = bufs.b[B_Y0 + 1] + off;
+        const cd2* y2 = bufs.b[B_Y0 + 2] + off;
+        for (int e = threadIdx.x; e < N; e += TB) {
+          const cd2 a = y1[e];
+          y0[e] = a;
+          y1[e] = mk<double>(a.re + y2[e].re, a.im + y2[e].im);
+        }
+        break;
+      }
+      case V_ERROR: {
+        const cd2* Phi = bufs.b[op.a] + off;
+        const cd2* Chi = bufs.b[op.b] + off;
+        const int n = nd.dims[nd.order - 1], L = N / n;
+        const int64_t rs = rhoslot[pg.node];
+        double delta = 0.0, nrm = 0.0;
+        if (rs < 0) {
+          for (int e = threadIdx.x; e < N; e += TB) {
+            const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
+            delta += dr * dr + di * di;
+            nrm += Phi[e].re * Phi[e].re + Phi[e].im * Phi[e].im;
+          }
+          delta = block_sum_d(delta, red);
+          nrm = block_sum_d(nrm, red);
+        } else {
+          const cd2* rho = pool + rs;
+          // Delta = sum_{i,n,m} Re( conj(d(i,n)) rho(n,m) d(i,m) )
+          for (int e = threadIdx.x; e < N; e += TB) {
+            const int i = e % L, nn = e / L;
+            const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
+            double tr = 0.0, ti = 0.0;
+            for (int m = 0; m < n; ++m) {
+              const cd2 r = rho[nn + m * n];
+              const double er = Phi[i + m * L].re - Chi[i + m * L].re;
+              const double ei = Phi[i + m * L].im - Chi[i + m * L].im;
+              tr += r.re * er - r.im * ei;
+              ti += r.re * ei + r.im * er;
+            }
+            delta += dr * tr + di * ti;
+          }
+          delta = block_sum_d(delta, red);
+          double t = 0.0, tim = 0.0;
+          for (int m = 0; m < n; ++m) {
+            t += rho[m + m * n].re;
+            tim += rho[m + m * n].im;
+          }
+          nrm = sqrt(t * t + tim * tim);
+        }
+        if (threadIdx.x == 0) err[pg.node] = sqrt(fabs(delta / nrm));
+        break;
+      }
+      default:
+        break;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- orthogonal()
+// S = Phi^H Phi, SW(j,k) = sum_l U(k,l) sqrt(max(l_l,0)) conj(U(j,l)) (the
+// transposed square root), then iterated Gram-Schmidt of Phi in place
+// (Tensor_Implementation.h:806-853).  One wavefront per node.
+__global__ void __launch_bounds__(EW)
+    ortho_kernel(const OrthoItem* __restrict__ items, const DNode* __restrict__ nodes, cd2* P, cd2* sw,
+                 int nmax, int nspf) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cd2* Phi = reinterpret_cast<cd2*>(smem_raw);
+  cd2* A = Phi + nmax;
+  cd2* U = A + nspf * (nspf + 1);
+  __shared__ JacobiScratch js;
+  __shared__ double lam[64];
+  const OrthoItem it = items[blockIdx.x];
+  const DNode nd = nodes[it.node];
+  const int N = nd.N, n = nd.dims[nd.order - 1], L = N / n, ld = n + 1;
+  for (int e = threadIdx.x; e < N; e += EW) Phi[e] = P[nd.off + e];
+  __syncthreads();
+  for (int o = threadIdx.x; o < n * n; o += EW) {
+    const int j = o % n, k = o / n;
+    cd2 acc = czero<double>();
+    for (int i = 0; i < L; ++i) cfmac(acc, Phi[i + j * L], Phi[i + k * L]);
+    A[j + k * ld] = acc;
+  }
+  __syncthreads();
+  jacobi_eig(A, U, n, ld, js);
+  for (int i = threadIdx.x; i < n; i += EW) lam[i] = sqrt(fmax(A[i + i * ld].re, 0.0));
+  __syncthreads();
+  for (int o = threadIdx.x; o < n * n; o += EW) {
+    const int j = o % n, k = o / n;
+    cd2 acc = czero<double>();
+    for (int l = 0; l < n; ++l) {
+      const cd2 ukl = U[k + l * ld], ujl = U[j + l * ld];
+      // U(k,l) * s_l * conj(U(j,l))
+      acc.re += lam[l] * (ukl.re * ujl.re + ukl.im * ujl.im);
+      acc.im += lam[l] * (ukl.im * ujl.re - ukl.re * ujl.im);
+    }
+    sw[it.sw + j + k * n] = acc;
+  }
+  // Gram-Schmidt: lane owns rows i = lane + 64 t, so every reduction is a
+  // wavefront shuffle sum and no other lane touches its elements.
+  for (int k = 0; k < n; ++k) {
+    double accum = 1.0;
+    for (int iter = 0; iter < 15 && accum > 1e-12; ++iter) {
+      accum = 0.0;
+      for (int m = 0; m < k; ++m) {
+        double ore = 0.0, oim = 0.0;
+        for (int i = threadIdx.x; i < L; i += EW) {
+          const cd2 a = Phi[i + m * L], b = Phi[i + k * L];
+          ore += a.re * b.re + a.im * b.im;
+          oim += a.re * b.im - a.im * b.re;
+        }
+        ore = wave_sum(ore);
+        oim = wave_sum(oim);
+        accum += sqrt(ore * ore + oim * oim);
+        for (int i = threadIdx.x; i < L; i += EW) {
+          const cd2 a = Phi[i + m * L];
+          cd2& b = Phi[i + k * L];
+          b.re -= ore * a.re - oim * a.im;
+          b.im -= ore * a.im + oim * a.re;
+        }
+      }
+      double nr = 0.0;
+      for (int i = threadIdx.x; i < L; i += EW) nr += Phi[i + k * L].re * Phi[i + k * L].re + Phi[i + k * L].im * Phi[i + k * L].im;
+      nr = wave_sum(nr);
+      if (nr != 0.0) {
+        const double s = sqrt(nr);
+        for (int i = threadIdx.x; i < L; i += EW) {
+          Phi[i + k * L].re /= s;
+          Phi[i + k * L].im /= s;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < N; e += EW) P[nd.off + e] = Phi[e];
+}
+
+// parent <- SW_c x_{childIdx(c)} parent for its children, in node-address order
+__global__ void __launch_bounds__(TB)
+    apply_sw_kernel(const SwApply* __restrict__ items, const DNode* __restrict__ nodes, cd2* P, const cd2* sw,
+                    int nmax) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cd2* b0 = reinterpret_cast<cd2*>(smem_raw);
+  cd2* b1 = b0 + nmax;
+  const SwApply it = items[blockIdx.x];
+  const DNode nd = nodes[it.node];
+  const int N = nd.N;
+  for (int e = threadIdx.x; e < N; e += TB) b0[e] = P[nd.off + e];
+  __syncthreads();
+  cd2* cur = b0;
+  cd2* nxt = b1;
+  for (int f = 0; f < it.nf; ++f) {
+    mode_apply<double>(nxt, cur, sw + it.fmat[f], nd, it.fmode[f]);
+    __syncthreads();
+    cd2* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  for (int e = threadIdx.x; e < N; e += TB) P[nd.off + e] = cur[e];
+}
+
+int launch_programs(const NodeProg* progs, int64_t count, const VecOp* ops, const CmfBufs& bufs,
+                    const DNode* nodes, const cplx<double>* pool, const int64_t* rhoslot, double* err,
+                    hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  hipLaunchKernelGGL(program_kernel, dim3((unsigned)count), dim3(TB), 0, s, progs, ops, bufs, nodes, pool, rhoslot,
+                     err);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P, cplx<double>* sw,
+                 int nmax, int nspf, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  const size_t lds = ((size_t)nmax + 2 * (size_t)nspf * (nspf + 1)) * sizeof(cd2);
+  hipLaunchKernelGGL(ortho_kernel, dim3((unsigned)count), dim3(EW), lds, s, items, nodes, P, sw, nmax, nspf);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+int launch_apply_sw(const SwApply* items, int64_t count, const DNode* nodes, cplx<double>* P,
+                    const cplx<double>* sw, int nmax, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  const size_t lds = 2 * (size_t)nmax * sizeof(cd2);
+  hipLaunchKernelGGL(apply_sw_kernel, dim3((unsigned)count), dim3(TB), lds, s, items, nodes, P, sw, nmax);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+}  // namespace qtg

@@ -1,0 +1,65 @@
+// Device records and launchers of the CMF integrator: per-node vector
+// programs of the Bulirsch-Stoer state machines and the orthogonal() sweep.
+#pragma once
+#include "common.hpp"
+#include "tree_ops.hpp"
+
+namespace qtg {
+
+// arenas addressed by buffer id (each holds a whole tree in node-address layout)
+enum CmfBuf : int32_t {
+  B_P = 0,    // state being integrated in this CMF substep
+  B_Y0 = 1,   // yvec[0..4] of BS_integrator
+  B_YT0 = 6,  // ytab[0..6]
+  B_X = 13,   // derivative input
+  B_D = 14,   // derivative output
+  B_E1 = 15,  // error operands
+  B_E2 = 16,
+  kCmfBufs = 17
+};
+struct CmfBufs {
+  cplx<double>* b[kCmfBufs];
+};
+
+enum VecCode : int32_t {
+  V_COPY = 0,        // a <- b
+  V_AXPY = 1,        // a <- b + h c
+  V_ZERO = 2,        // a <- 0
+  V_MMID_STEP = 3,   // t = y0 + 2h y2; y0 <- y1; y1 <- t; y2 <- t
+  V_MMID_FINAL = 4,  // y0 <- (y0 + y1 + h y2) / 2
+  V_RZEXTR = 5,      // rational extrapolation step i (rzextr)
+  V_POST_EXTR = 6,   // y0 <- y1; y1 <- y1 + y2
+  V_ERROR = 7        // err[node] = LayerInterface::Error(a, b)
+};
+
+struct VecOp {
+  int32_t code, a, b, c;
+  int32_t i, mi, pad0, pad1;
+  double h;
+  double fx[7];
+};
+
+struct NodeProg {
+  int32_t node, op0, op1, pad;
+};
+
+struct OrthoItem {
+  int32_t node, pad;
+  int64_t sw;  // offset of S^{1/2 T} (n x n) in the square-root pool
+};
+
+struct SwApply {
+  int32_t node, nf;
+  int32_t fmode[4];
+  int64_t fmat[4];
+};
+
+int launch_programs(const NodeProg* progs, int64_t count, const VecOp* ops, const CmfBufs& bufs,
+                    const DNode* nodes, const cplx<double>* pool, const int64_t* rhoslot, double* err,
+                    hipStream_t s);
+int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P, cplx<double>* sw,
+                 int nmax, int nspf, hipStream_t s);
+int launch_apply_sw(const SwApply* items, int64_t count, const DNode* nodes, cplx<double>* P,
+                    const cplx<double>* sw, int nmax, hipStream_t s);
+
+}  // namespace qtg

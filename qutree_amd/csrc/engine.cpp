@@ -661,6 +661,14 @@ int Engine::build_t(hipStream_t s) {
   if ((rc = begin_mark(1, s))) return rc;
   if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, s))) return rc;
   if ((rc = end_mark(s))) return rc;
+  // aggregated Apply matrices depend only on the h and hole matrices
+  if (nComb_ > 0) {
+    if ((rc = begin_mark(5, s))) return rc;
+    if ((rc = launch_combine<T>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
+    if ((rc = launch_combine<T>(d_combOut2_, nComb2_, d_combTerm2_, d_pool_, s))) return rc;
+    if ((rc = end_mark(s))) return rc;
+    if (timing_) tfl_[5] += phaseFlops_[5];
+  }
   built_ = true;
   return QTG_OK;
 }
@@ -676,15 +684,7 @@ int Engine::build(hipStream_t s) {
 template <typename T>
 int Engine::apply_t(hipStream_t s, double ph_re, double ph_im) {
   const void* psi = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
-  int rc;
-  if (nComb_ > 0) {
-    if ((rc = begin_mark(5, s))) return rc;
-    if ((rc = launch_combine<T>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
-    if ((rc = launch_combine<T>(d_combOut2_, nComb2_, d_combTerm2_, d_pool_, s))) return rc;
-    if ((rc = end_mark(s))) return rc;
-    if (timing_) tfl_[5] += phaseFlops_[5];
-  }
-  rc = begin_mark(2, s);
+  int rc = begin_mark(2, s);
   if (rc) return rc;
   if ((rc = launch_apply<T>(d_chunks_, nChunks_, d_terms_, d_nodes_, psi, d_pool_, d_partial_, nmax_, s))) return rc;
   if ((rc = end_mark(s))) return rc;

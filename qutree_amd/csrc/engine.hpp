@@ -11,7 +11,11 @@
 
 namespace qtg {
 
+class Cmf;
+
 class Engine {
+  friend class Cmf;
+
  public:
   ~Engine();
   int create(const std::string& tree_text, int precision, bool host_only = false);

@@ -28,38 +28,11 @@
 #include <vector>
 
 #include "common.hpp"
+#include "jacobi.hpp"
+#include "node_tile.hpp"
 #include "tree_ops.hpp"
 
 namespace qtg {
-
-constexpr int TB = 256;
-
-__device__ inline void node_view(const DNode& nd, int k, int& bef, int& nk, int& aft) {
-  bef = 1;
-  aft = 1;
-  for (int i = 0; i < k; ++i) bef *= nd.dims[i];
-  nk = nd.dims[k];
-  for (int i = k + 1; i < nd.order; ++i) aft *= nd.dims[i];
-}
-
-// dst = M x_f src on an LDS-resident node tensor (M is n_f x n_f).
-template <typename T>
-__device__ void mode_apply(cplx<T>* dst, const cplx<T>* src, const cplx<T>* __restrict__ M, const DNode& nd,
-                           int f) {
-  int bef, nf, aft;
-  node_view(nd, f, bef, nf, aft);
-  const int N = nd.N;
-  for (int e = threadIdx.x; e < N; e += TB) {
-    const int b = e % bef;
-    const int q = e / bef;
-    const int k = q % nf;
-    const int a = q / nf;
-    const cplx<T>* s = src + b + a * bef * nf;
-    cplx<T> acc = czero<T>();
-    for (int m = 0; m < nf; ++m) cfma(acc, M[k + m * nf], s[m * bef]);
-    dst[e] = acc;
-  }
-}
 
 // out(j,l) = sum_r conj(X(r,j)) Y(r,l) over all indices but k.  scratch needs TB entries.
 template <typename T>
@@ -159,12 +132,13 @@ template <typename T, int EPT>
 __global__ void __launch_bounds__(TB)
     apply_kernel(const ApChunk* __restrict__ chunks, const ApTerm* __restrict__ terms,
                  const DNode* __restrict__ nodes, const cplx<T>* psi, const cplx<T>* __restrict__ pool,
-                 cplx<T>* partial, int nmax) {
+                 cplx<T>* partial, int nmax, const uint8_t* __restrict__ active) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* buf0 = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf1 = buf0 + nmax;
   cplx<T>* buf2 = buf1 + nmax;
   const ApChunk ch = chunks[blockIdx.x];
+  if (active && !active[ch.node]) return;  // uniform per workgroup
   const DNode nd = nodes[ch.node];
   const int N = nd.N;
   for (int e = threadIdx.x; e < N; e += TB) buf0[e] = psi[nd.off + e];
@@ -234,13 +208,14 @@ __global__ void __launch_bounds__(TB)
     finish_kernel(const FinNode* __restrict__ fin, const DNode* __restrict__ nodes,
                   const cplx<double>* __restrict__ psi64, const cplx<T>* __restrict__ partial,
                   const cplx<double>* __restrict__ pool64, cplx<double>* dpsi, cplx<double>* apply_out,
-                  double ph_re, double ph_im, cplx<double>* energy, int nmax) {
+                  double ph_re, double ph_im, cplx<double>* energy, int nmax, const uint8_t* __restrict__ active) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<double>* sY = reinterpret_cast<cplx<double>*>(smem_raw);
   cplx<double>* sA = sY + nmax;
   cplx<double>* sm = sA + nmax;  // two n x n matrices (n = SPFs of the node)
   __shared__ cplx<double> red[TB];
   const FinNode fn = fin[blockIdx.x];
+  if (active && !active[fn.node]) return;  // uniform per workgroup
   const DNode nd = nodes[fn.node];
   const int N = nd.N, n = nd.dims[nd.order - 1], L = N / n;
   for (int e = threadIdx.x; e < N; e += TB) {
@@ -361,21 +336,9 @@ __global__ void __launch_bounds__(TB)
   }
 }
 
-// --------------------------------------------------------- Jacobi eigensolver
-// Hermitian n x n (n <= 64), column-major in LDS (leading dimension n+1), one
-// wavefront per matrix.  Tournament (round-robin) ordering: each step rotates
-// up to 32 disjoint (p,q) pairs whose parameters come from the same matrix,
-// columns first, then rows.  Player at position j of step t:
-// j == 0 -> 0, else 1 + (j - 1 + t) mod (m - 1).  A pair is skipped when
-// |a_pq| <= eps_mach sqrt(|a_pp a_qq|) (relative accuracy of Demmel-Veselic;
-// eps_mach = 2.2e-16); sweeps stop when no pair rotates or after 20 sweeps.
-constexpr int EW = 64;
-
-__device__ inline double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, EW);
-  return v;
-}
-
+// ------------------------------------------- density eigen / regularised inverse
+// One wavefront per density matrix (jacobi.hpp), then
+// R = U diag(1/(l + eps e^{-l/eps})) U^H.
 template <typename T>
 __global__ void __launch_bounds__(EW)
     heev_reginv_kernel(const EigItem* __restrict__ items, const cplx<T>* __restrict__ pool,
@@ -384,9 +347,7 @@ __global__ void __launch_bounds__(EW)
   const int ldm = nmaxspf + 1;
   cplx<double>* A = reinterpret_cast<cplx<double>*>(smem_raw);
   cplx<double>* U = A + nmaxspf * ldm;
-  __shared__ double pc[32], ps[32];
-  __shared__ cplx<double> pph[32];
-  __shared__ int pp[32], pq[32];
+  __shared__ JacobiScratch js;
   __shared__ double lam[64];
   const EigItem it = items[blockIdx.x];
   const int n = it.n;
@@ -395,104 +356,13 @@ __global__ void __launch_bounds__(EW)
     const int i = o % n, j = o / n;
     const cplx<T> v = pool[it.in + o];
     A[i + j * ld] = mk<double>((double)v.re, (double)v.im);
-    U[i + j * ld] = mk<double>(i == j ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  // Hermitise (the input is Hermitian up to rounding)
-  for (int o = threadIdx.x; o < n * n; o += EW) {
-    const int i = o % n, j = o / n;
-    if (i < j) {
-      const cplx<double> a = A[i + j * ld], b = A[j + i * ld];
-      const cplx<double> m = mk<double>(0.5 * (a.re + b.re), 0.5 * (a.im - b.im));
-      A[i + j * ld] = m;
-      A[j + i * ld] = cconj(m);
-    } else if (i == j) {
-      A[i + i * ld].im = 0.0;
-    }
-  }
-  __syncthreads();
-  const int m = (n + 1) & ~1;  // even player count (a dummy when n is odd)
-  const int np = m / 2;
-  if (n > 1) {
-    for (int sweep = 0; sweep < 20; ++sweep) {
-      int rotated = 0;
-      for (int step = 0; step < m - 1; ++step) {
-        for (int i = threadIdx.x; i < np; i += EW) {
-          const int j0 = i, j1 = m - 1 - i;
-          int p = (j0 == 0) ? 0 : 1 + (j0 - 1 + step) % (m - 1);
-          int q = (j1 == 0) ? 0 : 1 + (j1 - 1 + step) % (m - 1);
-          if (p > q) {
-            const int t = p;
-            p = q;
-            q = t;
-          }
-          pp[i] = p;
-          pq[i] = q;
-          ps[i] = 0.0;
-          if (q < n) {
-            const cplx<double> apq = A[p + q * ld];
-            const double aa = sqrt(apq.re * apq.re + apq.im * apq.im);
-            const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
-            if (aa > 2.2e-16 * sqrt(fabs(app * aqq)) && aa > 1e-300) {
-              const double theta = (aqq - app) / (2.0 * aa);
-              const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-              const double c = 1.0 / sqrt(t * t + 1.0);
-              pc[i] = c;
-              ps[i] = t * c;
-              pph[i] = mk<double>(apq.re / aa, apq.im / aa);
-            }
-          }
-        }
-        __syncthreads();
-        int any = 0;
-        for (int i = 0; i < np; ++i) any |= (ps[i] != 0.0);
-        if (any) {
-          rotated = 1;
-          // columns: A <- A G, U <- U G  (G = [[c, s ph], [-s conj(ph), c]] on (p,q))
-          for (int o = threadIdx.x; o < np * n; o += EW) {
-            const int i = o / n, r = o % n;
-            const double s_ = ps[i];
-            if (s_ == 0.0) continue;
-            const int p = pp[i], q = pq[i];
-            const double c = pc[i];
-            const cplx<double> ph = pph[i];
-            const cplx<double> arp = A[r + p * ld], arq = A[r + q * ld];
-            A[r + p * ld] = mk<double>(c * arp.re - s_ * (ph.re * arq.re + ph.im * arq.im),
-                                       c * arp.im - s_ * (ph.re * arq.im - ph.im * arq.re));
-            A[r + q * ld] = mk<double>(s_ * (ph.re * arp.re - ph.im * arp.im) + c * arq.re,
-                                       s_ * (ph.re * arp.im + ph.im * arp.re) + c * arq.im);
-            const cplx<double> urp = U[r + p * ld], urq = U[r + q * ld];
-            U[r + p * ld] = mk<double>(c * urp.re - s_ * (ph.re * urq.re + ph.im * urq.im),
-                                       c * urp.im - s_ * (ph.re * urq.im - ph.im * urq.re));
-            U[r + q * ld] = mk<double>(s_ * (ph.re * urp.re - ph.im * urp.im) + c * urq.re,
-                                       s_ * (ph.re * urp.im + ph.im * urp.re) + c * urq.im);
-          }
-          __syncthreads();
-          // rows: A <- G^H A  (new p = c*apk - s*ph*aqk ; new q = s*conj(ph)*apk + c*aqk)
-          for (int o = threadIdx.x; o < np * n; o += EW) {
-            const int i = o / n, k = o % n;
-            const double s_ = ps[i];
-            if (s_ == 0.0) continue;
-            const int p = pp[i], q = pq[i];
-            const double c = pc[i];
-            const cplx<double> ph = pph[i];
-            const cplx<double> apk = A[p + k * ld], aqk = A[q + k * ld];
-            A[p + k * ld] = mk<double>(c * apk.re - s_ * (ph.re * aqk.re - ph.im * aqk.im),
-                                       c * apk.im - s_ * (ph.re * aqk.im + ph.im * aqk.re));
-            A[q + k * ld] = mk<double>(s_ * (ph.re * apk.re + ph.im * apk.im) + c * aqk.re,
-                                       s_ * (ph.re * apk.im - ph.im * apk.re) + c * aqk.im);
-          }
-        }
-        __syncthreads();
-      }
-      if (!rotated) break;
-    }
-  }
+  jacobi_eig(A, U, n, ld, js);
   for (int i = threadIdx.x; i < n; i += EW) lam[i] = A[i + i * ld].re;
   __syncthreads();
   if (it.ev >= 0)
     for (int i = threadIdx.x; i < n; i += EW) pool64[it.ev + i] = mk<double>(lam[i], 0.0);
-  // regularised inverse  R(i,j) = sum_k U(i,k) w_k conj(U(j,k))
   for (int o = threadIdx.x; o < n * n; o += EW) {
     const int i = o % n, j = o / n;
     cplx<double> acc = czero<double>();
@@ -572,13 +442,13 @@ int launch_sandwich(const SwItem* items, int64_t count, const DNode* nodes, cons
 
 template <typename T>
 int launch_apply(const ApChunk* chunks, int64_t nchunks, const ApTerm* terms, const DNode* nodes, const void* psi,
-                 const void* pool, void* partial, int nmax, hipStream_t s) {
+                 const void* pool, void* partial, int nmax, hipStream_t s, const uint8_t* active) {
   if (nchunks == 0) return QTG_OK;
   const size_t lds = 3 * (size_t)nmax * sizeof(cplx<T>);
   const int ept = (nmax + TB - 1) / TB;
 #define QTG_APPLY(E)                                                                                     \
   hipLaunchKernelGGL((apply_kernel<T, E>), dim3((unsigned)nchunks), dim3(TB), lds, s, chunks, terms, nodes, \
-                     (const cplx<T>*)psi, (const cplx<T>*)pool, (cplx<T>*)partial, nmax)
+                     (const cplx<T>*)psi, (const cplx<T>*)pool, (cplx<T>*)partial, nmax, active)
   if (ept <= 1) QTG_APPLY(1);
   else if (ept <= 2) QTG_APPLY(2);
   else if (ept <= 4) QTG_APPLY(4);
@@ -596,12 +466,13 @@ int launch_apply(const ApChunk* chunks, int64_t nchunks, const ApTerm* terms, co
 template <typename T>
 int launch_finish(const FinNode* fin, int64_t count, const DNode* nodes, const void* psi64, const void* partial,
                   const void* pool64, void* dpsi, void* apply_out, double ph_re, double ph_im, void* energy,
-                  int nmax, int nspf, hipStream_t s) {
+                  int nmax, int nspf, hipStream_t s, const uint8_t* active) {
   if (count == 0) return QTG_OK;
   const size_t lds = finish_lds_bytes(nmax, nspf);
   hipLaunchKernelGGL(finish_kernel<T>, dim3((unsigned)count), dim3(TB), lds, s, fin, nodes,
                      (const cplx<double>*)psi64, (const cplx<T>*)partial, (const cplx<double>*)pool64,
-                     (cplx<double>*)dpsi, (cplx<double>*)apply_out, ph_re, ph_im, (cplx<double>*)energy, nmax);
+                     (cplx<double>*)dpsi, (cplx<double>*)apply_out, ph_re, ph_im, (cplx<double>*)energy, nmax,
+                     active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -622,13 +493,13 @@ template int launch_sandwich<double>(const SwItem*, int64_t, const DNode*, const
 template int launch_sandwich<float>(const SwItem*, int64_t, const DNode*, const void*, const void*, void*, int,
                                     hipStream_t);
 template int launch_apply<double>(const ApChunk*, int64_t, const ApTerm*, const DNode*, const void*, const void*,
-                                  void*, int, hipStream_t);
+                                  void*, int, hipStream_t, const uint8_t*);
 template int launch_apply<float>(const ApChunk*, int64_t, const ApTerm*, const DNode*, const void*, const void*,
-                                 void*, int, hipStream_t);
+                                 void*, int, hipStream_t, const uint8_t*);
 template int launch_finish<double>(const FinNode*, int64_t, const DNode*, const void*, const void*, const void*,
-                                   void*, void*, double, double, void*, int, int, hipStream_t);
+                                   void*, void*, double, double, void*, int, int, hipStream_t, const uint8_t*);
 template int launch_finish<float>(const FinNode*, int64_t, const DNode*, const void*, const void*, const void*,
-                                  void*, void*, double, double, void*, int, int, hipStream_t);
+                                  void*, void*, double, double, void*, int, int, hipStream_t, const uint8_t*);
 template int launch_heev<double>(const EigItem*, int64_t, const void*, void*, double, int, hipStream_t);
 template int launch_heev<float>(const EigItem*, int64_t, const void*, void*, double, int, hipStream_t);
 

@@ -69,11 +69,11 @@ int launch_sandwich(const SwItem* items, int64_t count, const DNode* nodes, cons
                     void* pool, int nmax, hipStream_t s);
 template <typename T>
 int launch_apply(const ApChunk* chunks, int64_t nchunks, const ApTerm* terms, const DNode* nodes, const void* psi,
-                 const void* pool, void* partial, int nmax, hipStream_t s);
+                 const void* pool, void* partial, int nmax, hipStream_t s, const uint8_t* active = nullptr);
 template <typename T>
 int launch_finish(const FinNode* fin, int64_t count, const DNode* nodes, const void* psi64, const void* partial,
                   const void* pool64, void* dpsi, void* apply_out, double ph_re, double ph_im, void* energy,
-                  int nmax, int nspf, hipStream_t s);
+                  int nmax, int nspf, hipStream_t s, const uint8_t* active = nullptr);
 // dynamic LDS of finish_kernel: Y and A (FP64, nmax each) and two nspf^2 matrices
 inline size_t finish_lds_bytes(int nmax, int nspf) {
   return (2 * (size_t)nmax + 2 * (size_t)nspf * nspf) * 16;

@@ -174,6 +174,13 @@ def sweep64_model(seed: int = 64):
     return text, coupled_ho(64, seed, c_sigma=0.01, one_mode_matrix=True)
 
 
+def cmf64_model(seed: int = 64):
+    """The 64-mode sweep tree and Hamiltonian with displaced Gaussian leaves
+    (q0 = 1, p0 = 0) for a Hartree-product CMF start."""
+    text = balanced_tree_text(64, 16, [8, 10, 11, 13, 14, 16], wfr0=1.0)
+    return text, coupled_ho(64, seed, c_sigma=0.01, one_mode_matrix=True)
+
+
 def grad32_model(seed: int = 32):
     """32-mode anharmonic model: 8 DVR, SPF 8 at the bottom and 24 above."""
     text = balanced_tree_text(32, 8, [8, 24, 24, 24, 24])
