@@ -169,9 +169,14 @@ int qtg_engine_energy(qtg_engine engine, double* energy, qtg_stream stream);
 /* which: 0 rho[node], 1 rho^-1[node], 2 h-matrix[term][node], 3 hole[term][node] */
 int qtg_engine_get_matrix(qtg_engine engine, int which, int64_t term, int64_t node, double* out,
                           qtg_stream stream);
-/* FLOPs of one H|Psi> sweep (8 per complex multiply-add) and item counts
- * (items[0] sandwich items, items[1] apply chunks, items[2] apply terms). */
+/* FLOPs of one H|Psi> sweep of the reference algorithm (8 per complex
+ * multiply-add; one represent, hole and Apply contraction per (term, active
+ * node) as TreeFunctions and HamiltonianRepresentation::Apply loop) and item
+ * counts (items[0] sandwich items, items[1] apply chunks, items[2] apply terms). */
 int qtg_engine_counts(qtg_engine engine, double* sweep_flops, int64_t* items);
+/* FLOPs one sweep of this engine's plan executes (fewer than the reference
+ * count where equal slots are shared and Apply summands merged). */
+int qtg_engine_plan_flops(qtg_engine engine, double* executed);
 /* Host-only variants (no device is touched): plan statistics, state size and
  * the random initial state of a tree. */
 int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* coeff,
@@ -184,8 +189,9 @@ int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest,
  * around every launch group; read_timing returns, per kernel group
  * [0 sandwich (represent/holes/density), 1 eigensolver, 2 apply, 3 finish,
  * 4 precision conversion, 5 matrix combination], the summed milliseconds,
- * launch counts and algorithmic FLOPs since set_timing(1).  Arrays hold 6
- * entries. */
+ * launch counts and algorithmic FLOPs since set_timing(1); the grouped path
+ * adds 6 mode products (gm_grouped), 7 hole contractions (gc_grouped),
+ * 8 batched solves, 9 split-K reductions.  Arrays hold 10 entries. */
 int qtg_engine_set_timing(qtg_engine engine, int on);
 /* waits for the engine's own stream */
 int qtg_engine_synchronize(qtg_engine engine);
@@ -197,7 +203,10 @@ int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, dou
  * share all slots but one; 0 keeps one work item per (term, node) as the
  * reference loops do.
  * "graph" (default 1): Derivative() on the engine's stream is captured once
- * into a HIP graph and replayed; 0 launches eagerly. */
+ * into a HIP graph and replayed; 0 launches eagerly.
+ * "path" (set before qtg_engine_set_hamiltonian; default 1): 1 runs every
+ * node contraction as grouped MFMA tile GEMMs through a device workspace (any
+ * node size); 0 keeps whole node tensors in LDS (nodes up to ~150 KB). */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
 
 /* ----------------------------------------------------------- CMF integrator */

@@ -75,6 +75,7 @@ def lib():
     L.qtg_engine_energy.argtypes = [_vp, C.POINTER(C.c_double), _vp]
     L.qtg_engine_get_matrix.argtypes = [_vp, C.c_int, _i64, _i64, _dp, _vp]
     L.qtg_engine_counts.argtypes = [_vp, C.POINTER(C.c_double), _lp]
+    L.qtg_engine_plan_flops.argtypes = [_vp, C.POINTER(C.c_double)]
     L.qtg_engine_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
     L.qtg_engine_set_timing.argtypes = [_vp, C.c_int]
     L.qtg_engine_synchronize.argtypes = [_vp]
@@ -208,13 +209,15 @@ def _ci(z: np.ndarray) -> np.ndarray:
 class Engine:
     """Device-resident tree + SOP Hamiltonian (qtg_engine_*)."""
 
-    def __init__(self, tree_text: str, H=None, precision: int = 0, compress: bool = True, graph: bool = True):
+    def __init__(self, tree_text: str, H=None, precision: int = 0, compress: bool = True, graph: bool = True,
+                 path: int = 1):
         init()
         p = _vp()
         check(lib().qtg_engine_create(tree_text.encode(), precision, C.byref(p)))
         self.ptr = p.value
         check(lib().qtg_engine_set_option(self.ptr, b"compress", float(compress)))
         check(lib().qtg_engine_set_option(self.ptr, b"graph", float(graph)))
+        check(lib().qtg_engine_set_option(self.ptr, b"path", float(path)))
         self.precision = precision
         self.psi_size = lib().qtg_engine_psi_size(self.ptr)
         self.nnodes = lib().qtg_engine_num_nodes(self.ptr)
@@ -278,16 +281,22 @@ class Engine:
         check(lib().qtg_engine_counts(self.ptr, C.byref(f), items))
         return f.value, items
 
+    def executed_flops(self) -> float:
+        """FLOPs one sweep of this engine's plan executes."""
+        f = C.c_double()
+        check(lib().qtg_engine_plan_flops(self.ptr, C.byref(f)))
+        return f.value
+
     KERNELS = ("sandwich_kernel", "heev_reginv_kernel", "apply_kernel", "finish_kernel", "to_f32",
-               "combine_kernel")
+               "combine_kernel", "gm_grouped", "gc_grouped", "solve_kernel", "gc_reduce_kernel")
 
     def set_timing(self, on: bool) -> None:
         check(lib().qtg_engine_set_timing(self.ptr, int(on)))
 
     def read_timing(self):
-        ms = np.zeros(6)
-        n = np.zeros(6, np.int64)
-        fl = np.zeros(6)
+        ms = np.zeros(len(self.KERNELS))
+        n = np.zeros(len(self.KERNELS), np.int64)
+        fl = np.zeros(len(self.KERNELS))
         check(lib().qtg_engine_read_timing(self.ptr, ms, n, fl))
         return {k: dict(ms=float(ms[i]), launches=int(n[i]), flops=float(fl[i]))
                 for i, k in enumerate(self.KERNELS)}

@@ -120,12 +120,18 @@ int qtg_engine_get_matrix(qtg_engine e, int which, int64_t term, int64_t node, d
 int qtg_engine_counts(qtg_engine e, double* sweep_flops, int64_t* items) {
   if (!e) return QTG_EARG;
   auto* en = static_cast<Engine*>(e);
-  if (sweep_flops) *sweep_flops = en->sweep_flops();
+  if (sweep_flops) *sweep_flops = en->reference_flops();
   if (items) {
     items[0] = en->sandwich_items();
     items[1] = en->apply_terms_chunks();
     items[2] = en->apply_terms();
   }
+  return QTG_OK;
+}
+
+int qtg_engine_plan_flops(qtg_engine e, double* executed) {
+  if (!e || !executed) return QTG_EARG;
+  *executed = static_cast<Engine*>(e)->sweep_flops();
   return QTG_OK;
 }
 
@@ -138,6 +144,7 @@ int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* co
   int rc = en.create(tree_text, 0, true);
   if (rc) return rc;
   en.compress_ = compress != 0;
+  en.path_ = 0;  // counts only: the work-item plan needs no device structures
   rc = en.set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool);
   if (rc) return rc;
   return qtg_engine_counts(&en, sweep_flops, items);
@@ -163,8 +170,7 @@ int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest,
 
 int qtg_engine_synchronize(qtg_engine e) {
   if (!e) return QTG_EARG;
-  QTG_HIP_TRY(hipStreamSynchronize(static_cast<Engine*>(e)->stream()));
-  return QTG_OK;
+  return static_cast<Engine*>(e)->synchronize();
 }
 
 int qtg_engine_set_timing(qtg_engine e, int on) {
@@ -186,6 +192,14 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   }
   if (!std::strcmp(key, "compress")) {
     en->compress_ = value != 0.0;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "path")) {
+    if (value != 0.0 && value != 1.0) {
+      set_error("option path: 0 (LDS-resident node kernels) or 1 (grouped MFMA contractions)");
+      return QTG_EARG;
+    }
+    en->path_ = (int)value;
     return QTG_OK;
   }
   if (!std::strcmp(key, "graph")) {

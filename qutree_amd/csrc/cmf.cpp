@@ -340,12 +340,7 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
     if (nact == 0) break;  // only trailing accept copies were left
     ++nticks_;
     const uint8_t* dact = (const uint8_t*)(ds + o5);
-    if ((rc = launch_apply<double>(E.d_chunks_, E.nChunks_, E.d_terms_, E.d_nodes_, X_, E.d_pool_, E.d_partial_,
-                                   E.nmax_, s_, dact)))
-      return rc;
-    if ((rc = launch_finish<double>(E.d_fin_, E.nFin_, E.d_nodes_, X_, E.d_partial_, E.d_pool64_, D_, nullptr,
-                                    ph_re_, ph_im_, nullptr, E.nmax_, E.nspf_, s_, dact)))
-      return rc;
+    if ((rc = E.apply_finish((const double*)X_, (double*)D_, dact, ph_re_, ph_im_, nullptr, s_))) return rc;
     if ((rc = launch_programs((NodeProg*)(ds + o3), (int64_t)ppost.size(), (VecOp*)(ds + o4), bb, E.d_nodes_,
                               (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_)))
       return rc;
@@ -419,12 +414,7 @@ int Cmf::output(double time, CmfResult& res) {
   const size_t off = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode);
   QTG_HIP_TRY(hipMemcpyAsync(ds + off, act.data(), NN_, hipMemcpyHostToDevice, s_));
   QTG_HIP_TRY(hipStreamSynchronize(s_));
-  if ((rc = launch_apply<double>(E.d_chunks_, E.nChunks_, E.d_terms_, E.d_nodes_, P_, E.d_pool_, E.d_partial_,
-                                 E.nmax_, s_, ds + off)))
-    return rc;
-  if ((rc = launch_finish<double>(E.d_fin_, E.nFin_, E.d_nodes_, P_, E.d_partial_, E.d_pool64_, D_, nullptr, 1.0,
-                                  0.0, energy_, E.nmax_, E.nspf_, s_, ds + off)))
-    return rc;
+  if ((rc = E.apply_finish((const double*)P_, (double*)D_, ds + off, 1.0, 0.0, energy_, s_))) return rc;
   double e[4];
   QTG_HIP_TRY(hipMemcpyAsync(e, energy_, sizeof(e), hipMemcpyDeviceToHost, s_));
   QTG_HIP_TRY(hipStreamSynchronize(s_));

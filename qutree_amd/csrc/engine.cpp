@@ -32,6 +32,9 @@ Engine::~Engine() {
   invalidate_graph();
   release();
   for (auto e : evpool_) (void)hipEventDestroy(e);
+  if (evFork_) (void)hipEventDestroy(evFork_);
+  if (evHeev_) (void)hipEventDestroy(evHeev_);
+  if (side_) (void)hipStreamDestroy(side_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -76,10 +79,8 @@ int Engine::create(const std::string& tree_text, int precision, bool host_only) 
     nmax_ = std::max(nmax_, n.N);
     nspf_ = std::max(nspf_, n.n);
   }
-  const size_t es = c64_ ? 8 : 16;
-  if (work_lds_bytes(nmax_, es) > kLdsBudget || finish_lds_bytes(nmax_, nspf_) > kLdsBudget || nspf_ > 64) {
-    set_error("engine: node tensors of " + std::to_string(nmax_) + " elements / " + std::to_string(nspf_) +
-              " SPFs exceed the LDS-resident kernels of this build");
+  if (nspf_ > 64) {
+    set_error("engine: more than 64 SPFs at a node is not supported");
     return QTG_ESHAPE;
   }
   return QTG_OK;
@@ -88,6 +89,9 @@ int Engine::create(const std::string& tree_text, int precision, bool host_only) 
 int Engine::materialize() {
   invalidate_graph();
   if (!stream_) QTG_HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (!side_) QTG_HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+  if (!evFork_) QTG_HIP_TRY(hipEventCreateWithFlags(&evFork_, hipEventDisableTiming));
+  if (!evHeev_) QTG_HIP_TRY(hipEventCreateWithFlags(&evHeev_, hipEventDisableTiming));
   std::vector<DNode> dn;
   for (auto& n : tree_.nodes) {
     DNode d{};
@@ -168,6 +172,35 @@ int Engine::plan() {
       set_error("engine: nodes with more than two children are not supported");
       return QTG_ESHAPE;
     }
+  // FLOPs of the reference algorithm (one represent and one hole contraction
+  // per (term, active non-top node), one density item per non-top node, Apply
+  // per (term, active node), projector and rho^-1 per non-top node), the
+  // count a sweep is measured against whatever this plan shares or merges.
+  refFlops_ = 0.0;
+  for (int l = 0; l < L; ++l)
+    for (auto& v : tree_.nodes) {
+      if (!active_[l][v.addr]) continue;
+      double ch = 0.0;
+      if (v.bottom) ch = 8.0 * v.N * v.dims[0];
+      else
+        for (int c : v.children)
+          if (active_[l][c]) ch += 8.0 * v.N * tree_.nodes[c].n;
+      if (v.top()) {
+        refFlops_ += ch;
+        continue;
+      }
+      refFlops_ += 2.0 * (ch + 8.0 * v.N * v.n);  // represent + apply
+      const HNode& p = tree_.nodes[v.parent];
+      for (int sb : p.children)
+        if (sb != v.addr && active_[l][sb]) refFlops_ += 8.0 * p.N * tree_.nodes[sb].n;
+      if (!p.top()) refFlops_ += 8.0 * p.N * p.n;
+      refFlops_ += 8.0 * p.N * v.n;
+    }
+  for (auto& v : tree_.nodes) {
+    if (v.top()) continue;
+    const HNode& p = tree_.nodes[v.parent];
+    refFlops_ += 8.0 * p.N * v.n + (p.top() ? 0.0 : 8.0 * p.N * p.n) + 4 * 8.0 * v.N * v.n;
+  }
   auto nn = [&](int a) { return (int64_t)tree_.nodes[a].n * tree_.nodes[a].n; };
   // ---- pool layout (complex elements): leaf matrices, h slots, hole slots
   int64_t off = 0;
@@ -344,6 +377,9 @@ int Engine::plan() {
   std::vector<CombTerm> combTerm, combTerm2;
   double combFlops = 0.0;
   int64_t part = 0;
+  summ_.assign(NN, {});
+  smode_.assign(NN, {});
+  sdim_.assign(NN, {});
   for (auto& v : tree_.nodes) {
     FinNode fn{};
     fn.node = v.addr;
@@ -463,6 +499,10 @@ int Engine::plan() {
     } else {
       groups = tl;
     }
+    summ_[v.addr].clear();
+    for (auto& g : groups) summ_[v.addr].push_back(Summand{g.coef, g.ids});
+    smode_[v.addr] = smode;
+    sdim_[v.addr] = sdim;
     std::vector<ApTerm> vt;
     for (auto& g : groups) {
       ApTerm t{};
@@ -515,6 +555,18 @@ int Engine::plan() {
   h_terms_ = terms;
   h_chunks_ = chunks;
   h_fin_ = fin;
+  lflops_ = flops_;
+  if (path_ == 0) {
+    const size_t es = c64_ ? 8 : 16;
+    if (work_lds_bytes(nmax_, es) > kLdsBudget || finish_lds_bytes(nmax_, nspf_) > kLdsBudget || nspf_ > 64) {
+      set_error("engine: node tensors of " + std::to_string(nmax_) + " elements / " + std::to_string(nspf_) +
+                " SPFs exceed the LDS-resident kernels; use the grouped-contraction path");
+      return QTG_ESHAPE;
+    }
+  } else {
+    int grc = plan_gemm();
+    if (grc) return grc;
+  }
   if (host_only_) {
     planned_ = true;
     return QTG_OK;
@@ -540,10 +592,28 @@ int Engine::plan() {
   device_buffers_.push_back(d_combOut2_);
   if ((rc = up(&d_combTerm2_, combTerm2))) return rc;
   device_buffers_.push_back(d_combTerm2_);
+  if (path_ == 1) {
+    if ((rc = up(&d_gm_, h_gm_))) return rc;
+    device_buffers_.push_back(d_gm_);
+    if ((rc = up(&d_gc_, h_gc_))) return rc;
+    device_buffers_.push_back(d_gc_);
+    if ((rc = up(&d_solve_, h_solve_))) return rc;
+    device_buffers_.push_back(d_solve_);
+    if ((rc = up(&d_combOut3_, h_comb3_))) return rc;
+    device_buffers_.push_back(d_combOut3_);
+    if ((rc = up(&d_combTerm3_, h_combt3_))) return rc;
+    device_buffers_.push_back(d_combTerm3_);
+    if ((rc = up(&d_gmsplit_, h_gmsplit_))) return rc;
+    device_buffers_.push_back(d_gmsplit_);
+    if ((rc = up(&d_gcsplit_, h_gcsplit_))) return rc;
+    device_buffers_.push_back(d_gcsplit_);
+    if ((rc = alloc(&d_part_, 16 * std::max<int64_t>(partSize_, 1)))) return rc;
+  }
   const size_t es = c64_ ? 8 : 16;
+  if (path_ == 1 && (rc = alloc(&d_work_, es * std::max<int64_t>(workSize_, 1)))) return rc;
   if ((rc = alloc(&d_pool_, es * poolSize_))) return rc;
   if ((rc = alloc((void**)&d_pool64_, 16 * pool64Size_))) return rc;
-  if ((rc = alloc(&d_partial_, es * partialSize_))) return rc;
+  if ((rc = alloc(&d_partial_, es * (path_ == 1 ? 1 : partialSize_)))) return rc;
   // leaf matrices (constant)
   std::vector<double> lm;
   for (auto& m : sop_.leafMatrices)
@@ -551,13 +621,21 @@ int Engine::plan() {
       lm.push_back(z.real());
       lm.push_back(z.imag());
     }
-  if (!lm.empty()) {
+  auto put = [&](int64_t off, const std::vector<double>& v) -> int {
+    if (v.empty()) return QTG_OK;
     if (c64_) {
-      std::vector<float> lf(lm.begin(), lm.end());
-      QTG_HIP_TRY(hipMemcpy(d_pool_, lf.data(), sizeof(float) * lf.size(), hipMemcpyHostToDevice));
+      std::vector<float> lf(v.begin(), v.end());
+      QTG_HIP_TRY(hipMemcpy((char*)d_pool_ + 8 * off, lf.data(), sizeof(float) * lf.size(), hipMemcpyHostToDevice));
     } else {
-      QTG_HIP_TRY(hipMemcpy(d_pool_, lm.data(), sizeof(double) * lm.size(), hipMemcpyHostToDevice));
+      QTG_HIP_TRY(hipMemcpy((char*)d_pool_ + 16 * off, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
     }
+    return QTG_OK;
+  };
+  if ((rc = put(0, lm))) return rc;
+  for (auto& id : idSlots_) {
+    std::vector<double> e(2 * (size_t)id.second * id.second, 0.0);
+    for (int i = 0; i < id.second; ++i) e[2 * ((size_t)i + (size_t)i * id.second)] = 1.0;
+    if ((rc = put(id.first, e))) return rc;
   }
   planned_ = true;
   return QTG_OK;
@@ -630,17 +708,20 @@ int Engine::read_timing(double* ms, int64_t* launches, double* flops) {
   return QTG_OK;
 }
 
+int Engine::convert_psi(hipStream_t s) {
+  const int64_t n = 2 * tree_.total;
+  int rc = begin_mark(4, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_psi64_, d_psi32_, n);
+  QTG_LAUNCH_CHECK();
+  return end_mark(s);
+}
+
 template <typename T>
 int Engine::build_t(hipStream_t s) {
   const void* psi = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
   int rc;
-  if (c64_) {
-    const int64_t n = 2 * tree_.total;
-    if ((rc = begin_mark(4, s))) return rc;
-    hipLaunchKernelGGL(to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_psi64_, d_psi32_, n);
-    QTG_LAUNCH_CHECK();
-    if ((rc = end_mark(s))) return rc;
-  }
+  if (c64_ && (rc = convert_psi(s))) return rc;
   size_t li = 0;
   auto sw = [&](const std::pair<int64_t, int64_t>& r) -> int {
     const double fl = li < layerFlops_.size() ? layerFlops_[li] : 0.0;
@@ -678,6 +759,7 @@ int Engine::build(hipStream_t s) {
     set_error("engine: no Hamiltonian");
     return QTG_ESTATE;
   }
+  if (path_ == 1) return c64_ ? build_gemm<float>(s) : build_gemm<double>(s);
   return c64_ ? build_t<float>(s) : build_t<double>(s);
 }
 
@@ -701,6 +783,11 @@ int Engine::layer_derivative_all(hipStream_t s, double ph_re, double ph_im) {
   if (!built_) {
     set_error("engine: build() first");
     return QTG_ESTATE;
+  }
+  if (path_ == 1) {
+    const void* psiT = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
+    return c64_ ? apply_gemm<float>(s, psiT, d_psi64_, d_dpsi64_, nullptr, ph_re, ph_im)
+                : apply_gemm<double>(s, psiT, d_psi64_, d_dpsi64_, nullptr, ph_re, ph_im);
   }
   return c64_ ? apply_t<float>(s, ph_re, ph_im) : apply_t<double>(s, ph_re, ph_im);
 }
@@ -747,7 +834,8 @@ int Engine::derivative(hipStream_t s, double ph_re, double ph_im) {
 
 int Engine::energy(hipStream_t s, double* out) {
   double e[4];
-  QTG_HIP_TRY(hipMemcpyAsync(e, d_energy_, sizeof(e), hipMemcpyDeviceToHost, s));
+  const double* src = (path_ == 1) ? d_pool64_ + 2 * energySlot_ : d_energy_;
+  QTG_HIP_TRY(hipMemcpyAsync(e, src, sizeof(e), hipMemcpyDeviceToHost, s));
   QTG_HIP_TRY(hipStreamSynchronize(s));
   // <A_top|Apply(A_top)> / <A_top|A_top>
   const cd num(e[0], e[1]), den(e[2], e[3]);
@@ -768,6 +856,7 @@ int Engine::get_matrix(int which, int term, int node, double* out, hipStream_t s
     off = (which == 2) ? hslot_[term][node] : holeslot_[term][node];
   }
   if (off < 0) return QTG_EARG;
+  if (heevPending_) QTG_HIP_TRY(hipStreamWaitEvent(s, evHeev_, 0));  // rho^-1 of the side stream
   if (from64 || !c64_) {
     const void* base = from64 ? (const void*)d_pool64_ : d_pool_;
     QTG_HIP_TRY(hipMemcpyAsync(out, (const char*)base + 16 * off, 16 * (size_t)n * n, hipMemcpyDeviceToHost, s));

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gemm_ops.hpp"
 #include "model.hpp"
 #include "tree_ops.hpp"
 
@@ -34,7 +35,8 @@ class Engine {
 
   const HTree& tree() const { return tree_; }
   const HSop& sop() const { return sop_; }
-  double sweep_flops() const { return flops_; }
+  double sweep_flops() const { return flops_; }        // executed by this plan
+  double reference_flops() const { return refFlops_; }  // of the reference algorithm
   double* psi64() { return d_psi64_; }
   double* dpsi64() { return d_dpsi64_; }
   double* apply64() { return d_apply64_; }
@@ -47,11 +49,27 @@ class Engine {
   int chunkTermsCompressed_ = 4;
   bool compress_ = true;  // share equal h/hole matrices, aggregate Apply summands
   bool graph_ = true;     // replay Derivative() from a captured HIP graph
+  // 0: LDS-resident node kernels (node tensors up to ~150 KB);
+  // 1: grouped MFMA contractions through a global workspace (any node size)
+  int path_ = 1;
+  // Apply + projector + rho^-1 on an arbitrary FP64 state (complex128 engines):
+  // dpsi = -i phase rho^-1 (1 - P) Apply(psi) at every node whose active flag
+  // is set (all when active is null); energy (4 doubles: <A|HA>, <A|A> of the
+  // top node) is written when not null.  Uses the last build's matrices.
+  int apply_finish(const double* psi64, double* dpsi64, const uint8_t* active, double ph_re, double ph_im,
+                   double* energy_dev, hipStream_t s);
   hipStream_t stream() const { return stream_; }
+  // waits for the engine's streams (its own and the eigensolver's side stream)
+  int synchronize() {
+    QTG_HIP_TRY(hipStreamSynchronize(stream_));
+    if (side_) QTG_HIP_TRY(hipStreamSynchronize(side_));
+    return QTG_OK;
+  }
   void invalidate_graph();
   // Per-kernel timing with HIP events recorded on the launch stream around
   // every launch group: 0 sandwich, 1 heev, 2 apply, 3 finish, 4 convert, 5 combine.
-  static constexpr int kPhases = 6;
+  // 6 gm_grouped, 7 gc_grouped, 8 solve, 9 gc_reduce.
+  static constexpr int kPhases = 10;
   int set_timing(bool on);
   int read_timing(double* ms, int64_t* launches, double* flops);
 
@@ -61,6 +79,27 @@ class Engine {
   template <typename T>
   int apply_t(hipStream_t s, double ph_re, double ph_im);
   int plan();
+  int plan_gemm();
+  int convert_psi(hipStream_t s);  // FP64 state -> FP32 copy (complex64 engines)
+  template <typename T>
+  int build_gemm(hipStream_t s);
+  template <typename T>
+  int apply_gemm(hipStream_t s, const void* psiT, const double* psi64, double* dpsi64, const uint8_t* active,
+                 double ph_re, double ph_im);
+  // one launch of the grouped path
+  enum Buf { B_PSI = 0, B_WORK, B_POOL, B_PSI64, B_APPLY64, B_DPSI64, B_POOL64, B_COUNT };
+  struct GLaunch {
+    int kind;    // 0 mode products, 1 hole contractions, 2 solve
+    int prec;    // 0: engine precision, 1: engine precision in, FP64 out, 2: FP64
+    int64_t d0, nd, ntasks;
+    int64_t s0, ns;    // split descriptors of this launch (reduce pass)
+    int x0, x1, m, o;  // buffer ids
+    bool scaled;       // multiply alphas by the launch scale (-i phase)
+    double flops;
+  };
+  template <typename T>
+  int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
+                   hipStream_t s);
   int materialize();
   std::vector<std::pair<int, int>> restrict_in(int l, int a) const;
   std::vector<std::pair<int, int>> restrict_out(int l, int a) const;
@@ -80,11 +119,37 @@ class Engine {
   int nmax_ = 0, nspf_ = 1;
   std::vector<std::vector<char>> active_;
   std::vector<int64_t> leafOff_, rhoslot_, rhoinv_;
+  struct Summand {
+    cd coef;
+    std::vector<int64_t> ids;  // matrix per slot (-1: identity)
+  };
+  std::vector<std::vector<Summand>> summ_;    // Apply summands per node
+  std::vector<std::vector<int>> smode_, sdim_;  // slot legs and sizes per node
+  double lflops_ = 0.0;                       // sweep flops of the LDS path
+  // grouped path
+  std::vector<GmDesc> h_gm_, h_gmsplit_;
+  std::vector<GcDesc> h_gc_, h_gcsplit_;
+  int64_t partSize_ = 0;  // complex elements of the split-K partial tiles
+  GmDesc* d_gmsplit_ = nullptr;
+  GcDesc* d_gcsplit_ = nullptr;
+  void* d_part_ = nullptr;
+  std::vector<SolveItem> h_solve_;
+  std::vector<GLaunch> gBuild_, gApply_;
+  std::vector<std::pair<int64_t, int>> idSlots_;  // identity matrices in the pool (offset, n)
+  std::vector<CombOut> h_comb3_;
+  std::vector<CombTerm> h_combt3_;
+  int64_t workSize_ = 0, energySlot_ = -1, nComb3_ = 0;
+  GmDesc* d_gm_ = nullptr;
+  GcDesc* d_gc_ = nullptr;
+  SolveItem* d_solve_ = nullptr;
+  CombOut* d_combOut3_ = nullptr;
+  CombTerm* d_combTerm3_ = nullptr;
+  void* d_work_ = nullptr;
   std::vector<std::vector<int64_t>> hslot_, holeslot_;
   std::vector<std::pair<int64_t, int64_t>> repRange_, holeRange_;
   int64_t poolSize_ = 0, pool64Size_ = 0, partialSize_ = 0;
   int64_t nSw_ = 0, nChunks_ = 0, nFin_ = 0, nEig_ = 0, nComb_ = 0, nComb2_ = 0;
-  double flops_ = 0.0;
+  double flops_ = 0.0, refFlops_ = 0.0;
 
   DNode* d_nodes_ = nullptr;
   SwItem* d_sw_ = nullptr;
@@ -106,6 +171,9 @@ class Engine {
   double* d_energy_ = nullptr;
   std::vector<void*> device_buffers_;
   hipStream_t stream_ = nullptr;
+  hipStream_t side_ = nullptr;  // density eigensolver, overlapping Apply
+  hipEvent_t evFork_ = nullptr, evHeev_ = nullptr;
+  bool heevPending_ = false;
   hipGraph_t graph_h_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   double graph_ph_re_ = 0.0, graph_ph_im_ = 0.0;

@@ -1,0 +1,562 @@
+// Grouped-contraction path of the engine: every node contraction of a sweep
+// is a descriptor of the MFMA tile kernels (gemm_ops.hip), so node tensors of
+// any size work and each launch spreads over many wavefronts.
+//
+// Build.  A sandwich item out = <Psi_p| F_1 x F_2 (x F_3) |Psi_p> over all
+// legs but k (represent: k = last leg; holes and densities: k = the child's
+// leg) is factored as
+//     out(j,l) = sum_{legs != k} conj(Zb(.., j, ..)) Zk(.., l, ..),
+//     Zb = F_b^H x_b Psi_p   (factor on the lowest leg other than k),
+//     Zk = F_c x_c (F_d x_d Psi_p)   (the remaining factors),
+// and equal Zb / Zk tensors of a layer are computed once.  With shared slots
+// (equal term restrictions share one matrix) this turns the per-term work
+// of TreeFunctions::represent / contraction into a few mode-product launches
+// followed by one grouped hole-contraction launch per layer.
+//
+// Apply.  For node v with summands s (coefficient c_s, matrices on the
+// node's legs), the last leg L is split off:
+//     Apply(v) = sum_s c_s G_s x_L T_s,   T_s = (prod_{legs != L} F) Psi_v,
+// and the T_s are stored one after another, so the sum over s is one mode
+// product over the stacked index (s, p') with the stacked matrix
+// [c_1 G_1 | c_2 G_2 | ...] (G_s the hole matrix, identity at the top).
+//
+// Finish (FP64).  S = A^H A, X = A^H Y (grouped hole contractions), X <-
+// S^-1 X and W = rho^-1 X^T (batched solve), then
+//     dpsi = f rho^-1 x_L Y - f W x_L A,   f = -i phase,
+// which is -i phase rho^-1 (1 - P) Apply of HamiltonianRepresentation.cpp:53-72.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "engine.hpp"
+
+namespace qtg {
+
+namespace {
+
+constexpr int kGmSplitK = 128;  // K per task of a split mode product
+constexpr int kGcSplitK = 512;  // K per task of a split hole contraction
+
+void view(const HNode& v, int k, int64_t& bef, int& nk, int64_t& aft) {
+  bef = 1;
+  aft = 1;
+  for (int i = 0; i < k; ++i) bef *= v.dims[i];
+  nk = v.dims[k];
+  for (int i = k + 1; i < v.order(); ++i) aft *= v.dims[i];
+}
+
+GmDesc gm_desc(const HNode& v, int leg, int64_t x, int xsel, int64_t m, int64_t out, int op, int node) {
+  GmDesc g{};
+  int64_t bef, aft;
+  int nk;
+  view(v, leg, bef, nk, aft);
+  g.x = x;
+  g.m = m;
+  g.out = out;
+  g.bef = (int32_t)bef;
+  g.nin = nk;
+  g.nout = nk;
+  g.aft = (int32_t)aft;
+  g.op = op;
+  g.acc = 0;
+  g.are = 1.0;
+  g.aim = 0.0;
+  g.node = node;
+  g.xsel = xsel;
+  return g;
+}
+
+}  // namespace
+
+int Engine::plan_gemm() {
+  const int NN = (int)tree_.nodes.size();
+  h_gm_.clear();
+  h_gc_.clear();
+  h_gmsplit_.clear();
+  h_gcsplit_.clear();
+  partSize_ = 0;
+  h_solve_.clear();
+  gBuild_.clear();
+  gApply_.clear();
+  idSlots_.clear();
+  h_comb3_.clear();
+  h_combt3_.clear();
+  int64_t poolOff = poolSize_;
+  int64_t pool64Off = pool64Size_;
+  workSize_ = 0;
+  double flops = 0.0;
+  for (auto& x : phaseFlops_) x = 0.0;
+
+  auto push_gm = [&](std::vector<GLaunch>& L, std::vector<GmDesc>& descs, int prec, int x0, int x1, int m, int o,
+                     bool scaled) {
+    if (descs.empty()) return;
+    GLaunch gl{};
+    gl.kind = 0;
+    gl.prec = prec;
+    gl.d0 = (int64_t)h_gm_.size();
+    gl.nd = (int64_t)descs.size();
+    gl.s0 = (int64_t)h_gmsplit_.size();
+    int64_t t = 0, part = 0;
+    double fl = 0.0;
+    for (auto& d : descs) {
+      // long contractions (stacked Apply summands) are split along K into
+      // partial tiles that a second launch sums in a fixed order
+      if (d.op != GM_COPY && d.nin > 2 * kGmSplitK) {
+        d.ksplit = (d.nin + kGmSplitK - 1) / kGmSplitK;
+        d.part = part;
+        part += (int64_t)d.ksplit * d.bef * d.aft * d.nout;
+        h_gmsplit_.push_back(d);
+      }
+      d.tile0 = t;
+      t += gm_tasks(d);
+      if (d.op != GM_COPY) fl += 8.0 * d.bef * d.aft * (double)d.nin * d.nout;
+      h_gm_.push_back(d);
+    }
+    gl.ns = (int64_t)h_gmsplit_.size() - gl.s0;
+    partSize_ = std::max(partSize_, part);
+    gl.ntasks = t;
+    gl.x0 = x0;
+    gl.x1 = x1;
+    gl.m = m;
+    gl.o = o;
+    gl.scaled = scaled;
+    gl.flops = fl;
+    flops += fl;
+    L.push_back(gl);
+  };
+  auto push_gc = [&](std::vector<GLaunch>& L, std::vector<GcDesc>& descs, int prec, int x0, int x1, int o) {
+    if (descs.empty()) return;
+    GLaunch gl{};
+    gl.kind = 1;
+    gl.prec = prec;
+    gl.d0 = (int64_t)h_gc_.size();
+    gl.nd = (int64_t)descs.size();
+    gl.s0 = (int64_t)h_gcsplit_.size();
+    int64_t t = 0, part = 0;
+    double fl = 0.0;
+    for (auto& d : descs) {
+      const int64_t K = (int64_t)d.bef * d.aft;
+      if (K > 2 * kGcSplitK) {
+        d.ksplit = (int32_t)((K + kGcSplitK - 1) / kGcSplitK);
+        d.part = part;
+        part += (int64_t)d.ksplit * d.nx * d.ny;
+        h_gcsplit_.push_back(d);
+      }
+      d.tile0 = t;
+      t += gc_tasks(d);
+      fl += 8.0 * d.bef * d.aft * (double)d.nx * d.ny;
+      h_gc_.push_back(d);
+    }
+    gl.ns = (int64_t)h_gcsplit_.size() - gl.s0;
+    partSize_ = std::max(partSize_, part);
+    gl.ntasks = t;
+    gl.x0 = x0;
+    gl.x1 = x1;
+    gl.m = -1;
+    gl.o = o;
+    gl.scaled = false;
+    gl.flops = fl;
+    flops += fl;
+    L.push_back(gl);
+  };
+
+  // ---------------------------------------------------------------- build
+  auto sandwich_layer = [&](const std::pair<int64_t, int64_t>& r) {
+    if (r.second == r.first) return;
+    std::map<std::tuple<int, int, int64_t>, int64_t> zbra;             // (node, leg, mat) -> work offset
+    std::map<std::vector<int64_t>, int64_t> zket1, zket2;              // (node, leg, mat[, leg, mat])
+    std::vector<GmDesc> lv0, lv1;
+    std::vector<GcDesc> con;
+    int64_t w = 0;
+    for (int64_t i = r.first; i < r.second; ++i) {
+      const SwItem& it = h_items_[i];
+      const HNode& p = tree_.nodes[it.node];
+      const int k = it.k;
+      const int braleg = (k == 0) ? 1 : 0;
+      // bra tensor
+      int64_t xoff = p.off;
+      int xsel = 0;
+      std::vector<std::pair<int, int64_t>> ket;
+      for (int f = 0; f < it.nf; ++f) {
+        if (it.fmode[f] == braleg) {
+          auto key = std::make_tuple((int)it.node, braleg, it.fmat[f]);
+          auto z = zbra.find(key);
+          if (z == zbra.end()) {
+            lv0.push_back(gm_desc(p, braleg, p.off, 0, it.fmat[f], w, GM_H, it.node));
+            z = zbra.emplace(key, w).first;
+            w += p.N;
+          }
+          xoff = z->second;
+          xsel = 1;
+        } else {
+          ket.push_back({it.fmode[f], it.fmat[f]});
+        }
+      }
+      std::sort(ket.begin(), ket.end(), [](auto& a, auto& b) { return a.first > b.first; });
+      int64_t yoff = p.off;
+      int ysel = 0;
+      if (!ket.empty()) {
+        std::vector<int64_t> key1 = {it.node, ket[0].first, ket[0].second};
+        auto z = zket1.find(key1);
+        if (z == zket1.end()) {
+          lv0.push_back(gm_desc(p, ket[0].first, p.off, 0, ket[0].second, w, GM_N, it.node));
+          z = zket1.emplace(key1, w).first;
+          w += p.N;
+        }
+        yoff = z->second;
+        ysel = 1;
+        if (ket.size() > 1) {
+          std::vector<int64_t> key2 = key1;
+          key2.push_back(ket[1].first);
+          key2.push_back(ket[1].second);
+          auto z2 = zket2.find(key2);
+          if (z2 == zket2.end()) {
+            lv1.push_back(gm_desc(p, ket[1].first, yoff, 1, ket[1].second, w, GM_N, it.node));
+            z2 = zket2.emplace(key2, w).first;
+            w += p.N;
+          }
+          yoff = z2->second;
+        }
+      }
+      GcDesc g{};
+      int64_t bef, aft;
+      int nk;
+      view(p, k, bef, nk, aft);
+      g.x = xoff;
+      g.y = yoff;
+      g.out = it.out;
+      g.bef = (int32_t)bef;
+      g.nx = g.ny = nk;
+      g.aft = (int32_t)aft;
+      g.acc = 0;
+      g.node = it.node;
+      g.ksplit = 1;
+      g.sel = xsel | (ysel << 1);
+      con.push_back(g);
+    }
+    workSize_ = std::max(workSize_, w);
+    const double f0 = flops;
+    push_gm(gBuild_, lv0, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+    push_gm(gBuild_, lv1, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+    push_gc(gBuild_, con, 0, B_PSI, B_WORK, B_POOL);
+    (void)f0;
+  };
+  for (auto& r : repRange_) sandwich_layer(r);
+  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd]);
+
+  // ---------------------------------------------------------------- apply
+  std::map<int, int64_t> idmap;
+  auto identity = [&](int n) {
+    auto f = idmap.find(n);
+    if (f != idmap.end()) return f->second;
+    const int64_t o = poolOff;
+    poolOff += (int64_t)n * n;
+    idSlots_.push_back({o, n});
+    idmap[n] = o;
+    return o;
+  };
+  int maxLevels = 0;
+  for (auto& v : tree_.nodes) maxLevels = std::max(maxLevels, v.order() - 1);
+  std::vector<std::vector<GmDesc>> levels(maxLevels);
+  std::vector<GmDesc> stacked;
+  int64_t w = 0;
+  for (auto& v : tree_.nodes) {
+    const int L = v.order() - 1;
+    const int nL = v.dims[L];
+    const auto& sm = summ_[v.addr];
+    const auto& smode = smode_[v.addr];
+    if (sm.empty()) {
+      // no active term: Apply(v) = 0
+      GmDesc g = gm_desc(v, L, v.off, 0, 0, v.off, GM_COPY, v.addr);
+      g.are = 0.0;
+      stacked.push_back(g);
+      continue;
+    }
+    const int64_t stack0 = w;
+    w += (int64_t)sm.size() * v.N;
+    const int64_t mstack = poolOff;
+    poolOff += (int64_t)sm.size() * nL * nL;
+    // per summand: factor on each leg (-1 = none), the L-leg matrix
+    std::map<std::vector<int64_t>, int64_t> zmemo;  // chain key -> work offset
+    for (size_t s = 0; s < sm.size(); ++s) {
+      std::vector<int64_t> fac(L, -1);
+      int64_t gL = -1;
+      for (size_t sl = 0; sl < smode.size(); ++sl) {
+        if (sm[s].ids[sl] < 0) continue;
+        if (smode[sl] == L) gL = sm[s].ids[sl];
+        else fac[smode[sl]] = sm[s].ids[sl];
+      }
+      // chain from the highest leg below L down to leg 0; the last level
+      // writes into the stack slot and sits in the last launch
+      int64_t cur = v.off;
+      int cursel = 0;
+      std::vector<int64_t> key = {v.addr};
+      for (int leg = L - 1; leg >= 0; --leg) {
+        const int lvl = maxLevels - 1 - leg;  // leg 0 -> last launch
+        const bool last = (leg == 0);
+        const int64_t slot = stack0 + (int64_t)s * v.N;
+        if (fac[leg] < 0) {
+          if (last) levels[lvl].push_back(gm_desc(v, 0, cur, cursel, 0, slot, GM_COPY, v.addr));
+          continue;
+        }
+        key.push_back(leg);
+        key.push_back(fac[leg]);
+        if (last) {
+          levels[lvl].push_back(gm_desc(v, leg, cur, cursel, fac[leg], slot, GM_N, v.addr));
+        } else {
+          auto z = zmemo.find(key);
+          if (z == zmemo.end()) {
+            const int64_t o = w;
+            w += v.N;
+            levels[lvl].push_back(gm_desc(v, leg, cur, cursel, fac[leg], o, GM_N, v.addr));
+            z = zmemo.emplace(key, o).first;
+          }
+          cur = z->second;
+          cursel = 1;
+        }
+      }
+      // stacked matrix block s = c_s G_s
+      CombOut co{};
+      co.out = mstack + (int64_t)s * nL * nL;
+      co.n = nL;
+      co.t0 = (int32_t)h_combt3_.size();
+      CombTerm ct{};
+      ct.cre = sm[s].coef.real();
+      ct.cim = sm[s].coef.imag();
+      ct.src = (gL >= 0) ? gL : identity(nL);
+      h_combt3_.push_back(ct);
+      co.t1 = (int32_t)h_combt3_.size();
+      h_comb3_.push_back(co);
+    }
+    GmDesc g{};
+    g.x = stack0;
+    g.xsel = 1;
+    g.m = mstack;
+    g.out = v.off;
+    g.bef = (int32_t)(v.N / nL);
+    g.nin = (int32_t)(sm.size() * nL);
+    g.nout = nL;
+    g.aft = 1;
+    g.op = GM_N;
+    g.acc = 0;
+    g.are = 1.0;
+    g.aim = 0.0;
+    g.node = v.addr;
+    stacked.push_back(g);
+  }
+  workSize_ = std::max(workSize_, w);
+  const double fa0 = flops;
+  for (auto& lv : levels) push_gm(gApply_, lv, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+  push_gm(gApply_, stacked, 1, B_PSI, B_WORK, B_POOL, B_APPLY64, false);
+  phaseFlops_[2] = flops - fa0;
+
+  // --------------------------------------------------------------- finish
+  const double ff0 = flops;
+  std::vector<GcDesc> fin;
+  std::vector<GmDesc> g1, g2;
+  energySlot_ = pool64Off;
+  pool64Off += 2;
+  for (auto& v : tree_.nodes) {
+    const int L = v.order() - 1;
+    const int n = v.dims[L];
+    GcDesc g{};
+    g.bef = (int32_t)(v.N / n);
+    g.aft = 1;
+    g.nx = g.ny = n;
+    g.node = v.addr;
+    g.ksplit = 1;
+    g.x = g.y = v.off;
+    if (v.top()) {
+      // energy: <A|Y> and <A|A> of the top node (one state)
+      g.nx = g.ny = 1;
+      g.bef = (int32_t)v.N;
+      g.out = energySlot_;
+      g.sel = 2;
+      fin.push_back(g);
+      g.out = energySlot_ + 1;
+      g.sel = 0;
+      fin.push_back(g);
+      GmDesc c = gm_desc(v, L, v.off, 1, 0, v.off, GM_COPY, v.addr);
+      g1.push_back(c);
+      continue;
+    }
+    const int64_t so = pool64Off, xo = so + (int64_t)n * n, wo = xo + (int64_t)n * n;
+    pool64Off = wo + (int64_t)n * n;
+    g.out = so;
+    g.sel = 0;  // S = A^H A
+    fin.push_back(g);
+    g.out = xo;
+    g.sel = 2;  // X = A^H Y
+    fin.push_back(g);
+    SolveItem si{};
+    si.s = so;
+    si.x = xo;
+    si.r = rhoinv_[v.addr];
+    si.w = wo;
+    si.n = n;
+    si.node = v.addr;
+    h_solve_.push_back(si);
+    g1.push_back(gm_desc(v, L, v.off, 1, rhoinv_[v.addr], v.off, GM_N, v.addr));
+    GmDesc m2 = gm_desc(v, L, v.off, 0, wo, v.off, GM_N, v.addr);
+    m2.are = -1.0;
+    m2.acc = 1;
+    g2.push_back(m2);
+  }
+  push_gc(gApply_, fin, 2, B_PSI64, B_APPLY64, B_POOL64);
+  {
+    GLaunch gl{};
+    gl.kind = 2;
+    gl.prec = 2;
+    gl.nd = (int64_t)h_solve_.size();
+    gl.x0 = gl.x1 = gl.m = gl.o = B_POOL64;
+    double fl = 0.0;
+    for (auto& si : h_solve_) fl += 8.0 * 2.0 * si.n * si.n * si.n;
+    gl.flops = fl;
+    flops += fl;
+    gApply_.push_back(gl);
+  }
+  push_gm(gApply_, g1, 2, B_PSI64, B_APPLY64, B_POOL64, B_DPSI64, true);
+  push_gm(gApply_, g2, 2, B_PSI64, B_APPLY64, B_POOL64, B_DPSI64, true);
+  phaseFlops_[3] = flops - ff0;
+
+  // combine flops (aggregated slots of the shared plan, stacked blocks)
+  double comb = 0.0;
+  for (auto& co : h_comb3_) comb += 8.0 * co.n * co.n;
+  phaseFlops_[5] = comb;
+  flops += comb;
+  nComb3_ = (int64_t)h_comb3_.size();
+  poolSize_ = poolOff;
+  pool64Size_ = pool64Off;
+  flops_ = flops;
+  return QTG_OK;
+}
+
+template <typename T>
+int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre,
+                         double sim, hipStream_t s) {
+  int rc;
+  for (const auto& gl : L) {
+    if (gl.kind == 2 && heevPending_) {
+      // rho^-1 from the side stream of the last build
+      QTG_HIP_TRY(hipStreamWaitEvent(s, evHeev_, 0));
+      heevPending_ = false;
+    }
+    const double a = gl.scaled ? sre : 1.0, b = gl.scaled ? sim : 0.0;
+    if (gl.kind == 0) {
+      if ((rc = begin_mark(6, s))) return rc;
+      const GmDesc* d = d_gm_ + gl.d0;
+      if (gl.prec == 0)
+        rc = launch_gm<T, T>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o], d_part_,
+                             active, a, b, s);
+      else if (gl.prec == 1)
+        rc = launch_gm<T, double>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
+                                  d_part_, active, a, b, s);
+      else
+        rc = launch_gm<double, double>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
+                                       d_part_, active, a, b, s);
+      if (rc) return rc;
+      if (gl.ns > 0) {
+        // split descriptors skip inactive nodes in the tile pass; their
+        // reduce must too, so a masked launch reduces only active ones
+        const GmDesc* sd = d_gmsplit_ + gl.s0;
+        if (gl.prec == 0) rc = launch_gm_reduce<T>(sd, gl.ns, bases[gl.o], d_part_, active, s);
+        else rc = launch_gm_reduce<double>(sd, gl.ns, bases[gl.o], d_part_, active, s);
+        if (rc) return rc;
+      }
+      if (timing_) tfl_[6] += gl.flops;
+    } else if (gl.kind == 1) {
+      if ((rc = begin_mark(7, s))) return rc;
+      const GcDesc* d = d_gc_ + gl.d0;
+      if (gl.prec == 2)
+        rc = launch_gc<double>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], d_part_, active, s);
+      else
+        rc = launch_gc<T>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], d_part_, active, s);
+      if (rc) return rc;
+      if (gl.ns > 0) {
+        const GcDesc* sd = d_gcsplit_ + gl.s0;
+        if (gl.prec == 2) rc = launch_gc_reduce<double>(sd, gl.ns, bases[gl.o], d_part_, active, s);
+        else rc = launch_gc_reduce<T>(sd, gl.ns, bases[gl.o], d_part_, active, s);
+        if (rc) return rc;
+      }
+      if (timing_) tfl_[7] += gl.flops;
+    } else {
+      if ((rc = begin_mark(8, s))) return rc;
+      if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s))) return rc;
+      if (timing_) tfl_[8] += gl.flops;
+    }
+    if ((rc = end_mark(s))) return rc;
+  }
+  return QTG_OK;
+}
+
+template <typename T>
+int Engine::build_gemm(hipStream_t s) {
+  int rc;
+  void* psiT = c64_ ? (void*)d_psi32_ : (void*)d_psi64_;
+  if (c64_ && (rc = convert_psi(s))) return rc;
+  void* bases[B_COUNT] = {psiT, d_work_, d_pool_, d_psi64_, d_apply64_, d_dpsi64_, d_pool64_};
+  if ((rc = run_launches<T>(gBuild_, bases, nullptr, 1.0, 0.0, s))) return rc;
+  // The density eigensolver only feeds the solve and rho^-1 stages of the
+  // finish: it runs on a side stream, overlapping Apply and the finish's
+  // overlap contractions; apply_gemm joins before the solve.
+  QTG_HIP_TRY(hipEventRecord(evFork_, s));
+  QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
+  if ((rc = begin_mark(1, side_))) return rc;
+  if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
+  if ((rc = end_mark(side_))) return rc;
+  QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
+  heevPending_ = true;
+  if ((rc = begin_mark(5, s))) return rc;
+  if (nComb_ > 0) {
+    if ((rc = launch_combine<T>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
+    if ((rc = launch_combine<T>(d_combOut2_, nComb2_, d_combTerm2_, d_pool_, s))) return rc;
+  }
+  if ((rc = launch_combine<T>(d_combOut3_, nComb3_, d_combTerm3_, d_pool_, s))) return rc;
+  if ((rc = end_mark(s))) return rc;
+  if (timing_) tfl_[5] += phaseFlops_[5];
+  built_ = true;
+  return QTG_OK;
+}
+
+template <typename T>
+int Engine::apply_gemm(hipStream_t s, const void* psiT, const double* psi64, double* dpsi64, const uint8_t* active,
+                       double ph_re, double ph_im) {
+  void* bases[B_COUNT] = {(void*)psiT, d_work_, d_pool_, (void*)psi64, d_apply64_, dpsi64, d_pool64_};
+  // f = -i * phase
+  return run_launches<T>(gApply_, bases, active, ph_im, -ph_re, s);
+}
+
+int Engine::apply_finish(const double* psi64, double* dpsi64, const uint8_t* active, double ph_re, double ph_im,
+                         double* energy_dev, hipStream_t s) {
+  if (c64_) {
+    set_error("engine: apply_finish needs a complex128 engine");
+    return QTG_ESTATE;
+  }
+  if (!built_) {
+    set_error("engine: build() first");
+    return QTG_ESTATE;
+  }
+  int rc;
+  if (path_ == 1) {
+    if ((rc = apply_gemm<double>(s, psi64, psi64, dpsi64, active, ph_re, ph_im))) return rc;
+    if (energy_dev)
+      QTG_HIP_TRY(hipMemcpyAsync(energy_dev, d_pool64_ + 2 * energySlot_, 4 * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s));
+    return QTG_OK;
+  }
+  if ((rc = launch_apply<double>(d_chunks_, nChunks_, d_terms_, d_nodes_, psi64, d_pool_, d_partial_, nmax_, s,
+                                 active)))
+    return rc;
+  return launch_finish<double>(d_fin_, nFin_, d_nodes_, psi64, d_partial_, d_pool64_, dpsi64, nullptr, ph_re, ph_im,
+                               energy_dev, nmax_, nspf_, s, active);
+}
+
+template int Engine::build_gemm<double>(hipStream_t);
+template int Engine::build_gemm<float>(hipStream_t);
+template int Engine::apply_gemm<double>(hipStream_t, const void*, const double*, double*, const uint8_t*, double,
+                                        double);
+template int Engine::apply_gemm<float>(hipStream_t, const void*, const double*, double*, const uint8_t*, double,
+                                       double);
+
+}  // namespace qtg

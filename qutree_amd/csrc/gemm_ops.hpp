@@ -1,0 +1,88 @@
+// Grouped complex contractions of node tensors on MFMA (see gemm_ops.hip).
+//
+// Two descriptor kinds cover every dense contraction of the tree sweeps:
+//   GmDesc  mode-k product   out(b,i,a) = alpha sum_l op(M)(i,l) X(b,l,a) (+ out)
+//   GcDesc  hole contraction out(j,l)   = sum_{b,a} conj(X(b,j,a)) Y(b,l,a) (+ out)
+// Each descriptor is split into 16 x 16 output tiles; one wavefront computes
+// one tile with real-split complex MFMA (four real 16x16x4 products per
+// complex k-step).  A launch walks a flat task list: descriptors are sorted by
+// their first task index tile0, and a wavefront finds its descriptor by binary
+// search.
+#pragma once
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace qtg {
+
+enum GmOp : int32_t { GM_N = 0, GM_T = 1, GM_H = 2, GM_COPY = 3 };
+
+struct GmDesc {
+  int64_t x, m, out;                // complex-element offsets (input, matrix, output)
+  int32_t bef, nin, nout, aft;      // view of X on the contracted mode
+  int32_t op;                       // GmOp; GM_COPY: out = alpha X (nin == nout, m unused)
+  int32_t acc;                      // 1: accumulate into out
+  double are, aim;                  // alpha
+  int32_t node;                     // index into the active mask (-1: always run)
+  int32_t xsel;                     // X from the first (0) or second (1) input base
+  int32_t ksplit;                   // K chunks; > 1 writes partial tiles (alpha applied)
+  int32_t pad;
+  int64_t part;                     // partial-tile offset (R x nout per chunk)
+  int64_t tile0;                    // first task of this descriptor
+};
+
+struct GcDesc {
+  int64_t x, y, out;                // complex-element offsets (bra, ket, nx x ny output)
+  int32_t bef, nx, ny, aft;         // X is (bef, nx, aft), Y is (bef, ny, aft)
+  int32_t acc;                      // 1: accumulate into out
+  int32_t node;                     // active-mask index (-1: always run)
+  int32_t ksplit;                   // K chunks; > 1 writes partial tiles to the workspace
+  int32_t sel;                      // bit 0: X from the second input base; bit 1: Y likewise
+  int64_t part;                     // workspace offset of the ksplit partial outputs
+  int64_t tile0;
+};
+
+// number of tasks of one descriptor (16 x 16 tiles, times K chunks)
+inline int64_t gm_tasks(const GmDesc& d) {
+  const int64_t R = (int64_t)d.bef * d.aft;
+  return ((R + 15) / 16) * ((d.nout + 15) / 16) * (d.ksplit > 1 ? d.ksplit : 1);
+}
+inline int64_t gc_tasks(const GcDesc& d) {
+  return (int64_t)((d.nx + 15) / 16) * ((d.ny + 15) / 16) * (d.ksplit > 1 ? d.ksplit : 1);
+}
+
+// Launchers.  Input bases x0 / x1 (selected per descriptor), matrix base and
+// output base are cplx<T>* (output cplx<TO>*); `active` may be null.  Every
+// alpha of a mode-product launch is multiplied by the complex scale (sre, sim).
+template <typename T, typename TO>
+int launch_gm(const GmDesc* descs, int64_t ndesc, int64_t ntasks, const void* x0, const void* x1,
+              const void* mbase, void* obase, void* part, const uint8_t* active, double sre, double sim,
+              hipStream_t s);
+// Sums the ksplit partial tiles (cplx<TO>) of every split mode-product descriptor into its output.
+template <typename TO>
+int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, void* obase, const void* part, const uint8_t* active,
+                     hipStream_t s);
+template <typename T>
+int launch_gc(const GcDesc* descs, int64_t ndesc, int64_t ntasks, const void* x0, const void* x1, void* obase,
+              void* work, const uint8_t* active, hipStream_t s);
+// One descriptor (tile0 ignored), no active mask; no K split.
+template <typename T>
+int launch_gm_single(const GmDesc& d, const void* x, const void* m, void* o, hipStream_t s);
+template <typename T>
+int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hipStream_t s);
+// Sums the ksplit partial tiles of every split descriptor into its output.
+template <typename T>
+int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, void* obase, const void* work, const uint8_t* active,
+                     hipStream_t s);
+
+// Batched FP64 solve X <- S^-1 X (Gauss-Jordan with partial pivoting) for n x n
+// column-major S and X at pool offsets s, x; n <= 64.  When r >= 0 it also
+// forms W = R X^T at w (R the n x n matrix at r).
+struct SolveItem {
+  int64_t s, x, r, w;
+  int32_t n, node;
+};
+int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
+                 hipStream_t s);
+
+}  // namespace qtg

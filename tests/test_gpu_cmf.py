@@ -42,11 +42,12 @@ def test_cmf8_matches_oracle(start):
     assert np.allclose(got["times"], ref["times"], rtol=0, atol=1e-12)
     assert got["steps"] == ref["steps"] and got["rejected"] == ref["rejected"]
     assert got["derivs"] == ref["derivs"]
-    assert de < 1e-10
     # A Hartree start leaves the unoccupied SPFs with rho eigenvalues ~0, so
     # rho^-1 = 1/(lambda + 1e-10 e^{-lambda/1e-10}) multiplies their rounding
-    # noise by ~1e10 in both implementations; the autocorrelation then agrees
-    # to ~1e-10 instead of ~1e-14 (random, full-rank starts).
+    # noise by ~1e10 in both implementations; energies and autocorrelation
+    # then agree to ~1e-10 instead of ~1e-14 (random, full-rank starts), and
+    # how close depends on the summation order of each contraction.
+    assert de < (1e-9 if start == "hartree" else 1e-12)
     assert da < (1e-9 if start == "hartree" else 1e-12)
 
 

@@ -22,6 +22,9 @@ def _models():
         "f16": (W.balanced_tree_text(16, 16, [8, 10, 11, 13]), W.coupled_ho(16, 16, c_sigma=0.01,
                                                                              one_mode_matrix=True)),
         "sweep64": W.sweep64_model(),
+        # upper nodes of 24 x 24 x 24 (221 KB in complex128): beyond the
+        # LDS-resident kernels, grouped path only
+        "f8big": (W.balanced_tree_text(8, 8, [8, 24, 24]), W.coupled_quartic(8, 5, c_sigma=0.05)),
     }
 
 
@@ -54,17 +57,20 @@ def derivative_err(eng, om, got, ref, app_ref):
     return worst
 
 
+@pytest.mark.parametrize("path", [1, 0])
 @pytest.mark.parametrize("compress", [True, False])
-@pytest.mark.parametrize("name", ["f4", "f8quartic", "cmf8", "f16", "sweep64"])
-def test_derivative_matches_oracle(name, compress):
+@pytest.mark.parametrize("name", ["f4", "f8quartic", "cmf8", "f16", "sweep64", "f8big"])
+def test_derivative_matches_oracle(name, compress, path):
+    if name == "f8big" and path == 0:
+        pytest.skip("node tensors beyond the LDS-resident kernels")
     text, H = _models()[name]
-    print(f"case engine derivative {name} compress={compress}", flush=True)
+    print(f"case engine derivative {name} compress={compress} path={path}", flush=True)
     om = O.OracleModel(text, H)
     psi = om.fill_random(3, delta_lowest=False)
     ref = om.derivative(psi)
     e_ref = om.energy(psi)
     app_ref = om.apply_all(psi)
-    eng = Q.Engine(text, H, compress=compress)
+    eng = Q.Engine(text, H, compress=compress, path=path)
     eng.set_psi(psi)
     eng.derivative()
     got = eng.get(0)
@@ -75,15 +81,18 @@ def test_derivative_matches_oracle(name, compress):
     assert abs(eng.energy() - e_ref) < 1e-12 * abs(e_ref)
 
 
+@pytest.mark.parametrize("path", [1, 0])
 @pytest.mark.parametrize("compress", [True, False])
-@pytest.mark.parametrize("name", ["f4", "cmf8", "f16"])
-def test_representation_matrices_match_oracle(name, compress):
+@pytest.mark.parametrize("name", ["f4", "cmf8", "f16", "f8big"])
+def test_representation_matrices_match_oracle(name, compress, path):
+    if name == "f8big" and path == 0:
+        pytest.skip("node tensors beyond the LDS-resident kernels")
     text, H = _models()[name]
-    print(f"case engine matrices {name} compress={compress}", flush=True)
+    print(f"case engine matrices {name} compress={compress} path={path}", flush=True)
     om = O.OracleModel(text, H)
     psi = om.fill_random(4, delta_lowest=False)
     om.build(psi)
-    eng = Q.Engine(text, H, compress=compress)
+    eng = Q.Engine(text, H, compress=compress, path=path)
     eng.set_psi(psi)
     eng.build()
     Q.synchronize()
