@@ -34,6 +34,7 @@ Engine::~Engine() {
   for (auto e : evpool_) (void)hipEventDestroy(e);
   if (evFork_) (void)hipEventDestroy(evFork_);
   if (evHeev_) (void)hipEventDestroy(evHeev_);
+  if (evRho_) (void)hipEventDestroy(evRho_);
   if (side_) (void)hipStreamDestroy(side_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -92,6 +93,7 @@ int Engine::materialize() {
   if (!side_) QTG_HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   if (!evFork_) QTG_HIP_TRY(hipEventCreateWithFlags(&evFork_, hipEventDisableTiming));
   if (!evHeev_) QTG_HIP_TRY(hipEventCreateWithFlags(&evHeev_, hipEventDisableTiming));
+  if (!evRho_) QTG_HIP_TRY(hipEventCreateWithFlags(&evRho_, hipEventDisableTiming));
   std::vector<DNode> dn;
   for (auto& n : tree_.nodes) {
     DNode d{};
@@ -351,8 +353,13 @@ int Engine::plan() {
   nSw_ = (int64_t)items.size();
   phaseFlops_[0] = flops_;
   // ---- density eigen / inverse
+  // density eigenproblems, largest first (one workgroup each)
   std::vector<EigItem> eig;
-  for (auto& v : tree_.nodes) {
+  std::vector<const HNode*> bySize;
+  for (auto& v : tree_.nodes) bySize.push_back(&v);
+  std::stable_sort(bySize.begin(), bySize.end(), [](const HNode* a, const HNode* b) { return a->n > b->n; });
+  for (const HNode* vp : bySize) {
+    const HNode& v = *vp;
     if (v.top()) continue;
     if (v.n > 64) {
       set_error("engine: more than 64 SPFs at a node is not supported");
@@ -603,11 +610,16 @@ int Engine::plan() {
     device_buffers_.push_back(d_combOut3_);
     if ((rc = up(&d_combTerm3_, h_combt3_))) return rc;
     device_buffers_.push_back(d_combTerm3_);
+    if ((rc = up(&d_gmwg_, h_gmwg_))) return rc;
+    device_buffers_.push_back(d_gmwg_);
+    if ((rc = up(&d_gcwg_, h_gcwg_))) return rc;
+    device_buffers_.push_back(d_gcwg_);
     if ((rc = up(&d_gmsplit_, h_gmsplit_))) return rc;
     device_buffers_.push_back(d_gmsplit_);
     if ((rc = up(&d_gcsplit_, h_gcsplit_))) return rc;
     device_buffers_.push_back(d_gcsplit_);
     if ((rc = alloc(&d_part_, 16 * std::max<int64_t>(partSize_, 1)))) return rc;
+    if ((rc = alloc(&d_partRho_, 16 * std::max<int64_t>(partSize_, 1)))) return rc;
   }
   const size_t es = c64_ ? 8 : 16;
   if (path_ == 1 && (rc = alloc(&d_work_, es * std::max<int64_t>(workSize_, 1)))) return rc;

@@ -93,13 +93,15 @@ class Engine {
     int prec;    // 0: engine precision, 1: engine precision in, FP64 out, 2: FP64
     int64_t d0, nd, ntasks;
     int64_t s0, ns;    // split descriptors of this launch (reduce pass)
+    int64_t nrb;       // reduce blocks
+    int64_t wg0;       // first entry of the workgroup -> descriptor table
     int x0, x1, m, o;  // buffer ids
     bool scaled;       // multiply alphas by the launch scale (-i phase)
     double flops;
   };
   template <typename T>
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
-                   hipStream_t s);
+                   hipStream_t s, void* partbuf);
   int materialize();
   std::vector<std::pair<int, int>> restrict_in(int l, int a) const;
   std::vector<std::pair<int, int>> restrict_out(int l, int a) const;
@@ -130,11 +132,15 @@ class Engine {
   std::vector<GmDesc> h_gm_, h_gmsplit_;
   std::vector<GcDesc> h_gc_, h_gcsplit_;
   int64_t partSize_ = 0;  // complex elements of the split-K partial tiles
+  std::vector<int32_t> h_gmwg_, h_gcwg_;
+  int32_t* d_gmwg_ = nullptr;
+  int32_t* d_gcwg_ = nullptr;
   GmDesc* d_gmsplit_ = nullptr;
   GcDesc* d_gcsplit_ = nullptr;
   void* d_part_ = nullptr;
+  void* d_partRho_ = nullptr;  // split-K partials of the side stream
   std::vector<SolveItem> h_solve_;
-  std::vector<GLaunch> gBuild_, gApply_;
+  std::vector<GLaunch> gBuild_, gHole_, gRho_, gApply_;
   std::vector<std::pair<int64_t, int>> idSlots_;  // identity matrices in the pool (offset, n)
   std::vector<CombOut> h_comb3_;
   std::vector<CombTerm> h_combt3_;
@@ -172,8 +178,9 @@ class Engine {
   std::vector<void*> device_buffers_;
   hipStream_t stream_ = nullptr;
   hipStream_t side_ = nullptr;  // density eigensolver, overlapping Apply
-  hipEvent_t evFork_ = nullptr, evHeev_ = nullptr;
+  hipEvent_t evFork_ = nullptr, evHeev_ = nullptr, evRho_ = nullptr;
   bool heevPending_ = false;
+
   hipGraph_t graph_h_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   double graph_ph_re_ = 0.0, graph_ph_im_ = 0.0;

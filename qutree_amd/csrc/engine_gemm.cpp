@@ -75,9 +75,13 @@ int Engine::plan_gemm() {
   h_gc_.clear();
   h_gmsplit_.clear();
   h_gcsplit_.clear();
+  h_gmwg_.clear();
+  h_gcwg_.clear();
   partSize_ = 0;
   h_solve_.clear();
   gBuild_.clear();
+  gHole_.clear();
+  gRho_.clear();
   gApply_.clear();
   idSlots_.clear();
   h_comb3_.clear();
@@ -97,7 +101,7 @@ int Engine::plan_gemm() {
     gl.d0 = (int64_t)h_gm_.size();
     gl.nd = (int64_t)descs.size();
     gl.s0 = (int64_t)h_gmsplit_.size();
-    int64_t t = 0, part = 0;
+    int64_t t = 0, part = 0, rblocks = 0;
     double fl = 0.0;
     for (auto& d : descs) {
       // long contractions (stacked Apply summands) are split along K into
@@ -106,7 +110,10 @@ int Engine::plan_gemm() {
         d.ksplit = (d.nin + kGmSplitK - 1) / kGmSplitK;
         d.part = part;
         part += (int64_t)d.ksplit * d.bef * d.aft * d.nout;
-        h_gmsplit_.push_back(d);
+        GmDesc r = d;
+        r.tile0 = rblocks;  // reduce blocks of 256 output elements
+        rblocks += ((int64_t)d.bef * d.aft * d.nout + 255) / 256;
+        h_gmsplit_.push_back(r);
       }
       d.tile0 = t;
       t += gm_tasks(d);
@@ -114,7 +121,10 @@ int Engine::plan_gemm() {
       h_gm_.push_back(d);
     }
     gl.ns = (int64_t)h_gmsplit_.size() - gl.s0;
+    gl.nrb = rblocks;
     partSize_ = std::max(partSize_, part);
+    gl.wg0 = (int64_t)h_gmwg_.size();
+    gemm_wg_table(h_gm_.data() + gl.d0, gl.nd, t, h_gmwg_);
     gl.ntasks = t;
     gl.x0 = x0;
     gl.x1 = x1;
@@ -133,7 +143,7 @@ int Engine::plan_gemm() {
     gl.d0 = (int64_t)h_gc_.size();
     gl.nd = (int64_t)descs.size();
     gl.s0 = (int64_t)h_gcsplit_.size();
-    int64_t t = 0, part = 0;
+    int64_t t = 0, part = 0, rblocks = 0;
     double fl = 0.0;
     for (auto& d : descs) {
       const int64_t K = (int64_t)d.bef * d.aft;
@@ -141,7 +151,10 @@ int Engine::plan_gemm() {
         d.ksplit = (int32_t)((K + kGcSplitK - 1) / kGcSplitK);
         d.part = part;
         part += (int64_t)d.ksplit * d.nx * d.ny;
-        h_gcsplit_.push_back(d);
+        GcDesc r = d;
+        r.tile0 = rblocks;
+        rblocks += ((int64_t)d.nx * d.ny + 255) / 256;
+        h_gcsplit_.push_back(r);
       }
       d.tile0 = t;
       t += gc_tasks(d);
@@ -149,7 +162,10 @@ int Engine::plan_gemm() {
       h_gc_.push_back(d);
     }
     gl.ns = (int64_t)h_gcsplit_.size() - gl.s0;
+    gl.nrb = rblocks;
     partSize_ = std::max(partSize_, part);
+    gl.wg0 = (int64_t)h_gcwg_.size();
+    gemm_wg_table(h_gc_.data() + gl.d0, gl.nd, t, h_gcwg_);
     gl.ntasks = t;
     gl.x0 = x0;
     gl.x1 = x1;
@@ -162,15 +178,23 @@ int Engine::plan_gemm() {
   };
 
   // ---------------------------------------------------------------- build
-  auto sandwich_layer = [&](const std::pair<int64_t, int64_t>& r) {
+  // items of a layer; rho: 1 keeps only density items, 0 drops them, -1 keeps all
+  auto is_rho = [&](const SwItem& it) {
+    const HNode& p = tree_.nodes[it.node];
+    if (it.k >= (int)p.children.size() || p.bottom) return false;
+    return it.out == rhoslot_[p.children[it.k]];
+  };
+  int64_t wbase = 0, wmax = 0;
+  auto sandwich_layer = [&](const std::pair<int64_t, int64_t>& r, int rho, std::vector<GLaunch>& L) {
     if (r.second == r.first) return;
     std::map<std::tuple<int, int, int64_t>, int64_t> zbra;             // (node, leg, mat) -> work offset
     std::map<std::vector<int64_t>, int64_t> zket1, zket2;              // (node, leg, mat[, leg, mat])
     std::vector<GmDesc> lv0, lv1;
     std::vector<GcDesc> con;
-    int64_t w = 0;
+    int64_t w = wbase;
     for (int64_t i = r.first; i < r.second; ++i) {
       const SwItem& it = h_items_[i];
+      if (rho >= 0 && (int)is_rho(it) != rho) continue;
       const HNode& p = tree_.nodes[it.node];
       const int k = it.k;
       const int braleg = (k == 0) ? 1 : 0;
@@ -235,15 +259,19 @@ int Engine::plan_gemm() {
       g.sel = xsel | (ysel << 1);
       con.push_back(g);
     }
-    workSize_ = std::max(workSize_, w);
-    const double f0 = flops;
-    push_gm(gBuild_, lv0, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
-    push_gm(gBuild_, lv1, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
-    push_gc(gBuild_, con, 0, B_PSI, B_WORK, B_POOL);
-    (void)f0;
+    wmax = std::max(wmax, w);
+    push_gm(L, lv0, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+    push_gm(L, lv1, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+    push_gc(L, con, 0, B_PSI, B_WORK, B_POOL);
   };
-  for (auto& r : repRange_) sandwich_layer(r);
-  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd]);
+  // represent (bottom-up) and the holes of the terms (top-down) on the engine
+  // stream; the density tree, which needs neither, runs top-down on a side
+  // stream with its own workspace region and feeds the eigensolver early
+  for (auto& r : repRange_) sandwich_layer(r, -1, gBuild_);
+  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 0, gHole_);
+  wbase = wmax;
+  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 1, gRho_);
+  workSize_ = std::max(workSize_, wmax);
 
   // ---------------------------------------------------------------- apply
   std::map<int, int64_t> idmap;
@@ -434,7 +462,7 @@ int Engine::plan_gemm() {
 
 template <typename T>
 int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre,
-                         double sim, hipStream_t s) {
+                         double sim, hipStream_t s, void* partbuf) {
   int rc;
   for (const auto& gl : L) {
     if (gl.kind == 2 && heevPending_) {
@@ -446,37 +474,39 @@ int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, cons
     if (gl.kind == 0) {
       if ((rc = begin_mark(6, s))) return rc;
       const GmDesc* d = d_gm_ + gl.d0;
+      const int32_t* wg = d_gmwg_ + gl.wg0;
       if (gl.prec == 0)
-        rc = launch_gm<T, T>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o], d_part_,
+        rc = launch_gm<T, T>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o], d_part_,
                              active, a, b, s);
       else if (gl.prec == 1)
-        rc = launch_gm<T, double>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
-                                  d_part_, active, a, b, s);
+        rc = launch_gm<T, double>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
+                                  partbuf, active, a, b, s);
       else
-        rc = launch_gm<double, double>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
-                                       d_part_, active, a, b, s);
+        rc = launch_gm<double, double>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
+                                       partbuf, active, a, b, s);
       if (rc) return rc;
       if (gl.ns > 0) {
         // split descriptors skip inactive nodes in the tile pass; their
         // reduce must too, so a masked launch reduces only active ones
         const GmDesc* sd = d_gmsplit_ + gl.s0;
-        if (gl.prec == 0) rc = launch_gm_reduce<T>(sd, gl.ns, bases[gl.o], d_part_, active, s);
-        else rc = launch_gm_reduce<double>(sd, gl.ns, bases[gl.o], d_part_, active, s);
+        if (gl.prec == 0) rc = launch_gm_reduce<T>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
+        else rc = launch_gm_reduce<double>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
         if (rc) return rc;
       }
       if (timing_) tfl_[6] += gl.flops;
     } else if (gl.kind == 1) {
       if ((rc = begin_mark(7, s))) return rc;
       const GcDesc* d = d_gc_ + gl.d0;
+      const int32_t* wg = d_gcwg_ + gl.wg0;
       if (gl.prec == 2)
-        rc = launch_gc<double>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], d_part_, active, s);
+        rc = launch_gc<double>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], partbuf, active, s);
       else
-        rc = launch_gc<T>(d, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], d_part_, active, s);
+        rc = launch_gc<T>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], partbuf, active, s);
       if (rc) return rc;
       if (gl.ns > 0) {
         const GcDesc* sd = d_gcsplit_ + gl.s0;
-        if (gl.prec == 2) rc = launch_gc_reduce<double>(sd, gl.ns, bases[gl.o], d_part_, active, s);
-        else rc = launch_gc_reduce<T>(sd, gl.ns, bases[gl.o], d_part_, active, s);
+        if (gl.prec == 2) rc = launch_gc_reduce<double>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
+        else rc = launch_gc_reduce<T>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
         if (rc) return rc;
       }
       if (timing_) tfl_[7] += gl.flops;
@@ -496,17 +526,22 @@ int Engine::build_gemm(hipStream_t s) {
   void* psiT = c64_ ? (void*)d_psi32_ : (void*)d_psi64_;
   if (c64_ && (rc = convert_psi(s))) return rc;
   void* bases[B_COUNT] = {psiT, d_work_, d_pool_, d_psi64_, d_apply64_, d_dpsi64_, d_pool64_};
-  if ((rc = run_launches<T>(gBuild_, bases, nullptr, 1.0, 0.0, s))) return rc;
-  // The density eigensolver only feeds the solve and rho^-1 stages of the
-  // finish: it runs on a side stream, overlapping Apply and the finish's
-  // overlap contractions; apply_gemm joins before the solve.
+  // Side stream: density tree, then the eigensolver (which only feeds the
+  // solve and rho^-1 stages of the finish).  Engine stream: represent, then
+  // (after the densities, which some hole items use as parent factor) the
+  // holes, the combinations and Apply; apply_gemm joins before the solve.
   QTG_HIP_TRY(hipEventRecord(evFork_, s));
   QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
+  if ((rc = run_launches<T>(gRho_, bases, nullptr, 1.0, 0.0, side_, d_partRho_))) return rc;
+  QTG_HIP_TRY(hipEventRecord(evRho_, side_));
   if ((rc = begin_mark(1, side_))) return rc;
   if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
   if ((rc = end_mark(side_))) return rc;
   QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
   heevPending_ = true;
+  if ((rc = run_launches<T>(gBuild_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  QTG_HIP_TRY(hipStreamWaitEvent(s, evRho_, 0));
+  if ((rc = run_launches<T>(gHole_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
   if ((rc = begin_mark(5, s))) return rc;
   if (nComb_ > 0) {
     if ((rc = launch_combine<T>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
@@ -524,7 +559,7 @@ int Engine::apply_gemm(hipStream_t s, const void* psiT, const double* psi64, dou
                        double ph_re, double ph_im) {
   void* bases[B_COUNT] = {(void*)psiT, d_work_, d_pool_, (void*)psi64, d_apply64_, dpsi64, d_pool64_};
   // f = -i * phase
-  return run_launches<T>(gApply_, bases, active, ph_im, -ph_re, s);
+  return run_launches<T>(gApply_, bases, active, ph_im, -ph_re, s, d_part_);
 }
 
 int Engine::apply_finish(const double* psi64, double* dpsi64, const uint8_t* active, double ph_re, double ph_im,

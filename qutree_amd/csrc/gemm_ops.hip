@@ -62,7 +62,18 @@ __device__ inline int find_desc(const D* __restrict__ descs, int ndesc, int64_t 
   return lo;
 }
 
-constexpr int kWaves = 4;  // wavefronts (tasks) per workgroup
+constexpr int kWaves = kGemmWaves;  // wavefronts (tasks) per workgroup
+
+// Descriptor of a task: wgdesc[blockIdx.x] is the descriptor of the
+// workgroup's first task (precomputed on the host), and a task lies at most
+// kWaves - 1 descriptors further.
+template <typename D>
+__device__ inline int task_desc(const D* __restrict__ descs, int ndesc, const int32_t* __restrict__ wgdesc,
+                                int64_t task) {
+  int di = wgdesc[blockIdx.x];
+  while (di + 1 < ndesc && descs[di + 1].tile0 <= task) ++di;
+  return di;
+}
 
 // ------------------------------------------------------------- mode product
 // Output tile: 16 rows of the flattened (b, a) index x 16 output columns i.
@@ -157,15 +168,18 @@ __device__ inline void gm_task(const GmDesc& d, int64_t t, const cplx<T>* __rest
   }
 }
 
-// one workgroup per split descriptor: sum the partial tiles in chunk order
+// Sums the partial tiles of split descriptors in chunk order.  The split
+// descriptors carry, in tile0, the first 256-element block of their output:
+// each workgroup owns one block of one descriptor.
 template <typename TO>
 __global__ void __launch_bounds__(256)
-    gm_reduce_kernel(const GmDesc* __restrict__ descs, cplx<TO>* O, const cplx<TO>* __restrict__ part,
+    gm_reduce_kernel(const GmDesc* __restrict__ descs, int ndesc, cplx<TO>* O, const cplx<TO>* __restrict__ part,
                      const uint8_t* __restrict__ active) {
-  const GmDesc d = descs[blockIdx.x];
+  const GmDesc d = descs[find_desc(descs, ndesc, blockIdx.x)];
   if (active && d.node >= 0 && !active[d.node]) return;
   const int64_t bef = d.bef, R = bef * d.aft, n = R * d.nout;
-  for (int64_t e = threadIdx.x; e < n; e += 256) {
+  const int64_t e = (blockIdx.x - d.tile0) * 256 + threadIdx.x;
+  if (e < n) {
     cplx<TO> s = part[d.part + e];
     for (int c = 1; c < d.ksplit; ++c) s = cadd(s, part[d.part + c * n + e]);
     const int64_t r = e % R, jj = e / R;
@@ -177,12 +191,12 @@ __global__ void __launch_bounds__(256)
 
 template <typename T, typename TO>
 __global__ void __launch_bounds__(64 * kWaves)
-    gm_grouped(const GmDesc* __restrict__ descs, int ndesc, int64_t ntasks, const cplx<T>* __restrict__ X0,
-               const cplx<T>* __restrict__ X1, const cplx<T>* __restrict__ M, cplx<TO>* O, cplx<TO>* part,
-               const uint8_t* __restrict__ active, double sre, double sim) {
+    gm_grouped(const GmDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
+               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, const cplx<T>* __restrict__ M,
+               cplx<TO>* O, cplx<TO>* part, const uint8_t* __restrict__ active, double sre, double sim) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;  // uniform per wavefront
-  const GmDesc d = descs[find_desc(descs, ndesc, task)];
+  const GmDesc d = descs[task_desc(descs, ndesc, wgdesc, task)];
   if (active && d.node >= 0 && !active[d.node]) return;
   gm_task<T, TO>(d, task - d.tile0, d.xsel ? X1 : X0, M, O, part, sre, sim);
 }
@@ -222,15 +236,14 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
   int64_t b = kc % bef, a = kc / bef;
   v4 cr0 = {0, 0, 0, 0}, ci0 = {0, 0, 0, 0}, cr1 = {0, 0, 0, 0}, ci1 = {0, 0, 0, 0};
   const cplx<T> z = czero<T>();
-  // four k-steps per iteration: all loads are issued before the MFMAs
-  for (int64_t k0 = kbeg; k0 < kend; k0 += 16) {
-    cplx<T> xv4[4], yv4[4];
+  // four k-steps per iteration; the loads of the next iteration are issued
+  // before the MFMAs of the current one (double-buffered registers)
+  auto load4 = [&](cplx<T>* xr, cplx<T>* yr) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const bool kv = kc < kend;
-      xv4[u] = (xv && kv) ? xp[b + a * xs] : z;
-      yv4[u] = (yv && kv) ? yp[b + a * ys] : z;
-      xv4[u].im = -xv4[u].im;
+      xr[u] = (xv && kv) ? xp[b + a * xs] : z;
+      yr[u] = (yv && kv) ? yp[b + a * ys] : z;
       kc += 4;
       b += 4;
       if (b >= bef) {
@@ -239,10 +252,26 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
         b -= q * bef;
       }
     }
-    cmma<T>(cr0, ci0, xv4[0], yv4[0]);
-    if (k0 + 4 < kend) cmma<T>(cr1, ci1, xv4[1], yv4[1]);
-    if (k0 + 8 < kend) cmma<T>(cr0, ci0, xv4[2], yv4[2]);
-    if (k0 + 12 < kend) cmma<T>(cr1, ci1, xv4[3], yv4[3]);
+  };
+  cplx<T> xc[4], yc[4];
+  if (kbeg < kend) load4(xc, yc);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += 16) {
+    cplx<T> xn[4], yn[4];
+    const bool more = k0 + 16 < kend;
+    if (more) load4(xn, yn);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xc[u].im = -xc[u].im;
+    cmma<T>(cr0, ci0, xc[0], yc[0]);
+    if (k0 + 4 < kend) cmma<T>(cr1, ci1, xc[1], yc[1]);
+    if (k0 + 8 < kend) cmma<T>(cr0, ci0, xc[2], yc[2]);
+    if (k0 + 12 < kend) cmma<T>(cr1, ci1, xc[3], yc[3]);
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xc[u] = xn[u];
+        yc[u] = yn[u];
+      }
+    }
   }
   const int col = ly;
 #pragma unroll
@@ -263,11 +292,12 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
 
 template <typename T>
 __global__ void __launch_bounds__(64 * kWaves)
-    gc_grouped(const GcDesc* __restrict__ descs, int ndesc, int64_t ntasks, const cplx<T>* __restrict__ X0,
-               const cplx<T>* __restrict__ X1, cplx<T>* O, cplx<T>* work, const uint8_t* __restrict__ active) {
+    gc_grouped(const GcDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
+               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, cplx<T>* O, cplx<T>* work,
+               const uint8_t* __restrict__ active) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;
-  const GcDesc d = descs[find_desc(descs, ndesc, task)];
+  const GcDesc d = descs[task_desc(descs, ndesc, wgdesc, task)];
   if (active && d.node >= 0 && !active[d.node]) return;
   gc_task<T>(d, task - d.tile0, (d.sel & 1) ? X1 : X0, (d.sel & 2) ? X1 : X0, O, work);
 }
@@ -280,15 +310,16 @@ __global__ void __launch_bounds__(64 * kWaves)
   gc_task<T>(d, task, X, Y, O, O);  // no K split here: the workspace is never written
 }
 
-// one workgroup per split descriptor: sum the partial tiles in chunk order
+// as gm_reduce_kernel, for hole contractions
 template <typename T>
 __global__ void __launch_bounds__(256)
-    gc_reduce_kernel(const GcDesc* __restrict__ descs, cplx<T>* O, const cplx<T>* __restrict__ work,
+    gc_reduce_kernel(const GcDesc* __restrict__ descs, int ndesc, cplx<T>* O, const cplx<T>* __restrict__ work,
                      const uint8_t* __restrict__ active) {
-  const GcDesc d = descs[blockIdx.x];
+  const GcDesc d = descs[find_desc(descs, ndesc, blockIdx.x)];
   if (active && d.node >= 0 && !active[d.node]) return;
   const int64_t n = (int64_t)d.nx * d.ny;
-  for (int64_t e = threadIdx.x; e < n; e += 256) {
+  const int64_t e = (blockIdx.x - d.tile0) * 256 + threadIdx.x;
+  if (e < n) {
     cplx<T> s = work[d.part + e];
     for (int c = 1; c < d.ksplit; ++c) s = cadd(s, work[d.part + c * n + e]);
     if (d.acc) s = cadd(s, O[d.out + e]);
@@ -377,12 +408,12 @@ __global__ void __launch_bounds__(256)
 }  // namespace
 
 template <typename T, typename TO>
-int launch_gm(const GmDesc* descs, int64_t ndesc, int64_t ntasks, const void* x0, const void* x1,
-              const void* mbase, void* obase, void* part, const uint8_t* active, double sre, double sim,
-              hipStream_t s) {
+int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
+              const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
+              double sim, hipStream_t s) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((gm_grouped<T, TO>), dim3(blocks), dim3(64 * kWaves), 0, s, descs, (int)ndesc, ntasks,
+  hipLaunchKernelGGL((gm_grouped<T, TO>), dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc, ntasks,
                      (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<T>*)mbase, (cplx<TO>*)obase,
                      (cplx<TO>*)part, active, sre, sim);
   QTG_LAUNCH_CHECK();
@@ -390,11 +421,11 @@ int launch_gm(const GmDesc* descs, int64_t ndesc, int64_t ntasks, const void* x0
 }
 
 template <typename T>
-int launch_gc(const GcDesc* descs, int64_t ndesc, int64_t ntasks, const void* x0, const void* x1, void* obase,
-              void* work, const uint8_t* active, hipStream_t s) {
+int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
+              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL(gc_grouped<T>, dim3(blocks), dim3(64 * kWaves), 0, s, descs, (int)ndesc, ntasks,
+  hipLaunchKernelGGL(gc_grouped<T>, dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc, ntasks,
                      (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<T>*)obase, (cplx<T>*)work, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
@@ -425,21 +456,21 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 }
 
 template <typename TO>
-int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, void* obase, const void* part, const uint8_t* active,
-                     hipStream_t s) {
-  if (ndesc <= 0) return QTG_OK;
-  hipLaunchKernelGGL(gm_reduce_kernel<TO>, dim3((unsigned)ndesc), dim3(256), 0, s, descs, (cplx<TO>*)obase,
-                     (const cplx<TO>*)part, active);
+int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* part,
+                     const uint8_t* active, hipStream_t s) {
+  if (ndesc <= 0 || nblocks <= 0) return QTG_OK;
+  hipLaunchKernelGGL(gm_reduce_kernel<TO>, dim3((unsigned)nblocks), dim3(256), 0, s, descs, (int)ndesc,
+                     (cplx<TO>*)obase, (const cplx<TO>*)part, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 template <typename T>
-int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, void* obase, const void* work, const uint8_t* active,
-                     hipStream_t s) {
-  if (ndesc <= 0) return QTG_OK;
-  hipLaunchKernelGGL(gc_reduce_kernel<T>, dim3((unsigned)ndesc), dim3(256), 0, s, descs, (cplx<T>*)obase,
-                     (const cplx<T>*)work, active);
+int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* work,
+                     const uint8_t* active, hipStream_t s) {
+  if (ndesc <= 0 || nblocks <= 0) return QTG_OK;
+  hipLaunchKernelGGL(gc_reduce_kernel<T>, dim3((unsigned)nblocks), dim3(256), 0, s, descs, (int)ndesc,
+                     (cplx<T>*)obase, (const cplx<T>*)work, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -453,23 +484,27 @@ int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax
   return QTG_OK;
 }
 
-template int launch_gm<double, double>(const GmDesc*, int64_t, int64_t, const void*, const void*,
-    const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
-template int launch_gm<float, float>(const GmDesc*, int64_t, int64_t, const void*, const void*,
-    const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
-template int launch_gm<float, double>(const GmDesc*, int64_t, int64_t, const void*, const void*,
-    const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
-template int launch_gc<double>(const GcDesc*, int64_t, int64_t, const void*, const void*, void*, void*,
-    const uint8_t*, hipStream_t);
-template int launch_gc<float>(const GcDesc*, int64_t, int64_t, const void*, const void*, void*, void*,
-    const uint8_t*, hipStream_t);
+template int launch_gm<double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
+    const void*, const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
+template int launch_gm<float, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
+    const void*, const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
+template int launch_gm<float, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
+    const void*, const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
+template int launch_gc<double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*, void*,
+    void*, const uint8_t*, hipStream_t);
+template int launch_gc<float>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*, void*,
+    void*, const uint8_t*, hipStream_t);
 template int launch_gm_single<double>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gm_single<float>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gc_single<double>(const GcDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gc_single<float>(const GcDesc&, const void*, const void*, void*, hipStream_t);
-template int launch_gm_reduce<double>(const GmDesc*, int64_t, void*, const void*, const uint8_t*, hipStream_t);
-template int launch_gm_reduce<float>(const GmDesc*, int64_t, void*, const void*, const uint8_t*, hipStream_t);
-template int launch_gc_reduce<double>(const GcDesc*, int64_t, void*, const void*, const uint8_t*, hipStream_t);
-template int launch_gc_reduce<float>(const GcDesc*, int64_t, void*, const void*, const uint8_t*, hipStream_t);
+template int launch_gm_reduce<double>(const GmDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
+                                      hipStream_t);
+template int launch_gm_reduce<float>(const GmDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
+                                     hipStream_t);
+template int launch_gc_reduce<double>(const GcDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
+                                      hipStream_t);
+template int launch_gc_reduce<float>(const GcDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
+                                     hipStream_t);
 
 }  // namespace qtg

@@ -10,12 +10,15 @@
 // search.
 #pragma once
 #include <cstdint>
+#include <vector>
 
 #include "common.hpp"
 
 namespace qtg {
 
 enum GmOp : int32_t { GM_N = 0, GM_T = 1, GM_H = 2, GM_COPY = 3 };
+
+constexpr int kGemmWaves = 4;  // tasks (wavefronts) per workgroup of the grouped kernels
 
 struct GmDesc {
   int64_t x, m, out;                // complex-element offsets (input, matrix, output)
@@ -54,26 +57,41 @@ inline int64_t gc_tasks(const GcDesc& d) {
 // Launchers.  Input bases x0 / x1 (selected per descriptor), matrix base and
 // output base are cplx<T>* (output cplx<TO>*); `active` may be null.  Every
 // alpha of a mode-product launch is multiplied by the complex scale (sre, sim).
+// wgdesc[b]: descriptor of the first task (kGemmWaves * b) of workgroup b
+// (see gemm_wg_table).
 template <typename T, typename TO>
-int launch_gm(const GmDesc* descs, int64_t ndesc, int64_t ntasks, const void* x0, const void* x1,
-              const void* mbase, void* obase, void* part, const uint8_t* active, double sre, double sim,
-              hipStream_t s);
-// Sums the ksplit partial tiles (cplx<TO>) of every split mode-product descriptor into its output.
+int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
+              const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
+              double sim, hipStream_t s);
+// Sums the ksplit partial tiles (cplx<TO>) of split mode-product descriptors
+// into their outputs.  These descriptor copies carry in tile0 their first
+// block of 256 output elements; nblocks is the total.
 template <typename TO>
-int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, void* obase, const void* part, const uint8_t* active,
-                     hipStream_t s);
+int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* part,
+                     const uint8_t* active, hipStream_t s);
 template <typename T>
-int launch_gc(const GcDesc* descs, int64_t ndesc, int64_t ntasks, const void* x0, const void* x1, void* obase,
-              void* work, const uint8_t* active, hipStream_t s);
+int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
+              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s);
+
+// Appends to `out` the first-descriptor index of every workgroup of a launch
+// over `descs` (tile0 already assigned, ntasks in total).
+template <typename D>
+inline void gemm_wg_table(const D* descs, int64_t ndesc, int64_t ntasks, std::vector<int32_t>& out) {
+  int64_t di = 0;
+  for (int64_t t = 0; t < ntasks; t += kGemmWaves) {
+    while (di + 1 < ndesc && descs[di + 1].tile0 <= t) ++di;
+    out.push_back((int32_t)di);
+  }
+}
 // One descriptor (tile0 ignored), no active mask; no K split.
 template <typename T>
 int launch_gm_single(const GmDesc& d, const void* x, const void* m, void* o, hipStream_t s);
 template <typename T>
 int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hipStream_t s);
-// Sums the ksplit partial tiles of every split descriptor into its output.
+// As launch_gm_reduce, for hole contractions.
 template <typename T>
-int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, void* obase, const void* work, const uint8_t* active,
-                     hipStream_t s);
+int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* work,
+                     const uint8_t* active, hipStream_t s);
 
 // Batched FP64 solve X <- S^-1 X (Gauss-Jordan with partial pivoting) for n x n
 // column-major S and X at pool offsets s, x; n <= 64.  When r >= 0 it also

@@ -2,8 +2,9 @@
 // held in LDS, column-major with leading dimension ld.  All lanes of a
 // 64-lane workgroup call it.  Tournament (round-robin) ordering: each step
 // rotates up to 32 disjoint (p,q) pairs whose parameters come from the same
-// matrix, columns first, then rows; the lanes of one pair compute its
-// rotation in registers and a wavefront vote replaces any shared flag.
+// matrix; one pass applies both sides at once, each lane owning 2 x 2 blocks
+// (rows of pair i, columns of pair j), and a wavefront vote skips steps in
+// which no pair rotates.
 // Player at position j of step t: j == 0 -> 0, else 1 + (j - 1 + t) mod (m - 1).  A pair is skipped when
 // |a_pq| <= eps_mach sqrt(|a_pp a_qq|) (the relative-accuracy criterion of
 // Demmel and Veselic) or |a_pq| <= 1e-24 max_i |a_ii|; a rotated pair gets
@@ -19,7 +20,11 @@ namespace qtg {
 
 constexpr int EW = 64;
 
-struct JacobiScratch {};  // rotation parameters live in registers
+struct JacobiScratch {
+  double pc[32], ps[32];
+  cplx<double> pph[32];
+  int pp[32], pq[32];
+};
 
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, EW);
@@ -51,79 +56,106 @@ __device__ inline void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int l
   double scale = 0.0;
   for (int i = 0; i < n; ++i) scale = fmax(scale, fabs(A[i + i * ld].re));
   const double floor_abs = 1e-24 * scale;
-  // lane -> (pair i, row group g): the G lanes of a pair compute its
-  // rotation redundantly in registers and share the row / column updates
-  const int G = EW / np;
-  const int pi = threadIdx.x % np, g = threadIdx.x / np;
-  const bool lane_on = g < G;
   for (int sweep = 0; sweep < 30; ++sweep) {
     int rotated = 0;
     for (int step = 0; step < m - 1; ++step) {
-      const int j0 = pi, j1 = m - 1 - pi;
-      int p = (j0 == 0) ? 0 : 1 + (j0 - 1 + step) % (m - 1);
-      int q = (j1 == 0) ? 0 : 1 + (j1 - 1 + step) % (m - 1);
-      if (p > q) {
-        const int t = p;
-        p = q;
-        q = t;
-      }
-      double c = 1.0, sn = 0.0;
-      cplx<double> ph = mk<double>(1.0, 0.0);
-      if (lane_on && q < n) {
-        const cplx<double> apq = A[p + q * ld];
-        const double aa = sqrt(apq.re * apq.re + apq.im * apq.im);
-        const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
-        if (aa > 2.2e-16 * sqrt(fabs(app * aqq)) && aa > floor_abs && aa > 1e-300) {
-          const double theta = (aqq - app) / (2.0 * aa);
-          const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          c = 1.0 / sqrt(t * t + 1.0);
-          sn = t * c;
-          ph = mk<double>(apq.re / aa, apq.im / aa);
+      // phase 1: lane i < np computes the rotation of pair i
+      int flag = 0;
+      for (int i = threadIdx.x; i < np; i += EW) {
+        const int j0 = i, j1 = m - 1 - i;
+        int p = (j0 == 0) ? 0 : 1 + (j0 - 1 + step) % (m - 1);
+        int q = (j1 == 0) ? 0 : 1 + (j1 - 1 + step) % (m - 1);
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
         }
+        double c = 1.0, sn = 0.0;
+        cplx<double> ph = mk<double>(1.0, 0.0);
+        if (q < n) {
+          const cplx<double> apq = A[p + q * ld];
+          const double aa = sqrt(apq.re * apq.re + apq.im * apq.im);
+          const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
+          if (aa > 2.2e-16 * sqrt(fabs(app * aqq)) && aa > floor_abs && aa > 1e-300) {
+            const double theta = (aqq - app) / (2.0 * aa);
+            const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            sn = t * c;
+            ph = mk<double>(apq.re / aa, apq.im / aa);
+            flag = 1;
+          }
+        }
+        js.pp[i] = p;
+        js.pq[i] = q;
+        js.pc[i] = c;
+        js.ps[i] = sn;
+        js.pph[i] = ph;
       }
-      // the parameter reads above precede every write below in this
-      // wavefront's program order; one barrier keeps that explicit
+      if (!__any(flag)) continue;  // uniform: no pair rotates in this step
+      rotated = 1;
       __syncthreads();
-      const bool act = lane_on && sn != 0.0;
-      if (__any(act)) {
-        rotated = 1;
-        // columns: A <- A G, U <- U G  (G = [[c, s ph], [-s conj(ph), c]] on (p,q))
-        if (act)
-          for (int r = g; r < n; r += G) {
-            const cplx<double> arp = A[r + p * ld], arq = A[r + q * ld];
-            A[r + p * ld] = mk<double>(c * arp.re - sn * (ph.re * arq.re + ph.im * arq.im),
-                                       c * arp.im - sn * (ph.re * arq.im - ph.im * arq.re));
-            A[r + q * ld] = mk<double>(sn * (ph.re * arp.re - ph.im * arp.im) + c * arq.re,
-                                       sn * (ph.re * arp.im + ph.im * arp.re) + c * arq.im);
-            const cplx<double> urp = U[r + p * ld], urq = U[r + q * ld];
-            U[r + p * ld] = mk<double>(c * urp.re - sn * (ph.re * urq.re + ph.im * urq.im),
-                                       c * urp.im - sn * (ph.re * urq.im - ph.im * urq.re));
-            U[r + q * ld] = mk<double>(sn * (ph.re * urp.re - ph.im * urp.im) + c * urq.re,
-                                       sn * (ph.re * urp.im + ph.im * urp.re) + c * urq.im);
-          }
-        __syncthreads();
-        // rows: A <- G^H A  (new p = c*apk - s*ph*aqk ; new q = s*conj(ph)*apk + c*aqk)
-        if (act)
-          for (int k = g; k < n; k += G) {
-            const cplx<double> apk = A[p + k * ld], aqk = A[q + k * ld];
-            cplx<double> np_ = mk<double>(c * apk.re - sn * (ph.re * aqk.re - ph.im * aqk.im),
-                                          c * apk.im - sn * (ph.re * aqk.im + ph.im * aqk.re));
-            cplx<double> nq_ = mk<double>(sn * (ph.re * apk.re + ph.im * apk.im) + c * aqk.re,
-                                          sn * (ph.re * apk.im - ph.im * apk.re) + c * aqk.im);
-            // the rotated pair: exact zeros off the diagonal, real diagonal
-            if (k == q) {
-              np_ = czero<double>();
-              nq_.im = 0.0;
-            }
-            if (k == p) {
-              nq_ = czero<double>();
-              np_.im = 0.0;
-            }
-            A[p + k * ld] = np_;
-            A[q + k * ld] = nq_;
-          }
-        __syncthreads();
+      // phase 2: every 2x2 block (rows of pair bi, columns of pair bj) gets
+      // A <- G_bi^H A G_bj, and U <- U G_bj on the same rows, where
+      // G = [[c, s ph], [-s conj(ph), c]] on (p, q)
+      for (int o = threadIdx.x; o < np * np; o += EW) {
+        const int bi = o % np, bj = o / np;
+        const int p = js.pp[bi], q = js.pq[bi], pc = js.pp[bj], qc = js.pq[bj];
+        if (p >= n) continue;
+        const bool qv = q < n, qcv = qc < n;
+        const double ci = js.pc[bi], si = js.ps[bi], cj = js.pc[bj], sj = js.ps[bj];
+        const cplx<double> hi = js.pph[bi], hj = js.pph[bj];
+        // rows p, q x columns pc, qc
+        cplx<double> a00 = A[p + pc * ld];
+        cplx<double> a01 = qcv ? A[p + qc * ld] : czero<double>();
+        cplx<double> a10 = qv ? A[q + pc * ld] : czero<double>();
+        cplx<double> a11 = (qv && qcv) ? A[q + qc * ld] : czero<double>();
+        cplx<double> u00 = U[p + pc * ld];
+        cplx<double> u01 = qcv ? U[p + qc * ld] : czero<double>();
+        cplx<double> u10 = qv ? U[q + pc * ld] : czero<double>();
+        cplx<double> u11 = (qv && qcv) ? U[q + qc * ld] : czero<double>();
+        // right: X <- X G_bj (columns pc, qc)
+        auto right = [&](cplx<double>& x0, cplx<double>& x1) {
+          const cplx<double> a = x0, b = x1;
+          x0 = mk<double>(cj * a.re - sj * (hj.re * b.re + hj.im * b.im),
+                          cj * a.im - sj * (hj.re * b.im - hj.im * b.re));
+          x1 = mk<double>(sj * (hj.re * a.re - hj.im * a.im) + cj * b.re,
+                          sj * (hj.re * a.im + hj.im * a.re) + cj * b.im);
+        };
+        // left: X <- G_bi^H X (rows p, q)
+        auto left = [&](cplx<double>& x0, cplx<double>& x1) {
+          const cplx<double> a = x0, b = x1;
+          x0 = mk<double>(ci * a.re - si * (hi.re * b.re - hi.im * b.im),
+                          ci * a.im - si * (hi.re * b.im + hi.im * b.re));
+          x1 = mk<double>(si * (hi.re * a.re + hi.im * a.im) + ci * b.re,
+                          si * (hi.re * a.im - hi.im * a.re) + ci * b.im);
+        };
+        if (sj != 0.0) {
+          right(a00, a01);
+          right(a10, a11);
+          right(u00, u01);
+          right(u10, u11);
+        }
+        if (si != 0.0) {
+          left(a00, a10);
+          left(a01, a11);
+        }
+        if (bi == bj && si != 0.0) {
+          // the rotated pair: exact zeros off the diagonal, real diagonal
+          a01 = czero<double>();
+          a10 = czero<double>();
+          a00.im = 0.0;
+          a11.im = 0.0;
+        }
+        A[p + pc * ld] = a00;
+        if (qcv) A[p + qc * ld] = a01;
+        if (qv) A[q + pc * ld] = a10;
+        if (qv && qcv) A[q + qc * ld] = a11;
+        U[p + pc * ld] = u00;
+        if (qcv) U[p + qc * ld] = u01;
+        if (qv) U[q + pc * ld] = u10;
+        if (qv && qcv) U[q + qc * ld] = u11;
       }
+      __syncthreads();
     }
     if (!rotated) break;
   }
