@@ -143,12 +143,25 @@ int Engine::plan_gemm() {
     gl.d0 = (int64_t)h_gc_.size();
     gl.nd = (int64_t)descs.size();
     gl.s0 = (int64_t)h_gcsplit_.size();
+    // descriptors sharing a bra tensor sit next to each other, so the
+    // wavefronts of one workgroup mostly read the same X through L1
+    std::stable_sort(descs.begin(), descs.end(), [](const GcDesc& a, const GcDesc& b) {
+      return std::make_tuple(a.sel & 1, a.x, a.y) < std::make_tuple(b.sel & 1, b.x, b.y);
+    });
+    // launches with few tiles split long reductions so that every SIMD holds
+    // several wavefronts (a 16 x 16 tile is one wavefront)
+    int64_t tiles = 0;
+    for (auto& d : descs) tiles += gc_tasks(d);
+    const int64_t want = 2048;
     int64_t t = 0, part = 0, rblocks = 0;
     double fl = 0.0;
     for (auto& d : descs) {
       const int64_t K = (int64_t)d.bef * d.aft;
-      if (K > 2 * kGcSplitK) {
-        d.ksplit = (int32_t)((K + kGcSplitK - 1) / kGcSplitK);
+      int64_t ks = 1;
+      if (K > 2 * kGcSplitK) ks = (K + kGcSplitK - 1) / kGcSplitK;
+      if (tiles < 512 && K >= 128) ks = std::max<int64_t>(ks, std::min<int64_t>((want + tiles - 1) / tiles, K / 48));
+      if (ks > 1) {
+        d.ksplit = (int32_t)ks;
         d.part = part;
         part += (int64_t)d.ksplit * d.nx * d.ny;
         GcDesc r = d;

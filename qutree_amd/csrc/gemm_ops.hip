@@ -328,6 +328,9 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------------- solve
+// One workgroup per system: Gauss-Jordan with partial pivoting on [S | X] in
+// LDS; the pivot (largest modulus, lowest row on ties) comes from an argmax
+// over the first wavefront.
 __global__ void __launch_bounds__(256)
     solve_kernel(const SolveItem* __restrict__ items, cplx<double>* pool, const uint8_t* __restrict__ active) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -343,20 +346,27 @@ __global__ void __launch_bounds__(256)
     Xm[o] = pool[it.x + o];
   }
   __syncthreads();
-  // Gauss-Jordan with partial pivoting on [S | X] (column-major, n x n each)
   for (int col = 0; col < n; ++col) {
-    if (threadIdx.x == 0) {
-      int p = col;
+    if (threadIdx.x < 64) {
       double best = -1.0;
-      for (int r = col; r < n; ++r) {
+      int bi = col;
+      for (int r = col + (int)threadIdx.x; r < n; r += 64) {
         const cplx<double> v = Sm[r + col * n];
         const double a = v.re * v.re + v.im * v.im;
         if (a > best) {
           best = a;
-          p = r;
+          bi = r;
         }
       }
-      piv = p;
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (threadIdx.x == 0) piv = bi;
     }
     __syncthreads();
     const int p = piv;
