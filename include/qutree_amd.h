@@ -174,6 +174,16 @@ int qtg_engine_get_matrix(qtg_engine engine, int which, int64_t term, int64_t no
  * node) as TreeFunctions and HamiltonianRepresentation::Apply loop) and item
  * counts (items[0] sandwich items, items[1] apply chunks, items[2] apply terms). */
 int qtg_engine_counts(qtg_engine engine, double* sweep_flops, int64_t* items);
+/* Gradient of the energy functional E = <Psi|H|Psi> / <Psi|Psi> with respect
+ * to conj(A) at every node (complex128, grouped path), for the state set by
+ * qtg_engine_set_psi with orthonormal non-top nodes:
+ *   dE/dA_v* = (Apply(v) + M_v x_last A_v - E rho_v x_last A_v) / <A_top|A_top>,
+ * Apply as in HamiltonianRepresentation.cpp:6-43, M_v the mean field of the
+ * terms inactive at v (which the projector of LayerDerivative removes).  No
+ * reference counterpart; rho^-1 (1 - P) of it is the reference's
+ * imaginary-time LayerDerivative.  grad_host (engine layout, complex128
+ * interleaved) and energy may be null. */
+int qtg_engine_gradient(qtg_engine engine, double* grad_host, double* energy, qtg_stream stream);
 /* FLOPs one sweep of this engine's plan executes (fewer than the reference
  * count where equal slots are shared and Apply summands merged). */
 int qtg_engine_plan_flops(qtg_engine engine, double* executed);

@@ -1064,6 +1064,63 @@ Tensor LayerDerivative(const SOP& H, const Tensor& Phi, const HamRep& r, const T
   return dPhi;
 }
 
+// Gradient of E = <Psi|H|Psi>/<Psi|Psi> with respect to conj(A_v) at every
+// node, for a tree whose non-top nodes are orthonormal (the gauge in which
+// HamiltonianRepresentation works):
+//   dE/dA_v* = (H_eff,v A_v - E rho_v x_last A_v) / <A_top|A_top>,
+//   H_eff,v A_v = Apply(v) + M_v x_last A_v,
+// where Apply is HamiltonianRepresentation::Apply (the terms active at v)
+// and M_v = sum over the terms inactive at v of c_t hole_t(v), their mean
+// field, obtained top-down from the parent p:
+//   M_v = contraction(A_p, sum_{t active at p, not at v} c_t apply_t(A_p)
+//                          + M_p x_last A_p, childIdx(v)).
+// The reference drops the inactive terms because (1 - P) removes them from
+// the MCTDH derivative; the energy gradient keeps them.  No reference
+// counterpart: this pins the device gradient, and finite differences of
+// the energy pin this function.
+TensorTree EnergyGradient(const SOP& H, const TensorTree& P, const HamRep& r, const Tree& t, double& E) {
+  const Node& top = t.nodes[t.top()];
+  const cd num = Expectation(H, P, r, t)(0, 0);
+  const cd Nrm = dotProduct(P[top.address], P[top.address])(0, 0);
+  E = std::real(num / Nrm);
+  MatrixTree M(t.nodes.size());
+  for (int a = (int)t.nodes.size() - 1; a >= 0; --a) {  // parents before children
+    const Node& n = t.nodes[a];
+    if (n.isTop()) continue;
+    const Node& p = t.nodes[n.parent];
+    const Tensor& Ap = P[p.address];
+    Tensor Y(Ap.s);
+    for (size_t l = 0; l < H.terms.size(); ++l) {
+      if (!r.hMats[l].active(p.address) || r.hMats[l].active(a)) continue;
+      Tensor Psi(Ap.s);
+      for (size_t i = 0; i < Psi.c.size(); ++i) Psi[i] = H.coeff[l] * Ap[i];
+      Psi = applyNode(r.hMats[l], &r.hCon[l], Psi, t, p, n.childIdx);
+      for (size_t i = 0; i < Y.c.size(); ++i) Y[i] += Psi[i];
+    }
+    if (!p.isTop()) {
+      Tensor K = matrixTensor(M[p.address], Ap, Ap.s.lastIdx());
+      for (size_t i = 0; i < Y.c.size(); ++i) Y[i] += K[i];
+    }
+    M[a] = contraction(Ap, Y, n.childIdx);
+  }
+  TensorTree G = makeTensorTree(t);
+  for (auto& n : t.nodes) {
+    const Tensor& A = P[n.address];
+    Tensor g = Apply(H, A, r, t, n);
+    if (n.isTop()) {
+      for (size_t i = 0; i < g.c.size(); ++i) g[i] -= E * A[i];
+    } else {
+      Mat X = M[n.address];
+      const Mat& rho = r.rho[n.address];
+      for (size_t i = 0; i < X.c.size(); ++i) X.c[i] -= E * rho.c[i];
+      multStateAB(g, X, A, false);
+    }
+    for (auto& x : g.c) x /= Nrm;
+    G[n.address] = g;
+  }
+  return G;
+}
+
 // LayerInterface::Error (LayerInterface.cpp:6-31)
 double LayerError(const Tensor& Phi, const Tensor& Chi, const HamRep& r, const Node& node) {
   double Delta = 0., norm = 1.;
@@ -1629,6 +1686,24 @@ int orc_derivative(void* h, const double* psi, double* dpsi, double phase_re, do
     for (auto& n : m->tree.nodes)
       D[n.address] = LayerDerivative(m->H, P[n.address], m->rep, m->tree, n, cd(phase_re, phase_im));
     m->pack(D, dpsi);
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+// Energy gradient dE/dA* at every node (orthonormal lower nodes): build, then
+// EnergyGradient.  *energy receives E.
+int orc_gradient(void* h, const double* psi, double* grad, double eps, double* energy) {
+  Model* m = (Model*)h;
+  try {
+    TensorTree P = m->unpack(psi);
+    build(m->rep, m->H, P, m->tree, eps);
+    m->built = true;
+    double E = 0.0;
+    TensorTree G = EnergyGradient(m->H, P, m->rep, m->tree, E);
+    m->pack(G, grad);
+    if (energy) *energy = E;
     return 0;
   } catch (std::exception& e) {
     g_err = e.what();

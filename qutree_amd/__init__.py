@@ -76,6 +76,7 @@ def lib():
     L.qtg_engine_get_matrix.argtypes = [_vp, C.c_int, _i64, _i64, _dp, _vp]
     L.qtg_engine_counts.argtypes = [_vp, C.POINTER(C.c_double), _lp]
     L.qtg_engine_plan_flops.argtypes = [_vp, C.POINTER(C.c_double)]
+    L.qtg_engine_gradient.argtypes = [_vp, _dp, C.POINTER(C.c_double), _vp]
     L.qtg_engine_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
     L.qtg_engine_set_timing.argtypes = [_vp, C.c_int]
     L.qtg_engine_synchronize.argtypes = [_vp]
@@ -280,6 +281,14 @@ class Engine:
         items = np.zeros(3, np.int64)
         check(lib().qtg_engine_counts(self.ptr, C.byref(f), items))
         return f.value, items
+
+    def gradient(self):
+        """(dE/dA* at every node, E) for the state set by set_psi (orthonormal
+        non-top nodes)."""
+        out = np.zeros(2 * self.psi_size)
+        e = C.c_double()
+        check(lib().qtg_engine_gradient(self.ptr, out, C.byref(e), None))
+        return out.view(np.complex128).copy(), e.value
 
     def executed_flops(self) -> float:
         """FLOPs one sweep of this engine's plan executes."""

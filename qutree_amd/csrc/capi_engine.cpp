@@ -129,6 +129,12 @@ int qtg_engine_counts(qtg_engine e, double* sweep_flops, int64_t* items) {
   return QTG_OK;
 }
 
+int qtg_engine_gradient(qtg_engine e, double* grad_host, double* energy, qtg_stream s) {
+  if (!e) return QTG_EARG;
+  auto* en = static_cast<Engine*>(e);
+  return en->gradient(ES(e, s), grad_host, energy);
+}
+
 int qtg_engine_plan_flops(qtg_engine e, double* executed) {
   if (!e || !executed) return QTG_EARG;
   *executed = static_cast<Engine*>(e)->sweep_flops();

@@ -50,6 +50,7 @@ void Engine::release() {
   for (void* p : device_buffers_)
     if (p) (void)hipFree(p);
   device_buffers_.clear();
+  d_grad64_ = nullptr;
 }
 
 int Engine::alloc(void** p, size_t bytes) {
@@ -380,8 +381,14 @@ int Engine::plan() {
   std::vector<ApTerm> terms;
   std::vector<ApChunk> chunks;
   std::vector<FinNode> fin;
-  std::vector<CombOut> combOut, combOut2;
-  std::vector<CombTerm> combTerm, combTerm2;
+  std::vector<CombOut>& combOut = h_comb1_;
+  std::vector<CombOut>& combOut2 = h_comb2_;
+  std::vector<CombTerm>& combTerm = h_combt1_;
+  std::vector<CombTerm>& combTerm2 = h_combt2_;
+  combOut.clear();
+  combOut2.clear();
+  combTerm.clear();
+  combTerm2.clear();
   double combFlops = 0.0;
   int64_t part = 0;
   summ_.assign(NN, {});
@@ -409,14 +416,10 @@ int Engine::plan() {
       sdim.push_back(v.n);
     }
     const int S = (int)smode.size();
-    struct G {
-      std::vector<int64_t> ids;
-      cd coef;
-    };
-    std::vector<G> tl;
+    std::vector<Summand> tl;
     for (int l = 0; l < L; ++l) {
       if (!active_[l][v.addr]) continue;
-      G g;
+      Summand g;
       g.coef = sop_.terms[l].coeff;
       if (v.bottom) {
         g.ids.push_back(leafmat(l, v.leaf.mode));
@@ -426,88 +429,8 @@ int Engine::plan() {
       if (!v.top()) g.ids.push_back(holeslot_[l][v.addr]);
       tl.push_back(g);
     }
-    std::vector<G> groups;
-    if (compress_) {
-      std::vector<char> alive(tl.size(), 1);
-      size_t nalive = tl.size();
-      while (nalive > 1) {
-        int best_s = -1;
-        std::vector<int> best;
-        for (int sl = 0; sl < S; ++sl) {
-          std::map<std::vector<int64_t>, std::vector<int>> gm;
-          for (size_t t = 0; t < tl.size(); ++t) {
-            if (!alive[t]) continue;
-            std::vector<int64_t> k = tl[t].ids;
-            k.erase(k.begin() + sl);
-            gm[k].push_back((int)t);
-          }
-          for (auto& kv : gm)
-            if (kv.second.size() > std::max<size_t>(1, best.size())) {
-              best = kv.second;
-              best_s = sl;
-            }
-        }
-        if (best_s < 0) break;
-        // new aggregated matrix for slot best_s
-        // Long sums are split into partial sums of <= kCombChunk terms
-        // (first launch) that a second launch adds, so no workgroup walks a
-        // long serial chain of loads.
-        const int64_t d = sdim[best_s];
-        const size_t kCombChunk = 32;
-        std::vector<int64_t> parts;
-        for (size_t c0 = 0; c0 < best.size(); c0 += kCombChunk) {
-          CombOut co{};
-          co.out = off;
-          co.n = (int32_t)d;
-          co.t0 = (int32_t)combTerm.size();
-          for (size_t q = c0; q < std::min(best.size(), c0 + kCombChunk); ++q) {
-            const int t = best[q];
-            CombTerm ct{};
-            ct.cre = tl[t].coef.real();
-            ct.cim = tl[t].coef.imag();
-            ct.src = tl[t].ids[best_s];
-            combTerm.push_back(ct);
-            alive[t] = 0;
-            --nalive;
-          }
-          co.t1 = (int32_t)combTerm.size();
-          combOut.push_back(co);
-          parts.push_back(off);
-          off += d * d;
-        }
-        combFlops += 8.0 * d * d * best.size();
-        int64_t result = parts[0];
-        if (parts.size() > 1) {
-          CombOut co{};
-          co.out = off;
-          co.n = (int32_t)d;
-          co.t0 = (int32_t)combTerm2.size();
-          for (int64_t pidx : parts) {
-            CombTerm ct{};
-            ct.cre = 1.0;
-            ct.cim = 0.0;
-            ct.src = pidx;
-            combTerm2.push_back(ct);
-          }
-          co.t1 = (int32_t)combTerm2.size();
-          combOut2.push_back(co);
-          combFlops += 8.0 * d * d * parts.size();
-          result = off;
-          off += d * d;
-        }
-        G g;
-        g.ids = tl[best[0]].ids;
-        g.ids[best_s] = result;
-        g.coef = cd(1.0, 0.0);
-        groups.push_back(g);
-      }
-      for (size_t t = 0; t < tl.size(); ++t)
-        if (alive[t]) groups.push_back(tl[t]);
-    } else {
-      groups = tl;
-    }
-    summ_[v.addr].clear();
-    for (auto& g : groups) summ_[v.addr].push_back(Summand{g.coef, g.ids});
+    std::vector<Summand> groups = compress_ ? aggregate(tl, sdim, off, combFlops) : tl;
+    summ_[v.addr] = groups;
     smode_[v.addr] = smode;
     sdim_[v.addr] = sdim;
     std::vector<ApTerm> vt;
@@ -546,6 +469,7 @@ int Engine::plan() {
   nFin_ = (int64_t)fin.size();
   nComb_ = (int64_t)combOut.size();
   nComb2_ = (int64_t)combOut2.size();
+  combFlops12_ = combFlops;
   {
     double proj = 0.0;
     for (auto& v : tree_.nodes)
@@ -610,6 +534,16 @@ int Engine::plan() {
     device_buffers_.push_back(d_combOut3_);
     if ((rc = up(&d_combTerm3_, h_combt3_))) return rc;
     device_buffers_.push_back(d_combTerm3_);
+    for (auto pr : {std::make_pair(&d_combG1_, &h_combG1_), std::make_pair(&d_combG2_, &h_combG2_),
+                    std::make_pair(&d_combG3_, &h_combG3_), std::make_pair(&d_combGE_, &h_combGE_)}) {
+      if ((rc = up(pr.first, *pr.second))) return rc;
+      device_buffers_.push_back(*pr.first);
+    }
+    for (auto pr : {std::make_pair(&d_combtG1_, &h_combtG1_), std::make_pair(&d_combtG2_, &h_combtG2_),
+                    std::make_pair(&d_combtG3_, &h_combtG3_), std::make_pair(&d_combtGE_, &h_combtGE_)}) {
+      if ((rc = up(pr.first, *pr.second))) return rc;
+      device_buffers_.push_back(*pr.first);
+    }
     if ((rc = up(&d_gmwg_, h_gmwg_))) return rc;
     device_buffers_.push_back(d_gmwg_);
     if ((rc = up(&d_gcwg_, h_gcwg_))) return rc;
@@ -651,6 +585,90 @@ int Engine::plan() {
   }
   planned_ = true;
   return QTG_OK;
+}
+
+// Greedy merge of Kronecker-product summands: terms that agree on every slot
+// but one become one summand whose free slot is the coefficient-weighted sum
+// of their matrices (largest group first).  The sums are combination work of
+// the build; long sums are split into partial sums of <= 32 terms (first
+// combine launch) that a second launch adds, so no workgroup walks a long
+// serial chain of loads.
+std::vector<Engine::Summand> Engine::aggregate(const std::vector<Summand>& tl, const std::vector<int>& sdim,
+                                               int64_t& off, double& combFlops) {
+  std::vector<Summand> groups;
+  const int S = (int)sdim.size();
+  std::vector<char> alive(tl.size(), 1);
+  size_t nalive = tl.size();
+  while (nalive > 1) {
+    int best_s = -1;
+    std::vector<int> best;
+    for (int sl = 0; sl < S; ++sl) {
+      std::map<std::vector<int64_t>, std::vector<int>> gm;
+      for (size_t t = 0; t < tl.size(); ++t) {
+        if (!alive[t]) continue;
+        std::vector<int64_t> k = tl[t].ids;
+        k.erase(k.begin() + sl);
+        gm[k].push_back((int)t);
+      }
+      for (auto& kv : gm)
+        if (kv.second.size() > std::max<size_t>(1, best.size())) {
+          best = kv.second;
+          best_s = sl;
+        }
+    }
+    if (best_s < 0) break;
+    const int64_t d = sdim[best_s];
+    const size_t kCombChunk = 32;
+    std::vector<int64_t> parts;
+    for (size_t c0 = 0; c0 < best.size(); c0 += kCombChunk) {
+      CombOut co{};
+      co.out = off;
+      co.n = (int32_t)d;
+      co.t0 = (int32_t)h_combt1_.size();
+      for (size_t q = c0; q < std::min(best.size(), c0 + kCombChunk); ++q) {
+        const int t = best[q];
+        CombTerm ct{};
+        ct.cre = tl[t].coef.real();
+        ct.cim = tl[t].coef.imag();
+        ct.src = tl[t].ids[best_s];
+        h_combt1_.push_back(ct);
+        alive[t] = 0;
+        --nalive;
+      }
+      co.t1 = (int32_t)h_combt1_.size();
+      h_comb1_.push_back(co);
+      parts.push_back(off);
+      off += d * d;
+    }
+    combFlops += 8.0 * d * d * best.size();
+    int64_t result = parts[0];
+    if (parts.size() > 1) {
+      CombOut co{};
+      co.out = off;
+      co.n = (int32_t)d;
+      co.t0 = (int32_t)h_combt2_.size();
+      for (int64_t pidx : parts) {
+        CombTerm ct{};
+        ct.cre = 1.0;
+        ct.cim = 0.0;
+        ct.src = pidx;
+        h_combt2_.push_back(ct);
+      }
+      co.t1 = (int32_t)h_combt2_.size();
+      h_comb2_.push_back(co);
+      combFlops += 8.0 * d * d * parts.size();
+      result = off;
+      off += d * d;
+    }
+    Summand g;
+    g.ids = tl[best[0]].ids;
+    g.ids[best_s] = result;
+    g.coef = cd(1.0, 0.0);
+    groups.push_back(g);
+  }
+  for (size_t t = 0; t < tl.size(); ++t)
+    if (alive[t]) groups.push_back(tl[t]);
+  return groups;
 }
 
 int Engine::set_psi(const double* host, hipStream_t s) {

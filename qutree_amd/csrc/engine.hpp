@@ -58,6 +58,10 @@ class Engine {
   // top node) is written when not null.  Uses the last build's matrices.
   int apply_finish(const double* psi64, double* dpsi64, const uint8_t* active, double ph_re, double ph_im,
                    double* energy_dev, hipStream_t s);
+  // dE/dA* at every node (complex128 engines, grouped path) of the state set
+  // by set_psi, whose non-top nodes are orthonormal; grad_host receives the
+  // gradient in the engine's layout, energy the expectation value.
+  int gradient(hipStream_t s, double* grad_host, double* energy);
   hipStream_t stream() const { return stream_; }
   // waits for the engine's streams (its own and the eigensolver's side stream)
   int synchronize() {
@@ -126,6 +130,21 @@ class Engine {
     std::vector<int64_t> ids;  // matrix per slot (-1: identity)
   };
   std::vector<std::vector<Summand>> summ_;    // Apply summands per node
+  std::vector<Summand> aggregate(const std::vector<Summand>& tl, const std::vector<int>& sdim, int64_t& off,
+                                 double& combFlops);
+  std::vector<CombOut> h_comb1_, h_comb2_;  // aggregated slots: partial sums, sums of partials
+  std::vector<CombTerm> h_combt1_, h_combt2_;
+  double combFlops12_ = 0.0;
+  // energy gradient (grouped path)
+  std::vector<int64_t> mslot_, gslot_;
+  std::vector<CombOut> h_combG1_, h_combG2_, h_combG3_, h_combGE_;
+  std::vector<CombTerm> h_combtG1_, h_combtG2_, h_combtG3_, h_combtGE_;
+  std::vector<GLaunch> gGradPre_, gGradFin_;
+  std::vector<std::vector<GLaunch>> gGradDepth_;
+  double* d_grad64_ = nullptr;
+  double gradFlops_ = 0.0;
+  CombOut *d_combG1_ = nullptr, *d_combG2_ = nullptr, *d_combG3_ = nullptr, *d_combGE_ = nullptr;
+  CombTerm *d_combtG1_ = nullptr, *d_combtG2_ = nullptr, *d_combtG3_ = nullptr, *d_combtGE_ = nullptr;
   std::vector<std::vector<int>> smode_, sdim_;  // slot legs and sizes per node
   double lflops_ = 0.0;                       // sweep flops of the LDS path
   // grouped path

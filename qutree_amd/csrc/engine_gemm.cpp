@@ -462,14 +462,215 @@ int Engine::plan_gemm() {
   phaseFlops_[3] = flops - ff0;
 
   // combine flops (aggregated slots of the shared plan, stacked blocks)
-  double comb = 0.0;
+  double comb = combFlops12_;
   for (auto& co : h_comb3_) comb += 8.0 * co.n * co.n;
   phaseFlops_[5] = comb;
   flops += comb;
   nComb3_ = (int64_t)h_comb3_.size();
+  flops_ = flops;
+
+  // ------------------------------------------------------------- gradient
+  // dE/dA_v* = (H_eff,v A_v - E rho_v x_L A_v) / <Psi|Psi> (orthonormal
+  // lower nodes).  H_eff,v A_v = Apply(v) + M_v x_L A_v, where Apply(v) is
+  // HamiltonianRepresentation::Apply (terms active at v) and M_v, the mean
+  // field of the terms inactive at v, follows top-down:
+  //   M_v = <Psi_p| [ sum_{t active at p, not at v} c_t (x_sib h_t) x hole_t(p)
+  //                   + M_p x_par ] |Psi_p>  over all legs of p but v's.
+  // The first part is an Apply-like stacked sum at p (computed for all nodes
+  // at once), the second a mode product on the parent leg that accumulates
+  // into it once M_p exists; then one grouped hole contraction per depth.
+  h_combG1_.clear();
+  h_combG2_.clear();
+  h_combG3_.clear();
+  h_combtG3_.clear();
+  h_combGE_.clear();
+  h_combtGE_.clear();
+  gGradPre_.clear();
+  gGradFin_.clear();
+  gGradDepth_.assign(tree_.maxDepth + 1, {});
+  mslot_.assign(NN, -1);
+  for (auto& v : tree_.nodes)
+    if (!v.top()) {
+      mslot_[v.addr] = poolOff;
+      poolOff += (int64_t)v.n * v.n;
+    }
+  {
+    // the aggregation helper appends to the build's combination lists; keep
+    // the gradient's apart (they run only when a gradient is asked for)
+    std::vector<CombOut> c1 = h_comb1_, c2 = h_comb2_;
+    std::vector<CombTerm> t1 = h_combt1_, t2 = h_combt2_;
+    h_comb1_.clear();
+    h_comb2_.clear();
+    h_combt1_.clear();
+    h_combt2_.clear();
+    double gcomb = 0.0;
+    std::vector<std::vector<GmDesc>> glevels(maxLevels);
+    std::vector<GmDesc> gstacked;
+    std::vector<std::vector<GmDesc>> gacc(tree_.maxDepth + 1);
+    std::vector<std::vector<GcDesc>> gcon(tree_.maxDepth + 1);
+    int64_t gw = 0;
+    const int L_ = (int)sop_.terms.size();
+    for (auto& v : tree_.nodes) {
+      if (v.top()) continue;
+      const HNode& p = tree_.nodes[v.parent];
+      const int Lp = p.order() - 1;
+      const int nL = p.dims[Lp];
+      // summands at p of the terms active at p and inactive at v
+      std::vector<Summand> tl;
+      for (int l = 0; l < L_; ++l) {
+        if (!active_[l][p.addr] || active_[l][v.addr]) continue;
+        Summand g;
+        g.coef = sop_.terms[l].coeff;
+        for (int c : p.children) g.ids.push_back(active_[l][c] ? hslot_[l][c] : -1);
+        if (!p.top()) g.ids.push_back(holeslot_[l][p.addr]);
+        tl.push_back(g);
+      }
+      std::vector<Summand> sm = compress_ ? aggregate(tl, sdim_[p.addr], poolOff, gcomb) : tl;
+      const int64_t yv = gw;  // Y_v (N_p elements)
+      gw += p.N;
+      if (sm.empty()) {
+        GmDesc z = gm_desc(p, Lp, p.off, 0, 0, yv, GM_COPY, -1);
+        z.are = 0.0;
+        z.xsel = 0;
+        gstacked.push_back(z);
+      } else {
+        const int64_t stack0 = gw;
+        gw += (int64_t)sm.size() * p.N;
+        const int64_t mstack = poolOff;
+        poolOff += (int64_t)sm.size() * nL * nL;
+        std::map<std::vector<int64_t>, int64_t> zmemo;
+        for (size_t s = 0; s < sm.size(); ++s) {
+          std::vector<int64_t> fac(Lp, -1);
+          int64_t gL = -1;
+          for (size_t sl = 0; sl < smode_[p.addr].size(); ++sl) {
+            if (sm[s].ids[sl] < 0) continue;
+            if (smode_[p.addr][sl] == Lp) gL = sm[s].ids[sl];
+            else fac[smode_[p.addr][sl]] = sm[s].ids[sl];
+          }
+          int64_t cur = p.off;
+          int cursel = 0;
+          std::vector<int64_t> key = {p.addr};
+          for (int leg = Lp - 1; leg >= 0; --leg) {
+            const int lvl = maxLevels - 1 - leg;
+            const bool last = (leg == 0);
+            const int64_t slot = stack0 + (int64_t)s * p.N;
+            if (fac[leg] < 0) {
+              if (last) glevels[lvl].push_back(gm_desc(p, 0, cur, cursel, 0, slot, GM_COPY, -1));
+              continue;
+            }
+            key.push_back(leg);
+            key.push_back(fac[leg]);
+            if (last) {
+              glevels[lvl].push_back(gm_desc(p, leg, cur, cursel, fac[leg], slot, GM_N, -1));
+            } else {
+              auto z = zmemo.find(key);
+              if (z == zmemo.end()) {
+                const int64_t o = gw;
+                gw += p.N;
+                glevels[lvl].push_back(gm_desc(p, leg, cur, cursel, fac[leg], o, GM_N, -1));
+                z = zmemo.emplace(key, o).first;
+              }
+              cur = z->second;
+              cursel = 1;
+            }
+          }
+          CombOut co{};
+          co.out = mstack + (int64_t)s * nL * nL;
+          co.n = nL;
+          co.t0 = (int32_t)h_combtG3_.size();
+          CombTerm ct{};
+          ct.cre = sm[s].coef.real();
+          ct.cim = sm[s].coef.imag();
+          ct.src = (gL >= 0) ? gL : identity(nL);
+          h_combtG3_.push_back(ct);
+          co.t1 = (int32_t)h_combtG3_.size();
+          h_combG3_.push_back(co);
+        }
+        GmDesc g{};
+        g.x = stack0;
+        g.xsel = 1;
+        g.m = mstack;
+        g.out = yv;
+        g.bef = (int32_t)(p.N / nL);
+        g.nin = (int32_t)(sm.size() * nL);
+        g.nout = nL;
+        g.aft = 1;
+        g.op = GM_N;
+        g.are = 1.0;
+        g.node = -1;
+        gstacked.push_back(g);
+      }
+      // Y_v += M_p x_par Psi_p, once M_p exists (depth of p)
+      if (!p.top()) {
+        GmDesc a = gm_desc(p, Lp, p.off, 0, mslot_[p.addr], yv, GM_N, -1);
+        a.acc = 1;
+        gacc[v.depth].push_back(a);
+      }
+      // M_v = <Psi_p| Y_v> over the legs of p but v's
+      GcDesc c{};
+      int64_t bef, aft;
+      int nk;
+      view(p, v.childIdx, bef, nk, aft);
+      c.x = p.off;
+      c.y = yv;
+      c.out = mslot_[v.addr];
+      c.bef = (int32_t)bef;
+      c.nx = c.ny = nk;
+      c.aft = (int32_t)aft;
+      c.node = -1;
+      c.ksplit = 1;
+      c.sel = 2;  // X = Psi (first base), Y = workspace (second base)
+      gcon[v.depth].push_back(c);
+    }
+    workSize_ = std::max(workSize_, gw);
+    h_combG1_ = h_comb1_;
+    h_combG2_ = h_comb2_;
+    h_combtG1_ = h_combt1_;
+    h_combtG2_ = h_combt2_;
+    h_comb1_ = c1;
+    h_comb2_ = c2;
+    h_combt1_ = t1;
+    h_combt2_ = t2;
+    for (auto& lv : glevels) push_gm(gGradPre_, lv, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+    push_gm(gGradPre_, gstacked, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+    for (int d = 1; d <= tree_.maxDepth; ++d) {
+      push_gm(gGradDepth_[d], gacc[d], 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+      push_gc(gGradDepth_[d], gcon[d], 0, B_PSI, B_WORK, B_POOL);
+    }
+    // G_v = M_v - E rho_v (E set at run time), then
+    // grad = (Apply + G_v x_L A) / <A_top|A_top>; at the top (Apply - E A) / N
+    std::vector<GmDesc> f1, f2;
+    gslot_.clear();
+    for (auto& v : tree_.nodes) {
+      const int L = v.order() - 1;
+      f1.push_back(gm_desc(v, L, v.off, 1, 0, v.off, GM_COPY, -1));
+      // G_v = M_v - E rho_v; at the top (no parent leg, rho = 1) G = -E
+      gslot_.push_back(poolOff);
+      CombOut co{};
+      co.out = poolOff;
+      co.n = v.dims[L];
+      co.t0 = (int32_t)h_combtGE_.size();
+      CombTerm a{};
+      a.cre = v.top() ? 0.0 : 1.0;
+      a.src = v.top() ? -1 : mslot_[v.addr];
+      CombTerm b{};
+      b.cre = 0.0;  // -E, set at run time
+      b.src = v.top() ? -1 : rhoslot_[v.addr];
+      h_combtGE_.push_back(a);
+      h_combtGE_.push_back(b);
+      co.t1 = (int32_t)h_combtGE_.size();
+      h_combGE_.push_back(co);
+      GmDesc g = gm_desc(v, L, v.off, 0, poolOff, v.off, GM_N, -1);
+      g.acc = 1;
+      f2.push_back(g);
+      poolOff += (int64_t)v.dims[L] * v.dims[L];
+    }
+    push_gm(gGradFin_, f1, 2, B_PSI64, B_APPLY64, B_POOL, B_DPSI64, true);
+    push_gm(gGradFin_, f2, 2, B_PSI64, B_APPLY64, B_POOL, B_DPSI64, true);
+    gradFlops_ = gcomb;
+  }
   poolSize_ = poolOff;
   pool64Size_ = pool64Off;
-  flops_ = flops;
   return QTG_OK;
 }
 
@@ -598,6 +799,44 @@ int Engine::apply_finish(const double* psi64, double* dpsi64, const uint8_t* act
     return rc;
   return launch_finish<double>(d_fin_, nFin_, d_nodes_, psi64, d_partial_, d_pool64_, dpsi64, nullptr, ph_re, ph_im,
                                energy_dev, nmax_, nspf_, s, active);
+}
+
+int Engine::gradient(hipStream_t s, double* grad_host, double* energy) {
+  if (c64_ || path_ != 1) {
+    set_error("engine: the energy gradient needs a complex128 engine on the grouped path");
+    return QTG_ESTATE;
+  }
+  if (!planned_) {
+    set_error("engine: no Hamiltonian");
+    return QTG_ESTATE;
+  }
+  int rc = derivative(s, 1.0, 0.0);
+  if (rc) return rc;
+  double e[4];
+  QTG_HIP_TRY(hipMemcpyAsync(e, d_pool64_ + 2 * energySlot_, sizeof(e), hipMemcpyDeviceToHost, s));
+  QTG_HIP_TRY(hipStreamSynchronize(s));
+  const cd num(e[0], e[1]), den(e[2], e[3]);
+  const double E = (num / den).real(), Nrm = den.real();
+  if (energy) *energy = E;
+  if (!d_grad64_) {
+    if ((rc = alloc((void**)&d_grad64_, sizeof(double) * 2 * tree_.total))) return rc;
+  }
+  for (size_t i = 1; i < h_combtGE_.size(); i += 2) h_combtGE_[i].cre = -E;
+  QTG_HIP_TRY(hipMemcpy(d_combtGE_, h_combtGE_.data(), sizeof(CombTerm) * h_combtGE_.size(), hipMemcpyHostToDevice));
+  void* bases[B_COUNT] = {d_psi64_, d_work_, d_pool_, d_psi64_, d_apply64_, d_grad64_, d_pool64_};
+  if ((rc = launch_combine<double>(d_combG1_, (int64_t)h_combG1_.size(), d_combtG1_, d_pool_, s))) return rc;
+  if ((rc = launch_combine<double>(d_combG2_, (int64_t)h_combG2_.size(), d_combtG2_, d_pool_, s))) return rc;
+  if ((rc = launch_combine<double>(d_combG3_, (int64_t)h_combG3_.size(), d_combtG3_, d_pool_, s))) return rc;
+  if ((rc = run_launches<double>(gGradPre_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  for (size_t d = 1; d < gGradDepth_.size(); ++d)
+    if ((rc = run_launches<double>(gGradDepth_[d], bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  if ((rc = launch_combine<double>(d_combGE_, (int64_t)h_combGE_.size(), d_combtGE_, d_pool_, s))) return rc;
+  if ((rc = run_launches<double>(gGradFin_, bases, nullptr, 1.0 / Nrm, 0.0, s, d_part_))) return rc;
+  if (grad_host) {
+    QTG_HIP_TRY(hipMemcpyAsync(grad_host, d_grad64_, sizeof(double) * 2 * tree_.total, hipMemcpyDeviceToHost, s));
+    QTG_HIP_TRY(hipStreamSynchronize(s));
+  }
+  return QTG_OK;
 }
 
 template int Engine::build_gemm<double>(hipStream_t);

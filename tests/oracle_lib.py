@@ -60,6 +60,7 @@ def lib():
     L.orc_get_matrix.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_long, dp]
     L.orc_derivative.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double]
     L.orc_apply_all.argtypes = [C.c_void_p, dp, dp]
+    L.orc_gradient.argtypes = [C.c_void_p, dp, dp, C.c_double, C.POINTER(C.c_double)]
     L.orc_energy.argtypes = [C.c_void_p, dp]
     L.orc_energy.restype = C.c_double
     L.orc_dot_product_top.argtypes = [C.c_void_p, dp, dp, dp]
@@ -209,6 +210,14 @@ class OracleModel:
         if lib().orc_derivative(self.h, cview(psi), out, complex(phase).real, complex(phase).imag, eps):
             raise RuntimeError(lib().orc_last_error().decode())
         return as_complex(out).copy()
+
+    def gradient(self, psi, eps=1e-10):
+        """dE/dA* at every node (orthonormal lower nodes) and the energy."""
+        out = np.zeros(2 * self.psi_size)
+        e = C.c_double()
+        if lib().orc_gradient(self.h, cview(psi), out, eps, C.byref(e)):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return as_complex(out).copy(), e.value
 
     def apply_all(self, psi):
         out = np.zeros(2 * self.psi_size)
