@@ -240,6 +240,24 @@ int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend,
                           int64_t max_steps, int64_t max_out, double* times, double* energies,
                           double* autocorr, int64_t* stats);
 
+/* ------------------------------------------------------------------- TTNO */
+/* Tree tensor network operator on an engine's tree (TensorTreeOperator,
+ * include/TreeOperators/TensorOperators/TensorTreeOperator.h:23-58; complex128
+ * engines).  bond[a] = bond dimension of node a (1 at the top); B = the
+ * operator tensors in node-address order, complex128 interleaved: bottom
+ * (dim^2, D_a) with the leaf matrices as columns (toMatrix(B, l),
+ * TensorTreeOperator.cpp:40-47), upper (D_child0, ..., D_childk, D_a).
+ * qtg_ttno_apply: TTOrepresentation::calculate (TTOrepresentation.h:81-103)
+ * with <bra| . |ket>, TTOcontraction::calculate (TTOcontraction.h:58-112),
+ * then apply() (TTOcontraction_Implementation.h:7-30) at every node to the
+ * ket's node tensors into out (engine layout); expect (2 doubles) receives
+ * <bra|H|ket>.  bra, ket, out are host arrays; null out/expect skip them. */
+typedef void* qtg_ttno;
+int qtg_ttno_create(qtg_engine engine, const int64_t* bond, const double* B, qtg_ttno* ttno);
+int qtg_ttno_destroy(qtg_ttno ttno);
+int qtg_ttno_apply(qtg_ttno ttno, const double* bra, const double* ket, double* out, double* expect,
+                   qtg_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1121,6 +1121,111 @@ TensorTree EnergyGradient(const SOP& H, const TensorTree& P, const HamRep& r, co
   return G;
 }
 
+// ---------------------------------------------------------------- TTNO
+// Tree tensor network operator (TensorTreeOperator.h:23-58): one tensor per
+// node of the operator tree, bottom (dim^2, D_v) with the leaf matrices as
+// columns (toMatrix(B, l), TensorTreeOperator.cpp:40-47), upper
+// (D_c1, ..., D_ck, D_v), top (D_c1, ..., 1).
+static vector<size_t> indexMapping(size_t L, const Shape& s) {
+  vector<size_t> ls(s.d.size());
+  for (size_t k = 0; k < s.d.size(); ++k) {
+    ls[k] = L % s.d[k];
+    L /= s.d[k];
+  }
+  return ls;
+}
+
+static Mat leafColumn(const Tensor& B, size_t l, size_t dim) {
+  Mat h(dim, dim);
+  for (size_t I = 0; I < dim * dim; ++I) h.c[I] = B(I, l);
+  return h;
+}
+
+// TTOrepresentation::calculate (TTOrepresentation.h:81-103): bottom-up
+// rep[v][l] = <Psi_v| (operator of bond index l) |Chi_v>.
+vector<vector<Mat>> ttoRepresentation(const TensorTree& Psi, const vector<Tensor>& H, const TensorTree& Chi,
+                                      const Tree& t) {
+  vector<vector<Mat>> rep(t.nodes.size());
+  for (const Node& node : t.nodes) {
+    const int a = node.address;
+    const Tensor& B = H[a];
+    const size_t D = B.s.d.back(), n = node.shape.d.back();
+    rep[a].assign(D, Mat(n, n));
+    if (node.bottom) {
+      for (size_t l = 0; l < D; ++l) {
+        Tensor hChi = matrixTensor(leafColumn(B, l, node.leaf.dim), Chi[a], 0);
+        Mat hij = dotProduct(Psi[a], hChi);
+        for (size_t i = 0; i < hij.c.size(); ++i) rep[a][l].c[i] += hij.c[i];
+      }
+    } else {
+      for (size_t L = 0; L < B.s.total; ++L) {
+        vector<size_t> ls = indexMapping(L, B.s);
+        Tensor A = Chi[a];
+        for (size_t k = 0; k < node.children.size(); ++k) A = matrixTensor(rep[node.children[k]][ls[k]], A, k);
+        for (auto& x : A.c) x *= B[L];
+        Mat hij = dotProduct(Psi[a], A);
+        for (size_t i = 0; i < hij.c.size(); ++i) rep[a][ls.back()].c[i] += hij.c[i];
+      }
+    }
+  }
+  return rep;
+}
+
+// TTOcontraction::calculate (TTOcontraction.h:58-112): top-down
+// con[v][l] = contraction over the legs of the parent but v's of
+// <Psi_p| siblings' rep x B_p x con[p] |Chi_p> with v's bond index l.
+vector<vector<Mat>> ttoContraction(const TensorTree& Psi, const vector<Tensor>& H,
+                                   const vector<vector<Mat>>& rep, const TensorTree& Chi, const Tree& t) {
+  vector<vector<Mat>> con(t.nodes.size());
+  for (int a = (int)t.nodes.size() - 1; a >= 0; --a) {
+    const Node& node = t.nodes[a];
+    if (node.isTop()) continue;
+    const Node& p = t.nodes[node.parent];
+    const Tensor& B = H[p.address];
+    const size_t k = node.childIdx, n = node.shape.d.back();
+    con[a].assign(H[a].s.d.back(), Mat(n, n));
+    for (size_t L = 0; L < B.s.total; ++L) {
+      vector<size_t> ls = indexMapping(L, B.s);
+      Tensor A = Chi[p.address];
+      for (size_t j = 0; j < p.children.size(); ++j) {
+        if (j == k) continue;
+        A = matrixTensor(rep[p.children[j]][ls[j]], A, j);
+      }
+      for (auto& x : A.c) x *= B[L];
+      if (!p.isTop()) A = matrixTensor(con[p.address][ls.back()], A, A.s.lastIdx());
+      Mat hk = contraction(Psi[p.address], A, k);
+      for (size_t i = 0; i < hk.c.size(); ++i) con[a][ls[k]].c[i] += hk.c[i];
+    }
+  }
+  return con;
+}
+
+// apply(Phi, H, rep, con, node) (TTOcontraction_Implementation.h:7-30)
+Tensor ttoApply(const Tensor& Phi, const vector<Tensor>& H, const vector<vector<Mat>>& rep,
+                const vector<vector<Mat>>& con, const Tree& t, const Node& node) {
+  Tensor HPhi(Phi.s);
+  const Tensor& B = H[node.address];
+  const double eps = 1e-12;
+  if (node.bottom) {
+    for (size_t l = 0; l < B.s.d.back(); ++l) {
+      Tensor A = matrixTensor(leafColumn(B, l, node.leaf.dim), Phi, 0);
+      if (!node.isTop()) A = matrixTensor(con[node.address][l], A, A.s.lastIdx());
+      for (size_t i = 0; i < A.c.size(); ++i) HPhi[i] += A[i];
+    }
+  } else {
+    for (size_t L = 0; L < B.s.total; ++L) {
+      if (std::abs(B[L]) < eps) continue;
+      vector<size_t> ls = indexMapping(L, B.s);
+      Tensor A = Phi;
+      for (size_t k = 0; k < node.children.size(); ++k) A = matrixTensor(rep[node.children[k]][ls[k]], A, k);
+      for (auto& x : A.c) x *= B[L];
+      if (!node.isTop()) A = matrixTensor(con[node.address][ls.back()], A, A.s.lastIdx());
+      for (size_t i = 0; i < A.c.size(); ++i) HPhi[i] += A[i];
+    }
+  }
+  return HPhi;
+}
+
 // LayerInterface::Error (LayerInterface.cpp:6-31)
 double LayerError(const Tensor& Phi, const Tensor& Chi, const HamRep& r, const Node& node) {
   double Delta = 0., norm = 1.;
@@ -1704,6 +1809,48 @@ int orc_gradient(void* h, const double* psi, double* grad, double eps, double* e
     TensorTree G = EnergyGradient(m->H, P, m->rep, m->tree, E);
     m->pack(G, grad);
     if (energy) *energy = E;
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+// TTNO application at every node: representation (bra, ket), contraction,
+// then apply() to the ket's node tensors.  bond[a] = D_a (1 at the top);
+// B holds the operator tensors in node-address order, shapes as above.
+// expect receives rep[top][0] = <bra|H|ket> (top with one state).
+int orc_ttno_apply(void* h, const long* bond, const double* B, const double* bra, const double* ket, double* out,
+                   double* expect) {
+  Model* m = (Model*)h;
+  try {
+    const Tree& t = m->tree;
+    vector<Tensor> H;
+    size_t o = 0;
+    for (const Node& n : t.nodes) {
+      vector<size_t> d;
+      if (n.bottom) d.push_back(n.leaf.dim * n.leaf.dim);
+      else
+        for (int c : n.children) d.push_back((size_t)bond[c]);
+      d.push_back((size_t)bond[n.address]);
+      if (n.isTop() && bond[n.address] != 1) throw std::runtime_error("ttno: top bond dimension must be 1");
+      Tensor T{Shape(d)};
+      for (auto& x : T.c) {
+        x = cd(B[2 * o], B[2 * o + 1]);
+        ++o;
+      }
+      H.push_back(T);
+    }
+    TensorTree Psi = m->unpack(bra), Chi = m->unpack(ket);
+    auto rep = ttoRepresentation(Psi, H, Chi, t);
+    auto con = ttoContraction(Psi, H, rep, Chi, t);
+    TensorTree D = makeTensorTree(t);
+    for (const Node& n : t.nodes) D[n.address] = ttoApply(Chi[n.address], H, rep, con, t, n);
+    m->pack(D, out);
+    if (expect) {
+      const Mat& e = rep[t.top()][0];
+      expect[0] = e.c[0].real();
+      expect[1] = e.c[0].imag();
+    }
     return 0;
   } catch (std::exception& e) {
     g_err = e.what();

@@ -77,6 +77,9 @@ def lib():
     L.qtg_engine_counts.argtypes = [_vp, C.POINTER(C.c_double), _lp]
     L.qtg_engine_plan_flops.argtypes = [_vp, C.POINTER(C.c_double)]
     L.qtg_engine_gradient.argtypes = [_vp, _dp, C.POINTER(C.c_double), _vp]
+    L.qtg_ttno_create.argtypes = [_vp, _lp, _dp, C.POINTER(_vp)]
+    L.qtg_ttno_destroy.argtypes = [_vp]
+    L.qtg_ttno_apply.argtypes = [_vp, _dp, _dp, _dp, _dp, _vp]
     L.qtg_engine_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
     L.qtg_engine_set_timing.argtypes = [_vp, C.c_int]
     L.qtg_engine_synchronize.argtypes = [_vp]
@@ -357,5 +360,31 @@ class CMF:
             if self.ptr:
                 lib().qtg_cmf_destroy(self.ptr)
                 self.ptr = None
+        except Exception:
+            pass
+
+
+class TTNO:
+    """Tree tensor network operator on an engine's tree (qtg_ttno_*).
+    bond: D per node address (1 at the top); B: the operator tensors
+    concatenated in node-address order (complex128)."""
+
+    def __init__(self, engine: Engine, bond, B):
+        self.engine = engine  # keeps the engine alive
+        p = _vp()
+        check(lib().qtg_ttno_create(engine.ptr, np.ascontiguousarray(bond, np.int64), _ci(B), C.byref(p)))
+        self.ptr = p.value
+
+    def apply(self, bra: np.ndarray, ket: np.ndarray):
+        """(TTNO applied at every node to ket's node tensors, <bra|H|ket>)."""
+        out = np.zeros(2 * self.engine.psi_size)
+        e = np.zeros(2)
+        check(lib().qtg_ttno_apply(self.ptr, _ci(bra), _ci(ket), out, e, None))
+        return out.view(np.complex128).copy(), complex(e[0], e[1])
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qtg_ttno_destroy(self.ptr)
         except Exception:
             pass

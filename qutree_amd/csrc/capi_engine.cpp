@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "cmf.hpp"
+#include "ttno.hpp"
 #include "engine.hpp"
 
 using namespace qtg;
@@ -276,6 +277,30 @@ int64_t qtg_cmf_integrate(qtg_cmf c, double* psi_host, double t0, double tend, d
     stats[4] = (int64_t)r.ticks;
   }
   return (int64_t)r.times.size();
+}
+
+// ----------------------------------------------------------------------- TTNO
+int qtg_ttno_create(qtg_engine e, const int64_t* bond, const double* B, qtg_ttno* ttno) {
+  if (!e || !bond || !B || !ttno) return QTG_EARG;
+  auto* t = new Ttno(*static_cast<Engine*>(e));
+  int rc = t->create(bond, B);
+  if (rc) {
+    delete t;
+    return rc;
+  }
+  *ttno = t;
+  return QTG_OK;
+}
+
+int qtg_ttno_destroy(qtg_ttno ttno) {
+  delete static_cast<Ttno*>(ttno);
+  return QTG_OK;
+}
+
+int qtg_ttno_apply(qtg_ttno ttno, const double* bra, const double* ket, double* out, double* expect,
+                   qtg_stream stream) {
+  if (!ttno || !bra || !ket) return QTG_EARG;
+  return static_cast<Ttno*>(ttno)->apply(bra, ket, out, expect, S(stream));
 }
 
 }  // extern "C"

@@ -61,6 +61,7 @@ def lib():
     L.orc_derivative.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double]
     L.orc_apply_all.argtypes = [C.c_void_p, dp, dp]
     L.orc_gradient.argtypes = [C.c_void_p, dp, dp, C.c_double, C.POINTER(C.c_double)]
+    L.orc_ttno_apply.argtypes = [C.c_void_p, lp, dp, dp, dp, dp, dp]
     L.orc_energy.argtypes = [C.c_void_p, dp]
     L.orc_energy.restype = C.c_double
     L.orc_dot_product_top.argtypes = [C.c_void_p, dp, dp, dp]
@@ -218,6 +219,17 @@ class OracleModel:
         if lib().orc_gradient(self.h, cview(psi), out, eps, C.byref(e)):
             raise RuntimeError(lib().orc_last_error().decode())
         return as_complex(out).copy(), e.value
+
+    def ttno_apply(self, bond, B, bra, ket):
+        """TTNO application at every node to the ket's node tensors (rep with
+        bra, ket) and <bra|H|ket>.  bond: D per node address; B: the operator
+        tensors concatenated in address order (complex)."""
+        out = np.zeros(2 * self.psi_size)
+        e = np.zeros(2)
+        if lib().orc_ttno_apply(self.h, np.ascontiguousarray(bond, np.int64), cview(np.ascontiguousarray(B)),
+                                cview(bra), cview(ket), out, e):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return as_complex(out).copy(), complex(e[0], e[1])
 
     def apply_all(self, psi):
         out = np.zeros(2 * self.psi_size)
