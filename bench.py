@@ -65,6 +65,114 @@ def time_form(Q, text, H, psi, precision, compress, graph, steps, path=1):
             "executed_flops": flops}
 
 
+def gradient_leg(Q, W, gd_steps):
+    """Energy gradient dE/dA* on the 32-mode anharmonic model (8 DVR, SPF 8
+    then 24): gradient evaluations per second (sweep + mean-field pass +
+    gradient assembly + copy to the host) and plain gradient descent from a
+    random state with QR re-orthonormalisation on the host."""
+    from qutree_amd.gauge import gradient_descent
+
+    text, H = W.grad32_model()
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(32, delta_lowest=False)
+    eng.set_psi(psi)
+    eng.gradient()
+    n = 10
+    t0 = time.perf_counter()
+    for _ in range(n):
+        eng.gradient()
+    t1 = time.perf_counter()
+    t2 = time.perf_counter()
+    _, energies = gradient_descent(eng, psi, gd_steps, 0.05)
+    t3 = time.perf_counter()
+    return {"workload": "32-mode quartic model, 8 DVR, SPF 8 then 24 (24x24x24 nodes), 32 + 496 terms",
+            "gradient_evals_per_s": round(n / (t1 - t0), 3), "ms_per_gradient": round((t1 - t0) / n * 1e3, 3),
+            "gd_steps": gd_steps, "gd_eta": 0.05, "gd_seconds": round(t3 - t2, 3),
+            "energy_start": energies[0], "energy_end": energies[-1],
+            "monotone": bool(all(b < a for a, b in zip(energies, energies[1:])))}
+
+
+def big_tree_leg(Q, W, steps=3):
+    """One H|Psi> sweep on the 128-mode tree (8 DVR, SPF 8..48 toward the root,
+    128 + 8128 coupled-quartic terms) in complex128 and complex64 bulk."""
+    text, H = W.cmf128_model()
+    res = {"workload": "128-mode binary tree, 8 DVR, SPF 8,14,21,27,34,41,48, 128 + 8128 terms"}
+    e64 = None
+    psi = None
+    for prec, name in ((0, "complex128"), (1, "complex64_bulk")):
+        eng = Q.Engine(text, H, precision=prec)
+        if psi is None:
+            psi = eng.fill_random(128, delta_lowest=False)
+        eng.set_psi(psi)
+        eng.derivative(sync=False)
+        Q.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.derivative(sync=False)
+        Q.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        e = eng.energy()
+        r = {"sweeps_per_s": round(1.0 / dt, 3), "ms_per_sweep": round(dt * 1e3, 3), "energy": e,
+             "executed_flops": eng.executed_flops(), "tflops_per_s_executed": round(eng.executed_flops() / dt / 1e12, 3)}
+        if e64 is None:
+            e64 = e
+            r["reference_flops"] = eng.counts()[0]
+        else:
+            r["energy_rel_err_vs_complex128"] = abs(e - e64) / abs(e64)
+        res[name] = r
+        del eng
+    return res
+
+
+def microbench_leg(Q, W, reps=5):
+    """Node-level grouped kernels on 4096 order-3 tensors with legs from
+    {8,16,24,32,48,64}: matrixTensor on a random mode and the hole
+    contraction(A, B, k), complex128, with their rooflines."""
+    import numpy as np
+
+    shapes, modes = W.microbench_shapes()
+    count = len(modes)
+    sizes = np.prod(shapes, axis=1)
+    offT = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    nk = shapes[np.arange(count), modes]
+    offM = np.concatenate([[0], np.cumsum(nk * nk)[:-1]]).astype(np.int64)
+    tot, totM = int(sizes.sum()), int((nk * nk).sum())
+    mp = np.zeros(count, Q.MP_DESC)
+    hc = np.zeros(count, Q.HC_DESC)
+    for t in range(count):
+        d, k = shapes[t], modes[t]
+        before, after = int(np.prod(d[:k])), int(np.prod(d[k + 1:]))
+        mp[t] = (offT[t], offM[t], offT[t], before, d[k], d[k], after, 0, 0)
+        hc[t] = (offM[t], offT[t], offT[t], before, d[k], d[k], after, 0, 0)
+    # synthetic values (timing only; parity is covered by the test suite)
+    rng = np.random.default_rng(4096)
+    block = rng.standard_normal(2 * 4096)
+    fill = np.resize(block, 2 * tot).view(np.complex128)
+    dA = Q.DeviceBuffer.from_array(fill)
+    dB = Q.DeviceBuffer.from_array(fill)
+    dC = Q.DeviceBuffer(16 * tot)
+    dM = Q.DeviceBuffer.from_array(np.resize(block, 2 * totM).view(np.complex128))
+    dS = Q.DeviceBuffer(16 * totM)
+    out = {"workload": f"{count} order-3 tensors, legs from {{8,16,24,32,48,64}}, complex128",
+           "elements": tot}
+    for name, plan, args in (("matrixTensor", Q.Plan("mode_product", mp), (dC, dM, dA)),
+                             ("contraction", Q.Plan("hole", hc), (dS, dA, dB))):
+        plan.execute(*args)
+        Q.synchronize()
+        tm = Q.Timer()
+        tm.start()
+        for _ in range(reps):
+            plan.execute(*args)
+        ms = tm.stop() / reps
+        fl, by = plan.counts()
+        tf, gbs = fl / (ms * 1e-3) / 1e12, by / (ms * 1e-3) / 1e9
+        t_roof = max(by / 8e12, fl / 78.6e12)
+        out[name] = {"ms": round(ms, 4), "tflops_per_s": round(tf, 3), "gb_per_s": round(gbs, 1),
+                     "flops": fl, "bytes": by, "roofline_frac": round(t_roof / (ms * 1e-3), 4),
+                     "bound": "hbm" if by / 8e12 > fl / 78.6e12 else "mfma"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +183,9 @@ def main() -> int:
     ap.add_argument("--cmf-steps", type=int, default=6, help="CMF steps timed on the 64-mode tree (0: skip)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip timing the plain per-term complex128 form and the complex64 bulk form")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the gradient, 128-mode and node-microbench legs")
+    ap.add_argument("--gd-steps", type=int, default=200, help="gradient-descent steps of the gradient leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -193,6 +304,10 @@ def main() -> int:
                       "rejected": r["rejected"], "builds": r["builds"], "node_derivatives": r["derivs"],
                       "ticks": r["ticks"], "ms_per_step": round((c1 - c0) / r["steps"] * 1e3, 3),
                       "workload": "64-mode tree, displaced Hartree start (q0 = 1), dt = 0.1, cmf 1e-4, bs 1e-5"}
+    if not args.no_extras and args.precision == "complex128":
+        out["gradient"] = gradient_leg(Q, W, args.gd_steps)
+        out["tree128"] = big_tree_leg(Q, W)
+        out["node_microbench"] = microbench_leg(Q, W)
     if not args.no_cpu:
         base, e_ref = cpu_baseline(text, H, psi, energy)
         out["cpu_baseline"] = base

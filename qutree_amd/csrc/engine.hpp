@@ -37,6 +37,8 @@ class Engine {
   const HSop& sop() const { return sop_; }
   double sweep_flops() const { return flops_; }        // executed by this plan
   double reference_flops() const { return refFlops_; }  // of the reference algorithm
+  int64_t work_elems() const { return workSize_; }       // device workspace (complex elements)
+  int64_t pool_elems() const { return poolSize_; }       // matrix pool (complex elements)
   double* psi64() { return d_psi64_; }
   double* dpsi64() { return d_dpsi64_; }
   double* apply64() { return d_apply64_; }

@@ -1,46 +1,2 @@
-"""Gauge helpers for tree states on the host (numpy): orthonormalise a node by
-QR and push R into the parent, which leaves the represented state unchanged.
-Used to evaluate the energy of perturbed states, whose perturbed node is no
-longer orthonormal, in the gauge every energy routine here assumes."""
-import numpy as np
-
-
-def node_view(psi, node):
-    dims = node["dims"]
-    N = int(np.prod(dims))
-    return psi[node["off"]:node["off"] + N].reshape(dims, order="F")
-
-
-def push_up(psi, nodes, a):
-    """QR of node a (rows: all indices but the last), R into the parent on
-    a's leg.  Returns the parent address (or -1 at the top)."""
-    n = nodes[a]
-    p = n["parent"]
-    if p < 0:
-        return -1
-    dims = n["dims"]
-    N, k = int(np.prod(dims)), dims[-1]
-    A = psi[n["off"]:n["off"] + N].reshape((N // k, k), order="F")
-    Q, R = np.linalg.qr(A)
-    psi[n["off"]:n["off"] + N] = Q.reshape(-1, order="F")
-    P = node_view(psi, nodes[p]).copy()
-    leg = n["childIdx"]
-    P = np.moveaxis(np.tensordot(R, P, axes=([1], [leg])), 0, leg)
-    pn = nodes[p]
-    psi[pn["off"]:pn["off"] + int(np.prod(pn["dims"]))] = P.reshape(-1, order="F")
-    return p
-
-
-def canonicalize_path(psi, nodes, a):
-    """Orthonormalise node a and its ancestors below the top (in place)."""
-    while a >= 0 and nodes[a]["parent"] >= 0:
-        a = push_up(psi, nodes, a)
-    return psi
-
-
-def canonicalize_all(psi, nodes):
-    """Orthonormalise every non-top node, children before parents (in place)."""
-    for a in range(len(nodes)):
-        if nodes[a]["parent"] >= 0:
-            push_up(psi, nodes, a)
-    return psi
+"""Re-export of the package's host gauge helpers for the tests."""
+from qutree_amd.gauge import canonicalize_all, canonicalize_path, node_view, push_up  # noqa: F401
