@@ -277,33 +277,55 @@ def main() -> int:
         plain = time_form(Q, text, H, psi, 0, False, False, max(3, args.steps // 10), path=0)
         lds = time_form(Q, text, H, psi, 0, True, True, args.steps, path=0)
         c64 = time_form(Q, text, H, psi, 1, True, True, args.steps)
-        for f in (plain, lds, c64):
+        split = time_form(Q, text, H, psi, 2, True, True, args.steps)
+        for f in (plain, lds, c64, split):
             f["energy_rel_err_vs_complex128"] = abs(f["energy"] - energy) / abs(energy)
         plain["form"] = "complex128, per-term work items, whole node tensors in LDS, eager launches"
         lds["form"] = "complex128, shared slots, whole node tensors in LDS, HIP graph"
         c64["form"] = ("complex64 bulk MFMA contractions (FP64 rho^-1, projector, output), shared slots, "
                        "HIP graph")
+        split["form"] = ("complex64 bulk MFMA contractions, FP64 for the large-coefficient terms and the "
+                         "densities (and rho^-1, projector, output), shared slots, HIP graph")
         out["forms"] = {"plain_complex128": plain, "lds_complex128": lds, "complex64_bulk": c64,
+                        "complex64_split": split,
                         "speedup_vs_plain": round(value / plain["sweeps_per_s"], 3),
-                        "complex64_speedup_vs_plain": round(c64["sweeps_per_s"] / plain["sweeps_per_s"], 3)}
+                        "complex64_speedup_vs_plain": round(c64["sweeps_per_s"] / plain["sweeps_per_s"], 3),
+                        "complex64_split_speedup_vs_plain": round(split["sweeps_per_s"] / plain["sweeps_per_s"], 3)}
     if args.cmf_steps > 0 and args.precision == "complex128":
         # CMF steps (CMFIntegrator::Integrate iterations: 5 half/quarter
         # substeps of per-node Bulirsch-Stoer, 2-3 builds, error control,
         # orthogonal) on the same tree from a displaced Hartree product
         text_c, H_c = W.cmf64_model()
-        engc = Q.Engine(text_c, H_c, graph=False)
-        psi_c = engc.fill_random(64, delta_lowest=True)
-        cmf = Q.CMF(engc)
-        cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=1)  # warm-up
-        Q.synchronize()
-        c0 = time.perf_counter()
-        r = cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=args.cmf_steps)
-        Q.synchronize()
-        c1 = time.perf_counter()
-        out["cmf"] = {"steps_per_s": round(r["steps"] / (c1 - c0), 3), "steps": r["steps"],
-                      "rejected": r["rejected"], "builds": r["builds"], "node_derivatives": r["derivs"],
-                      "ticks": r["ticks"], "ms_per_step": round((c1 - c0) / r["steps"] * 1e3, 3),
-                      "workload": "64-mode tree, displaced Hartree start (q0 = 1), dt = 0.1, cmf 1e-4, bs 1e-5"}
+
+        def cmf_form(precision, path, compress):
+            engc = Q.Engine(text_c, H_c, precision=precision, compress=compress, graph=False, path=path)
+            psi_c = engc.fill_random(64, delta_lowest=True)
+            cmf = Q.CMF(engc)
+            cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=1)  # warm-up
+            Q.synchronize()
+            c0 = time.perf_counter()
+            r = cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=args.cmf_steps)
+            Q.synchronize()
+            c1 = time.perf_counter()
+            return {"steps_per_s": round(r["steps"] / (c1 - c0), 3), "steps": r["steps"],
+                    "rejected": r["rejected"], "builds": r["builds"], "node_derivatives": r["derivs"],
+                    "ticks": r["ticks"], "ms_per_step": round((c1 - c0) / r["steps"] * 1e3, 3),
+                    "energy_final": float(r["energies"][-1]) if len(r["energies"]) else None}, r
+
+        main_cmf, r64 = cmf_form(0, 1, True)
+        main_cmf["workload"] = "64-mode tree, displaced Hartree start (q0 = 1), dt = 0.1, cmf 1e-4, bs 1e-5"
+        out["cmf"] = main_cmf
+        if not args.no_forms:
+            plain_cmf, _ = cmf_form(0, 0, False)
+            split_cmf, rs = cmf_form(2, 1, True)
+            e_ref = np.asarray(r64["energies"])
+            n = min(len(e_ref), len(rs["energies"]))
+            split_cmf["energy_rel_err_vs_complex128"] = (
+                float(np.max(np.abs(np.asarray(rs["energies"][:n]) - e_ref[:n]) / np.abs(e_ref[:n]))) if n else None)
+            out["cmf_forms"] = {"plain_complex128": plain_cmf, "complex64_split": split_cmf,
+                                "speedup_vs_plain": round(main_cmf["steps_per_s"] / plain_cmf["steps_per_s"], 3),
+                                "complex64_split_speedup_vs_plain":
+                                    round(split_cmf["steps_per_s"] / plain_cmf["steps_per_s"], 3)}
     if not args.no_extras and args.precision == "complex128":
         out["gradient"] = gradient_leg(Q, W, args.gd_steps)
         out["tree128"] = big_tree_leg(Q, W)

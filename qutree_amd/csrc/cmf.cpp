@@ -40,8 +40,8 @@ int Cmf::dalloc(void** p, size_t bytes) {
 }
 
 int Cmf::init() {
-  if (E.c64_) {
-    set_error("cmf: the integrator runs on a complex128 engine");
+  if (E.c64_ && E.path_ != 1) {
+    set_error("cmf: complex64 engines integrate on the grouped-contraction path");
     return QTG_ESTATE;
   }
   if (!E.planned_) {

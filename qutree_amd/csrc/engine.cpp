@@ -70,11 +70,12 @@ int Engine::create(const std::string& tree_text, int precision, bool host_only) 
     set_error(std::string("tree: ") + e.what());
     return QTG_ESHAPE;
   }
-  if (precision != 0 && precision != 1) {
-    set_error("precision must be 0 (complex128) or 1 (complex64 bulk)");
+  if (precision < 0 || precision > 2) {
+    set_error("precision must be 0 (complex128), 1 (complex64 bulk) or 2 (complex64 bulk, FP64 large terms)");
     return QTG_EARG;
   }
-  c64_ = precision == 1;
+  c64_ = precision != 0;
+  precision_ = precision;
   nmax_ = 256;
   nspf_ = 1;
   for (auto& n : tree_.nodes) {
@@ -219,11 +220,19 @@ int Engine::plan() {
   std::vector<std::vector<std::pair<int64_t, int>>> hent(NN), qent(NN);
   typedef std::vector<std::pair<int, int>> Key;
   std::vector<std::map<Key, int64_t>> hkey(NN), qkey(NN);
+  // large-coefficient terms (|c| >= 0.1 max |c|): their slots run in FP64 in
+  // the split-precision form
+  double cmax = 0.0;
+  for (auto& tm : sop_.terms) cmax = std::max(cmax, std::abs(tm.coeff));
+  std::vector<char> big(L, 0);
+  for (int l = 0; l < L; ++l) big[l] = std::abs(sop_.terms[l].coeff) >= 0.1 * cmax;
+  highSlots_.clear();
   for (int a = 0; a < NN; ++a) {
     const HNode& v = tree_.nodes[a];
     if (v.top()) continue;
     // density matrix = hole of the empty outside restriction
     rhoslot_[a] = off;
+    highSlots_.insert(off);
     if (compress_) qkey[a][Key()] = off;
     qent[a].push_back({off, -1});
     off += nn(a);
@@ -240,6 +249,7 @@ int Engine::plan() {
         } else {
           hslot_[l][a] = hi->second;
         }
+        if (big[l]) highSlots_.insert(hslot_[l][a]);
         auto qi = qkey[a].find(ko);
         if (qi == qkey[a].end()) {
           qkey[a][ko] = off;
@@ -249,6 +259,7 @@ int Engine::plan() {
         } else {
           holeslot_[l][a] = qi->second;
         }
+        if (big[l]) highSlots_.insert(holeslot_[l][a]);
       } else {
         hslot_[l][a] = off;
         hent[a].push_back({off, l});
@@ -256,6 +267,10 @@ int Engine::plan() {
         holeslot_[l][a] = off;
         qent[a].push_back({off, l});
         off += nn(a);
+        if (big[l]) {
+          highSlots_.insert(hslot_[l][a]);
+          highSlots_.insert(holeslot_[l][a]);
+        }
       }
     }
   }
@@ -489,6 +504,10 @@ int Engine::plan() {
   lflops_ = flops_;
   if (path_ == 0) {
     const size_t es = c64_ ? 8 : 16;
+    if (precision_ == 2) {
+      set_error("engine: the split-precision form runs on the grouped-contraction path");
+      return QTG_EARG;
+    }
     if (work_lds_bytes(nmax_, es) > kLdsBudget || finish_lds_bytes(nmax_, nspf_) > kLdsBudget || nspf_ > 64) {
       set_error("engine: node tensors of " + std::to_string(nmax_) + " elements / " + std::to_string(nspf_) +
                 " SPFs exceed the LDS-resident kernels; use the grouped-contraction path");
@@ -555,8 +574,10 @@ int Engine::plan() {
     if ((rc = alloc(&d_part_, 16 * std::max<int64_t>(partSize_, 1)))) return rc;
     if ((rc = alloc(&d_partRho_, 16 * std::max<int64_t>(partSize_, 1)))) return rc;
   }
-  const size_t es = c64_ ? 8 : 16;
-  if (path_ == 1 && (rc = alloc(&d_work_, es * std::max<int64_t>(workSize_, 1)))) return rc;
+  // the grouped path keeps every matrix in FP64 (FP32 kernels round on load)
+  const size_t es = (c64_ && path_ == 0) ? 8 : 16;
+  if (path_ == 1 && (rc = alloc(&d_work_, 16 * std::max<int64_t>(workSize_, 1)))) return rc;
+  if (path_ == 1 && c64_ && (rc = alloc(&d_work32_, 8 * std::max<int64_t>(work32Size_, 1)))) return rc;
   if ((rc = alloc(&d_pool_, es * poolSize_))) return rc;
   if ((rc = alloc((void**)&d_pool64_, 16 * pool64Size_))) return rc;
   if ((rc = alloc(&d_partial_, es * (path_ == 1 ? 1 : partialSize_)))) return rc;
@@ -569,7 +590,7 @@ int Engine::plan() {
     }
   auto put = [&](int64_t off, const std::vector<double>& v) -> int {
     if (v.empty()) return QTG_OK;
-    if (c64_) {
+    if (es == 8) {
       std::vector<float> lf(v.begin(), v.end());
       QTG_HIP_TRY(hipMemcpy((char*)d_pool_ + 8 * off, lf.data(), sizeof(float) * lf.size(), hipMemcpyHostToDevice));
     } else {
@@ -637,6 +658,8 @@ std::vector<Engine::Summand> Engine::aggregate(const std::vector<Summand>& tl, c
       }
       co.t1 = (int32_t)h_combt1_.size();
       h_comb1_.push_back(co);
+      for (int32_t q = co.t0; q < co.t1; ++q)
+        if (h_combt1_[q].src >= 0 && highSlots_.count(h_combt1_[q].src)) highSlots_.insert(off);
       parts.push_back(off);
       off += d * d;
     }
@@ -657,6 +680,8 @@ std::vector<Engine::Summand> Engine::aggregate(const std::vector<Summand>& tl, c
       co.t1 = (int32_t)h_combt2_.size();
       h_comb2_.push_back(co);
       combFlops += 8.0 * d * d * parts.size();
+      for (int64_t pidx : parts)
+        if (highSlots_.count(pidx)) highSlots_.insert(off);
       result = off;
       off += d * d;
     }
@@ -738,6 +763,13 @@ int Engine::read_timing(double* ms, int64_t* launches, double* flops) {
   return QTG_OK;
 }
 
+int launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return QTG_OK;
+  hipLaunchKernelGGL(to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
 int Engine::convert_psi(hipStream_t s) {
   const int64_t n = 2 * tree_.total;
   int rc = begin_mark(4, s);
@@ -789,7 +821,7 @@ int Engine::build(hipStream_t s) {
     set_error("engine: no Hamiltonian");
     return QTG_ESTATE;
   }
-  if (path_ == 1) return c64_ ? build_gemm<float>(s) : build_gemm<double>(s);
+  if (path_ == 1) return build_gemm(s);
   return c64_ ? build_t<float>(s) : build_t<double>(s);
 }
 
@@ -815,9 +847,7 @@ int Engine::layer_derivative_all(hipStream_t s, double ph_re, double ph_im) {
     return QTG_ESTATE;
   }
   if (path_ == 1) {
-    const void* psiT = c64_ ? (const void*)d_psi32_ : (const void*)d_psi64_;
-    return c64_ ? apply_gemm<float>(s, psiT, d_psi64_, d_dpsi64_, nullptr, ph_re, ph_im)
-                : apply_gemm<double>(s, psiT, d_psi64_, d_dpsi64_, nullptr, ph_re, ph_im);
+    return apply_gemm(s, d_psi64_, d_dpsi64_, nullptr, ph_re, ph_im);
   }
   return c64_ ? apply_t<float>(s, ph_re, ph_im) : apply_t<double>(s, ph_re, ph_im);
 }
@@ -887,7 +917,7 @@ int Engine::get_matrix(int which, int term, int node, double* out, hipStream_t s
   }
   if (off < 0) return QTG_EARG;
   if (heevPending_) QTG_HIP_TRY(hipStreamWaitEvent(s, evHeev_, 0));  // rho^-1 of the side stream
-  if (from64 || !c64_) {
+  if (from64 || !c64_ || path_ == 1) {
     const void* base = from64 ? (const void*)d_pool64_ : d_pool_;
     QTG_HIP_TRY(hipMemcpyAsync(out, (const char*)base + 16 * off, 16 * (size_t)n * n, hipMemcpyDeviceToHost, s));
     QTG_HIP_TRY(hipStreamSynchronize(s));

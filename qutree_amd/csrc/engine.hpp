@@ -1,6 +1,7 @@
 // Device-resident tree engine (see engine.cpp).
 #pragma once
 #include <map>
+#include <set>
 #include <string>
 #include <utility>
 #include <vector>
@@ -13,6 +14,9 @@
 namespace qtg {
 
 class Cmf;
+
+// out[i] = (float)in[i] for n doubles
+int launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 
 class Engine {
   friend class Cmf;
@@ -87,16 +91,18 @@ class Engine {
   int plan();
   int plan_gemm();
   int convert_psi(hipStream_t s);  // FP64 state -> FP32 copy (complex64 engines)
-  template <typename T>
   int build_gemm(hipStream_t s);
-  template <typename T>
-  int apply_gemm(hipStream_t s, const void* psiT, const double* psi64, double* dpsi64, const uint8_t* active,
-                 double ph_re, double ph_im);
-  // one launch of the grouped path
-  enum Buf { B_PSI = 0, B_WORK, B_POOL, B_PSI64, B_APPLY64, B_DPSI64, B_POOL64, B_COUNT };
+  int apply_gemm(hipStream_t s, const double* psi64, double* dpsi64, const uint8_t* active, double ph_re,
+                 double ph_im);
+  // one launch of the grouped path; matrices always live in the FP64 pool
+  enum Buf { B_PSI32 = 0, B_WORK32, B_POOL, B_PSI64, B_APPLY64, B_DPSI64, B_POOL64, B_WORK64, B_COUNT };
+  // P64: FP64 operands and outputs; P32: FP32 tensors (matrices rounded on
+  // load), FP32 mode-product outputs, FP64 contraction outputs; P32W: FP32
+  // mode product writing FP64 (the stacked Apply sum)
+  enum Prec { P64 = 0, P32 = 1, P32W = 2 };
   struct GLaunch {
     int kind;    // 0 mode products, 1 hole contractions, 2 solve
-    int prec;    // 0: engine precision, 1: engine precision in, FP64 out, 2: FP64
+    int prec;    // Prec
     int64_t d0, nd, ntasks;
     int64_t s0, ns;    // split descriptors of this launch (reduce pass)
     int64_t nrb;       // reduce blocks
@@ -105,7 +111,6 @@ class Engine {
     bool scaled;       // multiply alphas by the launch scale (-i phase)
     double flops;
   };
-  template <typename T>
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
                    hipStream_t s, void* partbuf);
   int materialize();
@@ -119,6 +124,7 @@ class Engine {
   HTree tree_;
   HSop sop_;
   bool c64_ = false, planned_ = false, built_ = false, host_only_ = false;
+  int precision_ = 0;  // 0 complex128, 1 complex64 bulk, 2 complex64 bulk with FP64 for large terms
   std::vector<SwItem> h_items_;
   std::vector<EigItem> h_eig_;
   std::vector<ApTerm> h_terms_;
@@ -165,7 +171,11 @@ class Engine {
   std::vector<std::pair<int64_t, int>> idSlots_;  // identity matrices in the pool (offset, n)
   std::vector<CombOut> h_comb3_;
   std::vector<CombTerm> h_combt3_;
-  int64_t workSize_ = 0, energySlot_ = -1, nComb3_ = 0;
+  int64_t workSize_ = 0, work32Size_ = 0, energySlot_ = -1, nComb3_ = 0;
+  void* d_work32_ = nullptr;
+  // pool slots computed in FP64 by the split-precision form: those serving a
+  // large-coefficient term, and the densities
+  std::set<int64_t> highSlots_;
   GmDesc* d_gm_ = nullptr;
   GcDesc* d_gc_ = nullptr;
   SolveItem* d_solve_ = nullptr;

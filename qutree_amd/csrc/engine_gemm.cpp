@@ -197,32 +197,42 @@ int Engine::plan_gemm() {
     if (it.k >= (int)p.children.size() || p.bottom) return false;
     return it.out == rhoslot_[p.children[it.k]];
   };
-  int64_t wbase = 0, wmax = 0;
+  // Each item runs in FP64 (complex128 engines; in the split form the items
+  // whose output serves a large-coefficient term, and the densities) or in
+  // FP32 (the complex64 bulk); equal Z tensors are shared per precision.
+  auto item_hi = [&](int64_t out) {
+    return precision_ == 0 || (precision_ == 2 && highSlots_.count(out) > 0);
+  };
+  int64_t wbase = 0, wbase32 = 0, wmax = 0, wmax32 = 0;
   auto sandwich_layer = [&](const std::pair<int64_t, int64_t>& r, int rho, std::vector<GLaunch>& L) {
     if (r.second == r.first) return;
-    std::map<std::tuple<int, int, int64_t>, int64_t> zbra;             // (node, leg, mat) -> work offset
-    std::map<std::vector<int64_t>, int64_t> zket1, zket2;              // (node, leg, mat[, leg, mat])
-    std::vector<GmDesc> lv0, lv1;
-    std::vector<GcDesc> con;
-    int64_t w = wbase;
+    struct Side {
+      std::map<std::tuple<int, int, int64_t>, int64_t> zbra;  // (node, leg, mat) -> work offset
+      std::map<std::vector<int64_t>, int64_t> zket1, zket2;   // (node, leg, mat[, leg, mat])
+      std::vector<GmDesc> lv0, lv1;
+      std::vector<GcDesc> con;
+      int64_t w = 0;
+    } sd[2];  // [0] FP32, [1] FP64
+    sd[0].w = wbase32;
+    sd[1].w = wbase;
     for (int64_t i = r.first; i < r.second; ++i) {
       const SwItem& it = h_items_[i];
       if (rho >= 0 && (int)is_rho(it) != rho) continue;
+      Side& S = sd[item_hi(it.out) ? 1 : 0];
       const HNode& p = tree_.nodes[it.node];
       const int k = it.k;
       const int braleg = (k == 0) ? 1 : 0;
-      // bra tensor
       int64_t xoff = p.off;
       int xsel = 0;
       std::vector<std::pair<int, int64_t>> ket;
       for (int f = 0; f < it.nf; ++f) {
         if (it.fmode[f] == braleg) {
           auto key = std::make_tuple((int)it.node, braleg, it.fmat[f]);
-          auto z = zbra.find(key);
-          if (z == zbra.end()) {
-            lv0.push_back(gm_desc(p, braleg, p.off, 0, it.fmat[f], w, GM_H, it.node));
-            z = zbra.emplace(key, w).first;
-            w += p.N;
+          auto z = S.zbra.find(key);
+          if (z == S.zbra.end()) {
+            S.lv0.push_back(gm_desc(p, braleg, p.off, 0, it.fmat[f], S.w, GM_H, it.node));
+            z = S.zbra.emplace(key, S.w).first;
+            S.w += p.N;
           }
           xoff = z->second;
           xsel = 1;
@@ -235,11 +245,11 @@ int Engine::plan_gemm() {
       int ysel = 0;
       if (!ket.empty()) {
         std::vector<int64_t> key1 = {it.node, ket[0].first, ket[0].second};
-        auto z = zket1.find(key1);
-        if (z == zket1.end()) {
-          lv0.push_back(gm_desc(p, ket[0].first, p.off, 0, ket[0].second, w, GM_N, it.node));
-          z = zket1.emplace(key1, w).first;
-          w += p.N;
+        auto z = S.zket1.find(key1);
+        if (z == S.zket1.end()) {
+          S.lv0.push_back(gm_desc(p, ket[0].first, p.off, 0, ket[0].second, S.w, GM_N, it.node));
+          z = S.zket1.emplace(key1, S.w).first;
+          S.w += p.N;
         }
         yoff = z->second;
         ysel = 1;
@@ -247,11 +257,11 @@ int Engine::plan_gemm() {
           std::vector<int64_t> key2 = key1;
           key2.push_back(ket[1].first);
           key2.push_back(ket[1].second);
-          auto z2 = zket2.find(key2);
-          if (z2 == zket2.end()) {
-            lv1.push_back(gm_desc(p, ket[1].first, yoff, 1, ket[1].second, w, GM_N, it.node));
-            z2 = zket2.emplace(key2, w).first;
-            w += p.N;
+          auto z2 = S.zket2.find(key2);
+          if (z2 == S.zket2.end()) {
+            S.lv1.push_back(gm_desc(p, ket[1].first, yoff, 1, ket[1].second, S.w, GM_N, it.node));
+            z2 = S.zket2.emplace(key2, S.w).first;
+            S.w += p.N;
           }
           yoff = z2->second;
         }
@@ -270,12 +280,16 @@ int Engine::plan_gemm() {
       g.node = it.node;
       g.ksplit = 1;
       g.sel = xsel | (ysel << 1);
-      con.push_back(g);
+      S.con.push_back(g);
     }
-    wmax = std::max(wmax, w);
-    push_gm(L, lv0, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
-    push_gm(L, lv1, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
-    push_gc(L, con, 0, B_PSI, B_WORK, B_POOL);
+    wmax32 = std::max(wmax32, sd[0].w);
+    wmax = std::max(wmax, sd[1].w);
+    push_gm(L, sd[1].lv0, P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+    push_gm(L, sd[0].lv0, P32, B_PSI32, B_WORK32, B_POOL, B_WORK32, false);
+    push_gm(L, sd[1].lv1, P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+    push_gm(L, sd[0].lv1, P32, B_PSI32, B_WORK32, B_POOL, B_WORK32, false);
+    push_gc(L, sd[1].con, P64, B_PSI64, B_WORK64, B_POOL);
+    push_gc(L, sd[0].con, P32, B_PSI32, B_WORK32, B_POOL);
   };
   // represent (bottom-up) and the holes of the terms (top-down) on the engine
   // stream; the density tree, which needs neither, runs top-down on a side
@@ -283,8 +297,10 @@ int Engine::plan_gemm() {
   for (auto& r : repRange_) sandwich_layer(r, -1, gBuild_);
   for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 0, gHole_);
   wbase = wmax;
+  wbase32 = wmax32;
   for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 1, gRho_);
   workSize_ = std::max(workSize_, wmax);
+  work32Size_ = std::max(work32Size_, wmax32);
 
   // ---------------------------------------------------------------- apply
   std::map<int, int64_t> idmap;
@@ -299,97 +315,116 @@ int Engine::plan_gemm() {
   };
   int maxLevels = 0;
   for (auto& v : tree_.nodes) maxLevels = std::max(maxLevels, v.order() - 1);
-  std::vector<std::vector<GmDesc>> levels(maxLevels);
-  std::vector<GmDesc> stacked;
-  int64_t w = 0;
+  // [precision][level]; [0] FP32, [1] FP64
+  std::vector<std::vector<GmDesc>> levels[2] = {std::vector<std::vector<GmDesc>>(maxLevels),
+                                                std::vector<std::vector<GmDesc>>(maxLevels)};
+  std::vector<GmDesc> stacked[2];
+  int64_t wa[2] = {0, 0};
+  auto summand_hi = [&](const Summand& s) {
+    if (precision_ == 0) return true;
+    if (precision_ == 1) return false;
+    for (int64_t id : s.ids)
+      if (id >= 0 && highSlots_.count(id)) return true;
+    return false;
+  };
   for (auto& v : tree_.nodes) {
     const int L = v.order() - 1;
     const int nL = v.dims[L];
-    const auto& sm = summ_[v.addr];
     const auto& smode = smode_[v.addr];
-    if (sm.empty()) {
+    std::vector<Summand> part[2];
+    for (auto& s : summ_[v.addr]) part[summand_hi(s) ? 1 : 0].push_back(s);
+    if (part[0].empty() && part[1].empty()) {
       // no active term: Apply(v) = 0
       GmDesc g = gm_desc(v, L, v.off, 0, 0, v.off, GM_COPY, v.addr);
       g.are = 0.0;
-      stacked.push_back(g);
+      stacked[1].push_back(g);
       continue;
     }
-    const int64_t stack0 = w;
-    w += (int64_t)sm.size() * v.N;
-    const int64_t mstack = poolOff;
-    poolOff += (int64_t)sm.size() * nL * nL;
-    // per summand: factor on each leg (-1 = none), the L-leg matrix
-    std::map<std::vector<int64_t>, int64_t> zmemo;  // chain key -> work offset
-    for (size_t s = 0; s < sm.size(); ++s) {
-      std::vector<int64_t> fac(L, -1);
-      int64_t gL = -1;
-      for (size_t sl = 0; sl < smode.size(); ++sl) {
-        if (sm[s].ids[sl] < 0) continue;
-        if (smode[sl] == L) gL = sm[s].ids[sl];
-        else fac[smode[sl]] = sm[s].ids[sl];
-      }
-      // chain from the highest leg below L down to leg 0; the last level
-      // writes into the stack slot and sits in the last launch
-      int64_t cur = v.off;
-      int cursel = 0;
-      std::vector<int64_t> key = {v.addr};
-      for (int leg = L - 1; leg >= 0; --leg) {
-        const int lvl = maxLevels - 1 - leg;  // leg 0 -> last launch
-        const bool last = (leg == 0);
-        const int64_t slot = stack0 + (int64_t)s * v.N;
-        if (fac[leg] < 0) {
-          if (last) levels[lvl].push_back(gm_desc(v, 0, cur, cursel, 0, slot, GM_COPY, v.addr));
-          continue;
+    for (int hp = 1; hp >= 0; --hp) {
+      const auto& sm = part[hp];
+      if (sm.empty()) continue;
+      int64_t& w = wa[hp];
+      const int64_t stack0 = w;
+      w += (int64_t)sm.size() * v.N;
+      const int64_t mstack = poolOff;
+      poolOff += (int64_t)sm.size() * nL * nL;
+      std::map<std::vector<int64_t>, int64_t> zmemo;  // chain key -> work offset
+      for (size_t s = 0; s < sm.size(); ++s) {
+        std::vector<int64_t> fac(L, -1);
+        int64_t gL = -1;
+        for (size_t sl = 0; sl < smode.size(); ++sl) {
+          if (sm[s].ids[sl] < 0) continue;
+          if (smode[sl] == L) gL = sm[s].ids[sl];
+          else fac[smode[sl]] = sm[s].ids[sl];
         }
-        key.push_back(leg);
-        key.push_back(fac[leg]);
-        if (last) {
-          levels[lvl].push_back(gm_desc(v, leg, cur, cursel, fac[leg], slot, GM_N, v.addr));
-        } else {
-          auto z = zmemo.find(key);
-          if (z == zmemo.end()) {
-            const int64_t o = w;
-            w += v.N;
-            levels[lvl].push_back(gm_desc(v, leg, cur, cursel, fac[leg], o, GM_N, v.addr));
-            z = zmemo.emplace(key, o).first;
+        // chain from the highest leg below L down to leg 0; the last level
+        // writes into the stack slot and sits in the last launch
+        int64_t cur = v.off;
+        int cursel = 0;
+        std::vector<int64_t> key = {v.addr};
+        for (int leg = L - 1; leg >= 0; --leg) {
+          const int lvl = maxLevels - 1 - leg;
+          const bool last = (leg == 0);
+          const int64_t slot = stack0 + (int64_t)s * v.N;
+          if (fac[leg] < 0) {
+            if (last) levels[hp][lvl].push_back(gm_desc(v, 0, cur, cursel, 0, slot, GM_COPY, v.addr));
+            continue;
           }
-          cur = z->second;
-          cursel = 1;
+          key.push_back(leg);
+          key.push_back(fac[leg]);
+          if (last) {
+            levels[hp][lvl].push_back(gm_desc(v, leg, cur, cursel, fac[leg], slot, GM_N, v.addr));
+          } else {
+            auto z = zmemo.find(key);
+            if (z == zmemo.end()) {
+              const int64_t o = w;
+              w += v.N;
+              levels[hp][lvl].push_back(gm_desc(v, leg, cur, cursel, fac[leg], o, GM_N, v.addr));
+              z = zmemo.emplace(key, o).first;
+            }
+            cur = z->second;
+            cursel = 1;
+          }
         }
+        // stacked matrix block s = c_s G_s
+        CombOut co{};
+        co.out = mstack + (int64_t)s * nL * nL;
+        co.n = nL;
+        co.t0 = (int32_t)h_combt3_.size();
+        CombTerm ct{};
+        ct.cre = sm[s].coef.real();
+        ct.cim = sm[s].coef.imag();
+        ct.src = (gL >= 0) ? gL : identity(nL);
+        h_combt3_.push_back(ct);
+        co.t1 = (int32_t)h_combt3_.size();
+        h_comb3_.push_back(co);
       }
-      // stacked matrix block s = c_s G_s
-      CombOut co{};
-      co.out = mstack + (int64_t)s * nL * nL;
-      co.n = nL;
-      co.t0 = (int32_t)h_combt3_.size();
-      CombTerm ct{};
-      ct.cre = sm[s].coef.real();
-      ct.cim = sm[s].coef.imag();
-      ct.src = (gL >= 0) ? gL : identity(nL);
-      h_combt3_.push_back(ct);
-      co.t1 = (int32_t)h_combt3_.size();
-      h_comb3_.push_back(co);
+      GmDesc g{};
+      g.x = stack0;
+      g.xsel = 1;
+      g.m = mstack;
+      g.out = v.off;
+      g.bef = (int32_t)(v.N / nL);
+      g.nin = (int32_t)(sm.size() * nL);
+      g.nout = nL;
+      g.aft = 1;
+      g.op = GM_N;
+      g.acc = (hp == 0 && !part[1].empty()) ? 1 : 0;  // the FP32 part adds to the FP64 part
+      g.are = 1.0;
+      g.aim = 0.0;
+      g.node = v.addr;
+      stacked[hp].push_back(g);
     }
-    GmDesc g{};
-    g.x = stack0;
-    g.xsel = 1;
-    g.m = mstack;
-    g.out = v.off;
-    g.bef = (int32_t)(v.N / nL);
-    g.nin = (int32_t)(sm.size() * nL);
-    g.nout = nL;
-    g.aft = 1;
-    g.op = GM_N;
-    g.acc = 0;
-    g.are = 1.0;
-    g.aim = 0.0;
-    g.node = v.addr;
-    stacked.push_back(g);
   }
-  workSize_ = std::max(workSize_, w);
+  workSize_ = std::max(workSize_, wa[1]);
+  work32Size_ = std::max(work32Size_, wa[0]);
   const double fa0 = flops;
-  for (auto& lv : levels) push_gm(gApply_, lv, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
-  push_gm(gApply_, stacked, 1, B_PSI, B_WORK, B_POOL, B_APPLY64, false);
+  for (int lv = 0; lv < maxLevels; ++lv) {
+    push_gm(gApply_, levels[1][lv], P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+    push_gm(gApply_, levels[0][lv], P32, B_PSI32, B_WORK32, B_POOL, B_WORK32, false);
+  }
+  push_gm(gApply_, stacked[1], P64, B_PSI64, B_WORK64, B_POOL, B_APPLY64, false);
+  push_gm(gApply_, stacked[0], P32W, B_PSI32, B_WORK32, B_POOL, B_APPLY64, false);
   phaseFlops_[2] = flops - fa0;
 
   // --------------------------------------------------------------- finish
@@ -444,11 +479,11 @@ int Engine::plan_gemm() {
     m2.acc = 1;
     g2.push_back(m2);
   }
-  push_gc(gApply_, fin, 2, B_PSI64, B_APPLY64, B_POOL64);
+  push_gc(gApply_, fin, P64, B_PSI64, B_APPLY64, B_POOL64);
   {
     GLaunch gl{};
     gl.kind = 2;
-    gl.prec = 2;
+    gl.prec = P64;
     gl.nd = (int64_t)h_solve_.size();
     gl.x0 = gl.x1 = gl.m = gl.o = B_POOL64;
     double fl = 0.0;
@@ -457,8 +492,8 @@ int Engine::plan_gemm() {
     flops += fl;
     gApply_.push_back(gl);
   }
-  push_gm(gApply_, g1, 2, B_PSI64, B_APPLY64, B_POOL64, B_DPSI64, true);
-  push_gm(gApply_, g2, 2, B_PSI64, B_APPLY64, B_POOL64, B_DPSI64, true);
+  push_gm(gApply_, g1, P64, B_PSI64, B_APPLY64, B_POOL64, B_DPSI64, true);
+  push_gm(gApply_, g2, P64, B_PSI64, B_APPLY64, B_POOL64, B_DPSI64, true);
   phaseFlops_[3] = flops - ff0;
 
   // combine flops (aggregated slots of the shared plan, stacked blocks)
@@ -631,11 +666,11 @@ int Engine::plan_gemm() {
     h_comb2_ = c2;
     h_combt1_ = t1;
     h_combt2_ = t2;
-    for (auto& lv : glevels) push_gm(gGradPre_, lv, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
-    push_gm(gGradPre_, gstacked, 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
+    for (auto& lv : glevels) push_gm(gGradPre_, lv, P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+    push_gm(gGradPre_, gstacked, P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
     for (int d = 1; d <= tree_.maxDepth; ++d) {
-      push_gm(gGradDepth_[d], gacc[d], 0, B_PSI, B_WORK, B_POOL, B_WORK, false);
-      push_gc(gGradDepth_[d], gcon[d], 0, B_PSI, B_WORK, B_POOL);
+      push_gm(gGradDepth_[d], gacc[d], P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+      push_gc(gGradDepth_[d], gcon[d], P64, B_PSI64, B_WORK64, B_POOL);
     }
     // G_v = M_v - E rho_v (E set at run time), then
     // grad = (Apply + G_v x_L A) / <A_top|A_top>; at the top (Apply - E A) / N
@@ -665,8 +700,8 @@ int Engine::plan_gemm() {
       f2.push_back(g);
       poolOff += (int64_t)v.dims[L] * v.dims[L];
     }
-    push_gm(gGradFin_, f1, 2, B_PSI64, B_APPLY64, B_POOL, B_DPSI64, true);
-    push_gm(gGradFin_, f2, 2, B_PSI64, B_APPLY64, B_POOL, B_DPSI64, true);
+    push_gm(gGradFin_, f1, P64, B_PSI64, B_APPLY64, B_POOL, B_DPSI64, true);
+    push_gm(gGradFin_, f2, P64, B_PSI64, B_APPLY64, B_POOL, B_DPSI64, true);
     gradFlops_ = gcomb;
   }
   poolSize_ = poolOff;
@@ -674,7 +709,6 @@ int Engine::plan_gemm() {
   return QTG_OK;
 }
 
-template <typename T>
 int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre,
                          double sim, hipStream_t s, void* partbuf) {
   int rc;
@@ -689,22 +723,20 @@ int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, cons
       if ((rc = begin_mark(6, s))) return rc;
       const GmDesc* d = d_gm_ + gl.d0;
       const int32_t* wg = d_gmwg_ + gl.wg0;
-      if (gl.prec == 0)
-        rc = launch_gm<T, T>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o], d_part_,
-                             active, a, b, s);
-      else if (gl.prec == 1)
-        rc = launch_gm<T, double>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
-                                  partbuf, active, a, b, s);
+      void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *m = bases[gl.m], *o = bases[gl.o];
+      if (gl.prec == P64)
+        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s);
+      else if (gl.prec == P32)
+        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s);
       else
-        rc = launch_gm<double, double>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
-                                       partbuf, active, a, b, s);
+        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s);
       if (rc) return rc;
       if (gl.ns > 0) {
         // split descriptors skip inactive nodes in the tile pass; their
         // reduce must too, so a masked launch reduces only active ones
         const GmDesc* sd = d_gmsplit_ + gl.s0;
-        if (gl.prec == 0) rc = launch_gm_reduce<T>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
-        else rc = launch_gm_reduce<double>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
+        if (gl.prec == P32) rc = launch_gm_reduce<float>(sd, gl.ns, gl.nrb, o, partbuf, active, s);
+        else rc = launch_gm_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s);
         if (rc) return rc;
       }
       if (timing_) tfl_[6] += gl.flops;
@@ -712,16 +744,13 @@ int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, cons
       if ((rc = begin_mark(7, s))) return rc;
       const GcDesc* d = d_gc_ + gl.d0;
       const int32_t* wg = d_gcwg_ + gl.wg0;
-      if (gl.prec == 2)
-        rc = launch_gc<double>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], partbuf, active, s);
-      else
-        rc = launch_gc<T>(d, wg, gl.nd, gl.ntasks, bases[gl.x0], bases[gl.x1], bases[gl.o], partbuf, active, s);
+      void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *o = bases[gl.o];
+      if (gl.prec == P64) rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s);
+      else rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s);
       if (rc) return rc;
       if (gl.ns > 0) {
         const GcDesc* sd = d_gcsplit_ + gl.s0;
-        if (gl.prec == 2) rc = launch_gc_reduce<double>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
-        else rc = launch_gc_reduce<T>(sd, gl.ns, gl.nrb, bases[gl.o], partbuf, active, s);
-        if (rc) return rc;
+        if ((rc = launch_gc_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s))) return rc;
       }
       if (timing_) tfl_[7] += gl.flops;
     } else {
@@ -734,61 +763,59 @@ int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, cons
   return QTG_OK;
 }
 
-template <typename T>
 int Engine::build_gemm(hipStream_t s) {
   int rc;
-  void* psiT = c64_ ? (void*)d_psi32_ : (void*)d_psi64_;
   if (c64_ && (rc = convert_psi(s))) return rc;
-  void* bases[B_COUNT] = {psiT, d_work_, d_pool_, d_psi64_, d_apply64_, d_dpsi64_, d_pool64_};
+  void* bases[B_COUNT] = {d_psi32_, d_work32_, d_pool_, d_psi64_, d_apply64_, d_dpsi64_, d_pool64_, d_work_};
   // Side stream: density tree, then the eigensolver (which only feeds the
   // solve and rho^-1 stages of the finish).  Engine stream: represent, then
   // (after the densities, which some hole items use as parent factor) the
   // holes, the combinations and Apply; apply_gemm joins before the solve.
   QTG_HIP_TRY(hipEventRecord(evFork_, s));
   QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
-  if ((rc = run_launches<T>(gRho_, bases, nullptr, 1.0, 0.0, side_, d_partRho_))) return rc;
+  if ((rc = run_launches(gRho_, bases, nullptr, 1.0, 0.0, side_, d_partRho_))) return rc;
   QTG_HIP_TRY(hipEventRecord(evRho_, side_));
   if ((rc = begin_mark(1, side_))) return rc;
-  if ((rc = launch_heev<T>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
+  if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
   if ((rc = end_mark(side_))) return rc;
   QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
   heevPending_ = true;
-  if ((rc = run_launches<T>(gBuild_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gBuild_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
   QTG_HIP_TRY(hipStreamWaitEvent(s, evRho_, 0));
-  if ((rc = run_launches<T>(gHole_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gHole_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
   if ((rc = begin_mark(5, s))) return rc;
   if (nComb_ > 0) {
-    if ((rc = launch_combine<T>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
-    if ((rc = launch_combine<T>(d_combOut2_, nComb2_, d_combTerm2_, d_pool_, s))) return rc;
+    if ((rc = launch_combine<double>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
+    if ((rc = launch_combine<double>(d_combOut2_, nComb2_, d_combTerm2_, d_pool_, s))) return rc;
   }
-  if ((rc = launch_combine<T>(d_combOut3_, nComb3_, d_combTerm3_, d_pool_, s))) return rc;
+  if ((rc = launch_combine<double>(d_combOut3_, nComb3_, d_combTerm3_, d_pool_, s))) return rc;
   if ((rc = end_mark(s))) return rc;
   if (timing_) tfl_[5] += phaseFlops_[5];
   built_ = true;
   return QTG_OK;
 }
 
-template <typename T>
-int Engine::apply_gemm(hipStream_t s, const void* psiT, const double* psi64, double* dpsi64, const uint8_t* active,
-                       double ph_re, double ph_im) {
-  void* bases[B_COUNT] = {(void*)psiT, d_work_, d_pool_, (void*)psi64, d_apply64_, dpsi64, d_pool64_};
+int Engine::apply_gemm(hipStream_t s, const double* psi64, double* dpsi64, const uint8_t* active, double ph_re,
+                       double ph_im) {
+  void* bases[B_COUNT] = {d_psi32_, d_work32_, d_pool_, (void*)psi64, d_apply64_, dpsi64, d_pool64_, d_work_};
   // f = -i * phase
-  return run_launches<T>(gApply_, bases, active, ph_im, -ph_re, s, d_part_);
+  return run_launches(gApply_, bases, active, ph_im, -ph_re, s, d_part_);
 }
 
 int Engine::apply_finish(const double* psi64, double* dpsi64, const uint8_t* active, double ph_re, double ph_im,
                          double* energy_dev, hipStream_t s) {
-  if (c64_) {
-    set_error("engine: apply_finish needs a complex128 engine");
-    return QTG_ESTATE;
-  }
   if (!built_) {
     set_error("engine: build() first");
     return QTG_ESTATE;
   }
   int rc;
   if (path_ == 1) {
-    if ((rc = apply_gemm<double>(s, psi64, psi64, dpsi64, active, ph_re, ph_im))) return rc;
+    if (c64_) {
+      // the FP32 bulk reads a rounded copy of this state
+      const int64_t n = 2 * tree_.total;
+      if ((rc = launch_to_f32(psi64, d_psi32_, n, s))) return rc;
+    }
+    if ((rc = apply_gemm(s, psi64, dpsi64, active, ph_re, ph_im))) return rc;
     if (energy_dev)
       QTG_HIP_TRY(hipMemcpyAsync(energy_dev, d_pool64_ + 2 * energySlot_, 4 * sizeof(double),
                                  hipMemcpyDeviceToDevice, s));
@@ -823,15 +850,15 @@ int Engine::gradient(hipStream_t s, double* grad_host, double* energy) {
   }
   for (size_t i = 1; i < h_combtGE_.size(); i += 2) h_combtGE_[i].cre = -E;
   QTG_HIP_TRY(hipMemcpy(d_combtGE_, h_combtGE_.data(), sizeof(CombTerm) * h_combtGE_.size(), hipMemcpyHostToDevice));
-  void* bases[B_COUNT] = {d_psi64_, d_work_, d_pool_, d_psi64_, d_apply64_, d_grad64_, d_pool64_};
+  void* bases[B_COUNT] = {d_psi32_, d_work32_, d_pool_, d_psi64_, d_apply64_, d_grad64_, d_pool64_, d_work_};
   if ((rc = launch_combine<double>(d_combG1_, (int64_t)h_combG1_.size(), d_combtG1_, d_pool_, s))) return rc;
   if ((rc = launch_combine<double>(d_combG2_, (int64_t)h_combG2_.size(), d_combtG2_, d_pool_, s))) return rc;
   if ((rc = launch_combine<double>(d_combG3_, (int64_t)h_combG3_.size(), d_combtG3_, d_pool_, s))) return rc;
-  if ((rc = run_launches<double>(gGradPre_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gGradPre_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
   for (size_t d = 1; d < gGradDepth_.size(); ++d)
-    if ((rc = run_launches<double>(gGradDepth_[d], bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+    if ((rc = run_launches(gGradDepth_[d], bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
   if ((rc = launch_combine<double>(d_combGE_, (int64_t)h_combGE_.size(), d_combtGE_, d_pool_, s))) return rc;
-  if ((rc = run_launches<double>(gGradFin_, bases, nullptr, 1.0 / Nrm, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gGradFin_, bases, nullptr, 1.0 / Nrm, 0.0, s, d_part_))) return rc;
   if (grad_host) {
     QTG_HIP_TRY(hipMemcpyAsync(grad_host, d_grad64_, sizeof(double) * 2 * tree_.total, hipMemcpyDeviceToHost, s));
     QTG_HIP_TRY(hipStreamSynchronize(s));
@@ -839,11 +866,5 @@ int Engine::gradient(hipStream_t s, double* grad_host, double* energy) {
   return QTG_OK;
 }
 
-template int Engine::build_gemm<double>(hipStream_t);
-template int Engine::build_gemm<float>(hipStream_t);
-template int Engine::apply_gemm<double>(hipStream_t, const void*, const double*, double*, const uint8_t*, double,
-                                        double);
-template int Engine::apply_gemm<float>(hipStream_t, const void*, const double*, double*, const uint8_t*, double,
-                                       double);
 
 }  // namespace qtg

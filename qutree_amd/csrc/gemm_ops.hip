@@ -77,9 +77,9 @@ __device__ inline int task_desc(const D* __restrict__ descs, int ndesc, const in
 
 // ------------------------------------------------------------- mode product
 // Output tile: 16 rows of the flattened (b, a) index x 16 output columns i.
-template <typename T, typename TO>
+template <typename T, typename TM, typename TO>
 __device__ inline void gm_task(const GmDesc& d, int64_t t, const cplx<T>* __restrict__ X,
-                               const cplx<T>* __restrict__ M, cplx<TO>* O, cplx<TO>* part, double sre,
+                               const cplx<TM>* __restrict__ M, cplx<TO>* O, cplx<TO>* part, double sre,
                                double sim) {
   typedef typename MF<T>::v4 v4;
   const int lane = threadIdx.x & 63;
@@ -119,7 +119,7 @@ __device__ inline void gm_task(const GmDesc& d, int64_t t, const cplx<T>* __rest
   // B operand: op(M)(j, k) for column j = ct*16 + li
   const int j = ct * 16 + li;
   const bool cv = j < nout;
-  const cplx<T>* mb;
+  const cplx<TM>* mb;
   int64_t mks;
   if (d.op == GM_N) {
     mb = M + d.m + j;  // M(j, k) at j + k nout
@@ -142,7 +142,12 @@ __device__ inline void gm_task(const GmDesc& d, int64_t t, const cplx<T>* __rest
       const int kx = k0 + 4 * u + kk;
       const bool kv = kx < kend;
       av[u] = (rv && kv) ? xa[(int64_t)kx * bef] : z;
-      bv[u] = (cv && kv) ? mb[(int64_t)kx * mks] : z;
+      if (cv && kv) {
+        const cplx<TM> mv = mb[(int64_t)kx * mks];
+        bv[u] = mk<T>((T)mv.re, (T)mv.im);
+      } else {
+        bv[u] = z;
+      }
       if (conjm) bv[u].im = -bv[u].im;
     }
     cmma<T>(cr0, ci0, av[0], bv[0]);
@@ -189,16 +194,16 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <typename T, typename TO>
+template <typename T, typename TM, typename TO>
 __global__ void __launch_bounds__(64 * kWaves)
     gm_grouped(const GmDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
-               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, const cplx<T>* __restrict__ M,
+               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, const cplx<TM>* __restrict__ M,
                cplx<TO>* O, cplx<TO>* part, const uint8_t* __restrict__ active, double sre, double sim) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;  // uniform per wavefront
   const GmDesc d = descs[task_desc(descs, ndesc, wgdesc, task)];
   if (active && d.node >= 0 && !active[d.node]) return;
-  gm_task<T, TO>(d, task - d.tile0, d.xsel ? X1 : X0, M, O, part, sre, sim);
+  gm_task<T, TM, TO>(d, task - d.tile0, d.xsel ? X1 : X0, M, O, part, sre, sim);
 }
 
 template <typename T>
@@ -206,14 +211,14 @@ __global__ void __launch_bounds__(64 * kWaves)
     gm_single(GmDesc d, int64_t ntasks, const cplx<T>* __restrict__ X, const cplx<T>* __restrict__ M, cplx<T>* O) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;
-  gm_task<T, T>(d, task, X, M, O, O, 1.0, 0.0);  // no K split here: the workspace is never written
+  gm_task<T, T, T>(d, task, X, M, O, O, 1.0, 0.0);  // no K split here: the workspace is never written
 }
 
 // --------------------------------------------------------- hole contraction
 // Output tile: rows j (bra index), columns l (ket index); K = flattened (b, a).
-template <typename T>
+template <typename T, typename TO>
 __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __restrict__ X,
-                               const cplx<T>* __restrict__ Y, cplx<T>* O, cplx<T>* work) {
+                               const cplx<T>* __restrict__ Y, cplx<TO>* O, cplx<TO>* work) {
   typedef typename MF<T>::v4 v4;
   const int lane = threadIdx.x & 63;
   const int split = d.ksplit > 1 ? d.ksplit : 1;
@@ -278,7 +283,7 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
   for (int q = 0; q < 4; ++q) {
     const int row = jt * 16 + MF<T>::drow(lane, q);
     if (row < nx && yv) {
-      cplx<T> v = mk<T>(cr0[q] + cr1[q], ci0[q] + ci1[q]);
+      cplx<TO> v = mk<TO>((TO)(cr0[q] + cr1[q]), (TO)(ci0[q] + ci1[q]));
       const int64_t e = row + (int64_t)col * nx;
       if (split > 1) {
         work[d.part + (int64_t)ks * nx * ny + e] = v;
@@ -290,16 +295,16 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
   }
 }
 
-template <typename T>
+template <typename T, typename TO>
 __global__ void __launch_bounds__(64 * kWaves)
     gc_grouped(const GcDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
-               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, cplx<T>* O, cplx<T>* work,
+               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, cplx<TO>* O, cplx<TO>* work,
                const uint8_t* __restrict__ active) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;
   const GcDesc d = descs[task_desc(descs, ndesc, wgdesc, task)];
   if (active && d.node >= 0 && !active[d.node]) return;
-  gc_task<T>(d, task - d.tile0, (d.sel & 1) ? X1 : X0, (d.sel & 2) ? X1 : X0, O, work);
+  gc_task<T, TO>(d, task - d.tile0, (d.sel & 1) ? X1 : X0, (d.sel & 2) ? X1 : X0, O, work);
 }
 
 template <typename T>
@@ -307,7 +312,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     gc_single(GcDesc d, int64_t ntasks, const cplx<T>* __restrict__ X, const cplx<T>* __restrict__ Y, cplx<T>* O) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;
-  gc_task<T>(d, task, X, Y, O, O);  // no K split here: the workspace is never written
+  gc_task<T, T>(d, task, X, Y, O, O);  // no K split here: the workspace is never written
 }
 
 // as gm_reduce_kernel, for hole contractions
@@ -417,26 +422,26 @@ __global__ void __launch_bounds__(256)
 
 }  // namespace
 
-template <typename T, typename TO>
+template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
               double sim, hipStream_t s) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((gm_grouped<T, TO>), dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc, ntasks,
-                     (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<T>*)mbase, (cplx<TO>*)obase,
+  hipLaunchKernelGGL((gm_grouped<T, TM, TO>), dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc,
+                     ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase, (cplx<TO>*)obase,
                      (cplx<TO>*)part, active, sre, sim);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
-template <typename T>
+template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL(gc_grouped<T>, dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc, ntasks,
-                     (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<T>*)obase, (cplx<T>*)work, active);
+  hipLaunchKernelGGL((gc_grouped<T, TO>), dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc, ntasks,
+                     (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, (cplx<TO>*)work, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -494,16 +499,24 @@ int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax
   return QTG_OK;
 }
 
-template int launch_gm<double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
-    const void*, const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
-template int launch_gm<float, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
-    const void*, const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
-template int launch_gm<float, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
-    const void*, const void*, void*, void*, const uint8_t*, double, double, hipStream_t);
-template int launch_gc<double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*, void*,
-    void*, const uint8_t*, hipStream_t);
-template int launch_gc<float>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*, void*,
-    void*, const uint8_t*, hipStream_t);
+template int launch_gm<double, double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
+                                               const void*, const void*, void*, void*, const uint8_t*, double,
+                                               double, hipStream_t);
+template int launch_gm<float, float, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
+                                            const void*, const void*, void*, void*, const uint8_t*, double, double,
+                                            hipStream_t);
+template int launch_gm<float, double, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
+                                             const void*, const void*, void*, void*, const uint8_t*, double,
+                                             double, hipStream_t);
+template int launch_gm<float, double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
+                                              const void*, const void*, void*, void*, const uint8_t*, double,
+                                              double, hipStream_t);
+template int launch_gc<double, double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
+                                       void*, void*, const uint8_t*, hipStream_t);
+template int launch_gc<float, float>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
+                                     void*, void*, const uint8_t*, hipStream_t);
+template int launch_gc<float, double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
+                                      void*, void*, const uint8_t*, hipStream_t);
 template int launch_gm_single<double>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gm_single<float>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gc_single<double>(const GcDesc&, const void*, const void*, void*, hipStream_t);

@@ -54,12 +54,13 @@ inline int64_t gc_tasks(const GcDesc& d) {
   return (int64_t)((d.nx + 15) / 16) * ((d.ny + 15) / 16) * (d.ksplit > 1 ? d.ksplit : 1);
 }
 
-// Launchers.  Input bases x0 / x1 (selected per descriptor), matrix base and
-// output base are cplx<T>* (output cplx<TO>*); `active` may be null.  Every
-// alpha of a mode-product launch is multiplied by the complex scale (sre, sim).
+// Launchers.  Input bases x0 / x1 (selected per descriptor) are cplx<T>*, the
+// matrix base cplx<TM>* (converted to T on load), the output cplx<TO>*; the
+// products accumulate in T.  `active` may be null.  Every alpha of a
+// mode-product launch is multiplied by the complex scale (sre, sim).
 // wgdesc[b]: descriptor of the first task (kGemmWaves * b) of workgroup b
 // (see gemm_wg_table).
-template <typename T, typename TO>
+template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
               double sim, hipStream_t s);
@@ -69,7 +70,7 @@ int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t
 template <typename TO>
 int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* part,
                      const uint8_t* active, hipStream_t s);
-template <typename T>
+template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s);
 

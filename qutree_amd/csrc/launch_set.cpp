@@ -125,14 +125,14 @@ int LaunchSet::run(void* const* bases, hipStream_t s) const {
   int rc;
   for (const auto& L : launches) {
     if (L.kind == 0) {
-      if ((rc = launch_gm<double, double>(d_gm + L.d0, d_gmwg + L.wg0, L.nd, L.ntasks, bases[L.x0], bases[L.x1],
+      if ((rc = launch_gm<double, double, double>(d_gm + L.d0, d_gmwg + L.wg0, L.nd, L.ntasks, bases[L.x0], bases[L.x1],
                                           bases[L.m], bases[L.o], d_part, nullptr, 1.0, 0.0, s)))
         return rc;
       if (L.ns > 0 &&
           (rc = launch_gm_reduce<double>(d_gmsplit + L.s0, L.ns, L.nrb, bases[L.o], d_part, nullptr, s)))
         return rc;
     } else {
-      if ((rc = launch_gc<double>(d_gc + L.d0, d_gcwg + L.wg0, L.nd, L.ntasks, bases[L.x0], bases[L.x1],
+      if ((rc = launch_gc<double, double>(d_gc + L.d0, d_gcwg + L.wg0, L.nd, L.ntasks, bases[L.x0], bases[L.x1],
                                   bases[L.o], d_part, nullptr, s)))
         return rc;
       if (L.ns > 0 &&

@@ -278,17 +278,17 @@ int qtg_plan_execute(qtg_plan plan, void* out_base, const void* first_base, cons
     // first = M base, second = B base
     const auto* g = (const GmDesc*)p->d_descs;
     const auto* wg = (const int32_t*)p->d_wg;
-    return p->c64 ? launch_gm<float, float>(g, wg, p->ndesc, p->ntasks, second_base, second_base, first_base,
+    return p->c64 ? launch_gm<float, float, float>(g, wg, p->ndesc, p->ntasks, second_base, second_base, first_base,
                                             out_base, out_base, nullptr, 1.0, 0.0, s)
-                  : launch_gm<double, double>(g, wg, p->ndesc, p->ntasks, second_base, second_base, first_base,
+                  : launch_gm<double, double, double>(g, wg, p->ndesc, p->ntasks, second_base, second_base, first_base,
                                               out_base, out_base, nullptr, 1.0, 0.0, s);
   }
   const auto* g = (const GcDesc*)p->d_descs;
   // X (bra) from the first base, Y (ket) from the second: sel = 2 on every descriptor
   const auto* wg = (const int32_t*)p->d_wg;
-  int rc = p->c64 ? launch_gc<float>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, p->d_work,
+  int rc = p->c64 ? launch_gc<float, float>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, p->d_work,
                                      nullptr, s)
-                  : launch_gc<double>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, p->d_work,
+                  : launch_gc<double, double>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, p->d_work,
                                       nullptr, s);
   if (rc || p->nsplit == 0) return rc;
   const auto* sp = (const GcDesc*)p->d_split;

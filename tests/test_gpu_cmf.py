@@ -84,3 +84,20 @@ def test_cmf_energy_conservation_and_norm():
     top = eng.nnodes - 1
     a = eng.node_tensor(got["psi"], top)
     assert abs(np.vdot(a, a).real - 1.0) < 1e-7  # BS tolerance 1e-8 per step, 267 steps
+
+
+@pytest.mark.parametrize("precision,tol", [(2, 1e-8), (1, 1e-6)])
+def test_cmf8_reduced_precision_against_oracle(precision, tol):
+    """CMF with complex64 bulk contractions (FP64 integrator, projector, rho^-1):
+    the split form (large-coefficient terms in FP64) stays within 1e-8 of the
+    FP64 trajectory, the all-FP32 bulk within 1e-6."""
+    text, H = W.cmf8_model()
+    print(f"case cmf cmf8 precision={precision}", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(2024, delta_lowest=False)
+    ref = om.cmf(psi, 0.0, 1.0, 0.1, 0.1)
+    eng = Q.Engine(text, H, precision=precision)
+    got = Q.CMF(eng).integrate(psi, 0.0, 1.0, 0.1, 0.1)
+    de, da = report(ref, got)
+    assert got["steps"] == ref["steps"]
+    assert de < tol and da < tol
