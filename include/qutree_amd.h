@@ -195,17 +195,21 @@ int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* co
                          int64_t* items);
 int64_t qtg_host_psi_size(const char* tree_text);
 int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest, double* psi_host);
-/* Kernel timing: while on, HIP events are recorded on the launch stream
- * around every launch group; read_timing returns, per kernel group
- * [0 sandwich (represent/holes/density), 1 eigensolver, 2 apply, 3 finish,
- * 4 precision conversion, 5 matrix combination], the summed milliseconds,
- * launch counts and algorithmic FLOPs since set_timing(1); the grouped path
- * adds 6 mode products (gm_grouped), 7 hole contractions (gc_grouped),
- * 8 batched solves, 9 split-K reductions.  Arrays hold 10 entries. */
+/* Kernel timing: while on, sweeps run eagerly behind a short device-side
+ * hold (so the host queues the whole sweep first) and HIP events time every
+ * launch on its stream; the grouped kernels carry the events in their own
+ * dispatch (hipExtLaunchKernelGGL), the others are bracketed.  read_timing
+ * returns, per kernel group [0 sandwich (represent/holes/density),
+ * 1 eigensolver, 2 apply, 3 finish, 4 precision conversion, 5 matrix
+ * combination], the summed milliseconds, launch counts, algorithmic FLOPs
+ * and (grouped kernels) algorithmic bytes since set_timing(1); the grouped
+ * path adds 6 mode products (gm_grouped), 7 hole contractions (gc_grouped),
+ * 8 batched solves, 9 split-K reductions.  Arrays hold 10 entries; any
+ * pointer may be null. */
 int qtg_engine_set_timing(qtg_engine engine, int on);
 /* waits for the engine's own stream */
 int qtg_engine_synchronize(qtg_engine engine);
-int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, double* flops);
+int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, double* flops, double* bytes);
 /* "rho_eps": regularisation of the density inverse (default 1e-10).
  * "compress" (set before qtg_engine_set_hamiltonian; default 1): compute each
  * distinct h-matrix / hole matrix once (they depend only on the term's

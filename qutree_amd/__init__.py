@@ -87,7 +87,7 @@ def lib():
     L.qtg_cmf_destroy.argtypes = [_vp]
     L.qtg_cmf_integrate.argtypes = [_vp, _dp] + [C.c_double] * 8 + [_i64, _i64, _dp, _dp, _dp, _lp]
     L.qtg_cmf_integrate.restype = _i64
-    L.qtg_engine_read_timing.argtypes = [_vp, _dp, _lp, _dp]
+    L.qtg_engine_read_timing.argtypes = [_vp, _dp, _lp, _dp, _dp]
     _lib = L
     return L
 
@@ -300,7 +300,7 @@ class Engine:
         return f.value
 
     KERNELS = ("sandwich_kernel", "heev_reginv_kernel", "apply_kernel", "finish_kernel", "to_f32",
-               "combine_kernel", "gm_grouped", "gc_grouped", "solve_kernel", "gc_reduce_kernel")
+               "combine_kernel", "gm_grouped", "gc_grouped", "solve_kernel", "split_reduce")
 
     def set_timing(self, on: bool) -> None:
         check(lib().qtg_engine_set_timing(self.ptr, int(on)))
@@ -309,8 +309,9 @@ class Engine:
         ms = np.zeros(len(self.KERNELS))
         n = np.zeros(len(self.KERNELS), np.int64)
         fl = np.zeros(len(self.KERNELS))
-        check(lib().qtg_engine_read_timing(self.ptr, ms, n, fl))
-        return {k: dict(ms=float(ms[i]), launches=int(n[i]), flops=float(fl[i]))
+        by = np.zeros(len(self.KERNELS))
+        check(lib().qtg_engine_read_timing(self.ptr, ms, n, fl, by))
+        return {k: dict(ms=float(ms[i]), launches=int(n[i]), flops=float(fl[i]), bytes=float(by[i]))
                 for i, k in enumerate(self.KERNELS)}
 
     def set_option(self, key: str, value: float) -> None:

@@ -185,9 +185,9 @@ int qtg_engine_set_timing(qtg_engine e, int on) {
   return static_cast<Engine*>(e)->set_timing(on != 0);
 }
 
-int qtg_engine_read_timing(qtg_engine e, double* ms, int64_t* launches, double* flops) {
+int qtg_engine_read_timing(qtg_engine e, double* ms, int64_t* launches, double* flops, double* bytes) {
   if (!e) return QTG_EARG;
-  return static_cast<Engine*>(e)->read_timing(ms, launches, flops);
+  return static_cast<Engine*>(e)->read_timing(ms, launches, flops, bytes);
 }
 
 int qtg_engine_set_option(qtg_engine e, const char* key, double value) {

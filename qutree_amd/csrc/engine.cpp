@@ -26,6 +26,18 @@ __global__ void to_f32(const double* __restrict__ in, float* __restrict__ out, i
   if (i < n) out[i] = (float)in[i];
 }
 
+// Holds the stream for about `ticks` periods of the constant-rate wall clock
+// (bounded by an iteration cap, so it always ends).  With kernel timing on,
+// an eager sweep starts behind it: the host enqueues the whole sweep while
+// the device waits, and the events then time kernels, not host gaps.
+__global__ void hold_kernel(uint64_t ticks) {
+  const uint64_t t0 = wall_clock64();
+  for (int i = 0; i < (1 << 20); ++i) {
+    if (wall_clock64() - t0 >= ticks) break;
+    __builtin_amdgcn_s_sleep(8);
+  }
+}
+
 }  // namespace
 
 Engine::~Engine() {
@@ -713,7 +725,7 @@ int Engine::set_timing(bool on) {
   timing_ = on;
   marks_.clear();
   evnext_ = 0;
-  for (int i = 0; i < kPhases; ++i) tms_[i] = tfl_[i] = 0.0, tn_[i] = 0;
+  for (int i = 0; i < kPhases; ++i) tms_[i] = tfl_[i] = tby_[i] = 0.0, tn_[i] = 0;
   return QTG_OK;
 }
 
@@ -728,6 +740,27 @@ int Engine::begin_mark(int phase, hipStream_t s) {
   evnext_ += 2;
   QTG_HIP_TRY(hipEventRecord(m.a, s));
   marks_.push_back(m);
+  return QTG_OK;
+}
+
+int Engine::kernel_mark(int phase, const KernelTiming** kt) {
+  *kt = nullptr;
+  if (!timing_) return QTG_OK;
+  // fold finished marks first so the event pool stays bounded
+  if (marks_.size() >= 512) {
+    int rc = collect_marks();
+    if (rc) return rc;
+  }
+  while (evpool_.size() < evnext_ + 2) {
+    hipEvent_t e;
+    QTG_HIP_TRY(hipEventCreate(&e));
+    evpool_.push_back(e);
+  }
+  Mark m{evpool_[evnext_], evpool_[evnext_ + 1], phase};
+  evnext_ += 2;
+  marks_.push_back(m);
+  ktiming_ = KernelTiming{m.a, m.b};
+  *kt = &ktiming_;
   return QTG_OK;
 }
 
@@ -752,13 +785,14 @@ int Engine::collect_marks() {
   return QTG_OK;
 }
 
-int Engine::read_timing(double* ms, int64_t* launches, double* flops) {
+int Engine::read_timing(double* ms, int64_t* launches, double* flops, double* bytes) {
   int rc = collect_marks();
   if (rc) return rc;
   for (int i = 0; i < kPhases; ++i) {
     if (ms) ms[i] = tms_[i];
     if (launches) launches[i] = tn_[i];
     if (flops) flops[i] = tfl_[i];
+    if (bytes) bytes[i] = tby_[i];
   }
   return QTG_OK;
 }
@@ -853,6 +887,15 @@ int Engine::layer_derivative_all(hipStream_t s, double ph_re, double ph_im) {
 }
 
 int Engine::derivative_eager(hipStream_t s, double ph_re, double ph_im) {
+  if (timing_) {
+    if (clockKHz_ == 0) {
+      int dev = 0;
+      QTG_HIP_TRY(hipGetDevice(&dev));
+      QTG_HIP_TRY(hipDeviceGetAttribute(&clockKHz_, hipDeviceAttributeWallClockRate, dev));
+    }
+    hipLaunchKernelGGL(hold_kernel, dim3(1), dim3(64), 0, s, (uint64_t)clockKHz_ * 3);  // 3 ms
+    QTG_LAUNCH_CHECK();
+  }
   int rc = build(s);
   if (rc) return rc;
   return layer_derivative_all(s, ph_re, ph_im);

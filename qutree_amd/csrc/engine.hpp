@@ -81,7 +81,7 @@ class Engine {
   // 6 gm_grouped, 7 gc_grouped, 8 solve, 9 gc_reduce.
   static constexpr int kPhases = 10;
   int set_timing(bool on);
-  int read_timing(double* ms, int64_t* launches, double* flops);
+  int read_timing(double* ms, int64_t* launches, double* flops, double* bytes);
 
  private:
   template <typename T>
@@ -110,6 +110,7 @@ class Engine {
     int x0, x1, m, o;  // buffer ids
     bool scaled;       // multiply alphas by the launch scale (-i phase)
     double flops;
+    double bytes;  // algorithmic HBM bytes (distinct inputs once, outputs)
   };
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
                    hipStream_t s, void* partbuf);
@@ -218,6 +219,7 @@ class Engine {
   int derivative_eager(hipStream_t s, double ph_re, double ph_im);
   // timing state
   bool timing_ = false;
+  int clockKHz_ = 0;  // wall-clock rate of the current device (kHz), read on first use
   struct Mark {
     hipEvent_t a, b;
     int phase;
@@ -225,7 +227,7 @@ class Engine {
   std::vector<Mark> marks_;
   std::vector<hipEvent_t> evpool_;
   size_t evnext_ = 0;
-  double tms_[kPhases] = {0}, tfl_[kPhases] = {0};
+  double tms_[kPhases] = {0}, tfl_[kPhases] = {0}, tby_[kPhases] = {0};
   int64_t tn_[kPhases] = {0};
   double phaseFlops_[kPhases] = {0};
   std::vector<double> layerFlops_;  // flops of each sandwich launch, in launch order
@@ -233,6 +235,10 @@ class Engine {
   int begin_mark(int phase, hipStream_t s);
   int end_mark(hipStream_t s);
   int collect_marks();
+  // While timing is on: a fresh event pair for the next kernel's own start
+  // and stop times (*kt stays null when timing is off).
+  int kernel_mark(int phase, const KernelTiming** kt);
+  KernelTiming ktiming_{};
 };
 
 }  // namespace qtg

@@ -103,6 +103,17 @@ int Engine::plan_gemm() {
     gl.s0 = (int64_t)h_gmsplit_.size();
     int64_t t = 0, part = 0, rblocks = 0;
     double fl = 0.0;
+    // algorithmic bytes: every distinct input tensor and matrix once, every
+    // output written once (and read once when accumulated)
+    const double ein = prec == P64 ? 16.0 : 8.0, eout = prec == P32 ? 8.0 : 16.0;
+    std::set<std::pair<int, int64_t>> seenX;
+    std::set<int64_t> seenM;
+    double by = 0.0;
+    for (auto& d : descs) {
+      if (seenX.insert({d.xsel, d.x}).second) by += ein * d.bef * (double)d.nin * d.aft;
+      if (d.op != GM_COPY && seenM.insert(d.m).second) by += 16.0 * d.nin * (double)d.nout;
+      by += eout * (d.acc ? 2.0 : 1.0) * d.bef * (double)d.nout * d.aft;
+    }
     for (auto& d : descs) {
       // long contractions (stacked Apply summands) are split along K into
       // partial tiles that a second launch sums in a fixed order
@@ -132,6 +143,7 @@ int Engine::plan_gemm() {
     gl.o = o;
     gl.scaled = scaled;
     gl.flops = fl;
+    gl.bytes = by;
     flops += fl;
     L.push_back(gl);
   };
@@ -153,6 +165,15 @@ int Engine::plan_gemm() {
     int64_t tiles = 0;
     for (auto& d : descs) tiles += gc_tasks(d);
     const int64_t want = 2048;
+    const double ein = prec == P64 ? 16.0 : 8.0;
+    std::set<std::pair<int, int64_t>> seen;
+    double by = 0.0;
+    for (auto& d : descs) {
+      const double K = (double)d.bef * d.aft;
+      if (seen.insert({d.sel & 1, d.x}).second) by += ein * K * d.nx;
+      if (seen.insert({(d.sel >> 1) & 1, d.y}).second) by += ein * K * d.ny;
+      by += 16.0 * (d.acc ? 2.0 : 1.0) * d.nx * (double)d.ny;
+    }
     int64_t t = 0, part = 0, rblocks = 0;
     double fl = 0.0;
     for (auto& d : descs) {
@@ -186,6 +207,7 @@ int Engine::plan_gemm() {
     gl.o = o;
     gl.scaled = false;
     gl.flops = fl;
+    gl.bytes = by;
     flops += fl;
     L.push_back(gl);
   };
@@ -719,46 +741,48 @@ int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, cons
       heevPending_ = false;
     }
     const double a = gl.scaled ? sre : 1.0, b = gl.scaled ? sim : 0.0;
+    const KernelTiming* kt = nullptr;  // set while kernel timing is on
     if (gl.kind == 0) {
-      if ((rc = begin_mark(6, s))) return rc;
+      if ((rc = kernel_mark(6, &kt))) return rc;
       const GmDesc* d = d_gm_ + gl.d0;
       const int32_t* wg = d_gmwg_ + gl.wg0;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *m = bases[gl.m], *o = bases[gl.o];
       if (gl.prec == P64)
-        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s);
+        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s, kt);
       else if (gl.prec == P32)
-        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s);
+        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s, kt);
       else
-        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s);
+        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s, kt);
       if (rc) return rc;
+      if (timing_) tfl_[6] += gl.flops, tby_[6] += gl.bytes;
       if (gl.ns > 0) {
         // split descriptors skip inactive nodes in the tile pass; their
         // reduce must too, so a masked launch reduces only active ones
+        if ((rc = kernel_mark(9, &kt))) return rc;
         const GmDesc* sd = d_gmsplit_ + gl.s0;
-        if (gl.prec == P32) rc = launch_gm_reduce<float>(sd, gl.ns, gl.nrb, o, partbuf, active, s);
-        else rc = launch_gm_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s);
+        if (gl.prec == P32) rc = launch_gm_reduce<float>(sd, gl.ns, gl.nrb, o, partbuf, active, s, kt);
+        else rc = launch_gm_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s, kt);
         if (rc) return rc;
       }
-      if (timing_) tfl_[6] += gl.flops;
     } else if (gl.kind == 1) {
-      if ((rc = begin_mark(7, s))) return rc;
+      if ((rc = kernel_mark(7, &kt))) return rc;
       const GcDesc* d = d_gc_ + gl.d0;
       const int32_t* wg = d_gcwg_ + gl.wg0;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *o = bases[gl.o];
-      if (gl.prec == P64) rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s);
-      else rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s);
+      if (gl.prec == P64) rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s, kt);
+      else rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s, kt);
       if (rc) return rc;
+      if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
       if (gl.ns > 0) {
+        if ((rc = kernel_mark(9, &kt))) return rc;
         const GcDesc* sd = d_gcsplit_ + gl.s0;
-        if ((rc = launch_gc_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s))) return rc;
+        if ((rc = launch_gc_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s, kt))) return rc;
       }
-      if (timing_) tfl_[7] += gl.flops;
     } else {
-      if ((rc = begin_mark(8, s))) return rc;
-      if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s))) return rc;
+      if ((rc = kernel_mark(8, &kt))) return rc;
+      if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s, kt))) return rc;
       if (timing_) tfl_[8] += gl.flops;
     }
-    if ((rc = end_mark(s))) return rc;
   }
   return QTG_OK;
 }

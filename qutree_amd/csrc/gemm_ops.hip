@@ -15,6 +15,8 @@
 // loads, so every wavefront runs the same number of k-steps and MFMA issue
 // stays uniform.  Two accumulator pairs alternate between k-steps so that
 // consecutive MFMAs do not depend on each other.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 #include "gemm_ops.hpp"
@@ -22,6 +24,15 @@
 namespace qtg {
 
 namespace {
+
+// Launches k; with kt set, through hipExtLaunchKernelGGL so that kt's events
+// carry the kernel's own start and stop times (no marker packets around it).
+template <typename K, typename... A>
+void launch_timed(K k, dim3 grid, dim3 block, unsigned lds, hipStream_t s, const KernelTiming* kt, A... a) {
+  if (kt) hipExtLaunchKernelGGL(k, grid, block, lds, s, kt->start, kt->stop, 0, a...);
+  else hipLaunchKernelGGL(k, grid, block, lds, s, a...);
+}
+
 
 template <typename T>
 struct MF;
@@ -425,23 +436,24 @@ __global__ void __launch_bounds__(256)
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
-              double sim, hipStream_t s) {
+              double sim, hipStream_t s, const KernelTiming* kt) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((gm_grouped<T, TM, TO>), dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc,
-                     ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase, (cplx<TO>*)obase,
-                     (cplx<TO>*)part, active, sre, sim);
+  launch_timed(gm_grouped<T, TM, TO>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc, ntasks,
+               (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase, (cplx<TO>*)obase, (cplx<TO>*)part,
+               active, sre, sim);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s) {
+              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s,
+              const KernelTiming* kt) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((gc_grouped<T, TO>), dim3(blocks), dim3(64 * kWaves), 0, s, descs, wgdesc, (int)ndesc, ntasks,
-                     (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, (cplx<TO>*)work, active);
+  launch_timed(gc_grouped<T, TO>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc, ntasks,
+               (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, (cplx<TO>*)work, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -472,62 +484,62 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 
 template <typename TO>
 int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* part,
-                     const uint8_t* active, hipStream_t s) {
+                     const uint8_t* active, hipStream_t s, const KernelTiming* kt) {
   if (ndesc <= 0 || nblocks <= 0) return QTG_OK;
-  hipLaunchKernelGGL(gm_reduce_kernel<TO>, dim3((unsigned)nblocks), dim3(256), 0, s, descs, (int)ndesc,
-                     (cplx<TO>*)obase, (const cplx<TO>*)part, active);
+  launch_timed(gm_reduce_kernel<TO>, dim3((unsigned)nblocks), dim3(256), 0, s, kt, descs, (int)ndesc,
+               (cplx<TO>*)obase, (const cplx<TO>*)part, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 template <typename T>
 int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* work,
-                     const uint8_t* active, hipStream_t s) {
+                     const uint8_t* active, hipStream_t s, const KernelTiming* kt) {
   if (ndesc <= 0 || nblocks <= 0) return QTG_OK;
-  hipLaunchKernelGGL(gc_reduce_kernel<T>, dim3((unsigned)nblocks), dim3(256), 0, s, descs, (int)ndesc,
-                     (cplx<T>*)obase, (const cplx<T>*)work, active);
+  launch_timed(gc_reduce_kernel<T>, dim3((unsigned)nblocks), dim3(256), 0, s, kt, descs, (int)ndesc,
+               (cplx<T>*)obase, (const cplx<T>*)work, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
-                 hipStream_t s) {
+                 hipStream_t s, const KernelTiming* kt) {
   if (n <= 0) return QTG_OK;
   const size_t lds = 2 * sizeof(cplx<double>) * (size_t)nmax * nmax;
-  hipLaunchKernelGGL(solve_kernel, dim3((unsigned)n), dim3(256), lds, s, items, pool, active);
+  launch_timed(solve_kernel, dim3((unsigned)n), dim3(256), (unsigned)lds, s, kt, items, pool, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 template int launch_gm<double, double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
                                                const void*, const void*, void*, void*, const uint8_t*, double,
-                                               double, hipStream_t);
+                                               double, hipStream_t, const KernelTiming*);
 template int launch_gm<float, float, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
                                             const void*, const void*, void*, void*, const uint8_t*, double, double,
-                                            hipStream_t);
+                                            hipStream_t, const KernelTiming*);
 template int launch_gm<float, double, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
                                              const void*, const void*, void*, void*, const uint8_t*, double,
-                                             double, hipStream_t);
+                                             double, hipStream_t, const KernelTiming*);
 template int launch_gm<float, double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
                                               const void*, const void*, void*, void*, const uint8_t*, double,
-                                              double, hipStream_t);
+                                              double, hipStream_t, const KernelTiming*);
 template int launch_gc<double, double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
-                                       void*, void*, const uint8_t*, hipStream_t);
+                                       void*, void*, const uint8_t*, hipStream_t, const KernelTiming*);
 template int launch_gc<float, float>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
-                                     void*, void*, const uint8_t*, hipStream_t);
+                                     void*, void*, const uint8_t*, hipStream_t, const KernelTiming*);
 template int launch_gc<float, double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
-                                      void*, void*, const uint8_t*, hipStream_t);
+                                      void*, void*, const uint8_t*, hipStream_t, const KernelTiming*);
 template int launch_gm_single<double>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gm_single<float>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gc_single<double>(const GcDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gc_single<float>(const GcDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gm_reduce<double>(const GmDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                      hipStream_t);
+                                      hipStream_t, const KernelTiming*);
 template int launch_gm_reduce<float>(const GmDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                     hipStream_t);
+                                     hipStream_t, const KernelTiming*);
 template int launch_gc_reduce<double>(const GcDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                      hipStream_t);
+                                      hipStream_t, const KernelTiming*);
 template int launch_gc_reduce<float>(const GcDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                     hipStream_t);
+                                     hipStream_t, const KernelTiming*);
 
 }  // namespace qtg

@@ -18,6 +18,12 @@ namespace qtg {
 
 enum GmOp : int32_t { GM_N = 0, GM_T = 1, GM_H = 2, GM_COPY = 3 };
 
+// Optional event pair a launcher records the kernel's own start and stop
+// times into (hipExtLaunchKernelGGL); null launches plainly.
+struct KernelTiming {
+  hipEvent_t start, stop;
+};
+
 constexpr int kGemmWaves = 4;  // tasks (wavefronts) per workgroup of the grouped kernels
 
 struct GmDesc {
@@ -63,16 +69,17 @@ inline int64_t gc_tasks(const GcDesc& d) {
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
-              double sim, hipStream_t s);
+              double sim, hipStream_t s, const KernelTiming* kt = nullptr);
 // Sums the ksplit partial tiles (cplx<TO>) of split mode-product descriptors
 // into their outputs.  These descriptor copies carry in tile0 their first
 // block of 256 output elements; nblocks is the total.
 template <typename TO>
 int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* part,
-                     const uint8_t* active, hipStream_t s);
+                     const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr);
 template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s);
+              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s,
+              const KernelTiming* kt = nullptr);
 
 // Appends to `out` the first-descriptor index of every workgroup of a launch
 // over `descs` (tile0 already assigned, ntasks in total).
@@ -92,7 +99,7 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 // As launch_gm_reduce, for hole contractions.
 template <typename T>
 int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* work,
-                     const uint8_t* active, hipStream_t s);
+                     const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr);
 
 // Batched FP64 solve X <- S^-1 X (Gauss-Jordan with partial pivoting) for n x n
 // column-major S and X at pool offsets s, x; n <= 64.  When r >= 0 it also
@@ -102,6 +109,6 @@ struct SolveItem {
   int32_t n, node;
 };
 int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
-                 hipStream_t s);
+                 hipStream_t s, const KernelTiming* kt = nullptr);
 
 }  // namespace qtg
