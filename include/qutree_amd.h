@@ -93,10 +93,10 @@ int qtg_hole_contraction_c(float* S, const float* A, const float* B, int64_t bef
                            int64_t nB, int64_t after, int accumulate, qtg_stream stream);
 
 /* Grouped variants: `count` independent problems of uneven shape launched
- * together.  A plan copies the descriptors to the device once and sizes the
- * split-K workspace of the grouped hole contraction; executing a plan
- * allocates nothing.  Offsets are in complex elements from the base pointers
- * passed at execution.  No reference counterpart: these batch the per-term,
+ * together (no two may write the same output).  A plan copies the
+ * descriptors to the device once; executing a plan allocates nothing.
+ * Offsets are in complex elements from the base pointers passed at
+ * execution.  No reference counterpart: these batch the per-term,
  * per-node calls of TreeFunctions
  * (include/TreeClasses/SparseMatrixTreeFunctions_Implementation.h:18-276). */
 typedef struct {
@@ -220,7 +220,11 @@ int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, dou
  * into a HIP graph and replayed; 0 launches eagerly.
  * "path" (set before qtg_engine_set_hamiltonian; default 1): 1 runs every
  * node contraction as grouped MFMA tile GEMMs through a device workspace (any
- * node size); 0 keeps whole node tensors in LDS (nodes up to ~150 KB). */
+ * node size); 0 keeps whole node tensors in LDS (nodes up to ~150 KB).
+ * "split_waves", "split_min_k" (set before qtg_engine_set_hamiltonian;
+ * defaults 2048, 16): a grouped hole-contraction launch with fewer 16 x 16
+ * tiles than split_waves splits its reductions along K into chunks of at
+ * least split_min_k until it holds about split_waves wavefronts. */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
 
 /* ----------------------------------------------------------- CMF integrator */

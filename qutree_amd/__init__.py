@@ -214,7 +214,9 @@ class Engine:
     """Device-resident tree + SOP Hamiltonian (qtg_engine_*)."""
 
     def __init__(self, tree_text: str, H=None, precision: int = 0, compress: bool = True, graph: bool = True,
-                 path: int = 1):
+                 path: int = 1, **options):
+        """`options`: further qtg_engine_set_option keys (split_waves,
+        split_min_k, rho_eps), applied before the Hamiltonian is planned."""
         init()
         p = _vp()
         check(lib().qtg_engine_create(tree_text.encode(), precision, C.byref(p)))
@@ -222,6 +224,8 @@ class Engine:
         check(lib().qtg_engine_set_option(self.ptr, b"compress", float(compress)))
         check(lib().qtg_engine_set_option(self.ptr, b"graph", float(graph)))
         check(lib().qtg_engine_set_option(self.ptr, b"path", float(path)))
+        for k, v in options.items():
+            check(lib().qtg_engine_set_option(self.ptr, k.encode(), float(v)))
         self.precision = precision
         self.psi_size = lib().qtg_engine_psi_size(self.ptr)
         self.nnodes = lib().qtg_engine_num_nodes(self.ptr)

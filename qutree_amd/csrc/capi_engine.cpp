@@ -209,6 +209,22 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
     en->path_ = (int)value;
     return QTG_OK;
   }
+  if (!std::strcmp(key, "split_waves")) {
+    if (value < 1.0) {
+      set_error("option split_waves: at least 1");
+      return QTG_EARG;
+    }
+    en->splitWaves_ = (int64_t)value;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "split_min_k")) {
+    if (value < 4.0) {
+      set_error("option split_min_k: at least 4");
+      return QTG_EARG;
+    }
+    en->splitMinK_ = (int)value;
+    return QTG_OK;
+  }
   if (!std::strcmp(key, "graph")) {
     en->graph_ = value != 0.0;
     en->invalidate_graph();

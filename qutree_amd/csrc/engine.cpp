@@ -579,12 +579,6 @@ int Engine::plan() {
     device_buffers_.push_back(d_gmwg_);
     if ((rc = up(&d_gcwg_, h_gcwg_))) return rc;
     device_buffers_.push_back(d_gcwg_);
-    if ((rc = up(&d_gmsplit_, h_gmsplit_))) return rc;
-    device_buffers_.push_back(d_gmsplit_);
-    if ((rc = up(&d_gcsplit_, h_gcsplit_))) return rc;
-    device_buffers_.push_back(d_gcsplit_);
-    if ((rc = alloc(&d_part_, 16 * std::max<int64_t>(partSize_, 1)))) return rc;
-    if ((rc = alloc(&d_partRho_, 16 * std::max<int64_t>(partSize_, 1)))) return rc;
   }
   // the grouped path keeps every matrix in FP64 (FP32 kernels round on load)
   const size_t es = (c64_ && path_ == 0) ? 8 : 16;

@@ -58,6 +58,8 @@ class Engine {
   // 0: LDS-resident node kernels (node tensors up to ~150 KB);
   // 1: grouped MFMA contractions through a global workspace (any node size)
   int path_ = 1;
+  int64_t splitWaves_ = 2048;  // wavefronts a grouped hole-contraction launch is split up to
+  int splitMinK_ = 16;         // shortest K chunk of such a split
   // Apply + projector + rho^-1 on an arbitrary FP64 state (complex128 engines):
   // dpsi = -i phase rho^-1 (1 - P) Apply(psi) at every node whose active flag
   // is set (all when active is null); energy (4 doubles: <A|HA>, <A|A> of the
@@ -78,7 +80,7 @@ class Engine {
   void invalidate_graph();
   // Per-kernel timing with HIP events recorded on the launch stream around
   // every launch group: 0 sandwich, 1 heev, 2 apply, 3 finish, 4 convert, 5 combine.
-  // 6 gm_grouped, 7 gc_grouped, 8 solve, 9 gc_reduce.
+  // 6 gm_grouped, 7 gc_grouped, 8 solve, 9 (unused: split-K sums run inside 6 and 7).
   static constexpr int kPhases = 10;
   int set_timing(bool on);
   int read_timing(double* ms, int64_t* launches, double* flops, double* bytes);
@@ -104,8 +106,6 @@ class Engine {
     int kind;    // 0 mode products, 1 hole contractions, 2 solve
     int prec;    // Prec
     int64_t d0, nd, ntasks;
-    int64_t s0, ns;    // split descriptors of this launch (reduce pass)
-    int64_t nrb;       // reduce blocks
     int64_t wg0;       // first entry of the workgroup -> descriptor table
     int x0, x1, m, o;  // buffer ids
     bool scaled;       // multiply alphas by the launch scale (-i phase)
@@ -113,7 +113,7 @@ class Engine {
     double bytes;  // algorithmic HBM bytes (distinct inputs once, outputs)
   };
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
-                   hipStream_t s, void* partbuf);
+                   hipStream_t s);
   int materialize();
   std::vector<std::pair<int, int>> restrict_in(int l, int a) const;
   std::vector<std::pair<int, int>> restrict_out(int l, int a) const;
@@ -157,16 +157,11 @@ class Engine {
   std::vector<std::vector<int>> smode_, sdim_;  // slot legs and sizes per node
   double lflops_ = 0.0;                       // sweep flops of the LDS path
   // grouped path
-  std::vector<GmDesc> h_gm_, h_gmsplit_;
-  std::vector<GcDesc> h_gc_, h_gcsplit_;
-  int64_t partSize_ = 0;  // complex elements of the split-K partial tiles
+  std::vector<GmDesc> h_gm_;
+  std::vector<GcDesc> h_gc_;
   std::vector<int32_t> h_gmwg_, h_gcwg_;
   int32_t* d_gmwg_ = nullptr;
   int32_t* d_gcwg_ = nullptr;
-  GmDesc* d_gmsplit_ = nullptr;
-  GcDesc* d_gcsplit_ = nullptr;
-  void* d_part_ = nullptr;
-  void* d_partRho_ = nullptr;  // split-K partials of the side stream
   std::vector<SolveItem> h_solve_;
   std::vector<GLaunch> gBuild_, gHole_, gRho_, gApply_;
   std::vector<std::pair<int64_t, int>> idSlots_;  // identity matrices in the pool (offset, n)

@@ -35,8 +35,8 @@ namespace qtg {
 
 namespace {
 
-constexpr int kGmSplitK = 128;  // K per task of a split mode product
-constexpr int kGcSplitK = 512;  // K per task of a split hole contraction
+constexpr int kGmSplitK = 128;  // mode products longer than 2 kGmSplitK split along K
+constexpr int kGcSplitK = 512;  // hole contractions longer than 2 kGcSplitK split along K
 
 void view(const HNode& v, int k, int64_t& bef, int& nk, int64_t& aft) {
   bef = 1;
@@ -73,11 +73,8 @@ int Engine::plan_gemm() {
   const int NN = (int)tree_.nodes.size();
   h_gm_.clear();
   h_gc_.clear();
-  h_gmsplit_.clear();
-  h_gcsplit_.clear();
   h_gmwg_.clear();
   h_gcwg_.clear();
-  partSize_ = 0;
   h_solve_.clear();
   gBuild_.clear();
   gHole_.clear();
@@ -100,8 +97,6 @@ int Engine::plan_gemm() {
     gl.prec = prec;
     gl.d0 = (int64_t)h_gm_.size();
     gl.nd = (int64_t)descs.size();
-    gl.s0 = (int64_t)h_gmsplit_.size();
-    int64_t t = 0, part = 0, rblocks = 0;
     double fl = 0.0;
     // algorithmic bytes: every distinct input tensor and matrix once, every
     // output written once (and read once when accumulated)
@@ -116,24 +111,12 @@ int Engine::plan_gemm() {
     }
     for (auto& d : descs) {
       // long contractions (stacked Apply summands) are split along K into
-      // partial tiles that a second launch sums in a fixed order
-      if (d.op != GM_COPY && d.nin > 2 * kGmSplitK) {
-        d.ksplit = (d.nin + kGmSplitK - 1) / kGmSplitK;
-        d.part = part;
-        part += (int64_t)d.ksplit * d.bef * d.aft * d.nout;
-        GmDesc r = d;
-        r.tile0 = rblocks;  // reduce blocks of 256 output elements
-        rblocks += ((int64_t)d.bef * d.aft * d.nout + 255) / 256;
-        h_gmsplit_.push_back(r);
-      }
-      d.tile0 = t;
-      t += gm_tasks(d);
+      // chunks that sum inside their workgroup
+      if (d.op != GM_COPY && d.nin > 2 * kGmSplitK) d.ksplit = split_chunks(d.nin, kGemmWaves, kGmSplitK / 2);
       if (d.op != GM_COPY) fl += 8.0 * d.bef * d.aft * (double)d.nin * d.nout;
-      h_gm_.push_back(d);
     }
-    gl.ns = (int64_t)h_gmsplit_.size() - gl.s0;
-    gl.nrb = rblocks;
-    partSize_ = std::max(partSize_, part);
+    const int64_t t = order_tasks(descs, gm_tasks);
+    h_gm_.insert(h_gm_.end(), descs.begin(), descs.end());
     gl.wg0 = (int64_t)h_gmwg_.size();
     gemm_wg_table(h_gm_.data() + gl.d0, gl.nd, t, h_gmwg_);
     gl.ntasks = t;
@@ -154,17 +137,17 @@ int Engine::plan_gemm() {
     gl.prec = prec;
     gl.d0 = (int64_t)h_gc_.size();
     gl.nd = (int64_t)descs.size();
-    gl.s0 = (int64_t)h_gcsplit_.size();
     // descriptors sharing a bra tensor sit next to each other, so the
     // wavefronts of one workgroup mostly read the same X through L1
     std::stable_sort(descs.begin(), descs.end(), [](const GcDesc& a, const GcDesc& b) {
       return std::make_tuple(a.sel & 1, a.x, a.y) < std::make_tuple(b.sel & 1, b.x, b.y);
     });
-    // launches with few tiles split long reductions so that every SIMD holds
-    // several wavefronts (a 16 x 16 tile is one wavefront)
+    // launches with few tiles split their reductions along K (2 or 4
+    // chunks of at least splitMinK_) toward about splitWaves_ wavefronts (a
+    // 16 x 16 tile is one); beyond 2 kGcSplitK a reduction is always split
     int64_t tiles = 0;
     for (auto& d : descs) tiles += gc_tasks(d);
-    const int64_t want = 2048;
+    const int64_t want = (splitWaves_ + tiles - 1) / tiles;
     const double ein = prec == P64 ? 16.0 : 8.0;
     std::set<std::pair<int, int64_t>> seen;
     double by = 0.0;
@@ -174,30 +157,14 @@ int Engine::plan_gemm() {
       if (seen.insert({(d.sel >> 1) & 1, d.y}).second) by += ein * K * d.ny;
       by += 16.0 * (d.acc ? 2.0 : 1.0) * d.nx * (double)d.ny;
     }
-    int64_t t = 0, part = 0, rblocks = 0;
     double fl = 0.0;
     for (auto& d : descs) {
       const int64_t K = (int64_t)d.bef * d.aft;
-      int64_t ks = 1;
-      if (K > 2 * kGcSplitK) ks = (K + kGcSplitK - 1) / kGcSplitK;
-      if (tiles < 512 && K >= 128) ks = std::max<int64_t>(ks, std::min<int64_t>((want + tiles - 1) / tiles, K / 48));
-      if (ks > 1) {
-        d.ksplit = (int32_t)ks;
-        d.part = part;
-        part += (int64_t)d.ksplit * d.nx * d.ny;
-        GcDesc r = d;
-        r.tile0 = rblocks;
-        rblocks += ((int64_t)d.nx * d.ny + 255) / 256;
-        h_gcsplit_.push_back(r);
-      }
-      d.tile0 = t;
-      t += gc_tasks(d);
+      d.ksplit = K > 2 * kGcSplitK ? split_chunks(K, kGemmWaves, 1) : split_chunks(K, want, splitMinK_);
       fl += 8.0 * d.bef * d.aft * (double)d.nx * d.ny;
-      h_gc_.push_back(d);
     }
-    gl.ns = (int64_t)h_gcsplit_.size() - gl.s0;
-    gl.nrb = rblocks;
-    partSize_ = std::max(partSize_, part);
+    const int64_t t = order_tasks(descs, gc_tasks);
+    h_gc_.insert(h_gc_.end(), descs.begin(), descs.end());
     gl.wg0 = (int64_t)h_gcwg_.size();
     gemm_wg_table(h_gc_.data() + gl.d0, gl.nd, t, h_gcwg_);
     gl.ntasks = t;
@@ -732,7 +699,7 @@ int Engine::plan_gemm() {
 }
 
 int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre,
-                         double sim, hipStream_t s, void* partbuf) {
+                         double sim, hipStream_t s) {
   int rc;
   for (const auto& gl : L) {
     if (gl.kind == 2 && heevPending_) {
@@ -748,36 +715,27 @@ int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, cons
       const int32_t* wg = d_gmwg_ + gl.wg0;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *m = bases[gl.m], *o = bases[gl.o];
       if (gl.prec == P64)
-        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s, kt);
+        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s,
+                                               kt);
       else if (gl.prec == P32)
-        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s, kt);
+        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s,
+                                             kt);
       else
-        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, partbuf, active, a, b, s, kt);
+        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s,
+                                              kt);
       if (rc) return rc;
       if (timing_) tfl_[6] += gl.flops, tby_[6] += gl.bytes;
-      if (gl.ns > 0) {
-        // split descriptors skip inactive nodes in the tile pass; their
-        // reduce must too, so a masked launch reduces only active ones
-        if ((rc = kernel_mark(9, &kt))) return rc;
-        const GmDesc* sd = d_gmsplit_ + gl.s0;
-        if (gl.prec == P32) rc = launch_gm_reduce<float>(sd, gl.ns, gl.nrb, o, partbuf, active, s, kt);
-        else rc = launch_gm_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s, kt);
-        if (rc) return rc;
-      }
     } else if (gl.kind == 1) {
       if ((rc = kernel_mark(7, &kt))) return rc;
       const GcDesc* d = d_gc_ + gl.d0;
       const int32_t* wg = d_gcwg_ + gl.wg0;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *o = bases[gl.o];
-      if (gl.prec == P64) rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s, kt);
-      else rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, partbuf, active, s, kt);
+      if (gl.prec == P64)
+        rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt);
+      else
+        rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt);
       if (rc) return rc;
       if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
-      if (gl.ns > 0) {
-        if ((rc = kernel_mark(9, &kt))) return rc;
-        const GcDesc* sd = d_gcsplit_ + gl.s0;
-        if ((rc = launch_gc_reduce<double>(sd, gl.ns, gl.nrb, o, partbuf, active, s, kt))) return rc;
-      }
     } else {
       if ((rc = kernel_mark(8, &kt))) return rc;
       if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s, kt))) return rc;
@@ -797,16 +755,16 @@ int Engine::build_gemm(hipStream_t s) {
   // holes, the combinations and Apply; apply_gemm joins before the solve.
   QTG_HIP_TRY(hipEventRecord(evFork_, s));
   QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
-  if ((rc = run_launches(gRho_, bases, nullptr, 1.0, 0.0, side_, d_partRho_))) return rc;
+  if ((rc = run_launches(gRho_, bases, nullptr, 1.0, 0.0, side_))) return rc;
   QTG_HIP_TRY(hipEventRecord(evRho_, side_));
   if ((rc = begin_mark(1, side_))) return rc;
   if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
   if ((rc = end_mark(side_))) return rc;
   QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
   heevPending_ = true;
-  if ((rc = run_launches(gBuild_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gBuild_, bases, nullptr, 1.0, 0.0, s))) return rc;
   QTG_HIP_TRY(hipStreamWaitEvent(s, evRho_, 0));
-  if ((rc = run_launches(gHole_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gHole_, bases, nullptr, 1.0, 0.0, s))) return rc;
   if ((rc = begin_mark(5, s))) return rc;
   if (nComb_ > 0) {
     if ((rc = launch_combine<double>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
@@ -823,7 +781,7 @@ int Engine::apply_gemm(hipStream_t s, const double* psi64, double* dpsi64, const
                        double ph_im) {
   void* bases[B_COUNT] = {d_psi32_, d_work32_, d_pool_, (void*)psi64, d_apply64_, dpsi64, d_pool64_, d_work_};
   // f = -i * phase
-  return run_launches(gApply_, bases, active, ph_im, -ph_re, s, d_part_);
+  return run_launches(gApply_, bases, active, ph_im, -ph_re, s);
 }
 
 int Engine::apply_finish(const double* psi64, double* dpsi64, const uint8_t* active, double ph_re, double ph_im,
@@ -878,11 +836,11 @@ int Engine::gradient(hipStream_t s, double* grad_host, double* energy) {
   if ((rc = launch_combine<double>(d_combG1_, (int64_t)h_combG1_.size(), d_combtG1_, d_pool_, s))) return rc;
   if ((rc = launch_combine<double>(d_combG2_, (int64_t)h_combG2_.size(), d_combtG2_, d_pool_, s))) return rc;
   if ((rc = launch_combine<double>(d_combG3_, (int64_t)h_combG3_.size(), d_combtG3_, d_pool_, s))) return rc;
-  if ((rc = run_launches(gGradPre_, bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gGradPre_, bases, nullptr, 1.0, 0.0, s))) return rc;
   for (size_t d = 1; d < gGradDepth_.size(); ++d)
-    if ((rc = run_launches(gGradDepth_[d], bases, nullptr, 1.0, 0.0, s, d_part_))) return rc;
+    if ((rc = run_launches(gGradDepth_[d], bases, nullptr, 1.0, 0.0, s))) return rc;
   if ((rc = launch_combine<double>(d_combGE_, (int64_t)h_combGE_.size(), d_combtGE_, d_pool_, s))) return rc;
-  if ((rc = run_launches(gGradFin_, bases, nullptr, 1.0 / Nrm, 0.0, s, d_part_))) return rc;
+  if ((rc = run_launches(gGradFin_, bases, nullptr, 1.0 / Nrm, 0.0, s))) return rc;
   if (grad_host) {
     QTG_HIP_TRY(hipMemcpyAsync(grad_host, d_grad64_, sizeof(double) * 2 * tree_.total, hipMemcpyDeviceToHost, s));
     QTG_HIP_TRY(hipStreamSynchronize(s));

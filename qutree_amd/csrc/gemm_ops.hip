@@ -62,17 +62,6 @@ __device__ inline void cmma(typename MF<T>::v4& cr, typename MF<T>::v4& ci, cons
   ci = MF<T>::mma(a.im, b.re, ci);
 }
 
-template <typename D>
-__device__ inline int find_desc(const D* __restrict__ descs, int ndesc, int64_t task) {
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].tile0 <= task) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 constexpr int kWaves = kGemmWaves;  // wavefronts (tasks) per workgroup
 
 // Descriptor of a task: wgdesc[blockIdx.x] is the descriptor of the
@@ -86,49 +75,88 @@ __device__ inline int task_desc(const D* __restrict__ descs, int ndesc, const in
   return di;
 }
 
+// Split-K inside a workgroup: the ksplit (2 or 4) chunks of a split tile are
+// consecutive tasks starting at a multiple of ksplit, so they are wavefronts
+// of one workgroup.  Each leaves its (alpha-scaled, output-precision) partial
+// tile in LDS and the chunk-0 wavefront sums them in chunk order after the
+// workgroup barrier: deterministic, no atomics, no second launch.  Every
+// wavefront of a workgroup reaches the barrier, including idle ones.
+template <typename TO>
+__device__ inline void reduce_chunks(cplx<TO>* red, bool mine, int split, int ks, cplx<TO> (&v)[4]) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (mine && split > 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave * 256 + q * 64 + lane] = v[q];
+  }
+  __syncthreads();
+  if (mine && split > 1 && ks == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cplx<TO> x = red[wave * 256 + q * 64 + lane];
+      for (int c = 1; c < split; ++c) x = cadd(x, red[(wave + c) * 256 + q * 64 + lane]);
+      v[q] = x;
+    }
+  }
+}
+
 // ------------------------------------------------------------- mode product
 // Output tile: 16 rows of the flattened (b, a) index x 16 output columns i.
+// Tile index and K chunk of task t of descriptor d.
+struct GmTile {
+  int64_t rt;
+  int ct, ks;
+};
+__device__ inline GmTile gm_tile(const GmDesc& d, int64_t t) {
+  const int split = d.ksplit > 1 ? d.ksplit : 1;
+  GmTile g;
+  g.ks = (int)(t % split);
+  t /= split;
+  const int CT = (d.nout + 15) >> 4;
+  g.rt = t / CT;
+  g.ct = (int)(t - g.rt * CT);
+  return g;
+}
+
+// out(b,i,a) = alpha X(b,i,a) for one 16 x 16 tile, four columns per lane
+template <typename T, typename TO>
+__device__ inline void gm_copy(const GmDesc& d, const GmTile& g, const cplx<T>* __restrict__ X, cplx<TO>* O,
+                               double sre, double sim) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const int64_t bef = d.bef, R = bef * d.aft;
+  const T are = (T)(d.are * sre - d.aim * sim), aim = (T)(d.are * sim + d.aim * sre);
+  for (int q = 0; q < 4; ++q) {
+    const int64_t r = g.rt * 16 + li;
+    const int col = g.ct * 16 + kk * 4 + q;
+    if (r < R && col < d.nout) {
+      const int64_t b = r % bef, a = r / bef;
+      const cplx<T> x = X[d.x + b + (int64_t)col * bef + a * bef * d.nin];
+      const int64_t o = d.out + b + (int64_t)col * bef + a * bef * d.nout;
+      cplx<TO> v = mk<TO>((TO)(are * x.re - aim * x.im), (TO)(are * x.im + aim * x.re));
+      if (d.acc) v = cadd(v, O[o]);
+      O[o] = v;
+    }
+  }
+}
+
+// This chunk's share of the product tile, alpha-scaled, in the output
+// precision: v[q] is row rt*16 + drow(lane, q), column ct*16 + lane % 16.
 template <typename T, typename TM, typename TO>
-__device__ inline void gm_task(const GmDesc& d, int64_t t, const cplx<T>* __restrict__ X,
-                               const cplx<TM>* __restrict__ M, cplx<TO>* O, cplx<TO>* part, double sre,
-                               double sim) {
+__device__ inline void gm_product(const GmDesc& d, const GmTile& g, const cplx<T>* __restrict__ X,
+                                  const cplx<TM>* __restrict__ M, double sre, double sim, cplx<TO> (&v)[4]) {
   typedef typename MF<T>::v4 v4;
   const int lane = threadIdx.x & 63;
   const int split = d.ksplit > 1 ? d.ksplit : 1;
-  const int ks = (int)(t % split);
-  t /= split;
-  const int CT = (d.nout + 15) >> 4;
-  const int64_t rt = t / CT;
-  const int ct = (int)(t - rt * CT);
   const int64_t bef = d.bef, R = bef * d.aft;
   const int nin = d.nin, nout = d.nout;
   const int li = lane & 15, kk = lane >> 4;
   const T are = (T)(d.are * sre - d.aim * sim), aim = (T)(d.are * sim + d.aim * sre);
-
-  if (d.op == GM_COPY) {
-    // out(b,i,a) = alpha X(b,i,a): 16 rows x 16 columns, four per lane
-    for (int q = 0; q < 4; ++q) {
-      const int64_t r = rt * 16 + li;
-      const int col = ct * 16 + kk * 4 + q;
-      if (r < R && col < nout) {
-        const int64_t b = r % bef, a = r / bef;
-        const cplx<T> x = X[d.x + b + (int64_t)col * bef + a * bef * nin];
-        const int64_t o = d.out + b + (int64_t)col * bef + a * bef * nout;
-        cplx<TO> v = mk<TO>((TO)(are * x.re - aim * x.im), (TO)(are * x.im + aim * x.re));
-        if (d.acc) v = cadd(v, O[o]);
-        O[o] = v;
-      }
-    }
-    return;
-  }
-
   // A operand: row r = rt*16 + li of the flattened (b, a), column k
-  const int64_t r = rt * 16 + li;
+  const int64_t r = g.rt * 16 + li;
   const bool rv = r < R;
   const int64_t rb = rv ? r % bef : 0, ra = rv ? r / bef : 0;
   const cplx<T>* xa = X + d.x + rb + ra * bef * nin;
   // B operand: op(M)(j, k) for column j = ct*16 + li
-  const int j = ct * 16 + li;
+  const int j = g.ct * 16 + li;
   const bool cv = j < nout;
   const cplx<TM>* mb;
   int64_t mks;
@@ -140,9 +168,9 @@ __device__ inline void gm_task(const GmDesc& d, int64_t t, const cplx<T>* __rest
     mks = 1;
   }
   const bool conjm = d.op == GM_H;
-  // K range of this split (a multiple of 4 long, except the last)
+  // K range of this chunk (a multiple of 4 long, except the last)
   const int kchunk = ((nin + split - 1) / split + 3) & ~3;
-  const int kbeg = ks * kchunk, kend = min(nin, kbeg + kchunk);
+  const int kbeg = g.ks * kchunk, kend = min(nin, kbeg + kchunk);
   v4 cr0 = {0, 0, 0, 0}, ci0 = {0, 0, 0, 0}, cr1 = {0, 0, 0, 0}, ci1 = {0, 0, 0, 0};
   const cplx<T> z = czero<T>();
   // four k-steps per iteration: all loads are issued before the MFMAs
@@ -168,40 +196,27 @@ __device__ inline void gm_task(const GmDesc& d, int64_t t, const cplx<T>* __rest
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int64_t rr = rt * 16 + MF<T>::drow(lane, q);
-    if (rr < R && cv) {
-      const T vr = cr0[q] + cr1[q], vi = ci0[q] + ci1[q];
-      cplx<TO> v = mk<TO>((TO)(are * vr - aim * vi), (TO)(are * vi + aim * vr));
-      if (split > 1) {
-        part[d.part + (int64_t)ks * R * nout + rr + (int64_t)j * R] = v;
-      } else {
-        const int64_t b = rr % bef, a = rr / bef;
-        const int64_t o = d.out + b + (int64_t)j * bef + a * bef * nout;
-        if (d.acc) v = cadd(v, O[o]);
-        O[o] = v;
-      }
-    }
+    const T vr = cr0[q] + cr1[q], vi = ci0[q] + ci1[q];
+    v[q] = mk<TO>((TO)(are * vr - aim * vi), (TO)(are * vi + aim * vr));
   }
 }
 
-// Sums the partial tiles of split descriptors in chunk order.  The split
-// descriptors carry, in tile0, the first 256-element block of their output:
-// each workgroup owns one block of one descriptor.
-template <typename TO>
-__global__ void __launch_bounds__(256)
-    gm_reduce_kernel(const GmDesc* __restrict__ descs, int ndesc, cplx<TO>* O, const cplx<TO>* __restrict__ part,
-                     const uint8_t* __restrict__ active) {
-  const GmDesc d = descs[find_desc(descs, ndesc, blockIdx.x)];
-  if (active && d.node >= 0 && !active[d.node]) return;
-  const int64_t bef = d.bef, R = bef * d.aft, n = R * d.nout;
-  const int64_t e = (blockIdx.x - d.tile0) * 256 + threadIdx.x;
-  if (e < n) {
-    cplx<TO> s = part[d.part + e];
-    for (int c = 1; c < d.ksplit; ++c) s = cadd(s, part[d.part + c * n + e]);
-    const int64_t r = e % R, jj = e / R;
-    const int64_t o = d.out + r % bef + jj * bef + (r / bef) * bef * d.nout;
-    if (d.acc) s = cadd(s, O[o]);
-    O[o] = s;
+template <typename T, typename TO>
+__device__ inline void gm_store(const GmDesc& d, const GmTile& g, const cplx<TO> (&v)[4], cplx<TO>* O) {
+  const int lane = threadIdx.x & 63;
+  const int64_t bef = d.bef, R = bef * d.aft;
+  const int j = g.ct * 16 + (lane & 15);
+  if (j >= d.nout) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t rr = g.rt * 16 + MF<T>::drow(lane, q);
+    if (rr < R) {
+      const int64_t b = rr % bef, a = rr / bef;
+      const int64_t o = d.out + b + (int64_t)j * bef + a * bef * d.nout;
+      cplx<TO> x = v[q];
+      if (d.acc) x = cadd(x, O[o]);
+      O[o] = x;
+    }
   }
 }
 
@@ -209,47 +224,83 @@ template <typename T, typename TM, typename TO>
 __global__ void __launch_bounds__(64 * kWaves)
     gm_grouped(const GmDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
                const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, const cplx<TM>* __restrict__ M,
-               cplx<TO>* O, cplx<TO>* part, const uint8_t* __restrict__ active, double sre, double sim) {
+               cplx<TO>* O, const uint8_t* __restrict__ active, double sre, double sim) {
+  __shared__ cplx<TO> red[kWaves * 256];
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-  if (task >= ntasks) return;  // uniform per wavefront
-  const GmDesc d = descs[task_desc(descs, ndesc, wgdesc, task)];
-  if (active && d.node >= 0 && !active[d.node]) return;
-  gm_task<T, TM, TO>(d, task - d.tile0, d.xsel ? X1 : X0, M, O, part, sre, sim);
+  bool mine = task < ntasks;  // uniform per wavefront
+  GmDesc d{};
+  if (mine) {
+    d = descs[task_desc(descs, ndesc, wgdesc, task)];
+    mine = !(active && d.node >= 0 && !active[d.node]);
+  }
+  GmTile g{0, 0, 0};
+  if (mine) g = gm_tile(d, task - d.tile0);
+  const int split = mine && d.ksplit > 1 ? d.ksplit : 1;
+  const cplx<T>* X = d.xsel ? X1 : X0;
+  cplx<TO> v[4];
+  if (mine && d.op == GM_COPY) {
+    gm_copy<T, TO>(d, g, X, O, sre, sim);
+    mine = false;
+  }
+  if (mine) gm_product<T, TM, TO>(d, g, X, M, sre, sim, v);
+  reduce_chunks<TO>(red, mine, split, g.ks, v);
+  if (mine && g.ks == 0) gm_store<T, TO>(d, g, v, O);
 }
 
+// one descriptor, no K split
 template <typename T>
 __global__ void __launch_bounds__(64 * kWaves)
     gm_single(GmDesc d, int64_t ntasks, const cplx<T>* __restrict__ X, const cplx<T>* __restrict__ M, cplx<T>* O) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;
-  gm_task<T, T, T>(d, task, X, M, O, O, 1.0, 0.0);  // no K split here: the workspace is never written
+  const GmTile g = gm_tile(d, task);
+  if (d.op == GM_COPY) {
+    gm_copy<T, T>(d, g, X, O, 1.0, 0.0);
+    return;
+  }
+  cplx<T> v[4];
+  gm_product<T, T, T>(d, g, X, M, 1.0, 0.0, v);
+  gm_store<T, T>(d, g, v, O);
 }
 
 // --------------------------------------------------------- hole contraction
 // Output tile: rows j (bra index), columns l (ket index); K = flattened (b, a).
+struct GcTile {
+  int jt, lt, ks;
+};
+__device__ inline GcTile gc_tile(const GcDesc& d, int64_t t) {
+  const int split = d.ksplit > 1 ? d.ksplit : 1;
+  GcTile g;
+  g.ks = (int)(t % split);
+  t /= split;
+  const int TJ = (d.nx + 15) >> 4;
+  g.jt = (int)(t % TJ);
+  g.lt = (int)(t / TJ);
+  return g;
+}
+
+// This chunk's share of the tile in the output precision: v[q] is row
+// jt*16 + drow(lane, q), column lt*16 + lane % 16.
 template <typename T, typename TO>
-__device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __restrict__ X,
-                               const cplx<T>* __restrict__ Y, cplx<TO>* O, cplx<TO>* work) {
+__device__ inline void gc_product(const GcDesc& d, const GcTile& g, const cplx<T>* __restrict__ X,
+                                  const cplx<T>* __restrict__ Y, cplx<TO> (&v)[4]) {
   typedef typename MF<T>::v4 v4;
   const int lane = threadIdx.x & 63;
   const int split = d.ksplit > 1 ? d.ksplit : 1;
-  const int ks = (int)(t % split);
-  t /= split;
-  const int TJ = (d.nx + 15) >> 4;
-  const int jt = (int)(t % TJ), lt = (int)(t / TJ);
-  const int64_t bef = d.bef, K = bef * d.aft;
+  // K = bef * aft < 2^31 for every node tensor, so the k walk runs in 32 bits
+  const int bef = d.bef, K = d.bef * d.aft;
   const int nx = d.nx, ny = d.ny;
   // K range of this chunk (a multiple of 4 long, except the last)
-  const int64_t chunk = ((K + split - 1) / split + 3) & ~(int64_t)3;
-  const int64_t kbeg = ks * chunk, kend = std::min(K, kbeg + chunk);
+  const int chunk = ((K + split - 1) / split + 3) & ~3;
+  const int kbeg = g.ks * chunk, kend = min(K, kbeg + chunk);
   const int li = lane & 15, kk = lane >> 4;
-  const int jx = jt * 16 + li, ly = lt * 16 + li;
+  const int jx = g.jt * 16 + li, ly = g.lt * 16 + li;
   const bool xv = jx < nx, yv = ly < ny;
   const cplx<T>* xp = X + d.x + (int64_t)jx * bef;
   const cplx<T>* yp = Y + d.y + (int64_t)ly * bef;
-  const int64_t xs = bef * nx, ys = bef * ny;  // stride of a
-  int64_t kc = kbeg + kk;
-  int64_t b = kc % bef, a = kc / bef;
+  const int xs = bef * nx, ys = bef * ny;  // stride of a
+  int kc = kbeg + kk;
+  int b = kc % bef, a = kc / bef;
   v4 cr0 = {0, 0, 0, 0}, ci0 = {0, 0, 0, 0}, cr1 = {0, 0, 0, 0}, ci1 = {0, 0, 0, 0};
   const cplx<T> z = czero<T>();
   // four k-steps per iteration; the loads of the next iteration are issued
@@ -258,20 +309,19 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const bool kv = kc < kend;
-      xr[u] = (xv && kv) ? xp[b + a * xs] : z;
-      yr[u] = (yv && kv) ? yp[b + a * ys] : z;
+      xr[u] = (xv && kv) ? xp[b + (int64_t)a * xs] : z;
+      yr[u] = (yv && kv) ? yp[b + (int64_t)a * ys] : z;
       kc += 4;
       b += 4;
-      if (b >= bef) {
-        const int64_t q = b / bef;
-        a += q;
-        b -= q * bef;
+      while (b >= bef) {  // at most 4 / bef rounds
+        b -= bef;
+        ++a;
       }
     }
   };
   cplx<T> xc[4], yc[4];
   if (kbeg < kend) load4(xc, yc);
-  for (int64_t k0 = kbeg; k0 < kend; k0 += 16) {
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
     cplx<T> xn[4], yn[4];
     const bool more = k0 + 16 < kend;
     if (more) load4(xn, yn);
@@ -289,19 +339,23 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
       }
     }
   }
-  const int col = ly;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = mk<TO>((TO)(cr0[q] + cr1[q]), (TO)(ci0[q] + ci1[q]));
+}
+
+template <typename T, typename TO>
+__device__ inline void gc_store(const GcDesc& d, const GcTile& g, const cplx<TO> (&v)[4], cplx<TO>* O) {
+  const int lane = threadIdx.x & 63;
+  const int col = g.lt * 16 + (lane & 15);
+  if (col >= d.ny) return;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int row = jt * 16 + MF<T>::drow(lane, q);
-    if (row < nx && yv) {
-      cplx<TO> v = mk<TO>((TO)(cr0[q] + cr1[q]), (TO)(ci0[q] + ci1[q]));
-      const int64_t e = row + (int64_t)col * nx;
-      if (split > 1) {
-        work[d.part + (int64_t)ks * nx * ny + e] = v;
-      } else {
-        if (d.acc) v = cadd(v, O[d.out + e]);
-        O[d.out + e] = v;
-      }
+    const int row = g.jt * 16 + MF<T>::drow(lane, q);
+    if (row < d.nx) {
+      const int64_t e = d.out + row + (int64_t)col * d.nx;
+      cplx<TO> x = v[q];
+      if (d.acc) x = cadd(x, O[e]);
+      O[e] = x;
     }
   }
 }
@@ -309,38 +363,35 @@ __device__ inline void gc_task(const GcDesc& d, int64_t t, const cplx<T>* __rest
 template <typename T, typename TO>
 __global__ void __launch_bounds__(64 * kWaves)
     gc_grouped(const GcDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
-               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, cplx<TO>* O, cplx<TO>* work,
+               const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, cplx<TO>* O,
                const uint8_t* __restrict__ active) {
+  __shared__ cplx<TO> red[kWaves * 256];
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-  if (task >= ntasks) return;
-  const GcDesc d = descs[task_desc(descs, ndesc, wgdesc, task)];
-  if (active && d.node >= 0 && !active[d.node]) return;
-  gc_task<T, TO>(d, task - d.tile0, (d.sel & 1) ? X1 : X0, (d.sel & 2) ? X1 : X0, O, work);
+  bool mine = task < ntasks;  // uniform per wavefront
+  GcDesc d{};
+  if (mine) {
+    d = descs[task_desc(descs, ndesc, wgdesc, task)];
+    mine = !(active && d.node >= 0 && !active[d.node]);
+  }
+  GcTile g{0, 0, 0};
+  if (mine) g = gc_tile(d, task - d.tile0);
+  const int split = mine && d.ksplit > 1 ? d.ksplit : 1;
+  cplx<TO> v[4];
+  if (mine) gc_product<T, TO>(d, g, (d.sel & 1) ? X1 : X0, (d.sel & 2) ? X1 : X0, v);
+  reduce_chunks<TO>(red, mine, split, g.ks, v);
+  if (mine && g.ks == 0) gc_store<T, TO>(d, g, v, O);
 }
 
+// one descriptor, no K split
 template <typename T>
 __global__ void __launch_bounds__(64 * kWaves)
     gc_single(GcDesc d, int64_t ntasks, const cplx<T>* __restrict__ X, const cplx<T>* __restrict__ Y, cplx<T>* O) {
   const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (task >= ntasks) return;
-  gc_task<T, T>(d, task, X, Y, O, O);  // no K split here: the workspace is never written
-}
-
-// as gm_reduce_kernel, for hole contractions
-template <typename T>
-__global__ void __launch_bounds__(256)
-    gc_reduce_kernel(const GcDesc* __restrict__ descs, int ndesc, cplx<T>* O, const cplx<T>* __restrict__ work,
-                     const uint8_t* __restrict__ active) {
-  const GcDesc d = descs[find_desc(descs, ndesc, blockIdx.x)];
-  if (active && d.node >= 0 && !active[d.node]) return;
-  const int64_t n = (int64_t)d.nx * d.ny;
-  const int64_t e = (blockIdx.x - d.tile0) * 256 + threadIdx.x;
-  if (e < n) {
-    cplx<T> s = work[d.part + e];
-    for (int c = 1; c < d.ksplit; ++c) s = cadd(s, work[d.part + c * n + e]);
-    if (d.acc) s = cadd(s, O[d.out + e]);
-    O[d.out + e] = s;
-  }
+  const GcTile g = gc_tile(d, task);
+  cplx<T> v[4];
+  gc_product<T, T>(d, g, X, Y, v);
+  gc_store<T, T>(d, g, v, O);
 }
 
 // ------------------------------------------------------------------- solve
@@ -435,31 +486,33 @@ __global__ void __launch_bounds__(256)
 
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
-              double sim, hipStream_t s, const KernelTiming* kt) {
+              const void* x1, const void* mbase, void* obase, const uint8_t* active, double sre, double sim,
+              hipStream_t s, const KernelTiming* kt) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
   launch_timed(gm_grouped<T, TM, TO>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc, ntasks,
-               (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase, (cplx<TO>*)obase, (cplx<TO>*)part,
-               active, sre, sim);
+               (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase, (cplx<TO>*)obase, active, sre, sim);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s,
-              const KernelTiming* kt) {
+              const void* x1, void* obase, const uint8_t* active, hipStream_t s, const KernelTiming* kt) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
   launch_timed(gc_grouped<T, TO>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc, ntasks,
-               (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, (cplx<TO>*)work, active);
+               (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 template <typename T>
 int launch_gm_single(const GmDesc& d, const void* x, const void* m, void* o, hipStream_t s) {
+  if (d.ksplit > 1) {
+    set_error("single mode product: no K split");
+    return QTG_EARG;
+  }
   const int64_t nt = gm_tasks(d);
   if (nt <= 0) return QTG_OK;
   hipLaunchKernelGGL(gm_single<T>, dim3((unsigned)((nt + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0, s, d, nt,
@@ -471,33 +524,13 @@ int launch_gm_single(const GmDesc& d, const void* x, const void* m, void* o, hip
 template <typename T>
 int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hipStream_t s) {
   if (d.ksplit > 1) {
-    set_error("single hole contraction: no K split without a workspace");
+    set_error("single hole contraction: no K split");
     return QTG_EARG;
   }
   const int64_t nt = gc_tasks(d);
   if (nt <= 0) return QTG_OK;
   hipLaunchKernelGGL(gc_single<T>, dim3((unsigned)((nt + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0, s, d, nt,
                      (const cplx<T>*)x, (const cplx<T>*)y, (cplx<T>*)o);
-  QTG_LAUNCH_CHECK();
-  return QTG_OK;
-}
-
-template <typename TO>
-int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* part,
-                     const uint8_t* active, hipStream_t s, const KernelTiming* kt) {
-  if (ndesc <= 0 || nblocks <= 0) return QTG_OK;
-  launch_timed(gm_reduce_kernel<TO>, dim3((unsigned)nblocks), dim3(256), 0, s, kt, descs, (int)ndesc,
-               (cplx<TO>*)obase, (const cplx<TO>*)part, active);
-  QTG_LAUNCH_CHECK();
-  return QTG_OK;
-}
-
-template <typename T>
-int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* work,
-                     const uint8_t* active, hipStream_t s, const KernelTiming* kt) {
-  if (ndesc <= 0 || nblocks <= 0) return QTG_OK;
-  launch_timed(gc_reduce_kernel<T>, dim3((unsigned)nblocks), dim3(256), 0, s, kt, descs, (int)ndesc,
-               (cplx<T>*)obase, (const cplx<T>*)work, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -511,35 +544,25 @@ int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax
   return QTG_OK;
 }
 
-template int launch_gm<double, double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
-                                               const void*, const void*, void*, void*, const uint8_t*, double,
-                                               double, hipStream_t, const KernelTiming*);
-template int launch_gm<float, float, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
-                                            const void*, const void*, void*, void*, const uint8_t*, double, double,
-                                            hipStream_t, const KernelTiming*);
-template int launch_gm<float, double, float>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
-                                             const void*, const void*, void*, void*, const uint8_t*, double,
-                                             double, hipStream_t, const KernelTiming*);
-template int launch_gm<float, double, double>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*,
-                                              const void*, const void*, void*, void*, const uint8_t*, double,
-                                              double, hipStream_t, const KernelTiming*);
-template int launch_gc<double, double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
-                                       void*, void*, const uint8_t*, hipStream_t, const KernelTiming*);
-template int launch_gc<float, float>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
-                                     void*, void*, const uint8_t*, hipStream_t, const KernelTiming*);
-template int launch_gc<float, double>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,
-                                      void*, void*, const uint8_t*, hipStream_t, const KernelTiming*);
+#define QTG_GM(T, TM, TO)                                                                                      \
+  template int launch_gm<T, TM, TO>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,  \
+                                    const void*, void*, const uint8_t*, double, double, hipStream_t,            \
+                                    const KernelTiming*);
+QTG_GM(double, double, double)
+QTG_GM(float, float, float)
+QTG_GM(float, double, float)
+QTG_GM(float, double, double)
+#undef QTG_GM
+#define QTG_GC(T, TO)                                                                                           \
+  template int launch_gc<T, TO>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*, void*, \
+                                const uint8_t*, hipStream_t, const KernelTiming*);
+QTG_GC(double, double)
+QTG_GC(float, float)
+QTG_GC(float, double)
+#undef QTG_GC
 template int launch_gm_single<double>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gm_single<float>(const GmDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gc_single<double>(const GcDesc&, const void*, const void*, void*, hipStream_t);
 template int launch_gc_single<float>(const GcDesc&, const void*, const void*, void*, hipStream_t);
-template int launch_gm_reduce<double>(const GmDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                      hipStream_t, const KernelTiming*);
-template int launch_gm_reduce<float>(const GmDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                     hipStream_t, const KernelTiming*);
-template int launch_gc_reduce<double>(const GcDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                      hipStream_t, const KernelTiming*);
-template int launch_gc_reduce<float>(const GcDesc*, int64_t, int64_t, void*, const void*, const uint8_t*,
-                                     hipStream_t, const KernelTiming*);
 
 }  // namespace qtg

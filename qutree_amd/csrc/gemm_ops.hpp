@@ -6,9 +6,11 @@
 // Each descriptor is split into 16 x 16 output tiles; one wavefront computes
 // one tile with real-split complex MFMA (four real 16x16x4 products per
 // complex k-step).  A launch walks a flat task list: descriptors are sorted by
-// their first task index tile0, and a wavefront finds its descriptor by binary
-// search.
+// their first task index tile0, and a wavefront finds its descriptor from a
+// per-workgroup table.  Long reductions split along K into 2 or 4 chunks
+// (more wavefronts in flight) that sum in LDS inside their workgroup.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -34,9 +36,8 @@ struct GmDesc {
   double are, aim;                  // alpha
   int32_t node;                     // index into the active mask (-1: always run)
   int32_t xsel;                     // X from the first (0) or second (1) input base
-  int32_t ksplit;                   // K chunks; > 1 writes partial tiles (alpha applied)
+  int32_t ksplit;                   // K chunks per tile: 1, 2 or 4 (see order_tasks)
   int32_t pad;
-  int64_t part;                     // partial-tile offset (R x nout per chunk)
   int64_t tile0;                    // first task of this descriptor
 };
 
@@ -45,9 +46,8 @@ struct GcDesc {
   int32_t bef, nx, ny, aft;         // X is (bef, nx, aft), Y is (bef, ny, aft)
   int32_t acc;                      // 1: accumulate into out
   int32_t node;                     // active-mask index (-1: always run)
-  int32_t ksplit;                   // K chunks; > 1 writes partial tiles to the workspace
+  int32_t ksplit;                   // K chunks per tile: 1, 2 or 4 (see order_tasks)
   int32_t sel;                      // bit 0: X from the second input base; bit 1: Y likewise
-  int64_t part;                     // workspace offset of the ksplit partial outputs
   int64_t tile0;
 };
 
@@ -65,21 +65,40 @@ inline int64_t gc_tasks(const GcDesc& d) {
 // products accumulate in T.  `active` may be null.  Every alpha of a
 // mode-product launch is multiplied by the complex scale (sre, sim).
 // wgdesc[b]: descriptor of the first task (kGemmWaves * b) of workgroup b
-// (see gemm_wg_table).
+// (see gemm_wg_table).  The chunks of a split tile sum in LDS inside their
+// workgroup, which needs the task order of order_tasks.  No two descriptors
+// of one launch may write the same output.
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, const void* mbase, void* obase, void* part, const uint8_t* active, double sre,
-              double sim, hipStream_t s, const KernelTiming* kt = nullptr);
-// Sums the ksplit partial tiles (cplx<TO>) of split mode-product descriptors
-// into their outputs.  These descriptor copies carry in tile0 their first
-// block of 256 output elements; nblocks is the total.
-template <typename TO>
-int launch_gm_reduce(const GmDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* part,
-                     const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr);
+              const void* x1, const void* mbase, void* obase, const uint8_t* active, double sre, double sim,
+              hipStream_t s, const KernelTiming* kt = nullptr);
 template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, void* obase, void* work, const uint8_t* active, hipStream_t s,
-              const KernelTiming* kt = nullptr);
+              const void* x1, void* obase, const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr);
+
+// K split of a reduction of length K into kGemmWaves-bounded chunks: the
+// largest of 1, 2, 4 not above `want` whose chunks stay at least min_k long.
+inline int32_t split_chunks(int64_t K, int64_t want, int64_t min_k) {
+  int32_t s = 1;
+  while (s < kGemmWaves && 2 * s <= want && K / (2 * s) >= min_k) s *= 2;
+  return s;
+}
+
+// Orders descriptors by ksplit (4, then 2, then 1; stable within a class)
+// and assigns tile0, so that the chunks of every split tile are consecutive
+// tasks starting at a multiple of ksplit, i.e. wavefronts of one workgroup.
+// Returns the launch's task count.
+template <typename D>
+inline int64_t order_tasks(std::vector<D>& descs, int64_t (*tasks)(const D&)) {
+  std::stable_sort(descs.begin(), descs.end(),
+                   [](const D& a, const D& b) { return (a.ksplit > 1 ? a.ksplit : 1) > (b.ksplit > 1 ? b.ksplit : 1); });
+  int64_t t = 0;
+  for (auto& d : descs) {
+    d.tile0 = t;
+    t += tasks(d);
+  }
+  return t;
+}
 
 // Appends to `out` the first-descriptor index of every workgroup of a launch
 // over `descs` (tile0 already assigned, ntasks in total).
@@ -96,10 +115,6 @@ template <typename T>
 int launch_gm_single(const GmDesc& d, const void* x, const void* m, void* o, hipStream_t s);
 template <typename T>
 int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hipStream_t s);
-// As launch_gm_reduce, for hole contractions.
-template <typename T>
-int launch_gc_reduce(const GcDesc* descs, int64_t ndesc, int64_t nblocks, void* obase, const void* work,
-                     const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr);
 
 // Batched FP64 solve X <- S^-1 X (Gauss-Jordan with partial pivoting) for n x n
 // column-major S and X at pool offsets s, x; n <= 64.  When r >= 0 it also

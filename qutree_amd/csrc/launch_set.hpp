@@ -12,24 +12,20 @@ namespace qtg {
 struct LaunchSet {
   struct Launch {
     int kind;  // 0 mode products, 1 hole contractions
-    int64_t d0, nd, ntasks, s0, ns, nrb, wg0;
+    int64_t d0, nd, ntasks, wg0;
     int x0, x1, m, o;
     double flops;
   };
-  std::vector<GmDesc> gm, gmsplit;
-  std::vector<GcDesc> gc, gcsplit;
+  std::vector<GmDesc> gm;
+  std::vector<GcDesc> gc;
   std::vector<int32_t> gmwg, gcwg;
   std::vector<Launch> launches;
-  int64_t partSize = 0;
   double flops = 0.0;
 
   GmDesc* d_gm = nullptr;
   GcDesc* d_gc = nullptr;
-  GmDesc* d_gmsplit = nullptr;
-  GcDesc* d_gcsplit = nullptr;
   int32_t* d_gmwg = nullptr;
   int32_t* d_gcwg = nullptr;
-  void* d_part = nullptr;
 
   ~LaunchSet() { release(); }
   void release();

@@ -4,8 +4,8 @@
 // run on the MFMA tile kernels of gemm_ops.hip: a mode product is a GEMM over
 // the flattened (before, after) rows, a hole contraction a GEMM over the
 // flattened (before, after) reduction index.  Grouped hole contractions with
-// long reductions are split along K into partial tiles that a second kernel
-// sums in chunk order (deterministic, no atomics).
+// long reductions are split along K into 2 or 4 chunks that sum in LDS inside
+// their workgroup, in chunk order (deterministic).
 #include <algorithm>
 #include <climits>
 #include <vector>
@@ -100,14 +100,12 @@ int hole_one(const qtg_hole_contraction_desc& d, cplx<R>* Sm, const cplx<R>* A, 
 struct Plan {
   int kind = 0;  // 0 mode product, 1 hole contraction
   bool c64 = false;
-  int64_t ndesc = 0, ntasks = 0, nsplit = 0, nblocks = 0;
+  int64_t ndesc = 0, ntasks = 0;
   void* d_descs = nullptr;
-  void* d_wg = nullptr;     // first descriptor of every workgroup
-  void* d_split = nullptr;  // split hole descriptors (reduce pass)
-  void* d_work = nullptr;
+  void* d_wg = nullptr;  // first descriptor of every workgroup
   double flops = 0, bytes = 0;
   ~Plan() {
-    for (void* p : {d_descs, d_wg, d_split, d_work})
+    for (void* p : {d_descs, d_wg})
       if (p) (void)hipFree(p);
   }
 };
@@ -218,8 +216,7 @@ int qtg_hole_contraction_plan_create(qtg_plan* plan, const qtg_hole_contraction_
   p->kind = 1;
   p->c64 = complex64 != 0;
   const double es = p->c64 ? 8.0 : 16.0;
-  std::vector<GcDesc> gv, split;
-  int64_t tasks = 0, work = 0, blocks = 0;
+  std::vector<GcDesc> gv;
   for (int64_t i = 0; i < count; ++i) {
     GcDesc g;
     int rc = to_gc(descs[i], g);
@@ -228,39 +225,23 @@ int qtg_hole_contraction_plan_create(qtg_plan* plan, const qtg_hole_contraction_
       return rc;
     }
     const int64_t K = (int64_t)g.bef * g.aft;
-    if (K > 2 * kSplitK) {
-      g.ksplit = (int32_t)((K + kSplitK - 1) / kSplitK);
-      g.part = work;
-      work += (int64_t)g.ksplit * g.nx * g.ny;
-      GcDesc r = g;
-      r.tile0 = blocks;  // reduce blocks of 256 output elements
-      blocks += ((int64_t)g.nx * g.ny + 255) / 256;
-      split.push_back(r);
-    }
-    g.tile0 = tasks;
-    tasks += gc_tasks(g);
+    if (K > 2 * kSplitK) g.ksplit = split_chunks(K, kGemmWaves, 1);
     gv.push_back(g);
     const auto& d = descs[i];
     p->flops += 8.0 * K * d.nA * d.nB;
     p->bytes += es * (K * (d.nA + d.nB) + d.nA * d.nB * (d.accumulate ? 2 : 1));
   }
+  const int64_t tasks = order_tasks(gv, gc_tasks);
   std::vector<int32_t> wg;
   gemm_wg_table(gv.data(), (int64_t)gv.size(), tasks, wg);
   int rc = upload(&p->d_descs, gv);
   if (!rc) rc = upload(&p->d_wg, wg);
-  if (!rc) rc = upload(&p->d_split, split);
-  if (!rc && work > 0 && hipMalloc(&p->d_work, (size_t)work * (p->c64 ? 8 : 16)) != hipSuccess) {
-    set_error("plan: workspace allocation failed");
-    rc = QTG_ENOMEM;
-  }
   if (rc) {
     delete p;
     return rc;
   }
   p->ndesc = count;
   p->ntasks = tasks;
-  p->nsplit = (int64_t)split.size();
-  p->nblocks = blocks;
   *plan = p;
   return QTG_OK;
 }
@@ -278,22 +259,16 @@ int qtg_plan_execute(qtg_plan plan, void* out_base, const void* first_base, cons
     // first = M base, second = B base
     const auto* g = (const GmDesc*)p->d_descs;
     const auto* wg = (const int32_t*)p->d_wg;
-    return p->c64 ? launch_gm<float, float, float>(g, wg, p->ndesc, p->ntasks, second_base, second_base, first_base,
-                                            out_base, out_base, nullptr, 1.0, 0.0, s)
-                  : launch_gm<double, double, double>(g, wg, p->ndesc, p->ntasks, second_base, second_base, first_base,
-                                              out_base, out_base, nullptr, 1.0, 0.0, s);
+    return p->c64 ? launch_gm<float, float, float>(g, wg, p->ndesc, p->ntasks, second_base, second_base,
+                                                   first_base, out_base, nullptr, 1.0, 0.0, s)
+                  : launch_gm<double, double, double>(g, wg, p->ndesc, p->ntasks, second_base, second_base,
+                                                      first_base, out_base, nullptr, 1.0, 0.0, s);
   }
   const auto* g = (const GcDesc*)p->d_descs;
   // X (bra) from the first base, Y (ket) from the second: sel = 2 on every descriptor
   const auto* wg = (const int32_t*)p->d_wg;
-  int rc = p->c64 ? launch_gc<float, float>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, p->d_work,
-                                     nullptr, s)
-                  : launch_gc<double, double>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, p->d_work,
-                                      nullptr, s);
-  if (rc || p->nsplit == 0) return rc;
-  const auto* sp = (const GcDesc*)p->d_split;
-  return p->c64 ? launch_gc_reduce<float>(sp, p->nsplit, p->nblocks, out_base, p->d_work, nullptr, s)
-                : launch_gc_reduce<double>(sp, p->nsplit, p->nblocks, out_base, p->d_work, nullptr, s);
+  return p->c64 ? launch_gc<float, float>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, nullptr, s)
+                : launch_gc<double, double>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, nullptr, s);
 }
 
 int qtg_plan_destroy(qtg_plan plan) {
