@@ -67,6 +67,20 @@ constexpr int kWaves = kGemmWaves;  // wavefronts (tasks) per workgroup
 // Descriptor of a task: wgdesc[blockIdx.x] is the descriptor of the
 // workgroup's first task (precomputed on the host), and a task lies at most
 // kWaves - 1 descriptors further.
+// Wavefront index in the workgroup, as a wave-uniform (scalar) value, so that
+// descriptor loads and address arithmetic stay on the scalar unit.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// base[off] when ok, else zero.  The load itself is unconditional, from
+// base[0] when !ok (base is the start of an operand, always in bounds): a
+// guarded load would let the compiler select between the operand and a stack
+// zero and fall back to generic (flat) loads.
+template <typename T>
+__device__ inline cplx<T> ldz(const cplx<T>* __restrict__ base, int64_t off, bool ok) {
+  const cplx<T> v = base[ok ? off : 0];
+  return ok ? v : czero<T>();
+}
+
 template <typename D>
 __device__ inline int task_desc(const D* __restrict__ descs, int ndesc, const int32_t* __restrict__ wgdesc,
                                 int64_t task) {
@@ -83,7 +97,7 @@ __device__ inline int task_desc(const D* __restrict__ descs, int ndesc, const in
 // wavefront of a workgroup reaches the barrier, including idle ones.
 template <typename TO>
 __device__ inline void reduce_chunks(cplx<TO>* red, bool mine, int split, int ks, cplx<TO> (&v)[4]) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = wave_id(), lane = threadIdx.x & 63;
   if (mine && split > 1) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) red[wave * 256 + q * 64 + lane] = v[q];
@@ -153,26 +167,20 @@ __device__ inline void gm_product(const GmDesc& d, const GmTile& g, const cplx<T
   // A operand: row r = rt*16 + li of the flattened (b, a), column k
   const int64_t r = g.rt * 16 + li;
   const bool rv = r < R;
-  const int64_t rb = rv ? r % bef : 0, ra = rv ? r / bef : 0;
-  const cplx<T>* xa = X + d.x + rb + ra * bef * nin;
+  const int64_t xoff = rv ? r % bef + (r / bef) * bef * nin : 0;
+  const cplx<T>* xa = X + d.x;
   // B operand: op(M)(j, k) for column j = ct*16 + li
   const int j = g.ct * 16 + li;
   const bool cv = j < nout;
-  const cplx<TM>* mb;
-  int64_t mks;
-  if (d.op == GM_N) {
-    mb = M + d.m + j;  // M(j, k) at j + k nout
-    mks = nout;
-  } else {
-    mb = M + d.m + (int64_t)j * nin;  // M(k, j) at k + j nin
-    mks = 1;
-  }
+  const cplx<TM>* mb = M + d.m;
+  // M(j, k) at j + k nout (N), M(k, j) at k + j nin (T, H)
+  const int64_t moff = !cv ? 0 : d.op == GM_N ? j : (int64_t)j * nin;
+  const int64_t mks = d.op == GM_N ? nout : 1;
   const bool conjm = d.op == GM_H;
   // K range of this chunk (a multiple of 4 long, except the last)
   const int kchunk = ((nin + split - 1) / split + 3) & ~3;
   const int kbeg = g.ks * kchunk, kend = min(nin, kbeg + kchunk);
   v4 cr0 = {0, 0, 0, 0}, ci0 = {0, 0, 0, 0}, cr1 = {0, 0, 0, 0}, ci1 = {0, 0, 0, 0};
-  const cplx<T> z = czero<T>();
   // four k-steps per iteration: all loads are issued before the MFMAs
   for (int k0 = kbeg; k0 < kend; k0 += 16) {
     cplx<T> av[4], bv[4];
@@ -180,14 +188,9 @@ __device__ inline void gm_product(const GmDesc& d, const GmTile& g, const cplx<T
     for (int u = 0; u < 4; ++u) {
       const int kx = k0 + 4 * u + kk;
       const bool kv = kx < kend;
-      av[u] = (rv && kv) ? xa[(int64_t)kx * bef] : z;
-      if (cv && kv) {
-        const cplx<TM> mv = mb[(int64_t)kx * mks];
-        bv[u] = mk<T>((T)mv.re, (T)mv.im);
-      } else {
-        bv[u] = z;
-      }
-      if (conjm) bv[u].im = -bv[u].im;
+      av[u] = ldz(xa, xoff + (int64_t)kx * bef, rv && kv);
+      const cplx<TM> mv = ldz(mb, moff + (int64_t)kx * mks, cv && kv);
+      bv[u] = mk<T>((T)mv.re, conjm ? -(T)mv.im : (T)mv.im);
     }
     cmma<T>(cr0, ci0, av[0], bv[0]);
     if (k0 + 4 < kend) cmma<T>(cr1, ci1, av[1], bv[1]);
@@ -226,7 +229,7 @@ __global__ void __launch_bounds__(64 * kWaves)
                const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, const cplx<TM>* __restrict__ M,
                cplx<TO>* O, const uint8_t* __restrict__ active, double sre, double sim) {
   __shared__ cplx<TO> red[kWaves * 256];
-  const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t task = (int64_t)blockIdx.x * kWaves + wave_id();
   bool mine = task < ntasks;  // uniform per wavefront
   GmDesc d{};
   if (mine) {
@@ -251,7 +254,7 @@ __global__ void __launch_bounds__(64 * kWaves)
 template <typename T>
 __global__ void __launch_bounds__(64 * kWaves)
     gm_single(GmDesc d, int64_t ntasks, const cplx<T>* __restrict__ X, const cplx<T>* __restrict__ M, cplx<T>* O) {
-  const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t task = (int64_t)blockIdx.x * kWaves + wave_id();
   if (task >= ntasks) return;
   const GmTile g = gm_tile(d, task);
   if (d.op == GM_COPY) {
@@ -296,27 +299,29 @@ __device__ inline void gc_product(const GcDesc& d, const GcTile& g, const cplx<T
   const int li = lane & 15, kk = lane >> 4;
   const int jx = g.jt * 16 + li, ly = g.lt * 16 + li;
   const bool xv = jx < nx, yv = ly < ny;
-  const cplx<T>* xp = X + d.x + (int64_t)jx * bef;
-  const cplx<T>* yp = Y + d.y + (int64_t)ly * bef;
+  const cplx<T>* xp = X + d.x;
+  const cplx<T>* yp = Y + d.y;
+  const int64_t xrow = (int64_t)jx * bef, yrow = (int64_t)ly * bef;
   const int xs = bef * nx, ys = bef * ny;  // stride of a
+  // k = b + a bef advances by 4 per load: b by 4 % bef, a by 4 / bef, plus a carry
+  const int q4 = 4 / bef, r4 = 4 % bef;
   int kc = kbeg + kk;
   int b = kc % bef, a = kc / bef;
   v4 cr0 = {0, 0, 0, 0}, ci0 = {0, 0, 0, 0}, cr1 = {0, 0, 0, 0}, ci1 = {0, 0, 0, 0};
-  const cplx<T> z = czero<T>();
   // four k-steps per iteration; the loads of the next iteration are issued
   // before the MFMAs of the current one (double-buffered registers)
   auto load4 = [&](cplx<T>* xr, cplx<T>* yr) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const bool kv = kc < kend;
-      xr[u] = (xv && kv) ? xp[b + (int64_t)a * xs] : z;
-      yr[u] = (yv && kv) ? yp[b + (int64_t)a * ys] : z;
+      xr[u] = ldz(xp, xrow + b + (int64_t)a * xs, xv && kv);
+      yr[u] = ldz(yp, yrow + b + (int64_t)a * ys, yv && kv);
       kc += 4;
-      b += 4;
-      while (b >= bef) {  // at most 4 / bef rounds
-        b -= bef;
-        ++a;
-      }
+      b += r4;
+      a += q4;
+      const bool carry = b >= bef;
+      b = carry ? b - bef : b;
+      a = carry ? a + 1 : a;
     }
   };
   cplx<T> xc[4], yc[4];
@@ -366,7 +371,7 @@ __global__ void __launch_bounds__(64 * kWaves)
                const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, cplx<TO>* O,
                const uint8_t* __restrict__ active) {
   __shared__ cplx<TO> red[kWaves * 256];
-  const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t task = (int64_t)blockIdx.x * kWaves + wave_id();
   bool mine = task < ntasks;  // uniform per wavefront
   GcDesc d{};
   if (mine) {
@@ -386,7 +391,7 @@ __global__ void __launch_bounds__(64 * kWaves)
 template <typename T>
 __global__ void __launch_bounds__(64 * kWaves)
     gc_single(GcDesc d, int64_t ntasks, const cplx<T>* __restrict__ X, const cplx<T>* __restrict__ Y, cplx<T>* O) {
-  const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t task = (int64_t)blockIdx.x * kWaves + wave_id();
   if (task >= ntasks) return;
   const GcTile g = gc_tile(d, task);
   cplx<T> v[4];
