@@ -129,7 +129,9 @@ int qtg_plan_counts(qtg_plan plan, double* flops, double* bytes);
  * leaves).  The wave function is the concatenation of the node tensors in
  * node-address order (post-order, last child first, Tree.cpp:92-115), complex128
  * interleaved.  precision: 0 = complex128 throughout, 1 = complex64 bulk
- * contractions with FP64 density inverse, projector and derivative output. */
+ * contractions with FP64 density inverse, projector and derivative output,
+ * 2 = as 1 but with the slots of the large-coefficient terms (|c| >= 0.1
+ * max |c|) and the densities in FP64 (grouped path only). */
 typedef void* qtg_engine;
 int qtg_engine_create(const char* tree_text, int precision, qtg_engine* engine);
 int qtg_engine_destroy(qtg_engine engine);
