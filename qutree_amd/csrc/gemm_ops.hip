@@ -487,6 +487,78 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Systems with n <= 48: ping-pong Gauss-Jordan.  Each column step reads the
+// previous [S | X] buffer and writes the next one, so a step is one barrier:
+// every thread finds the pivot itself (broadcast reads of the pivot column,
+// largest modulus, lowest row on ties), then forms its elements of the
+// swapped, scaled and eliminated matrix directly from the old values.  The
+// arithmetic per element is solve_kernel's.
+__global__ void __launch_bounds__(256)
+    solve_pp_kernel(const SolveItem* __restrict__ items, cplx<double>* pool, const uint8_t* __restrict__ active) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const SolveItem it = items[blockIdx.x];
+  if (active && it.node >= 0 && !active[it.node]) return;
+  const int n = it.n, nn = n * n;
+  cplx<double>* buf[2] = {reinterpret_cast<cplx<double>*>(smem_raw),
+                          reinterpret_cast<cplx<double>*>(smem_raw) + 2 * nn};
+  for (int o = threadIdx.x; o < nn; o += 256) {
+    buf[0][o] = pool[it.s + o];
+    buf[0][nn + o] = pool[it.x + o];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int c = 0; c < n; ++c) {
+    const cplx<double>* M = buf[cur];
+    cplx<double>* N = buf[cur ^ 1];
+    const cplx<double>* Mc = M + c * n;  // pivot column
+    int p = c;
+    double best = -1.0;
+    for (int r = c; r < n; ++r) {
+      const cplx<double> v = Mc[r];
+      const double a = v.re * v.re + v.im * v.im;
+      if (a > best) {
+        best = a;
+        p = r;
+      }
+    }
+    const cplx<double> dg = Mc[p];
+    const double dd = dg.re * dg.re + dg.im * dg.im;
+    const cplx<double> dinv = mk<double>(dg.re / dd, -dg.im / dd);
+    for (int o = threadIdx.x; o < 2 * nn; o += 256) {
+      const int r = o % n, k = o / n;
+      const cplx<double>* Mk = M + k * n;
+      // pivot row after the swap and scaling: old row p times dinv
+      const cplx<double> v = cmul(Mk[p], dinv);
+      cplx<double> w;
+      if (r == c) {
+        w = v;
+      } else {
+        const int rs = (r == p) ? c : r;  // row r after the swap
+        const cplx<double> f = Mc[rs];
+        w = Mk[rs];
+        w.re -= f.re * v.re - f.im * v.im;
+        w.im -= f.re * v.im + f.im * v.re;
+      }
+      N[o] = w;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  const cplx<double>* X = buf[cur] + nn;
+  for (int o = threadIdx.x; o < nn; o += 256) pool[it.x + o] = X[o];
+  if (it.r < 0) return;
+  // W(k, m) = sum_l R(k, l) X(m, l); R staged in the free buffer
+  cplx<double>* Rm = buf[cur ^ 1];
+  for (int o = threadIdx.x; o < nn; o += 256) Rm[o] = pool[it.r + o];
+  __syncthreads();
+  for (int o = threadIdx.x; o < nn; o += 256) {
+    const int k = o % n, m = o / n;
+    cplx<double> acc = czero<double>();
+    for (int l = 0; l < n; ++l) cfma(acc, Rm[k + l * n], X[m + l * n]);
+    pool[it.w + o] = acc;
+  }
+}
+
 }  // namespace
 
 template <typename T, typename TM, typename TO>
@@ -543,6 +615,12 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
                  hipStream_t s, const KernelTiming* kt) {
   if (n <= 0) return QTG_OK;
+  if (nmax <= 48) {
+    const size_t lds = 4 * sizeof(cplx<double>) * (size_t)nmax * nmax;
+    launch_timed(solve_pp_kernel, dim3((unsigned)n), dim3(256), (unsigned)lds, s, kt, items, pool, active);
+    QTG_LAUNCH_CHECK();
+    return QTG_OK;
+  }
   const size_t lds = 2 * sizeof(cplx<double>) * (size_t)nmax * nmax;
   launch_timed(solve_kernel, dim3((unsigned)n), dim3(256), (unsigned)lds, s, kt, items, pool, active);
   QTG_LAUNCH_CHECK();

@@ -117,8 +117,9 @@ template <typename T>
 int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hipStream_t s);
 
 // Batched FP64 solve X <- S^-1 X (Gauss-Jordan with partial pivoting) for n x n
-// column-major S and X at pool offsets s, x; n <= 64.  When r >= 0 it also
-// forms W = R X^T at w (R the n x n matrix at r).
+// column-major S and X at pool offsets s, x; n <= nmax <= 64 (one barrier
+// per column step when nmax <= 48).  When r >= 0 it also forms W = R X^T at w (R
+// the n x n matrix at r).
 struct SolveItem {
   int64_t s, x, r, w;
   int32_t n, node;
