@@ -27,6 +27,23 @@ __device__ inline double block_sum_d(double v, double* red) {
   return r;
 }
 
+// b.b[i] for a wave-uniform i.  Indexing the by-value kernel argument with a
+// runtime index would copy the whole pointer table to scratch memory; a
+// switch reads each entry from the argument segment directly.
+__device__ inline cd2* bp(const CmfBufs& b, int i) {
+  switch (i) {
+#define QTG_BP(k) \
+  case k:         \
+    return b.b[k];
+    QTG_BP(0) QTG_BP(1) QTG_BP(2) QTG_BP(3) QTG_BP(4) QTG_BP(5) QTG_BP(6) QTG_BP(7) QTG_BP(8)
+    QTG_BP(9) QTG_BP(10) QTG_BP(11) QTG_BP(12) QTG_BP(13) QTG_BP(14) QTG_BP(15) QTG_BP(16)
+#undef QTG_BP
+    default:
+      return nullptr;
+  }
+}
+static_assert(kCmfBufs == 17, "bp() covers every buffer");
+
 __global__ void __launch_bounds__(TB)
     program_kernel(const NodeProg* __restrict__ progs, const VecOp* __restrict__ ops, CmfBufs bufs,
                    const DNode* __restrict__ nodes, const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot,
@@ -40,28 +57,28 @@ __global__ void __launch_bounds__(TB)
     const VecOp op = ops[k];
     switch (op.code) {
       case V_COPY: {
-        cd2* A = bufs.b[op.a] + off;
-        const cd2* B = bufs.b[op.b] + off;
+        cd2* A = bp(bufs, op.a) + off;
+        const cd2* B = bp(bufs, op.b) + off;
         for (int e = threadIdx.x; e < N; e += TB) A[e] = B[e];
         break;
       }
       case V_AXPY: {
-        cd2* A = bufs.b[op.a] + off;
-        const cd2* B = bufs.b[op.b] + off;
-        const cd2* Cc = bufs.b[op.c] + off;
+        cd2* A = bp(bufs, op.a) + off;
+        const cd2* B = bp(bufs, op.b) + off;
+        const cd2* Cc = bp(bufs, op.c) + off;
         for (int e = threadIdx.x; e < N; e += TB)
           A[e] = mk<double>(B[e].re + op.h * Cc[e].re, B[e].im + op.h * Cc[e].im);
         break;
       }
       case V_ZERO: {
-        cd2* A = bufs.b[op.a] + off;
+        cd2* A = bp(bufs, op.a) + off;
         for (int e = threadIdx.x; e < N; e += TB) A[e] = czero<double>();
         break;
       }
       case V_MMID_STEP: {
-        cd2* y0 = bufs.b[B_Y0] + off;
-        cd2* y1 = bufs.b[B_Y0 + 1] + off;
-        cd2* y2 = bufs.b[B_Y0 + 2] + off;
+        cd2* y0 = bp(bufs, B_Y0) + off;
+        cd2* y1 = bp(bufs, B_Y0 + 1) + off;
+        cd2* y2 = bp(bufs, B_Y0 + 2) + off;
         for (int e = threadIdx.x; e < N; e += TB) {
           const cd2 t = mk<double>(y0[e].re + 2.0 * op.h * y2[e].re, y0[e].im + 2.0 * op.h * y2[e].im);
           y0[e] = y1[e];
@@ -71,9 +88,9 @@ __global__ void __launch_bounds__(TB)
         break;
       }
       case V_MMID_FINAL: {
-        cd2* y0 = bufs.b[B_Y0] + off;
-        const cd2* y1 = bufs.b[B_Y0 + 1] + off;
-        const cd2* y2 = bufs.b[B_Y0 + 2] + off;
+        cd2* y0 = bp(bufs, B_Y0) + off;
+        const cd2* y1 = bp(bufs, B_Y0 + 1) + off;
+        const cd2* y2 = bp(bufs, B_Y0 + 2) + off;
         for (int e = threadIdx.x; e < N; e += TB)
           y0[e] = mk<double>((y0[e].re + y1[e].re + op.h * y2[e].re) / 2.0,
                              (y0[e].im + y1[e].im + op.h * y2[e].im) / 2.0);
@@ -81,10 +98,10 @@ __global__ void __launch_bounds__(TB)
       }
       case V_RZEXTR: {
         // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2
-        cd2* yest = bufs.b[B_Y0] + off;
-        cd2* ysav = bufs.b[B_Y0 + 1] + off;
-        cd2* dy = bufs.b[B_Y0 + 2] + off;
-        cd2* yt0 = bufs.b[B_YT0] + off;
+        cd2* yest = bp(bufs, B_Y0) + off;
+        cd2* ysav = bp(bufs, B_Y0 + 1) + off;
+        cd2* dy = bp(bufs, B_Y0 + 2) + off;
+        cd2* yt0 = bp(bufs, B_YT0) + off;
         for (int e = threadIdx.x; e < N; e += TB) {
           if (op.i == 0) {
             const cd2 v = yest[e];
@@ -99,7 +116,8 @@ __global__ void __launch_bounds__(TB)
           cd2 c = yy;
           yt0[e] = yy;
           for (int k2 = 1; k2 < op.mi; ++k2) {
-            const cd2 bi = mk<double>(op.fx[k2] * v.re, op.fx[k2] * v.im);
+            const double fx = ops[k].fx[k2];  // from memory: a dynamic index into a copy would go to scratch
+            const cd2 bi = mk<double>(fx * v.re, fx * v.im);
             cd2 b = mk<double>(bi.re - c.re, bi.im - c.im);
             if (b.re != 0.0 || b.im != 0.0) {
               b = cdiv(mk<double>(c.re - v.re, c.im - v.im), b);
@@ -108,7 +126,7 @@ __global__ void __launch_bounds__(TB)
             } else {
               ddy = v;
             }
-            cd2* ytk = bufs.b[B_YT0 + k2] + off;
+            cd2* ytk = bp(bufs, B_YT0 + k2) + off;
             v = ytk[e];
             ytk[e] = ddy;
             yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
@@ -119,9 +137,9 @@ __global__ void __launch_bounds__(TB)
         break;
       }
       case V_POST_EXTR: {
-        cd2* y0 = bufs.b[B_Y0] + off;
-        cd2* y1 = bufs.b[B_Y0 + 1] + off;
-        const cd2* y2 = bufs.b[B_Y0 + 2] + off;
+        cd2* y0 = bp(bufs, B_Y0) + off;
+        cd2* y1 = bp(bufs, B_Y0 + 1) + off;
+        const cd2* y2 = bp(bufs, B_Y0 + 2) + off;
         for (int e = threadIdx.x; e < N; e += TB) {
           const cd2 a = y1[e];
           y0[e] = a;
@@ -130,9 +148,9 @@ __global__ void __launch_bounds__(TB)
         break;
       }
       case V_ERROR: {
-        const cd2* Phi = bufs.b[op.a] + off;
-        const cd2* Chi = bufs.b[op.b] + off;
-        const int n = nd.dims[nd.order - 1], L = N / n;
+        const cd2* Phi = bp(bufs, op.a) + off;
+        const cd2* Chi = bp(bufs, op.b) + off;
+        const int n = nodes[pg.node].dims[nd.order - 1], L = N / n;
         const int64_t rs = rhoslot[pg.node];
         double delta = 0.0, nrm = 0.0;
         if (rs < 0) {
@@ -192,7 +210,7 @@ __global__ void __launch_bounds__(EW)
   __shared__ double lam[64];
   const OrthoItem it = items[blockIdx.x];
   const DNode nd = nodes[it.node];
-  const int N = nd.N, n = nd.dims[nd.order - 1], L = N / n, ld = n + 1;
+  const int N = nd.N, n = nodes[it.node].dims[nd.order - 1], L = N / n, ld = n + 1;
   for (int e = threadIdx.x; e < N; e += EW) Phi[e] = P[nd.off + e];
   __syncthreads();
   for (int o = threadIdx.x; o < n * n; o += EW) {

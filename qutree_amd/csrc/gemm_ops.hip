@@ -499,17 +499,18 @@ __global__ void __launch_bounds__(256)
   const SolveItem it = items[blockIdx.x];
   if (active && it.node >= 0 && !active[it.node]) return;
   const int n = it.n, nn = n * n;
-  cplx<double>* buf[2] = {reinterpret_cast<cplx<double>*>(smem_raw),
-                          reinterpret_cast<cplx<double>*>(smem_raw) + 2 * nn};
+  // the two buffers are addressed from the LDS base (an array of pointers
+  // would turn every access into a generic one)
+  cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
   for (int o = threadIdx.x; o < nn; o += 256) {
-    buf[0][o] = pool[it.s + o];
-    buf[0][nn + o] = pool[it.x + o];
+    lds[o] = pool[it.s + o];
+    lds[nn + o] = pool[it.x + o];
   }
   __syncthreads();
   int cur = 0;
   for (int c = 0; c < n; ++c) {
-    const cplx<double>* M = buf[cur];
-    cplx<double>* N = buf[cur ^ 1];
+    const cplx<double>* M = lds + cur * 2 * nn;
+    cplx<double>* N = lds + (cur ^ 1) * 2 * nn;
     const cplx<double>* Mc = M + c * n;  // pivot column
     int p = c;
     double best = -1.0;
@@ -544,11 +545,11 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     cur ^= 1;
   }
-  const cplx<double>* X = buf[cur] + nn;
+  const cplx<double>* X = lds + cur * 2 * nn + nn;
   for (int o = threadIdx.x; o < nn; o += 256) pool[it.x + o] = X[o];
   if (it.r < 0) return;
   // W(k, m) = sum_l R(k, l) X(m, l); R staged in the free buffer
-  cplx<double>* Rm = buf[cur ^ 1];
+  cplx<double>* Rm = lds + (cur ^ 1) * 2 * nn;
   for (int o = threadIdx.x; o < nn; o += 256) Rm[o] = pool[it.r + o];
   __syncthreads();
   for (int o = threadIdx.x; o < nn; o += 256) {
