@@ -27,6 +27,10 @@ int qtg_init(int device) {
     return QTG_EDEVICE;
   }
   QTG_HIP_TRY(hipSetDevice(device));
+  // host waits spin instead of yielding: the CMF integrator synchronises
+  // once per lockstep tick (~100 us apart), where a yield costs tens of us
+  (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+  (void)hipGetLastError();
   return QTG_OK;
 }
 
