@@ -417,7 +417,11 @@ int Engine::plan_gemm() {
   phaseFlops_[2] = flops - fa0;
 
   // --------------------------------------------------------------- finish
+  // With SPF counts up to 48 the solve launch also writes the projected
+  // derivative of every node (and the top node's scaled Apply); larger nodes
+  // use two more mode-product launches for it.
   const double ff0 = flops;
+  const bool fusedFinish = nspf_ <= 48;
   std::vector<GcDesc> fin;
   std::vector<GmDesc> g1, g2;
   energySlot_ = pool64Off;
@@ -442,8 +446,17 @@ int Engine::plan_gemm() {
       g.out = energySlot_ + 1;
       g.sel = 0;
       fin.push_back(g);
-      GmDesc c = gm_desc(v, L, v.off, 1, 0, v.off, GM_COPY, v.addr);
-      g1.push_back(c);
+      if (fusedFinish) {
+        SolveItem si{};
+        si.s = si.x = si.r = si.w = si.a = -1;
+        si.y = si.out = v.off;
+        si.n = n;
+        si.rows = (int32_t)(v.N / n);
+        si.node = v.addr;
+        h_solve_.push_back(si);
+      } else {
+        g1.push_back(gm_desc(v, L, v.off, 1, 0, v.off, GM_COPY, v.addr));
+      }
       continue;
     }
     const int64_t so = pool64Off, xo = so + (int64_t)n * n, wo = xo + (int64_t)n * n;
@@ -460,8 +473,12 @@ int Engine::plan_gemm() {
     si.r = rhoinv_[v.addr];
     si.w = wo;
     si.n = n;
+    si.rows = (int32_t)(v.N / n);
     si.node = v.addr;
+    si.a = si.y = v.off;
+    si.out = fusedFinish ? v.off : -1;
     h_solve_.push_back(si);
+    if (fusedFinish) continue;
     g1.push_back(gm_desc(v, L, v.off, 1, rhoinv_[v.addr], v.off, GM_N, v.addr));
     GmDesc m2 = gm_desc(v, L, v.off, 0, wo, v.off, GM_N, v.addr);
     m2.are = -1.0;
@@ -475,8 +492,12 @@ int Engine::plan_gemm() {
     gl.prec = P64;
     gl.nd = (int64_t)h_solve_.size();
     gl.x0 = gl.x1 = gl.m = gl.o = B_POOL64;
+    gl.scaled = fusedFinish;
     double fl = 0.0;
-    for (auto& si : h_solve_) fl += 8.0 * 2.0 * si.n * si.n * si.n;
+    for (auto& si : h_solve_) {
+      if (si.s >= 0) fl += 8.0 * 2.0 * si.n * si.n * si.n;
+      if (si.s >= 0 && si.out >= 0) fl += 8.0 * 2.0 * si.n * si.n * (double)si.rows;  // the projected derivative
+    }
     gl.flops = fl;
     flops += fl;
     gApply_.push_back(gl);
@@ -738,7 +759,10 @@ int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, cons
       if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
     } else {
       if ((rc = kernel_mark(8, &kt))) return rc;
-      if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s, kt))) return rc;
+      if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s, kt,
+                             (const cplx<double>*)bases[B_PSI64], (const cplx<double>*)bases[B_APPLY64],
+                             (cplx<double>*)bases[B_DPSI64], a, b)))
+        return rc;
       if (timing_) tfl_[8] += gl.flops;
     }
   }

@@ -493,16 +493,26 @@ __global__ void __launch_bounds__(256)
 // largest modulus, lowest row on ties), then forms its elements of the
 // swapped, scaled and eliminated matrix directly from the old values.  The
 // arithmetic per element is solve_kernel's.
-__global__ void __launch_bounds__(256)
-    solve_pp_kernel(const SolveItem* __restrict__ items, cplx<double>* pool, const uint8_t* __restrict__ active) {
+constexpr int kSolveThreads = 512;  // 8 wavefronts: the derivative tiles of a node run side by side
+
+__global__ void __launch_bounds__(kSolveThreads)
+    solve_pp_kernel(const SolveItem* __restrict__ items, cplx<double>* pool, const uint8_t* __restrict__ active,
+                    const cplx<double>* __restrict__ abase, const cplx<double>* __restrict__ ybase,
+                    cplx<double>* obase, double fre, double fim) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const SolveItem it = items[blockIdx.x];
   if (active && it.node >= 0 && !active[it.node]) return;
   const int n = it.n, nn = n * n;
+  const cplx<double> f = mk<double>(fre, fim);
+  if (it.s < 0) {
+    // the top node: D = f Y
+    for (int e = threadIdx.x; e < it.rows * n; e += kSolveThreads) obase[it.out + e] = cmul(f, ybase[it.y + e]);
+    return;
+  }
   // the two buffers are addressed from the LDS base (an array of pointers
   // would turn every access into a generic one)
   cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
-  for (int o = threadIdx.x; o < nn; o += 256) {
+  for (int o = threadIdx.x; o < nn; o += kSolveThreads) {
     lds[o] = pool[it.s + o];
     lds[nn + o] = pool[it.x + o];
   }
@@ -525,7 +535,7 @@ __global__ void __launch_bounds__(256)
     const cplx<double> dg = Mc[p];
     const double dd = dg.re * dg.re + dg.im * dg.im;
     const cplx<double> dinv = mk<double>(dg.re / dd, -dg.im / dd);
-    for (int o = threadIdx.x; o < 2 * nn; o += 256) {
+    for (int o = threadIdx.x; o < 2 * nn; o += kSolveThreads) {
       const int r = o % n, k = o / n;
       const cplx<double>* Mk = M + k * n;
       // pivot row after the swap and scaling: old row p times dinv
@@ -535,10 +545,10 @@ __global__ void __launch_bounds__(256)
         w = v;
       } else {
         const int rs = (r == p) ? c : r;  // row r after the swap
-        const cplx<double> f = Mc[rs];
+        const cplx<double> fr = Mc[rs];
         w = Mk[rs];
-        w.re -= f.re * v.re - f.im * v.im;
-        w.im -= f.re * v.im + f.im * v.re;
+        w.re -= fr.re * v.re - fr.im * v.im;
+        w.im -= fr.re * v.im + fr.im * v.re;
       }
       N[o] = w;
     }
@@ -546,17 +556,44 @@ __global__ void __launch_bounds__(256)
     cur ^= 1;
   }
   const cplx<double>* X = lds + cur * 2 * nn + nn;
-  for (int o = threadIdx.x; o < nn; o += 256) pool[it.x + o] = X[o];
+  for (int o = threadIdx.x; o < nn; o += kSolveThreads) pool[it.x + o] = X[o];
   if (it.r < 0) return;
   // W(k, m) = sum_l R(k, l) X(m, l); R staged in the free buffer
   cplx<double>* Rm = lds + (cur ^ 1) * 2 * nn;
-  for (int o = threadIdx.x; o < nn; o += 256) Rm[o] = pool[it.r + o];
+  for (int o = threadIdx.x; o < nn; o += kSolveThreads) Rm[o] = pool[it.r + o];
   __syncthreads();
-  for (int o = threadIdx.x; o < nn; o += 256) {
+  for (int o = threadIdx.x; o < nn; o += kSolveThreads) {
     const int k = o % n, m = o / n;
     cplx<double> acc = czero<double>();
     for (int l = 0; l < n; ++l) cfma(acc, Rm[k + l * n], X[m + l * n]);
     pool[it.w + o] = acc;
+  }
+  if (it.out < 0) return;
+  __syncthreads();  // W, written to the pool above, is read back by this workgroup's MFMA tiles
+  // D = f (R x_last Y) - f (W x_last A): the mode products of the grouped
+  // path on the node's (rows x n) tiles, one wavefront per tile
+  GmDesc gy{};
+  gy.bef = it.rows;
+  gy.nin = gy.nout = n;
+  gy.aft = 1;
+  gy.op = GM_N;
+  gy.are = 1.0;
+  gy.ksplit = 1;
+  gy.x = it.y;
+  gy.m = it.r;
+  gy.out = it.out;
+  GmDesc ga = gy;
+  ga.x = it.a;
+  ga.m = it.w;
+  const int64_t tiles = gm_tasks(gy);
+  for (int64_t t = wave_id(); t < tiles; t += kSolveThreads / 64) {
+    const GmTile gt = gm_tile(gy, t);
+    cplx<double> vy[4], va[4];
+    gm_product<double, double, double>(gy, gt, ybase, pool, fre, fim, vy);
+    gm_product<double, double, double>(ga, gt, abase, pool, -fre, -fim, va);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) vy[q] = cadd(vy[q], va[q]);
+    gm_store<double, double>(gy, gt, vy, obase);
   }
 }
 
@@ -614,11 +651,13 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 }
 
 int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
-                 hipStream_t s, const KernelTiming* kt) {
+                 hipStream_t s, const KernelTiming* kt, const cplx<double>* abase, const cplx<double>* ybase,
+                 cplx<double>* obase, double fre, double fim) {
   if (n <= 0) return QTG_OK;
   if (nmax <= 48) {
     const size_t lds = 4 * sizeof(cplx<double>) * (size_t)nmax * nmax;
-    launch_timed(solve_pp_kernel, dim3((unsigned)n), dim3(256), (unsigned)lds, s, kt, items, pool, active);
+    launch_timed(solve_pp_kernel, dim3((unsigned)n), dim3(kSolveThreads), (unsigned)lds, s, kt, items, pool, active,
+                 abase, ybase, obase, fre, fim);
     QTG_LAUNCH_CHECK();
     return QTG_OK;
   }

@@ -52,11 +52,11 @@ struct GcDesc {
 };
 
 // number of tasks of one descriptor (16 x 16 tiles, times K chunks)
-inline int64_t gm_tasks(const GmDesc& d) {
+__host__ __device__ inline int64_t gm_tasks(const GmDesc& d) {
   const int64_t R = (int64_t)d.bef * d.aft;
   return ((R + 15) / 16) * ((d.nout + 15) / 16) * (d.ksplit > 1 ? d.ksplit : 1);
 }
-inline int64_t gc_tasks(const GcDesc& d) {
+__host__ __device__ inline int64_t gc_tasks(const GcDesc& d) {
   return (int64_t)((d.nx + 15) / 16) * ((d.ny + 15) / 16) * (d.ksplit > 1 ? d.ksplit : 1);
 }
 
@@ -118,13 +118,20 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 
 // Batched FP64 solve X <- S^-1 X (Gauss-Jordan with partial pivoting) for n x n
 // column-major S and X at pool offsets s, x; n <= nmax <= 64 (one barrier
-// per column step when nmax <= 48).  When r >= 0 it also forms W = R X^T at w (R
-// the n x n matrix at r).
+// per column step when nmax <= 48).  When r >= 0 it also forms W = R X^T at w
+// (R the n x n matrix at r).  When out >= 0 (nmax <= 48 only) it then writes
+// the projected derivative of the node, rows L = N / n, column-major:
+//   D(:, i) = f (sum_l R(i, l) Y(:, l) - sum_l W(i, l) A(:, l))
+// with A at a, Y at y, D at out in their own base pointers and f the launch
+// scale; an item with s < 0 only writes D = f Y (the top node).
 struct SolveItem {
   int64_t s, x, r, w;
-  int32_t n, node;
+  int64_t a, y, out;
+  int32_t n, node, rows, pad;
 };
 int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
-                 hipStream_t s, const KernelTiming* kt = nullptr);
+                 hipStream_t s, const KernelTiming* kt = nullptr, const cplx<double>* abase = nullptr,
+                 const cplx<double>* ybase = nullptr, cplx<double>* obase = nullptr, double fre = 0.0,
+                 double fim = 0.0);
 
 }  // namespace qtg
