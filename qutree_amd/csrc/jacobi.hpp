@@ -31,6 +31,91 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Rotation of tournament pair i at step `step` (m players): its (p, q) with
+// p < q and, when |a_pq| passes the criteria above, the parameters of
+// G = [[c, s ph], [-s conj(ph), c]].  Returns whether the pair rotates.
+__device__ inline bool jacobi_pair(const cplx<double>* A, int n, int ld, int m, int step, int i, double floor_abs,
+                                   int& p, int& q, double& c, double& sn, cplx<double>& ph) {
+  const int j0 = i, j1 = m - 1 - i;
+  p = (j0 == 0) ? 0 : 1 + (j0 - 1 + step) % (m - 1);
+  q = (j1 == 0) ? 0 : 1 + (j1 - 1 + step) % (m - 1);
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+  c = 1.0;
+  sn = 0.0;
+  ph = mk<double>(1.0, 0.0);
+  if (q >= n) return false;
+  const cplx<double> apq = A[p + q * ld];
+  const double aa = sqrt(apq.re * apq.re + apq.im * apq.im);
+  const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
+  if (!(aa > 2.2e-16 * sqrt(fabs(app * aqq)) && aa > floor_abs && aa > 1e-300)) return false;
+  const double theta = (aqq - app) / (2.0 * aa);
+  const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+  c = 1.0 / sqrt(t * t + 1.0);
+  sn = t * c;
+  ph = mk<double>(apq.re / aa, apq.im / aa);
+  return true;
+}
+
+// The 2 x 2 block (rows p, q of pair bi; columns pc, qc of pair bj) of
+// A <- G_bi^H A G_bj and U <- U G_bj, with G = [[c, s ph], [-s conj(ph), c]]
+// on (p, q); a rotated diagonal block gets exact zeros off the diagonal and a
+// real diagonal.  Pairs with an index beyond n are padding.
+__device__ inline void jacobi_block(cplx<double>* A, cplx<double>* U, int n, int ld, bool diag, int p, int q,
+                                    double ci, double si, cplx<double> hi, int pc, int qc, double cj, double sj,
+                                    cplx<double> hj) {
+  if (p >= n) return;
+  const bool qv = q < n, qcv = qc < n;
+  // rows p, q x columns pc, qc
+  cplx<double> a00 = A[p + pc * ld];
+  cplx<double> a01 = qcv ? A[p + qc * ld] : czero<double>();
+  cplx<double> a10 = qv ? A[q + pc * ld] : czero<double>();
+  cplx<double> a11 = (qv && qcv) ? A[q + qc * ld] : czero<double>();
+  cplx<double> u00 = U[p + pc * ld];
+  cplx<double> u01 = qcv ? U[p + qc * ld] : czero<double>();
+  cplx<double> u10 = qv ? U[q + pc * ld] : czero<double>();
+  cplx<double> u11 = (qv && qcv) ? U[q + qc * ld] : czero<double>();
+  // right: X <- X G_bj (columns pc, qc)
+  auto right = [&](cplx<double>& x0, cplx<double>& x1) {
+    const cplx<double> a = x0, b = x1;
+    x0 = mk<double>(cj * a.re - sj * (hj.re * b.re + hj.im * b.im), cj * a.im - sj * (hj.re * b.im - hj.im * b.re));
+    x1 = mk<double>(sj * (hj.re * a.re - hj.im * a.im) + cj * b.re, sj * (hj.re * a.im + hj.im * a.re) + cj * b.im);
+  };
+  // left: X <- G_bi^H X (rows p, q)
+  auto left = [&](cplx<double>& x0, cplx<double>& x1) {
+    const cplx<double> a = x0, b = x1;
+    x0 = mk<double>(ci * a.re - si * (hi.re * b.re - hi.im * b.im), ci * a.im - si * (hi.re * b.im + hi.im * b.re));
+    x1 = mk<double>(si * (hi.re * a.re + hi.im * a.im) + ci * b.re, si * (hi.re * a.im - hi.im * a.re) + ci * b.im);
+  };
+  if (sj != 0.0) {
+    right(a00, a01);
+    right(a10, a11);
+    right(u00, u01);
+    right(u10, u11);
+  }
+  if (si != 0.0) {
+    left(a00, a10);
+    left(a01, a11);
+  }
+  if (diag && si != 0.0) {
+    a01 = czero<double>();
+    a10 = czero<double>();
+    a00.im = 0.0;
+    a11.im = 0.0;
+  }
+  A[p + pc * ld] = a00;
+  if (qcv) A[p + qc * ld] = a01;
+  if (qv) A[q + pc * ld] = a10;
+  if (qv && qcv) A[q + qc * ld] = a11;
+  U[p + pc * ld] = u00;
+  if (qcv) U[p + qc * ld] = u01;
+  if (qv) U[q + pc * ld] = u10;
+  if (qv && qcv) U[q + qc * ld] = u11;
+}
+
 __device__ inline void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int ld, JacobiScratch& js) {
   for (int o = threadIdx.x; o < n * n; o += EW) {
     const int i = o % n, j = o / n;
@@ -56,35 +141,43 @@ __device__ inline void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int l
   double scale = 0.0;
   for (int i = 0; i < n; ++i) scale = fmax(scale, fabs(A[i + i * ld].re));
   const double floor_abs = 1e-24 * scale;
+  if (np * np <= EW) {
+    // one 2 x 2 block per lane: every lane derives the rotations of its row
+    // and column pairs itself (the same inputs, so the same parameters in
+    // every lane), all reads of a step precede its writes in the wavefront,
+    // and a step needs one barrier
+    const int bi = threadIdx.x % np, bj = threadIdx.x / np;
+    const bool mine = threadIdx.x < np * np;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+      int rotated = 0;
+      for (int step = 0; step < m - 1; ++step) {
+        int p = 0, q = 0, pc = 0, qc = 0;
+        double ci = 1.0, si = 0.0, cj = 1.0, sj = 0.0;
+        cplx<double> hi = mk<double>(1.0, 0.0), hj = hi;
+        int flag = 0;
+        if (mine) {
+          if (jacobi_pair(A, n, ld, m, step, bi, floor_abs, p, q, ci, si, hi)) flag = 1;
+          jacobi_pair(A, n, ld, m, step, bj, floor_abs, pc, qc, cj, sj, hj);
+        }
+        if (!__any(flag)) continue;  // uniform: no pair rotates in this step
+        rotated = 1;
+        if (mine) jacobi_block(A, U, n, ld, bi == bj, p, q, ci, si, hi, pc, qc, cj, sj, hj);
+        __syncthreads();
+      }
+      if (!rotated) break;
+    }
+    return;
+  }
   for (int sweep = 0; sweep < 30; ++sweep) {
     int rotated = 0;
     for (int step = 0; step < m - 1; ++step) {
       // phase 1: lane i < np computes the rotation of pair i
       int flag = 0;
       for (int i = threadIdx.x; i < np; i += EW) {
-        const int j0 = i, j1 = m - 1 - i;
-        int p = (j0 == 0) ? 0 : 1 + (j0 - 1 + step) % (m - 1);
-        int q = (j1 == 0) ? 0 : 1 + (j1 - 1 + step) % (m - 1);
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        double c = 1.0, sn = 0.0;
-        cplx<double> ph = mk<double>(1.0, 0.0);
-        if (q < n) {
-          const cplx<double> apq = A[p + q * ld];
-          const double aa = sqrt(apq.re * apq.re + apq.im * apq.im);
-          const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
-          if (aa > 2.2e-16 * sqrt(fabs(app * aqq)) && aa > floor_abs && aa > 1e-300) {
-            const double theta = (aqq - app) / (2.0 * aa);
-            const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            sn = t * c;
-            ph = mk<double>(apq.re / aa, apq.im / aa);
-            flag = 1;
-          }
-        }
+        int p, q;
+        double c, sn;
+        cplx<double> ph;
+        if (jacobi_pair(A, n, ld, m, step, i, floor_abs, p, q, c, sn, ph)) flag = 1;
         js.pp[i] = p;
         js.pq[i] = q;
         js.pc[i] = c;
@@ -99,61 +192,8 @@ __device__ inline void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int l
       // G = [[c, s ph], [-s conj(ph), c]] on (p, q)
       for (int o = threadIdx.x; o < np * np; o += EW) {
         const int bi = o % np, bj = o / np;
-        const int p = js.pp[bi], q = js.pq[bi], pc = js.pp[bj], qc = js.pq[bj];
-        if (p >= n) continue;
-        const bool qv = q < n, qcv = qc < n;
-        const double ci = js.pc[bi], si = js.ps[bi], cj = js.pc[bj], sj = js.ps[bj];
-        const cplx<double> hi = js.pph[bi], hj = js.pph[bj];
-        // rows p, q x columns pc, qc
-        cplx<double> a00 = A[p + pc * ld];
-        cplx<double> a01 = qcv ? A[p + qc * ld] : czero<double>();
-        cplx<double> a10 = qv ? A[q + pc * ld] : czero<double>();
-        cplx<double> a11 = (qv && qcv) ? A[q + qc * ld] : czero<double>();
-        cplx<double> u00 = U[p + pc * ld];
-        cplx<double> u01 = qcv ? U[p + qc * ld] : czero<double>();
-        cplx<double> u10 = qv ? U[q + pc * ld] : czero<double>();
-        cplx<double> u11 = (qv && qcv) ? U[q + qc * ld] : czero<double>();
-        // right: X <- X G_bj (columns pc, qc)
-        auto right = [&](cplx<double>& x0, cplx<double>& x1) {
-          const cplx<double> a = x0, b = x1;
-          x0 = mk<double>(cj * a.re - sj * (hj.re * b.re + hj.im * b.im),
-                          cj * a.im - sj * (hj.re * b.im - hj.im * b.re));
-          x1 = mk<double>(sj * (hj.re * a.re - hj.im * a.im) + cj * b.re,
-                          sj * (hj.re * a.im + hj.im * a.re) + cj * b.im);
-        };
-        // left: X <- G_bi^H X (rows p, q)
-        auto left = [&](cplx<double>& x0, cplx<double>& x1) {
-          const cplx<double> a = x0, b = x1;
-          x0 = mk<double>(ci * a.re - si * (hi.re * b.re - hi.im * b.im),
-                          ci * a.im - si * (hi.re * b.im + hi.im * b.re));
-          x1 = mk<double>(si * (hi.re * a.re + hi.im * a.im) + ci * b.re,
-                          si * (hi.re * a.im - hi.im * a.re) + ci * b.im);
-        };
-        if (sj != 0.0) {
-          right(a00, a01);
-          right(a10, a11);
-          right(u00, u01);
-          right(u10, u11);
-        }
-        if (si != 0.0) {
-          left(a00, a10);
-          left(a01, a11);
-        }
-        if (bi == bj && si != 0.0) {
-          // the rotated pair: exact zeros off the diagonal, real diagonal
-          a01 = czero<double>();
-          a10 = czero<double>();
-          a00.im = 0.0;
-          a11.im = 0.0;
-        }
-        A[p + pc * ld] = a00;
-        if (qcv) A[p + qc * ld] = a01;
-        if (qv) A[q + pc * ld] = a10;
-        if (qv && qcv) A[q + qc * ld] = a11;
-        U[p + pc * ld] = u00;
-        if (qcv) U[p + qc * ld] = u01;
-        if (qv) U[q + pc * ld] = u10;
-        if (qv && qcv) U[q + qc * ld] = u11;
+        jacobi_block(A, U, n, ld, bi == bj, js.pp[bi], js.pq[bi], js.pc[bi], js.ps[bi], js.pph[bi], js.pp[bj],
+                     js.pq[bj], js.pc[bj], js.ps[bj], js.pph[bj]);
       }
       __syncthreads();
     }
