@@ -113,7 +113,12 @@ class Engine {
     double bytes;  // algorithmic HBM bytes (distinct inputs once, outputs)
   };
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
-                   hipStream_t s);
+                   hipStream_t s) {
+    return run_launches(L, 0, L.size(), bases, active, sre, sim, s);
+  }
+  // launches [i0, i1) of L
+  int run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, void* const* bases, const uint8_t* active,
+                   double sre, double sim, hipStream_t s);
   int materialize();
   std::vector<std::pair<int, int>> restrict_in(int l, int a) const;
   std::vector<std::pair<int, int>> restrict_out(int l, int a) const;

@@ -719,10 +719,11 @@ int Engine::plan_gemm() {
   return QTG_OK;
 }
 
-int Engine::run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre,
-                         double sim, hipStream_t s) {
+int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, void* const* bases,
+                         const uint8_t* active, double sre, double sim, hipStream_t s) {
   int rc;
-  for (const auto& gl : L) {
+  for (size_t li = i0; li < std::min(i1, L.size()); ++li) {
+    const GLaunch& gl = L[li];
     if (gl.kind == 2 && heevPending_) {
       // rho^-1 from the side stream of the last build
       QTG_HIP_TRY(hipStreamWaitEvent(s, evHeev_, 0));
@@ -777,16 +778,24 @@ int Engine::build_gemm(hipStream_t s) {
   // solve and rho^-1 stages of the finish).  Engine stream: represent, then
   // (after the densities, which some hole items use as parent factor) the
   // holes, the combinations and Apply; apply_gemm joins before the solve.
+  // The two streams' launches are issued alternately: a replayed graph
+  // enqueues its nodes in the order they were recorded, so neither chain
+  // waits behind the host-side enqueue of the other.
   QTG_HIP_TRY(hipEventRecord(evFork_, s));
   QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
-  if ((rc = run_launches(gRho_, bases, nullptr, 1.0, 0.0, side_))) return rc;
-  QTG_HIP_TRY(hipEventRecord(evRho_, side_));
-  if ((rc = begin_mark(1, side_))) return rc;
-  if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
-  if ((rc = end_mark(side_))) return rc;
-  QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
-  heevPending_ = true;
-  if ((rc = run_launches(gBuild_, bases, nullptr, 1.0, 0.0, s))) return rc;
+  for (size_t i = 0; i <= std::max(gRho_.size(), gBuild_.size()); ++i) {
+    if ((rc = run_launches(gBuild_, i, i + 1, bases, nullptr, 1.0, 0.0, s))) return rc;
+    if (i < gRho_.size()) {
+      if ((rc = run_launches(gRho_, i, i + 1, bases, nullptr, 1.0, 0.0, side_))) return rc;
+    } else if (i == gRho_.size()) {
+      QTG_HIP_TRY(hipEventRecord(evRho_, side_));
+      if ((rc = begin_mark(1, side_))) return rc;
+      if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
+      if ((rc = end_mark(side_))) return rc;
+      QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
+      heevPending_ = true;
+    }
+  }
   QTG_HIP_TRY(hipStreamWaitEvent(s, evRho_, 0));
   if ((rc = run_launches(gHole_, bases, nullptr, 1.0, 0.0, s))) return rc;
   if ((rc = begin_mark(5, s))) return rc;
