@@ -205,12 +205,12 @@ __global__ void __launch_bounds__(EW)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cd2* Phi = reinterpret_cast<cd2*>(smem_raw);
   cd2* A = Phi + nmax;
-  cd2* U = A + nspf * (nspf + 1);
+  cd2* U = A + jacobi_elems(nspf);
   __shared__ JacobiScratch js;
   __shared__ double lam[64];
   const OrthoItem it = items[blockIdx.x];
   const DNode nd = nodes[it.node];
-  const int N = nd.N, n = nodes[it.node].dims[nd.order - 1], L = N / n, ld = n + 1;
+  const int N = nd.N, n = nodes[it.node].dims[nd.order - 1], L = N / n, ld = jacobi_ld(n);
   for (int e = threadIdx.x; e < N; e += EW) Phi[e] = P[nd.off + e];
   __syncthreads();
   for (int o = threadIdx.x; o < n * n; o += EW) {
@@ -310,7 +310,7 @@ int launch_programs(const NodeProg* progs, int64_t count, const VecOp* ops, cons
 int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P, cplx<double>* sw,
                  int nmax, int nspf, hipStream_t s) {
   if (count == 0) return QTG_OK;
-  const size_t lds = ((size_t)nmax + 2 * (size_t)nspf * (nspf + 1)) * sizeof(cd2);
+  const size_t lds = ((size_t)nmax + 2 * (size_t)jacobi_elems(nspf)) * sizeof(cd2);
   hipLaunchKernelGGL(ortho_kernel, dim3((unsigned)count), dim3(EW), lds, s, items, nodes, P, sw, nmax, nspf);
   QTG_LAUNCH_CHECK();
   return QTG_OK;

@@ -1,5 +1,6 @@
 // Wavefront Jacobi eigensolver for small Hermitian matrices (FP64, n <= 64)
-// held in LDS, column-major with leading dimension ld.  All lanes of a
+// held in LDS, column-major with leading dimension jacobi_ld(n) (even order,
+// see jacobi_dim).  All lanes of a
 // 64-lane workgroup call it.  Tournament (round-robin) ordering: each step
 // rotates up to 32 disjoint (p,q) pairs whose parameters come from the same
 // matrix; one pass applies both sides at once, each lane owning 2 x 2 blocks
@@ -19,6 +20,13 @@
 namespace qtg {
 
 constexpr int EW = 64;
+
+// Matrices are padded to an even order m (a zero row and column when n is
+// odd) and stored with leading dimension m + 1; a caller's LDS holds
+// jacobi_elems(nmax) complex elements per matrix.
+__host__ __device__ constexpr int jacobi_dim(int n) { return (n + 1) & ~1; }
+__host__ __device__ constexpr int jacobi_ld(int n) { return jacobi_dim(n) + 1; }
+__host__ __device__ constexpr int jacobi_elems(int n) { return jacobi_dim(n) * jacobi_ld(n); }
 
 struct JacobiScratch {
   double pc[32], ps[32];
@@ -57,6 +65,36 @@ __device__ inline bool jacobi_pair(const cplx<double>* A, int n, int ld, int m, 
   c = 1.0 / sqrt(t * t + 1.0);
   sn = t * c;
   ph = mk<double>(apq.re / aa, apq.im / aa);
+  return true;
+}
+
+// jacobi_pair with fewer FP64 divides and square roots: 1/|a_pq| and
+// 1/sqrt(1 + t^2) are reciprocal square roots, the tests compare squares.
+__device__ inline bool jacobi_pair_fast(const cplx<double>* A, int n, int ld, int m, int step, int i,
+                                        double floor2, int& p, int& q, double& c, double& sn, cplx<double>& ph) {
+  const int j0 = i, j1 = m - 1 - i;
+  p = (j0 == 0) ? 0 : 1 + (j0 - 1 + step) % (m - 1);
+  q = (j1 == 0) ? 0 : 1 + (j1 - 1 + step) % (m - 1);
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+  c = 1.0;
+  sn = 0.0;
+  ph = mk<double>(1.0, 0.0);
+  if (q >= n) return false;
+  const cplx<double> apq = A[p + q * ld];
+  const double r2 = apq.re * apq.re + apq.im * apq.im;
+  const double app = A[p + p * ld].re, aqq = A[q + q * ld].re;
+  // |a_pq| > eps sqrt|a_pp a_qq|, |a_pq| > floor, |a_pq| > 1e-140
+  if (!(r2 > 4.84e-32 * fabs(app * aqq) && r2 > floor2 && r2 > 1e-280)) return false;
+  const double inv = rsqrt(r2);
+  const double theta = 0.5 * (aqq - app) * inv;
+  const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+  c = rsqrt(t * t + 1.0);
+  sn = t * c;
+  ph = mk<double>(apq.re * inv, apq.im * inv);
   return true;
 }
 
@@ -116,10 +154,58 @@ __device__ inline void jacobi_block(cplx<double>* A, cplx<double>* U, int n, int
   if (qv && qcv) U[q + qc * ld] = u11;
 }
 
-__device__ inline void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int ld, JacobiScratch& js) {
-  for (int o = threadIdx.x; o < n * n; o += EW) {
-    const int i = o % n, j = o / n;
+// The fused step's block update: rows p, q and columns pc, qc all lie in the
+// padded m x m matrices, so no index is guarded.
+__device__ __forceinline__ void jacobi_block_full(cplx<double>* A, cplx<double>* U, int ld, bool diag, int p, int q,
+                                                  double ci, double si, cplx<double> hi, int pc, int qc, double cj,
+                                                  double sj, cplx<double> hj) {
+  cplx<double> a00 = A[p + pc * ld], a01 = A[p + qc * ld], a10 = A[q + pc * ld], a11 = A[q + qc * ld];
+  cplx<double> u00 = U[p + pc * ld], u01 = U[p + qc * ld], u10 = U[q + pc * ld], u11 = U[q + qc * ld];
+  auto right = [&](cplx<double>& x0, cplx<double>& x1) {
+    const cplx<double> a = x0, b = x1;
+    x0 = mk<double>(cj * a.re - sj * (hj.re * b.re + hj.im * b.im), cj * a.im - sj * (hj.re * b.im - hj.im * b.re));
+    x1 = mk<double>(sj * (hj.re * a.re - hj.im * a.im) + cj * b.re, sj * (hj.re * a.im + hj.im * a.re) + cj * b.im);
+  };
+  auto left = [&](cplx<double>& x0, cplx<double>& x1) {
+    const cplx<double> a = x0, b = x1;
+    x0 = mk<double>(ci * a.re - si * (hi.re * b.re - hi.im * b.im), ci * a.im - si * (hi.re * b.im + hi.im * b.re));
+    x1 = mk<double>(si * (hi.re * a.re + hi.im * a.im) + ci * b.re, si * (hi.re * a.im - hi.im * a.re) + ci * b.im);
+  };
+  if (sj != 0.0) {
+    right(a00, a01);
+    right(a10, a11);
+    right(u00, u01);
+    right(u10, u11);
+    U[p + pc * ld] = u00;
+    U[p + qc * ld] = u01;
+    U[q + pc * ld] = u10;
+    U[q + qc * ld] = u11;
+  }
+  if (si != 0.0) {
+    left(a00, a10);
+    left(a01, a11);
+    if (diag) {
+      a01 = czero<double>();
+      a10 = czero<double>();
+      a00.im = 0.0;
+      a11.im = 0.0;
+    }
+  }
+  if (sj != 0.0 || si != 0.0) {
+    A[p + pc * ld] = a00;
+    A[p + qc * ld] = a01;
+    A[q + pc * ld] = a10;
+    A[q + qc * ld] = a11;
+  }
+}
+
+// ld must be jacobi_ld(n); the padding row and column (odd n) are set here.
+__device__ __forceinline__ void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int ld, JacobiScratch& js) {
+  const int md = jacobi_dim(n);
+  for (int o = threadIdx.x; o < md * md; o += EW) {
+    const int i = o % md, j = o / md;
     U[i + j * ld] = mk<double>(i == j ? 1.0 : 0.0, 0.0);
+    if (i >= n || j >= n) A[i + j * ld] = czero<double>();
   }
   __syncthreads();
   // Hermitise (the input is Hermitian up to rounding)
@@ -148,6 +234,7 @@ __device__ inline void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int l
     // and a step needs one barrier
     const int bi = threadIdx.x % np, bj = threadIdx.x / np;
     const bool mine = threadIdx.x < np * np;
+    const double floor2 = floor_abs * floor_abs;
     for (int sweep = 0; sweep < 30; ++sweep) {
       int rotated = 0;
       for (int step = 0; step < m - 1; ++step) {
@@ -156,12 +243,14 @@ __device__ inline void jacobi_eig(cplx<double>* A, cplx<double>* U, int n, int l
         cplx<double> hi = mk<double>(1.0, 0.0), hj = hi;
         int flag = 0;
         if (mine) {
-          if (jacobi_pair(A, n, ld, m, step, bi, floor_abs, p, q, ci, si, hi)) flag = 1;
-          jacobi_pair(A, n, ld, m, step, bj, floor_abs, pc, qc, cj, sj, hj);
+          if (jacobi_pair_fast(A, n, ld, m, step, bi, floor2, p, q, ci, si, hi)) flag = 1;
+          jacobi_pair_fast(A, n, ld, m, step, bj, floor2, pc, qc, cj, sj, hj);
         }
+        // the wavefront's reads of this step stay ahead of its writes
+        asm volatile("" ::: "memory");
         if (!__any(flag)) continue;  // uniform: no pair rotates in this step
         rotated = 1;
-        if (mine) jacobi_block(A, U, n, ld, bi == bj, p, q, ci, si, hi, pc, qc, cj, sj, hj);
+        if (mine) jacobi_block_full(A, U, ld, bi == bj, p, q, ci, si, hi, pc, qc, cj, sj, hj);
         __syncthreads();
       }
       if (!rotated) break;

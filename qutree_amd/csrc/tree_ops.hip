@@ -344,14 +344,13 @@ __global__ void __launch_bounds__(EW)
     heev_reginv_kernel(const EigItem* __restrict__ items, const cplx<T>* __restrict__ pool,
                        cplx<double>* pool64, double eps, int nmaxspf) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int ldm = nmaxspf + 1;
   cplx<double>* A = reinterpret_cast<cplx<double>*>(smem_raw);
-  cplx<double>* U = A + nmaxspf * ldm;
+  cplx<double>* U = A + jacobi_elems(nmaxspf);
   __shared__ JacobiScratch js;
-  __shared__ double lam[64];
+  __shared__ double lam[64], wgt[64];
   const EigItem it = items[blockIdx.x];
   const int n = it.n;
-  const int ld = n + 1;
+  const int ld = jacobi_ld(n);
   for (int o = threadIdx.x; o < n * n; o += EW) {
     const int i = o % n, j = o / n;
     const cplx<T> v = pool[it.in + o];
@@ -359,7 +358,11 @@ __global__ void __launch_bounds__(EW)
   }
   __syncthreads();
   jacobi_eig(A, U, n, ld, js);
-  for (int i = threadIdx.x; i < n; i += EW) lam[i] = A[i + i * ld].re;
+  for (int i = threadIdx.x; i < n; i += EW) {
+    const double l = A[i + i * ld].re;
+    lam[i] = l;
+    wgt[i] = 1.0 / (l + eps * exp(-l / eps));
+  }
   __syncthreads();
   if (it.ev >= 0)
     for (int i = threadIdx.x; i < n; i += EW) pool64[it.ev + i] = mk<double>(lam[i], 0.0);
@@ -367,8 +370,7 @@ __global__ void __launch_bounds__(EW)
     const int i = o % n, j = o / n;
     cplx<double> acc = czero<double>();
     for (int k = 0; k < n; ++k) {
-      const double l = lam[k];
-      const double w = 1.0 / (l + eps * exp(-l / eps));
+      const double w = wgt[k];
       const cplx<double> u = U[i + k * ld], v = U[j + k * ld];
       acc.re += w * (u.re * v.re + u.im * v.im);
       acc.im += w * (u.im * v.re - u.re * v.im);
@@ -481,7 +483,7 @@ template <typename T>
 int launch_heev(const EigItem* items, int64_t count, const void* pool, void* pool64, double eps, int nspf,
                 hipStream_t s) {
   if (count == 0) return QTG_OK;
-  const size_t lds = 2 * (size_t)nspf * (nspf + 1) * sizeof(cplx<double>);
+  const size_t lds = 2 * (size_t)jacobi_elems(nspf) * sizeof(cplx<double>);
   hipLaunchKernelGGL(heev_reginv_kernel<T>, dim3((unsigned)count), dim3(EW), lds, s, items, (const cplx<T>*)pool,
                      (cplx<double>*)pool64, eps, nspf);
   QTG_LAUNCH_CHECK();
