@@ -28,6 +28,8 @@ Cmf::~Cmf() {
   for (void* p : dev_)
     if (p) (void)hipFree(p);
   if (host_stage_) (void)hipHostFree(host_stage_);
+  for (hipEvent_t e : evStage_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 int Cmf::dalloc(void** p, size_t bytes) {
@@ -69,12 +71,13 @@ int Cmf::init() {
   QTG_HIP_TRY(hipMemcpy(rhoslot_, rs.data(), sizeof(int64_t) * NN_, hipMemcpyHostToDevice));
   // staging for per-tick programs and the active mask
   stage_bytes_ = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode) + NN_ + 64;
-  if (hipHostMalloc(&host_stage_, stage_bytes_) != hipSuccess) {
+  if (hipHostMalloc(&host_stage_, 2 * stage_bytes_) != hipSuccess) {
     set_error("cmf: pinned allocation failed");
     return QTG_ENOMEM;
   }
   if ((rc = dalloc(&dev_stage_, stage_bytes_))) return rc;
   QTG_HIP_TRY(hipHostMalloc((void**)&err_h_, sizeof(double) * NN_));
+  for (hipEvent_t& e : evStage_) QTG_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // orthogonal(): per height, the nodes to orthonormalise and the parents
   // receiving their square roots (children in node-address order)
   int64_t swoff = 0;
@@ -284,8 +287,8 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
     bs_[v].xend = t1;
   }
   const CmfBufs bb = bufs(P);
-  unsigned char* hs = (unsigned char*)host_stage_;
   unsigned char* ds = (unsigned char*)dev_stage_;
+  int half = 0;
   std::vector<NodeProg> ppre, ppost;
   std::vector<VecOp> opre, opost, pre, post;
   std::vector<uint8_t> act(NN_);
@@ -328,12 +331,20 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
       set_error("cmf: staging overflow");
       return QTG_ESTATE;
     }
+    // The host runs ahead of the device while no node waits for an error:
+    // the two halves of the pinned staging take turns, and a half is
+    // rewritten only after the device has read its last upload.  The device
+    // region is reused in stream order.
+    unsigned char* hs = (unsigned char*)host_stage_ + half * stage_bytes_;
+    QTG_HIP_TRY(hipEventSynchronize(evStage_[half]));
     std::memcpy(hs + o1, ppre.data(), sizeof(NodeProg) * ppre.size());
     std::memcpy(hs + o2, opre.data(), sizeof(VecOp) * opre.size());
     std::memcpy(hs + o3, ppost.data(), sizeof(NodeProg) * ppost.size());
     std::memcpy(hs + o4, opost.data(), sizeof(VecOp) * opost.size());
     std::memcpy(hs + o5, act.data(), NN_);
     QTG_HIP_TRY(hipMemcpyAsync(ds, hs, oend, hipMemcpyHostToDevice, s_));
+    QTG_HIP_TRY(hipEventRecord(evStage_[half], s_));
+    half ^= 1;
     int rc = launch_programs((NodeProg*)(ds + o1), (int64_t)ppre.size(), (VecOp*)(ds + o2), bb, E.d_nodes_,
                              (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
     if (rc) return rc;
@@ -344,9 +355,11 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
     if ((rc = launch_programs((NodeProg*)(ds + o3), (int64_t)ppost.size(), (VecOp*)(ds + o4), bb, E.d_nodes_,
                               (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_)))
       return rc;
-    if (need_err) QTG_HIP_TRY(hipMemcpyAsync(err_h_, err_, sizeof(double) * NN_, hipMemcpyDeviceToHost, s_));
-    // the pinned staging is rewritten next tick, and the errors are read
-    QTG_HIP_TRY(hipStreamSynchronize(s_));
+    if (need_err) {
+      // the next tick's decisions read the errors
+      QTG_HIP_TRY(hipMemcpyAsync(err_h_, err_, sizeof(double) * NN_, hipMemcpyDeviceToHost, s_));
+      QTG_HIP_TRY(hipStreamSynchronize(s_));
+    }
   }
   QTG_HIP_TRY(hipStreamSynchronize(s_));
   return QTG_OK;
@@ -356,6 +369,7 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
 int Cmf::error(double* out) {
   unsigned char* hs = (unsigned char*)host_stage_;
   unsigned char* ds = (unsigned char*)dev_stage_;
+  QTG_HIP_TRY(hipEventSynchronize(evStage_[0]));
   const size_t progB = sizeof(NodeProg) * NN_;
   NodeProg* hp = (NodeProg*)hs;
   VecOp* ho = (VecOp*)(hs + progB);

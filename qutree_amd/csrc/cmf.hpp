@@ -59,9 +59,10 @@ class Cmf {
   double* err_ = nullptr;
   double* err_h_ = nullptr;
   int64_t* rhoslot_ = nullptr;
-  void* host_stage_ = nullptr;
+  void* host_stage_ = nullptr;  // two halves of stage_bytes_, used in turn by the ticks
   void* dev_stage_ = nullptr;
   size_t stage_bytes_ = 0;
+  hipEvent_t evStage_[2] = {nullptr, nullptr};  // the upload from each half has been read
   cplx<double>* sw_ = nullptr;
   OrthoItem* d_oi_ = nullptr;
   SwApply* d_sa_ = nullptr;

@@ -44,11 +44,91 @@ __device__ inline cd2* bp(const CmfBufs& b, int i) {
 }
 static_assert(kCmfBufs == 17, "bp() covers every buffer");
 
+// Elementwise ops: Op::load(e) reads every operand of element e, and
+// Op::store(e, v) writes its results.  The buffers of one op are different
+// arenas, but the compiler cannot prove it, so a store would order every later
+// load behind it: a full pass loads kPass elements per thread before storing
+// any, and the remainder goes one element at a time.
+constexpr int kPass = 4;
+template <class Op>
+__device__ __forceinline__ void elementwise(int N, const Op& op) {
+  int e0 = threadIdx.x;
+  for (; e0 + (kPass - 1) * TB < N; e0 += kPass * TB) {
+    typename Op::Val v[kPass];
+#pragma unroll
+    for (int u = 0; u < kPass; ++u) v[u] = op.load(e0 + u * TB);
+#pragma unroll
+    for (int u = 0; u < kPass; ++u) op.store(e0 + u * TB, v[u]);
+  }
+  for (int e = e0; e < N; e += TB) op.store(e, op.load(e));
+}
+
+struct CopyOp {  // a <- b
+  cd2* A;
+  const cd2* B;
+  struct Val {
+    cd2 b;
+  };
+  __device__ Val load(int e) const { return {B[e]}; }
+  __device__ void store(int e, const Val& v) const { A[e] = v.b; }
+};
+struct AxpyOp {  // a <- b + h c
+  cd2* A;
+  const cd2 *B, *C;
+  double h;
+  struct Val {
+    cd2 b, c;
+  };
+  __device__ Val load(int e) const { return {B[e], C[e]}; }
+  __device__ void store(int e, const Val& v) const {
+    A[e] = mk<double>(v.b.re + h * v.c.re, v.b.im + h * v.c.im);
+  }
+};
+struct MmidStepOp {  // t = y0 + 2h y2; y0 <- y1; y1 <- t; y2 <- t
+  cd2 *y0, *y1, *y2;
+  double h;
+  struct Val {
+    cd2 a, b, c;
+  };
+  __device__ Val load(int e) const { return {y0[e], y1[e], y2[e]}; }
+  __device__ void store(int e, const Val& v) const {
+    const cd2 t = mk<double>(v.a.re + 2.0 * h * v.c.re, v.a.im + 2.0 * h * v.c.im);
+    y0[e] = v.b;
+    y1[e] = t;
+    y2[e] = t;
+  }
+};
+struct MmidFinalOp {  // y0 <- (y0 + y1 + h y2) / 2
+  cd2* y0;
+  const cd2 *y1, *y2;
+  double h;
+  struct Val {
+    cd2 a, b, c;
+  };
+  __device__ Val load(int e) const { return {y0[e], y1[e], y2[e]}; }
+  __device__ void store(int e, const Val& v) const {
+    y0[e] = mk<double>((v.a.re + v.b.re + h * v.c.re) / 2.0, (v.a.im + v.b.im + h * v.c.im) / 2.0);
+  }
+};
+struct PostExtrOp {  // y0 <- y1; y1 <- y1 + y2
+  cd2 *y0, *y1;
+  const cd2* y2;
+  struct Val {
+    cd2 a, c;
+  };
+  __device__ Val load(int e) const { return {y1[e], y2[e]}; }
+  __device__ void store(int e, const Val& v) const {
+    y0[e] = v.a;
+    y1[e] = mk<double>(v.a.re + v.c.re, v.a.im + v.c.im);
+  }
+};
+
 __global__ void __launch_bounds__(TB)
     program_kernel(const NodeProg* __restrict__ progs, const VecOp* __restrict__ ops, CmfBufs bufs,
                    const DNode* __restrict__ nodes, const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot,
                    double* err) {
   __shared__ double red[TB];
+  __shared__ cd2 rhos[16 * 16];
   const NodeProg pg = progs[blockIdx.x];
   const DNode nd = nodes[pg.node];
   const int N = nd.N;
@@ -56,52 +136,38 @@ __global__ void __launch_bounds__(TB)
   for (int k = pg.op0; k < pg.op1; ++k) {
     const VecOp op = ops[k];
     switch (op.code) {
-      case V_COPY: {
-        cd2* A = bp(bufs, op.a) + off;
-        const cd2* B = bp(bufs, op.b) + off;
-        for (int e = threadIdx.x; e < N; e += TB) A[e] = B[e];
+      case V_COPY:
+        elementwise(N, CopyOp{bp(bufs, op.a) + off, bp(bufs, op.b) + off});
         break;
-      }
-      case V_AXPY: {
-        cd2* A = bp(bufs, op.a) + off;
-        const cd2* B = bp(bufs, op.b) + off;
-        const cd2* Cc = bp(bufs, op.c) + off;
-        for (int e = threadIdx.x; e < N; e += TB)
-          A[e] = mk<double>(B[e].re + op.h * Cc[e].re, B[e].im + op.h * Cc[e].im);
+      case V_AXPY:
+        elementwise(N, AxpyOp{bp(bufs, op.a) + off, bp(bufs, op.b) + off, bp(bufs, op.c) + off, op.h});
         break;
-      }
       case V_ZERO: {
         cd2* A = bp(bufs, op.a) + off;
         for (int e = threadIdx.x; e < N; e += TB) A[e] = czero<double>();
         break;
       }
-      case V_MMID_STEP: {
-        cd2* y0 = bp(bufs, B_Y0) + off;
-        cd2* y1 = bp(bufs, B_Y0 + 1) + off;
-        cd2* y2 = bp(bufs, B_Y0 + 2) + off;
-        for (int e = threadIdx.x; e < N; e += TB) {
-          const cd2 t = mk<double>(y0[e].re + 2.0 * op.h * y2[e].re, y0[e].im + 2.0 * op.h * y2[e].im);
-          y0[e] = y1[e];
-          y1[e] = t;
-          y2[e] = t;
-        }
+      case V_MMID_STEP:
+        elementwise(N, MmidStepOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off, op.h});
         break;
-      }
-      case V_MMID_FINAL: {
-        cd2* y0 = bp(bufs, B_Y0) + off;
-        const cd2* y1 = bp(bufs, B_Y0 + 1) + off;
-        const cd2* y2 = bp(bufs, B_Y0 + 2) + off;
-        for (int e = threadIdx.x; e < N; e += TB)
-          y0[e] = mk<double>((y0[e].re + y1[e].re + op.h * y2[e].re) / 2.0,
-                             (y0[e].im + y1[e].im + op.h * y2[e].im) / 2.0);
+      case V_MMID_FINAL:
+        elementwise(N, MmidFinalOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off, op.h});
         break;
-      }
       case V_RZEXTR: {
-        // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2
+        // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2; one
+        // element per pass: its tableau column is loaded whole, then stored
         cd2* yest = bp(bufs, B_Y0) + off;
         cd2* ysav = bp(bufs, B_Y0 + 1) + off;
         cd2* dy = bp(bufs, B_Y0 + 2) + off;
         cd2* yt0 = bp(bufs, B_YT0) + off;
+        constexpr int kTab = 7;  // ytab[0..6]
+        const int mi = op.mi < kTab ? op.mi : kTab;
+        double fx[kTab];
+#pragma unroll
+        for (int k2 = 1; k2 < kTab; ++k2) fx[k2] = ops[k].fx[k2];
+        cd2* yt[kTab];
+#pragma unroll
+        for (int k2 = 0; k2 < kTab; ++k2) yt[k2] = bp(bufs, B_YT0 + k2) + off;
         for (int e = threadIdx.x; e < N; e += TB) {
           if (op.i == 0) {
             const cd2 v = yest[e];
@@ -110,43 +176,42 @@ __global__ void __launch_bounds__(TB)
             yt0[e] = v;
             continue;
           }
+          cd2 col[kTab];
+#pragma unroll
+          for (int k2 = 1; k2 < kTab; ++k2) col[k2] = yt[k2 < mi ? k2 : 0][e];
           cd2 ddy = czero<double>();
           cd2 yy = yest[e];
           cd2 v = yt0[e];
           cd2 c = yy;
           yt0[e] = yy;
-          for (int k2 = 1; k2 < op.mi; ++k2) {
-            const double fx = ops[k].fx[k2];  // from memory: a dynamic index into a copy would go to scratch
-            const cd2 bi = mk<double>(fx * v.re, fx * v.im);
-            cd2 b = mk<double>(bi.re - c.re, bi.im - c.im);
-            if (b.re != 0.0 || b.im != 0.0) {
-              b = cdiv(mk<double>(c.re - v.re, c.im - v.im), b);
-              ddy = cmul(c, b);
-              c = cmul(bi, b);
-            } else {
-              ddy = v;
+#pragma unroll
+          for (int k2 = 1; k2 < kTab; ++k2) {
+            if (k2 < mi) {
+              const cd2 bi = mk<double>(fx[k2] * v.re, fx[k2] * v.im);
+              cd2 b = mk<double>(bi.re - c.re, bi.im - c.im);
+              if (b.re != 0.0 || b.im != 0.0) {
+                b = cdiv(mk<double>(c.re - v.re, c.im - v.im), b);
+                ddy = cmul(c, b);
+                c = cmul(bi, b);
+              } else {
+                ddy = v;
+              }
+              v = col[k2];
+              col[k2] = ddy;
+              yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
             }
-            cd2* ytk = bp(bufs, B_YT0 + k2) + off;
-            v = ytk[e];
-            ytk[e] = ddy;
-            yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
           }
+#pragma unroll
+          for (int k2 = 1; k2 < kTab; ++k2)
+            if (k2 < mi) yt[k2][e] = col[k2];
           dy[e] = ddy;
           ysav[e] = yy;
         }
         break;
       }
-      case V_POST_EXTR: {
-        cd2* y0 = bp(bufs, B_Y0) + off;
-        cd2* y1 = bp(bufs, B_Y0 + 1) + off;
-        const cd2* y2 = bp(bufs, B_Y0 + 2) + off;
-        for (int e = threadIdx.x; e < N; e += TB) {
-          const cd2 a = y1[e];
-          y0[e] = a;
-          y1[e] = mk<double>(a.re + y2[e].re, a.im + y2[e].im);
-        }
+      case V_POST_EXTR:
+        elementwise(N, PostExtrOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off});
         break;
-      }
       case V_ERROR: {
         const cd2* Phi = bp(bufs, op.a) + off;
         const cd2* Chi = bp(bufs, op.b) + off;
@@ -164,18 +229,48 @@ __global__ void __launch_bounds__(TB)
         } else {
           const cd2* rho = pool + rs;
           // Delta = sum_{i,n,m} Re( conj(d(i,n)) rho(n,m) d(i,m) )
-          for (int e = threadIdx.x; e < N; e += TB) {
-            const int i = e % L, nn = e / L;
-            const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
-            double tr = 0.0, ti = 0.0;
-            for (int m = 0; m < n; ++m) {
-              const cd2 r = rho[nn + m * n];
-              const double er = Phi[i + m * L].re - Chi[i + m * L].re;
-              const double ei = Phi[i + m * L].im - Chi[i + m * L].im;
-              tr += r.re * er - r.im * ei;
-              ti += r.re * ei + r.im * er;
+          if (n <= 16) {
+            // a thread per row i: d(i, :) in registers, rho in LDS
+            for (int o = threadIdx.x; o < n * n; o += TB) rhos[o] = rho[o];
+            __syncthreads();
+            for (int i = threadIdx.x; i < L; i += TB) {
+              cd2 d[16];
+#pragma unroll
+              for (int m = 0; m < 16; ++m) {
+                const int mm = m < n ? m : 0;
+                const cd2 p = Phi[i + mm * L], q = Chi[i + mm * L];
+                d[m] = mk<double>(p.re - q.re, p.im - q.im);
+              }
+#pragma unroll
+              for (int nn = 0; nn < 16; ++nn) {
+                if (nn < n) {
+                  double tr = 0.0, ti = 0.0;
+#pragma unroll
+                  for (int m = 0; m < 16; ++m) {
+                    if (m < n) {
+                      const cd2 r = rhos[nn + m * n];
+                      tr += r.re * d[m].re - r.im * d[m].im;
+                      ti += r.re * d[m].im + r.im * d[m].re;
+                    }
+                  }
+                  delta += d[nn].re * tr + d[nn].im * ti;
+                }
+              }
             }
-            delta += dr * tr + di * ti;
+          } else {
+            for (int e = threadIdx.x; e < N; e += TB) {
+              const int i = e % L, nn = e / L;
+              const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
+              double tr = 0.0, ti = 0.0;
+              for (int m = 0; m < n; ++m) {
+                const cd2 r = rho[nn + m * n];
+                const double er = Phi[i + m * L].re - Chi[i + m * L].re;
+                const double ei = Phi[i + m * L].im - Chi[i + m * L].im;
+                tr += r.re * er - r.im * ei;
+                ti += r.re * ei + r.im * er;
+              }
+              delta += dr * tr + di * ti;
+            }
           }
           delta = block_sum_d(delta, red);
           double t = 0.0, tim = 0.0;
