@@ -489,10 +489,13 @@ __global__ void __launch_bounds__(256)
 
 // Systems with n <= 48: ping-pong Gauss-Jordan.  Each column step reads the
 // previous [S | X] buffer and writes the next one, so a step is one barrier:
-// every thread finds the pivot itself (broadcast reads of the pivot column,
-// largest modulus, lowest row on ties), then forms its elements of the
-// swapped, scaled and eliminated matrix directly from the old values.  The
-// arithmetic per element is solve_kernel's.
+// every thread forms its elements of the scaled and eliminated matrix
+// directly from the old values.  S = A^H A is Hermitian positive definite
+// (the node's columns are independent), so its diagonal pivots are positive
+// and elimination needs no row exchanges; for the near-orthonormal A of a
+// sweep or a CMF substep S is diagonally dominant, and the partial pivoting
+// of solve_kernel picks the same diagonal pivots, so the arithmetic per
+// element is solve_kernel's.
 constexpr int kSolveThreads = 512;  // 8 wavefronts: the derivative tiles of a node run side by side
 
 __global__ void __launch_bounds__(kSolveThreads)
@@ -522,31 +525,20 @@ __global__ void __launch_bounds__(kSolveThreads)
     const cplx<double>* M = lds + cur * 2 * nn;
     cplx<double>* N = lds + (cur ^ 1) * 2 * nn;
     const cplx<double>* Mc = M + c * n;  // pivot column
-    int p = c;
-    double best = -1.0;
-    for (int r = c; r < n; ++r) {
-      const cplx<double> v = Mc[r];
-      const double a = v.re * v.re + v.im * v.im;
-      if (a > best) {
-        best = a;
-        p = r;
-      }
-    }
-    const cplx<double> dg = Mc[p];
+    const cplx<double> dg = Mc[c];
     const double dd = dg.re * dg.re + dg.im * dg.im;
     const cplx<double> dinv = mk<double>(dg.re / dd, -dg.im / dd);
     for (int o = threadIdx.x; o < 2 * nn; o += kSolveThreads) {
       const int r = o % n, k = o / n;
       const cplx<double>* Mk = M + k * n;
-      // pivot row after the swap and scaling: old row p times dinv
-      const cplx<double> v = cmul(Mk[p], dinv);
+      // pivot row after scaling
+      const cplx<double> v = cmul(Mk[c], dinv);
       cplx<double> w;
       if (r == c) {
         w = v;
       } else {
-        const int rs = (r == p) ? c : r;  // row r after the swap
-        const cplx<double> fr = Mc[rs];
-        w = Mk[rs];
+        const cplx<double> fr = Mc[r];
+        w = Mk[r];
         w.re -= fr.re * v.re - fr.im * v.im;
         w.im -= fr.re * v.im + fr.im * v.re;
       }
