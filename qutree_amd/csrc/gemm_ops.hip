@@ -521,28 +521,48 @@ __global__ void __launch_bounds__(kSolveThreads)
   }
   __syncthreads();
   int cur = 0;
-  for (int c = 0; c < n; ++c) {
+  // Two columns per barrier: each thread derives the values of the first
+  // column step it needs (its row, the second pivot row and column) with the
+  // single step's arithmetic, then applies the second step to them.
+  for (int c = 0; c < n; c += 2) {
     const cplx<double>* M = lds + cur * 2 * nn;
     cplx<double>* N = lds + (cur ^ 1) * 2 * nn;
     const cplx<double>* Mc = M + c * n;  // pivot column
     const cplx<double> dg = Mc[c];
     const double dd = dg.re * dg.re + dg.im * dg.im;
     const cplx<double> dinv = mk<double>(dg.re / dd, -dg.im / dd);
-    for (int o = threadIdx.x; o < 2 * nn; o += kSolveThreads) {
-      const int r = o % n, k = o / n;
-      const cplx<double>* Mk = M + k * n;
-      // pivot row after scaling
-      const cplx<double> v = cmul(Mk[c], dinv);
-      cplx<double> w;
-      if (r == c) {
-        w = v;
-      } else {
-        const cplx<double> fr = Mc[r];
-        w = Mk[r];
-        w.re -= fr.re * v.re - fr.im * v.im;
-        w.im -= fr.re * v.im + fr.im * v.re;
+    // [S | X](rr, col) after column step c
+    auto step1 = [&](int rr, int col) {
+      const cplx<double>* Mk = M + col * n;
+      const cplx<double> v = cmul(Mk[c], dinv);  // pivot row after scaling
+      if (rr == c) return v;
+      const cplx<double> fr = Mc[rr];
+      cplx<double> w = Mk[rr];
+      w.re -= fr.re * v.re - fr.im * v.im;
+      w.im -= fr.re * v.im + fr.im * v.re;
+      return w;
+    };
+    if (c + 1 == n) {
+      for (int o = threadIdx.x; o < 2 * nn; o += kSolveThreads) N[o] = step1(o % n, o / n);
+    } else {
+      const int c1 = c + 1;
+      const cplx<double> dg1 = step1(c1, c1);
+      const double dd1 = dg1.re * dg1.re + dg1.im * dg1.im;
+      const cplx<double> dinv1 = mk<double>(dg1.re / dd1, -dg1.im / dd1);
+      for (int o = threadIdx.x; o < 2 * nn; o += kSolveThreads) {
+        const int r = o % n, k = o / n;
+        const cplx<double> v = cmul(step1(c1, k), dinv1);
+        cplx<double> w;
+        if (r == c1) {
+          w = v;
+        } else {
+          const cplx<double> fr = step1(r, c1);
+          w = step1(r, k);
+          w.re -= fr.re * v.re - fr.im * v.im;
+          w.im -= fr.re * v.im + fr.im * v.re;
+        }
+        N[o] = w;
       }
-      N[o] = w;
     }
     __syncthreads();
     cur ^= 1;
@@ -550,8 +570,10 @@ __global__ void __launch_bounds__(kSolveThreads)
   const cplx<double>* X = lds + cur * 2 * nn + nn;
   for (int o = threadIdx.x; o < nn; o += kSolveThreads) pool[it.x + o] = X[o];
   if (it.r < 0) return;
-  // W(k, m) = sum_l R(k, l) X(m, l); R staged in the free buffer
+  // W(k, m) = sum_l R(k, l) X(m, l); R staged in the free buffer, W kept
+  // beside it for the derivative tiles
   cplx<double>* Rm = lds + (cur ^ 1) * 2 * nn;
+  cplx<double>* Wm = Rm + nn;
   for (int o = threadIdx.x; o < nn; o += kSolveThreads) Rm[o] = pool[it.r + o];
   __syncthreads();
   for (int o = threadIdx.x; o < nn; o += kSolveThreads) {
@@ -559,33 +581,65 @@ __global__ void __launch_bounds__(kSolveThreads)
     cplx<double> acc = czero<double>();
     for (int l = 0; l < n; ++l) cfma(acc, Rm[k + l * n], X[m + l * n]);
     pool[it.w + o] = acc;
+    Wm[o] = acc;
   }
   if (it.out < 0) return;
-  __syncthreads();  // W, written to the pool above, is read back by this workgroup's MFMA tiles
-  // D = f (R x_last Y) - f (W x_last A): the mode products of the grouped
-  // path on the node's (rows x n) tiles, one wavefront per tile
-  GmDesc gy{};
-  gy.bef = it.rows;
-  gy.nin = gy.nout = n;
-  gy.aft = 1;
-  gy.op = GM_N;
-  gy.are = 1.0;
-  gy.ksplit = 1;
-  gy.x = it.y;
-  gy.m = it.r;
-  gy.out = it.out;
-  GmDesc ga = gy;
-  ga.x = it.a;
-  ga.m = it.w;
-  const int64_t tiles = gm_tasks(gy);
+  __syncthreads();
+  // D = f (R x_last Y) - f (W x_last A) on the node's (rows x n) tiles, one
+  // wavefront per 16 x 16 tile: the two products of a tile load their Y and A
+  // operands together, R and W come from LDS, and each product keeps the
+  // accumulators and alpha scaling of the grouped mode product.
+  typedef MF<double>::v4 v4;
+  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const int rows = it.rows, CT = (n + 15) >> 4;
+  const int64_t tiles = (int64_t)((rows + 15) >> 4) * CT;
+  const cplx<double>* ya = ybase + it.y;
+  const cplx<double>* aa = abase + it.a;
   for (int64_t t = wave_id(); t < tiles; t += kSolveThreads / 64) {
-    const GmTile gt = gm_tile(gy, t);
-    cplx<double> vy[4], va[4];
-    gm_product<double, double, double>(gy, gt, ybase, pool, fre, fim, vy);
-    gm_product<double, double, double>(ga, gt, abase, pool, -fre, -fim, va);
+    const int rt = (int)(t / CT), ct = (int)(t - (int64_t)rt * CT);
+    const int r = rt * 16 + li, j = ct * 16 + li;
+    const bool rv = r < rows, cv = j < n;
+    v4 yr0 = {0, 0, 0, 0}, yi0 = yr0, yr1 = yr0, yi1 = yr0, ar0 = yr0, ai0 = yr0, ar1 = yr0, ai1 = yr0;
+    for (int k0 = 0; k0 < n; k0 += 16) {
+      cplx<double> ay[4], av[4], by[4], bw[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) vy[q] = cadd(vy[q], va[q]);
-    gm_store<double, double>(gy, gt, vy, obase);
+      for (int u = 0; u < 4; ++u) {
+        const int kx = k0 + 4 * u + kk;
+        const bool kv = kx < n;
+        ay[u] = ldz(ya, r + (int64_t)kx * rows, rv && kv);
+        av[u] = ldz(aa, r + (int64_t)kx * rows, rv && kv);
+        const int mo = (cv && kv) ? j + kx * n : 0;
+        const cplx<double> rm = Rm[mo], wm = Wm[mo];
+        by[u] = (cv && kv) ? rm : czero<double>();
+        bw[u] = (cv && kv) ? wm : czero<double>();
+      }
+      cmma<double>(yr0, yi0, ay[0], by[0]);
+      cmma<double>(ar0, ai0, av[0], bw[0]);
+      if (k0 + 4 < n) {
+        cmma<double>(yr1, yi1, ay[1], by[1]);
+        cmma<double>(ar1, ai1, av[1], bw[1]);
+      }
+      if (k0 + 8 < n) {
+        cmma<double>(yr0, yi0, ay[2], by[2]);
+        cmma<double>(ar0, ai0, av[2], bw[2]);
+      }
+      if (k0 + 12 < n) {
+        cmma<double>(yr1, yi1, ay[3], by[3]);
+        cmma<double>(ar1, ai1, av[3], bw[3]);
+      }
+    }
+    if (!cv) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = rt * 16 + MF<double>::drow(lane, q);
+      if (rr >= rows) continue;
+      const double vr = yr0[q] + yr1[q], vi = yi0[q] + yi1[q];
+      const double wr = ar0[q] + ar1[q], wi = ai0[q] + ai1[q];
+      // f Y R^T and (-f) A W^T, each scaled as the mode product scales it
+      const cplx<double> py = mk<double>(fre * vr - fim * vi, fre * vi + fim * vr);
+      const cplx<double> pa = mk<double>(-fre * wr + fim * wi, -fre * wi - fim * wr);
+      obase[it.out + rr + (int64_t)j * rows] = cadd(py, pa);
+    }
   }
 }
 
