@@ -244,6 +244,18 @@ def main() -> int:
     sweep_s = (t1 - t0) / args.steps
     value = 1.0 / sweep_s
 
+    # the same sweep with host buffers at the boundary: the state copied in
+    # from host memory and the derivative copied back out, per sweep (the
+    # PCIe-inclusive rate; not the headline value)
+    n_host = max(5, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(n_host):
+        eng.set_psi(psi)
+        eng.derivative(sync=False)
+        eng.get(0)
+    t1 = time.perf_counter()
+    host_rate = n_host / (t1 - t0)
+
     # the same K sweeps launched eagerly with HIP events around every launch
     # on the engine's streams (each sweep queued behind a short device-side
     # hold, so the events time kernels rather than host launch gaps):
@@ -306,6 +318,9 @@ def main() -> int:
         "kernels": {k: {"ms_per_step": round(v["ms"] / args.steps, 4),
                         "launches_per_step": v["launches"] / args.steps} for k, v in timing.items()},
         "energy": energy,
+        "host_buffers": {"sweeps_per_s": round(host_rate, 3),
+                         "per_sweep": "state in from host memory, derivative out to host memory "
+                                      f"({16 * int(eng.psi_size)} bytes each way)"},
         "roofline": roofline,
     }
     if not args.no_forms:
