@@ -64,9 +64,6 @@ __device__ inline void cmma(typename MF<T>::v4& cr, typename MF<T>::v4& ci, cons
 
 constexpr int kWaves = kGemmWaves;  // wavefronts (tasks) per workgroup
 
-// Descriptor of a task: wgdesc[blockIdx.x] is the descriptor of the
-// workgroup's first task (precomputed on the host), and a task lies at most
-// kWaves - 1 descriptors further.
 // Wavefront index in the workgroup, as a wave-uniform (scalar) value, so that
 // descriptor loads and address arithmetic stay on the scalar unit.
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -81,13 +78,8 @@ __device__ inline cplx<T> ldz(const cplx<T>* __restrict__ base, int64_t off, boo
   return ok ? v : czero<T>();
 }
 
-template <typename D>
-__device__ inline int task_desc(const D* __restrict__ descs, int ndesc, const int32_t* __restrict__ wgdesc,
-                                int64_t task) {
-  int di = wgdesc[blockIdx.x];
-  while (di + 1 < ndesc && descs[di + 1].tile0 <= task) ++di;
-  return di;
-}
+// Descriptor of a task: one load from the per-task table built on the host.
+__device__ inline int task_desc(const int32_t* __restrict__ wgdesc, int64_t task) { return wgdesc[task]; }
 
 // Split-K inside a workgroup: the ksplit (2 or 4) chunks of a split tile are
 // consecutive tasks starting at a multiple of ksplit, so they are wavefronts
@@ -233,7 +225,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   bool mine = task < ntasks;  // uniform per wavefront
   GmDesc d{};
   if (mine) {
-    d = descs[task_desc(descs, ndesc, wgdesc, task)];
+    d = descs[task_desc(wgdesc, task)];
     mine = !(active && d.node >= 0 && !active[d.node]);
   }
   GmTile g{0, 0, 0};
@@ -375,7 +367,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   bool mine = task < ntasks;  // uniform per wavefront
   GcDesc d{};
   if (mine) {
-    d = descs[task_desc(descs, ndesc, wgdesc, task)];
+    d = descs[task_desc(wgdesc, task)];
     mine = !(active && d.node >= 0 && !active[d.node]);
   }
   GcTile g{0, 0, 0};

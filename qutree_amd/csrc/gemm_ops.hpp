@@ -64,10 +64,10 @@ __host__ __device__ inline int64_t gc_tasks(const GcDesc& d) {
 // matrix base cplx<TM>* (converted to T on load), the output cplx<TO>*; the
 // products accumulate in T.  `active` may be null.  Every alpha of a
 // mode-product launch is multiplied by the complex scale (sre, sim).
-// wgdesc[b]: descriptor of the first task (kGemmWaves * b) of workgroup b
-// (see gemm_wg_table).  The chunks of a split tile sum in LDS inside their
-// workgroup, which needs the task order of order_tasks.  No two descriptors
-// of one launch may write the same output.
+// wgdesc[t]: descriptor of task t (see gemm_wg_table), so a wavefront finds
+// its descriptor with one load.  The chunks of a split tile sum in LDS inside
+// their workgroup, which needs the task order of order_tasks.  No two
+// descriptors of one launch may write the same output.
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, const uint8_t* active, double sre, double sim,
@@ -100,12 +100,12 @@ inline int64_t order_tasks(std::vector<D>& descs, int64_t (*tasks)(const D&)) {
   return t;
 }
 
-// Appends to `out` the first-descriptor index of every workgroup of a launch
-// over `descs` (tile0 already assigned, ntasks in total).
+// Appends to `out` the descriptor index of every task of a launch over
+// `descs` (tile0 already assigned, ntasks in total).
 template <typename D>
 inline void gemm_wg_table(const D* descs, int64_t ndesc, int64_t ntasks, std::vector<int32_t>& out) {
   int64_t di = 0;
-  for (int64_t t = 0; t < ntasks; t += kGemmWaves) {
+  for (int64_t t = 0; t < ntasks; ++t) {
     while (di + 1 < ndesc && descs[di + 1].tile0 <= t) ++di;
     out.push_back((int32_t)di);
   }
