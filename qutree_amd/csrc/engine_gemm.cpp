@@ -506,9 +506,52 @@ int Engine::plan_gemm() {
   push_gm(gApply_, g2, P64, B_PSI64, B_APPLY64, B_POOL64, B_DPSI64, true);
   phaseFlops_[3] = flops - ff0;
 
-  // combine flops (aggregated slots of the shared plan, stacked blocks)
-  double comb = combFlops12_;
-  for (auto& co : h_comb3_) comb += 8.0 * co.n * co.n;
+  // Two combination launches per sweep: the aggregated slots, then the sums
+  // over them together with the stacked Apply blocks.  A block whose matrix
+  // is such a sum is expanded into the sum's terms (over aggregated slots),
+  // so no output of the second launch feeds another.
+  {
+    std::map<int64_t, int32_t> sum2;  // output offset -> index in h_comb2_
+    for (size_t i = 0; i < h_comb2_.size(); ++i) sum2[h_comb2_[i].out] = (int32_t)i;
+    std::vector<CombOut> mo = h_comb2_;
+    std::vector<CombTerm> mt = h_combt2_;
+    for (const CombOut& co : h_comb3_) {
+      CombOut m = co;
+      m.t0 = (int32_t)mt.size();
+      for (int32_t q = co.t0; q < co.t1; ++q) {
+        const CombTerm& t = h_combt3_[q];
+        auto e = (t.src >= 0) ? sum2.find(t.src) : sum2.end();
+        if (e == sum2.end()) {
+          mt.push_back(t);
+          continue;
+        }
+        const CombOut& c2 = h_comb2_[e->second];
+        for (int32_t r = c2.t0; r < c2.t1; ++r) {
+          const CombTerm& u = h_combt2_[r];
+          CombTerm x{};
+          x.cre = t.cre * u.cre - t.cim * u.cim;
+          x.cim = t.cre * u.cim + t.cim * u.cre;
+          x.src = u.src;
+          mt.push_back(x);
+        }
+      }
+      m.t1 = (int32_t)mt.size();
+      mo.push_back(m);
+    }
+    std::set<int64_t> outs;
+    for (const CombOut& co : mo) outs.insert(co.out);
+    for (const CombTerm& t : mt)
+      if (t.src >= 0 && outs.count(t.src)) {
+        set_error("plan: a combination reads another output of its launch");
+        return QTG_ESTATE;
+      }
+    h_comb3_ = mo;
+    h_combt3_ = mt;
+  }
+  // combine flops (aggregated slots; sums over them and stacked blocks)
+  double comb = 0.0;
+  for (auto& co : h_comb1_) comb += 8.0 * co.n * co.n * (double)(co.t1 - co.t0);
+  for (auto& co : h_comb3_) comb += 8.0 * co.n * co.n * (double)(co.t1 - co.t0);
   phaseFlops_[5] = comb;
   flops += comb;
   nComb3_ = (int64_t)h_comb3_.size();
@@ -799,10 +842,8 @@ int Engine::build_gemm(hipStream_t s) {
   QTG_HIP_TRY(hipStreamWaitEvent(s, evRho_, 0));
   if ((rc = run_launches(gHole_, bases, nullptr, 1.0, 0.0, s))) return rc;
   if ((rc = begin_mark(5, s))) return rc;
-  if (nComb_ > 0) {
-    if ((rc = launch_combine<double>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
-    if ((rc = launch_combine<double>(d_combOut2_, nComb2_, d_combTerm2_, d_pool_, s))) return rc;
-  }
+  // the aggregated slots, then the sums over them with the stacked blocks
+  if ((rc = launch_combine<double>(d_combOut_, nComb_, d_combTerm_, d_pool_, s))) return rc;
   if ((rc = launch_combine<double>(d_combOut3_, nComb3_, d_combTerm3_, d_pool_, s))) return rc;
   if ((rc = end_mark(s))) return rc;
   if (timing_) tfl_[5] += phaseFlops_[5];
