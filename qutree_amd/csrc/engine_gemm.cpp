@@ -116,9 +116,11 @@ int Engine::plan_gemm() {
       if (d.op != GM_COPY) fl += 8.0 * d.bef * d.aft * (double)d.nin * d.nout;
     }
     const int64_t t = order_tasks(descs, gm_tasks);
-    h_gm_.insert(h_gm_.end(), descs.begin(), descs.end());
-    gl.wg0 = (int64_t)h_gmwg_.size();
-    gemm_wg_table(h_gm_.data() + gl.d0, gl.nd, t, h_gmwg_);
+    // one descriptor per task (its copy keeps tile0), so a wavefront's
+    // descriptor is one load at descs[task]
+    for (auto& d : descs) h_gm_.insert(h_gm_.end(), (size_t)gm_tasks(d), d);
+    gl.nd = t;
+    gl.wg0 = -1;
     gl.ntasks = t;
     gl.x0 = x0;
     gl.x1 = x1;
@@ -164,9 +166,9 @@ int Engine::plan_gemm() {
       fl += 8.0 * d.bef * d.aft * (double)d.nx * d.ny;
     }
     const int64_t t = order_tasks(descs, gc_tasks);
-    h_gc_.insert(h_gc_.end(), descs.begin(), descs.end());
-    gl.wg0 = (int64_t)h_gcwg_.size();
-    gemm_wg_table(h_gc_.data() + gl.d0, gl.nd, t, h_gcwg_);
+    for (auto& d : descs) h_gc_.insert(h_gc_.end(), (size_t)gc_tasks(d), d);
+    gl.nd = t;
+    gl.wg0 = -1;
     gl.ntasks = t;
     gl.x0 = x0;
     gl.x1 = x1;
@@ -777,7 +779,7 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
     if (gl.kind == 0) {
       if ((rc = kernel_mark(6, &kt))) return rc;
       const GmDesc* d = d_gm_ + gl.d0;
-      const int32_t* wg = d_gmwg_ + gl.wg0;
+      const int32_t* wg = gl.wg0 >= 0 ? d_gmwg_ + gl.wg0 : nullptr;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *m = bases[gl.m], *o = bases[gl.o];
       if (gl.prec == P64)
         rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s,
@@ -793,7 +795,7 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
     } else if (gl.kind == 1) {
       if ((rc = kernel_mark(7, &kt))) return rc;
       const GcDesc* d = d_gc_ + gl.d0;
-      const int32_t* wg = d_gcwg_ + gl.wg0;
+      const int32_t* wg = gl.wg0 >= 0 ? d_gcwg_ + gl.wg0 : nullptr;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *o = bases[gl.o];
       if (gl.prec == P64)
         rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt);

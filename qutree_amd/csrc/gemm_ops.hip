@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   bool mine = task < ntasks;  // uniform per wavefront
   GmDesc d{};
   if (mine) {
-    d = descs[task_desc(wgdesc, task)];
+    d = descs[wgdesc ? task_desc(wgdesc, task) : task];
     mine = !(active && d.node >= 0 && !active[d.node]);
   }
   GmTile g{0, 0, 0};
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   bool mine = task < ntasks;  // uniform per wavefront
   GcDesc d{};
   if (mine) {
-    d = descs[task_desc(wgdesc, task)];
+    d = descs[wgdesc ? task_desc(wgdesc, task) : task];
     mine = !(active && d.node >= 0 && !active[d.node]);
   }
   GcTile g{0, 0, 0};

@@ -64,8 +64,9 @@ __host__ __device__ inline int64_t gc_tasks(const GcDesc& d) {
 // matrix base cplx<TM>* (converted to T on load), the output cplx<TO>*; the
 // products accumulate in T.  `active` may be null.  Every alpha of a
 // mode-product launch is multiplied by the complex scale (sre, sim).
-// wgdesc[t]: descriptor of task t (see gemm_wg_table), so a wavefront finds
-// its descriptor with one load.  The chunks of a split tile sum in LDS inside
+// wgdesc[t]: descriptor of task t (see gemm_wg_table); null when descs holds
+// one descriptor per task (descs[t]).  Either way a wavefront finds its
+// descriptor without a search.  The chunks of a split tile sum in LDS inside
 // their workgroup, which needs the task order of order_tasks.  No two
 // descriptors of one launch may write the same output.
 template <typename T, typename TM, typename TO>
