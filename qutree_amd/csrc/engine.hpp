@@ -112,6 +112,7 @@ class Engine {
     double flops;
     double bytes;  // algorithmic HBM bytes (distinct inputs once, outputs)
   };
+  bool gramInSolve_ = false;  // the solve launch forms S, A^H Y and the energy terms
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
                    hipStream_t s) {
     return run_launches(L, 0, L.size(), bases, active, sre, sim, s);

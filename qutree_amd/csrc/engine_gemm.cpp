@@ -424,6 +424,9 @@ int Engine::plan_gemm() {
   // use two more mode-product launches for it.
   const double ff0 = flops;
   const bool fusedFinish = nspf_ <= 48;
+  // with SPF counts up to 16 the solve also forms S = A^H A, A^H Y and the
+  // top node's energy terms, and the Gram launch is dropped
+  gramInSolve_ = nspf_ <= 16;
   std::vector<GcDesc> fin;
   std::vector<GmDesc> g1, g2;
   energySlot_ = pool64Off;
@@ -450,7 +453,9 @@ int Engine::plan_gemm() {
       fin.push_back(g);
       if (fusedFinish) {
         SolveItem si{};
-        si.s = si.x = si.r = si.w = si.a = -1;
+        si.s = si.r = si.w = -1;
+        si.x = gramInSolve_ ? energySlot_ : -1;  // <A|Y>, <A|A>
+        si.a = v.off;
         si.y = si.out = v.off;
         si.n = n;
         si.rows = (int32_t)(v.N / n);
@@ -487,7 +492,7 @@ int Engine::plan_gemm() {
     m2.acc = 1;
     g2.push_back(m2);
   }
-  push_gc(gApply_, fin, P64, B_PSI64, B_APPLY64, B_POOL64);
+  if (!gramInSolve_) push_gc(gApply_, fin, P64, B_PSI64, B_APPLY64, B_POOL64);
   {
     GLaunch gl{};
     gl.kind = 2;
@@ -499,6 +504,7 @@ int Engine::plan_gemm() {
     for (auto& si : h_solve_) {
       if (si.s >= 0) fl += 8.0 * 2.0 * si.n * si.n * si.n;
       if (si.s >= 0 && si.out >= 0) fl += 8.0 * 2.0 * si.n * si.n * (double)si.rows;  // the projected derivative
+      if (gramInSolve_) fl += 8.0 * 2.0 * (double)si.rows * si.n * (si.s >= 0 ? si.n : 1);  // S, A^H Y / energy
     }
     gl.flops = fl;
     flops += fl;
@@ -807,7 +813,7 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
       if ((rc = kernel_mark(8, &kt))) return rc;
       if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s, kt,
                              (const cplx<double>*)bases[B_PSI64], (const cplx<double>*)bases[B_APPLY64],
-                             (cplx<double>*)bases[B_DPSI64], a, b)))
+                             (cplx<double>*)bases[B_DPSI64], a, b, gramInSolve_)))
         return rc;
       if (timing_) tfl_[8] += gl.flops;
     }

@@ -493,23 +493,79 @@ constexpr int kSolveThreads = 512;  // 8 wavefronts: the derivative tiles of a n
 __global__ void __launch_bounds__(kSolveThreads)
     solve_pp_kernel(const SolveItem* __restrict__ items, cplx<double>* pool, const uint8_t* __restrict__ active,
                     const cplx<double>* __restrict__ abase, const cplx<double>* __restrict__ ybase,
-                    cplx<double>* obase, double fre, double fim) {
+                    cplx<double>* obase, double fre, double fim, int nmax, int gram) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const SolveItem it = items[blockIdx.x];
   if (active && it.node >= 0 && !active[it.node]) return;
   const int n = it.n, nn = n * n;
   const cplx<double> f = mk<double>(fre, fim);
+  // the buffers are addressed from the LDS base (an array of pointers would
+  // turn every access into a generic one); the Gram partials follow the two
+  // [S | X] buffers
+  cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+  cplx<double>* part = lds + 4 * nmax * nmax;
+  const int lane = threadIdx.x & 63;
   if (it.s < 0) {
-    // the top node: D = f Y
-    for (int e = threadIdx.x; e < it.rows * n; e += kSolveThreads) obase[it.out + e] = cmul(f, ybase[it.y + e]);
+    // the top node: D = f Y; with gram also <A|Y> and <A|A>
+    cplx<double> ay = czero<double>(), aa = czero<double>();
+    for (int e = threadIdx.x; e < it.rows * n; e += kSolveThreads) {
+      const cplx<double> y = ybase[it.y + e];
+      obase[it.out + e] = cmul(f, y);
+      if (gram && it.x >= 0) {
+        const cplx<double> a = abase[it.a + e];
+        cfmac(ay, a, y);  // conj(a) y
+        cfmac(aa, a, a);
+      }
+    }
+    if (gram && it.x >= 0) {
+      part[threadIdx.x] = ay;
+      part[kSolveThreads + threadIdx.x] = aa;
+      __syncthreads();
+      for (int h = kSolveThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+          part[threadIdx.x] = cadd(part[threadIdx.x], part[threadIdx.x + h]);
+          part[kSolveThreads + threadIdx.x] = cadd(part[kSolveThreads + threadIdx.x], part[kSolveThreads + threadIdx.x + h]);
+        }
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        pool[it.x] = part[0];
+        pool[it.x + 1] = part[kSolveThreads];
+      }
+    }
     return;
   }
-  // the two buffers are addressed from the LDS base (an array of pointers
-  // would turn every access into a generic one)
-  cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
-  for (int o = threadIdx.x; o < nn; o += kSolveThreads) {
-    lds[o] = pool[it.s + o];
-    lds[nn + o] = pool[it.x + o];
+  if (gram) {
+    // S = A^H A (wavefronts 0-3) and X = A^H Y (4-7), each one 16 x 16 tile
+    // summed over four K chunks of the rows, partials added in chunk order
+    const int w = wave_id(), prod = w >> 2;
+    GcDesc gd{};
+    gd.x = it.a;
+    gd.y = prod ? it.y : it.a;
+    gd.bef = it.rows;
+    gd.aft = 1;
+    gd.nx = gd.ny = n;
+    gd.ksplit = 4;
+    const GcTile gt{0, 0, w & 3};
+    cplx<double> v[4];
+    gc_product<double, double>(gd, gt, abase, prod ? ybase : abase, v);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) part[w * 256 + q * 64 + lane] = v[q];
+    __syncthreads();
+    for (int o = threadIdx.x; o < 512; o += kSolveThreads) {
+      const int pr = o >> 8, idx = o & 255, q = idx >> 6, l = idx & 63;
+      const int row = MF<double>::drow(l, q), col = l & 15;
+      if (row < n && col < n) {
+        cplx<double> acc = part[(pr * 4) * 256 + idx];
+        for (int c = 1; c < 4; ++c) acc = cadd(acc, part[(pr * 4 + c) * 256 + idx]);
+        lds[pr * nn + row + col * n] = acc;
+      }
+    }
+  } else {
+    for (int o = threadIdx.x; o < nn; o += kSolveThreads) {
+      lds[o] = pool[it.s + o];
+      lds[nn + o] = pool[it.x + o];
+    }
   }
   __syncthreads();
   int cur = 0;
@@ -582,7 +638,7 @@ __global__ void __launch_bounds__(kSolveThreads)
   // operands together, R and W come from LDS, and each product keeps the
   // accumulators and alpha scaling of the grouped mode product.
   typedef MF<double>::v4 v4;
-  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const int li = lane & 15, kk = lane >> 4;
   const int rows = it.rows, CT = (n + 15) >> 4;
   const int64_t tiles = (int64_t)((rows + 15) >> 4) * CT;
   const cplx<double>* ya = ybase + it.y;
@@ -690,12 +746,19 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 
 int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
                  hipStream_t s, const KernelTiming* kt, const cplx<double>* abase, const cplx<double>* ybase,
-                 cplx<double>* obase, double fre, double fim) {
+                 cplx<double>* obase, double fre, double fim, bool gram) {
   if (n <= 0) return QTG_OK;
+  if (gram && nmax > 16) {
+    set_error("solve: the Gram matrices are formed in the solve only for n <= 16");
+    return QTG_EARG;
+  }
   if (nmax <= 48) {
-    const size_t lds = 4 * sizeof(cplx<double>) * (size_t)nmax * nmax;
+    // two [S | X] buffers, then (gram) the 8 partial 16 x 16 tiles, which
+    // also hold the top node's 2 x kSolveThreads partial sums
+    static_assert(2 * kSolveThreads <= 8 * 256, "gram partials");
+    const size_t lds = sizeof(cplx<double>) * (4 * (size_t)nmax * nmax + (gram ? 8 * 256 : 0));
     launch_timed(solve_pp_kernel, dim3((unsigned)n), dim3(kSolveThreads), (unsigned)lds, s, kt, items, pool, active,
-                 abase, ybase, obase, fre, fim);
+                 abase, ybase, obase, fre, fim, nmax, gram ? 1 : 0);
     QTG_LAUNCH_CHECK();
     return QTG_OK;
   }

@@ -125,6 +125,9 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
 //   D(:, i) = f (sum_l R(i, l) Y(:, l) - sum_l W(i, l) A(:, l))
 // with A at a, Y at y, D at out in their own base pointers and f the launch
 // scale; an item with s < 0 only writes D = f Y (the top node).
+// With gram (nmax <= 16 only) the solve forms S = A^H A and X = A^H Y itself
+// instead of reading them, and a top-node item with x >= 0 writes <A|Y> and
+// <A|A> (rows x n elements) to the pool at x and x + 1.
 struct SolveItem {
   int64_t s, x, r, w;
   int64_t a, y, out;
@@ -133,6 +136,6 @@ struct SolveItem {
 int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax, const uint8_t* active,
                  hipStream_t s, const KernelTiming* kt = nullptr, const cplx<double>* abase = nullptr,
                  const cplx<double>* ybase = nullptr, cplx<double>* obase = nullptr, double fre = 0.0,
-                 double fim = 0.0);
+                 double fim = 0.0, bool gram = false);
 
 }  // namespace qtg
