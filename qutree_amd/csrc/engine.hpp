@@ -111,6 +111,7 @@ class Engine {
     bool scaled;       // multiply alphas by the launch scale (-i phase)
     double flops;
     double bytes;  // algorithmic HBM bytes (distinct inputs once, outputs)
+    int waves;     // mode products: tasks per workgroup (0: kGemmWaves)
   };
   bool gramInSolve_ = false;  // the solve launch forms S, A^H Y and the energy terms
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,

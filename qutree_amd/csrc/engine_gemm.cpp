@@ -35,7 +35,7 @@ namespace qtg {
 
 namespace {
 
-constexpr int kGmSplitK = 128;  // mode products longer than 2 kGmSplitK split along K
+constexpr int kGmSplitK = 32;  // mode products longer than 2 kGmSplitK split along K, chunks >= kGmSplitK
 constexpr int kGcSplitK = 512;  // hole contractions longer than 2 kGcSplitK split along K
 
 void view(const HNode& v, int k, int64_t& bef, int& nk, int64_t& aft) {
@@ -98,6 +98,7 @@ int Engine::plan_gemm() {
     gl.d0 = (int64_t)h_gm_.size();
     gl.nd = (int64_t)descs.size();
     double fl = 0.0;
+    gl.waves = kGemmWaves;
     // algorithmic bytes: every distinct input tensor and matrix once, every
     // output written once (and read once when accumulated)
     const double ein = prec == P64 ? 16.0 : 8.0, eout = prec == P32 ? 8.0 : 16.0;
@@ -112,7 +113,10 @@ int Engine::plan_gemm() {
     for (auto& d : descs) {
       // long contractions (stacked Apply summands) are split along K into
       // chunks that sum inside their workgroup
-      if (d.op != GM_COPY && d.nin > 2 * kGmSplitK) d.ksplit = split_chunks(d.nin, kGemmWaves, kGmSplitK / 2);
+      if (d.op != GM_COPY && d.nin > 2 * kGmSplitK) {
+        d.ksplit = split_chunks(d.nin, kGemmWavesWide, kGmSplitK, kGemmWavesWide);
+        if (d.ksplit > kGemmWaves) gl.waves = kGemmWavesWide;
+      }
       if (d.op != GM_COPY) fl += 8.0 * d.bef * d.aft * (double)d.nin * d.nout;
     }
     const int64_t t = order_tasks(descs, gm_tasks);
@@ -787,15 +791,13 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
       const GmDesc* d = d_gm_ + gl.d0;
       const int32_t* wg = gl.wg0 >= 0 ? d_gmwg_ + gl.wg0 : nullptr;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *m = bases[gl.m], *o = bases[gl.o];
+      const int wv = gl.waves > 0 ? gl.waves : kGemmWaves;
       if (gl.prec == P64)
-        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s,
-                                               kt);
+        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv);
       else if (gl.prec == P32)
-        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s,
-                                             kt);
+        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv);
       else
-        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s,
-                                              kt);
+        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv);
       if (rc) return rc;
       if (timing_) tfl_[6] += gl.flops, tby_[6] += gl.bytes;
     } else if (gl.kind == 1) {

@@ -215,13 +215,13 @@ __device__ inline void gm_store(const GmDesc& d, const GmTile& g, const cplx<TO>
   }
 }
 
-template <typename T, typename TM, typename TO>
-__global__ void __launch_bounds__(64 * kWaves)
+template <typename T, typename TM, typename TO, int W>
+__global__ void __launch_bounds__(64 * W)
     gm_grouped(const GmDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
                const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, const cplx<TM>* __restrict__ M,
                cplx<TO>* O, const uint8_t* __restrict__ active, double sre, double sim) {
-  __shared__ cplx<TO> red[kWaves * 256];
-  const int64_t task = (int64_t)blockIdx.x * kWaves + wave_id();
+  __shared__ cplx<TO> red[W * 256];
+  const int64_t task = (int64_t)blockIdx.x * W + wave_id();
   bool mine = task < ntasks;  // uniform per wavefront
   GmDesc d{};
   if (mine) {
@@ -696,11 +696,19 @@ __global__ void __launch_bounds__(kSolveThreads)
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, const uint8_t* active, double sre, double sim,
-              hipStream_t s, const KernelTiming* kt) {
+              hipStream_t s, const KernelTiming* kt, int waves) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
-  const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  launch_timed(gm_grouped<T, TM, TO>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc, ntasks,
-               (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase, (cplx<TO>*)obase, active, sre, sim);
+  if (waves == kGemmWavesWide) {
+    const unsigned blocks = (unsigned)((ntasks + kGemmWavesWide - 1) / kGemmWavesWide);
+    launch_timed(gm_grouped<T, TM, TO, kGemmWavesWide>, dim3(blocks), dim3(64 * kGemmWavesWide), 0, s, kt, descs,
+                 wgdesc, (int)ndesc, ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase,
+                 (cplx<TO>*)obase, active, sre, sim);
+  } else {
+    const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
+    launch_timed(gm_grouped<T, TM, TO, kWaves>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc,
+                 (int)ndesc, ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase,
+                 (cplx<TO>*)obase, active, sre, sim);
+  }
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -771,7 +779,7 @@ int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax
 #define QTG_GM(T, TM, TO)                                                                                      \
   template int launch_gm<T, TM, TO>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,  \
                                     const void*, void*, const uint8_t*, double, double, hipStream_t,            \
-                                    const KernelTiming*);
+                                    const KernelTiming*, int);
 QTG_GM(double, double, double)
 QTG_GM(float, float, float)
 QTG_GM(float, double, float)

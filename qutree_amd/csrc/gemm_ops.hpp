@@ -27,6 +27,9 @@ struct KernelTiming {
 };
 
 constexpr int kGemmWaves = 4;  // tasks (wavefronts) per workgroup of the grouped kernels
+// mode-product launches whose reductions split into 8 chunks run workgroups
+// of this many wavefronts (launch_gm's `waves`)
+constexpr int kGemmWavesWide = 8;
 
 struct GmDesc {
   int64_t x, m, out;                // complex-element offsets (input, matrix, output)
@@ -72,20 +75,20 @@ __host__ __device__ inline int64_t gc_tasks(const GcDesc& d) {
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, const uint8_t* active, double sre, double sim,
-              hipStream_t s, const KernelTiming* kt = nullptr);
+              hipStream_t s, const KernelTiming* kt = nullptr, int waves = kGemmWaves);
 template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, void* obase, const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr);
 
-// K split of a reduction of length K into kGemmWaves-bounded chunks: the
-// largest of 1, 2, 4 not above `want` whose chunks stay at least min_k long.
-inline int32_t split_chunks(int64_t K, int64_t want, int64_t min_k) {
+// K split of a reduction of length K into at most maxs chunks: the largest
+// power of two not above `want` whose chunks stay at least min_k long.
+inline int32_t split_chunks(int64_t K, int64_t want, int64_t min_k, int32_t maxs = kGemmWaves) {
   int32_t s = 1;
-  while (s < kGemmWaves && 2 * s <= want && K / (2 * s) >= min_k) s *= 2;
+  while (s < maxs && 2 * s <= want && K / (2 * s) >= min_k) s *= 2;
   return s;
 }
 
-// Orders descriptors by ksplit (4, then 2, then 1; stable within a class)
+// Orders descriptors by ksplit (largest first; stable within a class)
 // and assigns tile0, so that the chunks of every split tile are consecutive
 // tasks starting at a multiple of ksplit, i.e. wavefronts of one workgroup.
 // Returns the launch's task count.
