@@ -207,3 +207,37 @@ def test_graph_replays_match_eager():
         # eager work on the graph engine between replays
         g.build()
         g.synchronize()
+
+
+@pytest.mark.parametrize("amp", [0.3, 0.003])
+@pytest.mark.parametrize("name", ["cmf8", "f16"])
+def test_derivative_non_orthonormal_state(name, amp):
+    """Lower node tensors A (I + amp R) with R random.  amp = 0.3: S = A^H A
+    is far from the identity and the projector solve eliminates (the
+    Newton-Schulz steps need n max|I - S| < 1/2); amp = 0.003: several
+    Newton-Schulz steps.  Same bounds as the orthonormal cases for Apply and
+    the derivative; the energy (the top node's expectation value) assumes
+    orthonormal lower nodes and is not compared."""
+    text, H = _models()[name]
+    print(f"case engine derivative non-orthonormal {name} amp={amp}", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(5, delta_lowest=False)
+    rng = np.random.default_rng(11)
+    for nd in om.nodes:
+        if nd["parent"] < 0:
+            continue
+        dims = nd["dims"]
+        size, k = int(np.prod(dims)), int(dims[-1])
+        X = psi[nd["off"]:nd["off"] + size].reshape(-1, k, order="F")
+        R = rng.normal(size=(k, k)) + 1j * rng.normal(size=(k, k))
+        psi[nd["off"]:nd["off"] + size] = (X @ (np.eye(k) + amp * R)).ravel(order="F")
+    ref = om.derivative(psi)
+    app_ref = om.apply_all(psi)
+    eng = Q.Engine(text, H)
+    eng.set_psi(psi)
+    eng.derivative()
+    got = eng.get(0)
+    assert node_rel(eng, eng.get(1), app_ref) < 1e-11
+    err = derivative_err(eng, om, got, ref, app_ref)
+    print(f"  derivative err {err:.3e}", flush=True)
+    assert err < 1e-12
