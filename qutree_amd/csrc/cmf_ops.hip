@@ -154,8 +154,7 @@ __global__ void __launch_bounds__(TB)
         elementwise(N, MmidFinalOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off, op.h});
         break;
       case V_RZEXTR: {
-        // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2; one
-        // element per pass: its tableau column is loaded whole, then stored
+        // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2
         cd2* yest = bp(bufs, B_Y0) + off;
         cd2* ysav = bp(bufs, B_Y0 + 1) + off;
         cd2* dy = bp(bufs, B_Y0 + 2) + off;
@@ -168,44 +167,63 @@ __global__ void __launch_bounds__(TB)
         cd2* yt[kTab];
 #pragma unroll
         for (int k2 = 0; k2 < kTab; ++k2) yt[k2] = bp(bufs, B_YT0 + k2) + off;
-        for (int e = threadIdx.x; e < N; e += TB) {
-          if (op.i == 0) {
+        if (op.i == 0) {
+          for (int e = threadIdx.x; e < N; e += TB) {
             const cd2 v = yest[e];
             ysav[e] = v;
             dy[e] = v;
             yt0[e] = v;
-            continue;
           }
-          cd2 col[kTab];
+          break;
+        }
+        // two elements per pass: both tableau columns are loaded before any
+        // store, and the two extrapolation chains are independent
+        for (int e0 = threadIdx.x; e0 < N; e0 += 2 * TB) {
+          const int e1 = e0 + TB;
+          const bool has1 = e1 < N;
+          const int ee[2] = {e0, has1 ? e1 : e0};
+          cd2 col[2][kTab], yy[2], v[2];
 #pragma unroll
-          for (int k2 = 1; k2 < kTab; ++k2) col[k2] = yt[k2 < mi ? k2 : 0][e];
-          cd2 ddy = czero<double>();
-          cd2 yy = yest[e];
-          cd2 v = yt0[e];
-          cd2 c = yy;
-          yt0[e] = yy;
+          for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int k2 = 1; k2 < kTab; ++k2) col[h][k2] = yt[k2 < mi ? k2 : 0][ee[h]];
+            yy[h] = yest[ee[h]];
+            v[h] = yt0[ee[h]];
+          }
+          const cd2 yst[2] = {yy[0], yy[1]};
+          cd2 ddy[2] = {czero<double>(), czero<double>()}, c[2] = {yy[0], yy[1]};
 #pragma unroll
           for (int k2 = 1; k2 < kTab; ++k2) {
             if (k2 < mi) {
-              const cd2 bi = mk<double>(fx[k2] * v.re, fx[k2] * v.im);
-              cd2 b = mk<double>(bi.re - c.re, bi.im - c.im);
-              if (b.re != 0.0 || b.im != 0.0) {
-                b = cdiv(mk<double>(c.re - v.re, c.im - v.im), b);
-                ddy = cmul(c, b);
-                c = cmul(bi, b);
-              } else {
-                ddy = v;
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const cd2 bi = mk<double>(fx[k2] * v[h].re, fx[k2] * v[h].im);
+                cd2 b = mk<double>(bi.re - c[h].re, bi.im - c[h].im);
+                if (b.re != 0.0 || b.im != 0.0) {
+                  b = cdiv(mk<double>(c[h].re - v[h].re, c[h].im - v[h].im), b);
+                  ddy[h] = cmul(c[h], b);
+                  c[h] = cmul(bi, b);
+                } else {
+                  ddy[h] = v[h];
+                }
+                v[h] = col[h][k2];
+                col[h][k2] = ddy[h];
+                yy[h] = mk<double>(yy[h].re + ddy[h].re, yy[h].im + ddy[h].im);
               }
-              v = col[k2];
-              col[k2] = ddy;
-              yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
             }
           }
 #pragma unroll
-          for (int k2 = 1; k2 < kTab; ++k2)
-            if (k2 < mi) yt[k2][e] = col[k2];
-          dy[e] = ddy;
-          ysav[e] = yy;
+          for (int h = 0; h < 2; ++h) {
+            if (h == 0 || has1) {
+              const int e = ee[h];
+              yt0[e] = yst[h];
+#pragma unroll
+              for (int k2 = 1; k2 < kTab; ++k2)
+                if (k2 < mi) yt[k2][e] = col[h][k2];
+              dy[e] = ddy[h];
+              ysav[e] = yy[h];
+            }
+          }
         }
         break;
       }
