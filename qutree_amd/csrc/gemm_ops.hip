@@ -479,10 +479,11 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Systems with n <= 48: ping-pong Gauss-Jordan.  Each column step reads the
-// previous [S | X] buffer and writes the next one, so a step is one barrier:
-// every thread forms its elements of the scaled and eliminated matrix
-// directly from the old values.  S = A^H A is Hermitian positive definite
+// Systems with n <= 48: ping-pong Gauss-Jordan.  Each pair of column steps
+// reads the previous [S | X] buffer and writes the next one, so it costs one
+// barrier: every thread forms its elements of the scaled and eliminated
+// matrix directly from the old values (near-identity Gram matrices formed in
+// the launch take Newton-Schulz steps instead, see below).  S = A^H A is Hermitian positive definite
 // (the node's columns are independent), so its diagonal pivots are positive
 // and elimination needs no row exchanges; for the near-orthonormal A of a
 // sweep or a CMF substep S is diagonally dominant, and the partial pivoting

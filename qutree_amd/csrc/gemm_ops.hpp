@@ -120,9 +120,11 @@ int launch_gm_single(const GmDesc& d, const void* x, const void* m, void* o, hip
 template <typename T>
 int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hipStream_t s);
 
-// Batched FP64 solve X <- S^-1 X (Gauss-Jordan with partial pivoting) for n x n
-// column-major S and X at pool offsets s, x; n <= nmax <= 64 (one barrier
-// per column step when nmax <= 48).  When r >= 0 it also forms W = R X^T at w
+// Batched FP64 solve X <- S^-1 X for n x n column-major S and X at pool
+// offsets s, x; n <= nmax <= 64.  nmax > 48: Gauss-Jordan with partial
+// pivoting.  nmax <= 48: Gauss-Jordan on the diagonal pivots of the positive
+// definite S = A^H A, two columns per barrier, or (gram, near-identity S)
+// Newton-Schulz steps.  When r >= 0 it also forms W = R X^T at w
 // (R the n x n matrix at r).  When out >= 0 (nmax <= 48 only) it then writes
 // the projected derivative of the node, rows L = N / n, column-major:
 //   D(:, i) = f (sum_l R(i, l) Y(:, l) - sum_l W(i, l) A(:, l))
