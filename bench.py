@@ -109,6 +109,50 @@ def gradient_leg(Q, W, gd_steps):
             "monotone": bool(all(b < a for a, b in zip(energies, energies[1:])))}
 
 
+EIG_ARGS = dict(dt=0.05, out=2.0, iterations=3, acc_cmf=1e-7, acc_bs=1e-8)
+
+
+def eigenstates_leg(Q, W):
+    """Imaginary-time eigenstates (Eigenstates.cpp:60-103) of 8 coupled HO
+    modes with three states at the top: wall time of the whole run, and the
+    levels against the closed form E_0 = sum W_k / 2, E_0 + W_k."""
+    import numpy as np
+
+    text, H, K = W.eig8_model(3)
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(11, delta_lowest=False)
+    cmf = Q.CMF(eng)
+    cmf.eigenstates(psi, EIG_ARGS["dt"], EIG_ARGS["out"], 0)  # warm-up
+    Q.synchronize()
+    t0 = time.perf_counter()
+    r = cmf.eigenstates(psi, **EIG_ARGS)
+    Q.synchronize()
+    dt = time.perf_counter() - t0
+    w = np.sqrt(np.linalg.eigvalsh(K))
+    exact = np.concatenate([[0.5 * w.sum()], 0.5 * w.sum() + w[:2]])
+    return {"workload": "8 coupled HO modes (w_i = 1 + 0.25 i), 4 SPFs, 3 states; "
+                        f"{EIG_ARGS['iterations']} iterations of imaginary-time CMF over {EIG_ARGS['out']} a.u., "
+                        f"cmf {EIG_ARGS['acc_cmf']}, bs {EIG_ARGS['acc_bs']}",
+            "seconds": round(dt, 3), "iterations_per_s": round(EIG_ARGS["iterations"] / dt, 3),
+            "cmf_steps": r["steps"], "cmf_steps_per_s": round(r["steps"] / dt, 1),
+            "node_derivatives": r["derivs"], "levels": [float(x) for x in r["energies"][-1]],
+            "levels_rel_err_vs_closed_form": [float(x) for x in np.abs(r["energies"][-1] - exact) / exact]}, psi
+
+
+def cpu_eigenstates(psi):
+    """The oracle on the same eigenstates run, one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from qutree_amd import workloads as W
+
+    text, H, _ = W.eig8_model(3)
+    om = O.OracleModel(text, H)
+    t0 = time.perf_counter()
+    r = om.eigenstates(psi, **EIG_ARGS)
+    dt = time.perf_counter() - t0
+    return {"seconds": round(dt, 3), "cmf_steps": r["steps"], "cores": 1, "kind": "port"}
+
+
 def big_tree_leg(Q, W, steps=3):
     """One H|Psi> sweep on the 128-mode tree (8 DVR, SPF 8..48 toward the root,
     128 + 8128 coupled-quartic terms) in complex128 and complex64 bulk."""
@@ -383,10 +427,15 @@ def main() -> int:
         out["gradient"] = gradient_leg(Q, W, args.gd_steps)
         out["tree128"] = big_tree_leg(Q, W)
         out["node_microbench"] = microbench_leg(Q, W)
+        out["eigenstates"], eig_psi = eigenstates_leg(Q, W)
     if not args.no_cpu:
         base, e_ref = cpu_baseline(text, H, psi, energy)
         out["cpu_baseline"] = base
         out["energy_rel_err_vs_fp64_oracle"] = abs(energy - e_ref) / abs(e_ref)
+        if "eigenstates" in out:
+            ce = cpu_eigenstates(eig_psi)
+            out["eigenstates"]["cpu_oracle"] = ce
+            out["eigenstates"]["speedup_vs_cpu_oracle"] = round(ce["seconds"] / out["eigenstates"]["seconds"], 2)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -250,6 +250,24 @@ int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend,
                           int64_t max_steps, int64_t max_out, double* times, double* energies,
                           double* autocorr, int64_t* stats);
 
+/* Imaginary-time eigenstates, Eigenstates(IntegratorVariables&)
+ * (src/Applications/Eigenstates.cpp:60-103) on a CMF integrator: Eigenstate()
+ * (build, diagonalise the top node's <Psi_i|H|Psi_j>, rotate the top node into
+ * its eigenbasis; Eigenstates.cpp:18-31), then `iterations` times: CMF with
+ * phase -i over [0, out - 1e-5] (the step size carries over between
+ * iterations, as job.dt does), propagatorEnergies (-log(ev(S_top)) / (2 out),
+ * ascending ev; Eigenstates.cpp:8-16), orthonormal() (Gram-Schmidt at every
+ * node), Eigenstate(), and converged(i) = sum_j |<Psi_i|Psi_prev_j>|^2 (the s(i)
+ * of Status, Eigenstates.cpp:33-45).  Eigenvector columns are phased so that
+ * their first component is real and non-negative.  psi_host: initial state in,
+ * final state out.  energies: (iterations + 1) x nstates ascending eigenvalues;
+ * propagator, converged: iterations x nstates.  stats (4 entries): CMF steps,
+ * rejected steps, CMF builds, node derivative evaluations; dt_end: the step
+ * size on return.  nstates <= 64.  Returns nstates, or a negative qtg_status. */
+int64_t qtg_cmf_eigenstates(qtg_cmf cmf, double* psi_host, double dt, double out, int64_t iterations, double acc_cmf,
+                            double acc_bs, int64_t max_steps, double* energies, double* propagator,
+                            double* converged, int64_t* stats, double* dt_end);
+
 /* ------------------------------------------------------------------- TTNO */
 /* Tree tensor network operator on an engine's tree (TensorTreeOperator,
  * include/TreeOperators/TensorOperators/TensorTreeOperator.h:23-58; complex128

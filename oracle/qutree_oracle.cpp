@@ -1405,6 +1405,7 @@ struct CMFResult {
   vector<cd> autocorr;
   size_t steps = 0, rejected = 0, builds = 0, derivs = 0;
   vector<double> dts;
+  double dt_end = 0.;  // step size on return (written back to job.dt, CMFIntegrator.cpp:172)
 };
 
 struct CMF {
@@ -1502,9 +1503,84 @@ struct CMF {
     }
     res.builds = nBuild;
     for (auto& b : bs) res.derivs += b.nDeriv;
+    res.dt_end = dt;
     return res;
   }
 };
+
+// ---------------------------------------------------------------------------
+// Imaginary-time eigenstates (src/Applications/Eigenstates.cpp:8-103)
+// ---------------------------------------------------------------------------
+// propagatorEnergies (Eigenstates.cpp:8-16): eigenvalues of S_top, ascending,
+// mapped to E_i = -log(ev_i) / (2 out).
+vector<double> propagatorEnergies(const TensorTree& P, const Tree& t, double out) {
+  MatrixTree S = dotProductTree(P, P, t);
+  Mat U;
+  vector<double> ev;
+  heev(S[t.top()], U, ev);
+  for (auto& e : ev) e = -std::log(e) / (2. * out);
+  return ev;
+}
+
+// orthonormal (TensorTree_Implementation.h:190-195): Gram-Schmidt at every node
+void orthonormal(TensorTree& P, const Tree& t) {
+  for (auto& n : t.nodes) gramSchmidt(P[n.address]);
+}
+
+// Eigenstate (Eigenstates.cpp:18-31): build the matrices, diagonalise the
+// top-node matrix <Psi_i|H|Psi_j> and rotate the top node into its eigenbasis.
+vector<double> eigenstate(TensorTree& P, const SOP& H, HamRep& rep, const Tree& t) {
+  build(rep, H, P, t, 1e-10);
+  Mat hm = Expectation(H, P, rep, t);
+  Mat U;
+  vector<double> ev;
+  heev(hm, U, ev);
+  const size_t top = t.top();
+  P[top] = multStateArTB(U, P[top]);
+  return ev;
+}
+
+struct EigenResult {
+  vector<double> energies, propagator, converged;  // per output, nstates each
+  size_t steps = 0, rejected = 0, builds = 0, derivs = 0;
+  double dt_end = 0.;
+};
+
+// Eigenstates (Eigenstates.cpp:60-103): iterations x (CMF in imaginary time
+// over [0, out - 1e-5], propagator energies, orthonormal, Eigenstate,
+// overlap with the previous iterate).  converged(i) = sum_j |<Psi_i|Psi'_j>|^2
+// (the s(i) of Status, Eigenstates.cpp:33-45).
+EigenResult eigenstates(TensorTree& P, const SOP& H, const Tree& t, double dt, double out, size_t iterations,
+                        double acc_cmf, double acc_bs, size_t max_steps) {
+  EigenResult res;
+  CMF cmf(H, t, cd(0., -1.));
+  HamRep rep = makeHamRep(H, t);
+  vector<double> e = eigenstate(P, H, rep, t);
+  res.energies.insert(res.energies.end(), e.begin(), e.end());
+  for (size_t it = 0; it < iterations; ++it) {
+    TensorTree last = P;
+    CMFResult r = cmf.integrate(P, 0., out - 1e-5, dt, out, acc_cmf, acc_bs, max_steps);
+    dt = r.dt_end;
+    res.steps += r.steps;
+    res.rejected += r.rejected;
+    res.derivs = r.derivs;  // the node integrators count across calls
+    vector<double> pe = propagatorEnergies(P, t, out);
+    res.propagator.insert(res.propagator.end(), pe.begin(), pe.end());
+    orthonormal(P, t);
+    e = eigenstate(P, H, rep, t);
+    res.energies.insert(res.energies.end(), e.begin(), e.end());
+    MatrixTree ov = dotProductTree(P, last, t);
+    const Mat& o = ov[t.top()];
+    for (size_t i = 0; i < o.d1; ++i) {
+      double s = 0.;
+      for (size_t j = 0; j < o.d2; ++j) s += std::norm(o(i, j));
+      res.converged.push_back(s);
+    }
+  }
+  res.builds = cmf.nBuild;
+  res.dt_end = dt;
+  return res;
+}
 
 // ---------------------------------------------------------------------------
 // Model = tree + SOP (+ state), the unit the C-ABI hands around.
@@ -1908,6 +1984,58 @@ long orc_cmf(void* h, double* psi, double t0, double tend, double dt, double out
       stats[3] = (long)r.derivs;
     }
     return (long)r.times.size();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Top-node matrices after a build from psi: H(i,j) = <Psi_i|H|Psi_j>
+// (Expectation) and S = dotProduct(Psi, Psi)[top], column-major interleaved.
+int orc_top_matrices(void* h, const double* psi, double* hmat, double* smat) {
+  Model* m = (Model*)h;
+  try {
+    TensorTree P = m->unpack(psi);
+    build(m->rep, m->H, P, m->tree, 1e-10);
+    m->built = true;
+    Mat hm = Expectation(m->H, P, m->rep, m->tree);
+    MatrixTree S = dotProductTree(P, P, m->tree);
+    const Mat& st = S[m->tree.top()];
+    for (size_t i = 0; i < hm.c.size(); ++i) {
+      hmat[2 * i] = hm.c[i].real();
+      hmat[2 * i + 1] = hm.c[i].imag();
+      smat[2 * i] = st.c[i].real();
+      smat[2 * i + 1] = st.c[i].imag();
+    }
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Imaginary-time eigenstates; psi in/out.  energies: (iterations + 1) x
+// nstates, propagator and converged: iterations x nstates.  stats: [steps,
+// rejected, builds, derivs].  Returns nstates, or -1 on error.
+long orc_eigenstates(void* h, double* psi, double dt, double out, long iterations, double acc_cmf, double acc_bs,
+                     long max_steps, double* energies, double* propagator, double* converged, long* stats,
+                     double* dt_end) {
+  Model* m = (Model*)h;
+  try {
+    TensorTree P = m->unpack(psi);
+    EigenResult r = eigenstates(P, m->H, m->tree, dt, out, (size_t)iterations, acc_cmf, acc_bs, (size_t)max_steps);
+    m->pack(P, psi);
+    std::copy(r.energies.begin(), r.energies.end(), energies);
+    std::copy(r.propagator.begin(), r.propagator.end(), propagator);
+    std::copy(r.converged.begin(), r.converged.end(), converged);
+    if (stats) {
+      stats[0] = (long)r.steps;
+      stats[1] = (long)r.rejected;
+      stats[2] = (long)r.builds;
+      stats[3] = (long)r.derivs;
+    }
+    if (dt_end) *dt_end = r.dt_end;
+    return (long)m->tree.nodes[m->tree.top()].shape.lastDim();
   } catch (std::exception& e) {
     g_err = e.what();
     return -1;

@@ -87,6 +87,9 @@ def lib():
     L.qtg_cmf_destroy.argtypes = [_vp]
     L.qtg_cmf_integrate.argtypes = [_vp, _dp] + [C.c_double] * 8 + [_i64, _i64, _dp, _dp, _dp, _lp]
     L.qtg_cmf_integrate.restype = _i64
+    L.qtg_cmf_eigenstates.argtypes = ([_vp, _dp, C.c_double, C.c_double, _i64, C.c_double, C.c_double, _i64, _dp,
+                                       _dp, _dp, _lp, C.POINTER(C.c_double)])
+    L.qtg_cmf_eigenstates.restype = _i64
     L.qtg_engine_read_timing.argtypes = [_vp, _dp, _lp, _dp, _dp]
     _lib = L
     return L
@@ -359,6 +362,26 @@ class CMF:
         return dict(psi=p.view(np.complex128).copy(), times=times[:n], energies=en[:n],
                     autocorr=ac.view(np.complex128)[:n].copy(), steps=int(stats[0]), rejected=int(stats[1]),
                     builds=int(stats[2]), derivs=int(stats[3]), ticks=int(stats[4]))
+
+    def eigenstates(self, psi, dt, out, iterations, acc_cmf=1e-6, acc_bs=1e-7, max_steps=10 ** 9):
+        """Imaginary-time eigenstates (Eigenstates.cpp:60-103) from psi:
+        energies (iterations + 1, nstates), propagator energies and converged
+        overlaps (iterations, nstates), the final state and the counters."""
+        ns = self.engine.nodes[-1]["dims"][-1]
+        p = _ci(psi).copy()
+        en = np.zeros((iterations + 1) * ns)
+        pe = np.zeros(max(iterations, 1) * ns)
+        cv = np.zeros(max(iterations, 1) * ns)
+        stats = np.zeros(4, np.int64)
+        dte = C.c_double()
+        n = lib().qtg_cmf_eigenstates(self.ptr, p, dt, out, iterations, acc_cmf, acc_bs, max_steps, en, pe, cv,
+                                      stats, C.byref(dte))
+        if n < 0:
+            check(-n)
+        return dict(psi=p.view(np.complex128).copy(), energies=en.reshape(iterations + 1, ns),
+                    propagator=pe[:iterations * ns].reshape(iterations, ns),
+                    converged=cv[:iterations * ns].reshape(iterations, ns), steps=int(stats[0]),
+                    rejected=int(stats[1]), builds=int(stats[2]), derivs=int(stats[3]), dt_end=dte.value)
 
     def __del__(self):
         try:

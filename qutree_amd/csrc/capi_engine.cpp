@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "cmf.hpp"
+#include <algorithm>
 #include "ttno.hpp"
 #include "engine.hpp"
 
@@ -293,6 +294,45 @@ int64_t qtg_cmf_integrate(qtg_cmf c, double* psi_host, double t0, double tend, d
     stats[4] = (int64_t)r.ticks;
   }
   return (int64_t)r.times.size();
+}
+
+int64_t qtg_cmf_eigenstates(qtg_cmf c, double* psi_host, double dt, double out, int64_t iterations, double acc_cmf,
+                            double acc_bs, int64_t max_steps, double* energies, double* propagator,
+                            double* converged, int64_t* stats, double* dt_end) {
+  if (!c || !psi_host || !energies || iterations < 0 || !(out > 1e-5)) {
+    set_error("cmf_eigenstates: null argument, negative iteration count or output interval <= 1e-5");
+    return -QTG_EARG;
+  }
+  if (iterations > 0 && (!propagator || !converged)) {
+    set_error("cmf_eigenstates: null output array");
+    return -QTG_EARG;
+  }
+  auto* cm = static_cast<Cmf*>(c);
+  const size_t bytes = sizeof(double) * 2 * cm->size();
+  if (hipMemcpyAsync(cm->state(), psi_host, bytes, hipMemcpyHostToDevice, cm->stream()) != hipSuccess ||
+      hipStreamSynchronize(cm->stream()) != hipSuccess) {
+    set_error("cmf_eigenstates: upload failed");
+    return -QTG_EDEVICE;
+  }
+  EigenResult r;
+  int rc = cm->eigenstates(dt, out, iterations, acc_cmf, acc_bs, max_steps, r);
+  if (rc) return -rc;
+  if (hipMemcpyAsync(psi_host, cm->state(), bytes, hipMemcpyDeviceToHost, cm->stream()) != hipSuccess ||
+      hipStreamSynchronize(cm->stream()) != hipSuccess) {
+    set_error("cmf_eigenstates: download failed");
+    return -QTG_EDEVICE;
+  }
+  std::copy(r.energies.begin(), r.energies.end(), energies);
+  std::copy(r.propagator.begin(), r.propagator.end(), propagator);
+  std::copy(r.converged.begin(), r.converged.end(), converged);
+  if (stats) {
+    stats[0] = (int64_t)r.steps;
+    stats[1] = (int64_t)r.rejected;
+    stats[2] = (int64_t)r.builds;
+    stats[3] = (int64_t)r.derivs;
+  }
+  if (dt_end) *dt_end = r.dt_end;
+  return cm->nstates();
 }
 
 // ----------------------------------------------------------------------- TTNO

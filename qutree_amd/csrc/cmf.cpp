@@ -495,6 +495,140 @@ int Cmf::integrate(double time, double timeend, double dt, double out, double ac
   res.builds = nbuild_;
   res.derivs = nderiv_;
   res.ticks = nticks_;
+  res.dt_end = dt;
+  return QTG_OK;
+}
+
+// ------------------------------------------------------------- eigenstates
+int Cmf::eig_init() {
+  if (PL_) return QTG_OK;
+  const HTree& t = E.tree_;
+  const HNode& top = t.nodes[NN_ - 1];
+  const int n = top.n;
+  if (n > 64) {
+    set_error("eigenstates: more than 64 states");
+    return QTG_EARG;
+  }
+  int rc;
+  if ((rc = dalloc((void**)&PL_, sizeof(cplx<double>) * total_))) return rc;
+  // [H_top | UT | scratch n x n]
+  if ((rc = dalloc((void**)&eigm_, sizeof(cplx<double>) * 3 * n * n))) return rc;
+  if ((rc = dalloc((void**)&eigv_, sizeof(double) * n))) return rc;
+  // <Psi_top| (H Psi)_top> on the state index
+  SwItem hx{};
+  hx.node = top.addr;
+  hx.k = top.order() - 1;
+  hx.nf = 0;
+  hx.bra = hx.ket = top.off;
+  hx.out = 0;
+  // C(:,j) = sum_k UT(j,k) B(:,k) on the state index
+  SwApply rot{};
+  rot.node = top.addr;
+  rot.nf = 1;
+  rot.fmode[0] = top.order() - 1;
+  rot.fmat[0] = (int64_t)n * n;
+  // Gram-Schmidt of every node
+  std::vector<OrthoItem> on(NN_);
+  for (int a = 0; a < NN_; ++a) {
+    on[a].node = a;
+    on[a].sw = -1;
+  }
+  if ((rc = dalloc((void**)&d_hx_, sizeof(SwItem)))) return rc;
+  if ((rc = dalloc((void**)&d_rot_, sizeof(SwApply)))) return rc;
+  if ((rc = dalloc((void**)&d_on_, sizeof(OrthoItem) * NN_))) return rc;
+  QTG_HIP_TRY(hipMemcpy(d_hx_, &hx, sizeof(hx), hipMemcpyHostToDevice));
+  QTG_HIP_TRY(hipMemcpy(d_rot_, &rot, sizeof(rot), hipMemcpyHostToDevice));
+  QTG_HIP_TRY(hipMemcpy(d_on_, on.data(), sizeof(OrthoItem) * NN_, hipMemcpyHostToDevice));
+  return QTG_OK;
+}
+
+// Apply and the top node's derivative of P with the matrices of the last
+// build, at the top node only: D_top = -i phase (H P)_top.
+int Cmf::apply_top(const cplx<double>* P, double ph_re, double ph_im, double* energy) {
+  std::vector<uint8_t> act(NN_, 0);
+  act[NN_ - 1] = 1;
+  unsigned char* ds = (unsigned char*)dev_stage_;
+  const size_t off = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode);
+  QTG_HIP_TRY(hipMemcpyAsync(ds + off, act.data(), NN_, hipMemcpyHostToDevice, s_));
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  return E.apply_finish((const double*)P, (double*)D_, ds + off, ph_re, ph_im, energy, s_);
+}
+
+// dotProduct tree (MatrixTreeFunctions_Implementation.h:16-39) of bra and
+// ket; the top node's n x n matrix stays at spool_ + sslot_[top].
+int Cmf::overlap_tree(const cplx<double>* bra, const cplx<double>* ket) {
+  for (size_t h = 0; h < ovRange_.size(); ++h) {
+    auto& r = ovRange_[h];
+    int rc = launch_sandwich<double>(d_ov_ + r.first, r.second - r.first, E.d_nodes_, bra, ket, spool_, ovNmax_[h],
+                                     s_);
+    if (rc) return rc;
+  }
+  return QTG_OK;
+}
+
+// Eigenstate (Eigenstates.cpp:18-31): build, Expectation (the top node's
+// <Psi_i|H|Psi_j>), its eigenvectors, Psi_top <- multStateArTB(U, Psi_top).
+int Cmf::eigenstate(std::vector<double>& ev) {
+  const HNode& top = E.tree_.nodes[NN_ - 1];
+  const int n = top.n;
+  int rc;
+  if ((rc = copy((cplx<double>*)E.d_psi64_, P_))) return rc;
+  if ((rc = E.build(s_))) return rc;
+  // phase i: D = -i * i * H Psi = H Psi
+  if ((rc = apply_top(P_, 0.0, 1.0, nullptr))) return rc;
+  if ((rc = launch_sandwich<double>(d_hx_, 1, E.d_nodes_, P_, D_, eigm_, ovNmax_.back(), s_))) return rc;
+  if ((rc = launch_eig_sorted(eigm_, n, eigm_ + (int64_t)n * n, eigv_, s_))) return rc;
+  if ((rc = launch_apply_sw(d_rot_, 1, E.d_nodes_, P_, eigm_, E.nmax_, s_))) return rc;
+  ev.assign(n, 0.0);
+  QTG_HIP_TRY(hipMemcpyAsync(ev.data(), eigv_, sizeof(double) * n, hipMemcpyDeviceToHost, s_));
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  return QTG_OK;
+}
+
+int Cmf::eigenstates(double dt, double out, int64_t iterations, double acc_cmf, double acc_bs, int64_t max_steps,
+                     EigenResult& res) {
+  int rc;
+  if ((rc = eig_init())) return rc;
+  const int n = E.tree_.nodes[NN_ - 1].n;
+  const cplx<double>* stop = spool_ + sslot_[NN_ - 1];
+  ph_re_ = 0.0;
+  ph_im_ = -1.0;
+  std::vector<double> ev;
+  if ((rc = eigenstate(ev))) return rc;
+  res.energies.insert(res.energies.end(), ev.begin(), ev.end());
+  std::vector<cd> ov((size_t)n * n);
+  for (int64_t it = 0; it < iterations; ++it) {
+    if ((rc = copy(PL_, P_))) return rc;
+    CmfResult r;
+    if ((rc = integrate(0.0, out - 1e-5, dt, out, acc_cmf, acc_bs, max_steps, r))) return rc;
+    dt = r.dt_end;
+    res.steps += r.steps;
+    res.rejected += r.rejected;
+    res.builds += r.builds;
+    res.derivs += r.derivs;
+    // propagatorEnergies (Eigenstates.cpp:8-16)
+    if ((rc = overlap_tree(P_, P_))) return rc;
+    if ((rc = launch_eig_sorted(stop, n, nullptr, eigv_, s_))) return rc;
+    ev.assign(n, 0.0);
+    QTG_HIP_TRY(hipMemcpyAsync(ev.data(), eigv_, sizeof(double) * n, hipMemcpyDeviceToHost, s_));
+    QTG_HIP_TRY(hipStreamSynchronize(s_));
+    for (double& e : ev) e = -std::log(e) / (2.0 * out);
+    res.propagator.insert(res.propagator.end(), ev.begin(), ev.end());
+    // orthonormal (TensorTree_Implementation.h:190-195)
+    if ((rc = launch_ortho(d_on_, NN_, E.d_nodes_, P_, sw_, E.nmax_, E.nspf_, s_))) return rc;
+    if ((rc = eigenstate(ev))) return rc;
+    res.energies.insert(res.energies.end(), ev.begin(), ev.end());
+    // overlap with the previous iterate: s(i) = sum_j |<Psi_i|Psi'_j>|^2
+    if ((rc = overlap_tree(P_, PL_))) return rc;
+    QTG_HIP_TRY(hipMemcpyAsync(ov.data(), stop, sizeof(cd) * n * n, hipMemcpyDeviceToHost, s_));
+    QTG_HIP_TRY(hipStreamSynchronize(s_));
+    for (int i = 0; i < n; ++i) {
+      double sum = 0.0;
+      for (int j = 0; j < n; ++j) sum += std::norm(ov[i + (size_t)j * n]);
+      res.converged.push_back(sum);
+    }
+  }
+  res.dt_end = dt;
   return QTG_OK;
 }
 

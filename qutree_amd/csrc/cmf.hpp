@@ -13,6 +13,13 @@ struct CmfResult {
   std::vector<double> times, energies;
   std::vector<cd> autocorr;
   size_t steps = 0, rejected = 0, builds = 0, derivs = 0, ticks = 0;
+  double dt_end = 0.0;  // step size on return (job.dt, CMFIntegrator.cpp:172)
+};
+
+struct EigenResult {
+  std::vector<double> energies, propagator, converged;  // nstates per entry
+  size_t steps = 0, rejected = 0, builds = 0, derivs = 0;
+  double dt_end = 0.0;
 };
 
 struct NodeBS {
@@ -34,9 +41,17 @@ class Cmf {
   hipStream_t stream() const { return s_; }
   int integrate(double time, double timeend, double dt, double out, double acc_cmf, double acc_bs,
                 int64_t max_steps, CmfResult& res);
+  // Eigenstates(IntegratorVariables&) (Eigenstates.cpp:60-103) from the state in P_
+  int eigenstates(double dt, double out, int64_t iterations, double acc_cmf, double acc_bs, int64_t max_steps,
+                  EigenResult& res);
+  int nstates() const { return E.tree_.nodes.back().n; }
   double ph_re_ = 1.0, ph_im_ = 0.0;  // propagation phase (1 = real time)
 
  private:
+  int eig_init();
+  int apply_top(const cplx<double>* P, double ph_re, double ph_im, double* energy);
+  int overlap_tree(const cplx<double>* bra, const cplx<double>* ket);
+  int eigenstate(std::vector<double>& ev);
   int dalloc(void** p, size_t bytes);
   CmfBufs bufs(cplx<double>* P) const;
   int copy(cplx<double>* dst, const cplx<double>* src);
@@ -73,6 +88,14 @@ class Cmf {
   std::vector<std::pair<int64_t, int64_t>> ovRange_;
   std::vector<int> ovNmax_;
   double* energy_ = nullptr;
+  // eigenstates: previous iterate, top matrix and its eigenvectors
+  // (UT at eigm_ + n^2), eigenvalues, and the launch items
+  cplx<double>* PL_ = nullptr;
+  cplx<double>* eigm_ = nullptr;
+  double* eigv_ = nullptr;
+  SwItem* d_hx_ = nullptr;
+  SwApply* d_rot_ = nullptr;
+  OrthoItem* d_on_ = nullptr;
   std::vector<NodeBS> bs_;
   std::vector<double> dtbs_;
   bool failed_ = false;

@@ -311,7 +311,9 @@ __global__ void __launch_bounds__(TB)
 // ------------------------------------------------------------- orthogonal()
 // S = Phi^H Phi, SW(j,k) = sum_l U(k,l) sqrt(max(l_l,0)) conj(U(j,l)) (the
 // transposed square root), then iterated Gram-Schmidt of Phi in place
-// (Tensor_Implementation.h:806-853).  One wavefront per node.
+// (Tensor_Implementation.h:806-853).  One wavefront per node.  Items with
+// sw < 0 take the Gram-Schmidt only (orthonormal(),
+// TensorTree_Implementation.h:190-195).
 __global__ void __launch_bounds__(EW)
     ortho_kernel(const OrthoItem* __restrict__ items, const DNode* __restrict__ nodes, cd2* P, cd2* sw,
                  int nmax, int nspf) {
@@ -326,26 +328,28 @@ __global__ void __launch_bounds__(EW)
   const int N = nd.N, n = nodes[it.node].dims[nd.order - 1], L = N / n, ld = jacobi_ld(n);
   for (int e = threadIdx.x; e < N; e += EW) Phi[e] = P[nd.off + e];
   __syncthreads();
-  for (int o = threadIdx.x; o < n * n; o += EW) {
-    const int j = o % n, k = o / n;
-    cd2 acc = czero<double>();
-    for (int i = 0; i < L; ++i) cfmac(acc, Phi[i + j * L], Phi[i + k * L]);
-    A[j + k * ld] = acc;
-  }
-  __syncthreads();
-  jacobi_eig(A, U, n, ld, js);
-  for (int i = threadIdx.x; i < n; i += EW) lam[i] = sqrt(fmax(A[i + i * ld].re, 0.0));
-  __syncthreads();
-  for (int o = threadIdx.x; o < n * n; o += EW) {
-    const int j = o % n, k = o / n;
-    cd2 acc = czero<double>();
-    for (int l = 0; l < n; ++l) {
-      const cd2 ukl = U[k + l * ld], ujl = U[j + l * ld];
-      // U(k,l) * s_l * conj(U(j,l))
-      acc.re += lam[l] * (ukl.re * ujl.re + ukl.im * ujl.im);
-      acc.im += lam[l] * (ukl.im * ujl.re - ukl.re * ujl.im);
+  if (it.sw >= 0) {  // uniform per workgroup
+    for (int o = threadIdx.x; o < n * n; o += EW) {
+      const int j = o % n, k = o / n;
+      cd2 acc = czero<double>();
+      for (int i = 0; i < L; ++i) cfmac(acc, Phi[i + j * L], Phi[i + k * L]);
+      A[j + k * ld] = acc;
     }
-    sw[it.sw + j + k * n] = acc;
+    __syncthreads();
+    jacobi_eig(A, U, n, ld, js);
+    for (int i = threadIdx.x; i < n; i += EW) lam[i] = sqrt(fmax(A[i + i * ld].re, 0.0));
+    __syncthreads();
+    for (int o = threadIdx.x; o < n * n; o += EW) {
+      const int j = o % n, k = o / n;
+      cd2 acc = czero<double>();
+      for (int l = 0; l < n; ++l) {
+        const cd2 ukl = U[k + l * ld], ujl = U[j + l * ld];
+        // U(k,l) * s_l * conj(U(j,l))
+        acc.re += lam[l] * (ukl.re * ujl.re + ukl.im * ujl.im);
+        acc.im += lam[l] * (ukl.im * ujl.re - ukl.re * ujl.im);
+      }
+      sw[it.sw + j + k * n] = acc;
+    }
   }
   // Gram-Schmidt: lane owns rows i = lane + 64 t, so every reduction is a
   // wavefront shuffle sum and no other lane touches its elements.
@@ -384,6 +388,56 @@ __global__ void __launch_bounds__(EW)
   }
   __syncthreads();
   for (int e = threadIdx.x; e < N; e += EW) P[nd.off + e] = Phi[e];
+}
+
+// ------------------------------------------------------ top eigenproblems
+// One wavefront: the Hermitian n x n matrix A (column-major) is diagonalised
+// (jacobi.hpp); eigenvalues ascending into ev (stable order), and the
+// eigenvector columns V(:,j) with the phase that makes V(0,j) real and
+// non-negative are written transposed, UT(j,k) = V(k,j), ready for a mode
+// product on the state index (multStateArTB, Tensor_Implementation.h:763-792:
+// C(:,j) = sum_k V(k,j) B(:,k)).  UT may be null.
+__global__ void __launch_bounds__(EW)
+    eig_sorted_kernel(const cd2* __restrict__ Ain, int n, cd2* UT, double* ev) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cd2* A = reinterpret_cast<cd2*>(smem_raw);
+  cd2* U = A + jacobi_elems(n);
+  __shared__ JacobiScratch js;
+  __shared__ int idx[64];
+  __shared__ cd2 ph[64];
+  const int ld = jacobi_ld(n);
+  for (int o = threadIdx.x; o < n * n; o += EW) A[o % n + (o / n) * ld] = Ain[o];
+  __syncthreads();
+  jacobi_eig(A, U, n, ld, js);
+  if (threadIdx.x == 0) {
+    // stable insertion sort of the eigenvalues
+    for (int j = 0; j < n; ++j) {
+      int k = j;
+      const double v = A[j + j * ld].re;
+      while (k > 0 && A[idx[k - 1] + idx[k - 1] * ld].re > v) {
+        idx[k] = idx[k - 1];
+        --k;
+      }
+      idx[k] = j;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += EW) {
+    const cd2 u0 = U[0 + idx[j] * ld];
+    const double a = sqrt(u0.re * u0.re + u0.im * u0.im);
+    cd2 p = mk<double>(1.0, 0.0);
+    if (a > 1e-300) p = mk<double>(u0.re / a, -u0.im / a);
+    // real(V(0,j)) < 0 flips the column
+    if (u0.re * p.re - u0.im * p.im < 0.0) p = mk<double>(-p.re, -p.im);
+    ph[j] = p;
+    ev[j] = A[idx[j] + idx[j] * ld].re;
+  }
+  __syncthreads();
+  if (UT)
+    for (int o = threadIdx.x; o < n * n; o += EW) {
+      const int k = o % n, j = o / n;  // V(k,j)
+      UT[j + k * n] = cmul(U[k + idx[j] * ld], ph[j]);
+    }
 }
 
 // parent <- SW_c x_{childIdx(c)} parent for its children, in node-address order
@@ -425,6 +479,17 @@ int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx
   if (count == 0) return QTG_OK;
   const size_t lds = ((size_t)nmax + 2 * (size_t)jacobi_elems(nspf)) * sizeof(cd2);
   hipLaunchKernelGGL(ortho_kernel, dim3((unsigned)count), dim3(EW), lds, s, items, nodes, P, sw, nmax, nspf);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+int launch_eig_sorted(const cplx<double>* A, int n, cplx<double>* UT, double* ev, hipStream_t s) {
+  if (n < 1 || n > 64) {
+    set_error("eig_sorted: order outside 1..64");
+    return QTG_EARG;
+  }
+  const size_t lds = 2 * (size_t)jacobi_elems(n) * sizeof(cd2);
+  hipLaunchKernelGGL(eig_sorted_kernel, dim3(1), dim3(EW), lds, s, A, n, UT, ev);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }

@@ -77,14 +77,15 @@ class Hamiltonian:
 
 
 def balanced_tree_text(n_leaves: int, dim_leaf: int, spf_by_layer: Sequence[int],
-                       omega: float = 1.0, r0: float = 0.0, wfr0: float = 0.0,
-                       wfomega: float = 1.0) -> str:
+                       omega=1.0, r0: float = 0.0, wfr0: float = 0.0,
+                       wfomega=1.0, nstates: int = 1) -> str:
     """Balanced binary tree over ``n_leaves`` (a power of two) HO-DVR leaves.
 
     spf_by_layer[0] is the SPF count of the bottom nodes, spf_by_layer[k] that
-    of nodes k layers above them; the top node carries one state.  Leaf modes
-    run 0..f-1 from left to right, as ``Tree::resetLeafModes`` numbers them
-    (src/TreeShape/Tree.cpp:35-46)."""
+    of nodes k layers above them; the top node carries ``nstates`` states.
+    ``omega`` and ``wfomega`` are one value for every mode or one per mode.
+    Leaf modes run 0..f-1 from left to right, as ``Tree::resetLeafModes``
+    numbers them (src/TreeShape/Tree.cpp:35-46)."""
     if n_leaves < 2 or n_leaves & (n_leaves - 1):
         raise ValueError("n_leaves must be a power of two >= 2")
     nlayers = int(np.log2(n_leaves))  # upper layers above the bottom nodes
@@ -97,7 +98,7 @@ def balanced_tree_text(n_leaves: int, dim_leaf: int, spf_by_layer: Sequence[int]
         # height 0 = bottom node
         tab = "\t" * indent
         if height == nlayers:
-            lines.append(f"{tab}1\t-2")
+            lines.append(f"{tab}{nstates}\t-2")
         else:
             lines.append(f"{tab}{spf_by_layer[height]}\t-{1 if height == 0 else 2}")
         if height == 0:
@@ -108,8 +109,10 @@ def balanced_tree_text(n_leaves: int, dim_leaf: int, spf_by_layer: Sequence[int]
         node(height - 1, indent + 1)
 
     node(nlayers, 0)
-    for _ in range(n_leaves):
-        lines.append(f"{omega} {r0} {wfr0} {wfomega}")
+    om = np.broadcast_to(np.asarray(omega, dtype=float), (n_leaves,))
+    wo = np.broadcast_to(np.asarray(wfomega, dtype=float), (n_leaves,))
+    for k in range(n_leaves):
+        lines.append(f"{float(om[k])!r} {r0} {wfr0} {float(wo[k])!r}")
     return "\n".join(lines) + "\n"
 
 
@@ -192,6 +195,27 @@ def cmf128_model(seed: int = 128):
     coupled-quartic SOP (128 + 8128 terms)."""
     text = balanced_tree_text(128, 8, [8, 14, 21, 27, 34, 41, 48])
     return text, coupled_quartic(128, seed)
+
+
+def eig8_model(nstates: int = 3, seed: int = 7):
+    """Imaginary-time eigenstates workload: 8 coupled HO modes with
+    frequencies w_i = 1 + 0.25 i (an HO-DVR of 8 points at each mode's own
+    frequency), pair couplings c_ij ~ N(0, 0.02^2), 4 SPFs per node and
+    ``nstates`` states at the top.  Returns the tree text, the Hamiltonian and
+    the force-constant matrix K (K_ii = w_i^2, K_ij = c_ij)."""
+    f = 8
+    w = 1.0 + 0.25 * np.arange(f)
+    rng = np.random.default_rng(seed)
+    H = Hamiltonian()
+    for i in range(f):
+        H.add_term(1.0, [(i, FACTOR_POLY, (1.0, 0.0, 0.5 * w[i] ** 2, 0.0))])
+    K = np.diag(w ** 2)
+    for i, j in _pairs(f):
+        c = rng.normal(0.0, 0.02)
+        K[i, j] = K[j, i] = c
+        H.add_term(c, [(i, FACTOR_X), (j, FACTOR_X)])
+    text = balanced_tree_text(f, 8, [4, 4, 4], omega=w, wfomega=w, nstates=nstates)
+    return text, H, K
 
 
 def microbench_shapes(count: int = 4096, seed: int = 4096):

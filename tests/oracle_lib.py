@@ -67,6 +67,10 @@ def lib():
     L.orc_dot_product_top.argtypes = [C.c_void_p, dp, dp, dp]
     L.orc_cmf.argtypes = ([C.c_void_p, dp] + [C.c_double] * 8 + [C.c_long, C.c_long, dp, dp, dp, lp])
     L.orc_cmf.restype = C.c_long
+    L.orc_eigenstates.argtypes = ([C.c_void_p, dp, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double,
+                                   C.c_long, dp, dp, dp, lp, C.POINTER(C.c_double)])
+    L.orc_eigenstates.restype = C.c_long
+    L.orc_top_matrices.argtypes = [C.c_void_p, dp, dp, dp]
     _lib = L
     return L
 
@@ -260,3 +264,32 @@ class OracleModel:
         return dict(psi=as_complex(p).copy(), times=times[:n], energies=en[:n],
                     autocorr=as_complex(ac)[:n].copy(), steps=int(stats[0]), rejected=int(stats[1]),
                     builds=int(stats[2]), derivs=int(stats[3]))
+
+    def top_matrices(self, psi):
+        """<Psi_i|H|Psi_j> and <Psi_i|Psi_j> of the top node's states."""
+        ns = self.nodes[-1]["dims"][-1]
+        hm = np.zeros(2 * ns * ns)
+        sm = np.zeros(2 * ns * ns)
+        if lib().orc_top_matrices(self.h, cview(psi), hm, sm):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return (as_complex(hm).reshape(ns, ns, order="F").copy(), as_complex(sm).reshape(ns, ns, order="F").copy())
+
+    def eigenstates(self, psi, dt, out, iterations, acc_cmf=1e-6, acc_bs=1e-7, max_steps=10 ** 9):
+        """Imaginary-time eigenstates (Eigenstates.cpp:60-103): energies
+        (iterations + 1, nstates), propagator energies and converged overlaps
+        (iterations, nstates), the final state and the counters."""
+        ns = self.nodes[-1]["dims"][-1]
+        p = cview(psi).copy()
+        en = np.zeros((iterations + 1) * ns)
+        pe = np.zeros(max(iterations, 1) * ns)
+        cv = np.zeros(max(iterations, 1) * ns)
+        stats = np.zeros(4, np.int64)
+        dte = C.c_double()
+        n = lib().orc_eigenstates(self.h, p, dt, out, iterations, acc_cmf, acc_bs, max_steps, en, pe, cv, stats,
+                                  C.byref(dte))
+        if n < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return dict(psi=as_complex(p).copy(), energies=en.reshape(iterations + 1, ns),
+                    propagator=pe[:iterations * ns].reshape(iterations, ns),
+                    converged=cv[:iterations * ns].reshape(iterations, ns), steps=int(stats[0]),
+                    rejected=int(stats[1]), builds=int(stats[2]), derivs=int(stats[3]), dt_end=dte.value)
