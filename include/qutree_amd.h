@@ -268,6 +268,35 @@ int64_t qtg_cmf_eigenstates(qtg_cmf cmf, double* psi_host, double dt, double out
                             double acc_bs, int64_t max_steps, double* energies, double* propagator,
                             double* converged, int64_t* stats, double* dt_end);
 
+/* ---------------------------------------------------------------- formats */
+/* QuTree's binary files, byte for byte (host only; complex<double>,
+ * little-endian).  TensorTree<cd>::write/read (TensorTree_Implementation.h:
+ * 79-132, Tensor_Implementation.h:204-258, TensorShape.cpp:68-111): "TTre",
+ * int32 node count, then per node in address order "TENS", "TDIM", int32
+ * order, int32 dims[order], int32 16, the coefficients (first index fastest).
+ * qtg_tensor_tree_write: psi in engine layout (node-address order, as
+ * qtg_engine_psi_size) of the tree in tree_text into buf; returns the byte
+ * count (buf may be null to ask for it; a smaller cap is QTG_EARG).
+ * qtg_tensor_tree_read: one TensorTree from buf, every node's order and
+ * dimensions checked against the tree (QTG_ESHAPE names the first mismatch);
+ * returns the bytes consumed, so that files the reference appended to
+ * (TensorTree::write(file, append)) are read tree by tree.  psi is undefined
+ * after an error.  Both return a negative qtg_status on error. */
+int64_t qtg_tensor_tree_write(const char* tree_text, const double* psi, unsigned char* buf, int64_t cap);
+int64_t qtg_tensor_tree_read(const char* tree_text, const unsigned char* buf, int64_t len, double* psi);
+/* MatrixTree<cd> ("MTre", MatrixTree.cpp:52-90) and SparseMatrixTree<cd>
+ * ("SMTr", SparseMatrixTree.cpp:27-70; sparse != 0): int32 count, then per
+ * matrix Matrix::write (Matrix_Implementation.h:365-378): "MATR", uint64
+ * dim1, uint64 dim2, int32 16, the coefficients (column-major).  dims holds
+ * (dim1, dim2) per matrix, mats the matrices back to back.  write: returns
+ * the byte count (buf may be null).  read: *count = number of matrices; at
+ * most max_count matrices and mats_cap complex coefficients are stored;
+ * returns the bytes consumed.  Negative qtg_status on error. */
+int64_t qtg_matrix_tree_write(int sparse, int64_t count, const int64_t* dims, const double* mats, unsigned char* buf,
+                              int64_t cap);
+int64_t qtg_matrix_tree_read(int sparse, const unsigned char* buf, int64_t len, int64_t max_count, int64_t* dims,
+                             double* mats, int64_t mats_cap, int64_t* count);
+
 /* ------------------------------------------------------------------- TTNO */
 /* Tree tensor network operator on an engine's tree (TensorTreeOperator,
  * include/TreeOperators/TensorOperators/TensorTreeOperator.h:23-58; complex128

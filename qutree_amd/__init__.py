@@ -90,6 +90,16 @@ def lib():
     L.qtg_cmf_eigenstates.argtypes = ([_vp, _dp, C.c_double, C.c_double, _i64, C.c_double, C.c_double, _i64, _dp,
                                        _dp, _dp, _lp, C.POINTER(C.c_double)])
     L.qtg_cmf_eigenstates.restype = _i64
+    L.qtg_host_psi_size.argtypes = [C.c_char_p]
+    L.qtg_host_psi_size.restype = _i64
+    L.qtg_tensor_tree_write.argtypes = [C.c_char_p, _dp, C.c_void_p, _i64]
+    L.qtg_tensor_tree_write.restype = _i64
+    L.qtg_tensor_tree_read.argtypes = [C.c_char_p, C.c_void_p, _i64, _dp]
+    L.qtg_tensor_tree_read.restype = _i64
+    L.qtg_matrix_tree_write.argtypes = [C.c_int, _i64, _lp, _dp, C.c_void_p, _i64]
+    L.qtg_matrix_tree_write.restype = _i64
+    L.qtg_matrix_tree_read.argtypes = [C.c_int, C.c_void_p, _i64, _i64, _lp, _dp, _i64, C.POINTER(_i64)]
+    L.qtg_matrix_tree_read.restype = _i64
     L.qtg_engine_read_timing.argtypes = [_vp, _dp, _lp, _dp, _dp]
     _lib = L
     return L
@@ -416,3 +426,68 @@ class TTNO:
                 lib().qtg_ttno_destroy(self.ptr)
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ formats
+def tensor_tree_bytes(tree_text: str, psi: np.ndarray) -> bytes:
+    """TensorTree<cd>::write (TensorTree_Implementation.h:79-92) of psi
+    (engine layout) on the tree in tree_text."""
+    p = _ci(psi)
+    n = lib().qtg_tensor_tree_write(tree_text.encode(), p, None, 0)
+    if n < 0:
+        check(-n)
+    buf = C.create_string_buffer(int(n))
+    m = lib().qtg_tensor_tree_write(tree_text.encode(), p, buf, n)
+    if m < 0:
+        check(-m)
+    return buf.raw[:m]
+
+
+def tensor_tree_from_bytes(tree_text: str, data: bytes, offset: int = 0):
+    """TensorTree<cd>::read of the tree starting at data[offset:]; returns
+    (psi, bytes consumed)."""
+    size = lib().qtg_host_psi_size(tree_text.encode())
+    if size < 0:
+        check(-size)
+    psi = np.zeros(2 * size)
+    view = (C.c_char * (len(data) - offset)).from_buffer_copy(data, offset)
+    n = lib().qtg_tensor_tree_read(tree_text.encode(), view, len(data) - offset, psi)
+    if n < 0:
+        check(-n)
+    return psi.view(np.complex128).copy(), int(n)
+
+
+def matrix_tree_bytes(mats, sparse: bool = False) -> bytes:
+    """MatrixTree ("MTre") or SparseMatrixTree ("SMTr") write of a list of
+    complex matrices, in node order."""
+    mats = [np.asarray(m, np.complex128) for m in mats]
+    dims = np.array([d for m in mats for d in m.shape], np.int64)
+    flat = (np.concatenate([m.ravel(order="F") for m in mats]) if mats else np.zeros(0, np.complex128))
+    fv = np.ascontiguousarray(flat).view(np.float64)
+    n = lib().qtg_matrix_tree_write(int(sparse), len(mats), dims, fv, None, 0)
+    if n < 0:
+        check(-n)
+    buf = C.create_string_buffer(int(n))
+    m = lib().qtg_matrix_tree_write(int(sparse), len(mats), dims, fv, buf, n)
+    if m < 0:
+        check(-m)
+    return buf.raw[:m]
+
+
+def matrix_tree_from_bytes(data: bytes, sparse: bool = False, max_count: int = 4096, max_coeffs: int = 1 << 24):
+    """Reads one MatrixTree / SparseMatrixTree; returns (matrices, bytes
+    consumed)."""
+    dims = np.zeros(2 * max_count, np.int64)
+    coef = np.zeros(2 * max_coeffs)
+    cnt = _i64()
+    view = (C.c_char * len(data)).from_buffer_copy(data)
+    n = lib().qtg_matrix_tree_read(int(sparse), view, len(data), max_count, dims, coef, max_coeffs, C.byref(cnt))
+    if n < 0:
+        check(-n)
+    z = coef.view(np.complex128)
+    out, off = [], 0
+    for k in range(cnt.value):
+        d1, d2 = int(dims[2 * k]), int(dims[2 * k + 1])
+        out.append(z[off:off + d1 * d2].reshape(d1, d2, order="F").copy())
+        off += d1 * d2
+    return out, int(n)
