@@ -474,9 +474,11 @@ def matrix_tree_bytes(mats, sparse: bool = False) -> bytes:
     return buf.raw[:m]
 
 
-def matrix_tree_from_bytes(data: bytes, sparse: bool = False, max_count: int = 4096, max_coeffs: int = 1 << 24):
+def matrix_tree_from_bytes(data: bytes, sparse: bool = False):
     """Reads one MatrixTree / SparseMatrixTree; returns (matrices, bytes
     consumed)."""
+    # a matrix record takes at least 24 bytes, a coefficient 16
+    max_count, max_coeffs = len(data) // 24 + 1, len(data) // 16 + 1
     dims = np.zeros(2 * max_count, np.int64)
     coef = np.zeros(2 * max_coeffs)
     cnt = _i64()

@@ -92,5 +92,23 @@ def test_matrix_tree_bytes_and_round_trip(sparse):
         assert x.shape == y.shape and np.array_equal(x, y)
     with pytest.raises(Q.QtgError, match="SMTr|MTre"):
         Q.matrix_tree_from_bytes(data, sparse=not sparse)
-    with pytest.raises(Q.QtgError, match="room for"):
-        Q.matrix_tree_from_bytes(data, sparse=sparse, max_count=2)
+    with pytest.raises(Q.QtgError, match="truncated"):
+        Q.matrix_tree_from_bytes(data[:-8], sparse=sparse)
+
+
+def test_matrix_tree_read_limits():
+    """The reader stores no more matrices or coefficients than it was given
+    room for."""
+    import ctypes as C
+
+    mats = [np.eye(3, dtype=np.complex128), np.ones((2, 2), np.complex128)]
+    data = Q.matrix_tree_bytes(mats)
+    view = (C.c_char * len(data)).from_buffer_copy(data)
+    dims = np.zeros(4, np.int64)
+    coef = np.zeros(2 * 13)
+    cnt = C.c_int64()
+    assert Q.lib().qtg_matrix_tree_read(0, view, len(data), 1, dims, coef, 13, C.byref(cnt)) == -3
+    assert cnt.value == 2
+    assert Q.lib().qtg_matrix_tree_read(0, view, len(data), 2, dims, coef, 12, C.byref(cnt)) == -3
+    assert Q.lib().qtg_matrix_tree_read(0, view, len(data), 2, dims, coef, 13, C.byref(cnt)) == len(data)
+    assert list(dims) == [3, 3, 2, 2]
