@@ -166,7 +166,9 @@ int qtg_engine_derivative(qtg_engine engine, double phase_re, double phase_im, q
 /* which: 0 = derivative, 1 = Apply() output (before projector), 2 = Psi */
 int qtg_engine_get_dpsi(qtg_engine engine, double* host, int which, qtg_stream stream);
 /* Expectation / S_top (HamiltonianRepresentation.cpp:45-51, CMFIntegrator.cpp:235-241)
- * of the last derivative evaluation (orthonormal lower nodes). */
+ * of the last derivative evaluation (orthonormal lower nodes).  With several
+ * states at the top node this is the state average trace(H) / trace(S); the
+ * levels of each state come from qtg_cmf_eigenstates. */
 int qtg_engine_energy(qtg_engine engine, double* energy, qtg_stream stream);
 /* which: 0 rho[node], 1 rho^-1[node], 2 h-matrix[term][node], 3 hole[term][node] */
 int qtg_engine_get_matrix(qtg_engine engine, int which, int64_t term, int64_t node, double* out,
@@ -238,7 +240,8 @@ int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
  * steps.  psi_host (complex128 interleaved, engine layout) is the initial
  * state on entry and the final state on return.  At t0 and after every
  * accepted step that reaches the next output time: times, energies
- * (Expectation / S_top) and autocorr (<Psi(t0)|Psi(t)>, interleaved).
+ * (h(0,0) / S(0,0) of Expectation and S_top: the first state's) and
+ * autocorr (<Psi_0(t0)|Psi_0(t)>, interleaved).
  * stats (5 entries): steps, rejected steps, builds, node derivative
  * evaluations, lockstep ticks.  Returns the number of outputs, or a negative
  * qtg_status. */

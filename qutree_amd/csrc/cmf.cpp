@@ -432,7 +432,18 @@ int Cmf::output(double time, CmfResult& res) {
   double e[4];
   QTG_HIP_TRY(hipMemcpyAsync(e, energy_, sizeof(e), hipMemcpyDeviceToHost, s_));
   QTG_HIP_TRY(hipStreamSynchronize(s_));
-  const cd num(e[0], e[1]), den(stop.re, stop.im);
+  cd num(e[0], e[1]);
+  const cd den(stop.re, stop.im);
+  if (E.tree_.nodes[NN_ - 1].n > 1) {
+    // several states: h(0,0) / S(0,0) of the first one, h(0,0) =
+    // <Psi_0|H Psi_0> = i <Psi_0|D_0> with the top node's D = -i H Psi
+    if ((rc = eig_init())) return rc;
+    if ((rc = launch_sandwich<double>(d_hx_, 1, E.d_nodes_, P_, D_, eigm_, ovNmax_.back(), s_))) return rc;
+    cplx<double> x;
+    QTG_HIP_TRY(hipMemcpyAsync(&x, eigm_, sizeof(x), hipMemcpyDeviceToHost, s_));
+    QTG_HIP_TRY(hipStreamSynchronize(s_));
+    num = cd(-x.im, x.re);
+  }
   res.times.push_back(time);
   res.energies.push_back((num / den).real());
   res.autocorr.push_back(cd(ac.re, ac.im));

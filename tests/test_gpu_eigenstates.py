@@ -92,3 +92,23 @@ def test_eigenstates_match_oracle(ns, iters, out):
         assert dc < 1e-9
         assert abs(got["dt_end"] - ref["dt_end"]) < 1e-3 * ref["dt_end"]
     assert ds < 1e-7
+
+
+def test_cmf_multi_state_matches_oracle():
+    """Real-time CMF with three states at the top: energies h(0,0)/S(0,0) and
+    autocorrelations of the first state, and the step statistics."""
+    text, H, _ = W.eig8_model(3)
+    print("case cmf eig8 nstates=3 real time", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(4, delta_lowest=False)
+    ref = om.cmf(psi, 0.0, 0.3, 0.1, 0.1)
+    eng = Q.Engine(text, H)
+    got = Q.CMF(eng).integrate(psi, 0.0, 0.3, 0.1, 0.1)
+    de = np.max(np.abs(got["energies"] - ref["energies"]) / np.abs(ref["energies"]))
+    da = np.max(np.abs(got["autocorr"] - ref["autocorr"]))
+    print(f"  outputs {len(got['times'])} steps {got['steps']}/{ref['steps']} derivs {got['derivs']}/"
+          f"{ref['derivs']}  max rel dE {de:.2e}  max |da| {da:.2e}", flush=True)
+    assert len(got["times"]) == len(ref["times"]) == 4
+    assert got["steps"] == ref["steps"] and got["derivs"] == ref["derivs"]
+    assert de < 1e-12
+    assert da < 1e-12
