@@ -186,7 +186,8 @@ int qtg_engine_counts(qtg_engine engine, double* sweep_flops, int64_t* items);
  * terms inactive at v (which the projector of LayerDerivative removes).  No
  * reference counterpart; rho^-1 (1 - P) of it is the reference's
  * imaginary-time LayerDerivative.  grad_host (engine layout, complex128
- * interleaved) and energy may be null. */
+ * interleaved) and energy may be null.  One state at the top node
+ * (QTG_ESHAPE otherwise). */
 int qtg_engine_gradient(qtg_engine engine, double* grad_host, double* energy, qtg_stream stream);
 /* FLOPs one sweep of this engine's plan executes (fewer than the reference
  * count where equal slots are shared and Apply summands merged). */

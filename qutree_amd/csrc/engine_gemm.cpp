@@ -903,6 +903,10 @@ int Engine::gradient(hipStream_t s, double* grad_host, double* energy) {
     set_error("engine: no Hamiltonian");
     return QTG_ESTATE;
   }
+  if (tree_.nodes.back().n != 1) {
+    set_error("engine: the energy gradient takes one state at the top node");
+    return QTG_ESHAPE;
+  }
   int rc = derivative(s, 1.0, 0.0);
   if (rc) return rc;
   double e[4];

@@ -112,3 +112,11 @@ def test_cmf_multi_state_matches_oracle():
     assert got["steps"] == ref["steps"] and got["derivs"] == ref["derivs"]
     assert de < 1e-12
     assert da < 1e-12
+
+
+def test_gradient_refuses_several_states():
+    text, H, _ = W.eig8_model(3)
+    eng = Q.Engine(text, H)
+    eng.set_psi(eng.fill_random(1, delta_lowest=False))
+    with pytest.raises(Q.QtgError, match="one state"):
+        eng.gradient()
