@@ -1990,6 +1990,34 @@ long orc_cmf(void* h, double* psi, double t0, double tend, double dt, double out
   }
 }
 
+// TensorTree<cd>::write (TensorTree_Implementation.h:79-92 with
+// Tensor::write, Tensor_Implementation.h:204-221, and TensorShape::write,
+// TensorShape.cpp:68-81) into buf; returns the byte count (buf may be null).
+long orc_tensor_tree_write(void* h, const double* psi, unsigned char* buf, long cap) {
+  Model* m = (Model*)h;
+  TensorTree P = m->unpack(psi);
+  std::string out;
+  auto put = [&](const void* p, size_t n) { out.append((const char*)p, n); };
+  put("TTre", 4);
+  const int32_t nn = (int32_t)P.size();
+  put(&nn, 4);
+  for (const Tensor& T : P) {
+    put("TENS", 4);
+    put("TDIM", 4);
+    const int32_t order = (int32_t)T.s.order();
+    put(&order, 4);
+    for (size_t k = 0; k < T.s.order(); ++k) {
+      const int32_t d = (int32_t)T.s.d[k];
+      put(&d, 4);
+    }
+    const int32_t size = sizeof(cd);
+    put(&size, 4);
+    for (const cd& x : T.c) put(&x, sizeof(cd));
+  }
+  if (buf && (long)out.size() <= cap) std::memcpy(buf, out.data(), out.size());
+  return (long)out.size();
+}
+
 // Top-node matrices after a build from psi: H(i,j) = <Psi_i|H|Psi_j>
 // (Expectation) and S = dotProduct(Psi, Psi)[top], column-major interleaved.
 int orc_top_matrices(void* h, const double* psi, double* hmat, double* smat) {

@@ -71,6 +71,8 @@ def lib():
                                    C.c_long, dp, dp, dp, lp, C.POINTER(C.c_double)])
     L.orc_eigenstates.restype = C.c_long
     L.orc_top_matrices.argtypes = [C.c_void_p, dp, dp, dp]
+    L.orc_tensor_tree_write.argtypes = [C.c_void_p, dp, C.c_void_p, C.c_long]
+    L.orc_tensor_tree_write.restype = C.c_long
     _lib = L
     return L
 
@@ -264,6 +266,13 @@ class OracleModel:
         return dict(psi=as_complex(p).copy(), times=times[:n], energies=en[:n],
                     autocorr=as_complex(ac)[:n].copy(), steps=int(stats[0]), rejected=int(stats[1]),
                     builds=int(stats[2]), derivs=int(stats[3]))
+
+    def tensor_tree_bytes(self, psi):
+        """TensorTree<cd>::write of psi, as the oracle restates it."""
+        n = lib().orc_tensor_tree_write(self.h, cview(psi), None, 0)
+        buf = C.create_string_buffer(n)
+        lib().orc_tensor_tree_write(self.h, cview(psi), buf, n)
+        return buf.raw[:n]
 
     def top_matrices(self, psi):
         """<Psi_i|H|Psi_j> and <Psi_i|Psi_j> of the top node's states."""

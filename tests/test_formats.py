@@ -112,3 +112,15 @@ def test_matrix_tree_read_limits():
     assert Q.lib().qtg_matrix_tree_read(0, view, len(data), 2, dims, coef, 12, C.byref(cnt)) == -3
     assert Q.lib().qtg_matrix_tree_read(0, view, len(data), 2, dims, coef, 13, C.byref(cnt)) == len(data)
     assert list(dims) == [3, 3, 2, 2]
+
+
+def test_tensor_tree_bytes_match_oracle_writer():
+    """The oracle's own restatement of TensorTree::write produces the same
+    bytes, and the C ABI reads them back."""
+    text, _, _ = W.eig8_model(3)
+    om = O.OracleModel(text)
+    psi = om.fill_random(9, delta_lowest=False)
+    data = om.tensor_tree_bytes(psi)
+    assert data == Q.tensor_tree_bytes(text, psi)
+    back, used = Q.tensor_tree_from_bytes(text, data)
+    assert used == len(data) and np.array_equal(back, psi)
