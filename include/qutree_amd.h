@@ -78,6 +78,17 @@ int qtg_ctmatvec_z(double* C, const double* B, const double* M, int64_t a, int64
 int qtg_rhomat_z(const double* bra, const double* ket, double* M, int64_t a, int64_t b, int64_t c,
                  qtg_stream stream);
 
+/* Real products, replace LA_lib.f rmatvec_ (LA_lib.f:314-379, bound in
+ * Tensor_Implementation.h:457 and called from matrixTensor for real M and B,
+ * :569-572) and trmatvec_ (LA_lib.f:402-467, bound in Matrix_Implementation.h:
+ * 459 and called from multAB for real square B, :481-483):
+ *   rmatvec:  C(b,k,c) (+)= sum_l M(k,l) B(b,l,c),
+ *   trmatvec: C(b,k,c) (+)= sum_l M(l,k) B(b,l,c),  all real FP64. */
+int qtg_rmatvec_d(double* C, const double* B, const double* M, int64_t a, int64_t b, int64_t c, int add,
+                  qtg_stream stream);
+int qtg_trmatvec_d(double* C, const double* B, const double* M, int64_t a, int64_t b, int64_t c, int add,
+                   qtg_stream stream);
+
 /* Rectangular mode product with matrixTensorBLAS semantics
  * (src/Core/TensorBLAS.cpp:124-173,232-271): M is nC x nB, B (before,nB,after),
  * C (before,nC,after).  op: 0 = M, 1 = M^T, 2 = M^H (M then nB x nC). */
@@ -121,6 +132,27 @@ int qtg_plan_destroy(qtg_plan plan);
 /* Total algorithmic FLOPs (8 per complex multiply-add) and minimum HBM bytes
  * (each operand read once, each result written once) of a plan. */
 int qtg_plan_counts(qtg_plan plan, double* flops, double* bytes);
+
+/* ------------------------------------------------------ small matrices, FP64 */
+/* Batches of `count` matrices stored back to back (device pointers,
+ * complex128 column-major), one workgroup per matrix; 1 <= n <= 64.
+ * qtg_heev_batch_z: Hermitian eigendecomposition A = U diag(lambda) U^H
+ * (Matrix::cDiag, include/Core/Matrix_Implementation.h:284-308): lambda (n
+ * doubles per matrix) ascending, the columns of U the eigenvectors, each
+ * phased so that its first component is real and >= 0 (the reference only
+ * flips columns whose first component has a negative real part; eigenvector
+ * phases of its Eigen solver are otherwise the solver's).  U or lambda may be
+ * null.  qtg_reg_inverse_batch_z: R = U diag(1 / (l + eps e^{-l/eps})) U^H
+ * (SpectralDecompositionTree.cpp:30-48, Vector_Implementation.h:266-280).
+ * qtg_inverse_batch_z: A^-1 by Gauss-Jordan with partial pivoting (Matrix::
+ * cInv, Matrix_Implementation.h:311-328).  qtg_qr_batch_z: the thin Q (m x n,
+ * m >= n) of a Householder QR (qr(Matrixcd), Matrix_Implementation.h:795-806;
+ * LAPACK zgeqrf/zungqr sign convention, R with diagonal -sign(Re a_jj) |.|);
+ * A and Q may be the same buffer. */
+int qtg_heev_batch_z(const double* A, double* U, double* lambda, int64_t n, int64_t count, qtg_stream stream);
+int qtg_reg_inverse_batch_z(const double* A, double* R, int64_t n, int64_t count, double eps, qtg_stream stream);
+int qtg_inverse_batch_z(const double* A, double* Ainv, int64_t n, int64_t count, qtg_stream stream);
+int qtg_qr_batch_z(const double* A, double* Q, int64_t m, int64_t n, int64_t count, qtg_stream stream);
 
 /* ------------------------------------------------------------- tree engine */
 /* A device-resident model of one tree and one sum-of-products Hamiltonian.

@@ -153,6 +153,30 @@ void tmatvec(cd* C, const cd* B, const cd* M, size_t a, size_t b, size_t c, bool
       }
 }
 
+// LA_lib.f:314-379 rmatvec: real mulpsi(i,k,j) (+)= matrix(k,l) * psi(i,l,j)
+void rmatvec(double* C, const double* B, const double* M, size_t a, size_t b, size_t c, bool add) {
+  if (!add)
+    for (size_t i = 0; i < a * b * c; ++i) C[i] = 0.;
+  for (size_t j = 0; j < c; ++j)
+    for (size_t k = 0; k < a; ++k)
+      for (size_t l = 0; l < a; ++l) {
+        const double m = M[l * a + k];
+        for (size_t i = 0; i < b; ++i) C[i + k * b + j * a * b] += m * B[i + l * b + j * a * b];
+      }
+}
+
+// LA_lib.f:402-467 trmatvec: real mulpsi(i,k,j) (+)= matrix(l,k) * psi(i,l,j)
+void trmatvec(double* C, const double* B, const double* M, size_t a, size_t b, size_t c, bool add) {
+  if (!add)
+    for (size_t i = 0; i < a * b * c; ++i) C[i] = 0.;
+  for (size_t j = 0; j < c; ++j)
+    for (size_t k = 0; k < a; ++k)
+      for (size_t l = 0; l < a; ++l) {
+        const double m = M[k * a + l];
+        for (size_t i = 0; i < b; ++i) C[i + k * b + j * a * b] += m * B[i + l * b + j * a * b];
+      }
+}
+
 // LA_lib.f:718-762 rhomat: matrix(j,i) = sum_{n,m} conjg(bra(n,j,m)) ket(n,i,m).
 // Always overwrites (LA_lib.f:729).
 void rhomat(const cd* bra, const cd* ket, cd* M, size_t a, size_t b, size_t c) {
@@ -397,6 +421,51 @@ Mat inverse(const Mat& Ain) {
     }
   }
   return I;
+}
+
+// Thin Q of a Householder QR of the m x n matrix A (m >= n): qr(Matrixcd)
+// (Matrix_Implementation.h:795-806) takes householderQ() * Identity(m, n) of
+// Eigen's HouseholderQR.  Eigen's makeHouseholder (and LAPACK zlarfg) choose
+// beta = -sign(Re alpha) |(alpha, x)| (negative for Re alpha = 0),
+// v = [1; x / (alpha - beta)], tau = (beta - alpha) / beta, and the identity
+// reflector when x = 0 and alpha is real; Q = H_0 H_1 ... H_{n-1}.
+Mat householderQ(const Mat& Ain) {
+  const size_t m = Ain.d1, n = Ain.d2;
+  Mat A = Ain;
+  vector<cd> tau(n, 0.);
+  for (size_t j = 0; j < n; ++j) {
+    double s = 0.;
+    for (size_t r = j + 1; r < m; ++r) s += std::norm(A(r, j));
+    const cd alpha = A(j, j);
+    if (s == 0. && std::imag(alpha) == 0.) continue;
+    const double nr = std::sqrt(std::norm(alpha) + s);
+    const double beta = std::real(alpha) >= 0. ? -nr : nr;
+    tau[j] = (beta - alpha) / beta;
+    const cd sc = 1. / (alpha - beta);
+    for (size_t r = j + 1; r < m; ++r) A(r, j) *= sc;
+    A(j, j) = beta;
+    // A(j:m, k) <- (I - conj(tau) v v^H) A(j:m, k)
+    for (size_t k = j + 1; k < n; ++k) {
+      cd w = A(j, k);
+      for (size_t r = j + 1; r < m; ++r) w += std::conj(A(r, j)) * A(r, k);
+      const cd f = std::conj(tau[j]) * w;
+      A(j, k) -= f;
+      for (size_t r = j + 1; r < m; ++r) A(r, k) -= A(r, j) * f;
+    }
+  }
+  // Q = H_0 ... H_{n-1} [I; 0], applied right to left
+  Mat Q(m, n);
+  for (size_t i = 0; i < n; ++i) Q(i, i) = 1.;
+  for (size_t jj = n; jj-- > 0;) {
+    for (size_t k = 0; k < n; ++k) {
+      cd w = Q(jj, k);
+      for (size_t r = jj + 1; r < m; ++r) w += std::conj(A(r, jj)) * Q(r, k);
+      const cd f = tau[jj] * w;
+      Q(jj, k) -= f;
+      for (size_t r = jj + 1; r < m; ++r) Q(r, k) -= A(r, jj) * f;
+    }
+  }
+  return Q;
 }
 
 // Regularised inverse U diag(1/(l + eps e^{-l/eps})) U^H
@@ -1686,6 +1755,24 @@ void orc_heev(const double* A, long n, double* U, double* ev) {
   heev(M, V, e);
   std::memcpy(U, V.c.data(), sizeof(cd) * n * n);
   for (long i = 0; i < n; ++i) ev[i] = e[i];
+}
+void orc_rmatvec(double* C, const double* B, const double* M, long a, long b, long c, int add) {
+  rmatvec(C, B, M, a, b, c, add != 0);
+}
+void orc_trmatvec(double* C, const double* B, const double* M, long a, long b, long c, int add) {
+  trmatvec(C, B, M, a, b, c, add != 0);
+}
+void orc_inverse(const double* A, long n, double* R) {
+  Mat M(n, n);
+  std::memcpy(M.c.data(), A, sizeof(cd) * n * n);
+  Mat X = inverse(M);
+  std::memcpy(R, X.c.data(), sizeof(cd) * n * n);
+}
+void orc_qr(const double* A, long m, long n, double* Q) {
+  Mat M(m, n);
+  std::memcpy(M.c.data(), A, sizeof(cd) * m * n);
+  Mat X = householderQ(M);
+  std::memcpy(Q, X.c.data(), sizeof(cd) * m * n);
 }
 void orc_reg_inverse(const double* A, long n, double eps, double* R) {
   Mat M(n, n);

@@ -194,8 +194,23 @@ int qtg_engine_read_timing(qtg_engine e, double* ms, int64_t* launches, double* 
 int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   if (!e || !key) return QTG_EARG;
   auto* en = static_cast<Engine*>(e);
+  // keys that shape the plan (buffer sizes, launch lists) are fixed by
+  // set_hamiltonian: changing them afterwards would run kernels on buffers
+  // sized for another plan
+  static const char* const planKeys[] = {"compress", "path", "split_waves", "split_min_k"};
+  for (const char* k : planKeys)
+    if (!std::strcmp(key, k) && en->planned()) {
+      set_error(std::string("option ") + key + ": set it before qtg_engine_set_hamiltonian");
+      return QTG_ESTATE;
+    }
   if (!std::strcmp(key, "rho_eps")) {
+    if (!(value > 0.0)) {
+      set_error("option rho_eps: must be positive");
+      return QTG_EARG;
+    }
     en->eps_ = value;
+    // a captured sweep holds the regulariser as a launch argument
+    en->invalidate_graph();
     return QTG_OK;
   }
   if (!std::strcmp(key, "compress")) {

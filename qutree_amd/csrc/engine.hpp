@@ -47,6 +47,7 @@ class Engine {
   double* dpsi64() { return d_dpsi64_; }
   double* apply64() { return d_apply64_; }
   bool complex64() const { return c64_; }
+  bool planned() const { return planned_; }
   int64_t sandwich_items() const { return nSw_; }
   int64_t apply_terms_chunks() const { return nChunks_; }
   int64_t apply_terms() const { return (int64_t)h_terms_.size(); }

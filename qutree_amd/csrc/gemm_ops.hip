@@ -112,15 +112,33 @@ struct GmTile {
   int64_t rt;
   int ct, ks;
 };
-__device__ inline GmTile gm_tile(const GmDesc& d, int64_t t) {
-  const int split = d.ksplit > 1 ? d.ksplit : 1;
+// A descriptor's tasks number fewer than 2^31 (a launch's grid), so the tile
+// arithmetic runs in 32 bits; ksplit is a power of two.
+__device__ inline GmTile gm_tile(const GmDesc& d, int64_t t64) {
+  const unsigned split = d.ksplit > 1 ? (unsigned)d.ksplit : 1u;
+  unsigned t = (unsigned)t64;
   GmTile g;
-  g.ks = (int)(t % split);
-  t /= split;
-  const int CT = (d.nout + 15) >> 4;
-  g.rt = t / CT;
-  g.ct = (int)(t - g.rt * CT);
+  g.ks = (int)(t & (split - 1));
+  t >>= __builtin_ctz(split);
+  const unsigned CT = (unsigned)(d.nout + 15) >> 4;
+  const unsigned rt = t / CT;
+  g.rt = rt;
+  g.ct = (int)(t - rt * CT);
   return g;
+}
+
+// (b, a) = (r % bef, r / bef) of a flattened (b, a) row: 32-bit division
+// when the rows of the view fit in 31 bits (every node tensor), else 64-bit.
+// R is uniform across the wavefront, so is the branch.
+__device__ inline void row_split(int64_t r, int64_t bef, int64_t R, int64_t& b, int64_t& a) {
+  if (R < ((int64_t)1 << 31)) {
+    const unsigned ur = (unsigned)r, ub = (unsigned)bef, q = ur / ub;
+    a = q;
+    b = ur - q * ub;
+  } else {
+    a = r / bef;
+    b = r - a * bef;
+  }
 }
 
 // out(b,i,a) = alpha X(b,i,a) for one 16 x 16 tile, four columns per lane
@@ -134,7 +152,8 @@ __device__ inline void gm_copy(const GmDesc& d, const GmTile& g, const cplx<T>* 
     const int64_t r = g.rt * 16 + li;
     const int col = g.ct * 16 + kk * 4 + q;
     if (r < R && col < d.nout) {
-      const int64_t b = r % bef, a = r / bef;
+      int64_t b, a;
+      row_split(r, bef, R, b, a);
       const cplx<T> x = X[d.x + b + (int64_t)col * bef + a * bef * d.nin];
       const int64_t o = d.out + b + (int64_t)col * bef + a * bef * d.nout;
       cplx<TO> v = mk<TO>((TO)(are * x.re - aim * x.im), (TO)(are * x.im + aim * x.re));
@@ -159,7 +178,9 @@ __device__ inline void gm_product(const GmDesc& d, const GmTile& g, const cplx<T
   // A operand: row r = rt*16 + li of the flattened (b, a), column k
   const int64_t r = g.rt * 16 + li;
   const bool rv = r < R;
-  const int64_t xoff = rv ? r % bef + (r / bef) * bef * nin : 0;
+  int64_t rb, ra;
+  row_split(rv ? r : 0, bef, R, rb, ra);
+  const int64_t xoff = rb + ra * bef * nin;
   const cplx<T>* xa = X + d.x;
   // B operand: op(M)(j, k) for column j = ct*16 + li
   const int j = g.ct * 16 + li;
@@ -202,15 +223,28 @@ __device__ inline void gm_store(const GmDesc& d, const GmTile& g, const cplx<TO>
   const int64_t bef = d.bef, R = bef * d.aft;
   const int j = g.ct * 16 + (lane & 15);
   if (j >= d.nout) return;
+  // the lane's four rows are rr0 + q * st: (b, a) of the first by division,
+  // the others by a carry step
+  const int st = MF<T>::drow(0, 1);
+  int64_t rr = g.rt * 16 + MF<T>::drow(lane, 0);
+  int64_t b, a;
+  row_split(rr < R ? rr : 0, bef, R, b, a);
+  const int qs = st / (int)min(bef, (int64_t)64), rs = st % (int)min(bef, (int64_t)64);
+  const int64_t ostr = bef * d.nout;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int64_t rr = g.rt * 16 + MF<T>::drow(lane, q);
     if (rr < R) {
-      const int64_t b = rr % bef, a = rr / bef;
-      const int64_t o = d.out + b + (int64_t)j * bef + a * bef * d.nout;
+      const int64_t o = d.out + b + (int64_t)j * bef + a * ostr;
       cplx<TO> x = v[q];
       if (d.acc) x = cadd(x, O[o]);
       O[o] = x;
+    }
+    rr += st;
+    b += rs;
+    a += qs;
+    if (b >= bef) {
+      b -= bef;
+      ++a;
     }
   }
 }
@@ -256,6 +290,51 @@ __global__ void __launch_bounds__(64 * kWaves)
   cplx<T> v[4];
   gm_product<T, T, T>(d, g, X, M, 1.0, 0.0, v);
   gm_store<T, T>(d, g, v, O);
+}
+
+// ------------------------------------------------------ real mode product
+// out(b,i,a) (+)= sum_l op(M)(i,l) X(b,l,a) on real FP64 data (the rmatvec /
+// trmatvec kernels of LA_lib.f): the tile of gm_product with one real MFMA
+// per k-step.  op is GM_N or GM_T.
+__global__ void __launch_bounds__(64 * kWaves)
+    gmr_single(GmDesc d, int64_t ntasks, const double* __restrict__ X, const double* __restrict__ M, double* O) {
+  typedef MF<double>::v4 v4;
+  const int64_t task = (int64_t)blockIdx.x * kWaves + wave_id();
+  if (task >= ntasks) return;
+  const GmTile g = gm_tile(d, task);
+  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const int64_t bef = d.bef, R = bef * d.aft;
+  const int nin = d.nin, nout = d.nout;
+  const int64_t r = g.rt * 16 + li;
+  const bool rv = r < R;
+  const int64_t xoff = rv ? r % bef + (r / bef) * bef * nin : 0;
+  const int j = g.ct * 16 + li;
+  const bool cv = j < nout;
+  const int64_t moff = !cv ? 0 : d.op == GM_N ? j : (int64_t)j * nin;
+  const int64_t mks = d.op == GM_N ? nout : 1;
+  v4 c0 = {0, 0, 0, 0}, c1 = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < nin; k0 += 8) {
+    const int ka = k0 + kk, kb = k0 + 4 + kk;
+    const bool va = ka < nin, vb = kb < nin;
+    // unconditional in-bounds loads (element 0 when the lane is idle)
+    const double xa = X[(rv && va) ? xoff + (int64_t)ka * bef : 0];
+    const double ma = M[(cv && va) ? moff + (int64_t)ka * mks : 0];
+    const double xb = X[(rv && vb) ? xoff + (int64_t)kb * bef : 0];
+    const double mb = M[(cv && vb) ? moff + (int64_t)kb * mks : 0];
+    c0 = MF<double>::mma((rv && va) ? xa : 0.0, (cv && va) ? ma : 0.0, c0);
+    if (k0 + 4 < nin) c1 = MF<double>::mma((rv && vb) ? xb : 0.0, (cv && vb) ? mb : 0.0, c1);
+  }
+  if (!cv) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t rr = g.rt * 16 + MF<double>::drow(lane, q);
+    if (rr < R) {
+      const int64_t b = rr % bef, a = rr / bef;
+      const int64_t o = d.out + b + (int64_t)j * bef + a * bef * nout;
+      const double v = c0[q] + c1[q];
+      O[o] = d.acc ? O[o] + v : v;
+    }
+  }
 }
 
 // --------------------------------------------------------- hole contraction
@@ -814,6 +893,19 @@ int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hip
   if (nt <= 0) return QTG_OK;
   hipLaunchKernelGGL(gc_single<T>, dim3((unsigned)((nt + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0, s, d, nt,
                      (const cplx<T>*)x, (const cplx<T>*)y, (cplx<T>*)o);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+int launch_gm_real(const GmDesc& d, const double* x, const double* m, double* o, hipStream_t s) {
+  if (d.ksplit > 1 || (d.op != GM_N && d.op != GM_T)) {
+    set_error("real mode product: op M or M^T, no K split");
+    return QTG_EARG;
+  }
+  const int64_t nt = gm_tasks(d);
+  if (nt <= 0) return QTG_OK;
+  hipLaunchKernelGGL(gmr_single, dim3((unsigned)((nt + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0, s, d, nt, x, m,
+                     o);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }

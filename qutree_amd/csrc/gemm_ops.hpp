@@ -120,6 +120,9 @@ int launch_gm_single(const GmDesc& d, const void* x, const void* m, void* o, hip
 template <typename T>
 int launch_gc_single(const GcDesc& d, const void* x, const void* y, void* o, hipStream_t s);
 
+// Real FP64 mode product (op GM_N or GM_T; one descriptor, no K split).
+int launch_gm_real(const GmDesc& d, const double* x, const double* m, double* o, hipStream_t s);
+
 // Batched FP64 solve X <- S^-1 X for n x n column-major S and X at pool
 // offsets s, x; n <= nmax <= 64.  nmax > 48: Gauss-Jordan with partial
 // pivoting.  nmax <= 48: Gauss-Jordan on the diagonal pivots of the positive

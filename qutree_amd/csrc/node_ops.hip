@@ -8,6 +8,7 @@
 // their workgroup, in chunk order (deterministic).
 #include <algorithm>
 #include <climits>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -27,8 +28,11 @@ int to_gm(const qtg_mode_product_desc& d, GmDesc& g) {
     set_error("mode product: need nB, nC, before, after >= 1 and op in {0,1,2}");
     return QTG_ESHAPE;
   }
+  // the tile kernels index rows of the flattened (before, after) in 64 bits
+  // and matrices in 32 bits; every product is checked before it is formed
   if (d.before > INT_MAX || d.after > INT_MAX || d.nB > INT_MAX || d.nC > INT_MAX ||
-      d.before * d.after > ((int64_t)1 << 40)) {
+      d.before * d.after > ((int64_t)1 << 40) || d.nB * d.nC >= ((int64_t)1 << 31) ||
+      d.before * d.after > (((int64_t)1 << 62) / std::max(d.nB, d.nC))) {
     set_error("mode product: dimensions out of range");
     return QTG_ESHAPE;
   }
@@ -54,8 +58,13 @@ int to_gc(const qtg_hole_contraction_desc& d, GcDesc& g) {
     set_error("hole contraction: dimensions must be >= 1");
     return QTG_ESHAPE;
   }
-  if (d.before > INT_MAX || d.after > INT_MAX || d.nA > INT_MAX || d.nB > INT_MAX) {
-    set_error("hole contraction: dimensions out of range");
+  // The reduction walks k = b + a before over K = before * after and steps a
+  // by before * n (gc_product): both stay in 32 bits only below 2^31, with
+  // 16 more for the last k-step's overrun.
+  const int64_t lim = (int64_t)1 << 31;
+  if (d.before >= lim || d.after >= lim || d.nA >= lim || d.nB >= lim || d.before * d.after + 16 >= lim ||
+      d.before * std::max(d.nA, d.nB) >= lim || d.nA * d.nB >= lim) {
+    set_error("hole contraction: before*after and before*max(nA, nB) must stay below 2^31");
     return QTG_ESHAPE;
   }
   g = GcDesc{};
@@ -164,6 +173,31 @@ int qtg_hole_contraction_c(float* Sm, const float* A, const float* B, int64_t be
 int qtg_rhomat_z(const double* bra, const double* ket, double* M, int64_t a, int64_t b, int64_t c,
                  qtg_stream stream) {
   return qtg_hole_contraction_z(M, bra, ket, b, a, a, c, 0, stream);
+}
+
+int qtg_rmatvec_d(double* C, const double* B, const double* M, int64_t a, int64_t b, int64_t c, int add,
+                  qtg_stream stream) {
+  if (!C || !B || !M) {
+    set_error("rmatvec: null pointer");
+    return QTG_EARG;
+  }
+  qtg_mode_product_desc d{0, 0, 0, b, a, a, c, 0, add};
+  GmDesc g;
+  int rc = to_gm(d, g);
+  if (rc) return rc;
+  return launch_gm_real(g, B, M, C, S(stream));
+}
+int qtg_trmatvec_d(double* C, const double* B, const double* M, int64_t a, int64_t b, int64_t c, int add,
+                   qtg_stream stream) {
+  if (!C || !B || !M) {
+    set_error("trmatvec: null pointer");
+    return QTG_EARG;
+  }
+  qtg_mode_product_desc d{0, 0, 0, b, a, a, c, 1, add};
+  GmDesc g;
+  int rc = to_gm(d, g);
+  if (rc) return rc;
+  return launch_gm_real(g, B, M, C, S(stream));
 }
 
 int qtg_mode_product_plan_create(qtg_plan* plan, const qtg_mode_product_desc* descs, int64_t count,

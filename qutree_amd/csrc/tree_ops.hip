@@ -337,8 +337,60 @@ __global__ void __launch_bounds__(TB)
 }
 
 // ------------------------------------------- density eigen / regularised inverse
-// One wavefront per density matrix (jacobi.hpp), then
-// R = U diag(1/(l + eps e^{-l/eps})) U^H.
+// One wavefront per density matrix.  R = U diag(1/(l + eps e^{-l/eps})) U^H
+// (SpectralDecompositionTree.cpp:30-48).  When rho is comfortably positive
+// definite the regulariser is below rounding and R is rho^-1 itself, which
+// n Gauss-Jordan steps give (fast path, no eigenvalues); otherwise the
+// tournament Jacobi solver (jacobi.hpp) diagonalises rho.
+//
+// Fast path: in-place Gauss-Jordan on the diagonal pivots of the hermitised
+// rho, ping-ponging between the two LDS buffers.  It is taken only when
+// every pivot is positive and finite and 1/||X||_F >= 50 eps for the result
+// X: 1/||X||_F bounds the smallest eigenvalue l_min from below, so
+// eps e^{-l/eps} <= e^{-50} l < 2e-22 l for every eigenvalue, far below the
+// rounding of either algorithm, and R = X to rounding.  It also asks for a
+// condition number below 1e6 (||X||_F tr(rho) <= 1e6, tr2 = tr(rho)^2), so
+// that rho^-1 carries at most ~1e-10 relative rounding either way; worse
+// conditioned densities (an SPF almost unoccupied) take the eigensolver.
+__device__ inline bool gj_inverse_certified(cplx<double>*& S, cplx<double>*& D, int n, int ld, double eps,
+                                            double tr2) {
+  const int nn = n * n;
+  for (int k = 0; k < n; ++k) {
+    const cplx<double> p = S[k + k * ld];  // the same element in every lane: a uniform test
+    if (!(p.re > 0.0) || !(p.re < 1e300) || !(fabs(p.im) < 1e300)) return false;
+    const double dd = p.re * p.re + p.im * p.im;
+    const cplx<double> pinv = mk<double>(p.re / dd, -p.im / dd);
+    for (int o = threadIdx.x; o < nn; o += EW) {
+      const int i = o % n, j = o / n;
+      const cplx<double> m = S[i + j * ld];
+      cplx<double> r;
+      if (i == k && j == k) {
+        r = pinv;
+      } else if (i == k) {
+        r = cmul(m, pinv);
+      } else if (j == k) {
+        const cplx<double> t = cmul(m, pinv);
+        r = mk<double>(-t.re, -t.im);
+      } else {
+        const cplx<double> t = cmul(cmul(S[i + k * ld], S[k + j * ld]), pinv);
+        r = mk<double>(m.re - t.re, m.im - t.im);
+      }
+      D[i + j * ld] = r;
+    }
+    __syncthreads();
+    cplx<double>* t = S;
+    S = D;
+    D = t;
+  }
+  double f = 0.0;
+  for (int o = threadIdx.x; o < nn; o += EW) {
+    const cplx<double> x = S[o % n + (o / n) * ld];
+    f += x.re * x.re + x.im * x.im;
+  }
+  f = wave_sum(f);
+  return f < 1e300 && f * (50.0 * eps) * (50.0 * eps) <= 1.0 && f * tr2 <= 1e12;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(EW)
     heev_reginv_kernel(const EigItem* __restrict__ items, const cplx<T>* __restrict__ pool,
@@ -351,6 +403,23 @@ __global__ void __launch_bounds__(EW)
   const EigItem it = items[blockIdx.x];
   const int n = it.n;
   const int ld = jacobi_ld(n);
+  if (it.ev < 0) {
+    for (int o = threadIdx.x; o < n * n; o += EW) {
+      const int i = o % n, j = o / n;
+      const cplx<T> a = pool[it.in + i + j * n], b = pool[it.in + j + i * n];
+      A[i + j * ld] = mk<double>(0.5 * ((double)a.re + (double)b.re), 0.5 * ((double)a.im - (double)b.im));
+    }
+    __syncthreads();
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += A[i + i * ld].re;
+    cplx<double>* S = A;
+    cplx<double>* D = U;
+    if (gj_inverse_certified(S, D, n, ld, eps, tr * tr)) {
+      for (int o = threadIdx.x; o < n * n; o += EW) pool64[it.out + o] = S[o % n + (o / n) * ld];
+      return;
+    }
+    __syncthreads();
+  }
   for (int o = threadIdx.x; o < n * n; o += EW) {
     const int i = o % n, j = o / n;
     const cplx<T> v = pool[it.in + o];

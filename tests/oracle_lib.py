@@ -18,13 +18,44 @@ dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 
+_STRICT_PATH = os.path.join(_ROOT, "oracle", "_build", "libqutree_oracle_strict.so")
+_strict = None
+_use_strict = False
+
+
+class strict_build:
+    """Within this context every oracle call (models created inside it
+    included) goes to the -O2, no-fast-math build of the same source: a
+    second summation order of the same algorithm."""
+
+    def __enter__(self):
+        global _use_strict
+        _use_strict = True
+        return self
+
+    def __exit__(self, *exc):
+        global _use_strict
+        _use_strict = False
+        return False
+
+
 def lib():
-    global _lib
+    global _lib, _strict
+    if _use_strict:
+        if _strict is None:
+            if not os.path.exists(_STRICT_PATH):
+                subprocess.check_call(["make", "-s", "-C", os.path.join(_ROOT, "oracle")])
+            _strict = _bind(C.CDLL(_STRICT_PATH))
+        return _strict
     if _lib is not None:
         return _lib
     if not os.path.exists(_LIB_PATH):
         subprocess.check_call(["make", "-s", "-C", os.path.join(_ROOT, "oracle")])
-    L = C.CDLL(_LIB_PATH)
+    _lib = _bind(C.CDLL(_LIB_PATH))
+    return _lib
+
+
+def _bind(L):
     L.orc_last_error.restype = C.c_char_p
     L.orc_matvec.argtypes = [dp, dp, dp, C.c_long, C.c_long, C.c_long, C.c_int]
     L.orc_tmatvec.argtypes = L.orc_matvec.argtypes
@@ -35,6 +66,10 @@ def lib():
     L.orc_microbench_batch.argtypes = [C.c_long, lp, lp, lp, lp, dp, dp, dp, dp, dp]
     L.orc_gram_schmidt.argtypes = [dp, C.c_long, C.c_long]
     L.orc_heev.argtypes = [dp, C.c_long, dp, dp]
+    L.orc_rmatvec.argtypes = L.orc_matvec.argtypes
+    L.orc_trmatvec.argtypes = L.orc_matvec.argtypes
+    L.orc_inverse.argtypes = [dp, C.c_long, dp]
+    L.orc_qr.argtypes = [dp, C.c_long, C.c_long, dp]
     L.orc_reg_inverse.argtypes = [dp, C.c_long, C.c_double, dp]
     L.orc_project_out.argtypes = [dp, dp, C.c_long, C.c_long, dp]
     L.orc_balanced_tree_text.argtypes = [C.c_long] * 4 + [C.c_double] * 4 + [C.c_char_p, C.c_long]
@@ -73,7 +108,6 @@ def lib():
     L.orc_top_matrices.argtypes = [C.c_void_p, dp, dp, dp]
     L.orc_tensor_tree_write.argtypes = [C.c_void_p, dp, C.c_void_p, C.c_long]
     L.orc_tensor_tree_write.restype = C.c_long
-    _lib = L
     return L
 
 
@@ -119,6 +153,7 @@ class OracleModel:
 
     def __init__(self, tree_text: str, H=None):
         L = lib()
+        self._L = L  # the build this model lives in (see strict_build)
         self.h = L.orc_model_create(tree_text.encode())
         if not self.h:
             raise RuntimeError(L.orc_last_error().decode())
@@ -134,7 +169,7 @@ class OracleModel:
 
     def _info(self, a):
         buf = np.zeros(64, np.int64)
-        k = lib().orc_model_node_info(self.h, a, buf, 64)
+        k = self._L.orc_model_node_info(self.h, a, buf, 64)
         v = buf[:k]
         order, nch = int(v[4]), int(v[5])
         return dict(address=a, parent=int(v[0]), childIdx=int(v[1]), bottom=bool(v[2]), layer=int(v[3]),
@@ -142,15 +177,15 @@ class OracleModel:
                     children=[int(x) for x in v[7 + order:7 + order + nch]])
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.orc_model_destroy(self.h)
+        if getattr(self, "h", None) and getattr(self, "_L", None) is not None:
+            self._L.orc_model_destroy(self.h)
             self.h = None
 
     def set_hamiltonian(self, H):
         c, toff, fm, fk, fp, pool = H.arrays()
-        r = lib().orc_model_set_hamiltonian(self.h, H.nterms, c, toff, fm, fk, fp, pool)
+        r = self._L.orc_model_set_hamiltonian(self.h, H.nterms, c, toff, fm, fk, fp, pool)
         if r:
-            raise RuntimeError(lib().orc_last_error().decode())
+            raise RuntimeError(self._L.orc_last_error().decode())
         self.nterms = H.nterms
 
     def leaf(self, mode):
@@ -159,17 +194,17 @@ class OracleModel:
         x = np.zeros(d)
         kin = np.zeros(d * d)
         tr = np.zeros(d * d)
-        lib().orc_model_leaf(self.h, mode, x, kin, tr)
+        self._L.orc_model_leaf(self.h, mode, x, kin, tr)
         return x, kin.reshape(d, d, order="F"), tr.reshape(d, d, order="F")
 
     def fill_random(self, seed, delta_lowest=True):
         psi = np.zeros(2 * self.psi_size)
-        lib().orc_fill_random(self.h, seed, int(delta_lowest), psi)
+        self._L.orc_fill_random(self.h, seed, int(delta_lowest), psi)
         return as_complex(psi).copy()
 
     def orthogonal(self, psi):
         p = cview(psi).copy()
-        lib().orc_orthogonal(self.h, p)
+        self._L.orc_orthogonal(self.h, p)
         return as_complex(p).copy()
 
     def node_tensor(self, psi, a):
@@ -179,23 +214,23 @@ class OracleModel:
 
     def sparse_tree(self, l):
         buf = np.zeros(self.nnodes, np.int64)
-        k = lib().orc_sparse_tree(self.h, l, buf)
+        k = self._L.orc_sparse_tree(self.h, l, buf)
         return buf[:k].tolist()
 
     def sparse_tree_modes(self, modes, tail=True):
         buf = np.zeros(self.nnodes, np.int64)
         m = np.asarray(modes, np.int64)
-        k = lib().orc_sparse_tree_modes(self.h, m, len(m), int(tail), buf)
+        k = self._L.orc_sparse_tree_modes(self.h, m, len(m), int(tail), buf)
         return buf[:k].tolist()
 
     def build(self, psi, eps=1e-10):
-        if lib().orc_build(self.h, cview(psi), eps):
-            raise RuntimeError(lib().orc_last_error().decode())
+        if self._L.orc_build(self.h, cview(psi), eps):
+            raise RuntimeError(self._L.orc_last_error().decode())
 
     def matrix(self, which, l, a):
         n = self.nodes[a]["dims"][-1]
         out = np.zeros(2 * n * n)
-        r = lib().orc_get_matrix(self.h, which, l, a, out)
+        r = self._L.orc_get_matrix(self.h, which, l, a, out)
         if r:
             return None
         return as_complex(out).reshape(n, n, order="F").copy()
@@ -214,16 +249,16 @@ class OracleModel:
 
     def derivative(self, psi, phase=1.0, eps=1e-10):
         out = np.zeros(2 * self.psi_size)
-        if lib().orc_derivative(self.h, cview(psi), out, complex(phase).real, complex(phase).imag, eps):
-            raise RuntimeError(lib().orc_last_error().decode())
+        if self._L.orc_derivative(self.h, cview(psi), out, complex(phase).real, complex(phase).imag, eps):
+            raise RuntimeError(self._L.orc_last_error().decode())
         return as_complex(out).copy()
 
     def gradient(self, psi, eps=1e-10):
         """dE/dA* at every node (orthonormal lower nodes) and the energy."""
         out = np.zeros(2 * self.psi_size)
         e = C.c_double()
-        if lib().orc_gradient(self.h, cview(psi), out, eps, C.byref(e)):
-            raise RuntimeError(lib().orc_last_error().decode())
+        if self._L.orc_gradient(self.h, cview(psi), out, eps, C.byref(e)):
+            raise RuntimeError(self._L.orc_last_error().decode())
         return as_complex(out).copy(), e.value
 
     def ttno_apply(self, bond, B, bra, ket):
@@ -232,23 +267,23 @@ class OracleModel:
         tensors concatenated in address order (complex)."""
         out = np.zeros(2 * self.psi_size)
         e = np.zeros(2)
-        if lib().orc_ttno_apply(self.h, np.ascontiguousarray(bond, np.int64), cview(np.ascontiguousarray(B)),
+        if self._L.orc_ttno_apply(self.h, np.ascontiguousarray(bond, np.int64), cview(np.ascontiguousarray(B)),
                                 cview(bra), cview(ket), out, e):
-            raise RuntimeError(lib().orc_last_error().decode())
+            raise RuntimeError(self._L.orc_last_error().decode())
         return as_complex(out).copy(), complex(e[0], e[1])
 
     def apply_all(self, psi):
         out = np.zeros(2 * self.psi_size)
-        if lib().orc_apply_all(self.h, cview(psi), out):
+        if self._L.orc_apply_all(self.h, cview(psi), out):
             raise RuntimeError("build first")
         return as_complex(out).copy()
 
     def energy(self, psi):
-        return lib().orc_energy(self.h, cview(psi))
+        return self._L.orc_energy(self.h, cview(psi))
 
     def overlap(self, bra, ket):
         out = np.zeros(2)
-        lib().orc_dot_product_top(self.h, cview(bra), cview(ket), out)
+        self._L.orc_dot_product_top(self.h, cview(bra), cview(ket), out)
         return complex(out[0], out[1])
 
     def cmf(self, psi, t0, tend, dt, out, acc_cmf=1e-4, acc_bs=1e-5, phase=1.0, max_steps=10 ** 9,
@@ -258,10 +293,10 @@ class OracleModel:
         en = np.zeros(max_out)
         ac = np.zeros(2 * max_out)
         stats = np.zeros(4, np.int64)
-        n = lib().orc_cmf(self.h, p, t0, tend, dt, out, acc_cmf, acc_bs, complex(phase).real,
+        n = self._L.orc_cmf(self.h, p, t0, tend, dt, out, acc_cmf, acc_bs, complex(phase).real,
                           complex(phase).imag, max_steps, max_out, times, en, ac, stats)
         if n < 0:
-            raise RuntimeError(lib().orc_last_error().decode())
+            raise RuntimeError(self._L.orc_last_error().decode())
         n = min(n, max_out)
         return dict(psi=as_complex(p).copy(), times=times[:n], energies=en[:n],
                     autocorr=as_complex(ac)[:n].copy(), steps=int(stats[0]), rejected=int(stats[1]),
@@ -269,9 +304,9 @@ class OracleModel:
 
     def tensor_tree_bytes(self, psi):
         """TensorTree<cd>::write of psi, as the oracle restates it."""
-        n = lib().orc_tensor_tree_write(self.h, cview(psi), None, 0)
+        n = self._L.orc_tensor_tree_write(self.h, cview(psi), None, 0)
         buf = C.create_string_buffer(n)
-        lib().orc_tensor_tree_write(self.h, cview(psi), buf, n)
+        self._L.orc_tensor_tree_write(self.h, cview(psi), buf, n)
         return buf.raw[:n]
 
     def top_matrices(self, psi):
@@ -279,8 +314,8 @@ class OracleModel:
         ns = self.nodes[-1]["dims"][-1]
         hm = np.zeros(2 * ns * ns)
         sm = np.zeros(2 * ns * ns)
-        if lib().orc_top_matrices(self.h, cview(psi), hm, sm):
-            raise RuntimeError(lib().orc_last_error().decode())
+        if self._L.orc_top_matrices(self.h, cview(psi), hm, sm):
+            raise RuntimeError(self._L.orc_last_error().decode())
         return (as_complex(hm).reshape(ns, ns, order="F").copy(), as_complex(sm).reshape(ns, ns, order="F").copy())
 
     def eigenstates(self, psi, dt, out, iterations, acc_cmf=1e-6, acc_bs=1e-7, max_steps=10 ** 9):
@@ -294,10 +329,10 @@ class OracleModel:
         cv = np.zeros(max(iterations, 1) * ns)
         stats = np.zeros(4, np.int64)
         dte = C.c_double()
-        n = lib().orc_eigenstates(self.h, p, dt, out, iterations, acc_cmf, acc_bs, max_steps, en, pe, cv, stats,
+        n = self._L.orc_eigenstates(self.h, p, dt, out, iterations, acc_cmf, acc_bs, max_steps, en, pe, cv, stats,
                                   C.byref(dte))
         if n < 0:
-            raise RuntimeError(lib().orc_last_error().decode())
+            raise RuntimeError(self._L.orc_last_error().decode())
         return dict(psi=as_complex(p).copy(), energies=en.reshape(iterations + 1, ns),
                     propagator=pe[:iterations * ns].reshape(iterations, ns),
                     converged=cv[:iterations * ns].reshape(iterations, ns), steps=int(stats[0]),

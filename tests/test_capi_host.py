@@ -184,3 +184,100 @@ def test_compressed_plan_counts_distinct_restrictions(model):
     assert items[0] == nrep + nhole
     if model == "sweep64":
         assert nrep + nhole < 45630 // 4
+
+
+def seam_functions():
+    text = open(os.path.join(ROOT, "include", "qutree_seam.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:void|int)\s+([a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_the_fortran_seam():
+    names = seam_functions()
+    assert set(names) == {"matvec_", "tmatvec_", "ctmatvec_", "rmatvec_", "trmatvec_", "rhomat_",
+                          "qtg_seam_status"}
+    L = Q.lib()
+    assert all(hasattr(L, n) for n in names)
+
+
+# The reference's own declarations of the Fortran kernels, as QuTree's call
+# sites see them (include/Core/Tensor_Implementation.h:450-459,
+# include/Core/Matrix_Implementation.h:454-461).  A translation unit that
+# declares them and then includes the seam header compiles only if every
+# prototype agrees, and links only if the library defines each one.
+REFERENCE_SEAM = """
+extern "C" {
+void matvec_(double *C, double *B, double *mat, int *a, int *b, int *c, int *add);
+void ctmatvec_(double *C, double *B, double *mat, int *a, int *b, int *c, int *add);
+void rmatvec_(double *C, double *B, double *mat, int *a, int *b, int *c, int *add);
+void rhomat_(double *Bra, double *Ket, double *M, int *a, int *b, int *c);
+void tmatvec_(double *C, double *B, double *mat, int *a, int *b, int *c, int *add);
+void trmatvec_(double *C, double *B, double *mat, int *a, int *b, int *c, int *add);
+}
+#include "qutree_seam.h"
+// call sites shaped like Tensor_Implementation.h:566 and Matrix_Implementation.h:478
+void call_sites(double* C, double* B, double* A, int a, int b, int c, int add) {
+  matvec_(C, B, A, &a, &b, &c, &add);
+  ctmatvec_(C, B, A, &a, &b, &c, &add);
+  rmatvec_(C, B, A, &a, &b, &c, &add);
+  tmatvec_(C, A, B, &a, &b, &c, &add);
+  trmatvec_(C, A, B, &a, &b, &c, &add);
+  rhomat_(B, A, C, &a, &b, &c);
+}
+int main() { return qtg_seam_status(); }
+"""
+
+
+def test_reference_call_sites_compile_and_link_against_the_seam(tmp_path):
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "seam_user.cpp"
+    src.write_text(REFERENCE_SEAM)
+    exe = tmp_path / "seam_user"
+    cmd = ["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+           Q.LIB_PATH, "-Wl,--unresolved-symbols=ignore-in-shared-libs"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    # a prototype that disagrees with the seam is a compile error
+    bad = tmp_path / "seam_bad.cpp"
+    bad.write_text(REFERENCE_SEAM.replace("void matvec_(double *C, double *B, double *mat, int *a, int *b, int *c, "
+                                          "int *add);", "void matvec_(double *C, double *B, double *mat, int *add, "
+                                          "int *a, int *b, long *c);", 1))
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(bad)],
+                       capture_output=True, text=True)
+    assert r.returncode != 0
+
+
+def test_shape_guards_reject_32bit_overflow_before_any_launch():
+    """Hole contractions index K = before*after and a*before*n in 32 bits;
+    mode products index matrices in 32 bits: shapes beyond that are QTG_ESHAPE
+    (checked on the host, before any device call)."""
+    L = Q.lib()
+    fake = 4096  # never dereferenced: the shape check comes first
+    big = 1 << 16
+    assert L.qtg_hole_contraction_z(fake, fake, fake, big, 2, 2, big // 2, 0, None) == 1   # before*after = 2^31
+    assert L.qtg_hole_contraction_z(fake, fake, fake, 1 << 20, 1 << 11, 2, 1, 0, None) == 1  # before*nA = 2^31
+    assert L.qtg_hole_contraction_z(fake, fake, fake, 1 << 20, 2, 1 << 11, 1, 0, None) == 1  # before*nB = 2^31
+    assert L.qtg_rhomat_z(fake, fake, fake, 2, big, big // 2, None) == 1
+    assert L.qtg_mode_product_z(fake, fake, fake, 1, 1 << 16, 1 << 15, 1, 0, 0, None) == 1  # nB*nC = 2^31
+    descs = np.zeros(1, Q.HC_DESC)
+    descs[0] = (0, 0, 0, big, 2, 2, big // 2, 0, 0)
+    p = C.c_void_p()
+    assert L.qtg_hole_contraction_plan_create(C.byref(p), descs.ctypes.data, 1, 0) == 1
+    # just inside the limit is accepted by the check (no launch: null pointer)
+    assert L.qtg_hole_contraction_z(None, fake, fake, big, 2, 2, big // 2 - 1, 0, None) == 3
+
+
+def test_small_matrix_shape_checks():
+    L = Q.lib()
+    fake = 4096
+    for fn in ("qtg_heev_batch_z", "qtg_inverse_batch_z"):
+        getattr(L, fn).argtypes = None
+    assert L.qtg_heev_batch_z(C.c_void_p(fake), C.c_void_p(fake), None, C.c_int64(65), C.c_int64(1), None) == 1
+    assert L.qtg_inverse_batch_z(C.c_void_p(fake), C.c_void_p(fake), C.c_int64(0), C.c_int64(1), None) == 1
+    L.qtg_qr_batch_z.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
+    assert L.qtg_qr_batch_z(fake, fake, 3, 4, 1, None) == 1  # m < n
+    L.qtg_reg_inverse_batch_z.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_void_p]
+    assert L.qtg_reg_inverse_batch_z(fake, fake, 4, 1, 0.0, None) == 3  # eps must be positive

@@ -30,25 +30,68 @@ def report(ref, got):
     return de, da
 
 
+def rho_weighted_diff(om, x, y):
+    """Largest LayerInterface::Error-style node difference
+    sqrt(sum conj(D) rho D / tr rho) over the non-top nodes
+    (src/TreeClasses/LayerInterface.cpp:6-31), rho of state x."""
+    om.build(x)
+    worst = 0.0
+    for k in range(om.nnodes - 1):
+        A, B = om.node_tensor(x, k), om.node_tensor(y, k)
+        n = A.shape[-1]
+        D = (A - B).reshape(-1, n, order="F")
+        rho = om.rho(k)
+        e = np.real(np.trace(D.conj().T @ D @ rho.T)) / np.real(np.trace(rho))
+        worst = max(worst, float(np.sqrt(abs(e))))
+    return worst
+
+
+def diffs(om, a, b):
+    top = om.nnodes - 1
+    return dict(de=float(np.max(np.abs(a["energies"] - b["energies"]) / np.abs(b["energies"]))),
+                da=float(np.max(np.abs(a["autocorr"] - b["autocorr"]))),
+                node=rho_weighted_diff(om, b["psi"], a["psi"]),
+                top=float(np.max(np.abs(om.node_tensor(a["psi"], top) - om.node_tensor(b["psi"], top)))))
+
+
 @pytest.mark.parametrize("start", ["hartree", "random"])
 def test_cmf8_matches_oracle(start):
+    """cfg1 at its full horizon: 8 coupled HO modes, dt = 0.1, outputs every
+    0.1 up to t = 10 (101 outputs).  Energies, autocorrelations, the
+    rho-weighted node differences and the top node are compared with the
+    oracle.  The tolerance is set by the oracle itself: the same oracle
+    source built at -O2 without fast-math (another summation order) drifts
+    from the -O3 -ffast-math build by a floor that a third correct FP64
+    implementation cannot beat.  From a random (full-rank) start that floor
+    is ~1e-14; from the Hartree start, whose unoccupied SPFs have density
+    eigenvalues ~0 that rho^-1 = 1/(l + 1e-10 e^{-l/1e-10}) amplifies by
+    ~1e10, it is ~2e-10 in the energy and ~3e-10 in the autocorrelation.
+    The GPU must stay within 3 times the measured floor (and never needs to
+    beat 1e-12)."""
     text, H = W.cmf8_model()
-    print(f"case cmf cmf8 {start}", flush=True)
+    print(f"case cmf cmf8 {start} t=10", flush=True)
     om = O.OracleModel(text, H)
     psi = om.fill_random(2024, delta_lowest=(start == "hartree"))
-    om, eng, ref, got = run_both(text, H, psi, 1.0, 0.1, 0.1)
-    de, da = report(ref, got)
-    assert len(got["times"]) == len(ref["times"]) == 11
+    ref = om.cmf(psi, 0.0, 10.0, 0.1, 0.1)
+    with O.strict_build():
+        om2 = O.OracleModel(text, H)
+        ref2 = om2.cmf(psi, 0.0, 10.0, 0.1, 0.1)
+        del om2
+    eng = Q.Engine(text, H)
+    got = Q.CMF(eng).integrate(psi, 0.0, 10.0, 0.1, 0.1)
+    report(ref, got)
+    assert len(got["times"]) == len(ref["times"]) == len(ref2["times"]) == 101
     assert np.allclose(got["times"], ref["times"], rtol=0, atol=1e-12)
     assert got["steps"] == ref["steps"] and got["rejected"] == ref["rejected"]
     assert got["derivs"] == ref["derivs"]
-    # A Hartree start leaves the unoccupied SPFs with rho eigenvalues ~0, so
-    # rho^-1 = 1/(lambda + 1e-10 e^{-lambda/1e-10}) multiplies their rounding
-    # noise by ~1e10 in both implementations; energies and autocorrelation
-    # then agree to ~1e-10 instead of ~1e-14 (random, full-rank starts), and
-    # how close depends on the summation order of each contraction.
-    assert de < (1e-9 if start == "hartree" else 1e-12)
-    assert da < (1e-9 if start == "hartree" else 1e-12)
+    floor = diffs(om, ref2, ref)
+    gpu = diffs(om, got, ref)
+    print(f"  floor (oracle -O2 vs -O3 -ffast-math) {floor}", flush=True)
+    print(f"  gpu vs oracle                          {gpu}", flush=True)
+    for k in ("de", "da", "node", "top"):
+        assert gpu[k] <= max(3.0 * floor[k], 1e-12), k
+    if start == "random":
+        assert floor["de"] < 1e-13 and gpu["de"] < 1e-12 and gpu["node"] < 1e-11
 
 
 def test_cmf_f16_random_short():

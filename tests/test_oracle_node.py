@@ -190,3 +190,65 @@ def test_project_out():
     assert np.max(np.abs(B.conj().T @ out)) < 1e-12 * np.max(np.abs(A))
     ref = A - B @ np.linalg.solve(B.conj().T @ B, B.conj().T @ A)
     assert np.max(np.abs(out - ref)) < 1e-12 * np.max(np.abs(A))
+
+
+@pytest.mark.parametrize("b,a,c", SHAPES)
+def test_real_matvec_bruteforce(b, a, c):
+    """rmatvec / trmatvec (LA_lib.f:314-379, 402-467) against numpy einsum on
+    real data, overwrite and accumulate."""
+    L = O.lib()
+    B = rng.normal(size=(b, a, c))
+    M = rng.normal(size=(a, a))
+    C0 = rng.normal(size=(b, a, c))
+    for fn, ref in ((L.orc_rmatvec, np.einsum("kl,blc->bkc", M, B)), (L.orc_trmatvec, np.einsum("lk,blc->bkc", M, B))):
+        for add in (0, 1):
+            out = C0.ravel(order="F").copy()
+            fn(out, np.ascontiguousarray(B.ravel(order="F")), np.ascontiguousarray(M.ravel(order="F")), a, b, c, add)
+            want = ref + (C0 if add else 0)
+            assert np.max(np.abs(out.reshape((b, a, c), order="F") - want)) < 1e-12 * max(1, np.max(np.abs(want)))
+
+
+def oracle_qr(A):
+    m, n = A.shape
+    Q = np.zeros(2 * m * n)
+    O.lib().orc_qr(O.cview(np.ascontiguousarray(A.ravel(order="F"))), m, n, Q)
+    return O.as_complex(Q).reshape(m, n, order="F")
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 1), (6, 6), (40, 7), (576, 24), (9, 8)])
+def test_householder_qr_matches_lapack(m, n):
+    """The oracle's thin Q equals LAPACK's (numpy.linalg.qr: zgeqrf + zungqr),
+    whose reflector convention Eigen's HouseholderQR shares, column for
+    column; QᴴQ = I and Qᴴ A is upper triangular with the diagonal sign
+    -sign(Re a) the convention fixes."""
+    A = crand(m, n)
+    Q = oracle_qr(A)
+    Qn, Rn = np.linalg.qr(A)
+    assert np.max(np.abs(Q - Qn)) < 1e-12
+    assert np.max(np.abs(Q.conj().T @ Q - np.eye(n))) < 1e-12
+    R = Q.conj().T @ A
+    assert np.max(np.abs(np.tril(R, -1))) < 1e-11 * np.max(np.abs(A))
+    # the first reflector's beta: -sign(Re a00) |A[:, 0]|
+    assert abs(R[0, 0] - (-np.sign(A[0, 0].real) * np.linalg.norm(A[:, 0]))) < 1e-11 * np.linalg.norm(A[:, 0])
+
+
+def test_householder_qr_special_cases():
+    # a column already zero below a real diagonal gets the identity reflector
+    # (tau = 0): Q keeps it and R keeps the sign of the diagonal
+    A = np.diag([2.0, -3.0, 0.5]).astype(np.complex128)
+    Q = oracle_qr(A)
+    assert np.max(np.abs(Q - np.eye(3))) < 1e-15
+    # a complex diagonal is not skipped (Eigen and LAPACK make R's diagonal real)
+    A = np.diag([1j, 2.0]).astype(np.complex128)
+    Q = oracle_qr(A)
+    R = Q.conj().T @ A
+    assert abs(R[0, 0].imag) < 1e-15 and R[0, 0].real < 0
+
+
+@pytest.mark.parametrize("n", [1, 3, 16, 48])
+def test_inverse_matches_numpy(n):
+    A = crand(n, n) + n * np.eye(n)
+    R = np.zeros(2 * n * n)
+    O.lib().orc_inverse(O.cview(np.ascontiguousarray(A.ravel(order="F"))), n, R)
+    X = O.as_complex(R).reshape(n, n, order="F")
+    assert np.max(np.abs(X - np.linalg.inv(A))) < 1e-12 * np.max(np.abs(np.linalg.inv(A)))
