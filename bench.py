@@ -43,25 +43,62 @@ def measured_traffic(kernel: str, dtype: str):
     return None
 
 
-def cpu_baseline(text, H, psi, gpu_energy, budget_s=12.0, max_sweeps=5):
-    """FP64 CPU restatement (oracle), single thread, timed on a bounded number
-    of sweeps of the same workload."""
+def host_cpu():
+    """Core count and model of the host CPU (os.cpu_count, /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"nproc": avail, "cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(text, H, psi, gpu_energy, repeats=5):
+    """FP64 CPU restatement of the reference path (the oracle, -O3
+    -ffast-math like the reference), timed on this host: one warm-up sweep,
+    then `repeats` sweeps, each timed alone with a steady clock; the median.
+    Single thread (the reference's default, OpenMP off), then the OpenMP
+    build of the same source over the independent nodes and SOP terms of a
+    sweep on every core this process may use."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import statistics
+
     import oracle_lib as O
 
-    om = O.OracleModel(text, H)
-    om.derivative(psi)  # warm-up: page in, build plans
-    n, t0 = 0, time.perf_counter()
-    while n < max_sweeps:
-        om.derivative(psi)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    e_ref = om.energy(psi)
-    return {"value": n / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full sweeps of the same 64-mode workload (FP64 oracle, -O3 -ffast-math, one thread)",
-            }, e_ref
+    def median_rate(variant):
+        def run():
+            om = O.OracleModel(text, H)
+            om.derivative(psi)  # warm-up: page in, build plans
+            times = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                om.derivative(psi)
+                times.append(time.perf_counter() - t0)
+            threads = om._L.orc_threads()
+            return 1.0 / statistics.median(times), times, threads, om.energy(psi)
+        if variant is None:
+            return run()
+        with O.build_variant(variant):
+            return run()
+
+    one, t1, th1, e_ref = median_rate(None)
+    many, tn, thn, _ = median_rate("omp")
+    cpu = host_cpu()
+    return {"value": one, "unit": "sweeps/s", "cores": th1, "kind": "port",
+            "sample": f"{repeats} full sweeps of the same 64-mode workload after one warm-up, median "
+                      "(FP64 oracle, -O3 -ffast-math, one thread)",
+            "sweep_seconds": [round(x, 4) for x in t1],
+            "all_cores": {"value": many, "unit": "sweeps/s", "cores": thn, "sweep_seconds": [round(x, 4) for x in tn],
+                          "note": "same oracle built with OpenMP over nodes and SOP terms"},
+            "host": cpu}, e_ref
 
 
 def time_form(Q, text, H, psi, precision, compress, graph, steps, path=1):
@@ -82,13 +119,52 @@ def time_form(Q, text, H, psi, precision, compress, graph, steps, path=1):
             "executed_flops": flops}
 
 
+def cmf128_leg(Q, W, steps):
+    """The precision run of the 128-mode tree (cfg5's workload): CMF at
+    dt = 0.05 with rho regularised at eps = 1e-8 from a random state, in
+    complex128 and in the split form (complex64 bulk contractions; FP64 for
+    the large terms, the densities, rho^-1, the projector and step control),
+    the same number of CMF iterations each: iterations per second and the
+    split form's energy and autocorrelation error against complex128 at the
+    common output times."""
+    import numpy as np
+    text, H = W.cmf128_model()
+    res = {"workload": f"128-mode tree, 128 + 8128 coupled-quartic terms, CMF dt = 0.05, outputs every 0.05, "
+                       f"eps 1e-8, {steps} iterations from fill_random(128)"}
+    ref = None
+    for prec, name in ((0, "complex128"), (2, "complex64_split")):
+        eng = Q.Engine(text, H, precision=prec, rho_eps=1e-8, graph=False)
+        psi = eng.fill_random(128, delta_lowest=False)
+        cmf = Q.CMF(eng)
+        cmf.integrate(psi, 0.0, 1000.0, 0.05, 0.05, max_steps=1)  # warm-up
+        Q.synchronize()
+        t0 = time.perf_counter()
+        r = cmf.integrate(psi, 0.0, 1000.0, 0.05, 0.05, max_steps=steps)
+        Q.synchronize()
+        dt = time.perf_counter() - t0
+        out = {"iterations_per_s": round(r["steps"] / dt, 3), "iterations": r["steps"], "rejected": r["rejected"],
+               "node_derivatives": r["derivs"], "seconds": round(dt, 3),
+               "t_reached": float(r["times"][-1]) if len(r["times"]) else None}
+        if ref is None:
+            ref = r
+        else:
+            n = min(len(ref["times"]), len(r["times"]))
+            if n:
+                out["energy_rel_err_vs_complex128"] = float(np.max(np.abs(r["energies"][:n] - ref["energies"][:n])
+                                                                  / np.abs(ref["energies"][:n])))
+                out["autocorr_err_vs_complex128"] = float(np.max(np.abs(r["autocorr"][:n] - ref["autocorr"][:n])))
+            out["speedup_vs_complex128"] = round(out["iterations_per_s"] / res["complex128"]["iterations_per_s"], 3)
+        res[name] = out
+        del cmf, eng
+    return res
+
+
 def gradient_leg(Q, W, gd_steps):
     """Energy gradient dE/dA* on the 32-mode anharmonic model (8 DVR, SPF 8
-    then 24): gradient evaluations per second (sweep + mean-field pass +
-    gradient assembly + copy to the host) and plain gradient descent from a
-    random state with QR re-orthonormalisation on the host."""
-    from qutree_amd.gauge import gradient_descent
-
+    then 24): gradient evaluations per second (forward sweep + mean-field
+    pass + gradient assembly + copy to the host), then plain gradient
+    descent from a random state with the QR gauge after every step, all on
+    the device (qtg_engine_gradient_descent)."""
     text, H = W.grad32_model()
     eng = Q.Engine(text, H)
     psi = eng.fill_random(32, delta_lowest=False)
@@ -99,12 +175,16 @@ def gradient_leg(Q, W, gd_steps):
     for _ in range(n):
         eng.gradient()
     t1 = time.perf_counter()
+    eng.gradient_descent(psi, 1, 0.05)  # warm-up
     t2 = time.perf_counter()
-    _, energies = gradient_descent(eng, psi, gd_steps, 0.05)
+    _, energies = eng.gradient_descent(psi, gd_steps, 0.05)
     t3 = time.perf_counter()
+    energies = [float(x) for x in energies]
     return {"workload": "32-mode quartic model, 8 DVR, SPF 8 then 24 (24x24x24 nodes), 32 + 496 terms",
             "gradient_evals_per_s": round(n / (t1 - t0), 3), "ms_per_gradient": round((t1 - t0) / n * 1e3, 3),
             "gd_steps": gd_steps, "gd_eta": 0.05, "gd_seconds": round(t3 - t2, 3),
+            "gd_ms_per_step": round((t3 - t2) / max(gd_steps, 1) * 1e3, 3),
+            "gd": "device: forward, backward, step and QR gauge per step, energies copied once",
             "energy_start": energies[0], "energy_end": energies[-1],
             "monotone": bool(all(b < a for a, b in zip(energies, energies[1:])))}
 
@@ -155,12 +235,13 @@ def cpu_eigenstates(psi):
 
 def big_tree_leg(Q, W, steps=3):
     """One H|Psi> sweep on the 128-mode tree (8 DVR, SPF 8..48 toward the root,
-    128 + 8128 coupled-quartic terms) in complex128 and complex64 bulk."""
+    128 + 8128 coupled-quartic terms) in complex128, complex64 bulk and the
+    split form."""
     text, H = W.cmf128_model()
     res = {"workload": "128-mode binary tree, 8 DVR, SPF 8,14,21,27,34,41,48, 128 + 8128 terms"}
     e64 = None
     psi = None
-    for prec, name in ((0, "complex128"), (1, "complex64_bulk")):
+    for prec, name in ((0, "complex128"), (1, "complex64_bulk"), (2, "complex64_split")):
         eng = Q.Engine(text, H, precision=prec)
         if psi is None:
             psi = eng.fill_random(128, delta_lowest=False)
@@ -241,12 +322,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", choices=["complex128", "complex64"], default="complex128")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cmf-steps", type=int, default=6, help="CMF steps timed on the 64-mode tree (0: skip)")
+    ap.add_argument("--cmf-steps", type=int, default=60,
+                    help="CMF iterations (Integrate loop passes, accepted or rejected) timed on the 64-mode tree "
+                         "(0: skip)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip timing the plain per-term complex128 form and the complex64 bulk form")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the gradient, 128-mode and node-microbench legs")
     ap.add_argument("--gd-steps", type=int, default=200, help="gradient-descent steps of the gradient leg")
+    ap.add_argument("--cmf128-steps", type=int, default=20,
+                    help="CMF iterations of the 128-mode precision leg (0: skip; the full run is 1000)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -398,15 +483,20 @@ def main() -> int:
             engc = Q.Engine(text_c, H_c, precision=precision, compress=compress, graph=False, path=path)
             psi_c = engc.fill_random(64, delta_lowest=True)
             cmf = Q.CMF(engc)
-            cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=1)  # warm-up
+            cmf.integrate(psi_c, 0.0, 10.0, 0.1, 0.1, max_steps=1)  # warm-up
             Q.synchronize()
             c0 = time.perf_counter()
-            r = cmf.integrate(psi_c, 0.0, 10.0, 0.1, 10.0, max_steps=args.cmf_steps)
+            r = cmf.integrate(psi_c, 0.0, 10.0, 0.1, 0.1, max_steps=args.cmf_steps)
             Q.synchronize()
             c1 = time.perf_counter()
-            return {"steps_per_s": round(r["steps"] / (c1 - c0), 3), "steps": r["steps"],
+            acc = r["steps"] - r["rejected"]
+            return {"steps_per_s": round(r["steps"] / (c1 - c0), 3),
+                    "iterations_per_s": round(r["steps"] / (c1 - c0), 3),
+                    "accepted_steps_per_s": round(acc / (c1 - c0), 3), "iterations": r["steps"],
+                    "steps": r["steps"], "accepted": acc,
                     "rejected": r["rejected"], "builds": r["builds"], "node_derivatives": r["derivs"],
-                    "ticks": r["ticks"], "ms_per_step": round((c1 - c0) / r["steps"] * 1e3, 3),
+                    "ticks": r["ticks"], "ms_per_iteration": round((c1 - c0) / r["steps"] * 1e3, 3),
+                    "seconds": round(c1 - c0, 4), "t_reached": float(r["times"][-1]) if len(r["times"]) else None,
                     "energy_final": float(r["energies"][-1]) if len(r["energies"]) else None}, r
 
         main_cmf, r64 = cmf_form(0, 1, True)
@@ -426,6 +516,8 @@ def main() -> int:
     if not args.no_extras and args.precision == "complex128":
         out["gradient"] = gradient_leg(Q, W, args.gd_steps)
         out["tree128"] = big_tree_leg(Q, W)
+        if args.cmf128_steps > 0:
+            out["cmf128"] = cmf128_leg(Q, W, args.cmf128_steps)
         out["node_microbench"] = microbench_leg(Q, W)
         out["eigenstates"], eig_psi = eigenstates_leg(Q, W)
     if not args.no_cpu:

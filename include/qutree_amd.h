@@ -221,6 +221,29 @@ int qtg_engine_counts(qtg_engine engine, double* sweep_flops, int64_t* items);
  * interleaved) and energy may be null.  One state at the top node
  * (QTG_ESHAPE otherwise). */
 int qtg_engine_gradient(qtg_engine engine, double* grad_host, double* energy, qtg_stream stream);
+/* The energy functional as a forward / backward pair (complex128 engine,
+ * grouped path, one state at the top; no reference counterpart).
+ * qtg_engine_forward: the sweep at phase 1 on the state set by
+ * qtg_engine_set_psi, leaving E and 1/<Psi|Psi> on the device; energy (may be
+ * null: then the call does not wait) receives E.  qtg_engine_backward:
+ * dE/dA* at every node into grad_host (engine layout; may be null).  With
+ * option "grad_recompute" 0 (default) backward uses the matrices of the last
+ * forward, which must have run on the same state; with 1 it recomputes them
+ * from the current state first, so nothing of the forward pass has to be
+ * kept in between (other sweeps may run).  qtg_engine_gradient = forward +
+ * backward.  qtg_engine_canonicalize: the QR gauge on the device: every
+ * non-top node orthonormalised by Householder QR, deepest layer first, its R
+ * pushed into the parent on the node's leg (the represented state is
+ * unchanged); psi_host (may be null) receives the state.
+ * qtg_engine_gradient_descent: `steps` steps A <- A - eta dE/dA* at every
+ * node, each followed by the QR gauge, all on the device; energies (steps + 1
+ * doubles, may be null) receives E before every step and after the last,
+ * psi_host (may be null) the final state. */
+int qtg_engine_forward(qtg_engine engine, double* energy, qtg_stream stream);
+int qtg_engine_backward(qtg_engine engine, double* grad_host, qtg_stream stream);
+int qtg_engine_canonicalize(qtg_engine engine, double* psi_host, qtg_stream stream);
+int qtg_engine_gradient_descent(qtg_engine engine, int64_t steps, double eta, double* energies, double* psi_host,
+                                qtg_stream stream);
 /* FLOPs one sweep of this engine's plan executes (fewer than the reference
  * count where equal slots are shared and Apply summands merged). */
 int qtg_engine_plan_flops(qtg_engine engine, double* executed);
@@ -232,6 +255,12 @@ int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* co
                          int64_t* items);
 int64_t qtg_host_psi_size(const char* tree_text);
 int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest, double* psi_host);
+/* The random state drawn from mt19937(seed) after `discard` outputs, for a
+ * generator that earlier states already drew from (the mctdh runner's
+ * shared rng_, mctdh.cpp:164-166,272-275); returns the number of generator
+ * outputs it drew, or a negative qtg_status. */
+int64_t qtg_host_fill_random_stream(const char* tree_text, uint64_t seed, int64_t discard, int delta_lowest,
+                                    double* psi_host);
 /* Kernel timing: while on, sweeps run eagerly behind a short device-side
  * hold (so the host queues the whole sweep first) and HIP events time every
  * launch on its stream; the grouped kernels carry the events in their own
@@ -263,6 +292,10 @@ int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, dou
  * tiles than split_waves splits its reductions along K into chunks of at
  * least split_min_k until it holds about split_waves wavefronts. */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
+/* The tuning-table line (qutree_amd/tuning/gfx950.txt, written by
+ * qutree_amd/autotune.py) applied to this engine's plan, as text into buf
+ * (cap bytes, NUL-terminated; empty when none matched). */
+int qtg_engine_tuning(qtg_engine engine, char* buf, int64_t cap);
 
 /* ----------------------------------------------------------- CMF integrator */
 /* CMFIntegrator(H, tree, phase).Integrate(ivars) (src/Applications/
@@ -280,6 +313,14 @@ int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
  * qtg_status. */
 typedef void* qtg_cmf;
 int qtg_cmf_create(qtg_engine engine, qtg_cmf* cmf);
+/* Psi.write(job.ofname, job.append_) of CMFIntegrator::Integrate
+ * (CMFIntegrator.cpp:46,153): while set, qtg_cmf_integrate writes the state
+ * as a TensorTree (the format of qtg_tensor_tree_write) at the start and at
+ * every output time.  append != 0 appends (creating the file); append == 0
+ * opens an existing file in place and overwrites it from its start without
+ * truncating, and writes nothing when the file is missing
+ * (TensorTree_Implementation.h:93-100, ios::in).  filename NULL: off. */
+int qtg_cmf_set_output_file(qtg_cmf cmf, const char* filename, int append);
 int qtg_cmf_destroy(qtg_cmf cmf);
 int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend, double dt, double out,
                           double acc_cmf, double acc_bs, double phase_re, double phase_im,

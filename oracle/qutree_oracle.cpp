@@ -22,6 +22,9 @@
 //     exact quadratic-Hamiltonian dynamics bound; the exact reference step
 //     sequence is parity unpinned (the reference cannot be built here).
 #include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cassert>
 #include <cmath>
 #include <complex>
@@ -1088,9 +1091,15 @@ HamRep makeHamRep(const SOP& H, const Tree& t) {
 void build(HamRep& r, const SOP& H, const TensorTree& P, const Tree& t, double eps) {
   r.rho = rhoTree(P, t);
   r.rhoInv.assign(t.nodes.size(), Mat());
-  for (auto& n : t.nodes) r.rhoInv[n.address] = regularizedInverse(r.rho[n.address], eps);
-  for (size_t l = 0; l < H.terms.size(); ++l) represent(r.hMats[l], H.terms[l], P, t);
-  for (size_t l = 0; l < H.terms.size(); ++l) holeContraction(r.hCon[l], r.hMats[l], P, t);
+  // The loops below are independent per node / per term.  The default build
+  // runs them on one thread like the reference; the all-core CPU baseline
+  // build (-fopenmp) spreads them over threads, with the same arithmetic.
+#pragma omp parallel for schedule(dynamic)
+  for (long a = 0; a < (long)t.nodes.size(); ++a) r.rhoInv[a] = regularizedInverse(r.rho[a], eps);
+#pragma omp parallel for schedule(dynamic)
+  for (long l = 0; l < (long)H.terms.size(); ++l) represent(r.hMats[l], H.terms[l], P, t);
+#pragma omp parallel for schedule(dynamic)
+  for (long l = 0; l < (long)H.terms.size(); ++l) holeContraction(r.hCon[l], r.hMats[l], P, t);
 }
 
 // Apply (HamiltonianRepresentation.cpp:6-43)
@@ -1943,6 +1952,15 @@ int orc_get_matrix(void* h, int which, long l, long addr, double* out) {
 
 // Derivative (HamiltonianRepresentation.cpp:74-81) for the whole tree:
 // build + LayerDerivative at every node.
+// Threads the build and the per-node loop use (1 unless built with OpenMP).
+int orc_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 int orc_derivative(void* h, const double* psi, double* dpsi, double phase_re, double phase_im,
                    double eps) {
   Model* m = (Model*)h;
@@ -1951,8 +1969,11 @@ int orc_derivative(void* h, const double* psi, double* dpsi, double phase_re, do
     build(m->rep, m->H, P, m->tree, eps);
     m->built = true;
     TensorTree D = makeTensorTree(m->tree);
-    for (auto& n : m->tree.nodes)
+#pragma omp parallel for schedule(dynamic)
+    for (long a = 0; a < (long)m->tree.nodes.size(); ++a) {
+      const Node& n = m->tree.nodes[a];
       D[n.address] = LayerDerivative(m->H, P[n.address], m->rep, m->tree, n, cd(phase_re, phase_im));
+    }
     m->pack(D, dpsi);
     return 0;
   } catch (std::exception& e) {

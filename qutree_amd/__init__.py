@@ -76,6 +76,12 @@ def lib():
     L.qtg_engine_get_matrix.argtypes = [_vp, C.c_int, _i64, _i64, _dp, _vp]
     L.qtg_engine_counts.argtypes = [_vp, C.POINTER(C.c_double), _lp]
     L.qtg_engine_plan_flops.argtypes = [_vp, C.POINTER(C.c_double)]
+    L.qtg_engine_forward.argtypes = [_vp, C.POINTER(C.c_double), _vp]
+    L.qtg_engine_backward.argtypes = [_vp, _vp, _vp]
+    L.qtg_engine_canonicalize.argtypes = [_vp, _vp, _vp]
+    L.qtg_engine_gradient_descent.argtypes = [_vp, _i64, C.c_double, _vp, _vp, _vp]
+    L.qtg_rmatvec_d.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, C.c_int, _vp]
+    L.qtg_trmatvec_d.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, C.c_int, _vp]
     L.qtg_engine_gradient.argtypes = [_vp, _dp, C.POINTER(C.c_double), _vp]
     L.qtg_ttno_create.argtypes = [_vp, _lp, _dp, C.POINTER(_vp)]
     L.qtg_ttno_destroy.argtypes = [_vp]
@@ -309,6 +315,35 @@ class Engine:
         e = C.c_double()
         check(lib().qtg_engine_gradient(self.ptr, out, C.byref(e), None))
         return out.view(np.complex128).copy(), e.value
+
+    def forward(self) -> float:
+        """The sweep at phase 1 on the current state; returns E (the
+        matrices stay on the device for backward())."""
+        e = C.c_double()
+        check(lib().qtg_engine_forward(self.ptr, C.byref(e), None))
+        return e.value
+
+    def backward(self) -> np.ndarray:
+        """dE/dA* at every node (after forward(), or with option
+        grad_recompute=1 from the current state)."""
+        out = np.zeros(2 * self.psi_size)
+        check(lib().qtg_engine_backward(self.ptr, out.ctypes.data, None))
+        return out.view(np.complex128).copy()
+
+    def canonicalize(self) -> np.ndarray:
+        """QR gauge of the current state on the device; returns it."""
+        out = np.zeros(2 * self.psi_size)
+        check(lib().qtg_engine_canonicalize(self.ptr, out.ctypes.data, None))
+        return out.view(np.complex128).copy()
+
+    def gradient_descent(self, psi, steps: int, eta: float):
+        """Device gradient descent from psi: (final state, energies before
+        every step and after the last)."""
+        self.set_psi(psi)
+        en = np.zeros(steps + 1)
+        out = np.zeros(2 * self.psi_size)
+        check(lib().qtg_engine_gradient_descent(self.ptr, steps, eta, en.ctypes.data, out.ctypes.data, None))
+        return out.view(np.complex128).copy(), en
 
     def executed_flops(self) -> float:
         """FLOPs one sweep of this engine's plan executes."""

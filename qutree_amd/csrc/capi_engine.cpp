@@ -1,4 +1,5 @@
 // C ABI of the tree engine (declared in include/qutree_amd.h).
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -137,6 +138,51 @@ int qtg_engine_gradient(qtg_engine e, double* grad_host, double* energy, qtg_str
   return en->gradient(ES(e, s), grad_host, energy);
 }
 
+int qtg_engine_tuning(qtg_engine e, char* buf, int64_t cap) {
+  if (!e) return QTG_EARG;
+  const std::string& t = static_cast<Engine*>(e)->tuned_from();
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>((size_t)cap - 1, t.size());
+    std::memcpy(buf, t.data(), n);
+    buf[n] = 0;
+  }
+  return QTG_OK;
+}
+
+int qtg_engine_forward(qtg_engine e, double* energy, qtg_stream s) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->forward(ES(e, s), energy);
+}
+
+int qtg_engine_backward(qtg_engine e, double* grad_host, qtg_stream s) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->backward(ES(e, s), grad_host);
+}
+
+int qtg_engine_gradient_descent(qtg_engine e, int64_t steps, double eta, double* energies, double* psi_host,
+                                qtg_stream s) {
+  if (!e) return QTG_EARG;
+  return static_cast<Engine*>(e)->gradient_descent(ES(e, s), steps, eta, energies, psi_host);
+}
+
+int qtg_engine_canonicalize(qtg_engine e, double* psi_host, qtg_stream s) {
+  if (!e) return QTG_EARG;
+  auto* en = static_cast<Engine*>(e);
+  if (!en->planned() || en->complex64()) {
+    set_error("canonicalize: needs a complex128 engine with a Hamiltonian");
+    return QTG_ESTATE;
+  }
+  hipStream_t st = ES(e, s);
+  int rc = en->canonicalize(st);
+  if (rc) return rc;
+  if (psi_host) {
+    QTG_HIP_TRY(hipMemcpyAsync(psi_host, en->psi64(), sizeof(double) * 2 * en->tree().total, hipMemcpyDeviceToHost,
+                               st));
+    QTG_HIP_TRY(hipStreamSynchronize(st));
+  }
+  return QTG_OK;
+}
+
 int qtg_engine_plan_flops(qtg_engine e, double* executed) {
   if (!e || !executed) return QTG_EARG;
   *executed = static_cast<Engine*>(e)->sweep_flops();
@@ -152,7 +198,9 @@ int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* co
   int rc = en.create(tree_text, 0, true);
   if (rc) return rc;
   en.compress_ = compress != 0;
-  en.path_ = 0;  // counts only: the work-item plan needs no device structures
+  // counts only: the work-item plan needs no device structures; with
+  // QTG_DUMP_PLAN set the grouped plan is made too and printed
+  en.path_ = std::getenv("QTG_DUMP_PLAN") ? 1 : 0;
   rc = en.set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool);
   if (rc) return rc;
   return qtg_engine_counts(&en, sweep_flops, items);
@@ -176,6 +224,18 @@ int qtg_host_fill_random(const char* tree_text, uint64_t seed, int delta_lowest,
   return QTG_OK;
 }
 
+int64_t qtg_host_fill_random_stream(const char* tree_text, uint64_t seed, int64_t discard, int delta_lowest,
+                                    double* psi_host) {
+  if (!tree_text || !psi_host || discard < 0) return -QTG_EARG;
+  Engine en;
+  int rc = en.create(tree_text, 0, true);
+  if (rc) return -rc;
+  std::vector<cd> psi;
+  const int64_t used = fill_random_stream(en.tree(), seed, discard, delta_lowest != 0, psi);
+  std::memcpy(psi_host, psi.data(), sizeof(cd) * psi.size());
+  return used;
+}
+
 int qtg_engine_synchronize(qtg_engine e) {
   if (!e) return QTG_EARG;
   return static_cast<Engine*>(e)->synchronize();
@@ -197,7 +257,8 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   // keys that shape the plan (buffer sizes, launch lists) are fixed by
   // set_hamiltonian: changing them afterwards would run kernels on buffers
   // sized for another plan
-  static const char* const planKeys[] = {"compress", "path", "split_waves", "split_min_k"};
+  static const char* const planKeys[] = {"compress",  "path",       "split_waves", "split_min_k",
+                                         "gc_long_k", "gm_split_k", "tuning"};
   for (const char* k : planKeys)
     if (!std::strcmp(key, k) && en->planned()) {
       set_error(std::string("option ") + key + ": set it before qtg_engine_set_hamiltonian");
@@ -231,6 +292,7 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
       return QTG_EARG;
     }
     en->splitWaves_ = (int64_t)value;
+    en->userSet_ |= 1;
     return QTG_OK;
   }
   if (!std::strcmp(key, "split_min_k")) {
@@ -239,6 +301,33 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
       return QTG_EARG;
     }
     en->splitMinK_ = (int)value;
+    en->userSet_ |= 2;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "gc_long_k")) {
+    if (value < 16.0) {
+      set_error("option gc_long_k: at least 16");
+      return QTG_EARG;
+    }
+    en->gcLongK_ = (int)value;
+    en->userSet_ |= 4;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "gm_split_k")) {
+    if (value < 16.0) {
+      set_error("option gm_split_k: at least 16");
+      return QTG_EARG;
+    }
+    en->gmSplitK_ = (int)value;
+    en->userSet_ |= 8;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "tuning")) {
+    en->tuning_ = value != 0.0;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "grad_recompute")) {
+    en->gradRecompute_ = value != 0.0;
     return QTG_OK;
   }
   if (!std::strcmp(key, "graph")) {
@@ -260,6 +349,14 @@ int qtg_cmf_create(qtg_engine e, qtg_cmf* out) {
     return rc;
   }
   *out = c;
+  return QTG_OK;
+}
+
+int qtg_cmf_set_output_file(qtg_cmf c, const char* filename, int append) {
+  if (!c) return QTG_EARG;
+  auto* cm = static_cast<Cmf*>(c);
+  cm->outFile_ = filename ? filename : "";
+  cm->outAppend_ = append != 0;
   return QTG_OK;
 }
 

@@ -10,9 +10,12 @@
 // extrapolation, error norm).  A tick is four launches (program, Apply,
 // finish, program) and one read of the per-node errors.
 #include "cmf.hpp"
+#include "jacobi.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstdio>
 #include <cstring>
 
 namespace qtg {
@@ -116,6 +119,84 @@ int Cmf::init() {
   if ((rc = dalloc((void**)&d_sa_, sizeof(SwApply) * std::max<size_t>(sa.size(), 1)))) return rc;
   if (!oi.empty()) QTG_HIP_TRY(hipMemcpy(d_oi_, oi.data(), sizeof(OrthoItem) * oi.size(), hipMemcpyHostToDevice));
   if (!sa.empty()) QTG_HIP_TRY(hipMemcpy(d_sa_, sa.data(), sizeof(SwApply) * sa.size(), hipMemcpyHostToDevice));
+  // Node tensors beyond the LDS budget of ortho_kernel: the global-memory
+  // kernel, and the square roots pushed into the parents by grouped mode
+  // products.  Per height, a parent takes the matrices of its children of
+  // that height leg by leg (children in address order), chained through two
+  // workspace regions and ending in the state; a parent with one such child
+  // ends with a copy.
+  // (QTG_ORTHO_GLOBAL=1 forces this path on any tree, which lets its parity
+  // be checked where the LDS kernel also runs)
+  const char* force = std::getenv("QTG_ORTHO_GLOBAL");
+  bigOrtho_ = (force && force[0] == '1') ||
+              (size_t)(E.nmax_ + 2 * jacobi_elems(E.nspf_)) * sizeof(cplx<double>) > kLdsBudget;
+  swLaunch_.clear();
+  if (bigOrtho_) {
+    std::vector<GmDesc> all;
+    int64_t wmax = 0;
+    for (int h = 0; h < t.maxHeight; ++h) {
+      std::vector<std::vector<GmDesc>> toWork, toState;
+      int64_t w = 0;
+      for (auto& p : t.nodes) {
+        if (p.bottom) continue;
+        std::vector<int> kids;
+        for (int c : p.children)
+          if (t.nodes[c].height == h) kids.push_back(c);
+        if (kids.empty()) continue;
+        std::sort(kids.begin(), kids.end());
+        const int levels = std::max<int>((int)kids.size(), 2);
+        if ((int)toWork.size() < levels) toWork.resize(levels), toState.resize(levels);
+        const int64_t wa = w, wb = w + p.N;
+        w += 2 * p.N;
+        int64_t cur = p.off;
+        int cursel = 0;
+        for (int c = 0; c < levels; ++c) {
+          const bool last = (c == levels - 1);
+          const bool copy = c >= (int)kids.size();
+          const int leg = copy ? p.order() - 1 : t.nodes[kids[c]].childIdx;
+          GmDesc g{};
+          int64_t bef = 1, aft = 1;
+          for (int i = 0; i < leg; ++i) bef *= p.dims[i];
+          for (int i = leg + 1; i < p.order(); ++i) aft *= p.dims[i];
+          g.x = cur;
+          g.xsel = cursel;
+          g.m = copy ? 0 : swslot[kids[c]];
+          g.out = last ? p.off : ((c % 2 == 0) ? wa : wb);
+          g.bef = (int32_t)bef;
+          g.nin = g.nout = p.dims[leg];
+          g.aft = (int32_t)aft;
+          g.op = copy ? GM_COPY : GM_N;
+          g.are = 1.0;
+          g.node = -1;
+          g.ksplit = 1;
+          (last ? toState : toWork)[c].push_back(g);
+          cur = g.out;
+          cursel = 1;
+        }
+      }
+      wmax = std::max(wmax, w);
+      swLaunch_.emplace_back();
+      for (size_t c = 0; c < toWork.size(); ++c)
+        for (int which = 0; which < 2; ++which) {
+          auto& L = which ? toState[c] : toWork[c];
+          if (L.empty()) continue;
+          SwLaunch sl{(int64_t)all.size(), 0, which == 1};
+          int64_t tasks = 0;
+          for (auto& g : L) {
+            g.tile0 = tasks;
+            const int64_t nt = gm_tasks(g);
+            tasks += nt;
+            all.insert(all.end(), (size_t)nt, g);
+          }
+          sl.ntasks = tasks;
+          swLaunch_.back().push_back(sl);
+        }
+    }
+    if ((rc = dalloc((void**)&d_swgm_, sizeof(GmDesc) * std::max<size_t>(all.size(), 1)))) return rc;
+    if (!all.empty()) QTG_HIP_TRY(hipMemcpy(d_swgm_, all.data(), sizeof(GmDesc) * all.size(), hipMemcpyHostToDevice));
+    swWorkSize_ = std::max<int64_t>(wmax, 1);
+    if ((rc = dalloc((void**)&swWork_, sizeof(cplx<double>) * swWorkSize_))) return rc;
+  }
   // overlap tree (TreeFunctions::dotProduct, MatrixTreeFunctions_Implementation.h:16-39)
   int64_t soff = 0;
   sslot_.assign(NN_, -1);
@@ -150,6 +231,92 @@ int Cmf::init() {
   }
   if ((rc = dalloc((void**)&d_ov_, sizeof(SwItem) * ov.size()))) return rc;
   QTG_HIP_TRY(hipMemcpy(d_ov_, ov.data(), sizeof(SwItem) * ov.size(), hipMemcpyHostToDevice));
+  // Overlap tree for node tensors beyond the LDS-resident sandwich kernel:
+  // per height, X_v = (S of the children on their legs) ket_v by mode
+  // products chained through the workspace (a bottom node copies its ket),
+  // then S_v = bra_v^H X_v over all legs but the last.
+  ovLaunch_.clear();
+  if (bigOrtho_) {
+    std::vector<GmDesc> gall;
+    std::vector<GcDesc> call;
+    int64_t wmax = 0;
+    for (int h = 0; h <= t.maxHeight; ++h) {
+      std::vector<std::vector<GmDesc>> lev;
+      std::vector<GcDesc> con;
+      int64_t w = 0;
+      for (auto& v : t.nodes) {
+        if (v.height != h) continue;
+        std::vector<int> kids = v.children;
+        std::sort(kids.begin(), kids.end());
+        const int levels = std::max<int>((int)kids.size(), 1);
+        if ((int)lev.size() < levels) lev.resize(levels);
+        const int64_t wa = w, wb = w + v.N;
+        w += 2 * v.N;
+        int64_t cur = v.off;
+        int cursel = 0;
+        for (int c = 0; c < levels; ++c) {
+          const bool copy = kids.empty();
+          const int leg = copy ? v.order() - 1 : t.nodes[kids[c]].childIdx;
+          GmDesc g{};
+          int64_t bef = 1, aft = 1;
+          for (int i = 0; i < leg; ++i) bef *= v.dims[i];
+          for (int i = leg + 1; i < v.order(); ++i) aft *= v.dims[i];
+          g.x = cur;
+          g.xsel = cursel;
+          g.m = copy ? 0 : sslot_[kids[c]];
+          g.out = (c % 2 == 0) ? wa : wb;
+          g.bef = (int32_t)bef;
+          g.nin = g.nout = v.dims[leg];
+          g.aft = (int32_t)aft;
+          g.op = copy ? GM_COPY : GM_N;
+          g.are = 1.0;
+          g.node = -1;
+          g.ksplit = 1;
+          lev[c].push_back(g);
+          cur = g.out;
+          cursel = 1;
+        }
+        GcDesc c{};
+        c.x = v.off;   // bra (first base)
+        c.y = cur;     // X (workspace, second base)
+        c.out = sslot_[v.addr];
+        c.bef = (int32_t)(v.N / v.n);
+        c.aft = 1;
+        c.nx = c.ny = v.n;
+        c.node = -1;
+        c.sel = 2;
+        const int64_t K = v.N / v.n;
+        c.ksplit = K > 1024 ? split_chunks(K, kGemmWaves, 1) : 1;
+        con.push_back(c);
+      }
+      wmax = std::max(wmax, w);
+      ovLaunch_.emplace_back();
+      for (auto& L : lev) {
+        if (L.empty()) continue;
+        OvLaunch ol{0, (int64_t)gall.size(), 0};
+        for (auto& g : L) {
+          g.tile0 = ol.ntasks;
+          const int64_t nt = gm_tasks(g);
+          ol.ntasks += nt;
+          gall.insert(gall.end(), (size_t)nt, g);
+        }
+        ovLaunch_.back().push_back(ol);
+      }
+      if (!con.empty()) {
+        OvLaunch ol{1, (int64_t)call.size(), 0};
+        ol.ntasks = order_tasks(con, gc_tasks);
+        for (auto& c : con) call.insert(call.end(), (size_t)gc_tasks(c), c);
+        ovLaunch_.back().push_back(ol);
+      }
+    }
+    if ((rc = dalloc((void**)&d_ovgm_, sizeof(GmDesc) * std::max<size_t>(gall.size(), 1)))) return rc;
+    if ((rc = dalloc((void**)&d_ovgc_, sizeof(GcDesc) * std::max<size_t>(call.size(), 1)))) return rc;
+    if (!gall.empty()) QTG_HIP_TRY(hipMemcpy(d_ovgm_, gall.data(), sizeof(GmDesc) * gall.size(), hipMemcpyHostToDevice));
+    if (!call.empty()) QTG_HIP_TRY(hipMemcpy(d_ovgc_, call.data(), sizeof(GcDesc) * call.size(), hipMemcpyHostToDevice));
+    if (wmax > swWorkSize_) {
+      if ((rc = dalloc((void**)&ovWork_, sizeof(cplx<double>) * wmax))) return rc;
+    }
+  }
   if ((rc = dalloc((void**)&energy_, sizeof(double) * 4))) return rc;
   return QTG_OK;
 }
@@ -391,6 +558,19 @@ int Cmf::error(double* out) {
 
 int Cmf::orthogonal(cplx<double>* P) {
   int rc;
+  if (bigOrtho_) {
+    for (size_t h = 0; h < orthoRange_.size(); ++h) {
+      const auto& r = orthoRange_[h];
+      if ((rc = launch_ortho_global(d_oi_ + r.first.first, r.first.second - r.first.first, E.d_nodes_, P, sw_,
+                                    E.nspf_, s_)))
+        return rc;
+      for (const SwLaunch& sl : swLaunch_[h])
+        if ((rc = launch_gm<double, double, double>(d_swgm_ + sl.d0, nullptr, sl.ntasks, sl.ntasks, P, swWork_, sw_,
+                                                    sl.toState ? (void*)P : (void*)swWork_, nullptr, 1.0, 0.0, s_)))
+          return rc;
+    }
+    return QTG_OK;
+  }
   for (auto& r : orthoRange_) {
     if ((rc = launch_ortho(d_oi_ + r.first.first, r.first.second - r.first.first, E.d_nodes_, P, sw_, E.nmax_,
                            E.nspf_, s_)))
@@ -403,6 +583,21 @@ int Cmf::orthogonal(cplx<double>* P) {
 }
 
 int Cmf::overlap_top(const cplx<double>* bra, const cplx<double>* ket, cplx<double>* out) {
+  if (bigOrtho_) {
+    cplx<double>* work = ovWork_ ? ovWork_ : swWork_;
+    for (auto& hl : ovLaunch_)
+      for (const OvLaunch& ol : hl) {
+        int rc = ol.kind == 0
+                     ? launch_gm<double, double, double>(d_ovgm_ + ol.d0, nullptr, ol.ntasks, ol.ntasks, ket, work,
+                                                         spool_, work, nullptr, 1.0, 0.0, s_)
+                     : launch_gc<double, double>(d_ovgc_ + ol.d0, nullptr, ol.ntasks, ol.ntasks, bra, work, spool_,
+                                                 nullptr, s_);
+        if (rc) return rc;
+      }
+    QTG_HIP_TRY(hipMemcpyAsync(out, spool_ + sslot_[NN_ - 1], sizeof(cplx<double>), hipMemcpyDeviceToHost, s_));
+    QTG_HIP_TRY(hipStreamSynchronize(s_));
+    return QTG_OK;
+  }
   for (size_t h = 0; h < ovRange_.size(); ++h) {
     auto& r = ovRange_[h];
     int rc = launch_sandwich<double>(d_ov_ + r.first, r.second - r.first, E.d_nodes_, bra, ket, spool_, ovNmax_[h],
@@ -416,6 +611,30 @@ int Cmf::overlap_top(const cplx<double>* bra, const cplx<double>* ket, cplx<doub
 
 // CMFIntegrator::Output: Expectation with the current matrices / S_top, and
 // the autocorrelation <Psi(0)|Psi(t)>.
+// TensorTree<cd>::write(filename, append) (TensorTree_Implementation.h:
+// 93-100): append opens for appending (created when missing); otherwise the
+// file is opened in place, in | out, so a missing file is not created and
+// an existing one is overwritten from its start without being truncated.
+int Cmf::write_state(const cplx<double>* P) {
+  if (outFile_.empty()) return QTG_OK;
+  std::vector<double> host(2 * (size_t)total_);
+  QTG_HIP_TRY(hipMemcpyAsync(host.data(), P, sizeof(double) * host.size(), hipMemcpyDeviceToHost, s_));
+  QTG_HIP_TRY(hipStreamSynchronize(s_));
+  const int64_t n = qtg_tensor_tree_write(E.tree_text().c_str(), host.data(), nullptr, 0);
+  if (n < 0) return (int)-n;
+  std::vector<unsigned char> buf((size_t)n);
+  if (qtg_tensor_tree_write(E.tree_text().c_str(), host.data(), buf.data(), n) != n) return QTG_EARG;
+  std::FILE* f = std::fopen(outFile_.c_str(), outAppend_ ? "ab" : "r+b");
+  if (!f) return QTG_OK;  // the reference's stream fails silently as well
+  const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+  std::fclose(f);
+  if (w != buf.size()) {
+    set_error("cmf: writing " + outFile_ + " failed");
+    return QTG_EARG;
+  }
+  return QTG_OK;
+}
+
 int Cmf::output(double time, CmfResult& res) {
   cplx<double> stop, ac;
   int rc = overlap_top(P_, P_, &stop);
@@ -460,6 +679,7 @@ int Cmf::integrate(double time, double timeend, double dt, double out, double ac
   double t_next = time + out;
   bool calc_mat = true;
   int rc;
+  if ((rc = write_state(P_))) return rc;
   if ((rc = copy(PS_, P_))) return rc;
   if ((rc = build(P_))) return rc;
   calc_mat = false;
@@ -496,6 +716,7 @@ int Cmf::integrate(double time, double timeend, double dt, double out, double ac
       time += dt;
       if ((rc = orthogonal(P_))) return rc;
       if (time + 1e-10 >= t_next) {
+        if ((rc = write_state(P_))) return rc;
         if ((rc = output(time, res))) return rc;
         t_next = time + out;
       }
@@ -589,7 +810,8 @@ int Cmf::eigenstate(std::vector<double>& ev) {
   if ((rc = apply_top(P_, 0.0, 1.0, nullptr))) return rc;
   if ((rc = launch_sandwich<double>(d_hx_, 1, E.d_nodes_, P_, D_, eigm_, ovNmax_.back(), s_))) return rc;
   if ((rc = launch_eig_sorted(eigm_, n, eigm_ + (int64_t)n * n, eigv_, s_))) return rc;
-  if ((rc = launch_apply_sw(d_rot_, 1, E.d_nodes_, P_, eigm_, E.nmax_, s_))) return rc;
+  // the top node only: its LDS buffers are sized by its own tensor
+  if ((rc = launch_apply_sw(d_rot_, 1, E.d_nodes_, P_, eigm_, E.tree_.nodes[NN_ - 1].N, s_))) return rc;
   ev.assign(n, 0.0);
   QTG_HIP_TRY(hipMemcpyAsync(ev.data(), eigv_, sizeof(double) * n, hipMemcpyDeviceToHost, s_));
   QTG_HIP_TRY(hipStreamSynchronize(s_));
@@ -626,7 +848,9 @@ int Cmf::eigenstates(double dt, double out, int64_t iterations, double acc_cmf, 
     for (double& e : ev) e = -std::log(e) / (2.0 * out);
     res.propagator.insert(res.propagator.end(), ev.begin(), ev.end());
     // orthonormal (TensorTree_Implementation.h:190-195)
-    if ((rc = launch_ortho(d_on_, NN_, E.d_nodes_, P_, sw_, E.nmax_, E.nspf_, s_))) return rc;
+    if ((rc = bigOrtho_ ? launch_ortho_global(d_on_, NN_, E.d_nodes_, P_, sw_, E.nspf_, s_)
+                        : launch_ortho(d_on_, NN_, E.d_nodes_, P_, sw_, E.nmax_, E.nspf_, s_)))
+      return rc;
     if ((rc = eigenstate(ev))) return rc;
     res.energies.insert(res.energies.end(), ev.begin(), ev.end());
     // overlap with the previous iterate: s(i) = sum_j |<Psi_i|Psi'_j>|^2

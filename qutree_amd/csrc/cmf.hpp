@@ -1,11 +1,13 @@
 // Device CMF integrator (see cmf.cpp).
 #pragma once
+#include <string>
 #include <complex>
 #include <utility>
 #include <vector>
 
 #include "cmf_ops.hpp"
 #include "engine.hpp"
+#include "gemm_ops.hpp"
 
 namespace qtg {
 
@@ -46,6 +48,11 @@ class Cmf {
                   EigenResult& res);
   int nstates() const { return E.tree_.nodes.back().n; }
   double ph_re_ = 1.0, ph_im_ = 0.0;  // propagation phase (1 = real time)
+  // Psi.write(job.ofname, job.append_) of CMFIntegrator::Integrate
+  // (CMFIntegrator.cpp:46,153): the state is written as a TensorTree at the
+  // start and at every output; empty name: off
+  std::string outFile_;
+  bool outAppend_ = true;
 
  private:
   int eig_init();
@@ -61,6 +68,7 @@ class Cmf {
   int cmfstep(cplx<double>* P, double t0, double t1, double eps);
   int error(double* out);
   int orthogonal(cplx<double>* P);
+  int write_state(const cplx<double>* P);
   int overlap_top(const cplx<double>* bra, const cplx<double>* ket, cplx<double>* out);
   int output(double time, CmfResult& res);
 
@@ -82,6 +90,29 @@ class Cmf {
   OrthoItem* d_oi_ = nullptr;
   SwApply* d_sa_ = nullptr;
   std::vector<std::pair<std::pair<int64_t, int64_t>, std::pair<int64_t, int64_t>>> orthoRange_;
+  // orthogonal() on node tensors too large for LDS: the node kernel works in
+  // global memory and the parents take the square roots by grouped mode
+  // products (per height and leg position, chained through swWork_)
+  bool bigOrtho_ = false;
+  struct SwLaunch {
+    int64_t d0, ntasks;
+    bool toState;  // the last leg of each parent: writes the state
+  };
+  std::vector<std::vector<SwLaunch>> swLaunch_;  // per height
+  GmDesc* d_swgm_ = nullptr;
+  cplx<double>* swWork_ = nullptr;
+  // the overlap tree on such trees: per height, mode-product levels (the
+  // children's overlap matrices on the ket, into the workspace) and one
+  // grouped contraction with the bra
+  struct OvLaunch {
+    int kind;  // 0 mode products, 1 contraction
+    int64_t d0, ntasks;
+  };
+  std::vector<std::vector<OvLaunch>> ovLaunch_;
+  GmDesc* d_ovgm_ = nullptr;
+  GcDesc* d_ovgc_ = nullptr;
+  int64_t swWorkSize_ = 0;
+  cplx<double>* ovWork_ = nullptr;  // when the overlap tree needs more than swWork_
   cplx<double>* spool_ = nullptr;
   std::vector<int64_t> sslot_;
   SwItem* d_ov_ = nullptr;

@@ -390,6 +390,90 @@ __global__ void __launch_bounds__(EW)
   for (int e = threadIdx.x; e < N; e += EW) P[nd.off + e] = Phi[e];
 }
 
+// orthogonal() for node tensors too large for LDS: the algorithm of
+// ortho_kernel with Phi read and written in place in global memory (only the
+// n x n Jacobi buffers in LDS).  One wavefront per node; a lane owns rows
+// i = lane + 64 t, so the Gram-Schmidt reductions are shuffle sums.
+__global__ void __launch_bounds__(EW)
+    ortho_global_kernel(const OrthoItem* __restrict__ items, const DNode* __restrict__ nodes, cd2* P, cd2* sw,
+                        int nspf) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cd2* A = reinterpret_cast<cd2*>(smem_raw);
+  cd2* U = A + jacobi_elems(nspf);
+  __shared__ JacobiScratch js;
+  __shared__ double lam[64];
+  const OrthoItem it = items[blockIdx.x];
+  const DNode nd = nodes[it.node];
+  const int N = nd.N, n = nd.dims[nd.order - 1], L = N / n, ld = jacobi_ld(n);
+  cd2* Phi = P + nd.off;
+  if (it.sw >= 0) {
+    // S(j,k) = sum_i conj(Phi(i,j)) Phi(i,k): lanes split the rows, every
+    // entry is a shuffle sum (coalesced column reads)
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) {
+        double re = 0.0, im = 0.0;
+        for (int i = threadIdx.x; i < L; i += EW) {
+          const cd2 a = Phi[i + j * L], b = Phi[i + k * L];
+          re += a.re * b.re + a.im * b.im;
+          im += a.re * b.im - a.im * b.re;
+        }
+        re = wave_sum(re);
+        im = wave_sum(im);
+        if (threadIdx.x == 0) A[j + k * ld] = mk<double>(re, im);
+      }
+    __syncthreads();
+    jacobi_eig(A, U, n, ld, js);
+    for (int i = threadIdx.x; i < n; i += EW) lam[i] = sqrt(fmax(A[i + i * ld].re, 0.0));
+    __syncthreads();
+    for (int o = threadIdx.x; o < n * n; o += EW) {
+      const int j = o % n, k = o / n;
+      cd2 acc = czero<double>();
+      for (int l = 0; l < n; ++l) {
+        const cd2 ukl = U[k + l * ld], ujl = U[j + l * ld];
+        acc.re += lam[l] * (ukl.re * ujl.re + ukl.im * ujl.im);
+        acc.im += lam[l] * (ukl.im * ujl.re - ukl.re * ujl.im);
+      }
+      sw[it.sw + j + k * n] = acc;
+    }
+  }
+  for (int k = 0; k < n; ++k) {
+    double accum = 1.0;
+    for (int iter = 0; iter < 15 && accum > 1e-12; ++iter) {
+      accum = 0.0;
+      for (int m = 0; m < k; ++m) {
+        double ore = 0.0, oim = 0.0;
+        for (int i = threadIdx.x; i < L; i += EW) {
+          const cd2 a = Phi[i + m * L], b = Phi[i + k * L];
+          ore += a.re * b.re + a.im * b.im;
+          oim += a.re * b.im - a.im * b.re;
+        }
+        ore = wave_sum(ore);
+        oim = wave_sum(oim);
+        accum += sqrt(ore * ore + oim * oim);
+        for (int i = threadIdx.x; i < L; i += EW) {
+          const cd2 a = Phi[i + m * L];
+          cd2 b = Phi[i + k * L];
+          b.re -= ore * a.re - oim * a.im;
+          b.im -= ore * a.im + oim * a.re;
+          Phi[i + k * L] = b;
+        }
+      }
+      double nr = 0.0;
+      for (int i = threadIdx.x; i < L; i += EW) nr += Phi[i + k * L].re * Phi[i + k * L].re + Phi[i + k * L].im * Phi[i + k * L].im;
+      nr = wave_sum(nr);
+      if (nr != 0.0) {
+        const double s = sqrt(nr);
+        for (int i = threadIdx.x; i < L; i += EW) {
+          cd2 b = Phi[i + k * L];
+          b.re /= s;
+          b.im /= s;
+          Phi[i + k * L] = b;
+        }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------ top eigenproblems
 // One wavefront: the Hermitian n x n matrix A (column-major) is diagonalised
 // (jacobi.hpp); eigenvalues ascending into ev (stable order), and the
@@ -479,6 +563,15 @@ int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx
   if (count == 0) return QTG_OK;
   const size_t lds = ((size_t)nmax + 2 * (size_t)jacobi_elems(nspf)) * sizeof(cd2);
   hipLaunchKernelGGL(ortho_kernel, dim3((unsigned)count), dim3(EW), lds, s, items, nodes, P, sw, nmax, nspf);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+int launch_ortho_global(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P,
+                        cplx<double>* sw, int nspf, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  const size_t lds = 2 * (size_t)jacobi_elems(nspf) * sizeof(cd2);
+  hipLaunchKernelGGL(ortho_global_kernel, dim3((unsigned)count), dim3(EW), lds, s, items, nodes, P, sw, nspf);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }

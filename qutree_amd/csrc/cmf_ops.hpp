@@ -59,6 +59,9 @@ int launch_programs(const NodeProg* progs, int64_t count, const VecOp* ops, cons
                     hipStream_t s);
 int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P, cplx<double>* sw,
                  int nmax, int nspf, hipStream_t s);
+// The same for node tensors of any size: Phi in place in global memory.
+int launch_ortho_global(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P,
+                        cplx<double>* sw, int nspf, hipStream_t s);
 // Hermitian eigenproblem of one n x n matrix (n <= 64): ascending
 // eigenvalues into ev, the eigenvectors transposed into UT (may be null).
 int launch_eig_sorted(const cplx<double>* A, int n, cplx<double>* UT, double* ev, hipStream_t s);

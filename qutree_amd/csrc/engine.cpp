@@ -63,6 +63,10 @@ void Engine::release() {
     if (p) (void)hipFree(p);
   device_buffers_.clear();
   d_grad64_ = nullptr;
+  d_escal_ = nullptr;
+  d_elog_ = nullptr;
+  elogCap_ = 0;
+  deviceBytes_ = 0;
 }
 
 int Engine::alloc(void** p, size_t bytes) {
@@ -71,11 +75,13 @@ int Engine::alloc(void** p, size_t bytes) {
     return QTG_ENOMEM;
   }
   device_buffers_.push_back(*p);
+  deviceBytes_ += (int64_t)std::max<size_t>(bytes, 16);
   return QTG_OK;
 }
 
 int Engine::create(const std::string& tree_text, int precision, bool host_only) {
   host_only_ = host_only;
+  treeText_ = tree_text;
   try {
     tree_ = parse_tree(tree_text);
   } catch (std::exception& e) {
@@ -136,6 +142,7 @@ int Engine::set_hamiltonian(int64_t nterms, const double* coeff, const int64_t* 
     set_error("hamiltonian: " + err);
     return QTG_EARG;
   }
+  apply_tuning();
   return plan();
 }
 
@@ -579,6 +586,8 @@ int Engine::plan() {
     device_buffers_.push_back(d_gmwg_);
     if ((rc = up(&d_gcwg_, h_gcwg_))) return rc;
     device_buffers_.push_back(d_gcwg_);
+    if ((rc = up(&d_canon_, h_canon_))) return rc;
+    device_buffers_.push_back(d_canon_);
   }
   // the grouped path keeps every matrix in FP64 (FP32 kernels round on load)
   const size_t es = (c64_ && path_ == 0) ? 8 : 16;

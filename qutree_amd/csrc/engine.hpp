@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "gemm_ops.hpp"
 #include "model.hpp"
+#include "smallmat.hpp"
 #include "tree_ops.hpp"
 
 namespace qtg {
@@ -38,6 +39,7 @@ class Engine {
   int get_matrix(int which, int term, int node, double* out, hipStream_t s);
 
   const HTree& tree() const { return tree_; }
+  const std::string& tree_text() const { return treeText_; }
   const HSop& sop() const { return sop_; }
   double sweep_flops() const { return flops_; }        // executed by this plan
   double reference_flops() const { return refFlops_; }  // of the reference algorithm
@@ -61,6 +63,15 @@ class Engine {
   int path_ = 1;
   int64_t splitWaves_ = 2048;  // wavefronts a grouped hole-contraction launch is split up to
   int splitMinK_ = 16;         // shortest K chunk of such a split
+  int gmSplitK_ = 32;    // mode products longer than 2 gmSplitK_ split along K, chunks >= gmSplitK_
+  int gcLongK_ = 512;    // hole contractions longer than 2 gcLongK_ always split along K
+  // Plan parameters from the tuning table (tuning.cpp) for this plan's
+  // class, unless the caller set them as options; tuning_ false skips it.
+  bool tuning_ = true;
+  unsigned userSet_ = 0;  // bits: options set explicitly (see tuning.cpp)
+  std::string tunedFrom_;  // the table line that was applied (empty: none)
+  void apply_tuning();
+  const std::string& tuned_from() const { return tunedFrom_; }
   // Apply + projector + rho^-1 on an arbitrary FP64 state (complex128 engines):
   // dpsi = -i phase rho^-1 (1 - P) Apply(psi) at every node whose active flag
   // is set (all when active is null); energy (4 doubles: <A|HA>, <A|A> of the
@@ -71,6 +82,27 @@ class Engine {
   // by set_psi, whose non-top nodes are orthonormal; grad_host receives the
   // gradient in the engine's layout, energy the expectation value.
   int gradient(hipStream_t s, double* grad_host, double* energy);
+  // The energy functional E(A) = <Psi|H|Psi> / <Psi|Psi> as a forward /
+  // backward pair (complex128 engines, grouped path, one state at the top):
+  // forward runs the sweep at phase 1 and leaves E and 1/<Psi|Psi> on the
+  // device (energy_host, when not null, receives E); backward writes dE/dA*
+  // into the engine's gradient buffer (grad_host, when not null, receives
+  // it).  With gradRecompute_ backward first recomputes the sweep's
+  // matrices from the current state, so nothing of the forward pass has to
+  // be kept in between (other sweeps may overwrite the matrix pool);
+  // otherwise it uses those of the last forward.
+  int forward(hipStream_t s, double* energy_host);
+  int backward(hipStream_t s, double* grad_host);
+  // steps of A <- A - eta dE/dA* at every node, each followed by the QR
+  // gauge (every non-top node orthonormalised, R pushed into its parent),
+  // all on the device.  energies_host (steps + 1) receives E before every
+  // step and after the last; psi_host (may be null) the final state.
+  int gradient_descent(hipStream_t s, int64_t steps, double eta, double* energies_host, double* psi_host);
+  // QR gauge of the state on the device (see gradient_descent)
+  int canonicalize(hipStream_t s);
+  bool gradRecompute_ = false;
+  // device bytes this engine holds
+  int64_t device_bytes() const { return deviceBytes_; }
   hipStream_t stream() const { return stream_; }
   // waits for the engine's streams (its own and the eigensolver's side stream)
   int synchronize() {
@@ -93,6 +125,7 @@ class Engine {
   int apply_t(hipStream_t s, double ph_re, double ph_im);
   int plan();
   int plan_gemm();
+  void dump_plan() const;
   int convert_psi(hipStream_t s);  // FP64 state -> FP32 copy (complex64 engines)
   int build_gemm(hipStream_t s);
   int apply_gemm(hipStream_t s, const double* psi64, double* dpsi64, const uint8_t* active, double ph_re,
@@ -131,6 +164,7 @@ class Engine {
   void release();
 
   HTree tree_;
+  std::string treeText_;
   HSop sop_;
   bool c64_ = false, planned_ = false, built_ = false, host_only_ = false;
   int precision_ = 0;  // 0 complex128, 1 complex64 bulk, 2 complex64 bulk with FP64 for large terms
@@ -159,6 +193,18 @@ class Engine {
   std::vector<GLaunch> gGradPre_, gGradFin_;
   std::vector<std::vector<GLaunch>> gGradDepth_;
   double* d_grad64_ = nullptr;
+  int backward_unscaled(hipStream_t s);
+  int grad_ready();  // checks and allocations of the gradient path
+  double* d_escal_ = nullptr;  // [E, 1 / <Psi|Psi>] of the last forward
+  double* d_elog_ = nullptr;   // energies of a device descent
+  int64_t elogCap_ = 0;
+  // canonicalisation: QR items per depth (deepest first) and the parent
+  // pushes of each depth, leg by leg
+  std::vector<SmItem> h_canon_;
+  SmItem* d_canon_ = nullptr;
+  std::vector<std::pair<int64_t, int64_t>> canonRange_;
+  std::vector<std::vector<GLaunch>> gCanon_;
+  int64_t deviceBytes_ = 0;
   double gradFlops_ = 0.0;
   CombOut *d_combG1_ = nullptr, *d_combG2_ = nullptr, *d_combG3_ = nullptr, *d_combGE_ = nullptr;
   CombTerm *d_combtG1_ = nullptr, *d_combtG2_ = nullptr, *d_combtG3_ = nullptr, *d_combtGE_ = nullptr;
@@ -171,7 +217,7 @@ class Engine {
   int32_t* d_gmwg_ = nullptr;
   int32_t* d_gcwg_ = nullptr;
   std::vector<SolveItem> h_solve_;
-  std::vector<GLaunch> gBuild_, gHole_, gRho_, gApply_;
+  std::vector<GLaunch> gBuild_, gHole_, gApply_;
   std::vector<std::pair<int64_t, int>> idSlots_;  // identity matrices in the pool (offset, n)
   std::vector<CombOut> h_comb3_;
   std::vector<CombTerm> h_combt3_;

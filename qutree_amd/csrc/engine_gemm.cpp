@@ -25,6 +25,8 @@
 //     dpsi = f rho^-1 x_L Y - f W x_L A,   f = -i phase,
 // which is -i phase rho^-1 (1 - P) Apply of HamiltonianRepresentation.cpp:53-72.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -35,8 +37,6 @@ namespace qtg {
 
 namespace {
 
-constexpr int kGmSplitK = 32;  // mode products longer than 2 kGmSplitK split along K, chunks >= kGmSplitK
-constexpr int kGcSplitK = 512;  // hole contractions longer than 2 kGcSplitK split along K
 
 void view(const HNode& v, int k, int64_t& bef, int& nk, int64_t& aft) {
   bef = 1;
@@ -78,7 +78,6 @@ int Engine::plan_gemm() {
   h_solve_.clear();
   gBuild_.clear();
   gHole_.clear();
-  gRho_.clear();
   gApply_.clear();
   idSlots_.clear();
   h_comb3_.clear();
@@ -113,8 +112,8 @@ int Engine::plan_gemm() {
     for (auto& d : descs) {
       // long contractions (stacked Apply summands) are split along K into
       // chunks that sum inside their workgroup
-      if (d.op != GM_COPY && d.nin > 2 * kGmSplitK) {
-        d.ksplit = split_chunks(d.nin, kGemmWavesWide, kGmSplitK, kGemmWavesWide);
+      if (d.op != GM_COPY && d.nin > 2 * gmSplitK_) {
+        d.ksplit = split_chunks(d.nin, kGemmWavesWide, gmSplitK_, kGemmWavesWide);
         if (d.ksplit > kGemmWaves) gl.waves = kGemmWavesWide;
       }
       if (d.op != GM_COPY) fl += 8.0 * d.bef * d.aft * (double)d.nin * d.nout;
@@ -150,7 +149,7 @@ int Engine::plan_gemm() {
     });
     // launches with few tiles split their reductions along K (2 or 4
     // chunks of at least splitMinK_) toward about splitWaves_ wavefronts (a
-    // 16 x 16 tile is one); beyond 2 kGcSplitK a reduction is always split
+    // 16 x 16 tile is one); beyond 2 gcLongK_ a reduction is always split
     int64_t tiles = 0;
     for (auto& d : descs) tiles += gc_tasks(d);
     const int64_t want = (splitWaves_ + tiles - 1) / tiles;
@@ -166,7 +165,7 @@ int Engine::plan_gemm() {
     double fl = 0.0;
     for (auto& d : descs) {
       const int64_t K = (int64_t)d.bef * d.aft;
-      d.ksplit = K > 2 * kGcSplitK ? split_chunks(K, kGemmWaves, 1) : split_chunks(K, want, splitMinK_);
+      d.ksplit = K > 2 * gcLongK_ ? split_chunks(K, kGemmWaves, 1) : split_chunks(K, want, splitMinK_);
       fl += 8.0 * d.bef * d.aft * (double)d.nx * d.ny;
     }
     const int64_t t = order_tasks(descs, gc_tasks);
@@ -199,7 +198,13 @@ int Engine::plan_gemm() {
     return precision_ == 0 || (precision_ == 2 && highSlots_.count(out) > 0);
   };
   int64_t wbase = 0, wbase32 = 0, wmax = 0, wmax32 = 0;
-  auto sandwich_layer = [&](const std::pair<int64_t, int64_t>& r, int rho, std::vector<GLaunch>& L) {
+  // descriptors of one layer: first factors, second factors, contractions
+  // ([0] FP32, [1] FP64)
+  struct LayerLists {
+    std::vector<GmDesc> lv0[2], lv1[2];
+    std::vector<GcDesc> con[2];
+  };
+  auto sandwich_layer = [&](const std::pair<int64_t, int64_t>& r, int rho, LayerLists& out) {
     if (r.second == r.first) return;
     struct Side {
       std::map<std::tuple<int, int, int64_t>, int64_t> zbra;  // (node, leg, mat) -> work offset
@@ -279,23 +284,51 @@ int Engine::plan_gemm() {
     }
     wmax32 = std::max(wmax32, sd[0].w);
     wmax = std::max(wmax, sd[1].w);
-    push_gm(L, sd[1].lv0, P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
-    push_gm(L, sd[0].lv0, P32, B_PSI32, B_WORK32, B_POOL, B_WORK32, false);
-    push_gm(L, sd[1].lv1, P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
-    push_gm(L, sd[0].lv1, P32, B_PSI32, B_WORK32, B_POOL, B_WORK32, false);
-    push_gc(L, sd[1].con, P64, B_PSI64, B_WORK64, B_POOL);
-    push_gc(L, sd[0].con, P32, B_PSI32, B_WORK32, B_POOL);
+    for (int hp = 0; hp < 2; ++hp) {
+      auto app = [](auto& dst, auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
+      app(out.lv0[hp], sd[hp].lv0);
+      app(out.lv1[hp], sd[hp].lv1);
+      app(out.con[hp], sd[hp].con);
+    }
   };
-  // represent (bottom-up) and the holes of the terms (top-down) on the engine
-  // stream; the density tree, which needs neither, runs top-down on a side
-  // stream with its own workspace region and feeds the eigensolver early
-  for (auto& r : repRange_) sandwich_layer(r, -1, gBuild_);
-  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 0, gHole_);
+  auto push_layer = [&](LayerLists& x, std::vector<GLaunch>& L) {
+    push_gm(L, x.lv0[1], P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+    push_gm(L, x.lv0[0], P32, B_PSI32, B_WORK32, B_POOL, B_WORK32, false);
+    push_gm(L, x.lv1[1], P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+    push_gm(L, x.lv1[0], P32, B_PSI32, B_WORK32, B_POOL, B_WORK32, false);
+    push_gc(L, x.con[1], P64, B_PSI64, B_WORK64, B_POOL);
+    push_gc(L, x.con[0], P32, B_PSI32, B_WORK32, B_POOL);
+  };
+  // Represent (bottom-up) and the density tree (top-down) are independent
+  // chains of the same depth: layer j of each runs in the same launches
+  // (the density layer in its own workspace region), so the densities are
+  // ready when represent ends.  The holes of the terms (top-down), which
+  // use the densities as the parent factor of terms acting only below a
+  // node, follow; the eigensolver of rho^-1 runs on a side stream beside them.
+  std::vector<LayerLists> rep(repRange_.size()), hol(holeRange_.size()), rho(holeRange_.size());
+  for (size_t i = 0; i < repRange_.size(); ++i) sandwich_layer(repRange_[i], -1, rep[i]);
+  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 0, hol[dd]);
   wbase = wmax;
   wbase32 = wmax32;
-  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 1, gRho_);
+  for (size_t dd = 1; dd < holeRange_.size(); ++dd) sandwich_layer(holeRange_[dd], 1, rho[dd]);
   workSize_ = std::max(workSize_, wmax);
   work32Size_ = std::max(work32Size_, wmax32);
+  {
+    const size_t nl = std::max(rep.size(), holeRange_.size() > 0 ? holeRange_.size() - 1 : 0);
+    for (size_t j = 0; j < nl; ++j) {
+      LayerLists m;
+      for (LayerLists* src : {j < rep.size() ? &rep[j] : nullptr, j + 1 < rho.size() ? &rho[j + 1] : nullptr}) {
+        if (!src) continue;
+        for (int hp = 0; hp < 2; ++hp) {
+          m.lv0[hp].insert(m.lv0[hp].end(), src->lv0[hp].begin(), src->lv0[hp].end());
+          m.lv1[hp].insert(m.lv1[hp].end(), src->lv1[hp].begin(), src->lv1[hp].end());
+          m.con[hp].insert(m.con[hp].end(), src->con[hp].begin(), src->con[hp].end());
+        }
+      }
+      push_layer(m, gBuild_);
+    }
+    for (size_t dd = 1; dd < hol.size(); ++dd) push_layer(hol[dd], gHole_);
+  }
 
   // ---------------------------------------------------------------- apply
   std::map<int, int64_t> idmap;
@@ -769,9 +802,105 @@ int Engine::plan_gemm() {
     push_gm(gGradFin_, f2, P64, B_PSI64, B_APPLY64, B_POOL, B_DPSI64, true);
     gradFlops_ = gcomb;
   }
+  // ----------------------------------------------------- canonicalisation
+  // The QR gauge on the device (qutree_amd.gauge.canonicalize_all's order
+  // and arithmetic): per depth, deepest first, one batched Householder QR of
+  // the non-top nodes of that depth (in place, R kept in the pool), then
+  // their parents take R on each child leg, one mode-product launch per leg
+  // position, chained through the workspace and ending in the state.
+  h_canon_.clear();
+  canonRange_.clear();
+  gCanon_.clear();
+  {
+    std::vector<int64_t> rslot(NN, -1);
+    for (auto& v : tree_.nodes)
+      if (!v.top()) {
+        rslot[v.addr] = poolOff;
+        poolOff += (int64_t)v.n * v.n;
+      }
+    for (int d = tree_.maxDepth; d >= 1; --d) {
+      const int64_t i0 = (int64_t)h_canon_.size();
+      for (auto& v : tree_.nodes)
+        if (!v.top() && v.depth == d) {
+          SmItem it{};
+          it.a = it.q = v.off;
+          it.e = rslot[v.addr];
+          it.m = v.N / v.n;
+          it.n = v.n;
+          h_canon_.push_back(it);
+        }
+      canonRange_.push_back({i0, (int64_t)h_canon_.size()});
+      // per leg position: descriptors writing the workspace, and those
+      // writing the state (the last leg of each parent)
+      std::vector<std::vector<GmDesc>> toWork, toPsi;
+      int64_t w = 0;
+      for (auto& p : tree_.nodes) {
+        if (p.bottom || p.depth != d - 1) continue;
+        const int nc = (int)p.children.size();
+        const int levels = std::max(nc, 2);
+        if ((int)toWork.size() < levels) toWork.resize(levels), toPsi.resize(levels);
+        const int64_t wa = w, wb = w + p.N;
+        w += 2 * p.N;
+        int64_t cur = p.off;
+        int cursel = 0;
+        for (int c = 0; c < levels; ++c) {
+          const bool last = (c == levels - 1);
+          const int64_t out = last ? p.off : ((c % 2 == 0) ? wa : wb);
+          GmDesc g = (c < nc) ? gm_desc(p, c, cur, cursel, rslot[p.children[c]], out, GM_N, -1)
+                              : gm_desc(p, p.order() - 1, cur, cursel, 0, out, GM_COPY, -1);
+          (last ? toPsi : toWork)[c].push_back(g);
+          cur = out;
+          cursel = 1;
+        }
+      }
+      workSize_ = std::max(workSize_, w);
+      gCanon_.emplace_back();
+      for (size_t c = 0; c < toWork.size(); ++c) {
+        push_gm(gCanon_.back(), toWork[c], P64, B_PSI64, B_WORK64, B_POOL, B_WORK64, false);
+        push_gm(gCanon_.back(), toPsi[c], P64, B_PSI64, B_WORK64, B_POOL, B_PSI64, false);
+      }
+    }
+  }
+
   poolSize_ = poolOff;
   pool64Size_ = pool64Off;
+  if (std::getenv("QTG_DUMP_PLAN")) dump_plan();
   return QTG_OK;
+}
+
+// QTG_DUMP_PLAN=1: the launch lists of the grouped path on stderr (one line
+// per launch: list, kind, precision, tasks, FLOPs, algorithmic bytes, and
+// the K range and output count of its descriptors).
+void Engine::dump_plan() const {
+  auto one = [&](const char* name, const std::vector<GLaunch>& L) {
+    for (size_t i = 0; i < L.size(); ++i) {
+      const GLaunch& g = L[i];
+      int kmin = 1 << 30, kmax = 0;
+      long outs = 0;
+      if (g.kind == 0) {
+        for (int64_t t = 0; t < g.ntasks; ++t) {
+          const GmDesc& d = h_gm_[g.d0 + t];
+          if (d.tile0 != t) continue;
+          kmin = std::min(kmin, d.nin);
+          kmax = std::max(kmax, d.nin);
+          ++outs;
+        }
+      } else if (g.kind == 1) {
+        for (int64_t t = 0; t < g.ntasks; ++t) {
+          const GcDesc& d = h_gc_[g.d0 + t];
+          if (d.tile0 != t) continue;
+          kmin = std::min(kmin, d.bef * d.aft);
+          kmax = std::max(kmax, d.bef * d.aft);
+          ++outs;
+        }
+      }
+      std::fprintf(stderr, "plan %-5s %2zu kind %d prec %d tasks %7lld descs %6ld K %d..%d flops %.3e bytes %.3e\n",
+                   name, i, g.kind, g.prec, (long long)g.ntasks, outs, kmin, kmax, g.flops, g.bytes);
+    }
+  };
+  one("build", gBuild_);
+  one("hole", gHole_);
+  one("apply", gApply_);
 }
 
 int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, void* const* bases,
@@ -827,29 +956,18 @@ int Engine::build_gemm(hipStream_t s) {
   int rc;
   if (c64_ && (rc = convert_psi(s))) return rc;
   void* bases[B_COUNT] = {d_psi32_, d_work32_, d_pool_, d_psi64_, d_apply64_, d_dpsi64_, d_pool64_, d_work_};
-  // Side stream: density tree, then the eigensolver (which only feeds the
-  // solve and rho^-1 stages of the finish).  Engine stream: represent, then
-  // (after the densities, which some hole items use as parent factor) the
-  // holes, the combinations and Apply; apply_gemm joins before the solve.
-  // The two streams' launches are issued alternately: a replayed graph
-  // enqueues its nodes in the order they were recorded, so neither chain
-  // waits behind the host-side enqueue of the other.
+  // represent and the density tree, layer by layer in shared launches
+  if ((rc = run_launches(gBuild_, bases, nullptr, 1.0, 0.0, s))) return rc;
+  // side stream: the eigensolver (it only feeds the solve of the finish,
+  // which apply_gemm makes wait for it); engine stream: the holes, the
+  // combinations and Apply
   QTG_HIP_TRY(hipEventRecord(evFork_, s));
   QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
-  for (size_t i = 0; i <= std::max(gRho_.size(), gBuild_.size()); ++i) {
-    if ((rc = run_launches(gBuild_, i, i + 1, bases, nullptr, 1.0, 0.0, s))) return rc;
-    if (i < gRho_.size()) {
-      if ((rc = run_launches(gRho_, i, i + 1, bases, nullptr, 1.0, 0.0, side_))) return rc;
-    } else if (i == gRho_.size()) {
-      QTG_HIP_TRY(hipEventRecord(evRho_, side_));
-      if ((rc = begin_mark(1, side_))) return rc;
-      if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
-      if ((rc = end_mark(side_))) return rc;
-      QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
-      heevPending_ = true;
-    }
-  }
-  QTG_HIP_TRY(hipStreamWaitEvent(s, evRho_, 0));
+  if ((rc = begin_mark(1, side_))) return rc;
+  if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
+  if ((rc = end_mark(side_))) return rc;
+  QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
+  heevPending_ = true;
   if ((rc = run_launches(gHole_, bases, nullptr, 1.0, 0.0, s))) return rc;
   if ((rc = begin_mark(5, s))) return rc;
   // the aggregated slots, then the sums over them with the stacked blocks
@@ -893,48 +1011,5 @@ int Engine::apply_finish(const double* psi64, double* dpsi64, const uint8_t* act
   return launch_finish<double>(d_fin_, nFin_, d_nodes_, psi64, d_partial_, d_pool64_, dpsi64, nullptr, ph_re, ph_im,
                                energy_dev, nmax_, nspf_, s, active);
 }
-
-int Engine::gradient(hipStream_t s, double* grad_host, double* energy) {
-  if (c64_ || path_ != 1) {
-    set_error("engine: the energy gradient needs a complex128 engine on the grouped path");
-    return QTG_ESTATE;
-  }
-  if (!planned_) {
-    set_error("engine: no Hamiltonian");
-    return QTG_ESTATE;
-  }
-  if (tree_.nodes.back().n != 1) {
-    set_error("engine: the energy gradient takes one state at the top node");
-    return QTG_ESHAPE;
-  }
-  int rc = derivative(s, 1.0, 0.0);
-  if (rc) return rc;
-  double e[4];
-  QTG_HIP_TRY(hipMemcpyAsync(e, d_pool64_ + 2 * energySlot_, sizeof(e), hipMemcpyDeviceToHost, s));
-  QTG_HIP_TRY(hipStreamSynchronize(s));
-  const cd num(e[0], e[1]), den(e[2], e[3]);
-  const double E = (num / den).real(), Nrm = den.real();
-  if (energy) *energy = E;
-  if (!d_grad64_) {
-    if ((rc = alloc((void**)&d_grad64_, sizeof(double) * 2 * tree_.total))) return rc;
-  }
-  for (size_t i = 1; i < h_combtGE_.size(); i += 2) h_combtGE_[i].cre = -E;
-  QTG_HIP_TRY(hipMemcpy(d_combtGE_, h_combtGE_.data(), sizeof(CombTerm) * h_combtGE_.size(), hipMemcpyHostToDevice));
-  void* bases[B_COUNT] = {d_psi32_, d_work32_, d_pool_, d_psi64_, d_apply64_, d_grad64_, d_pool64_, d_work_};
-  if ((rc = launch_combine<double>(d_combG1_, (int64_t)h_combG1_.size(), d_combtG1_, d_pool_, s))) return rc;
-  if ((rc = launch_combine<double>(d_combG2_, (int64_t)h_combG2_.size(), d_combtG2_, d_pool_, s))) return rc;
-  if ((rc = launch_combine<double>(d_combG3_, (int64_t)h_combG3_.size(), d_combtG3_, d_pool_, s))) return rc;
-  if ((rc = run_launches(gGradPre_, bases, nullptr, 1.0, 0.0, s))) return rc;
-  for (size_t d = 1; d < gGradDepth_.size(); ++d)
-    if ((rc = run_launches(gGradDepth_[d], bases, nullptr, 1.0, 0.0, s))) return rc;
-  if ((rc = launch_combine<double>(d_combGE_, (int64_t)h_combGE_.size(), d_combtGE_, d_pool_, s))) return rc;
-  if ((rc = run_launches(gGradFin_, bases, nullptr, 1.0 / Nrm, 0.0, s))) return rc;
-  if (grad_host) {
-    QTG_HIP_TRY(hipMemcpyAsync(grad_host, d_grad64_, sizeof(double) * 2 * tree_.total, hipMemcpyDeviceToHost, s));
-    QTG_HIP_TRY(hipStreamSynchronize(s));
-  }
-  return QTG_OK;
-}
-
 
 }  // namespace qtg

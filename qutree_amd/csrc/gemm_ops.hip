@@ -262,10 +262,10 @@ __global__ void __launch_bounds__(64 * W)
     d = descs[wgdesc ? task_desc(wgdesc, task) : task];
     mine = !(active && d.node >= 0 && !active[d.node]);
   }
+  const cplx<T>* X = d.xsel ? X1 : X0;
   GmTile g{0, 0, 0};
   if (mine) g = gm_tile(d, task - d.tile0);
   const int split = mine && d.ksplit > 1 ? d.ksplit : 1;
-  const cplx<T>* X = d.xsel ? X1 : X0;
   cplx<TO> v[4];
   if (mine && d.op == GM_COPY) {
     gm_copy<T, TO>(d, g, X, O, sre, sim);

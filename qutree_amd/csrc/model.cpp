@@ -325,8 +325,29 @@ void gram_schmidt(cd* A, int before, int n) {
   }
 }
 
+namespace {
+// an mt19937 that counts its outputs, so a caller can continue the stream
+struct CountingGen {
+  typedef std::mt19937::result_type result_type;
+  std::mt19937& g;
+  int64_t n = 0;
+  static constexpr result_type min() { return std::mt19937::min(); }
+  static constexpr result_type max() { return std::mt19937::max(); }
+  result_type operator()() {
+    ++n;
+    return g();
+  }
+};
+}  // namespace
+
 void fill_random(const HTree& tree, uint64_t seed, bool delta_lowest, std::vector<cd>& psi) {
-  std::mt19937 gen((std::mt19937::result_type)seed);
+  fill_random_stream(tree, seed, 0, delta_lowest, psi);
+}
+
+int64_t fill_random_stream(const HTree& tree, uint64_t seed, int64_t discard, bool delta_lowest, std::vector<cd>& psi) {
+  std::mt19937 mt((std::mt19937::result_type)seed);
+  mt.discard((unsigned long long)discard);
+  CountingGen gen{mt};
   psi.assign((size_t)tree.total, 0.0);
   for (const HNode& n : tree.nodes) {
     cd* A = psi.data() + n.off;
@@ -350,6 +371,7 @@ void fill_random(const HTree& tree, uint64_t seed, bool delta_lowest, std::vecto
     }
     gram_schmidt(A, L, n.n);
   }
+  return gen.n;
 }
 
 }  // namespace qtg

@@ -66,6 +66,11 @@ int build_sop(const HTree& tree, int64_t nterms, const double* coeff, const int6
 // TensorTree(rng, tree, delta_lowest): HO ground-state SPFs at the bottom,
 // uniform[-1,1] real coefficients above, Gram-Schmidt per node.
 void fill_random(const HTree& tree, uint64_t seed, bool delta_lowest, std::vector<cd>& psi);
+// The same drawn from mt19937(seed) after `discard` outputs; returns the
+// number of outputs it drew (TensorTree(mt19937&, tree, delta_lowest) on a
+// generator that earlier states already drew from).
+int64_t fill_random_stream(const HTree& tree, uint64_t seed, int64_t discard, bool delta_lowest,
+                           std::vector<cd>& psi);
 
 void gram_schmidt(cd* A, int before, int n);
 

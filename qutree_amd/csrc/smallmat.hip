@@ -192,7 +192,7 @@ constexpr int kQrWaves = kQrThreads / 64;
 // (zlarfg without its underflow rescaling).  Q = H_0 ... H_{n-1} [I; 0] by
 // backward accumulation (zung2r).  Called by all kQrThreads threads.
 namespace {
-__device__ void qr_inplace(z* W, int64_t m, int n, z* tau) {
+__device__ void qr_inplace(z* W, int64_t m, int n, z* tau, z* R) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int j = 0; j < n; ++j) {
     z* cj = W + (int64_t)j * m;
@@ -243,6 +243,15 @@ __device__ void qr_inplace(z* W, int64_t m, int n, z* tau) {
     }
     __syncthreads();
   }
+  // R (n x n, upper triangular) when asked for: the diagonal and above of
+  // the reduced matrix
+  if (R) {
+    for (int o = threadIdx.x; o < n * n; o += kQrThreads) {
+      const int i = o % n, j = o / n;
+      R[o] = i <= j ? W[i + (int64_t)j * m] : czero<double>();
+    }
+    __syncthreads();
+  }
   // thin Q, backward
   for (int i = n - 1; i >= 0; --i) {
     z* ci = W + (int64_t)i * m;
@@ -283,7 +292,7 @@ __device__ void qr_inplace(z* W, int64_t m, int n, z* tau) {
 }
 
 __global__ void __launch_bounds__(kQrThreads)
-    qr_batch_kernel(const SmItem* __restrict__ items, SmItem u, SmItem st, const z* in, z* out) {
+    qr_batch_kernel(const SmItem* __restrict__ items, SmItem u, SmItem st, const z* in, z* out, z* Rb) {
   __shared__ z tau[64];
   const SmItem it = item_of(items, u, st);
   const int64_t m = it.m;
@@ -294,7 +303,7 @@ __global__ void __launch_bounds__(kQrThreads)
     for (int64_t o = threadIdx.x; o < m * n; o += kQrThreads) W[o] = A[o];
     __syncthreads();
   }
-  qr_inplace(W, m, n, tau);
+  qr_inplace(W, m, n, tau, (Rb && it.e >= 0) ? Rb + it.e : nullptr);
 }
 
 }  // namespace
@@ -328,10 +337,10 @@ int launch_inverse_batch(const SmItem* items, const SmItem& u, const SmItem& st,
 }
 
 int launch_qr_batch(const SmItem* items, const SmItem& u, const SmItem& st, int64_t count, const void* in, void* out,
-                    hipStream_t s) {
+                    hipStream_t s, void* R) {
   if (count <= 0) return QTG_OK;
   hipLaunchKernelGGL(qr_batch_kernel, dim3((unsigned)count), dim3(kQrThreads), 0, s, items, u, st, (const z*)in,
-                     (z*)out);
+                     (z*)out, (z*)R);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -397,7 +406,9 @@ int qtg_inverse_batch_z(const double* A, double* Ainv, int64_t n, int64_t count,
 int qtg_qr_batch_z(const double* A, double* Q, int64_t m, int64_t n, int64_t count, qtg_stream stream) {
   int rc = sm_check(A, Q, m, n, count, "qr");
   if (rc) return rc;
-  return launch_qr_batch(nullptr, uniform(m, n), strides(m * n, m * n, 0), count, A, Q, S(stream));
+  SmItem u = uniform(m, n);
+  u.e = -1;  // no R
+  return launch_qr_batch(nullptr, u, strides(m * n, m * n, 0), count, A, Q, S(stream), nullptr);
 }
 
 }  // extern "C"

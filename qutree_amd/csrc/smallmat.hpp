@@ -21,8 +21,10 @@ int launch_heev_batch(const SmItem* items, const SmItem& u, const SmItem& st, in
                       const void* in, void* U, double* ev, void* R, double eps, hipStream_t s);
 int launch_inverse_batch(const SmItem* items, const SmItem& u, const SmItem& st, int64_t count, int nmax,
                          const void* in, void* out, hipStream_t s);
-// thin Q of every (m x n) item, n <= 64, m >= n; in and out may alias
+// thin Q of every (m x n) item, n <= 64, m >= n; in and out may alias.
+// With R non-null, an item with e >= 0 also writes its n x n upper
+// triangular R (A = Q R) at R + e.
 int launch_qr_batch(const SmItem* items, const SmItem& u, const SmItem& st, int64_t count, const void* in, void* out,
-                    hipStream_t s);
+                    hipStream_t s, void* R = nullptr);
 
 }  // namespace qtg

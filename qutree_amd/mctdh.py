@@ -1,0 +1,367 @@
+"""QuTree's ``mctdh`` job runner (contrib/mctdh/mctdh.cpp:148-275) on the
+device engine.
+
+A YAML file holds a ``run:`` list of jobs, executed in order on one state
+(``mctdh_state``: a tree, a Hamiltonian, named wave functions and one
+mt19937 random generator):
+
+``tree``
+    ``type: balanced`` with ``number_leaves``, ``dimension_leaves``,
+    ``dimension_nodes``, ``dimension_increment``, ``leaf_type``, ``omega``,
+    ``r0``, ``wfr0`` and the width under the reference's own key spelling
+    ``wfomoega`` (mctdh.cpp:87; so a ``wfomega`` key is ignored and the width
+    is 1), built as TreeFactory::balancedTree does (TreeFactory.cpp:52-202);
+    or ``type: compact`` with the ``Tree(istream)`` text under ``tree``.
+``hamiltonian``
+    ``name: coupled_ho``: CoupledHO (contrib/Hamiltonians/Operators/CoupledHO/
+    CoupledHO.cpp:7-66): T_k + (1/2) w^2 x_k^2 per mode, lambda^2 x_k x_{k+1}
+    around a ring, w = 4000 / 219474.6313705 and lambda = 2000 / 219474.6313705.
+``wavefunction``
+    ``type: create`` (``Hartree``, default true): TensorTree(rng, tree,
+    Hartree) from the shared generator, which keeps drawing where the last
+    state stopped; ``type: read``: the last TensorTree of ``filename``
+    (mctdh.cpp:151-162); ``type: save``: writes ``filename``.  Only the
+    state named ``Psi`` is propagated (mctdh.cpp:186-187).
+``seed``
+    ``seed``: the generator becomes mt19937(seed) (mctdh.cpp:272-275).  With
+    no seed job it is seeded from the clock, as in the reference
+    (mctdh.cpp:231); the seed used is reported.
+``cmf``
+    ``t`` (0), ``t_end`` (100 * 41.362), ``dt`` (1), ``out`` (41.362), ``cmf``
+    (1e-4), ``bs`` (1e-5), ``file_out`` ("out.dat"), ``append`` (true)
+    (mctdh.cpp:188-207): CMFIntegrator::Integrate, writing Psi to file_out at
+    the start and at every output (CMFIntegrator.cpp:46,153).
+``eigenstates``
+    The same keys: Eigenstates (Eigenstates.cpp:60-103) with
+    (t_end - t) / out iterations.
+
+Jobs outside the contraction path (``potential``/CDVR, ``scf``,
+``overlaps``, ``find_minimum``, ``cdvrtree``) raise ValueError.
+
+Usage: ``python -m qutree_amd.mctdh input.yaml``; ``run(config)`` returns the
+state with each job's results.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import workloads as W
+
+AU_PER_FS = 41.362
+CM = 219474.6313705
+
+
+# ------------------------------------------------------------------- trees
+class _Node:
+    __slots__ = ("children", "dims", "leaf_dim", "mode")
+
+    def __init__(self):
+        self.children: List["_Node"] = []
+        self.dims: List[int] = []
+        self.leaf_dim = 0
+        self.mode = -1
+
+    @property
+    def bottom(self):
+        return self.leaf_dim > 0
+
+
+def _postorder(n: _Node, out: List[_Node]) -> None:
+    # children before parents, the last child first (Node.cpp:225-239)
+    for c in reversed(n.children):
+        _postorder(c, out)
+    out.append(n)
+
+
+def balanced_tree(number_leaves: int, dimension_leaves: int = 2, dimension_nodes: int = 2,
+                  dimension_increment: int = 0, leaf_type: int = 0, omega: float = 1.0, r0: float = 0.0,
+                  wfr0: float = 0.0, wfomega: float = 1.0) -> str:
+    """TreeFactory::balancedTree (TreeFactory.cpp:52-202) as compact tree text:
+    bottom nodes of min(dim_nodes, dim_leaves) SPFs, neighbours paired layer
+    by layer (a dangling node moves up unpaired), one state at the top, SPF
+    counts dim_nodes + dim_increment * (layers above the bottom) capped at the
+    product of the children's, nodes that then contract nothing merged into
+    their parents (expandNodes), and leaf modes numbered in post-order from
+    f - 1 down (Tree::resetLeafModes, Tree.cpp:35-46)."""
+    if number_leaves < 1:
+        raise ValueError("number_leaves must be positive")
+    nodes = []
+    for _ in range(number_leaves):
+        b = _Node()
+        b.leaf_dim = dimension_leaves
+        b.dims = [dimension_leaves, min(dimension_nodes, dimension_leaves)]
+        nodes.append(b)
+    count = 0
+    while len(nodes) > 1:
+        groups = []
+        n_loop = len(nodes) // 2
+        for k in range(n_loop):
+            p = _Node()
+            p.children = [nodes[2 * k], nodes[2 * k + 1]]
+            prod = nodes[2 * k].dims[-1] * nodes[2 * k + 1].dims[-1]
+            p.dims = [c.dims[-1] for c in p.children] + [min(prod, dimension_nodes)]
+            groups.append(p)
+        n_rest = len(nodes) % 2
+        if n_loop == 0 and n_rest > 0:
+            p = _Node()
+            p.children = [nodes[0]]
+            p.dims = [nodes[0].dims[-1], min(nodes[0].dims[-1], dimension_nodes)]
+            groups.append(p)
+        else:
+            groups.extend(nodes[2 * n_loop:])
+        nodes = groups
+        count += 1
+        if count > 100:
+            raise ValueError("balanced tree: too many layers")
+    top = nodes[0]
+    top.dims[-1] = 1
+    # depth of every node below the top, and the leaf modes
+    depth: Dict[int, int] = {}
+
+    def walk(n: _Node, d: int) -> None:
+        depth[id(n)] = d
+        for c in n.children:
+            walk(c, d + 1)
+
+    walk(top, 0)
+    order: List[_Node] = []
+    _postorder(top, order)
+    f = sum(1 for n in order if n.bottom)
+    mode = f - 1
+    for n in order:
+        if n.bottom:
+            n.mode = mode
+            mode -= 1
+    nlayer = max(depth.values())
+    parent: Dict[int, tuple] = {}
+    for n in order:
+        for i, c in enumerate(n.children):
+            parent[id(c)] = (n, i)
+    for n in order:
+        if n is top or n.bottom:
+            continue
+        dim = dimension_nodes + (nlayer - depth[id(n)]) * dimension_increment
+        last_before = int(np.prod(n.dims[:-1]))
+        dim = min(dim, last_before)
+        n.dims[-1] = dim
+        p, i = parent[id(n)]
+        p.dims[i] = dim
+    # expandNodes: a non-bottom, non-top node whose SPF count equals the
+    # product of its children's dims is replaced by its children
+    changed = True
+    while changed:
+        changed = False
+        order = []
+        _postorder(top, order)
+        parent = {}
+        for n in order:
+            for i, c in enumerate(n.children):
+                parent[id(c)] = (n, i)
+        for n in order:
+            if n.bottom or n is top:
+                continue
+            if n.dims[-1] == int(np.prod(n.dims[:-1])):
+                p, i = parent[id(n)]
+                p.children = p.children[:i] + n.children + p.children[i + 1:]
+                p.dims = p.dims[:i] + n.dims[:-1] + p.dims[i + 1:]
+                changed = True
+                break
+    lines: List[str] = []
+
+    def emit(n: _Node, indent: int) -> None:
+        tab = "\t" * indent
+        if n.bottom:
+            lines.append(f"{tab}{n.dims[-1]}\t-1")
+            lines.append(f"{tab}\t{n.leaf_dim}\t{leaf_type}\t{n.mode}")
+            return
+        lines.append(f"{tab}{n.dims[-1]}\t-{len(n.children)}")
+        for c in n.children:
+            emit(c, indent + 1)
+
+    emit(top, 0)
+    for _ in range(f):
+        lines.append(f"{float(omega)!r} {float(r0)!r} {float(wfr0)!r} {float(wfomega)!r}")
+    return "\n".join(lines) + "\n"
+
+
+def tree_from_job(job: dict) -> str:
+    kind = job.get("type")
+    if kind == "balanced":
+        return balanced_tree(int(job["number_leaves"]), int(job.get("dimension_leaves", 2)),
+                             int(job.get("dimension_nodes", 2)), int(job.get("dimension_increment", 0)),
+                             int(job.get("leaf_type", 0)), float(job.get("omega", 1.0)), float(job.get("r0", 0.0)),
+                             float(job.get("wfr0", 0.0)), float(job.get("wfomoega", 1.0)))
+    if kind == "compact":
+        return str(job["tree"])
+    raise ValueError(f"tree job: type {kind!r} is not one of balanced, compact")
+
+
+def _leaf_count(tree_text: str) -> int:
+    from . import lib
+    L = lib()
+    L.qtg_host_psi_size.argtypes = [C.c_char_p]
+    L.qtg_host_psi_size.restype = C.c_int64
+    if L.qtg_host_psi_size(tree_text.encode()) < 0:
+        raise ValueError("tree text not accepted: " + L.qtg_last_error().decode())
+    # a leaf line is "dim type mode" after a "n -1" line
+    lines = [ln.split() for ln in tree_text.strip().splitlines()]
+    return sum(1 for a, b in zip(lines, lines[1:]) if len(a) == 2 and a[1] == "-1" and len(b) == 3)
+
+
+# ------------------------------------------------------------ Hamiltonians
+def coupled_ho(f: int) -> W.Hamiltonian:
+    """CoupledHO(tree) (CoupledHO.cpp:7-66)."""
+    lam, om = 2000.0 / CM, 4000.0 / CM
+    H = W.Hamiltonian()
+    for k in range(f):
+        H.add_term(1.0, [(k, W.FACTOR_KIN)])
+        H.add_term(0.5 * om * om, [(k, W.FACTOR_X2)])
+    for k in range(f):
+        H.add_term(lam * lam, [(k, W.FACTOR_X), ((k + 1) % f, W.FACTOR_X)])
+    return H
+
+
+HAMILTONIANS = {"coupled_ho": coupled_ho}
+
+
+# ------------------------------------------------------------------- state
+class State:
+    """mctdh_state (contrib/mctdh/mctdh_state.h): the tree, the Hamiltonian,
+    the named wave functions and the random generator (seed + outputs drawn)."""
+
+    def __init__(self, seed: Optional[int] = None):
+        self.tree: Optional[str] = None
+        self.H: Optional[W.Hamiltonian] = None
+        self.wavefunctions: Dict[str, np.ndarray] = {}
+        self.seed = int(time.time()) if seed is None else int(seed)
+        self.drawn = 0
+        self.results: List[dict] = []
+        self._engine = None
+
+    def engine(self):
+        from . import Engine
+        if self._engine is None:
+            if self.tree is None or self.H is None:
+                raise ValueError("a tree and a hamiltonian job must come first")
+            self._engine = Engine(self.tree, self.H)
+        return self._engine
+
+
+def _read_last_tree(tree_text: str, filename: str) -> np.ndarray:
+    from . import tensor_tree_from_bytes
+    data = open(filename, "rb").read()
+    off, psi = 0, None
+    while off < len(data):  # "read last wavefunction" (mctdh.cpp:159-161)
+        psi, used = tensor_tree_from_bytes(tree_text, data, off)
+        off += used
+    if psi is None:
+        raise ValueError(f"{filename}: no TensorTree in the file")
+    return psi
+
+
+def _ivar(job: dict) -> dict:
+    return dict(t=float(job.get("t", 0.0)), t_end=float(job.get("t_end", 100 * AU_PER_FS)),
+                out=float(job.get("out", AU_PER_FS)), dt=float(job.get("dt", 1.0)), cmf=float(job.get("cmf", 1e-4)),
+                bs=float(job.get("bs", 1e-5)), file_out=str(job.get("file_out", "out.dat")),
+                append=bool(job.get("append", True)))
+
+
+def run(config: dict, state: Optional[State] = None) -> State:
+    """Runs the jobs of a parsed configuration (``{"run": [job, ...]}``)."""
+    from . import CMF, check, lib, tensor_tree_bytes
+    jobs = config.get("run", [])
+    seed_given = any(j.get("job") == "seed" for j in jobs)
+    st = state or State(seed=None if not seed_given else 0)
+    for job in jobs:
+        kind = job.get("job")
+        if kind == "tree":
+            st.tree = tree_from_job(job)
+            st._engine = None
+        elif kind == "hamiltonian":
+            name = job.get("name")
+            if name not in HAMILTONIANS:
+                raise ValueError(f"hamiltonian job: {name!r} is not one of {sorted(HAMILTONIANS)}")
+            st.H = HAMILTONIANS[name](_leaf_count(st.tree))
+            st._engine = None
+        elif kind == "seed":
+            st.seed = int(job["seed"])
+            st.drawn = 0
+        elif kind == "wavefunction":
+            name, typ = job.get("name", "Psi"), job.get("type")
+            if typ == "create":
+                L = lib()
+                L.qtg_host_psi_size.argtypes = [C.c_char_p]
+                L.qtg_host_psi_size.restype = C.c_int64
+                size = L.qtg_host_psi_size(st.tree.encode())
+                if size < 0:
+                    check(-size)
+                out = np.zeros(2 * size)
+                L.qtg_host_fill_random_stream.argtypes = [C.c_char_p, C.c_uint64, C.c_int64, C.c_int, C.c_void_p]
+                L.qtg_host_fill_random_stream.restype = C.c_int64
+                used = L.qtg_host_fill_random_stream(st.tree.encode(), st.seed, st.drawn,
+                                                     int(bool(job.get("Hartree", True))), out.ctypes.data)
+                if used < 0:
+                    check(-used)
+                st.drawn += used
+                st.wavefunctions[name] = out.view(np.complex128).copy()
+            elif typ == "read":
+                if not job.get("filename"):
+                    raise ValueError("supply filename to save and read directive")
+                st.wavefunctions[name] = _read_last_tree(st.tree, job["filename"])
+            elif typ == "save":
+                if not job.get("filename"):
+                    raise ValueError("supply filename to save and read directive")
+                with open(job["filename"], "wb") as f:
+                    f.write(tensor_tree_bytes(st.tree, st.wavefunctions[name]))
+            else:
+                raise ValueError("wavefunction job: type must be one of read, create, save")
+        elif kind == "cmf":
+            iv = _ivar(job)
+            cmf = CMF(st.engine())
+            L = lib()
+            L.qtg_cmf_set_output_file.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+            check(L.qtg_cmf_set_output_file(cmf.ptr, iv["file_out"].encode(), int(iv["append"])))
+            r = cmf.integrate(st.wavefunctions["Psi"], iv["t"], iv["t_end"], iv["dt"], iv["out"], acc_cmf=iv["cmf"],
+                              acc_bs=iv["bs"])
+            st.wavefunctions["Psi"] = r["psi"]
+            st.results.append(dict(job="cmf", **{k: r[k] for k in ("times", "energies", "autocorr", "steps",
+                                                                       "rejected")}))
+        elif kind == "eigenstates":
+            iv = _ivar(job)
+            iters = int((iv["t_end"] - iv["t"]) / iv["out"])
+            r = CMF(st.engine()).eigenstates(st.wavefunctions["Psi"], iv["dt"], iv["out"], iters, acc_cmf=iv["cmf"],
+                                             acc_bs=iv["bs"])
+            st.wavefunctions["Psi"] = r["psi"]
+            st.results.append(dict(job="eigenstates", energies=r["energies"], converged=r["converged"]))
+        else:
+            raise ValueError(f"job {kind!r} is outside this engine's path (tree, hamiltonian, wavefunction, "
+                             "seed, cmf, eigenstates)")
+    return st
+
+
+def main(argv: List[str]) -> int:
+    import yaml
+    if len(argv) != 2:
+        print("usage: python -m qutree_amd.mctdh input.yaml", file=sys.stderr)
+        return 2
+    with open(argv[1]) as f:
+        cfg = yaml.safe_load(f)
+    st = run(cfg)
+    print(f"# rng seed {st.seed}")
+    for r in st.results:
+        if r["job"] == "cmf":
+            for t, e, a in zip(r["times"], r["energies"], r["autocorr"]):
+                print(f"{t:.6f} {e:.12f} {a.real:.12f} {a.imag:.12f}")
+        else:
+            print("levels", " ".join(f"{x:.10f}" for x in r["energies"][-1]))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))

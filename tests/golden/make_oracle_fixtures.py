@@ -59,6 +59,7 @@ def cmf64():
     print(f"cmf64: {r['steps']} steps, {len(r['times'])} outputs, {time.time() - t:.1f} s", flush=True)
     norms, ovl = node_checksums(om, r["psi"], psi0)
     top = om.nnodes - 1
+    om.build(r["psi"])  # Expectation reads the representation of the state it is given
     return dict(times=r["times"], energies=r["energies"], autocorr=r["autocorr"],
                 stats=np.array([r["steps"], r["rejected"], r["builds"], r["derivs"]], np.int64),
                 top=om.node_tensor(r["psi"], top).ravel(order="F"), norms=norms, overlaps=ovl,

@@ -18,35 +18,46 @@ dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 
-_STRICT_PATH = os.path.join(_ROOT, "oracle", "_build", "libqutree_oracle_strict.so")
-_strict = None
-_use_strict = False
+_VARIANTS = {"strict": os.path.join(_ROOT, "oracle", "_build", "libqutree_oracle_strict.so"),
+             "omp": os.path.join(_ROOT, "oracle", "_build", "libqutree_oracle_omp.so")}
+_variant_libs = {}
+_use = None
 
 
-class strict_build:
+class build_variant:
     """Within this context every oracle call (models created inside it
-    included) goes to the -O2, no-fast-math build of the same source: a
-    second summation order of the same algorithm."""
+    included) goes to another build of the same source: "strict" (-O2, no
+    fast-math: a second summation order of the same algorithm) or "omp"
+    (OpenMP over the independent nodes and SOP terms of a sweep: the
+    all-core CPU baseline)."""
+
+    def __init__(self, name):
+        assert name in _VARIANTS
+        self.name = name
 
     def __enter__(self):
-        global _use_strict
-        _use_strict = True
+        global _use
+        _use = self.name
         return self
 
     def __exit__(self, *exc):
-        global _use_strict
-        _use_strict = False
+        global _use
+        _use = None
         return False
 
 
+def strict_build():
+    return build_variant("strict")
+
+
 def lib():
-    global _lib, _strict
-    if _use_strict:
-        if _strict is None:
-            if not os.path.exists(_STRICT_PATH):
+    global _lib
+    if _use is not None:
+        if _use not in _variant_libs:
+            if not os.path.exists(_VARIANTS[_use]):
                 subprocess.check_call(["make", "-s", "-C", os.path.join(_ROOT, "oracle")])
-            _strict = _bind(C.CDLL(_STRICT_PATH))
-        return _strict
+            _variant_libs[_use] = _bind(C.CDLL(_VARIANTS[_use]))
+        return _variant_libs[_use]
     if _lib is not None:
         return _lib
     if not os.path.exists(_LIB_PATH):
@@ -66,6 +77,7 @@ def _bind(L):
     L.orc_microbench_batch.argtypes = [C.c_long, lp, lp, lp, lp, dp, dp, dp, dp, dp]
     L.orc_gram_schmidt.argtypes = [dp, C.c_long, C.c_long]
     L.orc_heev.argtypes = [dp, C.c_long, dp, dp]
+    L.orc_threads.restype = C.c_int
     L.orc_rmatvec.argtypes = L.orc_matvec.argtypes
     L.orc_trmatvec.argtypes = L.orc_matvec.argtypes
     L.orc_inverse.argtypes = [dp, C.c_long, dp]

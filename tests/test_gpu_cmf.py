@@ -144,3 +144,28 @@ def test_cmf8_reduced_precision_against_oracle(precision, tol):
     de, da = report(ref, got)
     assert got["steps"] == ref["steps"]
     assert de < tol and da < tol
+
+
+def test_cmf8_global_memory_orthogonal_matches_oracle():
+    """orthogonal() on node tensors too large for LDS runs in global memory
+    with the square roots pushed by grouped mode products; forced here on
+    the 8-mode tree, it must give the oracle's trajectory like the LDS
+    kernel does (random start, t = 1)."""
+    import os
+    text, H = W.cmf8_model()
+    print("case cmf cmf8 global-memory orthogonal", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(2024, delta_lowest=False)
+    ref = om.cmf(psi, 0.0, 1.0, 0.1, 0.1)
+    eng = Q.Engine(text, H)
+    os.environ["QTG_ORTHO_GLOBAL"] = "1"
+    try:
+        cmf = Q.CMF(eng)
+    finally:
+        del os.environ["QTG_ORTHO_GLOBAL"]
+    got = cmf.integrate(psi, 0.0, 1.0, 0.1, 0.1)
+    de, da = report(ref, got)
+    assert got["steps"] == ref["steps"] and got["derivs"] == ref["derivs"]
+    assert de < 1e-12 and da < 1e-12
+    top = om.nnodes - 1
+    assert np.max(np.abs(om.node_tensor(got["psi"], top) - om.node_tensor(ref["psi"], top))) < 1e-11

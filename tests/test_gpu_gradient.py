@@ -91,3 +91,104 @@ def test_gradient_descent_lowers_energy():
     energies.append(eng.energy())
     print(f"  energies {energies[0]:.8f} -> {energies[-1]:.8f}", flush=True)
     assert all(b < a for a, b in zip(energies, energies[1:]))
+
+
+def _perturbed_state(eng, seed, amp=0.05):
+    """A random orthonormal-gauge state moved off the gauge by a random
+    perturbation (what a gradient step leaves behind)."""
+    psi = eng.fill_random(seed, delta_lowest=False)
+    g = np.random.default_rng(seed)
+    return psi + amp * (g.normal(size=psi.size) + 1j * g.normal(size=psi.size))
+
+
+@pytest.mark.parametrize("name", ["f8quartic", "f8big"])
+def test_forward_backward_pair(name):
+    """forward() then backward() equals gradient() bit for bit and the
+    oracle's EnergyGradient to 1e-11; with grad_recompute=1, backward
+    recomputes the matrices from the state it is given even after sweeps at
+    another state overwrote them, and again equals the stored result bit for
+    bit."""
+    text, H = _models()[name]
+    print(f"case forward/backward {name}", flush=True)
+    om = O.OracleModel(text, H)
+    psi = om.fill_random(21, delta_lowest=False)
+    other = om.fill_random(22, delta_lowest=False)
+    ref, e_ref = om.gradient(psi)
+    eng = Q.Engine(text, H)
+    eng.set_psi(psi)
+    g0, e0 = eng.gradient()
+    eng.set_psi(psi)
+    e1 = eng.forward()
+    g1 = eng.backward()
+    assert e1 == e0 and np.array_equal(g1, g0)
+    assert abs(e1 - e_ref) < 1e-12 * abs(e_ref)
+    assert np.max(np.abs(g1 - ref)) < 1e-11 * np.max(np.abs(ref))
+    # stale matrices: a sweep at another state in between
+    eng.set_option("grad_recompute", 1)
+    eng.set_psi(psi)
+    eng.forward()
+    eng.set_psi(other)
+    eng.derivative()
+    eng.set_psi(psi)
+    g2 = eng.backward()
+    assert np.array_equal(g2, g0)
+    # without recompute the same sequence uses the other state's matrices
+    eng.set_option("grad_recompute", 0)
+    eng.set_psi(psi)
+    eng.forward()
+    eng.set_psi(other)
+    eng.derivative()
+    eng.set_psi(psi)
+    g3 = eng.backward()
+    assert not np.allclose(g3, g0)
+
+
+@pytest.mark.parametrize("name", ["cmf8", "f8big"])
+def test_device_qr_gauge_matches_host(name):
+    """The device QR gauge equals qutree_amd.gauge.canonicalize_all (numpy
+    QR, the same Householder convention) node for node, and leaves the
+    represented state unchanged (same energy)."""
+    text, H = _models()[name]
+    print(f"case device canonicalize {name}", flush=True)
+    eng = Q.Engine(text, H)
+    nodes = [dict(n, address=i) for i, n in enumerate(eng.nodes)]
+    psi = _perturbed_state(eng, 31)
+    ref = canonicalize_all(psi.copy(), nodes)
+    eng.set_psi(psi)
+    got = eng.canonicalize()
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+    for a, n in enumerate(eng.nodes):
+        if n["parent"] >= 0:
+            A = eng.node_tensor(got, a).reshape(-1, n["dims"][-1], order="F")
+            assert np.max(np.abs(A.conj().T @ A - np.eye(A.shape[1]))) < 1e-13
+
+
+def test_device_gradient_descent_matches_host_steps():
+    """Five device steps equal five host steps (gauge.gradient_descent:
+    engine gradient, numpy axpy, numpy QR gauge) to 1e-10 in the energies."""
+    text, H = W.grad32_model()
+    print("case device gradient descent vs host, grad32, 5 steps", flush=True)
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(32, delta_lowest=False)
+    from qutree_amd.gauge import gradient_descent
+    _, e_host = gradient_descent(eng, psi, 5, 0.05)
+    _, e_dev = eng.gradient_descent(psi, 5, 0.05)
+    print(f"  host {e_host}\n  dev  {list(e_dev)}", flush=True)
+    assert np.max(np.abs(np.array(e_host) - e_dev) / np.abs(e_dev)) < 1e-10
+
+
+def test_device_gradient_descent_200_steps():
+    """cfg4's 200 gradient-descent steps on the device: the energy falls at
+    every step, and the run takes well under the 2.4 s of the host loop."""
+    import time
+    text, H = W.grad32_model()
+    print("case device gradient descent grad32, 200 steps", flush=True)
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(32, delta_lowest=False)
+    eng.gradient_descent(psi, 1, 0.05)  # warm-up (graph capture)
+    t0 = time.perf_counter()
+    _, en = eng.gradient_descent(psi, 200, 0.05)
+    dt = time.perf_counter() - t0
+    print(f"  {dt:.3f} s, E {en[0]:.8f} -> {en[-1]:.8f}", flush=True)
+    assert np.all(np.diff(en) < 0)
+    assert dt < 2.4

@@ -1,0 +1,70 @@
+"""The mctdh runner's cmf job on the device: outputs and final state equal
+the oracle's CMF from the same seeded state, the TensorTree file receives
+the state at the start and at every output (appended), a read job resumes
+from the last tree, and append: false keeps the reference's in-place,
+no-create file mode (TensorTree_Implementation.h:93-100)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from qutree_amd import mctdh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def config(text, file_out, append=True, t_end=60.0):
+    return {"run": [{"job": "tree", "type": "compact", "tree": text}, {"job": "hamiltonian", "name": "coupled_ho"},
+                    {"job": "seed", "seed": 17},
+                    {"job": "wavefunction", "name": "Psi", "type": "create", "Hartree": False},
+                    {"job": "cmf", "t": 0.0, "t_end": t_end, "dt": 1.0, "out": 20.0, "file_out": file_out,
+                     "append": append}]}
+
+
+def test_cmf_job_matches_oracle_and_writes_every_output(tmp_path):
+    print("case mctdh cmf job", flush=True)
+    text = M.balanced_tree(8, 8, 4, 0, 0, 4000.0 / M.CM, 0.0, 1.0, 1.0)
+    f = str(tmp_path / "out.dat")
+    st = M.run(config(text, f))
+    r = st.results[-1]
+    om = O.OracleModel(text, M.coupled_ho(8))
+    psi0 = om.fill_random(17, delta_lowest=False)
+    ref = om.cmf(psi0, 0.0, 60.0, 1.0, 20.0)
+    assert len(r["times"]) == len(ref["times"]) == 4
+    assert np.max(np.abs(r["energies"] - ref["energies"]) / np.abs(ref["energies"])) < 1e-12
+    assert np.max(np.abs(r["autocorr"] - ref["autocorr"])) < 1e-12
+    # the file holds the start and the three outputs; its last tree is the
+    # final state
+    from qutree_amd import tensor_tree_from_bytes
+    data = open(f, "rb").read()
+    trees, off = [], 0
+    while off < len(data):
+        psi, used = tensor_tree_from_bytes(text, data, off)
+        trees.append(psi)
+        off += used
+    assert len(trees) == 4
+    assert np.array_equal(trees[0], psi0) or np.max(np.abs(trees[0] - psi0)) < 1e-14
+    assert np.array_equal(trees[-1], st.wavefunctions["Psi"])
+    # resume: a read job takes the last tree
+    st2 = M.run({"run": config(text, f)["run"][:2] + [{"job": "wavefunction", "name": "Psi", "type": "read",
+                                                        "filename": f}]})
+    assert np.array_equal(st2.wavefunctions["Psi"], st.wavefunctions["Psi"])
+
+
+def test_cmf_job_without_append_keeps_the_reference_file_mode(tmp_path):
+    print("case mctdh cmf job append false", flush=True)
+    text = M.balanced_tree(4, 6, 3, 0, 0, 4000.0 / M.CM, 0.0, 1.0, 1.0)
+    missing = str(tmp_path / "missing.dat")
+    M.run(config(text, missing, append=False, t_end=20.0))
+    assert not os.path.exists(missing)  # ios::in without app does not create
+    existing = str(tmp_path / "existing.dat")
+    with open(existing, "wb") as fh:
+        fh.write(b"\xab" * 100000)
+    st = M.run(config(text, existing, append=False, t_end=20.0))
+    data = open(existing, "rb").read()
+    assert len(data) == 100000  # overwritten from the start, not truncated
+    from qutree_amd import tensor_tree_from_bytes
+    psi, used = tensor_tree_from_bytes(text, data, 0)
+    assert np.array_equal(psi, st.wavefunctions["Psi"])
+    assert data[used:] == b"\xab" * (100000 - used)
