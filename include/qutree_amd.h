@@ -154,6 +154,19 @@ int qtg_reg_inverse_batch_z(const double* A, double* R, int64_t n, int64_t count
 int qtg_inverse_batch_z(const double* A, double* Ainv, int64_t n, int64_t count, qtg_stream stream);
 int qtg_qr_batch_z(const double* A, double* Q, int64_t m, int64_t n, int64_t count, qtg_stream stream);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc).  The same grouped
+ * hole contractions as qtg_hole_contraction_plan_create + qtg_plan_execute
+ * (complex128; X from A_base, Y from B_base, offsets in complex elements),
+ * but the descriptor tables are staged in a fixed device array of the library
+ * (about 1.3 MB per GPU, held until the process ends) instead of a plan's
+ * allocation: at most 4096 descriptors and 2^18 16 x 16 output tiles
+ * (QTG_ESHAPE beyond).  Allocates no device memory; returns when the result
+ * is written.  Every call on one GPU shares that array, so calls that overlap
+ * on one GPU (other host threads) race. */
+int qtg_hole_contraction_grouped_noalloc_z(const qtg_hole_contraction_desc* descs, int64_t count, double* S_base,
+                                           const double* A_base, const double* B_base, qtg_stream stream);
+
 /* ------------------------------------------------------------- tree engine */
 /* A device-resident model of one tree and one sum-of-products Hamiltonian.
  * The tree is QuTree's compact text (Tree(istream), src/TreeShape/Node.cpp:

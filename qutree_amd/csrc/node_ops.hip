@@ -8,6 +8,7 @@
 // their workgroup, in chunk order (deterministic).
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -125,6 +126,53 @@ int upload(void** dst, const std::vector<X>& v) {
   if (v.empty()) return QTG_OK;
   QTG_HIP_TRY(hipMalloc(dst, sizeof(X) * v.size()));
   QTG_HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
+  return QTG_OK;
+}
+
+// ------------------------------------------------ no-allocation variant
+// Device-resident staging of qtg_hole_contraction_grouped_noalloc_z: the
+// converted descriptors, then the per-task descriptor index table.  Each GPU
+// holds its own copy for the life of the process.
+constexpr int64_t kNoAllocMaxDescs = 4096;
+constexpr int64_t kNoAllocMaxTasks = 1 << 18;
+constexpr size_t kNoAllocTableOff = (size_t)kNoAllocMaxDescs * sizeof(GcDesc);
+constexpr size_t kNoAllocBytes = kNoAllocTableOff + (size_t)kNoAllocMaxTasks * sizeof(int32_t);
+static_assert(kNoAllocTableOff % 256 == 0, "table alignment");
+
+}  // namespace
+}  // namespace qtg
+
+// In host code this name is only a handle for hipGetSymbolAddress: kernels
+// receive the per-device address it returns (scratch_address).
+static __device__ __align__(256) unsigned char scratch[qtg::kNoAllocBytes];
+
+namespace qtg {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device with
+// that device current.
+int scratch_address(void** out) {
+  int dev = 0;
+  QTG_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("no-allocation contraction: device index out of range");
+    return QTG_EDEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("no-allocation contraction: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      return QTG_EDEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
   return QTG_OK;
 }
 
@@ -303,6 +351,51 @@ int qtg_plan_execute(qtg_plan plan, void* out_base, const void* first_base, cons
   const auto* wg = (const int32_t*)p->d_wg;
   return p->c64 ? launch_gc<float, float>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, nullptr, s)
                 : launch_gc<double, double>(g, wg, p->ndesc, p->ntasks, first_base, second_base, out_base, nullptr, s);
+}
+
+int qtg_hole_contraction_grouped_noalloc_z(const qtg_hole_contraction_desc* descs, int64_t count, double* S_base,
+                                           const double* A_base, const double* B_base, qtg_stream stream) {
+  if ((!descs && count > 0) || count < 0 || !S_base || !A_base || !B_base) {
+    set_error("no-allocation contraction: bad arguments");
+    return QTG_EARG;
+  }
+  if (count > kNoAllocMaxDescs) {
+    set_error("no-allocation contraction: more than " + std::to_string(kNoAllocMaxDescs) + " descriptors");
+    return QTG_ESHAPE;
+  }
+  if (count == 0) return QTG_OK;
+  // the same descriptors, K splits and task order as a plan of these descs
+  std::vector<GcDesc> gv;
+  gv.reserve((size_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    GcDesc g;
+    int rc = to_gc(descs[i], g);
+    if (rc) return rc;
+    const int64_t K = (int64_t)g.bef * g.aft;
+    if (K > 2 * kSplitK) g.ksplit = split_chunks(K, kGemmWaves, 1);
+    gv.push_back(g);
+  }
+  const int64_t tasks = order_tasks(gv, gc_tasks);
+  if (tasks > kNoAllocMaxTasks) {
+    set_error("no-allocation contraction: more than " + std::to_string(kNoAllocMaxTasks) + " 16 x 16 tiles");
+    return QTG_ESHAPE;
+  }
+  std::vector<int32_t> wg;
+  gemm_wg_table(gv.data(), (int64_t)gv.size(), tasks, wg);
+  void* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  hipStream_t s = S(stream);
+  GcDesc* d_descs = reinterpret_cast<GcDesc*>(base);
+  int32_t* d_wg = reinterpret_cast<int32_t*>(static_cast<unsigned char*>(base) + kNoAllocTableOff);
+  QTG_HIP_TRY(hipMemcpyAsync(d_descs, gv.data(), sizeof(GcDesc) * gv.size(), hipMemcpyHostToDevice, s));
+  QTG_HIP_TRY(hipMemcpyAsync(d_wg, wg.data(), sizeof(int32_t) * wg.size(), hipMemcpyHostToDevice, s));
+  rc = launch_gc<double, double>(d_descs, d_wg, count, tasks, A_base, B_base, S_base, nullptr, s);
+  if (rc) return rc;
+  // the staging tables are host temporaries and the device copy is shared by
+  // every call on this GPU: the call returns with the result written
+  QTG_HIP_TRY(hipStreamSynchronize(s));
+  return QTG_OK;
 }
 
 int qtg_plan_destroy(qtg_plan plan) {
