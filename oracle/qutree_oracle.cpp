@@ -1660,6 +1660,340 @@ EigenResult eigenstates(TensorTree& P, const SOP& H, const Tree& t, double dt, d
   return res;
 }
 
+// ---------------------------------------------------------------- SCF
+// Krylov-sweep SCF ground-state search (SCF.cpp:36-348) on the correction
+// matrices of HamiltonianRepresentation(H, tree, tree, tail = false)
+// (HamiltonianRepresentation.h:19-36,90-177, HamiltonianRepresentation.cpp:
+// 119-218, ActiveCounter.h:16-33).  Sparse trees are tail-cut: a term's
+// tree ends at its first node with more than one active child.
+struct ScfRep {
+  vector<SparseMatrixTree> hMats, hCon;
+  MatrixTree hCorr, hConCorr;
+  vector<vector<size_t>> nAct;  // [node][term]
+};
+
+static bool mloActs(const MLO& M, size_t mode) {
+  return std::find(M.modes.begin(), M.modes.end(), mode) != M.modes.end();
+}
+
+// nActives (HamiltonianRepresentation.cpp:121-138)
+static size_t scfActives(const SparseMatrixTree& hm, const MLO& M, const SparseMatrixTree& hc, const Node& n) {
+  size_t c = 0;
+  if (n.bottom) {
+    if (mloActs(M, n.leaf.mode)) ++c;
+  } else {
+    for (int ch : n.children)
+      if (hm.active(ch)) ++c;
+  }
+  if (n.isTop()) return c;
+  if (hc.active(n.address)) ++c;
+  return c;
+}
+
+ScfRep makeScfRep(const SOP& H, const Tree& t) {
+  ScfRep r;
+  for (auto& M : H.terms) {
+    SparseTree st = makeSparseTree(M.targetLeaves(), t, false);
+    r.hMats.push_back(makeSMT(st, t));
+    r.hCon.push_back(makeSMT(st, t));
+  }
+  for (auto& n : t.nodes) {
+    const size_t d = n.shape.lastDim();
+    r.hCorr.emplace_back(d, d);
+    r.hConCorr.emplace_back(d, d);
+    vector<size_t> na(H.terms.size());
+    for (size_t l = 0; l < H.terms.size(); ++l) na[l] = scfActives(r.hMats[l], H.terms[l], r.hCon[l], n);
+    r.nAct.push_back(na);
+  }
+  return r;
+}
+
+static size_t parentIdxOf(const Node& n) { return n.shape.lastIdx(); }
+
+// buildUpCorrection (HamiltonianRepresentation.cpp:140-168)
+static void scfUpCorrection(ScfRep& r, const SOP& H, const TensorTree& P, const Tree& t, const Node& n) {
+  const size_t d = n.shape.lastDim();
+  r.hCorr[n.address] = Mat(d, d);
+  if (!n.bottom) {
+    Tensor hA(n.shape);
+    for (size_t k = 0; k < n.children.size(); ++k) {
+      Tensor x = matrixTensor(r.hCorr[n.children[k]], P[n.address], k);
+      for (size_t i = 0; i < hA.c.size(); ++i) hA[i] += x[i];
+    }
+    r.hCorr[n.address] = contraction(P[n.address], hA, parentIdxOf(n));
+  }
+  if (!n.isTop()) {
+    for (size_t l = 0; l < H.terms.size(); ++l) {
+      if (r.nAct[n.parent][l] != 1) continue;
+      if (r.hMats[l].active(n.address)) {
+        const Mat& h = r.hMats[l][n.address];
+        for (size_t i = 0; i < h.c.size(); ++i) r.hCorr[n.address].c[i] += H.coeff[l] * h.c[i];
+      }
+    }
+  }
+}
+
+// buildDownCorrection (HamiltonianRepresentation.cpp:170-207)
+static void scfDownCorrection(ScfRep& r, const SOP& H, const TensorTree& P, const Tree& t, const Node& hole) {
+  if (hole.isTop()) return;
+  const Node& n = t.nodes[hole.parent];
+  const Tensor& A = P[n.address];
+  Tensor hA(n.shape);
+  for (size_t l = 0; l < H.terms.size(); ++l) {
+    if (r.nAct[n.address][l] < 2) continue;
+    if (r.hMats[l].active(hole.address)) continue;
+    // applyHole (SparseMatrixTreeFunctions_Implementation.h:355-374)
+    Tensor tmp = A;
+    for (size_t k = 0; k < n.children.size(); ++k) {
+      if ((int)k == hole.childIdx || !r.hMats[l].active(n.children[k])) continue;
+      tmp = matrixTensor(r.hMats[l][n.children[k]], tmp, k);
+    }
+    if (!n.isTop() && r.hCon[l].active(n.address)) tmp = matrixTensor(r.hCon[l][n.address], tmp, parentIdxOf(n));
+    for (size_t i = 0; i < hA.c.size(); ++i) hA[i] += H.coeff[l] * tmp[i];
+  }
+  for (size_t k = 0; k < n.children.size(); ++k) {
+    if ((int)k == hole.childIdx) continue;
+    Tensor x = matrixTensor(r.hCorr[n.children[k]], A, k);
+    for (size_t i = 0; i < hA.c.size(); ++i) hA[i] += x[i];
+  }
+  if (!n.isTop()) {
+    Tensor x = matrixTensor(r.hConCorr[n.address], A, parentIdxOf(n));
+    for (size_t i = 0; i < hA.c.size(); ++i) hA[i] += x[i];
+  }
+  r.hConCorr[hole.address] = contraction(A, hA, hole.childIdx);
+}
+
+// buildUp (HamiltonianRepresentation.h:90-105): representLayer of every
+// term (SparseMatrixTreeFunctions_Implementation.h:18-52) and the up
+// correction.
+static void scfBuildUp(ScfRep& r, const SOP& H, const TensorTree& P, const Tree& t, const Node& n) {
+  for (size_t l = 0; l < H.terms.size(); ++l) {
+    SparseMatrixTree& hm = r.hMats[l];
+    if (!hm.active(n.address)) continue;
+    Tensor hK = n.bottom ? mloApply(H.terms[l], P[n.address], n.leaf)
+                         : applyNode(hm, nullptr, P[n.address], t, n, (int)parentIdxOf(n));
+    hm[n.address] = contraction(P[n.address], hK, parentIdxOf(n));
+  }
+  scfUpCorrection(r, H, P, t, n);
+}
+
+// buildDown (HamiltonianRepresentation.h:107-129): contractionLayer of every
+// term active at the node (SparseMatrixTreeFunctions_Implementation.h:
+// 145-190) and the down correction.
+static void scfBuildDown(ScfRep& r, const SOP& H, const TensorTree& P, const Tree& t, const Node& n) {
+  if (!n.isTop()) {
+    for (size_t l = 0; l < H.terms.size(); ++l) {
+      SparseMatrixTree& hc = r.hCon[l];
+      if (!hc.active(n.address)) continue;
+      const Node& p = t.nodes[n.parent];
+      Tensor hK = applyNode(r.hMats[l], &hc, P[p.address], t, p, n.childIdx);
+      hc[n.address] = contraction(P[p.address], hK, n.childIdx);
+    }
+  }
+  scfDownCorrection(r, H, P, t, n);
+}
+
+// apply(HA, A, hMats, hCons, hCorr, hConCorr, ...) (SCF.cpp:36-81)
+Tensor scfApply(const ScfRep& r, const SOP& H, const Tensor& A, const Tree& t, const Node& n) {
+  Tensor HA(A.s);
+  for (size_t l = 0; l < H.terms.size(); ++l) {
+    Tensor hA;
+    if (n.bottom) {
+      if (!(r.hMats[l].active(n.address) && r.hCon[l].active(n.address))) continue;
+      Tensor MA = mloApply(H.terms[l], A, n.leaf);
+      hA = matrixTensor(r.hCon[l][n.address], MA, parentIdxOf(n));
+    } else {
+      if (r.nAct[n.address][l] <= 1) continue;
+      hA = applyNode(r.hMats[l], &r.hCon[l], A, t, n, -1);
+    }
+    for (size_t i = 0; i < HA.c.size(); ++i) HA[i] += H.coeff[l] * hA[i];
+  }
+  if (!n.bottom) {
+    for (size_t k = 0; k < n.children.size(); ++k) {
+      Tensor x = matrixTensor(r.hCorr[n.children[k]], A, k);
+      for (size_t i = 0; i < HA.c.size(); ++i) HA[i] += x[i];
+    }
+  }
+  if (!n.isTop()) {
+    Tensor x = matrixTensor(r.hConCorr[n.address], A, parentIdxOf(n));
+    for (size_t i = 0; i < HA.c.size(); ++i) HA[i] += x[i];
+  }
+  return HA;
+}
+
+// fullContraction = trace(dotProduct(A, B)) (SCF.cpp:97-100)
+static cd fullContraction(const Tensor& A, const Tensor& B) {
+  Mat S = dotProduct(A, B);
+  cd tr = 0.;
+  for (size_t i = 0; i < S.d1; ++i) tr += S(i, i);
+  return tr;
+}
+
+static double scfNormalize(Tensor& A) {  // SCF.cpp:102-107
+  const double nrm = std::sqrt(std::real(fullContraction(A, A)));
+  for (auto& x : A.c) x /= nrm;
+  return nrm;
+}
+
+struct Krylov {
+  vector<Tensor> space;
+  Mat U;
+  vector<double> E;
+};
+
+// solveKrylovSpace (SCF.cpp:134-196): Lanczos without re-orthogonalisation,
+// k = min(nKrylov, node size) vectors; the k x k tridiagonal matrix is
+// diagonalised with eigenvalues ascending and each eigenvector phased so that
+// its first component is real and non-negative (the eigensolver's sign is a
+// free choice: the reference's sweep is defined up to it, see scfSweep).
+Krylov solveKrylov(const ScfRep& r, const SOP& H, Tensor Psi, const Tree& t, const Node& n, size_t k) {
+  Krylov K;
+  K.space.assign(k, Tensor(Psi.s));
+  scfNormalize(Psi);
+  K.space[0] = Psi;
+  vector<double> alpha, beta;
+  Tensor HPsi = scfApply(r, H, Psi, t, n);
+  alpha.push_back(std::real(fullContraction(Psi, HPsi)));
+  for (size_t i = 0; i < Psi.c.size(); ++i) Psi[i] = HPsi[i] - alpha.back() * Psi[i];
+  for (size_t l = 1; l < k; ++l) {
+    beta.push_back(scfNormalize(Psi));
+    K.space[l] = Psi;
+    HPsi = scfApply(r, H, Psi, t, n);
+    alpha.push_back(std::real(fullContraction(Psi, HPsi)));
+    const Tensor& v = K.space[l];
+    const Tensor& vl = K.space[l - 1];
+    for (size_t i = 0; i < Psi.c.size(); ++i) Psi[i] = HPsi[i] - alpha.back() * v[i] - beta.back() * vl[i];
+  }
+  Mat T(k, k);
+  for (size_t l = 0; l < k; ++l) {
+    T(l, l) = alpha[l];
+    if (l == k - 1) break;
+    T(l + 1, l) = beta[l];
+    T(l, l + 1) = beta[l];
+  }
+  heev(T, K.U, K.E);
+  return K;
+}
+
+// imaginaryTimePropagation (SCF.cpp:198-222).  The ground-state weight
+// enters twice, as in the reference: once before the sum over k and once at
+// k = 0 inside it.
+static void scfItp(Tensor& Psi, const Krylov& K, double beta) {
+  const size_t k = K.E.size();
+  vector<cd> c(k, 0.);
+  for (size_t l = 0; l < k; ++l) {
+    c[l] += K.U(l, 0) * std::conj(K.U(0, 0));
+    for (size_t j = 0; j < k; ++j) {
+      const cd e = std::exp(-beta * (K.E[j] - K.E[0]));
+      c[l] += K.U(l, j) * e * std::conj(K.U(0, j));
+    }
+  }
+  Psi.zero();
+  for (size_t l = 0; l < k; ++l)
+    for (size_t i = 0; i < Psi.c.size(); ++i) Psi[i] += c[l] * K.space[l][i];
+  scfNormalize(Psi);
+}
+
+// diagonalizeKrylov (SCF.cpp:224-235) over the k vectors of this solve (the
+// reference sums over the nKrylov slots of the space, which equals k unless
+// a node has fewer than nKrylov entries).
+static void scfDiag(Tensor& Psi, const Krylov& K) {
+  Psi.zero();
+  for (size_t l = 0; l < K.E.size(); ++l)
+    for (size_t i = 0; i < Psi.c.size(); ++i) Psi[i] += K.U(l, 0) * K.space[l][i];
+}
+
+// adjacentIndex (SCF.cpp:237-256); to < 0 stands for the null pointer
+static size_t scfAdjacent(const Tree& t, const Node& from, int to) {
+  if (to < 0) return parentIdxOf(from);
+  for (size_t k = 0; k < from.children.size(); ++k)
+    if (from.children[k] == to) return k;
+  return parentIdxOf(from);
+}
+
+// qr(Tensor, mode) (Tensor_Implementation.h:864-870, toMatrix / toTensor
+// :961-1000): rows (before, after) with before fastest, columns the mode.
+Tensor qrTensor(const Tensor& A, size_t mode) {
+  const size_t bef = A.s.bef[mode], aft = A.s.aft[mode], n = A.s.d[mode];
+  Mat M(bef * aft, n);
+  for (size_t b = 0; b < bef; ++b)
+    for (size_t a = 0; a < n; ++a)
+      for (size_t f = 0; f < aft; ++f) M(b + f * bef, a) = A.c[b + bef * (a + n * f)];
+  Mat Q = householderQ(M);
+  Tensor R(A.s);
+  for (size_t b = 0; b < bef; ++b)
+    for (size_t a = 0; a < n; ++a)
+      for (size_t f = 0; f < aft; ++f) R.c[b + bef * (a + n * f)] = Q(b + f * bef, a);
+  return R;
+}
+
+// scf_sweep / addNodes (SCF.cpp:14-34): the visiting order of one iteration
+vector<int> scfSweepOrder(const Tree& t) {
+  vector<int> s;
+  std::function<void(int)> add = [&](int a) {
+    s.push_back(a);
+    const Node& n = t.nodes[a];
+    if (n.bottom) return;
+    for (int c : n.children) {
+      add(c);
+      s.push_back(a);
+    }
+  };
+  add(t.top());
+  return s;
+}
+
+// scf() (SCF.cpp:270-348).  energies receives the lowest Ritz value of
+// every Krylov solve (what the reference prints, before its unit
+// conversion).  The sweep is fixed up to the sign of each Krylov
+// ground-state eigenvector; with the phase convention of solveKrylov it is
+// deterministic.  With probe >= 0 the sweep stops before the Krylov solve of
+// microstep `probe` and returns its node address, the state then and
+// (in heff, N x N column-major) the dense effective Hamiltonian there.
+int scfSweep(const SOP& H, TensorTree& P, const Tree& t, size_t nIter, size_t nKrylov, size_t nITP, double beta,
+             vector<double>& energies, long probe, vector<cd>* heff) {
+  const vector<int> sweep = scfSweepOrder(t);
+  ScfRep r = makeScfRep(H, t);
+  for (auto& n : t.nodes)
+    if (!n.isTop()) scfBuildUp(r, H, P, t, n);
+  long step = 0;
+  for (size_t it = 0; it < nIter; ++it) {
+    for (size_t l = 0; l < sweep.size(); ++l) {
+      const Node& n = t.nodes[sweep[l]];
+      if (step++ == probe) {
+        const size_t N = n.shape.total;
+        heff->assign(N * N, 0.);
+        for (size_t j = 0; j < N; ++j) {
+          Tensor e(n.shape);
+          e[j] = 1.;
+          Tensor h = scfApply(r, H, e, t, n);
+          for (size_t i = 0; i < N; ++i) (*heff)[i + N * j] = h[i];
+        }
+        return n.address;
+      }
+      const size_t k = std::min(nKrylov, n.shape.total);
+      Krylov K = solveKrylov(r, H, P[n.address], t, n, k);
+      energies.push_back(K.E[0]);
+      if (it < nITP) scfItp(P[n.address], K, beta);
+      else scfDiag(P[n.address], K);
+      const int next = l + 1 < sweep.size() ? sweep[l + 1] : -1;
+      const size_t outIdx = scfAdjacent(t, n, next);
+      Tensor PsiW = P[n.address];
+      P[n.address] = qrTensor(P[n.address], outIdx);
+      if (next < 0) break;
+      const Node& nx = t.nodes[next];
+      Mat rm = contraction(P[n.address], PsiW, outIdx);
+      const size_t inIdx = scfAdjacent(t, nx, n.address);
+      P[next] = matrixTensor(rm, P[next], inIdx);
+      // buildSCF(H, Psi, node, next) (HamiltonianRepresentation.h:131-177)
+      const bool up = !n.isTop() ? (n.parent == next) : !(nx.parent == n.address);
+      if (up) scfBuildUp(r, H, P, t, n);
+      else scfBuildDown(r, H, P, t, nx);
+    }
+  }
+  return -1;
+}
+
 // ---------------------------------------------------------------------------
 // Model = tree + SOP (+ state), the unit the C-ABI hands around.
 // ---------------------------------------------------------------------------
@@ -2124,6 +2458,57 @@ long orc_tensor_tree_write(void* h, const double* psi, unsigned char* buf, long 
   }
   if (buf && (long)out.size() <= cap) std::memcpy(buf, out.data(), out.size());
   return (long)out.size();
+}
+
+// Krylov-sweep SCF (scfSweep): psi is updated in place; energies (cap
+// entries) receives the lowest Ritz value of every microstep.  Returns the
+// number of microsteps, -1 on error.
+long orc_scf(void* h, double* psi, long nIter, long nKrylov, long nITP, double beta, double* energies, long cap) {
+  Model* m = (Model*)h;
+  try {
+    TensorTree P = m->unpack(psi);
+    vector<double> e;
+    scfSweep(m->H, P, m->tree, (size_t)nIter, (size_t)nKrylov, (size_t)nITP, beta, e, -1, nullptr);
+    m->pack(P, psi);
+    for (long i = 0; i < (long)e.size() && i < cap; ++i) energies[i] = e[i];
+    return (long)e.size();
+  } catch (std::exception& ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
+// Runs the SCF sweep (nITP, beta as in orc_scf) up to microstep `step` and
+// stops before its Krylov
+// solve: psi receives the state then, heff (N x N complex, column-major,
+// cap complex entries) the dense effective Hamiltonian of the node it is at.
+// Returns that node's address.
+long orc_scf_probe(void* h, double* psi, long nKrylov, long nITP, double beta, long step, double* heff,
+                   long cap) {
+  Model* m = (Model*)h;
+  try {
+    TensorTree P = m->unpack(psi);
+    vector<double> e;
+    vector<cd> H;
+    const size_t iters = (size_t)(step / (long)scfSweepOrder(m->tree).size()) + 1;
+    const int a = scfSweep(m->H, P, m->tree, iters, (size_t)nKrylov, (size_t)nITP, beta, e, step, &H);
+    m->pack(P, psi);
+    for (long i = 0; i < (long)H.size() && i < cap; ++i) {
+      heff[2 * i] = H[i].real();
+      heff[2 * i + 1] = H[i].imag();
+    }
+    return a;
+  } catch (std::exception& ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
+// The node addresses one SCF iteration visits (scf_sweep); returns the count.
+long orc_scf_sweep_order(void* h, long* out, long cap) {
+  vector<int> s = scfSweepOrder(((Model*)h)->tree);
+  for (long i = 0; i < (long)s.size() && i < cap; ++i) out[i] = s[i];
+  return (long)s.size();
 }
 
 // Top-node matrices after a build from psi: H(i,j) = <Psi_i|H|Psi_j>

@@ -120,6 +120,12 @@ def _bind(L):
     L.orc_top_matrices.argtypes = [C.c_void_p, dp, dp, dp]
     L.orc_tensor_tree_write.argtypes = [C.c_void_p, dp, C.c_void_p, C.c_long]
     L.orc_tensor_tree_write.restype = C.c_long
+    L.orc_scf.argtypes = [C.c_void_p, dp, C.c_long, C.c_long, C.c_long, C.c_double, dp, C.c_long]
+    L.orc_scf.restype = C.c_long
+    L.orc_scf_probe.argtypes = [C.c_void_p, dp, C.c_long, C.c_long, C.c_double, C.c_long, dp, C.c_long]
+    L.orc_scf_probe.restype = C.c_long
+    L.orc_scf_sweep_order.argtypes = [C.c_void_p, lp, C.c_long]
+    L.orc_scf_sweep_order.restype = C.c_long
     return L
 
 
@@ -349,3 +355,31 @@ class OracleModel:
                     propagator=pe[:iterations * ns].reshape(iterations, ns),
                     converged=cv[:iterations * ns].reshape(iterations, ns), steps=int(stats[0]),
                     rejected=int(stats[1]), builds=int(stats[2]), derivs=int(stats[3]), dt_end=dte.value)
+
+    def scf_sweep_order(self):
+        buf = np.zeros(4 * self.nnodes, np.int64)
+        k = self._L.orc_scf_sweep_order(self.h, buf, len(buf))
+        return buf[:k].tolist()
+
+    def scf(self, psi, n_iter, n_krylov, n_itp=0, beta=1.0):
+        """Krylov-sweep SCF (SCF.cpp:270-348): the final state and the lowest
+        Ritz value of every microstep (n_iter x sweep length)."""
+        p = cview(psi).copy()
+        steps = len(self.scf_sweep_order())
+        e = np.zeros(max(n_iter * steps, 1))
+        k = self._L.orc_scf(self.h, p, n_iter, n_krylov, n_itp, beta, e, len(e))
+        if k < 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        return dict(psi=as_complex(p).copy(), energies=e[:k].reshape(n_iter, steps) if n_iter else e[:0])
+
+    def scf_probe(self, psi, n_krylov, step, n_itp=0, beta=1.0):
+        """State and dense effective Hamiltonian (N x N) at the node of SCF
+        microstep `step`, before its Krylov solve."""
+        p = cview(psi).copy()
+        N = max(int(np.prod(n["dims"])) for n in self.nodes)
+        h = np.zeros(2 * N * N)
+        a = self._L.orc_scf_probe(self.h, p, n_krylov, n_itp, beta, step, h, N * N)
+        if a < 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        Na = int(np.prod(self.nodes[a]["dims"]))
+        return a, as_complex(p).copy(), as_complex(h)[:Na * Na].reshape(Na, Na, order="F").copy()
