@@ -358,6 +358,47 @@ int64_t qtg_cmf_eigenstates(qtg_cmf cmf, double* psi_host, double dt, double out
                             double acc_bs, int64_t max_steps, double* energies, double* propagator,
                             double* converged, int64_t* stats, double* dt_end);
 
+/* ------------------------------------------------------------ SCF sweeps */
+/* Krylov-sweep SCF ground-state search, scf(SCF_parameters&)
+ * (Applications/SCF.cpp:270-348).  The engine's tree and Hamiltonian
+ * (complex128 engine, Hamiltonian set) are used with the correction
+ * matrices of HamiltonianRepresentation(H, tree, tree, false)
+ * (HamiltonianRepresentation.h:19-36,131-177, .cpp:119-218).
+ * qtg_scf_create plans a sweep with Krylov spaces of n_krylov (1..64)
+ * vectors, capped per node at the node tensor's size (SCF.cpp:301-302).
+ * qtg_scf_run performs n_iter iterations (SCF_parameters::nIter), the first
+ * n_itp of them imaginary-time updates with step beta (nITP, beta), the rest
+ * replacing each node by its lowest Krylov eigenvector; psi (engine layout,
+ * non-top nodes orthonormal as TensorTree(..., delta_lowest) leaves them) is
+ * the start on entry and the final state on return.  energies receives, per
+ * iteration and microstep, the lowest Ritz value in hartree (what the
+ * reference prints, before its unit conversion), at most cap values.
+ * Each Krylov ground-state eigenvector is phased with its first component
+ * real and non-negative (the reference leaves this sign to its
+ * eigensolver; energies do not depend on it).  Every node toward a
+ * neighbour must have at least as many rows as columns (QTG_ESHAPE). */
+typedef void* qtg_scf;
+int qtg_scf_create(qtg_engine engine, int64_t n_krylov, qtg_scf* scf);
+int qtg_scf_destroy(qtg_scf scf);
+/* microsteps per iteration (2 nodes - 1) */
+int64_t qtg_scf_sweep_length(qtg_scf scf);
+int qtg_scf_run(qtg_scf scf, double* psi_host, int64_t n_iter, int64_t n_itp, double beta, double* energies,
+                int64_t cap);
+/* "graph" (1: replay iterations from a captured HIP graph, the default; 0:
+ * launch eagerly); "launches" (read only via qtg_scf_info) */
+int qtg_scf_set_option(qtg_scf scf, const char* key, double value);
+/* info[0] microsteps per iteration, info[1] launches per iteration,
+ * info[2] device bytes */
+int qtg_scf_info(qtg_scf scf, int64_t* info);
+/* Host-only planning of the same sweep (no device needed): checks that every
+ * launch of the plan stays inside its buffers, that no two outputs of one
+ * launch overlap and that no launch writes what it reads.  info (6 entries):
+ * microsteps per iteration, launches per iteration, work and pool elements,
+ * mode-product and hole-contraction tasks. */
+int qtg_host_scf_plan(const char* tree_text, int64_t nterms, const double* coeff, const int64_t* term_off,
+                      const int64_t* fmode, const int64_t* fkind, const int64_t* fpool, const double* pool,
+                      int64_t n_krylov, int64_t* info);
+
 /* ---------------------------------------------------------------- formats */
 /* QuTree's binary files, byte for byte (host only; complex<double>,
  * little-endian).  TensorTree<cd>::write/read (TensorTree_Implementation.h:

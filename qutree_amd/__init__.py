@@ -91,6 +91,13 @@ def lib():
     L.qtg_engine_synchronize.argtypes = [_vp]
     L.qtg_cmf_create.argtypes = [_vp, C.POINTER(_vp)]
     L.qtg_cmf_destroy.argtypes = [_vp]
+    L.qtg_scf_create.argtypes = [_vp, C.c_int64, C.POINTER(_vp)]
+    L.qtg_scf_destroy.argtypes = [_vp]
+    L.qtg_scf_sweep_length.argtypes = [_vp]
+    L.qtg_scf_sweep_length.restype = C.c_int64
+    L.qtg_scf_run.argtypes = [_vp, _dp, C.c_int64, C.c_int64, C.c_double, _dp, C.c_int64]
+    L.qtg_scf_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
+    L.qtg_scf_info.argtypes = [_vp, _lp]
     L.qtg_cmf_integrate.argtypes = [_vp, _dp] + [C.c_double] * 8 + [_i64, _i64, _dp, _dp, _dp, _lp]
     L.qtg_cmf_integrate.restype = _i64
     L.qtg_cmf_eigenstates.argtypes = ([_vp, _dp, C.c_double, C.c_double, _i64, C.c_double, C.c_double, _i64, _dp,
@@ -432,6 +439,44 @@ class CMF:
         try:
             if self.ptr:
                 lib().qtg_cmf_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class SCF:
+    """Device Krylov-sweep SCF (qtg_scf_*, SCF.cpp:270-348) bound to a
+    complex128 Engine with its Hamiltonian set."""
+
+    def __init__(self, engine: Engine, n_krylov: int = 5):
+        p = _vp()
+        check(lib().qtg_scf_create(engine.ptr, n_krylov, C.byref(p)))
+        self.ptr = p.value
+        self.engine = engine
+        self.sweep_length = int(lib().qtg_scf_sweep_length(self.ptr))
+
+    def set_option(self, key: str, value) -> None:
+        check(lib().qtg_scf_set_option(self.ptr, key.encode(), float(value)))
+
+    def info(self):
+        v = np.zeros(3, np.int64)
+        check(lib().qtg_scf_info(self.ptr, v))
+        return dict(sweep_length=int(v[0]), launches=int(v[1]), device_bytes=int(v[2]))
+
+    def run(self, psi, n_iter, n_itp=0, beta=1.0):
+        """n_iter iterations (the first n_itp imaginary-time updates with
+        step beta): the final state and the lowest Ritz value of every
+        microstep, shape (n_iter, sweep length)."""
+        p = _ci(psi).copy()
+        e = np.zeros(max(n_iter * self.sweep_length, 1))
+        check(lib().qtg_scf_run(self.ptr, p, n_iter, n_itp, beta, e, len(e)))
+        return dict(psi=p.view(np.complex128).copy(), energies=e[:n_iter * self.sweep_length].reshape(
+            n_iter, self.sweep_length))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qtg_scf_destroy(self.ptr)
                 self.ptr = None
         except Exception:
             pass

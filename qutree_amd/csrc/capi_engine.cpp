@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "cmf.hpp"
+#include "scf.hpp"
 #include <algorithm>
 #include "ttno.hpp"
 #include "engine.hpp"
@@ -204,6 +205,25 @@ int qtg_host_plan_counts(const char* tree_text, int64_t nterms, const double* co
   rc = en.set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool);
   if (rc) return rc;
   return qtg_engine_counts(&en, sweep_flops, items);
+}
+
+int qtg_host_scf_plan(const char* tree_text, int64_t nterms, const double* coeff, const int64_t* term_off,
+                      const int64_t* fmode, const int64_t* fkind, const int64_t* fpool, const double* pool,
+                      int64_t n_krylov, int64_t* info) {
+  if (!tree_text || !info) return QTG_EARG;
+  Engine en;
+  int rc = en.create(tree_text, 0, true);
+  if (rc) return rc;
+  if ((rc = en.set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool))) return rc;
+  Scf sc(en);
+  if ((rc = sc.init(n_krylov))) return rc;
+  info[0] = sc.sweep_length();
+  info[1] = sc.launches_per_iteration();
+  info[2] = sc.work_elems();
+  info[3] = sc.pool_elems();
+  info[4] = sc.gm_tasks_total();
+  info[5] = sc.gc_tasks_total();
+  return QTG_OK;
 }
 
 int64_t qtg_host_psi_size(const char* tree_text) {
@@ -469,6 +489,54 @@ int qtg_ttno_apply(qtg_ttno ttno, const double* bra, const double* ket, double* 
                    qtg_stream stream) {
   if (!ttno || !bra || !ket) return QTG_EARG;
   return static_cast<Ttno*>(ttno)->apply(bra, ket, out, expect, S(stream));
+}
+
+int qtg_scf_create(qtg_engine e, int64_t n_krylov, qtg_scf* out) {
+  if (!e || !out) return QTG_EARG;
+  auto* sc = new Scf(*static_cast<Engine*>(e));
+  int rc = sc->init(n_krylov);
+  if (rc) {
+    delete sc;
+    return rc;
+  }
+  *out = sc;
+  return QTG_OK;
+}
+
+int qtg_scf_destroy(qtg_scf s) {
+  delete static_cast<Scf*>(s);
+  return QTG_OK;
+}
+
+int64_t qtg_scf_sweep_length(qtg_scf s) { return s ? static_cast<Scf*>(s)->sweep_length() : -QTG_EARG; }
+
+int qtg_scf_run(qtg_scf s, double* psi_host, int64_t n_iter, int64_t n_itp, double beta, double* energies,
+                int64_t cap) {
+  if (!s || !psi_host || n_iter < 0 || n_itp < 0 || cap < 0 || (cap > 0 && !energies)) {
+    set_error("scf_run: bad argument");
+    return QTG_EARG;
+  }
+  return static_cast<Scf*>(s)->run(psi_host, n_iter, n_itp, beta, energies, cap);
+}
+
+int qtg_scf_set_option(qtg_scf s, const char* key, double value) {
+  if (!s || !key) return QTG_EARG;
+  auto* sc = static_cast<Scf*>(s);
+  if (!std::strcmp(key, "graph")) {
+    sc->graph_ = value != 0.0;
+    return QTG_OK;
+  }
+  set_error(std::string("scf_set_option: unknown key ") + key);
+  return QTG_EARG;
+}
+
+int qtg_scf_info(qtg_scf s, int64_t* info) {
+  if (!s || !info) return QTG_EARG;
+  auto* sc = static_cast<Scf*>(s);
+  info[0] = sc->sweep_length();
+  info[1] = sc->launches_per_iteration();
+  info[2] = sc->device_bytes();
+  return QTG_OK;
 }
 
 }  // extern "C"

@@ -50,6 +50,7 @@ class Engine {
   double* apply64() { return d_apply64_; }
   bool complex64() const { return c64_; }
   bool planned() const { return planned_; }
+  bool host_only() const { return host_only_; }
   int64_t sandwich_items() const { return nSw_; }
   int64_t apply_terms_chunks() const { return nChunks_; }
   int64_t apply_terms() const { return (int64_t)h_terms_.size(); }

@@ -35,8 +35,14 @@ mt19937 random generator):
     The same keys: Eigenstates (Eigenstates.cpp:60-103) with
     (t_end - t) / out iterations.
 
-Jobs outside the contraction path (``potential``/CDVR, ``scf``,
-``overlaps``, ``find_minimum``, ``cdvrtree``) raise ValueError.
+``scf``
+    ``nIter`` (20), ``nKrylov`` (20), ``nITP`` (0), ``beta`` (1),
+    ``conversion`` (219474.6313705) (mctdh.cpp:205-217): the Krylov-sweep SCF
+    (SCF.cpp:270-348) on Psi; the lowest Ritz value of every microstep is
+    kept, times ``conversion`` as the reference prints it.
+
+Jobs outside the contraction path (``potential``/CDVR, ``overlaps``,
+``find_minimum``, ``cdvrtree``) raise ValueError.
 
 Usage: ``python -m qutree_amd.mctdh input.yaml``; ``run(config)`` returns the
 state with each job's results.
@@ -339,9 +345,17 @@ def run(config: dict, state: Optional[State] = None) -> State:
                                              acc_bs=iv["bs"])
             st.wavefunctions["Psi"] = r["psi"]
             st.results.append(dict(job="eigenstates", energies=r["energies"], converged=r["converged"]))
+        elif kind == "scf":
+            from . import SCF
+            n_iter, n_kry = int(job.get("nIter", 20)), int(job.get("nKrylov", 20))
+            n_itp, beta = int(job.get("nITP", 0)), float(job.get("beta", 1.0))
+            conv = float(job.get("conversion", CM))
+            r = SCF(st.engine(), n_kry).run(st.wavefunctions["Psi"], n_iter, n_itp=n_itp, beta=beta)
+            st.wavefunctions["Psi"] = r["psi"]
+            st.results.append(dict(job="scf", energies=r["energies"], printed=r["energies"] * conv))
         else:
             raise ValueError(f"job {kind!r} is outside this engine's path (tree, hamiltonian, wavefunction, "
-                             "seed, cmf, eigenstates)")
+                             "seed, cmf, eigenstates, scf)")
     return st
 
 
@@ -358,6 +372,9 @@ def main(argv: List[str]) -> int:
         if r["job"] == "cmf":
             for t, e, a in zip(r["times"], r["energies"], r["autocorr"]):
                 print(f"{t:.6f} {e:.12f} {a.real:.12f} {a.imag:.12f}")
+        elif r["job"] == "scf":
+            for it, row in enumerate(r["printed"]):
+                print(f"scf iteration {it}: {row[-1]:.10f}")
         else:
             print("levels", " ".join(f"{x:.10f}" for x in r["energies"][-1]))
     return 0

@@ -281,3 +281,33 @@ def test_small_matrix_shape_checks():
     assert L.qtg_qr_batch_z(fake, fake, 3, 4, 1, None) == 1  # m < n
     L.qtg_reg_inverse_batch_z.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_void_p]
     assert L.qtg_reg_inverse_batch_z(fake, fake, 4, 1, 0.0, None) == 3  # eps must be positive
+
+
+@pytest.mark.parametrize("model", ["small", "cmf8", "grad32", "sweep64", "ho128"])
+def test_scf_plan_stays_inside_its_buffers(model):
+    """The SCF sweep is planned on the host (no device) and every launch of
+    one iteration is checked: reads and writes inside their buffers, outputs
+    of one launch disjoint, no launch writing what it reads."""
+    from qutree_amd import mctdh as M
+    if model == "small":
+        text, H, k = W.balanced_tree_text(4, 3, [2, 3]), W.coupled_ho(4, 11, c_sigma=0.1), 4
+    elif model == "cmf8":
+        (text, H), k = W.cmf8_model()[:2], 10
+    elif model == "grad32":
+        (text, H), k = W.grad32_model()[:2], 5
+    elif model == "sweep64":
+        (text, H), k = W.sweep64_model()[:2], 5
+    else:
+        text, H, k = M.balanced_tree(128, 10, 4, 4, 0, 0.01822, 0.0, 0.0, 0.01822), M.coupled_ho(128), 10
+    L = _host()
+    L.qtg_host_scf_plan.argtypes = [C.c_char_p, C.c_int64, np.ctypeslib.ndpointer(np.float64)] + \
+        [np.ctypeslib.ndpointer(np.int64)] * 4 + [np.ctypeslib.ndpointer(np.float64), C.c_int64,
+                                                  np.ctypeslib.ndpointer(np.int64)]
+    c, toff, fm, fk, fp, pool = H.arrays()
+    info = np.zeros(6, np.int64)
+    rc = L.qtg_host_scf_plan(text.encode(), H.nterms, c, toff, fm, fk, fp, pool, k, info)
+    assert rc == 0, L.qtg_last_error().decode()
+    nnodes = O.OracleModel(text).nnodes
+    assert info[0] == 2 * nnodes - 1
+    assert info[1] > info[0] * k  # at least one product and one Lanczos step per Krylov vector
+

@@ -68,3 +68,24 @@ def test_cmf_job_without_append_keeps_the_reference_file_mode(tmp_path):
     psi, used = tensor_tree_from_bytes(text, data, 0)
     assert np.array_equal(psi, st.wavefunctions["Psi"])
     assert data[used:] == b"\xab" * (100000 - used)
+
+
+def test_scf_job_runs_the_reference_example_input():
+    """examples/mctdh/coupledHO.4.yaml of the reference, as its jobs are
+    written (compact tree, coupled_ho, create, scf with nIter 5 and
+    nKrylov 10), seeded: the energies equal the oracle's SCF from the same
+    state and are printed times the reference's conversion factor."""
+    print("case mctdh scf job coupledHO.4", flush=True)
+    tree = ("1 -2\n  10 -2\n    4 -1\n      10 0 0\n    4 -1\n      10 0 1\n  10 -2\n    4 -1\n      10 0 2\n"
+            "    4 -1\n      10 0 3\n" + "0.01822  0.0 0.0 0.0122\n" * 4)
+    cfg = {"run": [{"job": "tree", "type": "compact", "tree": tree}, {"job": "hamiltonian", "name": "coupled_ho"},
+                   {"job": "seed", "seed": 5}, {"job": "wavefunction", "name": "Psi", "type": "create"},
+                   {"job": "scf", "nIter": 5, "nKrylov": 10, "nITP": 0}]}
+    st = M.run(cfg)
+    r = st.results[-1]
+    om = O.OracleModel(tree, M.coupled_ho(4))
+    psi0 = om.fill_random(5, delta_lowest=True)
+    ref = om.scf(psi0, 5, 10)
+    assert r["energies"].shape == ref["energies"].shape
+    assert np.max(np.abs(r["energies"] - ref["energies"]) / np.abs(ref["energies"])) < 1e-10
+    assert np.allclose(r["printed"], r["energies"] * M.CM, rtol=1e-15)

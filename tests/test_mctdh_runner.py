@@ -89,3 +89,5 @@ def test_save_and_read_last_tree(tmp_path):
 def test_jobs_off_the_path_are_refused():
     with pytest.raises(ValueError):
         M.run({"run": [{"job": "potential", "name": "ch3"}]})
+    with pytest.raises(ValueError):
+        M.run({"run": [{"job": "overlaps"}]})

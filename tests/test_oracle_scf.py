@@ -159,3 +159,26 @@ def test_full_spf_sweep_reaches_exact_ground_state():
     assert abs(e[-1] - e0) < 1e-10 * abs(e0)
     v = D.DenseTree(m, r["psi"]).psi.ravel()
     assert abs(np.vdot(v, Hd @ v).real / np.vdot(v, v).real - e0) < 1e-10 * abs(e0)
+
+
+def test_reference_example_reaches_closed_form_ground_state():
+    """examples/mctdh/coupledHO.4.yaml of the reference (nIter 5, nKrylov 10)
+    from the Hartree start: the last energy lies above the exact ground
+    state of the 4 coupled oscillators (half the sum of the normal-mode
+    frequencies of V = 1/2 om^2 sum x^2 + lam^2 sum x_i x_{i+1}) by less
+    than the basis truncation."""
+    from qutree_amd import mctdh as M
+    tree = ("1 -2\n  10 -2\n    4 -1\n      10 0 0\n    4 -1\n      10 0 1\n  10 -2\n    4 -1\n"
+            "      10 0 2\n    4 -1\n      10 0 3\n" + "0.01822 0.0 0.0 0.0122\n" * 4)
+    m = O.OracleModel(tree, M.coupled_ho(4))
+    r = m.scf(m.fill_random(1234, delta_lowest=True), 5, 10)
+    lam, om = 2000.0 / M.CM, 4000.0 / M.CM
+    K = om ** 2 * np.eye(4)
+    for i in range(4):
+        K[i, (i + 1) % 4] += lam ** 2
+        K[(i + 1) % 4, i] += lam ** 2
+    e0 = 0.5 * np.sum(np.sqrt(np.linalg.eigvalsh(K)))
+    e = r["energies"].ravel()
+    assert np.all(np.diff(e) < 1e-14)
+    assert e0 * (1 - 1e-12) <= e[-1] < e0 * (1 + 2e-7)
+
