@@ -233,6 +233,78 @@ def cpu_eigenstates(psi):
     return {"seconds": round(dt, 3), "cmf_steps": r["steps"], "cores": 1, "kind": "port"}
 
 
+def scf_leg(Q, W, iters=10, cpu=True, eager=False):
+    """Krylov-sweep SCF (SCF.cpp:270-348) on the reference's own SCF example
+    input, examples/mctdh/CoupledHarmonicOscillator.128.4.4.yaml: 128 coupled
+    HO modes on a balanced tree (10 DVR points, 4 SPFs at the bottom rising
+    by 4 per layer), Hartree start, nKrylov 10, nIter 10; and the 32-mode
+    quartic model (8 DVR, SPF 8 then 24) with nKrylov 5.  Iterations per
+    second on the device (graph-replayed), the energy against the closed-form
+    ground state of the 128 normal modes, and the oracle timed on one
+    iteration of the same run (one thread)."""
+    import numpy as np
+    from qutree_amd import mctdh as M
+
+    text = M.balanced_tree(128, 10, 4, 4, 0, 0.01822, 0.0, 0.0, 0.01822)
+    H = M.coupled_ho(128)
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(128, delta_lowest=True)
+    scf = Q.SCF(eng, 10)
+    if eager:
+        scf.set_option("graph", 0)
+    scf.run(psi, 1)  # warm-up: plans and captures the iteration graph
+    t0 = time.perf_counter()
+    r = scf.run(psi, iters)
+    dt = time.perf_counter() - t0
+    info = scf.info()
+    lam, om = 2000.0 / M.CM, 4000.0 / M.CM
+    f = 128
+    e0 = 0.5 * float(np.sum(np.sqrt(om ** 2 + 2 * lam ** 2 * np.cos(2 * np.pi * np.arange(f) / f))))
+    e_it = [float(x) for x in r["energies"][:, -1]]
+    out = {"workload": "reference example CoupledHarmonicOscillator.128.4.4.yaml: 128 coupled HO modes, "
+                       "10 DVR, SPF 4 + 4 per layer, Hartree start, nKrylov 10",
+           "iterations": iters, "seconds": round(dt, 4), "iterations_per_s": round(iters / dt, 3),
+           "ms_per_iteration": round(dt / iters * 1e3, 3), "microsteps_per_iteration": info["sweep_length"],
+           "launches_per_iteration": info["launches"],
+           "us_per_launch": round(dt / iters / info["launches"] * 1e6, 3),
+           "energy_per_iteration": e_it, "closed_form_ground_state": e0,
+           "rel_gap_to_closed_form": (e_it[-1] - e0) / e0}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        omod = O.OracleModel(text, H)
+        c0 = time.perf_counter()
+        ref = omod.scf(psi, 1, 10)
+        c1 = time.perf_counter()
+        out["cpu_oracle"] = {"seconds_per_iteration": round(c1 - c0, 3), "cores": 1, "kind": "port",
+                             "sample": "one SCF iteration of the same run"}
+        out["speedup_vs_cpu_oracle"] = round((c1 - c0) / (dt / iters), 2)
+        out["first_iteration_rel_err_vs_oracle"] = float(
+            np.max(np.abs(r["energies"][0] - ref["energies"][0]) / np.abs(ref["energies"][0])))
+        # from the Hartree start the unoccupied SPF directions that each QR
+        # completes arbitrarily set the next node's search space, so rounding
+        # reaches the energies: the same oracle built without fast-math (a
+        # second summation order) measures that floor
+        with O.strict_build():
+            om2 = O.OracleModel(text, H)
+            ref2 = om2.scf(psi, 1, 10)
+            del om2
+        out["oracle_floor_first_iteration"] = float(
+            np.max(np.abs(ref2["energies"][0] - ref["energies"][0]) / np.abs(ref["energies"][0])))
+    text4, H4 = W.grad32_model()
+    eng4 = Q.Engine(text4, H4)
+    psi4 = eng4.fill_random(32, delta_lowest=True)
+    s4 = Q.SCF(eng4, 5)
+    s4.run(psi4, 1)
+    t0 = time.perf_counter()
+    r4 = s4.run(psi4, 3)
+    d4 = time.perf_counter() - t0
+    out["quartic32"] = {"workload": "32-mode quartic model, 8 DVR, SPF 8 then 24, nKrylov 5",
+                        "ms_per_iteration": round(d4 / 3 * 1e3, 3), "launches_per_iteration": s4.info()["launches"],
+                        "energy_per_iteration": [float(x) for x in r4["energies"][:, -1]]}
+    return out
+
+
 def big_tree_leg(Q, W, steps=3):
     """One H|Psi> sweep on the 128-mode tree (8 DVR, SPF 8..48 toward the root,
     128 + 8128 coupled-quartic terms) in complex128, complex64 bulk and the
@@ -334,15 +406,28 @@ def main() -> int:
                     help="CMF iterations of the 128-mode precision leg (0: skip; the full run is 1000)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
+    ap.add_argument("--scf-only", action="store_true",
+                    help="only run the SCF leg on the device with eager launches (no CPU oracle), for profiling")
     args = ap.parse_args()
+    if args.scf_only:
+        # eager launches: the profiler crashed while tracing the captured
+        # iteration graph (26k launches); kernel durations are the same
+        import qutree_amd as Q
+        from qutree_amd import workloads as W
+        print(json.dumps({"scf": scf_leg(Q, W, iters=3, cpu=False, eager=True)}))
+        return 0
     if args.copy_probe > 0:
-        import torch
+        import ctypes as C
+        import qutree_amd as Q
+        L = Q.lib()
         n = args.copy_probe << 20
-        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads n bytes, writes n bytes
-        torch.cuda.synchronize()
+        src, dst = C.c_void_p(), C.c_void_p()
+        Q.check(L.qtg_device_alloc(C.byref(src), n))
+        Q.check(L.qtg_device_alloc(C.byref(dst), n))
+        L.qtg_copy_d.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        for _ in range(5):  # the library's copy kernel: reads n bytes, writes n bytes
+            Q.check(L.qtg_copy_d(dst, src, n // 8, None))
+        Q.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "copies": 5}))
         return 0
     if args.gpus != 1:
@@ -520,6 +605,7 @@ def main() -> int:
             out["cmf128"] = cmf128_leg(Q, W, args.cmf128_steps)
         out["node_microbench"] = microbench_leg(Q, W)
         out["eigenstates"], eig_psi = eigenstates_leg(Q, W)
+        out["scf"] = scf_leg(Q, W, cpu=not args.no_cpu)
     if not args.no_cpu:
         base, e_ref = cpu_baseline(text, H, psi, energy)
         out["cpu_baseline"] = base

@@ -44,6 +44,10 @@ int qtg_init(int device);
 const char* qtg_last_error(void);
 int qtg_device_alloc(void** ptr, size_t bytes);
 int qtg_device_free(void* ptr);
+/* dst = src for n doubles on the device by the library's own copy kernel
+ * (n even, 16-byte aligned pointers): reads and writes 8 n bytes, which
+ * also calibrates the HBM byte counters of the profiles. */
+int qtg_copy_d(double* dst, const double* src, int64_t n, qtg_stream stream);
 int qtg_memcpy_h2d(void* dst, const void* src, size_t bytes, qtg_stream stream);
 int qtg_memcpy_d2h(void* dst, const void* src, size_t bytes, qtg_stream stream);
 int qtg_memset(void* dst, int value, size_t bytes, qtg_stream stream);

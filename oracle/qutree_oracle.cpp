@@ -1994,6 +1994,167 @@ int scfSweep(const SOP& H, TensorTree& P, const Tree& t, size_t nIter, size_t nK
   return -1;
 }
 
+// ------------------------------------------- SOP application fitted into the tree
+// applyOperatorIteration (ApplyFirstOrder.cpp:43-147): H|Psi> is fitted into
+// a tree of Psi's shape by alternating sweeps, with fidelity() of
+// TreeApplyOperator.cpp:22-35 and applyTop (:130-143).  Both files sum the
+// bare products sop(i) of the SOP: its coefficients are not applied.  The
+// start is TensorTree(gen, tree, false) from mt19937(seed) (the reference
+// seeds it from the clock, ApplyFirstOrder.cpp:57).
+
+// representLayer with Bra != Ket (SparseMatrixTreeFunctions_Implementation.h:18-52)
+static void representLayerBK(SparseMatrixTree& h, const MLO& M, const Tensor& Bra, const Tensor& Ket, const Tree& t,
+                             const Node& n) {
+  if (!h.active(n.address)) return;
+  Tensor hK = n.bottom ? mloApply(M, Ket, n.leaf) : applyNode(h, nullptr, Ket, t, n, (int)parentIdxOf(n));
+  h[n.address] = contraction(Bra, hK, parentIdxOf(n));
+}
+
+static void representBK(SparseMatrixTree& h, const MLO& M, const TensorTree& Bra, const TensorTree& Ket,
+                        const Tree& t) {
+  for (int a : h.st.nodes)
+    if (!t.nodes[a].isTop()) representLayerBK(h, M, Bra[a], Ket[a], t, t.nodes[a]);
+}
+
+// apply() with a density fallback for inactive holes
+// (SparseMatrixTreeFunctions_Implementation.h:299-352)
+static Tensor applyNodeRho(const SparseMatrixTree& mat, const SparseMatrixTree* holes, const MatrixTree* rho,
+                           Tensor Phi, const Tree& t, const Node& node, int skip) {
+  Tensor cur = Phi;
+  if (!node.bottom) {
+    for (size_t k = 0; k < node.children.size(); ++k) {
+      int c = node.children[k];
+      if ((int)k == skip || !mat.active(c)) continue;
+      cur = matrixTensor(mat[c], cur, k);
+    }
+  }
+  if (!node.isTop() && skip != (int)parentIdxOf(node) && holes != nullptr) {
+    if (mat.st.active(node.address) && holes->active(node.address))
+      cur = matrixTensor((*holes)[node.address], cur, node.shape.lastIdx());
+    else if (rho != nullptr)
+      cur = matrixTensor((*rho)[node.address], cur, node.shape.lastIdx());
+  }
+  return cur;
+}
+
+// contraction(holes, Bra, Ket, mats, &rho, stree, tree)
+// (SparseMatrixTreeFunctions_Implementation.h:145-190)
+static void holesRho(SparseMatrixTree& holes, const SparseMatrixTree& mats, const TensorTree& Bra,
+                     const TensorTree& Ket, const MatrixTree& rho, const Tree& t) {
+  for (int i = (int)holes.st.nodes.size() - 1; i >= 0; --i) {
+    const int a = holes.st.nodes[i];
+    const Node& n = t.nodes[a];
+    if (n.isTop()) {
+      holes[a] = identity(n.shape.lastDim());
+      continue;
+    }
+    const Node& p = t.nodes[n.parent];
+    Tensor hK = applyNodeRho(mats, &holes, &rho, Ket[p.address], t, p, n.childIdx);
+    holes[a] = contraction(Bra[p.address], hK, n.childIdx);
+  }
+}
+
+// apply(mats, Phi, M, node) (SparseMatrixTreeFunctions_Implementation.h:394-404)
+static Tensor applyLocal(const SparseMatrixTree& mats, const Tensor& Phi, const MLO& M, const Tree& t,
+                         const Node& n) {
+  if (!mats.active(n.address)) return Phi;
+  if (n.bottom) return mloApply(M, Phi, n.leaf);
+  return applyNode(mats, nullptr, Phi, t, n, (int)parentIdxOf(n));
+}
+
+// applyTop (TreeApplyOperator.cpp:130-143)
+static Tensor applyTopFit(const Tensor& A, const vector<SparseMatrixTree>& H, const SOP& sop, const Tree& t,
+                          const Node& n) {
+  Tensor out(A.s);
+  for (size_t l = 0; l < sop.terms.size(); ++l) {
+    Tensor x = applyLocal(H[l], A, sop.terms[l], t, n);
+    for (size_t i = 0; i < out.c.size(); ++i) out[i] += x[i];
+  }
+  return out;
+}
+
+// applyOperatorLocal (ApplyFirstOrder.cpp:120-140)
+static Tensor applyOperatorLocal(const Tensor& Phi, const MatrixTree& rho, const vector<SparseMatrixTree>& mats,
+                                 const vector<SparseMatrixTree>& holes, const SOP& sop, const Tree& t,
+                                 const Node& n) {
+  Tensor HPhi(Phi.s);
+  for (size_t l = 0; l < mats.size(); ++l) {
+    Tensor m = applyLocal(mats[l], Phi, sop.terms[l], t, n);
+    if (holes[l].active(n.address)) m = matrixTensor(holes[l][n.address], m, parentIdxOf(n));
+    else if (!n.isTop()) m = matrixTensor(rho[n.address], m, parentIdxOf(n));
+    for (size_t i = 0; i < HPhi.c.size(); ++i) HPhi[i] += m[i];
+  }
+  return qrTensor(HPhi, HPhi.s.lastIdx());
+}
+
+static vector<size_t> sopLeaves(const SOP& sop) {
+  vector<size_t> r;
+  for (auto& M : sop.terms)
+    for (size_t k : M.targetLeaves())
+      if (std::find(r.begin(), r.end(), k) == r.end()) r.push_back(k);
+  return r;
+}
+
+// fidelity(Psi, Psilast, u, tree) (TreeApplyOperator.cpp:22-35)
+double fitFidelity(const TensorTree& HPsi, const TensorTree& Psi, const SOP& sop, const Tree& t) {
+  SparseTree st = makeSparseTree(sopLeaves(sop), t, true);
+  vector<SparseMatrixTree> um;
+  for (size_t l = 0; l < sop.terms.size(); ++l) {
+    um.push_back(makeSMT(st, t));
+    representBK(um[l], sop.terms[l], HPsi, Psi, t);
+  }
+  const Node& top = t.nodes[t.top()];
+  Tensor U = applyTopFit(Psi[top.address], um, sop, t, top);
+  return std::norm(dotProduct(HPsi[top.address], U)(0, 0));
+}
+
+// applyOperatorIteration(Psi, rho, sop, tree) (ApplyFirstOrder.cpp:43-79) and
+// its sweep (:81-118).  fids receives the fidelity after every iteration.
+void applyOperatorFit(const SOP& sop, TensorTree& Psi, const Tree& t, uint64_t seed, size_t nIter,
+                      vector<double>& fids) {
+  if (sop.terms.size() == 1) {  // applyReference on the bottom tensors
+    for (auto& n : t.nodes)
+      if (n.bottom) Psi[n.address] = mloApply(sop.terms[0], Psi[n.address], n.leaf);
+    fids.push_back(1.0);
+    return;
+  }
+  SparseTree st = makeSparseTree(sopLeaves(sop), t, false);
+  MatrixTree rho = rhoTree(Psi, t);
+  std::mt19937 gen((unsigned)seed);
+  TensorTree HPsi = makeTensorTree(t);
+  fillRandom(HPsi, t, gen, false);
+  for (auto& n : t.nodes)
+    if (!st.active(n.address)) HPsi[n.address] = Psi[n.address];
+  double f = 0., f_last = 1.;
+  for (size_t it = 0; it < nIter; ++it) {
+    vector<SparseMatrixTree> Hm, Hh;
+    for (size_t l = 0; l < sop.terms.size(); ++l) {
+      Hm.push_back(makeSMT(st, t));
+      representBK(Hm[l], sop.terms[l], HPsi, Psi, t);
+    }
+    for (size_t l = 0; l < sop.terms.size(); ++l) {
+      Hh.push_back(makeSMT(st, t));
+      holesRho(Hh[l], Hm[l], HPsi, Psi, rho, t);
+    }
+    for (int a : st.nodes) {
+      const Node& n = t.nodes[a];
+      const bool top = n.isTop() || !st.active(n.parent);
+      if (!top) {
+        HPsi[a] = applyOperatorLocal(Psi[a], rho, Hm, Hh, sop, t, n);
+        for (size_t l = 0; l < sop.terms.size(); ++l) representLayerBK(Hm[l], sop.terms[l], HPsi[a], Psi[a], t, n);
+      } else {
+        HPsi[a] = applyTopFit(Psi[a], Hm, sop, t, n);
+        gramSchmidt(HPsi[a]);
+      }
+    }
+    f_last = f;
+    f = fitFidelity(HPsi, Psi, sop, t);
+    fids.push_back(f);
+    if ((1. - f) < 1e-6 || (f - f_last) < 1e-10) break;
+  }
+  Psi = HPsi;
+}
+
 // ---------------------------------------------------------------------------
 // Model = tree + SOP (+ state), the unit the C-ABI hands around.
 // ---------------------------------------------------------------------------
@@ -2509,6 +2670,24 @@ long orc_scf_sweep_order(void* h, long* out, long cap) {
   vector<int> s = scfSweepOrder(((Model*)h)->tree);
   for (long i = 0; i < (long)s.size() && i < cap; ++i) out[i] = s[i];
   return (long)s.size();
+}
+
+// SOP application fitted into the tree (applyOperatorFit): psi in, the fit
+// out; fids (cap entries) receives the fidelity of every iteration.  Returns
+// the number of iterations, -1 on error.
+long orc_apply_fit(void* h, double* psi, unsigned long seed, long n_iter, double* fids, long cap) {
+  Model* m = (Model*)h;
+  try {
+    TensorTree P = m->unpack(psi);
+    vector<double> f;
+    applyOperatorFit(m->H, P, m->tree, seed, (size_t)n_iter, f);
+    m->pack(P, psi);
+    for (long i = 0; i < (long)f.size() && i < cap; ++i) fids[i] = f[i];
+    return (long)f.size();
+  } catch (std::exception& ex) {
+    g_err = ex.what();
+    return -1;
+  }
 }
 
 // Top-node matrices after a build from psi: H(i,j) = <Psi_i|H|Psi_j>

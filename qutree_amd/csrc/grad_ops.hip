@@ -36,6 +36,12 @@ __global__ void copy_scalar_kernel(const double* __restrict__ src, double* dst) 
   if (threadIdx.x == 0) *dst = *src;
 }
 
+// dst = src, n2 double2 pairs (16-byte loads and stores), grid-stride
+__global__ void __launch_bounds__(256) copy_vec_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                       int64_t n2) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 }  // namespace
 
 int launch_energy_scalars(const cplx<double>* e, double* escal, CombTerm* terms, int64_t npairs, hipStream_t s) {
@@ -49,6 +55,19 @@ int launch_scaled_axpy(double* x, const double* y, int64_t n, double a, const do
   if (n <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
   hipLaunchKernelGGL(scaled_axpy_kernel, dim3(blocks), dim3(256), 0, s, x, y, n, a, escal);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+int launch_copy_vec(const double* src, double* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return QTG_OK;
+  if ((n & 1) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15)) {
+    set_error("copy: n must be even and both pointers 16-byte aligned");
+    return QTG_EARG;
+  }
+  const int64_t n2 = n / 2;
+  const int64_t blocks = std::min<int64_t>((n2 + 255) / 256, 8192);
+  hipLaunchKernelGGL(copy_vec_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const double2*)src, (double2*)dst, n2);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }

@@ -14,5 +14,7 @@ int launch_energy_scalars(const cplx<double>* e, double* escal, CombTerm* terms,
 // x += a escal[1] y (n doubles); y null: x *= escal[1]
 int launch_scaled_axpy(double* x, const double* y, int64_t n, double a, const double* escal, hipStream_t s);
 int launch_copy_scalar(const double* src, double* dst, hipStream_t s);
+// dst = src for n doubles (n even, 16-byte aligned pointers)
+int launch_copy_vec(const double* src, double* dst, int64_t n, hipStream_t s);
 
 }  // namespace qtg

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "grad_ops.hpp"
 
 namespace qtg {
 static thread_local std::string g_error;
@@ -46,6 +47,11 @@ int qtg_device_alloc(void** ptr, size_t bytes) {
 int qtg_device_free(void* ptr) {
   QTG_HIP_TRY(hipFree(ptr));
   return QTG_OK;
+}
+
+int qtg_copy_d(double* dst, const double* src, int64_t n, qtg_stream stream) {
+  if (!dst || !src || n < 0) return QTG_EARG;
+  return launch_copy_vec(src, dst, n, S(stream));
 }
 
 int qtg_memcpy_h2d(void* dst, const void* src, size_t bytes, qtg_stream stream) {

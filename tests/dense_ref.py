@@ -51,8 +51,13 @@ def term_factors(H: W.Hamiltonian, l, grids):
     out = []
     for f in range(H.term_off[l], H.term_off[l + 1]):
         mode, kind = H.fmode[f], H.fkind[f]
-        vals = H.pool[2 * H.fpool[f]: 2 * H.fpool[f] + 4] if kind == W.FACTOR_POLY else None
         x, kin = grids[mode]
+        if kind == W.FACTOR_MATRIX:  # d x d column-major, interleaved complex, at the pool offset
+            d = len(x)
+            raw = np.asarray(H.pool[2 * H.fpool[f]: 2 * H.fpool[f] + 2 * d * d])
+            out.append((mode, (raw[0::2] + 1j * raw[1::2]).reshape(d, d, order="F")))
+            continue
+        vals = H.pool[2 * H.fpool[f]: 2 * H.fpool[f] + 4] if kind == W.FACTOR_POLY else None
         out.append((mode, leaf_matrix(kind, vals, x, kin)))
     return out
 

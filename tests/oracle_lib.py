@@ -126,6 +126,8 @@ def _bind(L):
     L.orc_scf_probe.restype = C.c_long
     L.orc_scf_sweep_order.argtypes = [C.c_void_p, lp, C.c_long]
     L.orc_scf_sweep_order.restype = C.c_long
+    L.orc_apply_fit.argtypes = [C.c_void_p, dp, C.c_ulong, C.c_long, dp, C.c_long]
+    L.orc_apply_fit.restype = C.c_long
     return L
 
 
@@ -383,3 +385,15 @@ class OracleModel:
             raise RuntimeError(self._L.orc_last_error().decode())
         Na = int(np.prod(self.nodes[a]["dims"]))
         return a, as_complex(p).copy(), as_complex(h)[:Na * Na].reshape(Na, Na, order="F").copy()
+
+    def apply_fit(self, psi, seed, n_iter=100):
+        """H|Psi> fitted into a tree of Psi's shape (ApplyFirstOrder.cpp:43-147,
+        the SOP's coefficients not applied): the fit and the fidelity of
+        every iteration."""
+        p = cview(psi).copy()
+        f = np.zeros(max(n_iter, 1))
+        k = self._L.orc_apply_fit(self.h, p, seed, n_iter, f, len(f))
+        if k < 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        return dict(psi=as_complex(p).copy(), fidelities=f[:k].copy())
+
