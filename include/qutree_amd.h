@@ -403,6 +403,37 @@ int qtg_host_scf_plan(const char* tree_text, int64_t nterms, const double* coeff
                       const int64_t* fmode, const int64_t* fkind, const int64_t* fpool, const double* pool,
                       int64_t n_krylov, int64_t* info);
 
+/* ------------------------------------------- SOP application fitted into the tree */
+/* applyOperatorIteration(Psi, rho, sop, tree) (Applications/ApplyFirstOrder.cpp:
+ * 43-147): H|Psi> fitted into a tree of Psi's shape by alternating sweeps,
+ * with fidelity |<fit|H Psi>|^2 (TreeApplyOperator.cpp:22-35).  The operator
+ * is the engine's Hamiltonian (complex128 engine) with its coefficients NOT
+ * applied, as the reference sums the bare products sop(i): fold a
+ * coefficient into one of the product's leaf matrices.  The fit starts from
+ * TensorTree(mt19937(seed), tree, false) (the reference seeds from the
+ * clock); the loop stops when 1 - f < 1e-6, when f rose by less than 1e-10,
+ * or after max_iter iterations (the reference's 100).  psi_host holds the
+ * state on entry and the fit on return; fidelities (cap) the fidelity after
+ * each iteration, *iterations their number.  A one-product operator is
+ * applied to the bottom tensors directly (applyReference) with fidelity 1.
+ * Supported for operators whose tail-cut sparse tree reaches the top node,
+ * with one state there (QTG_ESHAPE otherwise). */
+typedef void* qtg_apply_fit;
+int qtg_apply_fit_create(qtg_engine engine, qtg_apply_fit* fit);
+int qtg_apply_fit_destroy(qtg_apply_fit fit);
+int qtg_apply_fit_run(qtg_apply_fit fit, double* psi_host, uint64_t seed, int64_t max_iter, double* fidelities,
+                      int64_t cap, int64_t* iterations);
+/* "graph" (1: replay iterations from a captured HIP graph, the default) */
+int qtg_apply_fit_set_option(qtg_apply_fit fit, const char* key, double value);
+/* info[0] launches of the first iteration's h-matrices, info[1] launches
+ * per iteration, info[2] device bytes */
+int qtg_apply_fit_info(qtg_apply_fit fit, int64_t* info);
+/* Host-only planning with the checks of qtg_host_scf_plan; info (4
+ * entries): launches per iteration, work and pool elements, GM tasks. */
+int qtg_host_apply_fit_plan(const char* tree_text, int64_t nterms, const double* coeff, const int64_t* term_off,
+                            const int64_t* fmode, const int64_t* fkind, const int64_t* fpool, const double* pool,
+                            int64_t* info);
+
 /* ---------------------------------------------------------------- formats */
 /* QuTree's binary files, byte for byte (host only; complex<double>,
  * little-endian).  TensorTree<cd>::write/read (TensorTree_Implementation.h:

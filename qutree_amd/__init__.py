@@ -98,6 +98,11 @@ def lib():
     L.qtg_scf_run.argtypes = [_vp, _dp, C.c_int64, C.c_int64, C.c_double, _dp, C.c_int64]
     L.qtg_scf_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
     L.qtg_scf_info.argtypes = [_vp, _lp]
+    L.qtg_apply_fit_create.argtypes = [_vp, C.POINTER(_vp)]
+    L.qtg_apply_fit_destroy.argtypes = [_vp]
+    L.qtg_apply_fit_run.argtypes = [_vp, _dp, C.c_uint64, C.c_int64, _dp, C.c_int64, C.POINTER(C.c_int64)]
+    L.qtg_apply_fit_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
+    L.qtg_apply_fit_info.argtypes = [_vp, _lp]
     L.qtg_cmf_integrate.argtypes = [_vp, _dp] + [C.c_double] * 8 + [_i64, _i64, _dp, _dp, _dp, _lp]
     L.qtg_cmf_integrate.restype = _i64
     L.qtg_cmf_eigenstates.argtypes = ([_vp, _dp, C.c_double, C.c_double, _i64, C.c_double, C.c_double, _i64, _dp,
@@ -477,6 +482,43 @@ class SCF:
         try:
             if self.ptr:
                 lib().qtg_scf_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class ApplyFit:
+    """H|Psi> fitted into a tree of Psi's shape on the device
+    (qtg_apply_fit_*, ApplyFirstOrder.cpp:43-147); the operator is the
+    engine's Hamiltonian with its coefficients not applied, as in the
+    reference."""
+
+    def __init__(self, engine: Engine):
+        p = _vp()
+        check(lib().qtg_apply_fit_create(engine.ptr, C.byref(p)))
+        self.ptr = p.value
+        self.engine = engine
+
+    def set_option(self, key: str, value) -> None:
+        check(lib().qtg_apply_fit_set_option(self.ptr, key.encode(), float(value)))
+
+    def info(self):
+        v = np.zeros(3, np.int64)
+        check(lib().qtg_apply_fit_info(self.ptr, v))
+        return dict(launches_first=int(v[0]), launches=int(v[1]), device_bytes=int(v[2]))
+
+    def run(self, psi, seed, max_iter=100):
+        """The fit of H|psi> and the fidelity after every iteration."""
+        p = _ci(psi).copy()
+        f = np.zeros(max_iter)
+        n = C.c_int64()
+        check(lib().qtg_apply_fit_run(self.ptr, p, seed, max_iter, f, max_iter, C.byref(n)))
+        return dict(psi=p.view(np.complex128).copy(), fidelities=f[:n.value].copy())
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qtg_apply_fit_destroy(self.ptr)
                 self.ptr = None
         except Exception:
             pass

@@ -12,7 +12,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqutree_amd.so")
 SOURCES = ["runtime.cpp", "gemm_ops.hip", "node_ops.hip", "tree_ops.hip", "cmf_ops.hip", "model.cpp", "engine.cpp", "engine_gemm.cpp", "launch_set.cpp", "ttno.cpp", "cmf.cpp",
            "capi_engine.cpp", "formats.cpp", "seam.cpp", "smallmat.hip", "grad_ops.hip", "gradient.cpp", "tuning.cpp",
-           "scf.cpp", "scf_ops.hip"]
+           "scf.cpp", "scf_ops.hip", "tree_prog.cpp", "apply_fit.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -5,6 +5,7 @@
 
 #include "cmf.hpp"
 #include "scf.hpp"
+#include "apply_fit.hpp"
 #include <algorithm>
 #include "ttno.hpp"
 #include "engine.hpp"
@@ -223,6 +224,23 @@ int qtg_host_scf_plan(const char* tree_text, int64_t nterms, const double* coeff
   info[3] = sc.pool_elems();
   info[4] = sc.gm_tasks_total();
   info[5] = sc.gc_tasks_total();
+  return QTG_OK;
+}
+
+int qtg_host_apply_fit_plan(const char* tree_text, int64_t nterms, const double* coeff, const int64_t* term_off,
+                            const int64_t* fmode, const int64_t* fkind, const int64_t* fpool, const double* pool,
+                            int64_t* info) {
+  if (!tree_text || !info) return QTG_EARG;
+  Engine en;
+  int rc = en.create(tree_text, 0, true);
+  if (rc) return rc;
+  if ((rc = en.set_hamiltonian(nterms, coeff, term_off, fmode, fkind, fpool, pool))) return rc;
+  ApplyFit f(en);
+  if ((rc = f.init())) return rc;
+  info[0] = f.launches_per_iteration();
+  info[1] = f.work_elems();
+  info[2] = f.pool_elems();
+  info[3] = f.gm_tasks_total();
   return QTG_OK;
 }
 
@@ -536,6 +554,51 @@ int qtg_scf_info(qtg_scf s, int64_t* info) {
   info[0] = sc->sweep_length();
   info[1] = sc->launches_per_iteration();
   info[2] = sc->device_bytes();
+  return QTG_OK;
+}
+
+int qtg_apply_fit_create(qtg_engine e, qtg_apply_fit* out) {
+  if (!e || !out) return QTG_EARG;
+  auto* f = new ApplyFit(*static_cast<Engine*>(e));
+  int rc = f->init();
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  *out = f;
+  return QTG_OK;
+}
+
+int qtg_apply_fit_destroy(qtg_apply_fit f) {
+  delete static_cast<ApplyFit*>(f);
+  return QTG_OK;
+}
+
+int qtg_apply_fit_run(qtg_apply_fit f, double* psi_host, uint64_t seed, int64_t max_iter, double* fidelities,
+                      int64_t cap, int64_t* iterations) {
+  if (!f || !psi_host || max_iter < 1 || cap < 0 || (cap > 0 && !fidelities)) {
+    set_error("apply_fit_run: bad argument");
+    return QTG_EARG;
+  }
+  return static_cast<ApplyFit*>(f)->run(psi_host, seed, max_iter, fidelities, cap, iterations);
+}
+
+int qtg_apply_fit_set_option(qtg_apply_fit f, const char* key, double value) {
+  if (!f || !key) return QTG_EARG;
+  if (!std::strcmp(key, "graph")) {
+    static_cast<ApplyFit*>(f)->graph_ = value != 0.0;
+    return QTG_OK;
+  }
+  set_error(std::string("apply_fit_set_option: unknown key ") + key);
+  return QTG_EARG;
+}
+
+int qtg_apply_fit_info(qtg_apply_fit f, int64_t* info) {
+  if (!f || !info) return QTG_EARG;
+  auto* a = static_cast<ApplyFit*>(f);
+  info[0] = a->launches_first();
+  info[1] = a->launches_per_iteration();
+  info[2] = a->device_bytes();
   return QTG_OK;
 }
 

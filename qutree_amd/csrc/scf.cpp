@@ -33,235 +33,11 @@
 #include <cstring>
 #include <functional>
 #include <string>
-#include <tuple>
 
 namespace qtg {
 
 Scf::~Scf() {
   if (exec_) hipGraphExecDestroy(exec_);
-  void* ptrs[] = {psi_, work_, pool_, ev_, scal_, elog_, par_, d_gm_, d_gc_, d_co_, d_ct_};
-  for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (s_) hipStreamDestroy(s_);
-}
-
-int64_t Scf::palloc(int64_t n) {
-  const int64_t o = poolSize_;
-  poolSize_ += n;
-  return o;
-}
-
-int64_t Scf::walloc(int64_t n) {
-  const int64_t o = progTop_;
-  progTop_ += n;
-  progMax_ = std::max(progMax_, progTop_);
-  return o;
-}
-
-int64_t Scf::leg_bef(int a, int k) const {
-  int64_t b = 1;
-  for (int i = 0; i < k; ++i) b *= E.tree().nodes[a].dims[i];
-  return b;
-}
-
-int64_t Scf::leg_aft(int a, int k) const {
-  const auto& d = E.tree().nodes[a].dims;
-  int64_t f = 1;
-  for (int i = k + 1; i < (int)d.size(); ++i) f *= d[i];
-  return f;
-}
-
-GmDesc Scf::gm_desc(int a, int leg, int64_t mat, int xsel, int64_t x, int64_t out) const {
-  const HNode& nd = E.tree().nodes[a];
-  GmDesc d{};
-  if (leg < 0) {  // copy of the whole tensor
-    d.bef = (int32_t)(nd.N / nd.n);
-    d.nin = d.nout = nd.n;
-    d.aft = 1;
-    d.op = GM_COPY;
-    d.m = 0;
-  } else {
-    d.bef = (int32_t)leg_bef(a, leg);
-    d.nin = d.nout = nd.dims[leg];
-    d.aft = (int32_t)leg_aft(a, leg);
-    d.op = GM_N;
-    d.m = mat;
-  }
-  d.x = x;
-  d.out = out;
-  d.acc = 0;
-  d.are = 1.0;
-  d.aim = 0.0;
-  d.node = -1;
-  d.xsel = xsel;
-  d.ksplit = 1;
-  return d;
-}
-
-GcDesc Scf::gc_desc(int a, int leg, int xsel, int64_t x, int ysel, int64_t y, int64_t out) const {
-  const HNode& nd = E.tree().nodes[a];
-  GcDesc d{};
-  d.x = x;
-  d.y = y;
-  d.out = out;
-  d.bef = (int32_t)leg_bef(a, leg);
-  d.nx = d.ny = nd.dims[leg];
-  d.aft = (int32_t)leg_aft(a, leg);
-  d.acc = 0;
-  d.node = -1;
-  d.ksplit = 1;
-  d.sel = (xsel ? 1 : 0) | (ysel ? 2 : 0);
-  return d;
-}
-
-void Scf::push_gm(std::vector<Step>& P, std::vector<GmDesc>& ds, int obuf) {
-  if (ds.empty()) return;
-  Step st;
-  st.kind = K_GM;
-  st.obuf = obuf;
-  st.waves = kGemmWaves;
-  for (auto& d : ds) {
-    // long reductions (the stacked sums over terms) split along K into
-    // chunks that sum inside their workgroup
-    if (d.op != GM_COPY && d.nin > 2 * E.gmSplitK_) {
-      d.ksplit = split_chunks(d.nin, kGemmWavesWide, E.gmSplitK_, kGemmWavesWide);
-      if (d.ksplit > kGemmWaves) st.waves = kGemmWavesWide;
-    }
-  }
-  const int64_t t = order_tasks(ds, gm_tasks);
-  st.d0 = (int64_t)h_gm_.size();
-  for (auto& d : ds) h_gm_.insert(h_gm_.end(), (size_t)gm_tasks(d), d);
-  st.n = t;
-  P.push_back(st);
-}
-
-void Scf::push_gc(std::vector<Step>& P, std::vector<GcDesc>& ds) {
-  if (ds.empty()) return;
-  // the split rule of the engine's grouped hole contractions
-  int64_t tiles = 0;
-  for (auto& d : ds) tiles += gc_tasks(d);
-  const int64_t want = (E.splitWaves_ + tiles - 1) / tiles;
-  for (auto& d : ds) {
-    const int64_t K = (int64_t)d.bef * d.aft;
-    d.ksplit = K > 2 * E.gcLongK_ ? split_chunks(K, kGemmWaves, 1) : split_chunks(K, want, E.splitMinK_);
-  }
-  const int64_t t = order_tasks(ds, gc_tasks);
-  Step st;
-  st.kind = K_GC;
-  st.d0 = (int64_t)h_gc_.size();
-  for (auto& d : ds) h_gc_.insert(h_gc_.end(), (size_t)gc_tasks(d), d);
-  st.n = t;
-  P.push_back(st);
-}
-
-void Scf::push_comb(std::vector<Step>& P, const std::vector<std::pair<int64_t, std::vector<CombTerm>>>& outs,
-                    int n) {
-  if (outs.empty()) return;
-  Step st;
-  st.kind = K_COMB;
-  st.d0 = (int64_t)h_co_.size();
-  for (auto& o : outs) {
-    CombOut co{};
-    co.out = o.first;
-    co.n = n;
-    co.t0 = (int32_t)h_ct_.size();
-    h_ct_.insert(h_ct_.end(), o.second.begin(), o.second.end());
-    co.t1 = (int32_t)h_ct_.size();
-    h_co_.push_back(co);
-  }
-  st.n = (int64_t)outs.size();
-  P.push_back(st);
-}
-
-// Mode-product chains on node a's tensor (input in base inbuf at `in`):
-// request r applies its factors in order and leaves the result at its
-// destination (work offset) or, with -1, in a temporary.  Requests sharing a
-// prefix share its products; depth d of all chains is one grouped launch.
-std::vector<int64_t> Scf::emit_chains(std::vector<Step>& P, int a, int inbuf, int64_t in,
-                                      const std::vector<std::pair<Chain, int64_t>>& reqs) {
-  const int64_t N = E.tree().nodes[a].N;
-  std::map<Chain, int64_t> loc;
-  size_t maxd = 0;
-  for (auto& r : reqs) maxd = std::max(maxd, r.first.size());
-  for (size_t d = 1; d <= maxd; ++d) {
-    std::vector<GmDesc> ds;
-    for (int pass = 0; pass < 2; ++pass) {  // requests ending at this depth first: they own a destination
-      for (auto& r : reqs) {
-        const Chain& c = r.first;
-        if (c.size() < d || (pass == 0) != (c.size() == d)) continue;
-        Chain pre(c.begin(), c.begin() + d);
-        if (loc.count(pre)) continue;
-        const int64_t out = (c.size() == d && r.second >= 0) ? r.second : walloc(N);
-        loc[pre] = out;
-        int xsel = 1;
-        int64_t x;
-        if (d == 1) {
-          xsel = inbuf;
-          x = in;
-        } else {
-          x = loc.at(Chain(c.begin(), c.begin() + (d - 1)));
-        }
-        ds.push_back(gm_desc(a, c[d - 1].first, c[d - 1].second, xsel, x, out));
-      }
-    }
-    push_gm(P, ds, 1);
-  }
-  std::vector<int64_t> outs;
-  for (auto& r : reqs) outs.push_back(r.first.empty() ? in : loc.at(r.first));
-  return outs;
-}
-
-int Scf::emit_stack(std::vector<Step>& comb, std::vector<Step>& body, int a, int inbuf, int64_t in,
-                    std::vector<StackEntry>& entries, bool entry0_is_input, int64_t stack_base, int64_t out) {
-  const HNode& nd = E.tree().nodes[a];
-  const int64_t N = nd.N, n = nd.n;
-  std::vector<std::pair<Chain, int64_t>> reqs;
-  for (size_t e = 0; e < entries.size(); ++e) {
-    if (e == 0 && entry0_is_input) continue;
-    Chain c = entries[e].chain;
-    if (c.empty()) c.push_back({-1, 0});  // the input itself: copy it into the stack
-    reqs.push_back({c, stack_base + (int64_t)e * N});
-  }
-  emit_chains(body, a, inbuf, in, reqs);
-  const size_t start = (entry0_is_input && entries[0].block.empty()) ? 1 : 0;
-  const int64_t Eff = (int64_t)entries.size() - (int64_t)start;
-  if (Eff <= 0) {
-    set_error("scf: a node with no contributions to its effective Hamiltonian");
-    return QTG_ESTATE;
-  }
-  bool ident = Eff <= 4;
-  for (size_t e = start; e < entries.size() && ident; ++e) {
-    const auto& b = entries[e].block;
-    ident = b.size() == 1 && b[0].src < 0 && b[0].cre == 1.0 && b[0].cim == 0.0;
-  }
-  int64_t m;
-  if (ident) {
-    m = ident_.at({(int)n, 4});
-  } else {
-    std::vector<std::pair<int64_t, std::vector<CombTerm>>> outs;
-    for (size_t e = start; e < entries.size(); ++e) outs.push_back({S_ + (int64_t)(e - start) * n * n, entries[e].block});
-    push_comb(comb, outs, (int)n);
-    Smax_ = std::max(Smax_, Eff * n * n);
-    m = S_;
-  }
-  GmDesc d{};
-  d.x = stack_base + (int64_t)start * N;
-  d.xsel = 1;
-  d.m = m;
-  d.out = out;
-  d.bef = (int32_t)(N / n);
-  d.nin = (int32_t)(n * Eff);
-  d.nout = (int32_t)n;
-  d.aft = 1;
-  d.op = GM_N;
-  d.acc = 0;
-  d.are = 1.0;
-  d.aim = 0.0;
-  d.node = -1;
-  d.ksplit = 1;
-  std::vector<GmDesc> ds{d};
-  push_gm(body, ds, 1);
-  return QTG_OK;
 }
 
 Scf::Key Scf::restrict_key(int l, int a, bool inside) const {
@@ -679,220 +455,78 @@ int Scf::init(int64_t n_krylov) {
     if ((rc = emit_microstep(sweep_[l], l + 1 < sweep_.size() ? sweep_[l + 1] : -1, (int)l))) return rc;
   workSize_ = progMax_;
   poolSize_ = S_ + Smax_;
-  if ((rc = validate())) return rc;
+  if ((rc = validate_steps(init_)) || (rc = validate_steps(iter_))) return rc;
   if (E.host_only()) return QTG_OK;  // plan and checks only
-  // device
-  QTG_HIP_TRY(hipStreamCreate(&s_));
-  auto dmalloc = [&](void** p, size_t bytes) -> int {
-    if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) {
-      set_error("scf: device allocation failed");
-      return QTG_ENOMEM;
-    }
-    bytes_ += (int64_t)bytes;
-    return QTG_OK;
-  };
-  if ((rc = dmalloc((void**)&psi_, 16 * (size_t)T.total)) || (rc = dmalloc((void**)&work_, 16 * (size_t)workSize_)) ||
-      (rc = dmalloc((void**)&pool_, 16 * (size_t)poolSize_)) || (rc = dmalloc((void**)&ev_, 8 * (size_t)kry_)) ||
-      (rc = dmalloc((void**)&scal_, 16 * (size_t)kry_)) ||
-      (rc = dmalloc((void**)&elog_, 8 * sweep_.size())) || (rc = dmalloc((void**)&par_, sizeof(ScfParams))) ||
-      (rc = dmalloc((void**)&d_gm_, sizeof(GmDesc) * h_gm_.size())) ||
-      (rc = dmalloc((void**)&d_gc_, sizeof(GcDesc) * h_gc_.size())) ||
-      (rc = dmalloc((void**)&d_co_, sizeof(CombOut) * h_co_.size())) ||
-      (rc = dmalloc((void**)&d_ct_, sizeof(CombTerm) * h_ct_.size())))
+  std::vector<std::pair<int64_t, const std::vector<cd>*>> init;
+  for (size_t i = 0; i < cdata.size(); ++i) init.push_back({leafOff_[i], cdata[i]});
+  if ((rc = materialize(init))) return rc;
+  if ((rc = dmalloc((void**)&ev_, 8 * (size_t)kry_)) || (rc = dmalloc((void**)&scal_, 16 * (size_t)kry_)) ||
+      (rc = dmalloc((void**)&elog_, 8 * sweep_.size())) || (rc = dmalloc((void**)&par_, sizeof(ScfParams))))
     return rc;
-  // the pool starts with the leaf matrices and the identity stacks, zeros elsewhere
-  std::vector<cd> host(poolSize_, cd(0.0, 0.0));
-  for (size_t i = 0; i < cdata.size(); ++i)
-    std::copy(cdata[i]->begin(), cdata[i]->end(), host.begin() + leafOff_[i]);
-  for (auto& id : ident_) {
-    const int n = id.first.first;
-    for (int b = 0; b < id.first.second; ++b)
-      for (int i = 0; i < n; ++i) host[id.second + (int64_t)b * n * n + i + (int64_t)i * n] = 1.0;
-  }
-  QTG_HIP_TRY(hipMemcpy(pool_, host.data(), 16 * (size_t)poolSize_, hipMemcpyHostToDevice));
-  QTG_HIP_TRY(hipMemset(work_, 0, 16 * (size_t)workSize_));
   QTG_HIP_TRY(hipMemset(scal_, 0, 16 * (size_t)kry_));
-  if (!h_gm_.empty()) QTG_HIP_TRY(hipMemcpy(d_gm_, h_gm_.data(), sizeof(GmDesc) * h_gm_.size(), hipMemcpyHostToDevice));
-  if (!h_gc_.empty()) QTG_HIP_TRY(hipMemcpy(d_gc_, h_gc_.data(), sizeof(GcDesc) * h_gc_.size(), hipMemcpyHostToDevice));
-  if (!h_co_.empty())
-    QTG_HIP_TRY(hipMemcpy(d_co_, h_co_.data(), sizeof(CombOut) * h_co_.size(), hipMemcpyHostToDevice));
-  if (!h_ct_.empty())
-    QTG_HIP_TRY(hipMemcpy(d_ct_, h_ct_.data(), sizeof(CombTerm) * h_ct_.size(), hipMemcpyHostToDevice));
   ready_ = true;
   return QTG_OK;
 }
 
-namespace {
-struct Span {
-  int buf;  // 0 psi, 1 work, 2 pool
-  int64_t lo, hi;
-};
-bool overlap(const Span& a, const Span& b) { return a.buf == b.buf && a.lo < b.hi && b.lo < a.hi; }
-}  // namespace
-
-int Scf::validate_steps(const std::vector<Step>& P) const {
-  const int64_t size[3] = {E.tree().total, workSize_, poolSize_};
+int Scf::validate_extra(const Step& st, std::vector<Span>& rd, std::vector<Span>& wr) const {
   auto fail = [&](const std::string& what) {
     set_error("scf plan check: " + what);
     return QTG_ESTATE;
   };
-  auto in_range = [&](const Span& s) { return s.lo >= 0 && s.lo <= s.hi && s.hi <= size[s.buf]; };
-  for (size_t si = 0; si < P.size(); ++si) {
-    const Step& st = P[si];
-    std::vector<Span> rd, wr;
-    switch (st.kind) {
-      case K_GM:
-        for (int64_t t = 0; t < st.n; ++t) {
-          const GmDesc& d = h_gm_.at(st.d0 + t);
-          if (t > 0 && d.tile0 == h_gm_[st.d0 + t - 1].tile0) continue;  // the same descriptor
-          const int64_t R = (int64_t)d.bef * d.aft;
-          rd.push_back({d.xsel ? 1 : 0, d.x, d.x + R * d.nin});
-          if (d.op != GM_COPY) rd.push_back({2, d.m, d.m + (int64_t)d.nin * d.nout});
-          else if (d.nin != d.nout) return fail("copy with nin != nout");
-          wr.push_back({st.obuf, d.out, d.out + R * d.nout});
-        }
-        break;
-      case K_GC:
-        for (int64_t t = 0; t < st.n; ++t) {
-          const GcDesc& d = h_gc_.at(st.d0 + t);
-          if (t > 0 && d.tile0 == h_gc_[st.d0 + t - 1].tile0) continue;
-          const int64_t K = (int64_t)d.bef * d.aft;
-          rd.push_back({(d.sel & 1) ? 1 : 0, d.x, d.x + K * d.nx});
-          rd.push_back({(d.sel & 2) ? 1 : 0, d.y, d.y + K * d.ny});
-          wr.push_back({2, d.out, d.out + (int64_t)d.nx * d.ny});
-        }
-        break;
-      case K_COMB:
-        for (int64_t i = 0; i < st.n; ++i) {
-          const CombOut& co = h_co_.at(st.d0 + i);
-          const int64_t n2 = (int64_t)co.n * co.n;
-          wr.push_back({2, co.out, co.out + n2});
-          for (int32_t t = co.t0; t < co.t1; ++t)
-            if (h_ct_.at(t).src >= 0) rd.push_back({2, h_ct_[t].src, h_ct_[t].src + n2});
-        }
-        break;
-      case K_LBEGIN:
-        rd.push_back({0, st.a, st.a + st.N});
-        wr.push_back({1, st.b, st.b + st.N});
-        wr.push_back({1, st.c, st.c + st.N});
-        break;
-      case K_LSTEP:
-        if (st.k < 1 || st.k > kry_ || st.j >= st.k) return fail("Krylov index");
-        rd.push_back({1, st.a, st.a + st.N});
-        wr.push_back({1, st.b, st.b + st.k * st.N});
-        wr.push_back({1, st.c, st.c + st.N});
-        wr.push_back({2, st.d, st.d + (int64_t)st.k * st.k});
-        break;
-      case K_HEEV:
-        if (st.k < 1 || st.k > kry_) return fail("Krylov size");
-        rd.push_back({2, st.a, st.a + (int64_t)st.k * st.k});
-        wr.push_back({2, st.b, st.b + (int64_t)st.k * st.k});
-        break;
-      case K_KCOMB:
-        if (st.slot < 0 || st.slot >= (int)sweep_.size()) return fail("energy slot");
-        rd.push_back({2, st.a, st.a + (int64_t)st.k * st.k});
-        rd.push_back({1, st.b, st.b + st.k * st.N});
-        wr.push_back({1, st.c, st.c + st.N});
-        break;
-      case K_PERM: {
-        const int64_t N = st.bef * st.nn * st.aft;
-        rd.push_back({st.abuf, st.a, st.a + N});
-        wr.push_back({st.cbuf, st.c, st.c + N});
-        break;
-      }
-      case K_QR:
-        if (st.nn < 1 || st.nn > 64 || st.bef < st.nn) return fail("QR shape");
-        rd.push_back({st.abuf, st.a, st.a + st.bef * st.nn});
-        if (!(st.abuf == st.cbuf && st.a == st.c)) wr.push_back({st.cbuf, st.c, st.c + st.bef * st.nn});
-        break;
-      default:
-        return fail("unknown step");
-    }
-    for (auto& r : rd)
-      if (!in_range(r)) return fail("read outside its buffer in step " + std::to_string(si));
-    for (auto& w : wr)
-      if (!in_range(w)) return fail("write outside its buffer in step " + std::to_string(si));
-    // within one launch: outputs disjoint, and no output over an input
-    // (the Lanczos step writes Krylov slot j + 1 next to the slots it
-    // reads, which its single workgroup orders itself)
-    if (st.kind == K_LSTEP) continue;
-    std::vector<Span> w = wr;
-    std::sort(w.begin(), w.end(), [](const Span& a, const Span& b) { return std::tie(a.buf, a.lo) < std::tie(b.buf, b.lo); });
-    for (size_t i = 1; i < w.size(); ++i)
-      if (overlap(w[i - 1], w[i])) return fail("overlapping outputs in step " + std::to_string(si));
-    for (auto& x : wr)
-      for (auto& r : rd)
-        if (overlap(x, r)) return fail("a launch writes what it reads in step " + std::to_string(si));
+  switch (st.kind) {
+    case K_LBEGIN:
+      rd.push_back({0, st.a, st.a + st.N});
+      wr.push_back({1, st.b, st.b + st.N});
+      wr.push_back({1, st.c, st.c + st.N});
+      return QTG_OK;
+    case K_LSTEP:
+      if (st.k < 1 || st.k > kry_ || st.j >= st.k) return fail("Krylov index");
+      rd.push_back({1, st.a, st.a + st.N});
+      wr.push_back({1, st.b, st.b + st.k * st.N});
+      wr.push_back({1, st.c, st.c + st.N});
+      wr.push_back({2, st.d, st.d + (int64_t)st.k * st.k});
+      return QTG_OK;
+    case K_HEEV:
+      if (st.k < 1 || st.k > kry_) return fail("Krylov size");
+      rd.push_back({2, st.a, st.a + (int64_t)st.k * st.k});
+      wr.push_back({2, st.b, st.b + (int64_t)st.k * st.k});
+      return QTG_OK;
+    case K_KCOMB:
+      if (st.slot < 0 || st.slot >= (int)sweep_.size()) return fail("energy slot");
+      rd.push_back({2, st.a, st.a + (int64_t)st.k * st.k});
+      rd.push_back({1, st.b, st.b + st.k * st.N});
+      wr.push_back({1, st.c, st.c + st.N});
+      return QTG_OK;
+    default:
+      return fail("unknown step");
   }
-  return QTG_OK;
 }
 
-int Scf::validate() const {
-  int rc;
-  if ((rc = validate_steps(init_))) return rc;
-  return validate_steps(iter_);
-}
-
-int Scf::run_steps(const std::vector<Step>& P, hipStream_t s) {
+int Scf::run_extra(const Step& st, hipStream_t s) {
   auto Z = [](double* b, int64_t off) { return (void*)(b + 2 * off); };
-  int rc = QTG_OK;
-  const SmItem zero{};
-  for (const Step& st : P) {
-    switch (st.kind) {
-      case K_GM:
-        rc = launch_gm<double, double, double>(d_gm_ + st.d0, nullptr, st.n, st.n, psi_, work_, pool_,
-                                               st.obuf ? (void*)work_ : (void*)psi_, nullptr, 1.0, 0.0, s, nullptr,
-                                               st.waves ? st.waves : kGemmWaves);
-        break;
-      case K_GC:
-        rc = launch_gc<double, double>(d_gc_ + st.d0, nullptr, st.n, st.n, psi_, work_, pool_, nullptr, s, nullptr);
-        break;
-      case K_COMB:
-        rc = launch_combine<double>(d_co_ + st.d0, st.n, d_ct_, pool_, s);
-        break;
-      case K_LBEGIN:
-        rc = launch_lanczos_begin(Z(psi_, st.a), st.N, Z(work_, st.b), Z(work_, st.c), s);
-        break;
-      case K_LSTEP:
-        rc = launch_lanczos_step(st.j, st.k, st.N, Z(work_, st.a), 1, Z(work_, st.b), Z(work_, st.c), scal_,
+  switch (st.kind) {
+    case K_LBEGIN:
+      return launch_lanczos_begin(Z(psi_, st.a), st.N, Z(work_, st.b), Z(work_, st.c), s);
+    case K_LSTEP:
+      return launch_lanczos_step(st.j, st.k, st.N, Z(work_, st.a), 1, Z(work_, st.b), Z(work_, st.c), scal_,
                                  Z(pool_, st.d), s);
-        break;
-      case K_HEEV: {
-        SmItem u{};
-        u.a = st.a;
-        u.q = st.b;
-        u.e = 0;
-        u.m = st.k;
-        u.n = st.k;
-        rc = launch_heev_batch(nullptr, u, zero, 1, st.k, pool_, pool_, ev_, nullptr, 0.0, s);
-        break;
-      }
-      case K_KCOMB:
-        rc = launch_krylov_combine(st.k, st.N, Z(pool_, st.a), ev_, Z(work_, st.b), Z(work_, st.c), par_, elog_,
-                                   st.slot, s);
-        break;
-      case K_PERM:
-        rc = launch_permute(Z(st.abuf ? work_ : psi_, st.a), Z(st.cbuf ? work_ : psi_, st.c), st.bef, st.nn, st.aft,
-                            st.dir, s);
-        break;
-      case K_QR: {
-        SmItem u{};
-        u.a = st.a;
-        u.q = st.c;
-        u.e = -1;
-        u.m = st.bef;
-        u.n = (int32_t)st.nn;
-        rc = launch_qr_batch(nullptr, u, zero, 1, st.abuf ? (void*)work_ : (void*)psi_,
-                             st.cbuf ? (void*)work_ : (void*)psi_, s);
-        break;
-      }
-      default:
-        set_error("scf: unknown step");
-        return QTG_ESTATE;
+    case K_HEEV: {
+      SmItem u{};
+      u.a = st.a;
+      u.q = st.b;
+      u.e = 0;
+      u.m = st.k;
+      u.n = st.k;
+      const SmItem zero{};
+      return launch_heev_batch(nullptr, u, zero, 1, st.k, pool_, pool_, ev_, nullptr, 0.0, s);
     }
-    if (rc) return rc;
+    case K_KCOMB:
+      return launch_krylov_combine(st.k, st.N, Z(pool_, st.a), ev_, Z(work_, st.b), Z(work_, st.c), par_, elog_,
+                                   st.slot, s);
+    default:
+      set_error("scf: unknown step");
+      return QTG_ESTATE;
   }
-  return QTG_OK;
 }
 
 int Scf::run(double* psi_host, int64_t n_iter, int64_t n_itp, double beta, double* energies, int64_t cap) {

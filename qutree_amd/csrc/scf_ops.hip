@@ -173,7 +173,26 @@ __global__ void permute_kernel(const z* __restrict__ in, z* __restrict__ out, in
   }
 }
 
+// out = in / |in|, nrm2[0] = |in|^2 (gramSchmidt of one column,
+// Tensor_Implementation.h:806-853)
+__global__ void __launch_bounds__(kTB)
+    normalize_kernel(const z* __restrict__ in, z* __restrict__ out, int64_t N, double* __restrict__ nrm2) {
+  __shared__ double red[kTB / 64 + 1];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < N; i += kTB) s += in[i].re * in[i].re + in[i].im * in[i].im;
+  const double t = block_sum(s, red);
+  const double nrm = sqrt(t);
+  for (int64_t i = threadIdx.x; i < N; i += kTB) out[i] = mk<double>(in[i].re / nrm, in[i].im / nrm);
+  if (threadIdx.x == 0) nrm2[0] = t;
+}
+
 }  // namespace
+
+int launch_normalize(const void* in, void* out, int64_t N, double* nrm2, hipStream_t s) {
+  hipLaunchKernelGGL(normalize_kernel, dim3(1), dim3(kTB), 0, s, (const z*)in, (z*)out, N, nrm2);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
 
 int launch_lanczos_begin(const void* psi, int64_t N, void* kry0, void* in, hipStream_t s) {
   hipLaunchKernelGGL(lanczos_begin_kernel, dim3(1), dim3(kTB), 0, s, (const z*)psi, N, (z*)kry0, (z*)in);

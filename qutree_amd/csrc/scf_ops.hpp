@@ -21,6 +21,8 @@ int launch_lanczos_step(int j, int k, int64_t N, const void* ha, int parts, void
                         void* T, hipStream_t s);
 int launch_krylov_combine(int k, int64_t N, const void* U, const double* ev, const void* kry, void* out,
                           const ScfParams* par, double* elog, int slot, hipStream_t s);
+// out = in / |in|, nrm2[0] = |in|^2 (one workgroup)
+int launch_normalize(const void* in, void* out, int64_t N, double* nrm2, hipStream_t s);
 // dir 0: tensor (bef, n, aft) -> matrix (bef aft, n); dir 1: back
 int launch_permute(const void* in, void* out, int64_t bef, int64_t n, int64_t aft, int dir, hipStream_t s);
 

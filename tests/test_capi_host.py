@@ -311,3 +311,31 @@ def test_scf_plan_stays_inside_its_buffers(model):
     assert info[0] == 2 * nnodes - 1
     assert info[1] > info[0] * k  # at least one product and one Lanczos step per Krylov vector
 
+
+
+@pytest.mark.parametrize("model", ["contractive4", "quartic8", "cmf8", "sweep64", "single"])
+def test_apply_fit_plan_stays_inside_its_buffers(model):
+    """The fitted SOP application is planned on the host and every launch of
+    the first h-matrix pass and of one iteration is checked like the SCF
+    plan."""
+    import test_oracle_apply_fit as TA
+    if model == "contractive4":
+        text, H = W.balanced_tree_text(4, 3, [2, 3]), TA.contraction_sop(4, 3, 4, 1)
+    elif model == "quartic8":
+        text, H = W.balanced_tree_text(8, 2, [2, 3, 3]), TA.contraction_sop(8, 2, 4, 2)
+    elif model == "cmf8":
+        text, H = W.cmf8_model()[:2]
+    elif model == "sweep64":
+        text, H = W.sweep64_model()[:2]
+    else:
+        text = W.balanced_tree_text(4, 3, [2, 3])
+        H = W.Hamiltonian()
+        H.add_term(1.0, [(1, W.FACTOR_X), (2, W.FACTOR_KIN)])
+    L = _host()
+    L.qtg_host_apply_fit_plan.argtypes = [C.c_char_p, C.c_int64, np.ctypeslib.ndpointer(np.float64)] + \
+        [np.ctypeslib.ndpointer(np.int64)] * 4 + [np.ctypeslib.ndpointer(np.float64), np.ctypeslib.ndpointer(np.int64)]
+    c, toff, fm, fk, fp, pool = H.arrays()
+    info = np.zeros(4, np.int64)
+    rc = L.qtg_host_apply_fit_plan(text.encode(), H.nterms, c, toff, fm, fk, fp, pool, info)
+    assert rc == 0, L.qtg_last_error().decode()
+    assert info[0] >= 1
