@@ -307,7 +307,17 @@ int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, dou
  * "split_waves", "split_min_k" (set before qtg_engine_set_hamiltonian;
  * defaults 2048, 16): a grouped hole-contraction launch with fewer 16 x 16
  * tiles than split_waves splits its reductions along K into chunks of at
- * least split_min_k until it holds about split_waves wavefronts. */
+ * least split_min_k until it holds about split_waves wavefronts.
+ * "gc_long_k" (before qtg_engine_set_hamiltonian; default 512): hole
+ * contractions longer than 2 gc_long_k always split along K.
+ * "gm_split_k" (before qtg_engine_set_hamiltonian; default 32): mode products
+ * longer than 2 gm_split_k split along K in chunks of at least gm_split_k.
+ * "tuning" (before qtg_engine_set_hamiltonian; default 1): the four split
+ * options not set explicitly come from the tuning table's nearest class
+ * (qtg_engine_tuning); 0 keeps the defaults.
+ * "grad_recompute" (default 0): see qtg_engine_backward.
+ * Plan options set after qtg_engine_set_hamiltonian return QTG_ESTATE;
+ * rho_eps must be positive and reaches a captured sweep. */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
 /* The tuning-table line (qutree_amd/tuning/gfx950.txt, written by
  * qutree_amd/autotune.py) applied to this engine's plan, as text into buf
