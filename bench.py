@@ -119,18 +119,22 @@ def time_form(Q, text, H, psi, precision, compress, graph, steps, path=1):
             "executed_flops": flops}
 
 
-def cmf128_leg(Q, W, steps):
+def cmf128_leg(Q, W, steps, chunks=1):
     """The precision run of the 128-mode tree (cfg5's workload): CMF at
     dt = 0.05 with rho regularised at eps = 1e-8 from a random state, in
     complex128 and in the split form (complex64 bulk contractions; FP64 for
     the large terms, the densities, rho^-1, the projector and step control),
     the same number of CMF iterations each: iterations per second and the
     split form's energy and autocorrelation error against complex128 at the
-    common output times."""
+    common output times.  With chunks > 1 each form runs as that many
+    integrate calls in a row (each continues from the last state and time;
+    the step control restarts at dt), with a progress line per chunk on
+    stderr; autocorrelations are then those within each chunk."""
     import numpy as np
     text, H = W.cmf128_model()
     res = {"workload": f"128-mode tree, 128 + 8128 coupled-quartic terms, CMF dt = 0.05, outputs every 0.05, "
-                       f"eps 1e-8, {steps} iterations from fill_random(128)"}
+                       f"eps 1e-8, {steps} iterations from fill_random(128)"
+                       + (f" in {chunks} integrate calls" if chunks > 1 else "")}
     ref = None
     for prec, name in ((0, "complex128"), (2, "complex64_split")):
         eng = Q.Engine(text, H, precision=prec, rho_eps=1e-8, graph=False)
@@ -138,13 +142,31 @@ def cmf128_leg(Q, W, steps):
         cmf = Q.CMF(eng)
         cmf.integrate(psi, 0.0, 1000.0, 0.05, 0.05, max_steps=1)  # warm-up
         Q.synchronize()
-        t0 = time.perf_counter()
-        r = cmf.integrate(psi, 0.0, 1000.0, 0.05, 0.05, max_steps=steps)
-        Q.synchronize()
-        dt = time.perf_counter() - t0
+        r = {"steps": 0, "rejected": 0, "derivs": 0, "times": [], "energies": [], "autocorr": []}
+        cur, t, dt = psi, 0.0, 0.0
+        for c in range(chunks):
+            n = steps // chunks + (steps % chunks if c == chunks - 1 else 0)
+            t0 = time.perf_counter()
+            rc = cmf.integrate(cur, t, 1000.0, 0.05, 0.05, max_steps=n)
+            Q.synchronize()
+            dt += time.perf_counter() - t0
+            skip = 0 if c == 0 else 1  # a later chunk's first output is the last chunk's end
+            for k in ("steps", "rejected", "derivs"):
+                r[k] += rc[k]
+            for k in ("times", "energies", "autocorr"):
+                r[k] += list(rc[k][skip:])
+            cur, t = rc["psi"], float(rc["times"][-1])
+            if chunks > 1:
+                print(f"bench: cmf128 {name} chunk {c + 1}/{chunks} t {t:.3f} steps {r['steps']} "
+                      f"{dt:.1f} s", file=sys.stderr, flush=True)
+        for k in ("times", "energies", "autocorr"):
+            r[k] = np.asarray(r[k])
         out = {"iterations_per_s": round(r["steps"] / dt, 3), "iterations": r["steps"], "rejected": r["rejected"],
                "node_derivatives": r["derivs"], "seconds": round(dt, 3),
                "t_reached": float(r["times"][-1]) if len(r["times"]) else None}
+        if len(r["energies"]):
+            out["energy_first"] = float(r["energies"][0])
+            out["energy_last"] = float(r["energies"][-1])
         if ref is None:
             ref = r
         else:
@@ -305,6 +327,52 @@ def scf_leg(Q, W, iters=10, cpu=True, eager=False):
     return out
 
 
+def apply_fit_leg(Q, W, cpu=True):
+    """The SOP application fitted into a tree (ApplyFirstOrder.cpp:43-147) on
+    the 32-mode tree of the gradient workload (8 DVR, SPF 8 then 24): 32
+    nearest-neighbour products A_i (x) B_{i+1} of random 8 x 8 leaf matrices
+    with spectral norm 0.15, from a random state and a seeded start, with the
+    reference's stopping rule (here after 4 iterations, when the fidelity
+    falls); iterations per second on the device and the oracle timed on the
+    first two iterations of the same run (one thread)."""
+    import numpy as np
+    text, _ = W.grad32_model()
+    rng = np.random.default_rng(5)
+    H = W.Hamiltonian()
+    for i in range(32):
+        mats = []
+        for _ in range(2):
+            A = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+            mats.append(0.15 * A / np.linalg.norm(A, 2))
+        H.add_term(1.0, [(i, W.FACTOR_MATRIX, mats[0]), ((i + 1) % 32, W.FACTOR_MATRIX, mats[1])])
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(64, delta_lowest=False)
+    fit = Q.ApplyFit(eng)
+    fit.run(psi, 3, 2)  # warm-up: plans and captures the iteration graph
+    t0 = time.perf_counter()
+    r = fit.run(psi, 3, 100)
+    dt = time.perf_counter() - t0
+    n = len(r["fidelities"])
+    out = {"workload": "32-mode tree (8 DVR, SPF 8 then 24), 32 nearest-neighbour products of random 8 x 8 "
+                       "matrices (spectral norm 0.15), random state, seed 3",
+           "iterations": n, "seconds": round(dt, 4), "iterations_per_s": round(n / dt, 3),
+           "ms_per_iteration": round(dt / n * 1e3, 3), "fidelity_final": float(r["fidelities"][-1]),
+           "launches_per_iteration": fit.info()["launches"]}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        om = O.OracleModel(text, H)
+        c0 = time.perf_counter()
+        ref = om.apply_fit(psi, 3, 2)
+        c1 = time.perf_counter()
+        out["cpu_oracle"] = {"seconds_per_iteration": round((c1 - c0) / len(ref["fidelities"]), 3), "cores": 1,
+                             "kind": "port", "sample": "the first two iterations of the same run"}
+        out["speedup_vs_cpu_oracle"] = round(out["cpu_oracle"]["seconds_per_iteration"] / (dt / n), 2)
+        out["fidelity_rel_err_vs_oracle_first_two"] = float(
+            np.max(np.abs(r["fidelities"][:2] - ref["fidelities"]) / np.abs(ref["fidelities"])))
+    return out
+
+
 def big_tree_leg(Q, W, steps=3):
     """One H|Psi> sweep on the 128-mode tree (8 DVR, SPF 8..48 toward the root,
     128 + 8128 coupled-quartic terms) in complex128, complex64 bulk and the
@@ -404,6 +472,10 @@ def main() -> int:
     ap.add_argument("--gd-steps", type=int, default=200, help="gradient-descent steps of the gradient leg")
     ap.add_argument("--cmf128-steps", type=int, default=20,
                     help="CMF iterations of the 128-mode precision leg (0: skip; the full run is 1000)")
+    ap.add_argument("--cmf128-chunks", type=int, default=1,
+                    help="run the 128-mode precision leg as this many integrate calls (progress on stderr)")
+    ap.add_argument("--cmf128-only", action="store_true",
+                    help="only run the 128-mode precision leg and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     ap.add_argument("--scf-only", action="store_true",
@@ -415,6 +487,11 @@ def main() -> int:
         import qutree_amd as Q
         from qutree_amd import workloads as W
         print(json.dumps({"scf": scf_leg(Q, W, iters=3, cpu=False, eager=True)}))
+        return 0
+    if args.cmf128_only:
+        import qutree_amd as Q
+        from qutree_amd import workloads as W
+        print(json.dumps({"cmf128": cmf128_leg(Q, W, args.cmf128_steps, args.cmf128_chunks)}))
         return 0
     if args.copy_probe > 0:
         import ctypes as C
@@ -599,13 +676,23 @@ def main() -> int:
                                 "complex64_split_speedup_vs_plain":
                                     round(split_cmf["steps_per_s"] / plain_cmf["steps_per_s"], 3)}
     if not args.no_extras and args.precision == "complex128":
+        def leg(name):  # a progress line per leg on stderr (stdout keeps the one JSON line)
+            print(f"bench: {name} done", file=sys.stderr, flush=True)
+
         out["gradient"] = gradient_leg(Q, W, args.gd_steps)
+        leg("gradient")
         out["tree128"] = big_tree_leg(Q, W)
+        leg("tree128")
         if args.cmf128_steps > 0:
-            out["cmf128"] = cmf128_leg(Q, W, args.cmf128_steps)
+            out["cmf128"] = cmf128_leg(Q, W, args.cmf128_steps, args.cmf128_chunks)
+            leg("cmf128")
         out["node_microbench"] = microbench_leg(Q, W)
         out["eigenstates"], eig_psi = eigenstates_leg(Q, W)
+        leg("microbench, eigenstates")
         out["scf"] = scf_leg(Q, W, cpu=not args.no_cpu)
+        leg("scf")
+        out["apply_fit"] = apply_fit_leg(Q, W, cpu=not args.no_cpu)
+        leg("apply_fit")
     if not args.no_cpu:
         base, e_ref = cpu_baseline(text, H, psi, energy)
         out["cpu_baseline"] = base

@@ -316,6 +316,8 @@ int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, dou
  * options not set explicitly come from the tuning table's nearest class
  * (qtg_engine_tuning); 0 keeps the defaults.
  * "grad_recompute" (default 0): see qtg_engine_backward.
+ * "heev_side" (default 1): 1 runs the grouped sweep's density eigensolver on
+ * a second stream, overlapping the hole contractions; 0 runs it in line.
  * Plan options set after qtg_engine_set_hamiltonian return QTG_ESTATE;
  * rho_eps must be positive and reaches a captured sweep. */
 int qtg_engine_set_option(qtg_engine engine, const char* key, double value);
@@ -348,6 +350,11 @@ int qtg_cmf_create(qtg_engine engine, qtg_cmf* cmf);
  * truncating, and writes nothing when the file is missing
  * (TensorTree_Implementation.h:93-100, ios::in).  filename NULL: off. */
 int qtg_cmf_set_output_file(qtg_cmf cmf, const char* filename, int append);
+/* "graph" (default 1): each lockstep tick (pre programs, masked Apply and
+ * finish, post programs) is replayed from a HIP graph captured once per
+ * state buffer and phase; only the tick's programs are uploaded.  0 launches
+ * every tick eagerly.  Results are the same either way. */
+int qtg_cmf_set_option(qtg_cmf cmf, const char* key, double value);
 int qtg_cmf_destroy(qtg_cmf cmf);
 int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend, double dt, double out,
                           double acc_cmf, double acc_bs, double phase_re, double phase_im,

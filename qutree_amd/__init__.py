@@ -91,6 +91,7 @@ def lib():
     L.qtg_engine_synchronize.argtypes = [_vp]
     L.qtg_cmf_create.argtypes = [_vp, C.POINTER(_vp)]
     L.qtg_cmf_destroy.argtypes = [_vp]
+    L.qtg_cmf_set_option.argtypes = [_vp, C.c_char_p, C.c_double]
     L.qtg_scf_create.argtypes = [_vp, C.c_int64, C.POINTER(_vp)]
     L.qtg_scf_destroy.argtypes = [_vp]
     L.qtg_scf_sweep_length.argtypes = [_vp]
@@ -403,6 +404,10 @@ class CMF:
         check(lib().qtg_cmf_create(engine.ptr, C.byref(p)))
         self.ptr = p.value
         self.engine = engine
+
+    def set_option(self, key: str, value) -> None:
+        """qtg_cmf_set_option (\"graph\": 1 replays ticks from HIP graphs)."""
+        check(lib().qtg_cmf_set_option(self.ptr, key.encode(), float(value)))
 
     def integrate(self, psi, t0, tend, dt, out, acc_cmf=1e-4, acc_bs=1e-5, phase=1.0, max_steps=10 ** 9,
                   max_out=100000):

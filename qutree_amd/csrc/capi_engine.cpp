@@ -373,6 +373,11 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
     en->invalidate_graph();
     return QTG_OK;
   }
+  if (!std::strcmp(key, "heev_side")) {
+    en->heevSide_ = value != 0.0;
+    en->invalidate_graph();
+    return QTG_OK;
+  }
   set_error(std::string("unknown option ") + key);
   return QTG_EARG;
 }
@@ -388,6 +393,17 @@ int qtg_cmf_create(qtg_engine e, qtg_cmf* out) {
   }
   *out = c;
   return QTG_OK;
+}
+
+int qtg_cmf_set_option(qtg_cmf c, const char* key, double value) {
+  if (!c || !key) return QTG_EARG;
+  auto* cm = static_cast<Cmf*>(c);
+  if (!std::strcmp(key, "graph")) {
+    cm->tickGraph_ = value != 0.0;
+    return QTG_OK;
+  }
+  set_error(std::string("unknown cmf option ") + key);
+  return QTG_EARG;
 }
 
 int qtg_cmf_set_output_file(qtg_cmf c, const char* filename, int append) {

@@ -28,6 +28,7 @@ constexpr int kOpsPerNode = 16;
 }  // namespace
 
 Cmf::~Cmf() {
+  drop_tick_graphs();
   for (void* p : dev_)
     if (p) (void)hipFree(p);
   if (host_stage_) (void)hipHostFree(host_stage_);
@@ -73,7 +74,7 @@ int Cmf::init() {
   if ((rc = dalloc((void**)&rhoslot_, sizeof(int64_t) * NN_))) return rc;
   QTG_HIP_TRY(hipMemcpy(rhoslot_, rs.data(), sizeof(int64_t) * NN_, hipMemcpyHostToDevice));
   // staging for per-tick programs and the active mask
-  stage_bytes_ = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode) + NN_ + 64;
+  stage_bytes_ = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode) + NN_ + 1024;
   if (hipHostMalloc(&host_stage_, 2 * stage_bytes_) != hipSuccess) {
     set_error("cmf: pinned allocation failed");
     return QTG_ENOMEM;
@@ -447,6 +448,85 @@ void Cmf::finish_mmid(int v, std::vector<VecOp>& post) {
   b.st = NodeBS::AWAIT_ERR;
 }
 
+// Fixed layout of the device staging of a tick: the active mask, every
+// node's pre and post program (empty ones have op0 == op1) and the ops they
+// index.  The launches of a tick then read fixed addresses, so the tick can
+// be replayed from a HIP graph; only the upload's length varies.
+Cmf::TickLayout Cmf::tick_layout() const {
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  TickLayout L;
+  L.pre = align((size_t)NN_);
+  L.post = L.pre + sizeof(NodeProg) * NN_;
+  L.ops = align(L.post + sizeof(NodeProg) * NN_);
+  return L;
+}
+
+// The launches of one tick on the staged programs: pre programs, the masked
+// Apply and finish, post programs.
+int Cmf::tick_launches(const CmfBufs& bb) {
+  const TickLayout L = tick_layout();
+  unsigned char* ds = (unsigned char*)dev_stage_;
+  const VecOp* ops = (const VecOp*)(ds + L.ops);
+  int rc = launch_programs((const NodeProg*)(ds + L.pre), NN_, ops, bb, E.d_nodes_, (const cplx<double>*)E.d_pool_,
+                           rhoslot_, err_, s_);
+  if (rc) return rc;
+  if ((rc = E.apply_finish((const double*)X_, (double*)D_, ds, ph_re_, ph_im_, nullptr, s_))) return rc;
+  return launch_programs((const NodeProg*)(ds + L.post), NN_, ops, bb, E.d_nodes_, (const cplx<double>*)E.d_pool_,
+                         rhoslot_, err_, s_);
+}
+
+void Cmf::drop_tick_graphs() {
+  for (hipGraphExec_t& g : tickExec_)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// One tick on the engine stream: replayed from the graph of its state
+// buffer (captured on first use, again after a phase change), or eagerly
+// while graphs are off or the engine times its kernels.
+int Cmf::run_tick(cplx<double>* P) {
+  // rho^-1 of the last build comes from the engine's side stream: wait for
+  // it here, outside any capture
+  if (E.heevPending_) {
+    QTG_HIP_TRY(hipStreamWaitEvent(s_, E.evHeev_, 0));
+    E.heevPending_ = false;
+  }
+  const CmfBufs bb = bufs(P);
+  if (!tickGraph_ || E.timing_) return tick_launches(bb);
+  if (tickPh_[0] != ph_re_ || tickPh_[1] != ph_im_) {
+    drop_tick_graphs();
+    tickPh_[0] = ph_re_;
+    tickPh_[1] = ph_im_;
+  }
+  const int w = (P == P2_) ? 1 : 0;
+  if (P != P_ && P != P2_) return tick_launches(bb);
+  if (!tickExec_[w]) {
+    QTG_HIP_TRY(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    int rc = tick_launches(bb);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s_, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error(std::string("cmf tick capture: ") + hipGetErrorString(e));
+      return QTG_EDEVICE;
+    }
+    hipError_t ie = hipGraphInstantiate(&tickExec_[w], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      tickExec_[w] = nullptr;
+      set_error(std::string("cmf tick graph: ") + hipGetErrorString(ie));
+      return QTG_EDEVICE;
+    }
+  }
+  QTG_HIP_TRY(hipGraphLaunch(tickExec_[w], s_));
+  return QTG_OK;
+}
+
 int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
   for (int v = 0; v < NN_; ++v) {
     bs_[v].st = NodeBS::NEED_START;
@@ -454,21 +534,20 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
     bs_[v].xend = t1;
   }
   const CmfBufs bb = bufs(P);
+  const TickLayout L = tick_layout();
   unsigned char* ds = (unsigned char*)dev_stage_;
   int half = 0;
-  std::vector<NodeProg> ppre, ppost;
-  std::vector<VecOp> opre, opost, pre, post;
+  std::vector<NodeProg> ppre(NN_), ppost(NN_);
+  std::vector<VecOp> ops, pre, post;
   std::vector<uint8_t> act(NN_);
-  auto align = [](size_t x) { return (x + 63) & ~size_t(63); };
   while (true) {
-    ppre.clear();
-    ppost.clear();
-    opre.clear();
-    opost.clear();
-    int nact = 0;
+    ops.clear();
+    int nact = 0, npre = 0;
     bool need_err = false;
     for (int v = 0; v < NN_; ++v) {
       act[v] = 0;
+      ppre[v] = NodeProg{v, 0, 0, 0};
+      ppost[v] = NodeProg{v, 0, 0, 0};
       if (bs_[v].st == NodeBS::DONE) continue;
       pre.clear();
       post.clear();
@@ -478,22 +557,21 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
         return QTG_ESTATE;
       }
       if (!pre.empty()) {
-        ppre.push_back(NodeProg{v, (int)opre.size(), (int)(opre.size() + pre.size()), 0});
-        opre.insert(opre.end(), pre.begin(), pre.end());
+        ppre[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + pre.size()), 0};
+        ops.insert(ops.end(), pre.begin(), pre.end());
+        ++npre;
       }
       if (!d) continue;
       act[v] = 1;
       ++nact;
       ++nderiv_;
-      ppost.push_back(NodeProg{v, (int)opost.size(), (int)(opost.size() + post.size()), 0});
+      ppost[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + post.size()), 0};
       for (auto& o : post)
         if (o.code == V_ERROR) need_err = true;
-      opost.insert(opost.end(), post.begin(), post.end());
+      ops.insert(ops.end(), post.begin(), post.end());
     }
-    // one compact upload: [pre progs][pre ops][post progs][post ops][active]
-    size_t o1 = 0, o2 = align(o1 + sizeof(NodeProg) * ppre.size());
-    size_t o3 = align(o2 + sizeof(VecOp) * opre.size()), o4 = align(o3 + sizeof(NodeProg) * ppost.size());
-    size_t o5 = align(o4 + sizeof(VecOp) * opost.size()), oend = o5 + NN_;
+    if (nact == 0 && npre == 0) break;
+    const size_t oend = L.ops + sizeof(VecOp) * ops.size();
     if (oend > stage_bytes_) {
       set_error("cmf: staging overflow");
       return QTG_ESTATE;
@@ -504,24 +582,23 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
     // region is reused in stream order.
     unsigned char* hs = (unsigned char*)host_stage_ + half * stage_bytes_;
     QTG_HIP_TRY(hipEventSynchronize(evStage_[half]));
-    std::memcpy(hs + o1, ppre.data(), sizeof(NodeProg) * ppre.size());
-    std::memcpy(hs + o2, opre.data(), sizeof(VecOp) * opre.size());
-    std::memcpy(hs + o3, ppost.data(), sizeof(NodeProg) * ppost.size());
-    std::memcpy(hs + o4, opost.data(), sizeof(VecOp) * opost.size());
-    std::memcpy(hs + o5, act.data(), NN_);
+    std::memcpy(hs, act.data(), NN_);
+    std::memcpy(hs + L.pre, ppre.data(), sizeof(NodeProg) * NN_);
+    std::memcpy(hs + L.post, ppost.data(), sizeof(NodeProg) * NN_);
+    std::memcpy(hs + L.ops, ops.data(), sizeof(VecOp) * ops.size());
     QTG_HIP_TRY(hipMemcpyAsync(ds, hs, oend, hipMemcpyHostToDevice, s_));
     QTG_HIP_TRY(hipEventRecord(evStage_[half], s_));
     half ^= 1;
-    int rc = launch_programs((NodeProg*)(ds + o1), (int64_t)ppre.size(), (VecOp*)(ds + o2), bb, E.d_nodes_,
-                             (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
-    if (rc) return rc;
-    if (nact == 0) break;  // only trailing accept copies were left
+    if (nact == 0) {
+      // only trailing accept copies were left
+      int rc = launch_programs((const NodeProg*)(ds + L.pre), NN_, (const VecOp*)(ds + L.ops), bb, E.d_nodes_,
+                               (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
+      if (rc) return rc;
+      break;
+    }
     ++nticks_;
-    const uint8_t* dact = (const uint8_t*)(ds + o5);
-    if ((rc = E.apply_finish((const double*)X_, (double*)D_, dact, ph_re_, ph_im_, nullptr, s_))) return rc;
-    if ((rc = launch_programs((NodeProg*)(ds + o3), (int64_t)ppost.size(), (VecOp*)(ds + o4), bb, E.d_nodes_,
-                              (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_)))
-      return rc;
+    int rc = run_tick(P);
+    if (rc) return rc;
     if (need_err) {
       // the next tick's decisions read the errors
       QTG_HIP_TRY(hipMemcpyAsync(err_h_, err_, sizeof(double) * NN_, hipMemcpyDeviceToHost, s_));
@@ -544,7 +621,7 @@ int Cmf::error(double* out) {
     hp[v] = NodeProg{v, v, v + 1, 0};
     ho[v] = op0(V_ERROR, B_E1, B_E2);
   }
-  QTG_HIP_TRY(hipMemcpyAsync(ds, hs, stage_bytes_, hipMemcpyHostToDevice, s_));
+  QTG_HIP_TRY(hipMemcpyAsync(ds, hs, progB + sizeof(VecOp) * NN_, hipMemcpyHostToDevice, s_));
   int rc = launch_programs((NodeProg*)ds, NN_, (VecOp*)(ds + progB), bufs(P_), E.d_nodes_,
                            (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
   if (rc) return rc;

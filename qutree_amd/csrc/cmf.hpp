@@ -53,6 +53,9 @@ class Cmf {
   // start and at every output; empty name: off
   std::string outFile_;
   bool outAppend_ = true;
+  // ticks replayed from HIP graphs (one per state buffer and phase); false
+  // launches every tick eagerly
+  bool tickGraph_ = true;
 
  private:
   int eig_init();
@@ -66,6 +69,15 @@ class Cmf {
   bool advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, double eps);
   void finish_mmid(int v, std::vector<VecOp>& post);
   int cmfstep(cplx<double>* P, double t0, double t1, double eps);
+  struct TickLayout {
+    size_t pre, post, ops;  // byte offsets in the staging; the active mask at 0
+  };
+  TickLayout tick_layout() const;
+  int tick_launches(const CmfBufs& bb);
+  int run_tick(cplx<double>* P);
+  void drop_tick_graphs();
+  hipGraphExec_t tickExec_[2] = {nullptr, nullptr};  // ticks on P_ and on P2_
+  double tickPh_[2] = {0.0, 0.0};                    // the phase they were captured with
   int error(double* out);
   int orthogonal(cplx<double>* P);
   int write_state(const cplx<double>* P);

@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(TB)
   __shared__ double red[TB];
   __shared__ cd2 rhos[16 * 16];
   const NodeProg pg = progs[blockIdx.x];
+  if (pg.op0 >= pg.op1) return;  // an empty program (the whole workgroup)
   const DNode nd = nodes[pg.node];
   const int N = nd.N;
   const int64_t off = nd.off;

@@ -102,6 +102,11 @@ class Engine {
   // QR gauge of the state on the device (see gradient_descent)
   int canonicalize(hipStream_t s);
   bool gradRecompute_ = false;
+  // the grouped sweep's eigensolver on the side stream (fork after the
+  // build, join before the solve) instead of in line on the engine stream;
+  // measured equal for replayed sweeps and faster for CMF, whose first tick
+  // after a build overlaps the eigensolver
+  bool heevSide_ = true;
   // device bytes this engine holds
   int64_t device_bytes() const { return deviceBytes_; }
   hipStream_t stream() const { return stream_; }

@@ -886,13 +886,17 @@ void Engine::dump_plan() const {
           ++outs;
         }
       } else if (g.kind == 1) {
+        std::set<std::pair<int, int64_t>> xs, ys;
         for (int64_t t = 0; t < g.ntasks; ++t) {
           const GcDesc& d = h_gc_[g.d0 + t];
           if (d.tile0 != t) continue;
           kmin = std::min(kmin, d.bef * d.aft);
           kmax = std::max(kmax, d.bef * d.aft);
+          xs.insert({d.sel & 1, d.x});
+          ys.insert({(d.sel >> 1) & 1, d.y});
           ++outs;
         }
+        std::fprintf(stderr, "plan %-5s %2zu distinct bra %zu ket %zu\n", name, i, xs.size(), ys.size());
       }
       std::fprintf(stderr, "plan %-5s %2zu kind %d prec %d tasks %7lld descs %6ld K %d..%d flops %.3e bytes %.3e\n",
                    name, i, g.kind, g.prec, (long long)g.ntasks, outs, kmin, kmax, g.flops, g.bytes);
@@ -901,6 +905,19 @@ void Engine::dump_plan() const {
   one("build", gBuild_);
   one("hole", gHole_);
   one("apply", gApply_);
+  // the two combination launches: outputs, terms, longest term list
+  auto comb = [](const char* name, const std::vector<CombOut>& co) {
+    int64_t terms = 0, tmax = 0, elems = 0;
+    for (const CombOut& c : co) {
+      terms += c.t1 - c.t0;
+      tmax = std::max<int64_t>(tmax, c.t1 - c.t0);
+      elems += (int64_t)c.n * c.n;
+    }
+    std::fprintf(stderr, "plan %-8s outputs %6zu terms %8lld longest %6lld elements %9lld\n", name, co.size(),
+                 (long long)terms, (long long)tmax, (long long)elems);
+  };
+  comb("combine1", h_comb1_);
+  comb("combine3", h_comb3_);
 }
 
 int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, void* const* bases,
@@ -958,16 +975,23 @@ int Engine::build_gemm(hipStream_t s) {
   void* bases[B_COUNT] = {d_psi32_, d_work32_, d_pool_, d_psi64_, d_apply64_, d_dpsi64_, d_pool64_, d_work_};
   // represent and the density tree, layer by layer in shared launches
   if ((rc = run_launches(gBuild_, bases, nullptr, 1.0, 0.0, s))) return rc;
-  // side stream: the eigensolver (it only feeds the solve of the finish,
-  // which apply_gemm makes wait for it); engine stream: the holes, the
-  // combinations and Apply
-  QTG_HIP_TRY(hipEventRecord(evFork_, s));
-  QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
-  if ((rc = begin_mark(1, side_))) return rc;
-  if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
-  if ((rc = end_mark(side_))) return rc;
-  QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
-  heevPending_ = true;
+  if (heevSide_) {
+    // side stream: the eigensolver (it only feeds the solve of the finish,
+    // which apply_gemm makes wait for it); engine stream: the holes, the
+    // combinations and Apply
+    QTG_HIP_TRY(hipEventRecord(evFork_, s));
+    QTG_HIP_TRY(hipStreamWaitEvent(side_, evFork_, 0));
+    if ((rc = begin_mark(1, side_))) return rc;
+    if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, side_))) return rc;
+    if ((rc = end_mark(side_))) return rc;
+    QTG_HIP_TRY(hipEventRecord(evHeev_, side_));
+    heevPending_ = true;
+  } else {
+    // in line on the engine stream
+    if ((rc = begin_mark(1, s))) return rc;
+    if ((rc = launch_heev<double>(d_eig_, nEig_, d_pool_, d_pool64_, eps_, nspf_, s))) return rc;
+    if ((rc = end_mark(s))) return rc;
+  }
   if ((rc = run_launches(gHole_, bases, nullptr, 1.0, 0.0, s))) return rc;
   if ((rc = begin_mark(5, s))) return rc;
   // the aggregated slots, then the sums over them with the stacked blocks

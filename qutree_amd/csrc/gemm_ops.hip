@@ -163,8 +163,17 @@ __device__ inline void gm_copy(const GmDesc& d, const GmTile& g, const cplx<T>* 
   }
 }
 
+// Output rows are contiguous in memory in runs of bef elements.  With
+// bef >= 4 the tile is accumulated transposed (the operands swap roles in
+// the MFMA), so that a lane holds one row and the 16 lanes of a store write
+// 16 consecutive rows; with bef < 4 a lane holds one column (consecutive
+// columns are bef apart).
+__device__ inline bool gm_rows_on_lanes(const GmDesc& d) { return d.bef >= 4; }
+
 // This chunk's share of the product tile, alpha-scaled, in the output
-// precision: v[q] is row rt*16 + drow(lane, q), column ct*16 + lane % 16.
+// precision: v[q] is row rt*16 + lane % 16, column ct*16 + drow(lane, q)
+// when gm_rows_on_lanes, else row rt*16 + drow(lane, q), column
+// ct*16 + lane % 16.
 template <typename T, typename TM, typename TO>
 __device__ inline void gm_product(const GmDesc& d, const GmTile& g, const cplx<T>* __restrict__ X,
                                   const cplx<TM>* __restrict__ M, double sre, double sim, cplx<TO> (&v)[4]) {
@@ -194,6 +203,10 @@ __device__ inline void gm_product(const GmDesc& d, const GmTile& g, const cplx<T
   const int kchunk = ((nin + split - 1) / split + 3) & ~3;
   const int kbeg = g.ks * kchunk, kend = min(nin, kbeg + kchunk);
   v4 cr0 = {0, 0, 0, 0}, ci0 = {0, 0, 0, 0}, cr1 = {0, 0, 0, 0}, ci1 = {0, 0, 0, 0};
+  // A lane's X value is element (row li, k kk) of X, its M value element
+  // (k kk, column li) of op(M)^T: passed as the MFMA's (A, B) they give the
+  // tile, as (B, A) its transpose.
+  const bool tr = gm_rows_on_lanes(d);
   // four k-steps per iteration: all loads are issued before the MFMAs
   for (int k0 = kbeg; k0 < kend; k0 += 16) {
     cplx<T> av[4], bv[4];
@@ -201,9 +214,11 @@ __device__ inline void gm_product(const GmDesc& d, const GmTile& g, const cplx<T
     for (int u = 0; u < 4; ++u) {
       const int kx = k0 + 4 * u + kk;
       const bool kv = kx < kend;
-      av[u] = ldz(xa, xoff + (int64_t)kx * bef, rv && kv);
+      const cplx<T> xv = ldz(xa, xoff + (int64_t)kx * bef, rv && kv);
       const cplx<TM> mv = ldz(mb, moff + (int64_t)kx * mks, cv && kv);
-      bv[u] = mk<T>((T)mv.re, conjm ? -(T)mv.im : (T)mv.im);
+      const cplx<T> m = mk<T>((T)mv.re, conjm ? -(T)mv.im : (T)mv.im);
+      av[u] = tr ? m : xv;
+      bv[u] = tr ? xv : m;
     }
     cmma<T>(cr0, ci0, av[0], bv[0]);
     if (k0 + 4 < kend) cmma<T>(cr1, ci1, av[1], bv[1]);
@@ -221,6 +236,25 @@ template <typename T, typename TO>
 __device__ inline void gm_store(const GmDesc& d, const GmTile& g, const cplx<TO> (&v)[4], cplx<TO>* O) {
   const int lane = threadIdx.x & 63;
   const int64_t bef = d.bef, R = bef * d.aft;
+  if (gm_rows_on_lanes(d)) {
+    // one row per lane, its four columns drow(lane, q)
+    const int64_t r = g.rt * 16 + (lane & 15);
+    if (r >= R) return;
+    int64_t b, a;
+    row_split(r, bef, R, b, a);
+    const int64_t o0 = d.out + b + a * bef * d.nout;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int jq = g.ct * 16 + MF<T>::drow(lane, q);
+      if (jq < d.nout) {
+        const int64_t o = o0 + (int64_t)jq * bef;
+        cplx<TO> x = v[q];
+        if (d.acc) x = cadd(x, O[o]);
+        O[o] = x;
+      }
+    }
+    return;
+  }
   const int j = g.ct * 16 + (lane & 15);
   if (j >= d.nout) return;
   // the lane's four rows are rr0 + q * st: (b, a) of the first by division,
@@ -249,6 +283,166 @@ __device__ inline void gm_store(const GmDesc& d, const GmTile& g, const cplx<TO>
   }
 }
 
+// ------------------------------------------------- mode product, lean form
+// A descriptor whose three tensors each span fewer than 2^27 elements (every
+// node tensor of the tree sweeps) takes this form: 32-bit element offsets
+// from wave-uniform bases (global loads and stores with a scalar base and a
+// 32-bit lane offset), rows and columns beyond the tile's range clamped into
+// it instead of predicated (their products land in tile rows and columns
+// that are never stored), and only the K tail zeroed.  The tile is
+// accumulated transposed when bef >= 4 (gm_rows_on_lanes).
+__device__ inline bool gm_lean_ok(const GmDesc& d) {
+  constexpr int64_t lim = (int64_t)1 << 27;
+  return d.op != GM_COPY && (int64_t)d.bef * d.nin * d.aft < lim && (int64_t)d.bef * d.nout * d.aft < lim &&
+         (int64_t)d.nin * d.nout < lim && (d.ksplit & (d.ksplit - 1)) == 0;
+}
+
+// base[idx] for idx < 2^27: a 32-bit byte offset from the base
+template <typename V>
+__device__ inline V ld32(const V* base, uint32_t idx) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + (idx * (uint32_t)sizeof(V)));
+}
+template <typename V>
+__device__ inline V* at32(V* base, uint32_t idx) {
+  return reinterpret_cast<V*>(reinterpret_cast<char*>(base) + (idx * (uint32_t)sizeof(V)));
+}
+
+// Lean tile coordinates: no division when the output has one column tile
+__device__ inline GmTile gm_tile_lean(const GmDesc& d, uint32_t t) {
+  GmTile g;
+  const int lg = d.ksplit > 1 ? __builtin_ctz((unsigned)d.ksplit) : 0;
+  g.ks = (int)(t & ((1u << lg) - 1u));
+  t >>= lg;
+  const unsigned CT = (unsigned)(d.nout + 15) >> 4;
+  if (CT == 1) {
+    g.rt = t;
+    g.ct = 0;
+  } else {
+    const unsigned rt = t / CT;
+    g.rt = rt;
+    g.ct = (int)(t - rt * CT);
+  }
+  return g;
+}
+
+// The K loop of the lean form.  TR: the tile is accumulated transposed (X
+// as the MFMA's B operand); CONJ: op(M) = M^H.  One accumulator pair: the
+// MFMAs of a k-step alternate between the real and imaginary part, and
+// consecutive MFMAs into the same accumulator chain without a stall.
+template <typename T, typename TM, bool TR, bool CONJ>
+__device__ inline void gm_lean_loop(const cplx<T>* __restrict__ xa, uint32_t xrow, uint32_t bef,
+                                    const cplx<TM>* __restrict__ mb, uint32_t mrow, uint32_t mks, int kbeg, int kend,
+                                    int kk, typename MF<T>::v4& cr, typename MF<T>::v4& ci) {
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+    cplx<T> xv[4], mv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kx = k0 + 4 * u + kk;
+      const bool kv = kx < kend;
+      const uint32_t kc = (uint32_t)(kv ? kx : kend - 1);
+      const cplx<T> x = ld32(xa, xrow + kc * bef);
+      const cplx<TM> m = ld32(mb, mrow + kc * mks);
+      xv[u] = mk<T>(kv ? x.re : (T)0, kv ? x.im : (T)0);
+      mv[u] = mk<T>((T)m.re, CONJ ? -(T)m.im : (T)m.im);
+    }
+    // every operand is taken here, so the compiler keeps the eight loads
+    // together ahead of the MFMAs instead of sinking them into the k-step
+    // guards below (one memory round trip per iteration, not four)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) asm volatile("" ::"v"(xv[u].re), "v"(xv[u].im), "v"(mv[u].re), "v"(mv[u].im));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (u > 0 && k0 + 4 * u >= kend) break;
+      const cplx<T>& a = TR ? mv[u] : xv[u];
+      const cplx<T>& b = TR ? xv[u] : mv[u];
+      cr = MF<T>::mma(a.re, b.re, cr);
+      ci = MF<T>::mma(a.re, b.im, ci);
+      cr = MF<T>::mma(-a.im, b.im, cr);
+      ci = MF<T>::mma(a.im, b.re, ci);
+    }
+  }
+}
+
+template <typename T, typename TM, typename TO>
+__device__ inline void gm_lean(const GmDesc& d, const GmTile& g, const cplx<T>* __restrict__ X,
+                               const cplx<TM>* __restrict__ M, double sre, double sim, cplx<TO> (&v)[4]) {
+  typedef typename MF<T>::v4 v4;
+  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const uint32_t bef = (uint32_t)d.bef, R = bef * (uint32_t)d.aft;
+  const int nin = d.nin, nout = d.nout;
+  const int lg = d.ksplit > 1 ? __builtin_ctz((unsigned)d.ksplit) : 0;
+  // K range of this chunk (a multiple of 4 long, except the last)
+  const int kchunk = (((nin + (1 << lg) - 1) >> lg) + 3) & ~3;
+  const int kbeg = g.ks * kchunk, kend = min(nin, kbeg + kchunk);
+  // the lane's row (clamped) and its (b, a)
+  const uint32_t r = (uint32_t)g.rt * 16u + (uint32_t)li;
+  const uint32_t rc = min(r, R - 1u);
+  const uint32_t a = rc / bef, b = rc - a * bef;
+  const uint32_t xrow = b + a * bef * (uint32_t)nin;
+  // the lane's column of op(M) (clamped): M(j, k) at j + k nout (N), M(k, j) at k + j nin (T, H)
+  const int j = min(g.ct * 16 + li, nout - 1);
+  const uint32_t mrow = d.op == GM_N ? (uint32_t)j : (uint32_t)j * (uint32_t)nin;
+  const uint32_t mks = d.op == GM_N ? (uint32_t)nout : 1u;
+  const cplx<T>* xa = X + d.x;
+  const cplx<TM>* mb = M + d.m;
+  v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  if (gm_rows_on_lanes(d)) {
+    if (d.op == GM_H) gm_lean_loop<T, TM, true, true>(xa, xrow, bef, mb, mrow, mks, kbeg, kend, kk, cr, ci);
+    else gm_lean_loop<T, TM, true, false>(xa, xrow, bef, mb, mrow, mks, kbeg, kend, kk, cr, ci);
+  } else {
+    if (d.op == GM_H) gm_lean_loop<T, TM, false, true>(xa, xrow, bef, mb, mrow, mks, kbeg, kend, kk, cr, ci);
+    else gm_lean_loop<T, TM, false, false>(xa, xrow, bef, mb, mrow, mks, kbeg, kend, kk, cr, ci);
+  }
+  const double fre = d.are * sre - d.aim * sim, fim = d.are * sim + d.aim * sre;
+  if (fre == 1.0 && fim == 0.0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = mk<TO>((TO)cr[q], (TO)ci[q]);
+  } else {
+    const T are = (T)fre, aim = (T)fim;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = mk<TO>((TO)(are * cr[q] - aim * ci[q]), (TO)(are * ci[q] + aim * cr[q]));
+  }
+}
+
+template <typename T, typename TO>
+__device__ inline void gm_lean_store(const GmDesc& d, const GmTile& g, const cplx<TO> (&v)[4], cplx<TO>* O) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t bef = (uint32_t)d.bef, R = bef * (uint32_t)d.aft, nout = (uint32_t)d.nout;
+  cplx<TO>* ob = O + d.out;
+  if (gm_rows_on_lanes(d)) {
+    // one row per lane, its four columns drow(lane, q)
+    const uint32_t r = (uint32_t)g.rt * 16u + (uint32_t)(lane & 15);
+    if (r >= R) return;
+    const uint32_t a = r / bef, b = r - a * bef;
+    const uint32_t o0 = b + a * bef * nout;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t jq = (uint32_t)(g.ct * 16 + MF<T>::drow(lane, q));
+      if (jq < nout) {
+        cplx<TO>* o = at32(ob, o0 + jq * bef);
+        cplx<TO> x = v[q];
+        if (d.acc) x = cadd(x, *o);
+        *o = x;
+      }
+    }
+    return;
+  }
+  // bef < 4: one column per lane, its four rows drow(lane, q)
+  const uint32_t j = (uint32_t)(g.ct * 16 + (lane & 15));
+  if (j >= nout) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t rr = (uint32_t)g.rt * 16u + (uint32_t)MF<T>::drow(lane, q);
+    if (rr < R) {
+      const uint32_t a = rr / bef, b = rr - a * bef;
+      cplx<TO>* o = at32(ob, b + j * bef + a * bef * nout);
+      cplx<TO> x = v[q];
+      if (d.acc) x = cadd(x, *o);
+      *o = x;
+    }
+  }
+}
+
 template <typename T, typename TM, typename TO, int W>
 __global__ void __launch_bounds__(64 * W)
     gm_grouped(const GmDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
@@ -264,16 +458,26 @@ __global__ void __launch_bounds__(64 * W)
   }
   const cplx<T>* X = d.xsel ? X1 : X0;
   GmTile g{0, 0, 0};
-  if (mine) g = gm_tile(d, task - d.tile0);
   const int split = mine && d.ksplit > 1 ? d.ksplit : 1;
   cplx<TO> v[4];
-  if (mine && d.op == GM_COPY) {
-    gm_copy<T, TO>(d, g, X, O, sre, sim);
-    mine = false;
+  const bool lean = mine && gm_lean_ok(d);
+  if (lean) {
+    g = gm_tile_lean(d, (uint32_t)(task - d.tile0));
+    gm_lean<T, TM, TO>(d, g, X, M, sre, sim, v);
+  } else if (mine) {
+    g = gm_tile(d, task - d.tile0);
+    if (d.op == GM_COPY) {
+      gm_copy<T, TO>(d, g, X, O, sre, sim);
+      mine = false;
+    } else {
+      gm_product<T, TM, TO>(d, g, X, M, sre, sim, v);
+    }
   }
-  if (mine) gm_product<T, TM, TO>(d, g, X, M, sre, sim, v);
   reduce_chunks<TO>(red, mine, split, g.ks, v);
-  if (mine && g.ks == 0) gm_store<T, TO>(d, g, v, O);
+  if (mine && g.ks == 0) {
+    if (lean) gm_lean_store<T, TO>(d, g, v, O);
+    else gm_store<T, TO>(d, g, v, O);
+  }
 }
 
 // one descriptor, no K split
@@ -419,6 +623,105 @@ __device__ inline void gc_product(const GcDesc& d, const GcTile& g, const cplx<T
   for (int q = 0; q < 4; ++q) v[q] = mk<TO>((TO)(cr0[q] + cr1[q]), (TO)(ci0[q] + ci1[q]));
 }
 
+// ---------------------------------------------- hole contraction, lean form
+// For X and Y tensors of fewer than 2^27 elements and a power-of-two K
+// split: 32-bit element offsets from wave-uniform bases, the lane's bra row
+// and ket column clamped into range (their products land in tile rows and
+// columns that are never stored), and only the K tail zeroed.  The running
+// position k = b + a bef of a lane advances by four per k-step as two
+// 32-bit offsets (X and Y) with one carry.
+__device__ inline bool gc_lean_ok(const GcDesc& d) {
+  constexpr int64_t lim = (int64_t)1 << 27;
+  return (int64_t)d.bef * d.nx * d.aft < lim && (int64_t)d.bef * d.ny * d.aft < lim &&
+         (d.ksplit & (d.ksplit - 1)) == 0;
+}
+
+__device__ inline GcTile gc_tile_lean(const GcDesc& d, uint32_t t) {
+  const int lg = d.ksplit > 1 ? __builtin_ctz((unsigned)d.ksplit) : 0;
+  GcTile g;
+  g.ks = (int)(t & ((1u << lg) - 1u));
+  t >>= lg;
+  const unsigned TJ = (unsigned)(d.nx + 15) >> 4;
+  if (TJ == 1) {
+    g.jt = 0;
+    g.lt = (int)t;
+  } else {
+    g.lt = (int)(t / TJ);
+    g.jt = (int)(t - (unsigned)g.lt * TJ);
+  }
+  return g;
+}
+
+template <typename T, typename TO>
+__device__ inline void gc_lean(const GcDesc& d, const GcTile& g, const cplx<T>* __restrict__ X,
+                               const cplx<T>* __restrict__ Y, cplx<TO> (&v)[4]) {
+  typedef typename MF<T>::v4 v4;
+  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const int lg = d.ksplit > 1 ? __builtin_ctz((unsigned)d.ksplit) : 0;
+  const uint32_t bef = (uint32_t)d.bef, aft = (uint32_t)d.aft;
+  const int K = d.bef * d.aft;
+  // K range of this chunk (a multiple of 4 long, except the last)
+  const int chunk = (((K + (1 << lg) - 1) >> lg) + 3) & ~3;
+  const int kbeg = g.ks * chunk, kend = min(K, kbeg + chunk);
+  const uint32_t jx = (uint32_t)min(g.jt * 16 + li, d.nx - 1), ly = (uint32_t)min(g.lt * 16 + li, d.ny - 1);
+  const uint32_t xs = bef * (uint32_t)d.nx, ys = bef * (uint32_t)d.ny;  // stride of a
+  const uint32_t xrow = jx * bef, yrow = ly * bef;
+  // the last element of the lane's row: reads past the K range stay inside
+  const uint32_t xlast = xrow + (bef - 1u) + (aft - 1u) * xs, ylast = yrow + (bef - 1u) + (aft - 1u) * ys;
+  const cplx<T>* xp = X + d.x;
+  const cplx<T>* yp = Y + d.y;
+  int kc = kbeg + kk;
+  uint32_t b = (uint32_t)kc % bef;
+  const uint32_t a0 = (uint32_t)kc / bef;
+  uint32_t px = b + a0 * xs, py = b + a0 * ys;
+  const uint32_t q4 = 4u / bef, r4 = 4u % bef;
+  const uint32_t dx = r4 + q4 * xs, dy = r4 + q4 * ys;
+  v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  // four k-steps per load: conj(X) zeroed past the chunk's end
+  auto load4 = [&](cplx<T>* xr, cplx<T>* yr) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool kv = kc < kend;
+      const cplx<T> x = ld32(xp, min(xrow + px, xlast));
+      yr[u] = ld32(yp, min(yrow + py, ylast));
+      xr[u] = mk<T>(kv ? x.re : (T)0, kv ? -x.im : (T)0);
+      kc += 4;
+      b += r4;
+      px += dx;
+      py += dy;
+      if (b >= bef) {
+        b -= bef;
+        px += xs - bef;
+        py += ys - bef;
+      }
+    }
+  };
+  cplx<T> xc[4], yc[4];
+  if (kbeg < kend) load4(xc, yc);
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+    cplx<T> xn[4], yn[4];
+    const bool more = k0 + 16 < kend;
+    if (more) load4(xn, yn);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (u > 0 && k0 + 4 * u >= kend) break;
+      cr = MF<T>::mma(xc[u].re, yc[u].re, cr);
+      ci = MF<T>::mma(xc[u].re, yc[u].im, ci);
+      cr = MF<T>::mma(-xc[u].im, yc[u].im, cr);
+      ci = MF<T>::mma(xc[u].im, yc[u].re, ci);
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xc[u] = xn[u];
+        yc[u] = yn[u];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = mk<TO>((TO)cr[q], (TO)ci[q]);
+}
+
 template <typename T, typename TO>
 __device__ inline void gc_store(const GcDesc& d, const GcTile& g, const cplx<TO> (&v)[4], cplx<TO>* O) {
   const int lane = threadIdx.x & 63;
@@ -450,10 +753,19 @@ __global__ void __launch_bounds__(64 * kWaves)
     mine = !(active && d.node >= 0 && !active[d.node]);
   }
   GcTile g{0, 0, 0};
-  if (mine) g = gc_tile(d, task - d.tile0);
   const int split = mine && d.ksplit > 1 ? d.ksplit : 1;
   cplx<TO> v[4];
-  if (mine) gc_product<T, TO>(d, g, (d.sel & 1) ? X1 : X0, (d.sel & 2) ? X1 : X0, v);
+  if (mine) {
+    const cplx<T>* xb = (d.sel & 1) ? X1 : X0;
+    const cplx<T>* yb = (d.sel & 2) ? X1 : X0;
+    if (gc_lean_ok(d)) {
+      g = gc_tile_lean(d, (uint32_t)(task - d.tile0));
+      gc_lean<T, TO>(d, g, xb, yb, v);
+    } else {
+      g = gc_tile(d, task - d.tile0);
+      gc_product<T, TO>(d, g, xb, yb, v);
+    }
+  }
   reduce_chunks<TO>(red, mine, split, g.ks, v);
   if (mine && g.ks == 0) gc_store<T, TO>(d, g, v, O);
 }

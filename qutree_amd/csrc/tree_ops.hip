@@ -360,22 +360,37 @@ __device__ inline bool gj_inverse_certified(cplx<double>*& S, cplx<double>*& D, 
     if (!(p.re > 0.0) || !(p.re < 1e300) || !(fabs(p.im) < 1e300)) return false;
     const double dd = p.re * p.re + p.im * p.im;
     const cplx<double> pinv = mk<double>(p.re / dd, -p.im / dd);
-    for (int o = threadIdx.x; o < nn; o += EW) {
-      const int i = o % n, j = o / n;
-      const cplx<double> m = S[i + j * ld];
-      cplx<double> r;
-      if (i == k && j == k) {
-        r = pinv;
-      } else if (i == k) {
-        r = cmul(m, pinv);
-      } else if (j == k) {
-        const cplx<double> t = cmul(m, pinv);
-        r = mk<double>(-t.re, -t.im);
-      } else {
-        const cplx<double> t = cmul(cmul(S[i + k * ld], S[k + j * ld]), pinv);
-        r = mk<double>(m.re - t.re, m.im - t.im);
+    // four elements per lane and pass: every LDS read of a pass is issued
+    // before its writes (they go to the other buffer)
+    for (int base = 0; base < nn; base += 4 * EW) {
+      cplx<double> m[4], sik[4], skj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int o = min(base + u * EW + (int)threadIdx.x, nn - 1);
+        const int i = o % n, j = o / n;
+        m[u] = S[i + j * ld];
+        sik[u] = S[i + k * ld];
+        skj[u] = S[k + j * ld];
       }
-      D[i + j * ld] = r;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int o = base + u * EW + (int)threadIdx.x;
+        if (o >= nn) continue;
+        const int i = o % n, j = o / n;
+        cplx<double> r;
+        if (i == k && j == k) {
+          r = pinv;
+        } else if (i == k) {
+          r = cmul(m[u], pinv);
+        } else if (j == k) {
+          const cplx<double> t = cmul(m[u], pinv);
+          r = mk<double>(-t.re, -t.im);
+        } else {
+          const cplx<double> t = cmul(cmul(sik[u], skj[u]), pinv);
+          r = mk<double>(m[u].re - t.re, m[u].im - t.im);
+        }
+        D[i + j * ld] = r;
+      }
     }
     __syncthreads();
     cplx<double>* t = S;
@@ -404,14 +419,30 @@ __global__ void __launch_bounds__(EW)
   const int n = it.n;
   const int ld = jacobi_ld(n);
   if (it.ev < 0) {
-    for (int o = threadIdx.x; o < n * n; o += EW) {
-      const int i = o % n, j = o / n;
-      const cplx<T> a = pool[it.in + i + j * n], b = pool[it.in + j + i * n];
-      A[i + j * ld] = mk<double>(0.5 * ((double)a.re + (double)b.re), 0.5 * ((double)a.im - (double)b.im));
+    // the hermitised rho, eight element pairs per lane loaded before any
+    // LDS write, so the global loads of a pass are in flight together
+    const int nn = n * n;
+    for (int base = 0; base < nn; base += 8 * EW) {
+      cplx<T> a[8], b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int o = min(base + u * EW + (int)threadIdx.x, nn - 1);
+        const int i = o % n, j = o / n;
+        a[u] = pool[it.in + i + j * n];
+        b[u] = pool[it.in + j + i * n];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int o = base + u * EW + (int)threadIdx.x;
+        if (o < nn)
+          A[o % n + (o / n) * ld] =
+              mk<double>(0.5 * ((double)a[u].re + (double)b[u].re), 0.5 * ((double)a[u].im - (double)b[u].im));
+      }
     }
     __syncthreads();
     double tr = 0.0;
-    for (int i = 0; i < n; ++i) tr += A[i + i * ld].re;
+    for (int i = threadIdx.x; i < n; i += EW) tr += A[i + i * ld].re;
+    tr = wave_sum(tr);
     cplx<double>* S = A;
     cplx<double>* D = U;
     if (gj_inverse_certified(S, D, n, ld, eps, tr * tr)) {
@@ -449,43 +480,68 @@ __global__ void __launch_bounds__(EW)
 }
 
 // ------------------------------------------------------------ combinations
+// Sum of term t into partial sum u: re/im[u] += c * (pool[src + e] or the
+// identity element id when src < 0).
+template <typename T>
+__device__ inline void comb_term(const CombTerm& ct, const cplx<T>& m, double id, double& re, double& im) {
+  const double mr = ct.src >= 0 ? (double)m.re : id, mi = ct.src >= 0 ? (double)m.im : 0.0;
+  re += ct.cre * mr - ct.cim * mi;
+  im += ct.cre * mi + ct.cim * mr;
+}
+
+// Element e of one output over its term list tl[0..nt): term 4q+u of a full
+// group of four goes to partial sum u, the last nt % 4 terms to sum 0.
+// Eight matrix loads are issued before their sums.  Terms without a source
+// load element e of the output itself (in bounds, unused).
+template <typename T>
+__device__ inline cplx<T> comb_element(const CombTerm* tl, int nt, const cplx<T>* pool, int64_t out, int e,
+                                       double id) {
+  double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+  int t = 0;
+  for (; t + 8 <= nt; t += 8) {
+    cplx<T> m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t src = tl[t + u].src;
+      m[u] = pool[(src >= 0 ? src : out) + e];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) comb_term(tl[t + u], m[u], id, re[u & 3], im[u & 3]);
+  }
+  for (; t + 4 <= nt; t += 4) {
+    cplx<T> m[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t src = tl[t + u].src;
+      m[u] = pool[(src >= 0 ? src : out) + e];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) comb_term(tl[t + u], m[u], id, re[u], im[u]);
+  }
+  for (; t < nt; ++t) {
+    const int64_t src = tl[t].src;
+    comb_term(tl[t], pool[(src >= 0 ? src : out) + e], id, re[0], im[0]);
+  }
+  return mk<T>((T)((re[0] + re[1]) + (re[2] + re[3])), (T)((im[0] + im[1]) + (im[2] + im[3])));
+}
+
+// One workgroup per output matrix.  Term lists of up to TB entries are
+// staged in LDS first (one coalesced load), so the matrix loads do not wait
+// on a term load each; longer lists are read from global memory.
 template <typename T>
 __global__ void __launch_bounds__(TB)
     combine_kernel(const CombOut* __restrict__ outs, const CombTerm* __restrict__ terms, cplx<T>* pool) {
+  __shared__ CombTerm st[TB];
   const CombOut co = outs[blockIdx.x];
-  const int n = co.n;
-  for (int e = threadIdx.x; e < n * n; e += TB) {
-    const int i = e % n, j = e / n;
-    const double id = (i == j) ? 1.0 : 0.0;
-    double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
-    int t = co.t0;
-    // four independent partial sums keep four loads in flight
-    for (; t + 4 <= co.t1; t += 4) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const CombTerm ct = terms[t + u];
-        double mr = id, mi = 0.0;
-        if (ct.src >= 0) {
-          const cplx<T> m = pool[ct.src + e];
-          mr = (double)m.re;
-          mi = (double)m.im;
-        }
-        re[u] += ct.cre * mr - ct.cim * mi;
-        im[u] += ct.cre * mi + ct.cim * mr;
-      }
-    }
-    for (; t < co.t1; ++t) {
-      const CombTerm ct = terms[t];
-      double mr = id, mi = 0.0;
-      if (ct.src >= 0) {
-        const cplx<T> m = pool[ct.src + e];
-        mr = (double)m.re;
-        mi = (double)m.im;
-      }
-      re[0] += ct.cre * mr - ct.cim * mi;
-      im[0] += ct.cre * mi + ct.cim * mr;
-    }
-    pool[co.out + e] = mk<T>((T)((re[0] + re[1]) + (re[2] + re[3])), (T)((im[0] + im[1]) + (im[2] + im[3])));
+  const int n = co.n, nt = co.t1 - co.t0;
+  if (nt <= TB) {
+    if ((int)threadIdx.x < nt) st[threadIdx.x] = terms[co.t0 + threadIdx.x];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += TB)
+      pool[co.out + e] = comb_element<T>(st, nt, pool, co.out, e, (e % n == e / n) ? 1.0 : 0.0);
+  } else {
+    for (int e = threadIdx.x; e < n * n; e += TB)
+      pool[co.out + e] = comb_element<T>(terms + co.t0, nt, pool, co.out, e, (e % n == e / n) ? 1.0 : 0.0);
   }
 }
 

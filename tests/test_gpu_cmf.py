@@ -169,3 +169,28 @@ def test_cmf8_global_memory_orthogonal_matches_oracle():
     assert de < 1e-12 and da < 1e-12
     top = om.nnodes - 1
     assert np.max(np.abs(om.node_tensor(got["psi"], top) - om.node_tensor(ref["psi"], top))) < 1e-11
+
+
+@pytest.mark.parametrize("model", ["cmf8", "cmf64"])
+def test_cmf_tick_graphs_equal_eager_ticks(model):
+    """Ticks replayed from the HIP graphs (default) and launched eagerly run
+    the same kernels on the same staged programs: energies, autocorrelations,
+    statistics and the final state agree bit for bit, also across a second
+    integrate call that continues from the first one's state."""
+    text, H = getattr(W, model + "_model")()
+    print(f"case cmf tick graph vs eager {model}", flush=True)
+    runs = []
+    for graph in (1, 0):
+        eng = Q.Engine(text, H)
+        psi = eng.fill_random(64 if model == "cmf64" else 8, delta_lowest=True)
+        cmf = Q.CMF(eng)
+        cmf.set_option("graph", graph)
+        a = cmf.integrate(psi, 0.0, 0.3, 0.1, 0.1)
+        b = cmf.integrate(a["psi"], 0.3, 0.5, 0.1, 0.1)
+        runs.append((a, b))
+    for x, y in zip(runs[0], runs[1]):
+        assert np.array_equal(x["energies"], y["energies"])
+        assert np.array_equal(x["autocorr"], y["autocorr"])
+        assert np.array_equal(x["psi"], y["psi"])
+        for k in ("steps", "rejected", "builds", "derivs", "ticks"):
+            assert x[k] == y[k], k

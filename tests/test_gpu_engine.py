@@ -209,6 +209,25 @@ def test_graph_replays_match_eager():
         g.synchronize()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_inline_eigensolver_equals_side_stream(graph):
+    """heev_side 0 runs the density eigensolver in line on the engine stream
+    instead of on the side stream: the same kernels on the same data, so the
+    derivative and the energy agree bit for bit."""
+    text, H = W.sweep64_model()
+    print(f"case engine heev_side 0 vs 1 graph={graph}", flush=True)
+    a = Q.Engine(text, H, graph=graph, heev_side=0)
+    b = Q.Engine(text, H, graph=graph, heev_side=1)
+    for seed in (5, 6):
+        psi = a.fill_random(seed, delta_lowest=False)
+        a.set_psi(psi)
+        b.set_psi(psi)
+        a.derivative()
+        b.derivative()
+        assert np.array_equal(a.get(0), b.get(0)), seed
+        assert a.energy() == b.energy()
+
+
 @pytest.mark.parametrize("amp", [0.3, 0.003])
 @pytest.mark.parametrize("name", ["cmf8", "f16"])
 def test_derivative_non_orthonormal_state(name, amp):
