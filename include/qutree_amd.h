@@ -350,10 +350,14 @@ int qtg_cmf_create(qtg_engine engine, qtg_cmf* cmf);
  * truncating, and writes nothing when the file is missing
  * (TensorTree_Implementation.h:93-100, ios::in).  filename NULL: off. */
 int qtg_cmf_set_output_file(qtg_cmf cmf, const char* filename, int append);
-/* "graph" (default 1): each lockstep tick (pre programs, masked Apply and
- * finish, post programs) is replayed from a HIP graph captured once per
- * state buffer and phase; only the tick's programs are uploaded.  0 launches
- * every tick eagerly.  Results are the same either way. */
+/* "graph" (default 0): 1 replays each lockstep tick (pre programs, masked
+ * Apply and finish, post programs) from a HIP graph captured once per state
+ * buffer and phase, uploading only the tick's programs; 0 launches every
+ * tick eagerly (measured faster on MI355X).  "fuse" (default 1): the vector
+ * op sequences a tick emits most (the Bulirsch-Stoer start, the modified
+ * midpoint steps, the final midpoint with rational extrapolation) run as one
+ * pass over the node each; 0 runs one pass per op.  Results are the same
+ * either way for both options. */
 int qtg_cmf_set_option(qtg_cmf cmf, const char* key, double value);
 int qtg_cmf_destroy(qtg_cmf cmf);
 int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend, double dt, double out,

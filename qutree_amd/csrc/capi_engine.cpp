@@ -402,6 +402,10 @@ int qtg_cmf_set_option(qtg_cmf c, const char* key, double value) {
     cm->tickGraph_ = value != 0.0;
     return QTG_OK;
   }
+  if (!std::strcmp(key, "fuse")) {
+    cm->fuseOps_ = value != 0.0;
+    return QTG_OK;
+  }
   set_error(std::string("unknown cmf option ") + key);
   return QTG_EARG;
 }

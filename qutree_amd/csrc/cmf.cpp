@@ -433,6 +433,52 @@ bool Cmf::advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, doub
   }
 }
 
+// The op sequences advance() emits most, each as one fused op (one pass
+// over the node; the same arithmetic per element).  Other sequences stay.
+static bool is_op(const VecOp& o, int code, int a = -1, int b = -1, int c = -1) {
+  return o.code == code && (a < 0 || o.a == a) && (b < 0 || o.b == b) && (c < 0 || o.c == c);
+}
+// the bsstep start: y0 = y3 = P, ytab[1..6] = 0, x = P
+static bool is_start(const std::vector<VecOp>& o, size_t i) {
+  if (o.size() < i + 9) return false;
+  if (!is_op(o[i], V_COPY, B_Y0, B_P) || !is_op(o[i + 1], V_COPY, B_Y0 + 3, B_P)) return false;
+  for (int k = 1; k < 7; ++k)
+    if (!is_op(o[i + 1 + k], V_ZERO, B_YT0 + k)) return false;
+  return is_op(o[i + 8], V_COPY, B_X, B_P);
+}
+static void fuse_ops(std::vector<VecOp>& o) {
+  std::vector<VecOp> r;
+  r.reserve(o.size());
+  size_t i = 0;
+  while (i < o.size()) {
+    if (is_op(o[i], V_COPY, B_P, B_Y0) && is_start(o, i + 1)) {  // accept, then the next start
+      r.push_back(op0(V_START, B_Y0, 1));
+      i += 10;
+    } else if (is_start(o, i)) {
+      r.push_back(op0(V_START, B_P, 0));
+      i += 9;
+    } else if (i + 3 <= o.size() && is_op(o[i], V_COPY, B_Y0, B_Y0 + 3) &&
+               is_op(o[i + 1], V_AXPY, B_Y0 + 1, B_Y0 + 3, B_Y0 + 4) && is_op(o[i + 2], V_COPY, B_X, B_Y0 + 1)) {
+      r.push_back(op0(V_MMID_BEGIN, 0, 0, 0, o[i + 1].h));
+      i += 3;
+    } else if (i + 2 <= o.size() && is_op(o[i], V_MMID_STEP) && is_op(o[i + 1], V_COPY, B_X, B_Y0 + 1)) {
+      r.push_back(op0(V_MMID_STEP_X, 0, 0, 0, o[i].h));
+      i += 2;
+    } else if (i + 4 <= o.size() && is_op(o[i], V_COPY, B_Y0 + 2, B_D) && is_op(o[i + 1], V_MMID_FINAL) &&
+               is_op(o[i + 2], V_RZEXTR) && is_op(o[i + 3], V_POST_EXTR)) {
+      VecOp f = o[i + 2];  // i, mi and fx of the extrapolation
+      f.code = V_FINAL_EXTR;
+      f.h = o[i + 1].h;
+      r.push_back(f);
+      i += 4;
+    } else {
+      r.push_back(o[i]);
+      ++i;
+    }
+  }
+  o.swap(r);
+}
+
 void Cmf::finish_mmid(int v, std::vector<VecOp>& post) {
   NodeBS& b = bs_[v];
   post.push_back(op0(V_MMID_FINAL, 0, 0, 0, b.hh));
@@ -555,6 +601,10 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
       if (failed_) {
         set_error("BS integrator: step size underflow");
         return QTG_ESTATE;
+      }
+      if (fuseOps_) {
+        fuse_ops(pre);
+        fuse_ops(post);
       }
       if (!pre.empty()) {
         ppre[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + pre.size()), 0};

@@ -54,8 +54,12 @@ class Cmf {
   std::string outFile_;
   bool outAppend_ = true;
   // ticks replayed from HIP graphs (one per state buffer and phase); false
-  // launches every tick eagerly
-  bool tickGraph_ = true;
+  // (default: measured faster on MI355X, where a graph launch per tick costs
+  // more than the tick's six kernel launches) launches every tick eagerly
+  bool tickGraph_ = false;
+  // the op sequences a tick emits most run as fused ops (one pass over the
+  // node each); false keeps one pass per op.  Results are the same.
+  bool fuseOps_ = true;
 
  private:
   int eig_init();

@@ -110,6 +110,54 @@ struct MmidFinalOp {  // y0 <- (y0 + y1 + h y2) / 2
     y0[e] = mk<double>((v.a.re + v.b.re + h * v.c.re) / 2.0, (v.a.im + v.b.im + h * v.c.im) / 2.0);
   }
 };
+struct StartOp {  // v = src; P <- v (when wp); y0, y3, x <- v; ytab[1..6] <- 0
+  const cd2* src;
+  cd2 *P, *y0, *y3, *x;
+  cd2* yt[6];
+  bool wp;
+  struct Val {
+    cd2 v;
+  };
+  __device__ Val load(int e) const { return {src[e]}; }
+  __device__ void store(int e, const Val& v) const {
+    if (wp) P[e] = v.v;
+    y0[e] = v.v;
+    y3[e] = v.v;
+    x[e] = v.v;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) yt[k][e] = czero<double>();
+  }
+};
+struct MmidBeginOp {  // y0 <- y3; t = y3 + h y4; y1 <- t; x <- t
+  cd2 *y0, *y1, *x;
+  const cd2 *y3, *y4;
+  double h;
+  struct Val {
+    cd2 b, c;
+  };
+  __device__ Val load(int e) const { return {y3[e], y4[e]}; }
+  __device__ void store(int e, const Val& v) const {
+    y0[e] = v.b;
+    const cd2 t = mk<double>(v.b.re + h * v.c.re, v.b.im + h * v.c.im);
+    y1[e] = t;
+    x[e] = t;
+  }
+};
+struct MmidStepXOp {  // t = y0 + 2h y2; y0 <- y1; y1 <- t; y2 <- t; x <- t
+  cd2 *y0, *y1, *y2, *x;
+  double h;
+  struct Val {
+    cd2 a, b, c;
+  };
+  __device__ Val load(int e) const { return {y0[e], y1[e], y2[e]}; }
+  __device__ void store(int e, const Val& v) const {
+    const cd2 t = mk<double>(v.a.re + 2.0 * h * v.c.re, v.a.im + 2.0 * h * v.c.im);
+    y0[e] = v.b;
+    y1[e] = t;
+    y2[e] = t;
+    x[e] = t;
+  }
+};
 struct PostExtrOp {  // y0 <- y1; y1 <- y1 + y2
   cd2 *y0, *y1;
   const cd2* y2;
@@ -231,6 +279,80 @@ __global__ void __launch_bounds__(TB)
       case V_POST_EXTR:
         elementwise(N, PostExtrOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off});
         break;
+      case V_START: {
+        StartOp so{bp(bufs, op.a) + off, bp(bufs, B_P) + off, bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 3) + off,
+                    bp(bufs, B_X) + off, {}, op.b != 0};
+#pragma unroll
+        for (int k2 = 0; k2 < 6; ++k2) so.yt[k2] = bp(bufs, B_YT0 + 1 + k2) + off;
+        elementwise(N, so);
+        break;
+      }
+      case V_MMID_BEGIN:
+        elementwise(N, MmidBeginOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_X) + off,
+                                   bp(bufs, B_Y0 + 3) + off, bp(bufs, B_Y0 + 4) + off, op.h});
+        break;
+      case V_MMID_STEP_X:
+        elementwise(N, MmidStepXOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off,
+                                   bp(bufs, B_X) + off, op.h});
+        break;
+      case V_FINAL_EXTR: {
+        // y2 <- d; y0 <- (y0 + y1 + h y2) / 2 (the midpoint estimate yest);
+        // rzextr at step i; then y0 <- ysav, y1 <- ysav + dy
+        cd2* y0 = bp(bufs, B_Y0) + off;
+        cd2* y1 = bp(bufs, B_Y0 + 1) + off;
+        cd2* y2 = bp(bufs, B_Y0 + 2) + off;
+        const cd2* dd = bp(bufs, B_D) + off;
+        constexpr int kTab = 7;  // ytab[0..6]
+        const int mi = op.mi < kTab ? op.mi : kTab;
+        const double h = op.h;
+        double fx[kTab];
+#pragma unroll
+        for (int k2 = 1; k2 < kTab; ++k2) fx[k2] = ops[k].fx[k2];
+        cd2* yt[kTab];
+#pragma unroll
+        for (int k2 = 0; k2 < kTab; ++k2) yt[k2] = bp(bufs, B_YT0 + k2) + off;
+        for (int e = threadIdx.x; e < N; e += TB) {
+          const cd2 d = dd[e], a = y0[e], b = y1[e];
+          cd2 col[kTab], v = czero<double>();
+          if (op.i > 0) {
+#pragma unroll
+            for (int k2 = 1; k2 < kTab; ++k2) col[k2] = yt[k2 < mi ? k2 : 0][e];
+            v = yt[0][e];
+          }
+          const cd2 yest = mk<double>((a.re + b.re + h * d.re) / 2.0, (a.im + b.im + h * d.im) / 2.0);
+          cd2 ysav = yest, dy = yest;
+          if (op.i > 0) {
+            cd2 ddy = czero<double>(), c = yest, yy = yest;
+#pragma unroll
+            for (int k2 = 1; k2 < kTab; ++k2) {
+              if (k2 < mi) {
+                const cd2 bi = mk<double>(fx[k2] * v.re, fx[k2] * v.im);
+                cd2 bb = mk<double>(bi.re - c.re, bi.im - c.im);
+                if (bb.re != 0.0 || bb.im != 0.0) {
+                  bb = cdiv(mk<double>(c.re - v.re, c.im - v.im), bb);
+                  ddy = cmul(c, bb);
+                  c = cmul(bi, bb);
+                } else {
+                  ddy = v;
+                }
+                v = col[k2];
+                col[k2] = ddy;
+                yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
+              }
+            }
+#pragma unroll
+            for (int k2 = 1; k2 < kTab; ++k2)
+              if (k2 < mi) yt[k2][e] = col[k2];
+            dy = ddy;
+            ysav = yy;
+          }
+          yt[0][e] = yest;
+          y2[e] = dy;
+          y0[e] = ysav;
+          y1[e] = mk<double>(ysav.re + dy.re, ysav.im + dy.im);
+        }
+        break;
+      }
       case V_ERROR: {
         const cd2* Phi = bp(bufs, op.a) + off;
         const cd2* Chi = bp(bufs, op.b) + off;

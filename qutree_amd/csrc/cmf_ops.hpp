@@ -29,7 +29,13 @@ enum VecCode : int32_t {
   V_MMID_FINAL = 4,  // y0 <- (y0 + y1 + h y2) / 2
   V_RZEXTR = 5,      // rational extrapolation step i (rzextr)
   V_POST_EXTR = 6,   // y0 <- y1; y1 <- y1 + y2
-  V_ERROR = 7        // err[node] = LayerInterface::Error(a, b)
+  V_ERROR = 7,       // err[node] = LayerInterface::Error(a, b)
+  // fused forms of the op sequences a tick emits most (one pass over the
+  // node instead of one per op; the same arithmetic per element)
+  V_START = 8,        // v = buffer a; P <- v when b; y0, y3, x <- v; ytab[1..6] <- 0
+  V_MMID_BEGIN = 9,   // y0 <- y3; t = y3 + h y4; y1 <- t; x <- t
+  V_MMID_STEP_X = 10, // V_MMID_STEP, then x <- y1
+  V_FINAL_EXTR = 11   // y2 <- d; V_MMID_FINAL; V_RZEXTR (i, mi, fx); V_POST_EXTR
 };
 
 struct VecOp {

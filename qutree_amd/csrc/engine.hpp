@@ -152,6 +152,7 @@ class Engine {
     double flops;
     double bytes;  // algorithmic HBM bytes (distinct inputs once, outputs)
     int waves;     // mode products: tasks per workgroup (0: kGemmWaves)
+    bool lean;     // every descriptor takes the lean kernel form
   };
   bool gramInSolve_ = false;  // the solve launch forms S, A^H Y and the energy terms
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,

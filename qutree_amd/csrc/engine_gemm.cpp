@@ -118,6 +118,7 @@ int Engine::plan_gemm() {
       }
       if (d.op != GM_COPY) fl += 8.0 * d.bef * d.aft * (double)d.nin * d.nout;
     }
+    gl.lean = std::all_of(descs.begin(), descs.end(), [](const GmDesc& d) { return gm_lean_ok(d); });
     const int64_t t = order_tasks(descs, gm_tasks);
     // one descriptor per task (its copy keeps tile0), so a wavefront's
     // descriptor is one load at descs[task]
@@ -168,6 +169,7 @@ int Engine::plan_gemm() {
       d.ksplit = K > 2 * gcLongK_ ? split_chunks(K, kGemmWaves, 1) : split_chunks(K, want, splitMinK_);
       fl += 8.0 * d.bef * d.aft * (double)d.nx * d.ny;
     }
+    gl.lean = std::all_of(descs.begin(), descs.end(), [](const GcDesc& d) { return gc_lean_ok(d); });
     const int64_t t = order_tasks(descs, gc_tasks);
     for (auto& d : descs) h_gc_.insert(h_gc_.end(), (size_t)gc_tasks(d), d);
     gl.nd = t;
@@ -939,11 +941,14 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *m = bases[gl.m], *o = bases[gl.o];
       const int wv = gl.waves > 0 ? gl.waves : kGemmWaves;
       if (gl.prec == P64)
-        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv);
+        rc = launch_gm<double, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv,
+                                               gl.lean);
       else if (gl.prec == P32)
-        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv);
+        rc = launch_gm<float, double, float>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv,
+                                             gl.lean);
       else
-        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv);
+        rc = launch_gm<float, double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, m, o, active, a, b, s, kt, wv,
+                                              gl.lean);
       if (rc) return rc;
       if (timing_) tfl_[6] += gl.flops, tby_[6] += gl.bytes;
     } else if (gl.kind == 1) {
@@ -952,9 +957,9 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
       const int32_t* wg = gl.wg0 >= 0 ? d_gcwg_ + gl.wg0 : nullptr;
       void *x0 = bases[gl.x0], *x1 = bases[gl.x1], *o = bases[gl.o];
       if (gl.prec == P64)
-        rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt);
+        rc = launch_gc<double, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt, gl.lean);
       else
-        rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt);
+        rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt, gl.lean);
       if (rc) return rc;
       if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
     } else {

@@ -18,6 +18,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "gemm_ops.hpp"
 
@@ -291,11 +292,6 @@ __device__ inline void gm_store(const GmDesc& d, const GmTile& g, const cplx<TO>
 // it instead of predicated (their products land in tile rows and columns
 // that are never stored), and only the K tail zeroed.  The tile is
 // accumulated transposed when bef >= 4 (gm_rows_on_lanes).
-__device__ inline bool gm_lean_ok(const GmDesc& d) {
-  constexpr int64_t lim = (int64_t)1 << 27;
-  return d.op != GM_COPY && (int64_t)d.bef * d.nin * d.aft < lim && (int64_t)d.bef * d.nout * d.aft < lim &&
-         (int64_t)d.nin * d.nout < lim && (d.ksplit & (d.ksplit - 1)) == 0;
-}
 
 // base[idx] for idx < 2^27: a 32-bit byte offset from the base
 template <typename V>
@@ -443,7 +439,8 @@ __device__ inline void gm_lean_store(const GmDesc& d, const GmTile& g, const cpl
   }
 }
 
-template <typename T, typename TM, typename TO, int W>
+// LEAN: every descriptor takes the lean form (checked on the host)
+template <typename T, typename TM, typename TO, int W, bool LEAN>
 __global__ void __launch_bounds__(64 * W)
     gm_grouped(const GmDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
                const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, const cplx<TM>* __restrict__ M,
@@ -460,11 +457,11 @@ __global__ void __launch_bounds__(64 * W)
   GmTile g{0, 0, 0};
   const int split = mine && d.ksplit > 1 ? d.ksplit : 1;
   cplx<TO> v[4];
-  const bool lean = mine && gm_lean_ok(d);
+  const bool lean = mine && (LEAN || gm_lean_ok(d));
   if (lean) {
     g = gm_tile_lean(d, (uint32_t)(task - d.tile0));
     gm_lean<T, TM, TO>(d, g, X, M, sre, sim, v);
-  } else if (mine) {
+  } else if (!LEAN && mine) {
     g = gm_tile(d, task - d.tile0);
     if (d.op == GM_COPY) {
       gm_copy<T, TO>(d, g, X, O, sre, sim);
@@ -475,7 +472,7 @@ __global__ void __launch_bounds__(64 * W)
   }
   reduce_chunks<TO>(red, mine, split, g.ks, v);
   if (mine && g.ks == 0) {
-    if (lean) gm_lean_store<T, TO>(d, g, v, O);
+    if (LEAN || lean) gm_lean_store<T, TO>(d, g, v, O);
     else gm_store<T, TO>(d, g, v, O);
   }
 }
@@ -630,11 +627,6 @@ __device__ inline void gc_product(const GcDesc& d, const GcTile& g, const cplx<T
 // columns that are never stored), and only the K tail zeroed.  The running
 // position k = b + a bef of a lane advances by four per k-step as two
 // 32-bit offsets (X and Y) with one carry.
-__device__ inline bool gc_lean_ok(const GcDesc& d) {
-  constexpr int64_t lim = (int64_t)1 << 27;
-  return (int64_t)d.bef * d.nx * d.aft < lim && (int64_t)d.bef * d.ny * d.aft < lim &&
-         (d.ksplit & (d.ksplit - 1)) == 0;
-}
 
 __device__ inline GcTile gc_tile_lean(const GcDesc& d, uint32_t t) {
   const int lg = d.ksplit > 1 ? __builtin_ctz((unsigned)d.ksplit) : 0;
@@ -722,6 +714,88 @@ __device__ inline void gc_lean(const GcDesc& d, const GcTile& g, const cplx<T>* 
   for (int q = 0; q < 4; ++q) v[q] = mk<TO>((TO)cr[q], (TO)ci[q]);
 }
 
+// Bra and ket of at most 8 columns (the bottom layers' 8 x 8 holes): lanes
+// 8..15 of every row group take the second half of the chunk's K range, so
+// the 16 x 16 MFMA tile holds the two halves' 8 x 8 products on its diagonal
+// blocks, summed across lanes at the end: half the k-steps (MFMAs and loads)
+// of gc_lean.  complex128 accumulator layout: lane l holds rows (l/16) + 4q,
+// column l % 16, so rows 8..15 of column c + 8 sit in v[q + 2] of lane l + 8.
+template <typename TO>
+__device__ inline void gc_lean_packed(const GcDesc& d, const GcTile& g, const cplx<double>* __restrict__ X,
+                                      const cplx<double>* __restrict__ Y, cplx<TO> (&v)[4]) {
+  typedef MF<double>::v4 v4;
+  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const int half = li >> 3, jr = li & 7;
+  const int lg = d.ksplit > 1 ? __builtin_ctz((unsigned)d.ksplit) : 0;
+  const uint32_t bef = (uint32_t)d.bef, aft = (uint32_t)d.aft;
+  const int K = d.bef * d.aft;
+  const int chunk = (((K + (1 << lg) - 1) >> lg) + 3) & ~3;
+  const int kbeg = g.ks * chunk, kend = min(K, kbeg + chunk);
+  // the two halves: [kbeg, kbeg + h0) and [kbeg + h0, kend), h0 a multiple of 4
+  const int h0 = ((((kend - kbeg) + 1) >> 1) + 3) & ~3;
+  const int klim = half ? kend : min(kend, kbeg + h0);
+  const uint32_t jx = (uint32_t)min(jr, d.nx - 1), ly = (uint32_t)min(jr, d.ny - 1);
+  const uint32_t xs = bef * (uint32_t)d.nx, ys = bef * (uint32_t)d.ny;
+  const uint32_t xrow = jx * bef, yrow = ly * bef;
+  const uint32_t xlast = xrow + (bef - 1u) + (aft - 1u) * xs, ylast = yrow + (bef - 1u) + (aft - 1u) * ys;
+  const cplx<double>* xp = X + d.x;
+  const cplx<double>* yp = Y + d.y;
+  int kc = kbeg + (half ? h0 : 0) + kk;
+  uint32_t b = (uint32_t)kc % bef;
+  const uint32_t a0 = (uint32_t)kc / bef;
+  uint32_t px = b + a0 * xs, py = b + a0 * ys;
+  const uint32_t q4 = 4u / bef, r4 = 4u % bef;
+  const uint32_t dx = r4 + q4 * xs, dy = r4 + q4 * ys;
+  v4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  auto load4 = [&](cplx<double>* xr, cplx<double>* yr) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool kv = kc < klim;
+      const cplx<double> x = ld32(xp, min(xrow + px, xlast));
+      yr[u] = ld32(yp, min(yrow + py, ylast));
+      xr[u] = mk<double>(kv ? x.re : 0.0, kv ? -x.im : 0.0);
+      kc += 4;
+      b += r4;
+      px += dx;
+      py += dy;
+      if (b >= bef) {
+        b -= bef;
+        px += xs - bef;
+        py += ys - bef;
+      }
+    }
+  };
+  cplx<double> xc[4], yc[4];
+  if (h0 > 0) load4(xc, yc);
+  for (int s0 = 0; s0 < h0; s0 += 16) {
+    cplx<double> xn[4], yn[4];
+    const bool more = s0 + 16 < h0;
+    if (more) load4(xn, yn);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (u > 0 && s0 + 4 * u >= h0) break;
+      cr = MF<double>::mma(xc[u].re, yc[u].re, cr);
+      ci = MF<double>::mma(xc[u].re, yc[u].im, ci);
+      cr = MF<double>::mma(-xc[u].im, yc[u].im, cr);
+      ci = MF<double>::mma(xc[u].im, yc[u].re, ci);
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xc[u] = xn[u];
+        yc[u] = yn[u];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const double r2 = __shfl_down(cr[q + 2], 8, 16), i2 = __shfl_down(ci[q + 2], 8, 16);
+    v[q] = mk<TO>((TO)(cr[q] + r2), (TO)(ci[q] + i2));
+  }
+  v[2] = czero<TO>();
+  v[3] = czero<TO>();
+}
+
 template <typename T, typename TO>
 __device__ inline void gc_store(const GcDesc& d, const GcTile& g, const cplx<TO> (&v)[4], cplx<TO>* O) {
   const int lane = threadIdx.x & 63;
@@ -739,7 +813,7 @@ __device__ inline void gc_store(const GcDesc& d, const GcTile& g, const cplx<TO>
   }
 }
 
-template <typename T, typename TO>
+template <typename T, typename TO, bool LEAN>
 __global__ void __launch_bounds__(64 * kWaves)
     gc_grouped(const GcDesc* __restrict__ descs, const int32_t* __restrict__ wgdesc, int ndesc, int64_t ntasks,
                const cplx<T>* __restrict__ X0, const cplx<T>* __restrict__ X1, cplx<TO>* O,
@@ -758,10 +832,15 @@ __global__ void __launch_bounds__(64 * kWaves)
   if (mine) {
     const cplx<T>* xb = (d.sel & 1) ? X1 : X0;
     const cplx<T>* yb = (d.sel & 2) ? X1 : X0;
-    if (gc_lean_ok(d)) {
+    if (LEAN || gc_lean_ok(d)) {
       g = gc_tile_lean(d, (uint32_t)(task - d.tile0));
-      gc_lean<T, TO>(d, g, xb, yb, v);
-    } else {
+      if constexpr (std::is_same<T, double>::value) {
+        if (d.nx <= 8 && d.ny <= 8) gc_lean_packed<TO>(d, g, xb, yb, v);
+        else gc_lean<T, TO>(d, g, xb, yb, v);
+      } else {
+        gc_lean<T, TO>(d, g, xb, yb, v);
+      }
+    } else if (!LEAN) {
       g = gc_tile(d, task - d.tile0);
       gc_product<T, TO>(d, g, xb, yb, v);
     }
@@ -1153,30 +1232,35 @@ __global__ void __launch_bounds__(kSolveThreads)
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, const uint8_t* active, double sre, double sim,
-              hipStream_t s, const KernelTiming* kt, int waves) {
+              hipStream_t s, const KernelTiming* kt, int waves, bool lean) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
+#define QTG_GM(W, L)                                                                                             \
+  launch_timed(gm_grouped<T, TM, TO, W, L>, dim3((unsigned)((ntasks + W - 1) / W)), dim3(64 * W), 0, s, kt, descs, \
+               wgdesc, (int)ndesc, ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase,       \
+               (cplx<TO>*)obase, active, sre, sim)
   if (waves == kGemmWavesWide) {
-    const unsigned blocks = (unsigned)((ntasks + kGemmWavesWide - 1) / kGemmWavesWide);
-    launch_timed(gm_grouped<T, TM, TO, kGemmWavesWide>, dim3(blocks), dim3(64 * kGemmWavesWide), 0, s, kt, descs,
-                 wgdesc, (int)ndesc, ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase,
-                 (cplx<TO>*)obase, active, sre, sim);
+    if (lean) QTG_GM(kGemmWavesWide, true);
+    else QTG_GM(kGemmWavesWide, false);
   } else {
-    const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-    launch_timed(gm_grouped<T, TM, TO, kWaves>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc,
-                 (int)ndesc, ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (const cplx<TM>*)mbase,
-                 (cplx<TO>*)obase, active, sre, sim);
+    if (lean) QTG_GM(kWaves, true);
+    else QTG_GM(kWaves, false);
   }
+#undef QTG_GM
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
 
 template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, void* obase, const uint8_t* active, hipStream_t s, const KernelTiming* kt) {
+              const void* x1, void* obase, const uint8_t* active, hipStream_t s, const KernelTiming* kt, bool lean) {
   if (ntasks <= 0 || ndesc <= 0) return QTG_OK;
   const unsigned blocks = (unsigned)((ntasks + kWaves - 1) / kWaves);
-  launch_timed(gc_grouped<T, TO>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc, ntasks,
-               (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, active);
+  if (lean)
+    launch_timed(gc_grouped<T, TO, true>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc,
+                 ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, active);
+  else
+    launch_timed(gc_grouped<T, TO, false>, dim3(blocks), dim3(64 * kWaves), 0, s, kt, descs, wgdesc, (int)ndesc,
+                 ntasks, (const cplx<T>*)x0, (const cplx<T>*)x1, (cplx<TO>*)obase, active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }
@@ -1249,7 +1333,7 @@ int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax
 #define QTG_GM(T, TM, TO)                                                                                      \
   template int launch_gm<T, TM, TO>(const GmDesc*, const int32_t*, int64_t, int64_t, const void*, const void*,  \
                                     const void*, void*, const uint8_t*, double, double, hipStream_t,            \
-                                    const KernelTiming*, int);
+                                    const KernelTiming*, int, bool);
 QTG_GM(double, double, double)
 QTG_GM(float, float, float)
 QTG_GM(float, double, float)
@@ -1257,7 +1341,7 @@ QTG_GM(float, double, double)
 #undef QTG_GM
 #define QTG_GC(T, TO)                                                                                           \
   template int launch_gc<T, TO>(const GcDesc*, const int32_t*, int64_t, int64_t, const void*, const void*, void*, \
-                                const uint8_t*, hipStream_t, const KernelTiming*);
+                                const uint8_t*, hipStream_t, const KernelTiming*, bool);
 QTG_GC(double, double)
 QTG_GC(float, float)
 QTG_GC(float, double)

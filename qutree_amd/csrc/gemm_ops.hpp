@@ -72,13 +72,32 @@ __host__ __device__ inline int64_t gc_tasks(const GcDesc& d) {
 // descriptor without a search.  The chunks of a split tile sum in LDS inside
 // their workgroup, which needs the task order of order_tasks.  No two
 // descriptors of one launch may write the same output.
+// lean: every descriptor of the launch passes gm_lean_ok / gc_lean_ok; the
+// launch then uses a kernel without the general path (fewer registers, more
+// wavefronts per SIMD).
 template <typename T, typename TM, typename TO>
 int launch_gm(const GmDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
               const void* x1, const void* mbase, void* obase, const uint8_t* active, double sre, double sim,
-              hipStream_t s, const KernelTiming* kt = nullptr, int waves = kGemmWaves);
+              hipStream_t s, const KernelTiming* kt = nullptr, int waves = kGemmWaves, bool lean = false);
 template <typename T, typename TO>
 int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t ntasks, const void* x0,
-              const void* x1, void* obase, const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr);
+              const void* x1, void* obase, const uint8_t* active, hipStream_t s, const KernelTiming* kt = nullptr,
+              bool lean = false);
+
+// The lean forms of the grouped kernels (gemm_ops.hip) take descriptors
+// whose tensors each span fewer than 2^27 elements, with a power-of-two K
+// split (and, for mode products, no plain copy): their element offsets fit
+// 32 bits.
+__host__ __device__ inline bool gm_lean_ok(const GmDesc& d) {
+  constexpr int64_t lim = (int64_t)1 << 27;
+  return d.op != GM_COPY && (int64_t)d.bef * d.nin * d.aft < lim && (int64_t)d.bef * d.nout * d.aft < lim &&
+         (int64_t)d.nin * d.nout < lim && (d.ksplit & (d.ksplit - 1)) == 0;
+}
+__host__ __device__ inline bool gc_lean_ok(const GcDesc& d) {
+  constexpr int64_t lim = (int64_t)1 << 27;
+  return (int64_t)d.bef * d.nx * d.aft < lim && (int64_t)d.bef * d.ny * d.aft < lim &&
+         (d.ksplit & (d.ksplit - 1)) == 0;
+}
 
 // K split of a reduction of length K into at most maxs chunks: the largest
 // power of two not above `want` whose chunks stay at least min_k long.

@@ -171,20 +171,23 @@ def test_cmf8_global_memory_orthogonal_matches_oracle():
     assert np.max(np.abs(om.node_tensor(got["psi"], top) - om.node_tensor(ref["psi"], top))) < 1e-11
 
 
+@pytest.mark.parametrize("opts", [{"graph": 1}, {"fuse": 0}, {"graph": 1, "fuse": 0}])
 @pytest.mark.parametrize("model", ["cmf8", "cmf64"])
-def test_cmf_tick_graphs_equal_eager_ticks(model):
-    """Ticks replayed from the HIP graphs (default) and launched eagerly run
-    the same kernels on the same staged programs: energies, autocorrelations,
-    statistics and the final state agree bit for bit, also across a second
-    integrate call that continues from the first one's state."""
+def test_cmf_tick_forms_equal_default(model, opts):
+    """Ticks replayed from HIP graphs, and per-op vector programs instead of
+    the fused op sequences, run the same arithmetic per element as the
+    default (eager ticks, fused ops): energies, autocorrelations, statistics
+    and the final state agree bit for bit, also across a second integrate
+    call that continues from the first one's state."""
     text, H = getattr(W, model + "_model")()
-    print(f"case cmf tick graph vs eager {model}", flush=True)
+    print(f"case cmf tick forms {model} {opts}", flush=True)
     runs = []
-    for graph in (1, 0):
+    for o in ({}, opts):
         eng = Q.Engine(text, H)
         psi = eng.fill_random(64 if model == "cmf64" else 8, delta_lowest=True)
         cmf = Q.CMF(eng)
-        cmf.set_option("graph", graph)
+        for k, v in o.items():
+            cmf.set_option(k, v)
         a = cmf.integrate(psi, 0.0, 0.3, 0.1, 0.1)
         b = cmf.integrate(a["psi"], 0.3, 0.5, 0.1, 0.1)
         runs.append((a, b))
