@@ -60,10 +60,17 @@ int Cmf::init() {
   s_ = E.stream_;
   const size_t arena = sizeof(cplx<double>) * total_;
   int rc;
-  for (cplx<double>** a : {&P_, &P0_, &P2_, &PS_, &X_, &D_})
+  for (cplx<double>** a : {&P_, &P0_, &P2_, &PS_})
     if ((rc = dalloc((void**)a, arena))) return rc;
   for (int k = 0; k < 5; ++k)
     if ((rc = dalloc((void**)&Y_[k], arena))) return rc;
+  // A tick's derivative input is the midpoint state yvec[1] and its output
+  // lands in yvec[2], where BS_integrator keeps the current derivative, so
+  // the copies x <- y1 and y2 <- d vanish (every op that fills x fills y1
+  // with the same values, and y2 is not read between a tick's Apply and the
+  // post program that would copy d into it).
+  X_ = Y_[1];
+  D_ = Y_[2];
   for (int k = 0; k < 7; ++k)
     if ((rc = dalloc((void**)&YT_[k], arena))) return rc;
   if ((rc = dalloc((void**)&err_, sizeof(double) * NN_))) return rc;
@@ -509,7 +516,7 @@ Cmf::TickLayout Cmf::tick_layout() const {
 
 // The launches of one tick on the staged programs: pre programs, the masked
 // Apply and finish, post programs.
-int Cmf::tick_launches(const CmfBufs& bb) {
+int Cmf::tick_launches(const CmfBufs& bb, bool post) {
   const TickLayout L = tick_layout();
   unsigned char* ds = (unsigned char*)dev_stage_;
   const VecOp* ops = (const VecOp*)(ds + L.ops);
@@ -517,6 +524,7 @@ int Cmf::tick_launches(const CmfBufs& bb) {
                            rhoslot_, err_, s_);
   if (rc) return rc;
   if ((rc = E.apply_finish((const double*)X_, (double*)D_, ds, ph_re_, ph_im_, nullptr, s_))) return rc;
+  if (!post) return QTG_OK;
   return launch_programs((const NodeProg*)(ds + L.post), NN_, ops, bb, E.d_nodes_, (const cplx<double>*)E.d_pool_,
                          rhoslot_, err_, s_);
 }
@@ -532,7 +540,7 @@ void Cmf::drop_tick_graphs() {
 // One tick on the engine stream: replayed from the graph of its state
 // buffer (captured on first use, again after a phase change), or eagerly
 // while graphs are off or the engine times its kernels.
-int Cmf::run_tick(cplx<double>* P) {
+int Cmf::run_tick(cplx<double>* P, bool post) {
   // rho^-1 of the last build comes from the engine's side stream: wait for
   // it here, outside any capture
   if (E.heevPending_) {
@@ -540,17 +548,17 @@ int Cmf::run_tick(cplx<double>* P) {
     E.heevPending_ = false;
   }
   const CmfBufs bb = bufs(P);
-  if (!tickGraph_ || E.timing_) return tick_launches(bb);
+  if (!tickGraph_ || E.timing_) return tick_launches(bb, post);
   if (tickPh_[0] != ph_re_ || tickPh_[1] != ph_im_) {
     drop_tick_graphs();
     tickPh_[0] = ph_re_;
     tickPh_[1] = ph_im_;
   }
   const int w = (P == P2_) ? 1 : 0;
-  if (P != P_ && P != P2_) return tick_launches(bb);
+  if (P != P_ && P != P2_) return tick_launches(bb, post);
   if (!tickExec_[w]) {
     QTG_HIP_TRY(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-    int rc = tick_launches(bb);
+    int rc = tick_launches(bb, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s_, &g);
     if (rc) {
@@ -588,7 +596,7 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
   std::vector<uint8_t> act(NN_);
   while (true) {
     ops.clear();
-    int nact = 0, npre = 0;
+    int nact = 0, npre = 0, npost = 0;
     bool need_err = false;
     for (int v = 0; v < NN_; ++v) {
       act[v] = 0;
@@ -606,6 +614,10 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
         fuse_ops(pre);
         fuse_ops(post);
       }
+      // copies between arenas that are the same buffer (x = y1, d = y2)
+      auto self_copy = [&](const VecOp& o) { return o.code == V_COPY && bb.b[o.a] == bb.b[o.b]; };
+      pre.erase(std::remove_if(pre.begin(), pre.end(), self_copy), pre.end());
+      post.erase(std::remove_if(post.begin(), post.end(), self_copy), post.end());
       if (!pre.empty()) {
         ppre[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + pre.size()), 0};
         ops.insert(ops.end(), pre.begin(), pre.end());
@@ -615,6 +627,7 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
       act[v] = 1;
       ++nact;
       ++nderiv_;
+      npost += post.empty() ? 0 : 1;
       ppost[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + post.size()), 0};
       for (auto& o : post)
         if (o.code == V_ERROR) need_err = true;
@@ -647,7 +660,7 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
       break;
     }
     ++nticks_;
-    int rc = run_tick(P);
+    int rc = run_tick(P, npost > 0);
     if (rc) return rc;
     if (need_err) {
       // the next tick's decisions read the errors

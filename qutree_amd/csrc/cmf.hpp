@@ -77,8 +77,9 @@ class Cmf {
     size_t pre, post, ops;  // byte offsets in the staging; the active mask at 0
   };
   TickLayout tick_layout() const;
-  int tick_launches(const CmfBufs& bb);
-  int run_tick(cplx<double>* P);
+  // post: launch the post programs (false when the tick has none)
+  int tick_launches(const CmfBufs& bb, bool post);
+  int run_tick(cplx<double>* P, bool post);
   void drop_tick_graphs();
   hipGraphExec_t tickExec_[2] = {nullptr, nullptr};  // ticks on P_ and on P2_
   double tickPh_[2] = {0.0, 0.0};                    // the phase they were captured with
