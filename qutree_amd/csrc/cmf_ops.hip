@@ -47,20 +47,22 @@ static_assert(kCmfBufs == 17, "bp() covers every buffer");
 // Elementwise ops: Op::load(e) reads every operand of element e, and
 // Op::store(e, v) writes its results.  The buffers of one op are different
 // arenas, but the compiler cannot prove it, so a store would order every later
-// load behind it: a full pass loads kPass elements per thread before storing
-// any, and the remainder goes one element at a time.
-constexpr int kPass = 4;
+// load behind it: a pass loads up to P elements per thread (the tail
+// predicated) before storing any, so a node of up to P * TB elements takes
+// one memory round trip per op.  P shrinks with the operands per element
+// (16 for one complex value, 8 for three) to stay in registers.
 template <class Op>
 __device__ __forceinline__ void elementwise(int N, const Op& op) {
-  int e0 = threadIdx.x;
-  for (; e0 + (kPass - 1) * TB < N; e0 += kPass * TB) {
-    typename Op::Val v[kPass];
+  constexpr int P = sizeof(typename Op::Val) <= 16 ? 16 : sizeof(typename Op::Val) <= 32 ? 12 : 8;
+  for (int e0 = threadIdx.x; e0 < N; e0 += P * TB) {
+    typename Op::Val v[P];
 #pragma unroll
-    for (int u = 0; u < kPass; ++u) v[u] = op.load(e0 + u * TB);
+    for (int u = 0; u < P; ++u)
+      if (e0 + u * TB < N) v[u] = op.load(e0 + u * TB);
 #pragma unroll
-    for (int u = 0; u < kPass; ++u) op.store(e0 + u * TB, v[u]);
+    for (int u = 0; u < P; ++u)
+      if (e0 + u * TB < N) op.store(e0 + u * TB, v[u]);
   }
-  for (int e = e0; e < N; e += TB) op.store(e, op.load(e));
 }
 
 struct CopyOp {  // a <- b

@@ -899,6 +899,15 @@ void Engine::dump_plan() const {
           ++outs;
         }
         std::fprintf(stderr, "plan %-5s %2zu distinct bra %zu ket %zu\n", name, i, xs.size(), ys.size());
+        std::map<std::tuple<int, int, int, int, int>, int> shapes;  // (nx, ny, bef, aft, ksplit) -> descriptors
+        for (int64_t t = 0; t < g.ntasks; ++t) {
+          const GcDesc& d = h_gc_[g.d0 + t];
+          if (d.tile0 == t) ++shapes[std::make_tuple(d.nx, d.ny, d.bef, d.aft, d.ksplit)];
+        }
+        for (auto& kv : shapes)
+          std::fprintf(stderr, "plan %-5s %2zu   nx %d ny %d bef %d aft %d ksplit %d: %d\n", name, i,
+                       std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first),
+                       std::get<4>(kv.first), kv.second);
       }
       std::fprintf(stderr, "plan %-5s %2zu kind %d prec %d tasks %7lld descs %6ld K %d..%d flops %.3e bytes %.3e\n",
                    name, i, g.kind, g.prec, (long long)g.ntasks, outs, kmin, kmax, g.flops, g.bytes);

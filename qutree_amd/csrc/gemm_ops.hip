@@ -400,6 +400,38 @@ __device__ inline void gm_lean(const GmDesc& d, const GmTile& g, const cplx<T>* 
   }
 }
 
+// out(b,i,a) = alpha X(b,i,a) for one 16 x 16 tile in the lean form: one
+// row per lane (16 consecutive rows per column), four columns per lane
+template <typename T, typename TO>
+__device__ inline void gm_lean_copy(const GmDesc& d, const GmTile& g, const cplx<T>* __restrict__ X, cplx<TO>* O,
+                                    double sre, double sim) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kk = lane >> 4;
+  const uint32_t bef = (uint32_t)d.bef, R = bef * (uint32_t)d.aft, nout = (uint32_t)d.nout;
+  const uint32_t r = (uint32_t)g.rt * 16u + (uint32_t)li;
+  if (r >= R) return;
+  const uint32_t a = r / bef, b = r - a * bef;
+  const uint32_t xo = b + a * bef * (uint32_t)d.nin, oo = b + a * bef * nout;
+  const cplx<T>* xa = X + d.x;
+  cplx<TO>* ob = O + d.out;
+  const T are = (T)(d.are * sre - d.aim * sim), aim = (T)(d.are * sim + d.aim * sre);
+  cplx<T> x[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t col = min((uint32_t)(g.ct * 16 + kk * 4 + q), nout - 1u);
+    x[q] = ld32(xa, xo + col * bef);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t col = (uint32_t)(g.ct * 16 + kk * 4 + q);
+    if (col < nout) {
+      cplx<TO>* o = at32(ob, oo + col * bef);
+      cplx<TO> v = mk<TO>((TO)(are * x[q].re - aim * x[q].im), (TO)(are * x[q].im + aim * x[q].re));
+      if (d.acc) v = cadd(v, *o);
+      *o = v;
+    }
+  }
+}
+
 template <typename T, typename TO>
 __device__ inline void gm_lean_store(const GmDesc& d, const GmTile& g, const cplx<TO> (&v)[4], cplx<TO>* O) {
   const int lane = threadIdx.x & 63;
@@ -460,7 +492,12 @@ __global__ void __launch_bounds__(64 * W)
   const bool lean = mine && (LEAN || gm_lean_ok(d));
   if (lean) {
     g = gm_tile_lean(d, (uint32_t)(task - d.tile0));
-    gm_lean<T, TM, TO>(d, g, X, M, sre, sim, v);
+    if (d.op == GM_COPY) {
+      gm_lean_copy<T, TO>(d, g, X, O, sre, sim);
+      mine = false;
+    } else {
+      gm_lean<T, TM, TO>(d, g, X, M, sre, sim, v);
+    }
   } else if (!LEAN && mine) {
     g = gm_tile(d, task - d.tile0);
     if (d.op == GM_COPY) {

@@ -86,11 +86,10 @@ int launch_gc(const GcDesc* descs, const int32_t* wgdesc, int64_t ndesc, int64_t
 
 // The lean forms of the grouped kernels (gemm_ops.hip) take descriptors
 // whose tensors each span fewer than 2^27 elements, with a power-of-two K
-// split (and, for mode products, no plain copy): their element offsets fit
-// 32 bits.
+// split: their element offsets fit 32 bits.
 __host__ __device__ inline bool gm_lean_ok(const GmDesc& d) {
   constexpr int64_t lim = (int64_t)1 << 27;
-  return d.op != GM_COPY && (int64_t)d.bef * d.nin * d.aft < lim && (int64_t)d.bef * d.nout * d.aft < lim &&
+  return (int64_t)d.bef * d.nin * d.aft < lim && (int64_t)d.bef * d.nout * d.aft < lim &&
          (int64_t)d.nin * d.nout < lim && (d.ksplit & (d.ksplit - 1)) == 0;
 }
 __host__ __device__ inline bool gc_lean_ok(const GcDesc& d) {
