@@ -316,6 +316,10 @@ int qtg_engine_read_timing(qtg_engine engine, double* ms, int64_t* launches, dou
  * options not set explicitly come from the tuning table's nearest class
  * (qtg_engine_tuning); 0 keeps the defaults.
  * "grad_recompute" (default 0): see qtg_engine_backward.
+ * "fuse_holes" (before qtg_engine_set_hamiltonian; default 1): complex128
+ * sweeps of binary trees whose hole outputs are at most 16 x 16 run each
+ * hole layer as one launch (ket tensors built in LDS beside the hole
+ * contractions that read them); 0 keeps two launches per layer.
  * "heev_side" (default 1): 1 runs the grouped sweep's density eigensolver on
  * a second stream, overlapping the hole contractions; 0 runs it in line.
  * Plan options set after qtg_engine_set_hamiltonian return QTG_ESTATE;

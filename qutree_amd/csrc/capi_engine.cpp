@@ -296,7 +296,7 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   // set_hamiltonian: changing them afterwards would run kernels on buffers
   // sized for another plan
   static const char* const planKeys[] = {"compress",  "path",       "split_waves", "split_min_k",
-                                         "gc_long_k", "gm_split_k", "tuning"};
+                                         "gc_long_k", "gm_split_k", "tuning",      "fuse_holes"};
   for (const char* k : planKeys)
     if (!std::strcmp(key, k) && en->planned()) {
       set_error(std::string("option ") + key + ": set it before qtg_engine_set_hamiltonian");
@@ -371,6 +371,10 @@ int qtg_engine_set_option(qtg_engine e, const char* key, double value) {
   if (!std::strcmp(key, "graph")) {
     en->graph_ = value != 0.0;
     en->invalidate_graph();
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "fuse_holes")) {
+    en->fuseHoles_ = value != 0.0;
     return QTG_OK;
   }
   if (!std::strcmp(key, "heev_side")) {

@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(EW)
   const OrthoItem it = items[blockIdx.x];
   const DNode nd = nodes[it.node];
   const int N = nd.N, n = nodes[it.node].dims[nd.order - 1], L = N / n, ld = jacobi_ld(n);
-  for (int e = threadIdx.x; e < N; e += EW) Phi[e] = P[nd.off + e];
+  stage_to_lds<EW>(Phi, P + nd.off, N);
   __syncthreads();
   if (it.sw >= 0) {  // uniform per workgroup
     for (int o = threadIdx.x; o < n * n; o += EW) {
@@ -659,7 +659,7 @@ __global__ void __launch_bounds__(TB)
   const SwApply it = items[blockIdx.x];
   const DNode nd = nodes[it.node];
   const int N = nd.N;
-  for (int e = threadIdx.x; e < N; e += TB) b0[e] = P[nd.off + e];
+  stage_to_lds<TB>(b0, P + nd.off, N);
   __syncthreads();
   cd2* cur = b0;
   cd2* nxt = b1;

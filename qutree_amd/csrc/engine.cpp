@@ -566,6 +566,10 @@ int Engine::plan() {
     device_buffers_.push_back(d_gm_);
     if ((rc = up(&d_gc_, h_gc_))) return rc;
     device_buffers_.push_back(d_gc_);
+    if ((rc = up(&d_hk_, h_hk_))) return rc;
+    device_buffers_.push_back(d_hk_);
+    if ((rc = up(&d_hi_, h_hi_))) return rc;
+    device_buffers_.push_back(d_hi_);
     if ((rc = up(&d_solve_, h_solve_))) return rc;
     device_buffers_.push_back(d_solve_);
     if ((rc = up(&d_combOut3_, h_comb3_))) return rc;

@@ -107,6 +107,11 @@ class Engine {
   // measured equal for replayed sweeps and faster for CMF, whose first tick
   // after a build overlaps the eigensolver
   bool heevSide_ = true;
+  // hole layers of complex128 binary-tree sweeps as one launch each
+  // (hole_fused_kernel: kets built in LDS, bras of all layers in one launch
+  // before them); false keeps a mode-product and a hole-contraction launch
+  // per layer
+  bool fuseHoles_ = true;
   // device bytes this engine holds
   int64_t device_bytes() const { return deviceBytes_; }
   hipStream_t stream() const { return stream_; }
@@ -153,6 +158,7 @@ class Engine {
     double bytes;  // algorithmic HBM bytes (distinct inputs once, outputs)
     int waves;     // mode products: tasks per workgroup (0: kGemmWaves)
     bool lean;     // every descriptor takes the lean kernel form
+    int64_t lds;   // fused hole layers (kind 3): dynamic LDS bytes per workgroup
   };
   bool gramInSolve_ = false;  // the solve launch forms S, A^H Y and the energy terms
   int run_launches(const std::vector<GLaunch>& L, void* const* bases, const uint8_t* active, double sre, double sim,
@@ -220,6 +226,10 @@ class Engine {
   // grouped path
   std::vector<GmDesc> h_gm_;
   std::vector<GcDesc> h_gc_;
+  std::vector<HoleKet> h_hk_;   // fused hole layers: kets and their items
+  std::vector<HoleItem> h_hi_;
+  HoleKet* d_hk_ = nullptr;
+  HoleItem* d_hi_ = nullptr;
   std::vector<int32_t> h_gmwg_, h_gcwg_;
   int32_t* d_gmwg_ = nullptr;
   int32_t* d_gcwg_ = nullptr;

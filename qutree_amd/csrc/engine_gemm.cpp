@@ -329,7 +329,164 @@ int Engine::plan_gemm() {
       }
       push_layer(m, gBuild_);
     }
-    for (size_t dd = 1; dd < hol.size(); ++dd) push_layer(hol[dd], gHole_);
+    // Hole layers.  Fused form (complex128, every ket a single factor, every
+    // output at most 16 x 16): one hole_fused launch per layer, each ket
+    // workgroup building its Z in LDS and running the items that read it.
+    // Measured on MI355X it pays only on layers of small parent tensors
+    // (at most kFuseMaxN elements; larger kets serialise too much work in one
+    // workgroup), so the other layers keep a mode-product and a
+    // hole-contraction launch.  The bras of the fused layers read only the
+    // state and the build's h-matrices: they ride in the mode-product launch
+    // of the last two-launch layer before the first fused one (the first
+    // layer is never fused).
+    h_hk_.clear();
+    h_hi_.clear();
+    constexpr int64_t kFuseMaxN = 1200;
+    const size_t NL = hol.size();
+    std::vector<char> fl(NL, 0);
+    std::vector<std::vector<HoleKet>> lk(NL);
+    std::vector<std::vector<HoleItem>> li(NL);
+    std::vector<int64_t> llds(NL, 0);
+    std::vector<double> lfl(NL, 0.0), lby(NL, 0.0);
+    std::vector<GmDesc> bras;
+    std::map<std::tuple<int, int, int64_t>, int64_t> braOff;
+    const int64_t bw0 = std::max(wmax, workSize_);  // bras after every other work tensor of the sweep
+    int64_t bw = bw0;
+    // the first layer keeps two launches: its mode products carry the bras
+    for (size_t dd = 2; fuseHoles_ && precision_ == 0 && dd < holeRange_.size(); ++dd) {
+      std::map<std::tuple<int, int, int64_t>, std::vector<HoleItem>> byKet;  // (node, leg, matrix)
+      bool ok = true;
+      std::vector<GmDesc> lb;  // this layer's new bras
+      std::map<std::tuple<int, int, int64_t>, int64_t> lbo;
+      int64_t lw = bw;
+      double fl_ = 0.0, by_ = 0.0;
+      for (int64_t i = holeRange_[dd].first; i < holeRange_[dd].second && ok; ++i) {
+        const SwItem& it = h_items_[i];
+        if (is_rho(it)) continue;
+        const HNode& p = tree_.nodes[it.node];
+        const int k = it.k, braleg = (k == 0) ? 1 : 0;
+        int64_t bmat = -1, kmat = -1;
+        int kleg = -1, nket = 0;
+        for (int f = 0; f < it.nf; ++f) {
+          if (it.fmode[f] == braleg) {
+            bmat = it.fmat[f];
+          } else {
+            kleg = it.fmode[f];
+            kmat = it.fmat[f];
+            ++nket;
+          }
+        }
+        int64_t bef, aft;
+        int nk;
+        view(p, k, bef, nk, aft);
+        if (nket > 1 || nk > 16 || p.N > kFuseMaxN) {
+          ok = false;
+          break;
+        }
+        HoleItem hi{};
+        if (bmat >= 0) {
+          const auto key = std::make_tuple((int)it.node, braleg, bmat);
+          auto f = braOff.find(key);
+          if (f == braOff.end()) {
+            f = lbo.find(key);
+            if (f == lbo.end()) {
+              lb.push_back(gm_desc(p, braleg, p.off, 0, bmat, lw, GM_H, it.node));
+              f = lbo.emplace(key, lw).first;
+              lw += p.N;
+            }
+          }
+          hi.x = f->second;
+          hi.sel = 1;
+        } else {
+          hi.x = p.off;
+          hi.sel = 0;
+        }
+        hi.out = it.out;
+        hi.bef = (int32_t)bef;
+        hi.n = nk;
+        hi.aft = (int32_t)aft;
+        byKet[std::make_tuple((int)it.node, kmat >= 0 ? kleg : -1, kmat)].push_back(hi);
+        fl_ += 8.0 * bef * aft * (double)nk * nk;
+        by_ += 16.0 * ((double)bef * aft * nk + (double)nk * nk);
+      }
+      if (!ok || byKet.empty()) continue;
+      fl[dd] = 1;
+      bras.insert(bras.end(), lb.begin(), lb.end());
+      braOff.insert(lbo.begin(), lbo.end());
+      bw = lw;
+      for (auto& kv : byKet) {
+        const HNode& p = tree_.nodes[std::get<0>(kv.first)];
+        HoleKet hk{};
+        hk.x = p.off;
+        hk.m = std::get<2>(kv.first);
+        hk.node = std::get<0>(kv.first);
+        if (hk.m >= 0) {
+          int64_t bef, aft;
+          int nl;
+          view(p, std::get<1>(kv.first), bef, nl, aft);
+          hk.bef = (int32_t)bef;
+          hk.nleg = nl;
+          hk.aft = (int32_t)aft;
+          fl_ += 8.0 * bef * aft * (double)nl * nl;
+          by_ += 16.0 * ((double)p.N + (double)nl * nl);
+        } else {
+          hk.bef = (int32_t)p.N;
+          hk.nleg = 1;
+          hk.aft = 1;
+          by_ += 16.0 * p.N;
+        }
+        // at most 8 items (two rounds of four wavefronts) per workgroup: a ket
+        // with more is built by several workgroups, each running a share
+        const int32_t i0 = (int32_t)li[dd].size();
+        li[dd].insert(li[dd].end(), kv.second.begin(), kv.second.end());
+        const int32_t i1 = (int32_t)li[dd].size();
+        for (int32_t a = i0; a < i1; a += 8) {
+          hk.it0 = a;
+          hk.it1 = std::min(i1, a + 8);
+          lk[dd].push_back(hk);
+        }
+        llds[dd] = std::max(llds[dd], hole_fused_lds(p.N));
+      }
+      lfl[dd] = fl_;
+      lby[dd] = by_;
+    }
+    workSize_ = std::max(workSize_, bw);
+    size_t first = NL;
+    for (size_t dd = 1; dd < NL; ++dd)
+      if (fl[dd]) {
+        first = dd;
+        break;
+      }
+    size_t host = 0;  // the two-launch layer whose mode products carry the bras
+    for (size_t dd = 1; dd < first && dd < NL; ++dd) host = dd;
+    if (first < NL) hol[host].lv0[1].insert(hol[host].lv0[1].end(), bras.begin(), bras.end());
+    for (size_t dd = 1; dd < NL; ++dd) {
+      if (!fl[dd]) {
+        push_layer(hol[dd], gHole_);
+        continue;
+      }
+      const int32_t ib = (int32_t)h_hi_.size();
+      for (HoleKet& hk : lk[dd]) {
+        hk.it0 += ib;
+        hk.it1 += ib;
+      }
+      GLaunch gl{};
+      gl.kind = 3;
+      gl.prec = P64;
+      gl.d0 = (int64_t)h_hk_.size();
+      gl.nd = gl.ntasks = (int64_t)lk[dd].size();
+      gl.x0 = B_PSI64;
+      gl.x1 = B_WORK64;
+      gl.m = B_POOL;
+      gl.o = B_POOL;
+      gl.lds = llds[dd];
+      gl.flops = lfl[dd];
+      gl.bytes = lby[dd];
+      flops += lfl[dd];
+      h_hk_.insert(h_hk_.end(), lk[dd].begin(), lk[dd].end());
+      h_hi_.insert(h_hi_.end(), li[dd].begin(), li[dd].end());
+      gHole_.push_back(gl);
+    }
   }
 
   // ---------------------------------------------------------------- apply
@@ -970,6 +1127,12 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
       else
         rc = launch_gc<float, double>(d, wg, gl.nd, gl.ntasks, x0, x1, o, active, s, kt, gl.lean);
       if (rc) return rc;
+      if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
+    } else if (gl.kind == 3) {
+      if ((rc = kernel_mark(7, &kt))) return rc;
+      if ((rc = launch_hole_fused(d_hk_ + gl.d0, gl.nd, d_hi_, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
+                                  gl.lds, active, s, kt)))
+        return rc;
       if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
     } else {
       if ((rc = kernel_mark(8, &kt))) return rc;

@@ -898,6 +898,98 @@ __global__ void __launch_bounds__(64 * kWaves)
   gc_store<T, T>(d, g, v, O);
 }
 
+// ------------------------------------------------------------ fused holes
+// One workgroup per ket: the four wavefronts build Z = M x_leg Psi_p in LDS
+// tile by tile (the lean mode product), then run the ket's hole
+// contractions with the bra from global memory and Z from LDS (the lean
+// hole contraction, both K halves in one tile for 8 x 8 outputs).  Two
+// items split K in two, one item in four (chunks summed in LDS as in the
+// grouped kernels); three or more take one wavefront each in turn.
+__global__ void __launch_bounds__(256)
+    hole_fused_kernel(const HoleKet* __restrict__ kets, const HoleItem* __restrict__ items,
+                      const cplx<double>* __restrict__ psi, const cplx<double>* __restrict__ work,
+                      const cplx<double>* __restrict__ pool, cplx<double>* opool,
+                      const uint8_t* __restrict__ active) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<double>* red = reinterpret_cast<cplx<double>*>(smem_raw);  // 4 x 256 partial tiles
+  cplx<double>* Z = red + 4 * 256;
+  const HoleKet kt = kets[blockIdx.x];
+  if (active && kt.node >= 0 && !active[kt.node]) return;  // the whole workgroup
+  const int wave = wave_id();
+  const int N = kt.bef * kt.nleg * kt.aft;
+  if (kt.m < 0) {
+    for (int base = 0; base < N; base += 8 * 256) {
+      cplx<double> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = psi[kt.x + min(base + u * 256 + (int)threadIdx.x, N - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = base + u * 256 + (int)threadIdx.x;
+        if (e < N) Z[e] = v[u];
+      }
+    }
+  } else {
+    GmDesc d{};
+    d.x = kt.x;
+    d.m = kt.m;
+    d.out = 0;
+    d.bef = kt.bef;
+    d.nin = d.nout = kt.nleg;
+    d.aft = kt.aft;
+    d.op = GM_N;
+    d.are = 1.0;
+    d.ksplit = 1;
+    const int tiles = ((kt.bef * kt.aft + 15) >> 4) * ((kt.nleg + 15) >> 4);
+    for (int t = wave; t < tiles; t += 4) {
+      const GmTile g = gm_tile_lean(d, (uint32_t)t);
+      cplx<double> v[4];
+      gm_lean<double, double, double>(d, g, psi, pool, 1.0, 0.0, v);
+      gm_lean_store<double, double>(d, g, v, Z);
+    }
+  }
+  __syncthreads();
+  const int nit = kt.it1 - kt.it0;
+  if (nit >= 3) {
+    for (int i = wave; i < nit; i += 4) {
+      const HoleItem it = items[kt.it0 + i];
+      GcDesc gd{};
+      gd.x = it.x;
+      gd.out = it.out;
+      gd.bef = it.bef;
+      gd.nx = gd.ny = it.n;
+      gd.aft = it.aft;
+      gd.ksplit = 1;
+      const GcTile g{0, 0, 0};
+      cplx<double> v[4];
+      const cplx<double>* X = (it.sel & 1) ? work : psi;
+      if (it.n <= 8) gc_lean_packed<double>(gd, g, X, Z, v);
+      else gc_lean<double, double>(gd, g, X, Z, v);
+      gc_store<double, double>(gd, g, v, opool);
+    }
+    return;
+  }
+  const int S = nit == 2 ? 2 : 4;
+  const int ii = wave / S, ks = wave % S;
+  const bool mine = ii < nit;
+  GcDesc gd{};
+  const GcTile g{0, 0, ks};
+  cplx<double> v[4];
+  if (mine) {
+    const HoleItem it = items[kt.it0 + ii];
+    gd.x = it.x;
+    gd.out = it.out;
+    gd.bef = it.bef;
+    gd.nx = gd.ny = it.n;
+    gd.aft = it.aft;
+    gd.ksplit = S;
+    const cplx<double>* X = (it.sel & 1) ? work : psi;
+    if (it.n <= 8) gc_lean_packed<double>(gd, g, X, Z, v);
+    else gc_lean<double, double>(gd, g, X, Z, v);
+  }
+  reduce_chunks<double>(red, mine, S, ks, v);
+  if (mine && ks == 0) gc_store<double, double>(gd, g, v, opool);
+}
+
 // ------------------------------------------------------------------- solve
 // One workgroup per system: Gauss-Jordan with partial pivoting on [S | X] in
 // LDS; the pivot (largest modulus, lowest row on ties) comes from an argmax
@@ -1363,6 +1455,17 @@ int launch_solve(const SolveItem* items, int64_t n, cplx<double>* pool, int nmax
   }
   const size_t lds = 2 * sizeof(cplx<double>) * (size_t)nmax * nmax;
   launch_timed(solve_kernel, dim3((unsigned)n), dim3(256), (unsigned)lds, s, kt, items, pool, active);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
+int launch_hole_fused(const HoleKet* kets, int64_t nkets, const HoleItem* items, const void* psi, const void* work,
+                      const void* pool, void* opool, int64_t lds_bytes, const uint8_t* active, hipStream_t s,
+                      const KernelTiming* kt) {
+  if (nkets <= 0) return QTG_OK;
+  launch_timed(hole_fused_kernel, dim3((unsigned)nkets), dim3(256), (unsigned)lds_bytes, s, kt, kets, items,
+               (const cplx<double>*)psi, (const cplx<double>*)work, (const cplx<double>*)pool, (cplx<double>*)opool,
+               active);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }

@@ -54,6 +54,28 @@ struct GcDesc {
   int64_t tile0;
 };
 
+// One hole-contraction layer in one launch (launch_hole_fused): a
+// workgroup per ket tensor Z = M x_leg Psi_p (or Psi_p itself), built in
+// LDS, and the hole contractions out = <X|Z> over all legs but one that
+// read it; the bras X were formed before the layer.  complex128; every
+// output at most 16 x 16.
+struct HoleKet {
+  int64_t x, m;            // Psi_p offset (state); pool offset of M (nleg x nleg, column-major), -1: Z = Psi_p
+  int32_t bef, nleg, aft;  // Psi_p viewed on the ket leg
+  int32_t node;            // active-mask index (-1: always run)
+  int32_t it0, it1;        // this ket's items
+};
+struct HoleItem {
+  int64_t x, out;       // bra offset (work buffer when sel & 1, else the state); n x n output in the pool
+  int32_t bef, n, aft;  // the node viewed on the contracted leg
+  int32_t sel;
+};
+int launch_hole_fused(const HoleKet* kets, int64_t nkets, const HoleItem* items, const void* psi, const void* work,
+                      const void* pool, void* opool, int64_t lds_bytes, const uint8_t* active, hipStream_t s,
+                      const KernelTiming* kt = nullptr);
+// LDS bytes of a hole_fused workgroup whose ket tensor has N elements
+inline int64_t hole_fused_lds(int64_t N) { return (4 * 256 + N) * 16; }
+
 // number of tasks of one descriptor (16 x 16 tiles, times K chunks)
 __host__ __device__ inline int64_t gm_tasks(const GmDesc& d) {
   const int64_t R = (int64_t)d.bef * d.aft;

@@ -8,6 +8,23 @@ namespace qtg {
 
 constexpr int TB = 256;
 
+// dst[e] = src[e] for e < N (N >= 1) by a workgroup of NT threads: eight
+// loads per thread are in flight before their LDS stores, instead of one
+// load round trip per element a thread copies.
+template <int NT, typename T>
+__device__ inline void stage_to_lds(cplx<T>* dst, const cplx<T>* __restrict__ src, int N) {
+  for (int base = 0; base < N; base += 8 * NT) {
+    cplx<T> v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[min(base + u * NT + (int)threadIdx.x, N - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * NT + (int)threadIdx.x;
+      if (e < N) dst[e] = v[u];
+    }
+  }
+}
+
 __device__ inline void node_view(const DNode& nd, int k, int& bef, int& nk, int& aft) {
   bef = 1;
   aft = 1;

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(TB)
   const DNode nd = nodes[it.node];
   const int N = nd.N;
   // ket into buf0; the bra stays in global memory unless it is the ket
-  for (int e = threadIdx.x; e < N; e += TB) buf0[e] = ket_base[it.ket + e];
+  stage_to_lds<TB>(buf0, ket_base + it.ket, N);
   const bool same = (bra_base == ket_base) && (it.bra == it.ket);
   __syncthreads();
   cplx<T>* cur = buf0;
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(TB)
     cplx<T>* b = (cur == buf0) ? buf1 : buf0;
     scratch = (cur == buf0) ? buf2 : ((cur == buf1) ? buf2 : buf1);
     __syncthreads();
-    for (int e = threadIdx.x; e < N; e += TB) b[e] = bra_base[it.bra + e];
+    stage_to_lds<TB>(b, bra_base + it.bra, N);
     __syncthreads();
     bra = b;
   }
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(TB)
   if (active && !active[ch.node]) return;  // uniform per workgroup
   const DNode nd = nodes[ch.node];
   const int N = nd.N;
-  for (int e = threadIdx.x; e < N; e += TB) buf0[e] = psi[nd.off + e];
+  stage_to_lds<TB>(buf0, psi + nd.off, N);
   cplx<T> acc[EPT];
 #pragma unroll
   for (int i = 0; i < EPT; ++i) acc[i] = czero<T>();
