@@ -287,7 +287,8 @@ int64_t qtg_host_fill_random_stream(const char* tree_text, uint64_t seed, int64_
  * combination], the summed milliseconds, launch counts, algorithmic FLOPs
  * and (grouped kernels) algorithmic bytes since set_timing(1); the grouped
  * path adds 6 mode products (gm_grouped), 7 hole contractions (gc_grouped),
- * 8 batched solves, 9 split-K reductions.  Arrays hold 10 entries; any
+ * 8 batched solves, 9 fused hole layers (hole_fused_kernel).  Arrays hold
+ * 10 entries; any
  * pointer may be null. */
 int qtg_engine_set_timing(qtg_engine engine, int on);
 /* waits for the engine's own stream */

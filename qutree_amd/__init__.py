@@ -365,7 +365,7 @@ class Engine:
         return f.value
 
     KERNELS = ("sandwich_kernel", "heev_reginv_kernel", "apply_kernel", "finish_kernel", "to_f32",
-               "combine_kernel", "gm_grouped", "gc_grouped", "solve_kernel", "split_reduce")
+               "combine_kernel", "gm_grouped", "gc_grouped", "solve_kernel", "hole_fused_kernel")
 
     def set_timing(self, on: bool) -> None:
         check(lib().qtg_engine_set_timing(self.ptr, int(on)))

@@ -1129,11 +1129,11 @@ int Engine::run_launches(const std::vector<GLaunch>& L, size_t i0, size_t i1, vo
       if (rc) return rc;
       if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
     } else if (gl.kind == 3) {
-      if ((rc = kernel_mark(7, &kt))) return rc;
+      if ((rc = kernel_mark(9, &kt))) return rc;
       if ((rc = launch_hole_fused(d_hk_ + gl.d0, gl.nd, d_hi_, bases[gl.x0], bases[gl.x1], bases[gl.m], bases[gl.o],
                                   gl.lds, active, s, kt)))
         return rc;
-      if (timing_) tfl_[7] += gl.flops, tby_[7] += gl.bytes;
+      if (timing_) tfl_[9] += gl.flops, tby_[9] += gl.bytes;
     } else {
       if ((rc = kernel_mark(8, &kt))) return rc;
       if ((rc = launch_solve(d_solve_, gl.nd, (cplx<double>*)bases[B_POOL64], nspf_, active, s, kt,
