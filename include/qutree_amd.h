@@ -361,8 +361,13 @@ int qtg_cmf_set_output_file(qtg_cmf cmf, const char* filename, int append);
  * tick eagerly (measured faster on MI355X).  "fuse" (default 1): the vector
  * op sequences a tick emits most (the Bulirsch-Stoer start, the modified
  * midpoint steps, the final midpoint with rational extrapolation) run as one
- * pass over the node each; 0 runs one pass per op.  Results are the same
- * either way for both options. */
+ * pass over the node each; 0 runs one pass per op.  "lookahead" (default
+ * 1): the tick after a tick that computes node errors is uploaded and
+ * launched before those errors reach the host, with both outcomes of each
+ * waiting node's accept test staged (the device picks one by the same
+ * comparison and sets the node's Apply mask); 0 waits for the errors before
+ * staging the next tick.  Results are the same either way for all three
+ * options. */
 int qtg_cmf_set_option(qtg_cmf cmf, const char* key, double value);
 int qtg_cmf_destroy(qtg_cmf cmf);
 int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend, double dt, double out,

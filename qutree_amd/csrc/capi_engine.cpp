@@ -410,6 +410,14 @@ int qtg_cmf_set_option(qtg_cmf c, const char* key, double value) {
     cm->fuseOps_ = value != 0.0;
     return QTG_OK;
   }
+  if (!std::strcmp(key, "lookahead")) {
+    cm->lookahead_ = value != 0.0;
+    return QTG_OK;
+  }
+  if (!std::strcmp(key, "device_bs")) {
+    cm->deviceBs_ = value != 0.0;
+    return QTG_OK;
+  }
   set_error(std::string("unknown cmf option ") + key);
   return QTG_EARG;
 }

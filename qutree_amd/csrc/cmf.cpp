@@ -21,10 +21,12 @@
 namespace qtg {
 
 namespace {
-constexpr int kMaxUse = 7, kMax = 11;
-constexpr double kShrink = 0.95, kGrow = 1.2;
-const int kSeq[kMax] = {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96};
+constexpr int kMaxUse = kBsMaxUse, kMax = kBsMax;
+constexpr double kShrink = kBsShrink, kGrow = kBsGrow;
+const int kSeq[kMax] = {bs_seq(0), bs_seq(1), bs_seq(2), bs_seq(3), bs_seq(4), bs_seq(5),
+                        bs_seq(6), bs_seq(7), bs_seq(8), bs_seq(9), bs_seq(10)};
 constexpr int kOpsPerNode = 16;
+constexpr int64_t kMaxTicks = int64_t(1) << 22;  // per interval of the device integrators
 }  // namespace
 
 Cmf::~Cmf() {
@@ -32,8 +34,12 @@ Cmf::~Cmf() {
   for (void* p : dev_)
     if (p) (void)hipFree(p);
   if (host_stage_) (void)hipHostFree(host_stage_);
+  if (err_h_) (void)hipHostFree(err_h_);
+  if (h_info_) (void)hipHostFree(h_info_);
+  if (evInfo_) (void)hipEventDestroy(evInfo_);
   for (hipEvent_t e : evStage_)
     if (e) (void)hipEventDestroy(e);
+  if (evErr_) (void)hipEventDestroy(evErr_);
 }
 
 int Cmf::dalloc(void** p, size_t bytes) {
@@ -81,7 +87,8 @@ int Cmf::init() {
   if ((rc = dalloc((void**)&rhoslot_, sizeof(int64_t) * NN_))) return rc;
   QTG_HIP_TRY(hipMemcpy(rhoslot_, rs.data(), sizeof(int64_t) * NN_, hipMemcpyHostToDevice));
   // staging for per-tick programs and the active mask
-  stage_bytes_ = 2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode) + NN_ + 1024;
+  // (each half starts on a 256-byte boundary)
+  stage_bytes_ = (2 * (sizeof(NodeProg) * NN_ + sizeof(VecOp) * NN_ * kOpsPerNode) + NN_ + 1024 + 255) & ~size_t(255);
   if (hipHostMalloc(&host_stage_, 2 * stage_bytes_) != hipSuccess) {
     set_error("cmf: pinned allocation failed");
     return QTG_ENOMEM;
@@ -89,6 +96,13 @@ int Cmf::init() {
   if ((rc = dalloc(&dev_stage_, stage_bytes_))) return rc;
   QTG_HIP_TRY(hipHostMalloc((void**)&err_h_, sizeof(double) * NN_));
   for (hipEvent_t& e : evStage_) QTG_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  QTG_HIP_TRY(hipEventCreateWithFlags(&evErr_, hipEventDisableTiming));
+  // the device integrators
+  if ((rc = dalloc((void**)&d_bs_, sizeof(BsNode) * NN_))) return rc;
+  if ((rc = dalloc((void**)&d_info_, sizeof(BsInfo)))) return rc;
+  if ((rc = dalloc((void**)&d_mask_, NN_))) return rc;
+  QTG_HIP_TRY(hipHostMalloc((void**)&h_info_, sizeof(BsInfo)));
+  QTG_HIP_TRY(hipEventCreateWithFlags(&evInfo_, hipEventDisableTiming));
   // orthogonal(): per height, the nodes to orthonormalise and the parents
   // receiving their square roots (children in node-address order)
   int64_t swoff = 0;
@@ -366,14 +380,15 @@ static VecOp op0(int code, int a = 0, int b = 0, int c = 0, double h = 0.0) {
 
 // Emits the node's ops up to and around its next derivative evaluation.
 // Returns false when the node has finished its interval.
-bool Cmf::advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, double eps) {
+// verdict: the outcome of a pending accept test (1 accept, 0 reject) in
+// place of the error read back from the device (-1).
+bool Cmf::advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, double eps, int verdict) {
   NodeBS& b = bs_[v];
   double& dt = dtbs_[v];
   while (true) {
     switch (b.st) {
       case NodeBS::AWAIT_ERR: {
-        const double e = err_h_[v];
-        if (e < eps) {
+        if (verdict < 0 ? err_h_[v] < eps : verdict == 1) {
           b.x += b.h;
           if (b.i == kMaxUse - 1) dt = b.h * kShrink;
           else if (b.i == kMaxUse - 2) dt = b.h * kGrow;
@@ -489,7 +504,10 @@ static void fuse_ops(std::vector<VecOp>& o) {
 void Cmf::finish_mmid(int v, std::vector<VecOp>& post) {
   NodeBS& b = bs_[v];
   post.push_back(op0(V_MMID_FINAL, 0, 0, 0, b.hh));
-  const double xest = std::pow(b.h / (1.0 * b.step), 2);
+  // pow(h / n, 2) of the reference (BS_integrator.h:121) as a product: the
+  // device integrators compute the same correctly rounded square
+  const double hs = b.h / (1.0 * b.step);
+  const double xest = hs * hs;
   b.xtab[b.i] = xest;
   VecOp r = op0(V_RZEXTR);
   r.i = b.i;
@@ -520,13 +538,16 @@ int Cmf::tick_launches(const CmfBufs& bb, bool post) {
   const TickLayout L = tick_layout();
   unsigned char* ds = (unsigned char*)dev_stage_;
   const VecOp* ops = (const VecOp*)(ds + L.ops);
-  int rc = launch_programs((const NodeProg*)(ds + L.pre), NN_, ops, bb, E.d_nodes_, (const cplx<double>*)E.d_pool_,
-                           rhoslot_, err_, s_);
+  const cplx<double>* pool = (const cplx<double>*)E.d_pool_;
+  int rc = launch_programs((const NodeProg*)(ds + L.pre), NN_, ops, bb, E.d_nodes_, pool, rhoslot_, err_, s_, ds);
   if (rc) return rc;
   if ((rc = E.apply_finish((const double*)X_, (double*)D_, ds, ph_re_, ph_im_, nullptr, s_))) return rc;
   if (!post) return QTG_OK;
-  return launch_programs((const NodeProg*)(ds + L.post), NN_, ops, bb, E.d_nodes_, (const cplx<double>*)E.d_pool_,
-                         rhoslot_, err_, s_);
+  return launch_programs((const NodeProg*)(ds + L.post), NN_, ops, bb, E.d_nodes_, pool, rhoslot_, err_, s_);
+}
+
+bool Cmf::tick_eager(const cplx<double>* P) const {
+  return !tickGraph_ || E.timing_ || (P != P_ && P != P2_);
 }
 
 void Cmf::drop_tick_graphs() {
@@ -548,14 +569,13 @@ int Cmf::run_tick(cplx<double>* P, bool post) {
     E.heevPending_ = false;
   }
   const CmfBufs bb = bufs(P);
-  if (!tickGraph_ || E.timing_) return tick_launches(bb, post);
+  if (tick_eager(P)) return tick_launches(bb, post);
   if (tickPh_[0] != ph_re_ || tickPh_[1] != ph_im_) {
     drop_tick_graphs();
     tickPh_[0] = ph_re_;
     tickPh_[1] = ph_im_;
   }
   const int w = (P == P2_) ? 1 : 0;
-  if (P != P_ && P != P2_) return tick_launches(bb, post);
   if (!tickExec_[w]) {
     QTG_HIP_TRY(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     int rc = tick_launches(bb, true);
@@ -581,7 +601,59 @@ int Cmf::run_tick(cplx<double>* P, bool post) {
   return QTG_OK;
 }
 
+// The same interval with every node's integrator on the device: a tick is
+// the masked Apply and one bs_tick launch (the tick's post ops, the
+// decisions, the next tick's pre ops and mask).  The host reads the
+// per-tick record only where the ticks known to have a derivative run out,
+// after it has launched the next tick (a tick without one does nothing).
+int Cmf::cmfstep_dev(cplx<double>* P, double t0, double t1, double eps) {
+  if (E.heevPending_) {
+    QTG_HIP_TRY(hipStreamWaitEvent(s_, E.evHeev_, 0));
+    E.heevPending_ = false;
+  }
+  const CmfBufs bb = bufs(P);
+  const cplx<double>* pool = (const cplx<double>*)E.d_pool_;
+  int slot = 0;  // the record of the next tick
+  auto boundary = [&](int init) {
+    slot = slot_;
+    slot_ = (slot_ + 1) % kBsRing;
+    return launch_bs_tick(d_bs_, NN_, bb, E.d_nodes_, pool, rhoslot_, err_, d_mask_, d_info_, slot, init, t0, t1,
+                          eps, s_);
+  };
+  int rc = boundary(1);
+  if (rc) return rc;
+  int64_t known = -1;  // ticks up to this one surely have a derivative
+  for (int64_t t = 0;; ++t) {
+    if (t > kMaxTicks) {
+      set_error("cmf: an interval took more than 2^22 ticks");
+      return QTG_ESTATE;
+    }
+    const bool check = t > known;
+    const int rec = slot;
+    if (check) {
+      QTG_HIP_TRY(hipMemcpyAsync(h_info_, d_info_, sizeof(BsInfo), hipMemcpyDeviceToHost, s_));
+      QTG_HIP_TRY(hipEventRecord(evInfo_, s_));
+    }
+    if ((rc = E.apply_finish((const double*)X_, (double*)D_, d_mask_, ph_re_, ph_im_, nullptr, s_))) return rc;
+    if ((rc = boundary(0))) return rc;
+    if (!check) continue;
+    QTG_HIP_TRY(hipEventSynchronize(evInfo_));
+    if (h_info_->fail) {
+      set_error("BS integrator: step size underflow");
+      return QTG_ESTATE;
+    }
+    if (h_info_->live[rec] == 0) {
+      // every node's interval ended with the tick before
+      nticks_ += t;
+      nderiv_ = (size_t)h_info_->derivs;
+      return QTG_OK;
+    }
+    known = t + h_info_->lead[rec] - 1;
+  }
+}
+
 int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
+  if (deviceBs_) return cmfstep_dev(P, t0, t1, eps);
   for (int v = 0; v < NN_; ++v) {
     bs_[v].st = NodeBS::NEED_START;
     bs_[v].x = t0;
@@ -592,48 +664,127 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
   unsigned char* ds = (unsigned char*)dev_stage_;
   int half = 0;
   std::vector<NodeProg> ppre(NN_), ppost(NN_);
-  std::vector<VecOp> ops, pre, post;
+  std::vector<VecOp> ops, pre, post, bpre[2], bpost[2];
   std::vector<uint8_t> act(NN_);
+  // Nodes whose accept test reads an error still on its way to the host.
+  // The next tick is staged with both outcomes of each such node's test
+  // (the program kernel picks one by the same comparison and writes the
+  // node's Apply mask byte); the host commits the picked branch once the
+  // error has arrived, while that tick runs.
+  std::vector<uint8_t> pending(NN_, 0);
+  struct Branch {
+    NodeBS b;
+    double dt;
+    bool active, fail;
+  };
+  std::vector<Branch> brA(NN_), brR(NN_);
+  std::vector<NodeBS> saveBs;
+  std::vector<double> saveDt;
+  bool errInFlight = false;
+  // copies between arenas that are the same buffer (x = y1, d = y2)
+  auto self_copy = [&](const VecOp& o) { return o.code == V_COPY && bb.b[o.a] == bb.b[o.b]; };
+  auto tidy = [&](std::vector<VecOp>& o) {
+    if (fuseOps_) fuse_ops(o);
+    o.erase(std::remove_if(o.begin(), o.end(), self_copy), o.end());
+  };
+  auto has_error_op = [](const std::vector<VecOp>& o) {
+    return std::any_of(o.begin(), o.end(), [](const VecOp& x) { return x.code == V_ERROR; });
+  };
+  auto prog = [&](int v, const std::vector<VecOp>& a, const std::vector<VecOp>& r, int32_t flags) {
+    NodeProg g{v, (int)ops.size(), (int)(ops.size() + a.size()), flags, 0, 0, eps};
+    ops.insert(ops.end(), a.begin(), a.end());
+    g.alt0 = (int)ops.size();
+    g.alt1 = (int)(ops.size() + r.size());
+    ops.insert(ops.end(), r.begin(), r.end());
+    return g;
+  };
+  auto wait_errors = [&]() -> int {
+    QTG_HIP_TRY(hipEventSynchronize(evErr_));
+    errInFlight = false;
+    return QTG_OK;
+  };
   while (true) {
-    ops.clear();
-    int nact = 0, npre = 0, npost = 0;
-    bool need_err = false;
-    for (int v = 0; v < NN_; ++v) {
-      act[v] = 0;
-      ppre[v] = NodeProg{v, 0, 0, 0};
-      ppost[v] = NodeProg{v, 0, 0, 0};
-      if (bs_[v].st == NodeBS::DONE) continue;
-      pre.clear();
-      post.clear();
-      const bool d = advance(v, pre, post, eps);
-      if (failed_) {
-        set_error("BS integrator: step size underflow");
-        return QTG_ESTATE;
-      }
-      if (fuseOps_) {
-        fuse_ops(pre);
-        fuse_ops(post);
-      }
-      // copies between arenas that are the same buffer (x = y1, d = y2)
-      auto self_copy = [&](const VecOp& o) { return o.code == V_COPY && bb.b[o.a] == bb.b[o.b]; };
-      pre.erase(std::remove_if(pre.begin(), pre.end(), self_copy), pre.end());
-      post.erase(std::remove_if(post.begin(), post.end(), self_copy), post.end());
-      if (!pre.empty()) {
-        ppre[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + pre.size()), 0};
-        ops.insert(ops.end(), pre.begin(), pre.end());
-        ++npre;
-      }
-      if (!d) continue;
-      act[v] = 1;
-      ++nact;
-      ++nderiv_;
-      npost += post.empty() ? 0 : 1;
-      ppost[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + post.size()), 0};
-      for (auto& o : post)
-        if (o.code == V_ERROR) need_err = true;
-      ops.insert(ops.end(), post.begin(), post.end());
+    bool look = lookahead_ && errInFlight;
+    if (look) {
+      saveBs = bs_;
+      saveDt = dtbs_;
     }
-    if (nact == 0 && npre == 0) break;
+    int nact, npre, npost;
+    bool need_err, branchAct;
+    while (true) {
+      ops.clear();
+      nact = npre = npost = 0;
+      need_err = branchAct = false;
+      for (int v = 0; v < NN_; ++v) {
+        act[v] = 0;
+        ppre[v] = NodeProg{v, 0, 0, 0, 0, 0, eps};
+        ppost[v] = NodeProg{v, 0, 0, 0, 0, 0, eps};
+        if (bs_[v].st == NodeBS::DONE) continue;
+        if (look && pending[v]) {
+          const NodeBS b0 = bs_[v];
+          const double dt0 = dtbs_[v];
+          Branch br[2];
+          for (int k = 0; k < 2; ++k) {  // 0 accept, 1 reject
+            bpre[k].clear();
+            bpost[k].clear();
+            const bool d = advance(v, bpre[k], bpost[k], eps, k == 0 ? 1 : 0);
+            br[k] = Branch{bs_[v], dtbs_[v], d, failed_};
+            failed_ = false;
+            bs_[v] = b0;
+            dtbs_[v] = dt0;
+            tidy(bpre[k]);
+            tidy(bpost[k]);
+            if (has_error_op(bpost[k])) look = false;  // a test right after a test: not staged ahead
+          }
+          if (!look) break;
+          brA[v] = br[0];
+          brR[v] = br[1];
+          branchAct = branchAct || br[0].active;
+          const int32_t fl = kSelect | (br[0].active ? kActAccept : 0) | (br[1].active ? kActReject : 0);
+          ppre[v] = prog(v, bpre[0], bpre[1], fl | kMask);
+          if (!bpost[0].empty() || !bpost[1].empty()) {
+            ppost[v] = prog(v, bpost[0], bpost[1], fl);
+            ++npost;
+          }
+          ++npre;
+          continue;
+        }
+        pre.clear();
+        post.clear();
+        const bool d = advance(v, pre, post, eps, -1);
+        if (failed_) {
+          set_error("BS integrator: step size underflow");
+          return QTG_ESTATE;
+        }
+        tidy(pre);
+        tidy(post);
+        if (!pre.empty()) {
+          ppre[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + pre.size()), 0, 0, 0, eps};
+          ops.insert(ops.end(), pre.begin(), pre.end());
+          ++npre;
+        }
+        if (!d) continue;
+        act[v] = 1;
+        ++nact;
+        ++nderiv_;
+        npost += post.empty() ? 0 : 1;
+        ppost[v] = NodeProg{v, (int)ops.size(), (int)(ops.size() + post.size()), 0, 0, 0, eps};
+        if (has_error_op(post)) need_err = true;
+        ops.insert(ops.end(), post.begin(), post.end());
+      }
+      // Staged ahead only when some node derivative is sure to run or
+      // follows an accept; otherwise (every node's step ends in this tick)
+      // the errors are read first and the tick is staged as it will run.
+      if (look && (nact > 0 || branchAct)) break;
+      if (!look && !errInFlight) break;
+      bs_ = saveBs;
+      dtbs_ = saveDt;
+      nderiv_ -= nact;
+      look = false;
+      if (int rc = wait_errors()) return rc;
+      std::fill(pending.begin(), pending.end(), 0);
+    }
+    if (!look && nact == 0 && npre == 0) break;
     const size_t oend = L.ops + sizeof(VecOp) * ops.size();
     if (oend > stage_bytes_) {
       set_error("cmf: staging overflow");
@@ -652,20 +803,43 @@ int Cmf::cmfstep(cplx<double>* P, double t0, double t1, double eps) {
     QTG_HIP_TRY(hipMemcpyAsync(ds, hs, oend, hipMemcpyHostToDevice, s_));
     QTG_HIP_TRY(hipEventRecord(evStage_[half], s_));
     half ^= 1;
-    if (nact == 0) {
+    int rc;
+    if (!look && nact == 0) {
       // only trailing accept copies were left
-      int rc = launch_programs((const NodeProg*)(ds + L.pre), NN_, (const VecOp*)(ds + L.ops), bb, E.d_nodes_,
-                               (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
+      rc = launch_programs((const NodeProg*)(ds + L.pre), NN_, (const VecOp*)(ds + L.ops), bb, E.d_nodes_,
+                           (const cplx<double>*)E.d_pool_, rhoslot_, err_, s_);
       if (rc) return rc;
       break;
     }
     ++nticks_;
-    int rc = run_tick(P, npost > 0);
-    if (rc) return rc;
+    if ((rc = run_tick(P, npost > 0))) return rc;
+    if (look) {
+      // the errors of the previous tick: commit the branch each node took
+      if ((rc = wait_errors())) return rc;
+      for (int v = 0; v < NN_; ++v) {
+        if (!pending[v]) continue;
+        pending[v] = 0;
+        const Branch& br = (err_h_[v] < eps) ? brA[v] : brR[v];
+        if (br.fail) {
+          set_error("BS integrator: step size underflow");
+          return QTG_ESTATE;
+        }
+        bs_[v] = br.b;
+        dtbs_[v] = br.dt;
+        if (br.active) ++nderiv_;
+      }
+    }
     if (need_err) {
       // the next tick's decisions read the errors
       QTG_HIP_TRY(hipMemcpyAsync(err_h_, err_, sizeof(double) * NN_, hipMemcpyDeviceToHost, s_));
-      QTG_HIP_TRY(hipStreamSynchronize(s_));
+      if (lookahead_) {
+        QTG_HIP_TRY(hipEventRecord(evErr_, s_));
+        errInFlight = true;
+        for (int v = 0; v < NN_; ++v)
+          if (bs_[v].st == NodeBS::AWAIT_ERR) pending[v] = 1;
+      } else {
+        QTG_HIP_TRY(hipStreamSynchronize(s_));
+      }
     }
   }
   QTG_HIP_TRY(hipStreamSynchronize(s_));
@@ -816,6 +990,19 @@ int Cmf::integrate(double time, double timeend, double dt, double out, double ac
   dtbs_.assign(NN_, dt / 2.0);
   failed_ = false;
   nbuild_ = nderiv_ = nticks_ = 0;
+  if (deviceBs_) {
+    // every node's integrator idle with step size dt / 2; the record cleared
+    std::vector<BsNode> bn(NN_);
+    for (BsNode& b : bn) {
+      b = BsNode{};
+      b.st = BS_DONE;
+      b.dt = dt / 2.0;
+    }
+    QTG_HIP_TRY(hipMemcpyAsync(d_bs_, bn.data(), sizeof(BsNode) * NN_, hipMemcpyHostToDevice, s_));
+    QTG_HIP_TRY(hipMemsetAsync(d_info_, 0, sizeof(BsInfo), s_));
+    QTG_HIP_TRY(hipStreamSynchronize(s_));
+    slot_ = 0;
+  }
   double t_next = time + out;
   bool calc_mat = true;
   int rc;

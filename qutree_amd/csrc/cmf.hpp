@@ -60,6 +60,14 @@ class Cmf {
   // the op sequences a tick emits most run as fused ops (one pass over the
   // node each); false keeps one pass per op.  Results are the same.
   bool fuseOps_ = true;
+  // a tick after an error tick is staged with both outcomes of each waiting
+  // node's accept test and launched before the errors reach the host; false
+  // waits for the errors first.  Results are the same.
+  bool lookahead_ = true;
+  // every node's Bulirsch-Stoer state machine runs on the device, inside the
+  // launch that runs its vector programs; false runs them on the host
+  // (above).  Results are the same.
+  bool deviceBs_ = true;
 
  private:
   int eig_init();
@@ -70,9 +78,10 @@ class Cmf {
   CmfBufs bufs(cplx<double>* P) const;
   int copy(cplx<double>* dst, const cplx<double>* src);
   int build(const cplx<double>* P);
-  bool advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, double eps);
+  bool advance(int v, std::vector<VecOp>& pre, std::vector<VecOp>& post, double eps, int verdict);
   void finish_mmid(int v, std::vector<VecOp>& post);
   int cmfstep(cplx<double>* P, double t0, double t1, double eps);
+  int cmfstep_dev(cplx<double>* P, double t0, double t1, double eps);
   struct TickLayout {
     size_t pre, post, ops;  // byte offsets in the staging; the active mask at 0
   };
@@ -80,6 +89,8 @@ class Cmf {
   // post: launch the post programs (false when the tick has none)
   int tick_launches(const CmfBufs& bb, bool post);
   int run_tick(cplx<double>* P, bool post);
+  // a tick on this state buffer is launched eagerly (not from a graph)
+  bool tick_eager(const cplx<double>* P) const;
   void drop_tick_graphs();
   hipGraphExec_t tickExec_[2] = {nullptr, nullptr};  // ticks on P_ and on P2_
   double tickPh_[2] = {0.0, 0.0};                    // the phase they were captured with
@@ -103,6 +114,15 @@ class Cmf {
   void* dev_stage_ = nullptr;
   size_t stage_bytes_ = 0;
   hipEvent_t evStage_[2] = {nullptr, nullptr};  // the upload from each half has been read
+  hipEvent_t evErr_ = nullptr;                   // the node errors have reached err_h_
+  // the device integrators, their per-tick record (and its host copy), the
+  // Apply mask they write, the next record slot
+  BsNode* d_bs_ = nullptr;
+  BsInfo* d_info_ = nullptr;
+  BsInfo* h_info_ = nullptr;
+  uint8_t* d_mask_ = nullptr;
+  int slot_ = 0;
+  hipEvent_t evInfo_ = nullptr;
   cplx<double>* sw_ = nullptr;
   OrthoItem* d_oi_ = nullptr;
   SwApply* d_sa_ = nullptr;

@@ -173,262 +173,283 @@ struct PostExtrOp {  // y0 <- y1; y1 <- y1 + y2
   }
 };
 
-__global__ void __launch_bounds__(TB)
-    program_kernel(const NodeProg* __restrict__ progs, const VecOp* __restrict__ ops, CmfBufs bufs,
-                   const DNode* __restrict__ nodes, const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot,
-                   double* err) {
-  __shared__ double red[TB];
-  __shared__ cd2 rhos[16 * 16];
-  const NodeProg pg = progs[blockIdx.x];
-  if (pg.op0 >= pg.op1) return;  // an empty program (the whole workgroup)
-  const DNode nd = nodes[pg.node];
+// One vector op on one node (the whole workgroup; the caller synchronises
+// the workgroup before the next op).
+__device__ __forceinline__ void exec_op(const VecOp* opp, const CmfBufs& bufs, const DNode* __restrict__ nodes,
+                                        const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot,
+                                        double* err, int node, const DNode& nd, double* red, cd2* rhos) {
   const int N = nd.N;
   const int64_t off = nd.off;
-  for (int k = pg.op0; k < pg.op1; ++k) {
-    const VecOp op = ops[k];
-    switch (op.code) {
-      case V_COPY:
-        elementwise(N, CopyOp{bp(bufs, op.a) + off, bp(bufs, op.b) + off});
-        break;
-      case V_AXPY:
-        elementwise(N, AxpyOp{bp(bufs, op.a) + off, bp(bufs, op.b) + off, bp(bufs, op.c) + off, op.h});
-        break;
-      case V_ZERO: {
-        cd2* A = bp(bufs, op.a) + off;
-        for (int e = threadIdx.x; e < N; e += TB) A[e] = czero<double>();
+  const VecOp op = *opp;
+  switch (op.code) {
+    case V_COPY:
+      elementwise(N, CopyOp{bp(bufs, op.a) + off, bp(bufs, op.b) + off});
+      break;
+    case V_AXPY:
+      elementwise(N, AxpyOp{bp(bufs, op.a) + off, bp(bufs, op.b) + off, bp(bufs, op.c) + off, op.h});
+      break;
+    case V_ZERO: {
+      cd2* A = bp(bufs, op.a) + off;
+      for (int e = threadIdx.x; e < N; e += TB) A[e] = czero<double>();
+      break;
+    }
+    case V_MMID_STEP:
+      elementwise(N, MmidStepOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off, op.h});
+      break;
+    case V_MMID_FINAL:
+      elementwise(N, MmidFinalOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off, op.h});
+      break;
+    case V_RZEXTR: {
+      // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2
+      cd2* yest = bp(bufs, B_Y0) + off;
+      cd2* ysav = bp(bufs, B_Y0 + 1) + off;
+      cd2* dy = bp(bufs, B_Y0 + 2) + off;
+      cd2* yt0 = bp(bufs, B_YT0) + off;
+      constexpr int kTab = 7;  // ytab[0..6]
+      const int mi = op.mi < kTab ? op.mi : kTab;
+      double fx[kTab];
+#pragma unroll
+      for (int k2 = 1; k2 < kTab; ++k2) fx[k2] = opp->fx[k2];
+      cd2* yt[kTab];
+#pragma unroll
+      for (int k2 = 0; k2 < kTab; ++k2) yt[k2] = bp(bufs, B_YT0 + k2) + off;
+      if (op.i == 0) {
+        for (int e = threadIdx.x; e < N; e += TB) {
+          const cd2 v = yest[e];
+          ysav[e] = v;
+          dy[e] = v;
+          yt0[e] = v;
+        }
         break;
       }
-      case V_MMID_STEP:
-        elementwise(N, MmidStepOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off, op.h});
-        break;
-      case V_MMID_FINAL:
-        elementwise(N, MmidFinalOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off, op.h});
-        break;
-      case V_RZEXTR: {
-        // BS_integrator::rzextr with yest = y0, ysav = y1, dy = y2
-        cd2* yest = bp(bufs, B_Y0) + off;
-        cd2* ysav = bp(bufs, B_Y0 + 1) + off;
-        cd2* dy = bp(bufs, B_Y0 + 2) + off;
-        cd2* yt0 = bp(bufs, B_YT0) + off;
-        constexpr int kTab = 7;  // ytab[0..6]
-        const int mi = op.mi < kTab ? op.mi : kTab;
-        double fx[kTab];
+      // two elements per pass: both tableau columns are loaded before any
+      // store, and the two extrapolation chains are independent
+      for (int e0 = threadIdx.x; e0 < N; e0 += 2 * TB) {
+        const int e1 = e0 + TB;
+        const bool has1 = e1 < N;
+        const int ee[2] = {e0, has1 ? e1 : e0};
+        cd2 col[2][kTab], yy[2], v[2];
 #pragma unroll
-        for (int k2 = 1; k2 < kTab; ++k2) fx[k2] = ops[k].fx[k2];
-        cd2* yt[kTab];
+        for (int h = 0; h < 2; ++h) {
 #pragma unroll
-        for (int k2 = 0; k2 < kTab; ++k2) yt[k2] = bp(bufs, B_YT0 + k2) + off;
-        if (op.i == 0) {
-          for (int e = threadIdx.x; e < N; e += TB) {
-            const cd2 v = yest[e];
-            ysav[e] = v;
-            dy[e] = v;
-            yt0[e] = v;
-          }
-          break;
+          for (int k2 = 1; k2 < kTab; ++k2) col[h][k2] = yt[k2 < mi ? k2 : 0][ee[h]];
+          yy[h] = yest[ee[h]];
+          v[h] = yt0[ee[h]];
         }
-        // two elements per pass: both tableau columns are loaded before any
-        // store, and the two extrapolation chains are independent
-        for (int e0 = threadIdx.x; e0 < N; e0 += 2 * TB) {
-          const int e1 = e0 + TB;
-          const bool has1 = e1 < N;
-          const int ee[2] = {e0, has1 ? e1 : e0};
-          cd2 col[2][kTab], yy[2], v[2];
+        const cd2 yst[2] = {yy[0], yy[1]};
+        cd2 ddy[2] = {czero<double>(), czero<double>()}, c[2] = {yy[0], yy[1]};
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
+        for (int k2 = 1; k2 < kTab; ++k2) {
+          if (k2 < mi) {
 #pragma unroll
-            for (int k2 = 1; k2 < kTab; ++k2) col[h][k2] = yt[k2 < mi ? k2 : 0][ee[h]];
-            yy[h] = yest[ee[h]];
-            v[h] = yt0[ee[h]];
+            for (int h = 0; h < 2; ++h) {
+              const cd2 bi = mk<double>(fx[k2] * v[h].re, fx[k2] * v[h].im);
+              cd2 b = mk<double>(bi.re - c[h].re, bi.im - c[h].im);
+              if (b.re != 0.0 || b.im != 0.0) {
+                b = cdiv(mk<double>(c[h].re - v[h].re, c[h].im - v[h].im), b);
+                ddy[h] = cmul(c[h], b);
+                c[h] = cmul(bi, b);
+              } else {
+                ddy[h] = v[h];
+              }
+              v[h] = col[h][k2];
+              col[h][k2] = ddy[h];
+              yy[h] = mk<double>(yy[h].re + ddy[h].re, yy[h].im + ddy[h].im);
+            }
           }
-          const cd2 yst[2] = {yy[0], yy[1]};
-          cd2 ddy[2] = {czero<double>(), czero<double>()}, c[2] = {yy[0], yy[1]};
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (h == 0 || has1) {
+            const int e = ee[h];
+            yt0[e] = yst[h];
+#pragma unroll
+            for (int k2 = 1; k2 < kTab; ++k2)
+              if (k2 < mi) yt[k2][e] = col[h][k2];
+            dy[e] = ddy[h];
+            ysav[e] = yy[h];
+          }
+        }
+      }
+      break;
+    }
+    case V_POST_EXTR:
+      elementwise(N, PostExtrOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off});
+      break;
+    case V_START: {
+      StartOp so{bp(bufs, op.a) + off, bp(bufs, B_P) + off, bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 3) + off,
+                  bp(bufs, B_X) + off, {}, op.b != 0};
+#pragma unroll
+      for (int k2 = 0; k2 < 6; ++k2) so.yt[k2] = bp(bufs, B_YT0 + 1 + k2) + off;
+      elementwise(N, so);
+      break;
+    }
+    case V_MMID_BEGIN:
+      elementwise(N, MmidBeginOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_X) + off,
+                                 bp(bufs, B_Y0 + 3) + off, bp(bufs, B_Y0 + 4) + off, op.h});
+      break;
+    case V_MMID_STEP_X:
+      elementwise(N, MmidStepXOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off,
+                                 bp(bufs, B_X) + off, op.h});
+      break;
+    case V_FINAL_EXTR: {
+      // y2 <- d; y0 <- (y0 + y1 + h y2) / 2 (the midpoint estimate yest);
+      // rzextr at step i; then y0 <- ysav, y1 <- ysav + dy
+      cd2* y0 = bp(bufs, B_Y0) + off;
+      cd2* y1 = bp(bufs, B_Y0 + 1) + off;
+      cd2* y2 = bp(bufs, B_Y0 + 2) + off;
+      const cd2* dd = bp(bufs, B_D) + off;
+      constexpr int kTab = 7;  // ytab[0..6]
+      const int mi = op.mi < kTab ? op.mi : kTab;
+      const double h = op.h;
+      double fx[kTab];
+#pragma unroll
+      for (int k2 = 1; k2 < kTab; ++k2) fx[k2] = opp->fx[k2];
+      cd2* yt[kTab];
+#pragma unroll
+      for (int k2 = 0; k2 < kTab; ++k2) yt[k2] = bp(bufs, B_YT0 + k2) + off;
+      for (int e = threadIdx.x; e < N; e += TB) {
+        const cd2 d = dd[e], a = y0[e], b = y1[e];
+        cd2 col[kTab], v = czero<double>();
+        if (op.i > 0) {
+#pragma unroll
+          for (int k2 = 1; k2 < kTab; ++k2) col[k2] = yt[k2 < mi ? k2 : 0][e];
+          v = yt[0][e];
+        }
+        const cd2 yest = mk<double>((a.re + b.re + h * d.re) / 2.0, (a.im + b.im + h * d.im) / 2.0);
+        cd2 ysav = yest, dy = yest;
+        if (op.i > 0) {
+          cd2 ddy = czero<double>(), c = yest, yy = yest;
 #pragma unroll
           for (int k2 = 1; k2 < kTab; ++k2) {
             if (k2 < mi) {
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const cd2 bi = mk<double>(fx[k2] * v[h].re, fx[k2] * v[h].im);
-                cd2 b = mk<double>(bi.re - c[h].re, bi.im - c[h].im);
-                if (b.re != 0.0 || b.im != 0.0) {
-                  b = cdiv(mk<double>(c[h].re - v[h].re, c[h].im - v[h].im), b);
-                  ddy[h] = cmul(c[h], b);
-                  c[h] = cmul(bi, b);
-                } else {
-                  ddy[h] = v[h];
-                }
-                v[h] = col[h][k2];
-                col[h][k2] = ddy[h];
-                yy[h] = mk<double>(yy[h].re + ddy[h].re, yy[h].im + ddy[h].im);
+              const cd2 bi = mk<double>(fx[k2] * v.re, fx[k2] * v.im);
+              cd2 bb = mk<double>(bi.re - c.re, bi.im - c.im);
+              if (bb.re != 0.0 || bb.im != 0.0) {
+                bb = cdiv(mk<double>(c.re - v.re, c.im - v.im), bb);
+                ddy = cmul(c, bb);
+                c = cmul(bi, bb);
+              } else {
+                ddy = v;
               }
+              v = col[k2];
+              col[k2] = ddy;
+              yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
             }
           }
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            if (h == 0 || has1) {
-              const int e = ee[h];
-              yt0[e] = yst[h];
-#pragma unroll
-              for (int k2 = 1; k2 < kTab; ++k2)
-                if (k2 < mi) yt[k2][e] = col[h][k2];
-              dy[e] = ddy[h];
-              ysav[e] = yy[h];
-            }
-          }
+          for (int k2 = 1; k2 < kTab; ++k2)
+            if (k2 < mi) yt[k2][e] = col[k2];
+          dy = ddy;
+          ysav = yy;
         }
-        break;
+        yt[0][e] = yest;
+        y2[e] = dy;
+        y0[e] = ysav;
+        y1[e] = mk<double>(ysav.re + dy.re, ysav.im + dy.im);
       }
-      case V_POST_EXTR:
-        elementwise(N, PostExtrOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off});
-        break;
-      case V_START: {
-        StartOp so{bp(bufs, op.a) + off, bp(bufs, B_P) + off, bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 3) + off,
-                    bp(bufs, B_X) + off, {}, op.b != 0};
-#pragma unroll
-        for (int k2 = 0; k2 < 6; ++k2) so.yt[k2] = bp(bufs, B_YT0 + 1 + k2) + off;
-        elementwise(N, so);
-        break;
-      }
-      case V_MMID_BEGIN:
-        elementwise(N, MmidBeginOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_X) + off,
-                                   bp(bufs, B_Y0 + 3) + off, bp(bufs, B_Y0 + 4) + off, op.h});
-        break;
-      case V_MMID_STEP_X:
-        elementwise(N, MmidStepXOp{bp(bufs, B_Y0) + off, bp(bufs, B_Y0 + 1) + off, bp(bufs, B_Y0 + 2) + off,
-                                   bp(bufs, B_X) + off, op.h});
-        break;
-      case V_FINAL_EXTR: {
-        // y2 <- d; y0 <- (y0 + y1 + h y2) / 2 (the midpoint estimate yest);
-        // rzextr at step i; then y0 <- ysav, y1 <- ysav + dy
-        cd2* y0 = bp(bufs, B_Y0) + off;
-        cd2* y1 = bp(bufs, B_Y0 + 1) + off;
-        cd2* y2 = bp(bufs, B_Y0 + 2) + off;
-        const cd2* dd = bp(bufs, B_D) + off;
-        constexpr int kTab = 7;  // ytab[0..6]
-        const int mi = op.mi < kTab ? op.mi : kTab;
-        const double h = op.h;
-        double fx[kTab];
-#pragma unroll
-        for (int k2 = 1; k2 < kTab; ++k2) fx[k2] = ops[k].fx[k2];
-        cd2* yt[kTab];
-#pragma unroll
-        for (int k2 = 0; k2 < kTab; ++k2) yt[k2] = bp(bufs, B_YT0 + k2) + off;
-        for (int e = threadIdx.x; e < N; e += TB) {
-          const cd2 d = dd[e], a = y0[e], b = y1[e];
-          cd2 col[kTab], v = czero<double>();
-          if (op.i > 0) {
-#pragma unroll
-            for (int k2 = 1; k2 < kTab; ++k2) col[k2] = yt[k2 < mi ? k2 : 0][e];
-            v = yt[0][e];
-          }
-          const cd2 yest = mk<double>((a.re + b.re + h * d.re) / 2.0, (a.im + b.im + h * d.im) / 2.0);
-          cd2 ysav = yest, dy = yest;
-          if (op.i > 0) {
-            cd2 ddy = czero<double>(), c = yest, yy = yest;
-#pragma unroll
-            for (int k2 = 1; k2 < kTab; ++k2) {
-              if (k2 < mi) {
-                const cd2 bi = mk<double>(fx[k2] * v.re, fx[k2] * v.im);
-                cd2 bb = mk<double>(bi.re - c.re, bi.im - c.im);
-                if (bb.re != 0.0 || bb.im != 0.0) {
-                  bb = cdiv(mk<double>(c.re - v.re, c.im - v.im), bb);
-                  ddy = cmul(c, bb);
-                  c = cmul(bi, bb);
-                } else {
-                  ddy = v;
-                }
-                v = col[k2];
-                col[k2] = ddy;
-                yy = mk<double>(yy.re + ddy.re, yy.im + ddy.im);
-              }
-            }
-#pragma unroll
-            for (int k2 = 1; k2 < kTab; ++k2)
-              if (k2 < mi) yt[k2][e] = col[k2];
-            dy = ddy;
-            ysav = yy;
-          }
-          yt[0][e] = yest;
-          y2[e] = dy;
-          y0[e] = ysav;
-          y1[e] = mk<double>(ysav.re + dy.re, ysav.im + dy.im);
-        }
-        break;
-      }
-      case V_ERROR: {
-        const cd2* Phi = bp(bufs, op.a) + off;
-        const cd2* Chi = bp(bufs, op.b) + off;
-        const int n = nodes[pg.node].dims[nd.order - 1], L = N / n;
-        const int64_t rs = rhoslot[pg.node];
-        double delta = 0.0, nrm = 0.0;
-        if (rs < 0) {
-          for (int e = threadIdx.x; e < N; e += TB) {
-            const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
-            delta += dr * dr + di * di;
-            nrm += Phi[e].re * Phi[e].re + Phi[e].im * Phi[e].im;
-          }
-          delta = block_sum_d(delta, red);
-          nrm = block_sum_d(nrm, red);
-        } else {
-          const cd2* rho = pool + rs;
-          // Delta = sum_{i,n,m} Re( conj(d(i,n)) rho(n,m) d(i,m) )
-          if (n <= 16) {
-            // a thread per row i: d(i, :) in registers, rho in LDS
-            for (int o = threadIdx.x; o < n * n; o += TB) rhos[o] = rho[o];
-            __syncthreads();
-            for (int i = threadIdx.x; i < L; i += TB) {
-              cd2 d[16];
-#pragma unroll
-              for (int m = 0; m < 16; ++m) {
-                const int mm = m < n ? m : 0;
-                const cd2 p = Phi[i + mm * L], q = Chi[i + mm * L];
-                d[m] = mk<double>(p.re - q.re, p.im - q.im);
-              }
-#pragma unroll
-              for (int nn = 0; nn < 16; ++nn) {
-                if (nn < n) {
-                  double tr = 0.0, ti = 0.0;
-#pragma unroll
-                  for (int m = 0; m < 16; ++m) {
-                    if (m < n) {
-                      const cd2 r = rhos[nn + m * n];
-                      tr += r.re * d[m].re - r.im * d[m].im;
-                      ti += r.re * d[m].im + r.im * d[m].re;
-                    }
-                  }
-                  delta += d[nn].re * tr + d[nn].im * ti;
-                }
-              }
-            }
-          } else {
-            for (int e = threadIdx.x; e < N; e += TB) {
-              const int i = e % L, nn = e / L;
-              const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
-              double tr = 0.0, ti = 0.0;
-              for (int m = 0; m < n; ++m) {
-                const cd2 r = rho[nn + m * n];
-                const double er = Phi[i + m * L].re - Chi[i + m * L].re;
-                const double ei = Phi[i + m * L].im - Chi[i + m * L].im;
-                tr += r.re * er - r.im * ei;
-                ti += r.re * ei + r.im * er;
-              }
-              delta += dr * tr + di * ti;
-            }
-          }
-          delta = block_sum_d(delta, red);
-          double t = 0.0, tim = 0.0;
-          for (int m = 0; m < n; ++m) {
-            t += rho[m + m * n].re;
-            tim += rho[m + m * n].im;
-          }
-          nrm = sqrt(t * t + tim * tim);
-        }
-        if (threadIdx.x == 0) err[pg.node] = sqrt(fabs(delta / nrm));
-        break;
-      }
-      default:
-        break;
+      break;
     }
+    case V_ERROR: {
+      const cd2* Phi = bp(bufs, op.a) + off;
+      const cd2* Chi = bp(bufs, op.b) + off;
+      const int n = nodes[node].dims[nd.order - 1], L = N / n;
+      const int64_t rs = rhoslot[node];
+      double delta = 0.0, nrm = 0.0;
+      if (rs < 0) {
+        for (int e = threadIdx.x; e < N; e += TB) {
+          const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
+          delta += dr * dr + di * di;
+          nrm += Phi[e].re * Phi[e].re + Phi[e].im * Phi[e].im;
+        }
+        delta = block_sum_d(delta, red);
+        nrm = block_sum_d(nrm, red);
+      } else {
+        const cd2* rho = pool + rs;
+        // Delta = sum_{i,n,m} Re( conj(d(i,n)) rho(n,m) d(i,m) )
+        if (n <= 16) {
+          // a thread per row i: d(i, :) in registers, rho in LDS
+          for (int o = threadIdx.x; o < n * n; o += TB) rhos[o] = rho[o];
+          __syncthreads();
+          for (int i = threadIdx.x; i < L; i += TB) {
+            cd2 d[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+              const int mm = m < n ? m : 0;
+              const cd2 p = Phi[i + mm * L], q = Chi[i + mm * L];
+              d[m] = mk<double>(p.re - q.re, p.im - q.im);
+            }
+#pragma unroll
+            for (int nn = 0; nn < 16; ++nn) {
+              if (nn < n) {
+                double tr = 0.0, ti = 0.0;
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                  if (m < n) {
+                    const cd2 r = rhos[nn + m * n];
+                    tr += r.re * d[m].re - r.im * d[m].im;
+                    ti += r.re * d[m].im + r.im * d[m].re;
+                  }
+                }
+                delta += d[nn].re * tr + d[nn].im * ti;
+              }
+            }
+          }
+        } else {
+          for (int e = threadIdx.x; e < N; e += TB) {
+            const int i = e % L, nn = e / L;
+            const double dr = Phi[e].re - Chi[e].re, di = Phi[e].im - Chi[e].im;
+            double tr = 0.0, ti = 0.0;
+            for (int m = 0; m < n; ++m) {
+              const cd2 r = rho[nn + m * n];
+              const double er = Phi[i + m * L].re - Chi[i + m * L].re;
+              const double ei = Phi[i + m * L].im - Chi[i + m * L].im;
+              tr += r.re * er - r.im * ei;
+              ti += r.re * ei + r.im * er;
+            }
+            delta += dr * tr + di * ti;
+          }
+        }
+        delta = block_sum_d(delta, red);
+        double t = 0.0, tim = 0.0;
+        for (int m = 0; m < n; ++m) {
+          t += rho[m + m * n].re;
+          tim += rho[m + m * n].im;
+        }
+        nrm = sqrt(t * t + tim * tim);
+      }
+      if (threadIdx.x == 0) err[node] = sqrt(fabs(delta / nrm));
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__global__ void __launch_bounds__(TB)
+    program_kernel(const NodeProg* __restrict__ progs, const VecOp* __restrict__ ops, CmfBufs bufs,
+                   const DNode* __restrict__ nodes, const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot,
+                   double* err, uint8_t* mask) {
+  __shared__ double red[TB];
+  __shared__ cd2 rhos[16 * 16];
+  const NodeProg pg = progs[blockIdx.x];
+  int op0 = pg.op0, op1 = pg.op1;
+  bool accept = true;
+  if (pg.flags & kSelect) {
+    // the accept test of the Bulirsch-Stoer step on the error that the
+    // previous tick wrote (the same comparison the host makes)
+    accept = err[pg.node] < pg.eps;
+    if (!accept) {
+      op0 = pg.alt0;
+      op1 = pg.alt1;
+    }
+  }
+  if ((pg.flags & kMask) && threadIdx.x == 0)
+    mask[pg.node] = (pg.flags & (accept ? kActAccept : kActReject)) ? 1 : 0;
+  if (op0 >= op1) return;  // an empty program (the whole workgroup)
+  const DNode nd = nodes[pg.node];
+  for (int k = op0; k < op1; ++k) {
+    exec_op(&ops[k], bufs, nodes, pool, rhoslot, err, pg.node, nd, red, rhos);
     __syncthreads();
   }
 }
@@ -673,12 +694,205 @@ __global__ void __launch_bounds__(TB)
   for (int e = threadIdx.x; e < N; e += TB) P[nd.off + e] = cur[e];
 }
 
+// ------------------------------------------------- device BS integrators
+namespace {
+__device__ inline void bs_op(VecOp* o, int code, int a = 0, int b = 0, int c = 0, double h = 0.0) {
+  o->code = code;
+  o->a = a;
+  o->b = b;
+  o->c = c;
+  o->i = o->mi = o->pad0 = o->pad1 = 0;
+  o->h = h;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) o->fx[k] = 0.0;
+}
+
+// The scalar state of one node's integrator, in registers.
+struct BsRegs {
+  int st, i, step, nsub;
+  double x, xend, h, hh, dt;
+};
+
+// The host integrator's advance() for one node (BS_integrator::Integrate,
+// bsstep and mmid, BS_integrator.h:113-256), emitting the fused op forms
+// the host's peephole makes of the same sequences, without copies between
+// aliased buffers.  Returns whether the node takes a derivative in the
+// tick; pre: the tick's one pre op (npre 0 or 1); post: the ops after the
+// derivative (npost); xtab: the node's extrapolation abscissae.  lead:
+// ticks from this one on in which the node surely takes a derivative.
+__device__ bool bs_advance(BsRegs& b, double* xtab, double e, double eps, const CmfBufs& bufs, VecOp* pre,
+                           int& npre, VecOp* post, int& npost, bool& fail, int& lead) {
+#pragma clang fp contract(off)
+  npre = npost = 0;
+  lead = 0;
+  bool acceptCopy = false;  // P <- y0 before the next start
+  while (true) {
+    if (b.st == BS_AWAIT_ERR) {
+      if (e < eps) {
+        b.x += b.h;
+        if (b.i == kBsMaxUse - 1) b.dt = b.h * kBsShrink;
+        else if (b.i == kBsMaxUse - 2) b.dt = b.h * kBsGrow;
+        else b.dt = (b.h * bs_seq(kBsMaxUse - 2)) / (1.0 * bs_seq(b.i));
+        acceptCopy = true;
+        b.st = BS_NEED_START;
+      } else {
+        ++b.i;
+        if (b.i >= kBsMax) {
+          b.h = b.h / 4.0;
+          for (int k = 0; k < (kBsMax - kBsMaxUse) / 2.0; ++k) b.h /= 2.0;
+          if (b.dt + b.h == b.dt) {
+            fail = true;
+            b.st = BS_DONE;
+            return false;
+          }
+          b.i = 0;
+        }
+        b.st = BS_START_MMID;
+      }
+      continue;
+    }
+    if (b.st == BS_NEED_START) {
+      if (!(b.xend - b.x > b.dt * 1e-6)) {
+        b.st = BS_DONE;
+        if (acceptCopy) {
+          bs_op(pre, V_COPY, B_P, B_Y0);
+          npre = 1;
+        }
+        return false;
+      }
+      if (b.x + b.dt > b.xend) b.dt = b.xend - b.x + 1e-10;
+      for (int k = 0; k < kBsMax; ++k) xtab[k] = 0.0;
+      // y0 = y3 = x = y (after P <- y0 on an accept), ytab[1..6] = 0
+      bs_op(pre, V_START, acceptCopy ? B_Y0 : B_P, acceptCopy ? 1 : 0);
+      npre = 1;
+      if (bufs.b[B_Y0 + 4] != bufs.b[B_D]) bs_op(post + npost++, V_COPY, B_Y0 + 4, B_D);
+      b.h = b.dt;
+      b.i = 0;
+      b.st = BS_START_MMID;
+      lead = 1 + bs_seq(0);
+      return true;
+    }
+    if (b.st != BS_START_MMID && b.st != BS_IN_MMID) return false;
+    if (b.st == BS_START_MMID) {
+      b.step = bs_seq(b.i);
+      b.hh = b.h / (1.0 * b.step);
+      bs_op(pre, V_MMID_BEGIN, 0, 0, 0, b.hh);
+      b.nsub = 1;
+      b.st = BS_IN_MMID;
+    } else {
+      bs_op(pre, V_MMID_STEP_X, 0, 0, 0, b.hh);
+      ++b.nsub;
+    }
+    npre = 1;
+    if (b.nsub == b.step) {
+      // the midpoint's final step, rzextr and the error (the copy of the
+      // derivative into y2 is part of the fused op)
+      const double hs = b.h / (1.0 * b.step);
+      const double xest = hs * hs;
+      xtab[b.i] = xest;
+      VecOp* f = post + npost++;
+      bs_op(f, V_FINAL_EXTR, 0, 0, 0, b.hh);
+      f->i = b.i;
+      f->mi = (b.i < kBsMaxUse) ? b.i + 1 : kBsMaxUse;
+      for (int k = 1; k < f->mi; ++k) f->fx[k] = xtab[b.i - k] / xest;
+      bs_op(post + npost++, V_ERROR, B_Y0, B_Y0 + 1);
+      b.st = BS_AWAIT_ERR;
+      lead = 1;
+    } else {
+      if (bufs.b[B_Y0 + 2] != bufs.b[B_D]) bs_op(post + npost++, V_COPY, B_Y0 + 2, B_D);
+      lead = 1 + b.step - b.nsub;
+    }
+    return true;
+  }
+}
+}  // namespace
+
+__global__ void __launch_bounds__(TB)
+    bs_tick_kernel(BsNode* __restrict__ st, CmfBufs bufs, const DNode* __restrict__ nodes,
+                   const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot, double* err, uint8_t* mask,
+                   BsInfo* info, int slot, int init, double t0, double t1, double eps) {
+  __shared__ double red[TB];
+  __shared__ cd2 rhos[16 * 16];
+  __shared__ VecOp pre;
+  __shared__ int npre;
+  const int v = blockIdx.x;
+  BsNode* b = st + v;
+  const DNode nd = nodes[v];
+  // the post ops of the tick that ran, then (k == np) the decisions and the
+  // next tick's pre op: one call site of the op code.  The count is read
+  // once for the workgroup before thread 0 can rewrite it.
+  __shared__ int snp;
+  if (threadIdx.x == 0) snp = init ? 0 : b->npost;
+  __syncthreads();
+  const int np = snp;
+  for (int k = 0; k <= np; ++k) {
+    const VecOp* opp = &pre;
+    if (k < np) {
+      opp = &b->post[k];
+    } else {
+      if (threadIdx.x == 0) {
+        BsRegs r{b->st, b->i, b->step, b->nsub, b->x, b->xend, b->h, b->hh, b->dt};
+        if (init) {
+          r.st = BS_NEED_START;
+          r.x = t0;
+          r.xend = t1;
+        }
+        // the node error the post ops just wrote (this thread wrote it)
+        const double e = (r.st == BS_AWAIT_ERR) ? err[v] : 0.0;
+        int nq = 0, lead = 0, n1 = 0;
+        bool fail = false;
+        const bool active = bs_advance(r, b->xtab, e, eps, bufs, &pre, n1, b->post, nq, fail, lead);
+        b->st = r.st;
+        b->i = r.i;
+        b->step = r.step;
+        b->nsub = r.nsub;
+        b->x = r.x;
+        b->xend = r.xend;
+        b->h = r.h;
+        b->hh = r.hh;
+        b->dt = r.dt;
+        b->npost = nq;
+        npre = n1;
+        mask[v] = active ? 1 : 0;
+        if (active) {
+          atomicAdd(&info->live[slot], 1);
+          atomicMax(&info->lead[slot], lead);
+          atomicAdd(&info->derivs, 1ull);
+        }
+        if (fail) atomicOr(&info->fail, 1);
+        if (v == 0) {
+          info->live[(slot + 1) % kBsRing] = 0;
+          info->lead[(slot + 1) % kBsRing] = 0;
+        }
+      }
+      __syncthreads();
+      if (!npre) break;
+    }
+    exec_op(opp, bufs, nodes, pool, rhoslot, err, v, nd, red, rhos);
+    __syncthreads();
+  }
+}
+
+int launch_bs_tick(BsNode* st, int64_t count, const CmfBufs& bufs, const DNode* nodes, const cplx<double>* pool,
+                   const int64_t* rhoslot, double* err, uint8_t* mask, BsInfo* info, int slot, int init,
+                   double t0, double t1, double eps, hipStream_t s) {
+  if (count == 0) return QTG_OK;
+  if (slot < 0 || slot >= kBsRing) {
+    set_error("bs_tick: ring slot out of range");
+    return QTG_EARG;
+  }
+  hipLaunchKernelGGL(bs_tick_kernel, dim3((unsigned)count), dim3(TB), 0, s, st, bufs, nodes, pool, rhoslot, err,
+                     mask, info, slot, init, t0, t1, eps);
+  QTG_LAUNCH_CHECK();
+  return QTG_OK;
+}
+
 int launch_programs(const NodeProg* progs, int64_t count, const VecOp* ops, const CmfBufs& bufs,
                     const DNode* nodes, const cplx<double>* pool, const int64_t* rhoslot, double* err,
-                    hipStream_t s) {
+                    hipStream_t s, uint8_t* mask) {
   if (count == 0) return QTG_OK;
   hipLaunchKernelGGL(program_kernel, dim3((unsigned)count), dim3(TB), 0, s, progs, ops, bufs, nodes, pool, rhoslot,
-                     err);
+                     err, mask);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }

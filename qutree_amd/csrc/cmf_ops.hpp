@@ -45,9 +45,16 @@ struct VecOp {
   double fx[7];
 };
 
+// A node's program: ops [op0, op1).  With kSelect in flags the node waits
+// for the error of its last extrapolation: ops [op0, op1) when err[node] <
+// eps (accept), else [alt0, alt1) (reject), and a pre program with kMask
+// writes the node's byte of the Apply mask (kActAccept / kActReject).
 struct NodeProg {
-  int32_t node, op0, op1, pad;
+  int32_t node, op0, op1, flags;
+  int32_t alt0, alt1;
+  double eps;
 };
+enum NodeProgFlags : int32_t { kSelect = 1, kActAccept = 2, kActReject = 4, kMask = 8 };
 
 struct OrthoItem {
   int32_t node, pad;
@@ -60,9 +67,54 @@ struct SwApply {
   int64_t fmat[4];
 };
 
+// Bulirsch-Stoer constants of BS_integrator (BS_integrator.h:34-37: maxuse_,
+// max_, shrink_, grow_, and its step sequence), shared by the host and the
+// device integrators.
+constexpr int kBsMaxUse = 7, kBsMax = 11;
+constexpr double kBsShrink = 0.95, kBsGrow = 1.2;
+__host__ __device__ inline int bs_seq(int i) {
+  constexpr int s[kBsMax] = {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96};
+  return s[i];
+}
+enum BsState : int32_t { BS_NEED_START = 0, BS_START_MMID, BS_IN_MMID, BS_AWAIT_ERR, BS_DONE };
+
+// One node's Bulirsch-Stoer integrator on the device: the state machine of
+// the host integrator, its step size, and the post ops its last derivative
+// evaluation left for the next tick boundary.
+struct BsNode {
+  int32_t st, i, step, nsub;
+  double x, xend, h, hh, dt;
+  double xtab[kBsMax];
+  int32_t npost, pad;
+  VecOp post[2];
+};
+// What the device integrators report per tick, in a ring of four ticks: the
+// nodes that take a derivative in the tick, how many ticks from that one on
+// surely have such a node, a sticky failure flag (step size underflow) and
+// the derivative evaluations since the record was cleared.
+constexpr int kBsRing = 4;
+struct BsInfo {
+  int32_t live[kBsRing];
+  int32_t lead[kBsRing];
+  int32_t fail, pad;
+  unsigned long long derivs;
+};
+
+// One tick boundary of every node's integrator (one workgroup per node): the
+// post ops of the tick that ran (none when init), the accept test on the
+// node error they computed, the next tick's pre op, the node's Apply mask
+// byte and the node's share of info->live/lead[slot].  init starts every
+// node's interval [t0, t1] (the step size carries over).  Also clears ring
+// entry slot + 1 for the next boundary.
+int launch_bs_tick(BsNode* st, int64_t count, const CmfBufs& bufs, const DNode* nodes, const cplx<double>* pool,
+                   const int64_t* rhoslot, double* err, uint8_t* mask, BsInfo* info, int slot, int init,
+                   double t0, double t1, double eps, hipStream_t s);
+
+// mask: the Apply mask that programs with kSelect | kMask write (may be null
+// when no program has kMask)
 int launch_programs(const NodeProg* progs, int64_t count, const VecOp* ops, const CmfBufs& bufs,
                     const DNode* nodes, const cplx<double>* pool, const int64_t* rhoslot, double* err,
-                    hipStream_t s);
+                    hipStream_t s, uint8_t* mask = nullptr);
 int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P, cplx<double>* sw,
                  int nmax, int nspf, hipStream_t s);
 // The same for node tensors of any size: Phi in place in global memory.

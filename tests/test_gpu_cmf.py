@@ -171,14 +171,19 @@ def test_cmf8_global_memory_orthogonal_matches_oracle():
     assert np.max(np.abs(om.node_tensor(got["psi"], top) - om.node_tensor(ref["psi"], top))) < 1e-11
 
 
-@pytest.mark.parametrize("opts", [{"graph": 1}, {"fuse": 0}, {"graph": 1, "fuse": 0}])
+@pytest.mark.parametrize("opts", [{"device_bs": 0}, {"device_bs": 0, "lookahead": 0}, {"device_bs": 0, "graph": 1},
+                                  {"device_bs": 0, "fuse": 0}, {"device_bs": 0, "graph": 1, "fuse": 0}])
 @pytest.mark.parametrize("model", ["cmf8", "cmf64"])
 def test_cmf_tick_forms_equal_default(model, opts):
-    """Ticks replayed from HIP graphs, and per-op vector programs instead of
-    the fused op sequences, run the same arithmetic per element as the
-    default (eager ticks, fused ops): energies, autocorrelations, statistics
-    and the final state agree bit for bit, also across a second integrate
-    call that continues from the first one's state."""
+    """The default ticks run every node's Bulirsch-Stoer state machine on the
+    device (one launch per tick boundary: post ops, accept test, next pre
+    op).  The host state machine in each of its forms (ticks staged ahead
+    with both outcomes of each accept test, or staged after the errors have
+    reached the host; eager ticks or ticks replayed from HIP graphs; fused
+    op sequences or one pass per op) runs the same arithmetic per element:
+    energies, autocorrelations, statistics and the final state agree bit
+    for bit, also across a second integrate call that continues from the
+    first one's state."""
     text, H = getattr(W, model + "_model")()
     print(f"case cmf tick forms {model} {opts}", flush=True)
     runs = []
