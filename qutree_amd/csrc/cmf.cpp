@@ -35,8 +35,8 @@ Cmf::~Cmf() {
     if (p) (void)hipFree(p);
   if (host_stage_) (void)hipHostFree(host_stage_);
   if (err_h_) (void)hipHostFree(err_h_);
-  if (h_info_) (void)hipHostFree(h_info_);
-  if (evInfo_) (void)hipEventDestroy(evInfo_);
+  if (h_rec_) (void)hipHostFree(h_rec_);
+  if (evRec_) (void)hipEventDestroy(evRec_);
   for (hipEvent_t e : evStage_)
     if (e) (void)hipEventDestroy(e);
   if (evErr_) (void)hipEventDestroy(evErr_);
@@ -99,10 +99,9 @@ int Cmf::init() {
   QTG_HIP_TRY(hipEventCreateWithFlags(&evErr_, hipEventDisableTiming));
   // the device integrators
   if ((rc = dalloc((void**)&d_bs_, sizeof(BsNode) * NN_))) return rc;
-  if ((rc = dalloc((void**)&d_info_, sizeof(BsInfo)))) return rc;
-  if ((rc = dalloc((void**)&d_mask_, NN_))) return rc;
-  QTG_HIP_TRY(hipHostMalloc((void**)&h_info_, sizeof(BsInfo)));
-  QTG_HIP_TRY(hipEventCreateWithFlags(&evInfo_, hipEventDisableTiming));
+  if ((rc = dalloc((void**)&d_rec_, (size_t)kBsRing * 2 * NN_))) return rc;
+  QTG_HIP_TRY(hipHostMalloc((void**)&h_rec_, 2 * (size_t)NN_));
+  QTG_HIP_TRY(hipEventCreateWithFlags(&evRec_, hipEventDisableTiming));
   // orthogonal(): per height, the nodes to orthonormalise and the parents
   // receiving their square roots (children in node-address order)
   int64_t swoff = 0;
@@ -613,12 +612,13 @@ int Cmf::cmfstep_dev(cplx<double>* P, double t0, double t1, double eps) {
   }
   const CmfBufs bb = bufs(P);
   const cplx<double>* pool = (const cplx<double>*)E.d_pool_;
+  const size_t recB = 2 * (size_t)NN_;
   int slot = 0;  // the record of the next tick
   auto boundary = [&](int init) {
     slot = slot_;
     slot_ = (slot_ + 1) % kBsRing;
-    return launch_bs_tick(d_bs_, NN_, bb, E.d_nodes_, pool, rhoslot_, err_, d_mask_, d_info_, slot, init, t0, t1,
-                          eps, s_);
+    return launch_bs_tick(d_bs_, NN_, bb, E.d_nodes_, pool, rhoslot_, err_, d_rec_ + slot * recB, init, t0, t1, eps,
+                          s_);
   };
   int rc = boundary(1);
   if (rc) return rc;
@@ -629,26 +629,32 @@ int Cmf::cmfstep_dev(cplx<double>* P, double t0, double t1, double eps) {
       return QTG_ESTATE;
     }
     const bool check = t > known;
-    const int rec = slot;
+    const uint8_t* rec = d_rec_ + slot * recB;
     if (check) {
-      QTG_HIP_TRY(hipMemcpyAsync(h_info_, d_info_, sizeof(BsInfo), hipMemcpyDeviceToHost, s_));
-      QTG_HIP_TRY(hipEventRecord(evInfo_, s_));
+      // the tick's record (a copy on a side stream measured slower)
+      QTG_HIP_TRY(hipMemcpyAsync(h_rec_, rec, recB, hipMemcpyDeviceToHost, s_));
+      QTG_HIP_TRY(hipEventRecord(evRec_, s_));
     }
-    if ((rc = E.apply_finish((const double*)X_, (double*)D_, d_mask_, ph_re_, ph_im_, nullptr, s_))) return rc;
+    if ((rc = E.apply_finish((const double*)X_, (double*)D_, rec, ph_re_, ph_im_, nullptr, s_))) return rc;
     if ((rc = boundary(0))) return rc;
     if (!check) continue;
-    QTG_HIP_TRY(hipEventSynchronize(evInfo_));
-    if (h_info_->fail) {
-      set_error("BS integrator: step size underflow");
-      return QTG_ESTATE;
+    QTG_HIP_TRY(hipEventSynchronize(evRec_));
+    int live = 0, lead = 0;
+    for (int v = 0; v < NN_; ++v) {
+      const uint8_t l = h_rec_[NN_ + v];
+      if (l == kBsFailed) {
+        set_error("BS integrator: step size underflow");
+        return QTG_ESTATE;
+      }
+      live += h_rec_[v];
+      lead = std::max(lead, (int)l);
     }
-    if (h_info_->live[rec] == 0) {
+    if (live == 0) {
       // every node's interval ended with the tick before
       nticks_ += t;
-      nderiv_ = (size_t)h_info_->derivs;
       return QTG_OK;
     }
-    known = t + h_info_->lead[rec] - 1;
+    known = t + lead - 1;
   }
 }
 
@@ -991,7 +997,7 @@ int Cmf::integrate(double time, double timeend, double dt, double out, double ac
   failed_ = false;
   nbuild_ = nderiv_ = nticks_ = 0;
   if (deviceBs_) {
-    // every node's integrator idle with step size dt / 2; the record cleared
+    // every node's integrator idle with step size dt / 2
     std::vector<BsNode> bn(NN_);
     for (BsNode& b : bn) {
       b = BsNode{};
@@ -999,9 +1005,7 @@ int Cmf::integrate(double time, double timeend, double dt, double out, double ac
       b.dt = dt / 2.0;
     }
     QTG_HIP_TRY(hipMemcpyAsync(d_bs_, bn.data(), sizeof(BsNode) * NN_, hipMemcpyHostToDevice, s_));
-    QTG_HIP_TRY(hipMemsetAsync(d_info_, 0, sizeof(BsInfo), s_));
     QTG_HIP_TRY(hipStreamSynchronize(s_));
-    slot_ = 0;
   }
   double t_next = time + out;
   bool calc_mat = true;
@@ -1051,6 +1055,13 @@ int Cmf::integrate(double time, double timeend, double dt, double out, double ac
     }
   }
   QTG_HIP_TRY(hipStreamSynchronize(s_));
+  if (deviceBs_) {
+    // the derivative evaluations the device integrators counted
+    std::vector<BsNode> bn(NN_);
+    QTG_HIP_TRY(hipMemcpy(bn.data(), d_bs_, sizeof(BsNode) * NN_, hipMemcpyDeviceToHost));
+    nderiv_ = 0;
+    for (const BsNode& b : bn) nderiv_ += b.nderiv;
+  }
   res.builds = nbuild_;
   res.derivs = nderiv_;
   res.ticks = nticks_;

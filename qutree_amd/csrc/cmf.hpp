@@ -115,14 +115,13 @@ class Cmf {
   size_t stage_bytes_ = 0;
   hipEvent_t evStage_[2] = {nullptr, nullptr};  // the upload from each half has been read
   hipEvent_t evErr_ = nullptr;                   // the node errors have reached err_h_
-  // the device integrators, their per-tick record (and its host copy), the
-  // Apply mask they write, the next record slot
+  // the device integrators, the ring of their per-tick records, the host
+  // copy of one record and its event
   BsNode* d_bs_ = nullptr;
-  BsInfo* d_info_ = nullptr;
-  BsInfo* h_info_ = nullptr;
-  uint8_t* d_mask_ = nullptr;
-  int slot_ = 0;
-  hipEvent_t evInfo_ = nullptr;
+  uint8_t* d_rec_ = nullptr;
+  uint8_t* h_rec_ = nullptr;
+  hipEvent_t evRec_ = nullptr;
+  int slot_ = 0;  // the record entry the next tick boundary writes
   cplx<double>* sw_ = nullptr;
   OrthoItem* d_oi_ = nullptr;
   SwApply* d_sa_ = nullptr;

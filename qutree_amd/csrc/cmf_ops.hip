@@ -3,6 +3,8 @@
 // the rho-weighted node error (LayerInterface::Error) and orthogonal()
 // (TensorTree_Implementation.h:143-171).  One workgroup per node.
 #include "cmf_ops.hpp"
+
+#include <cstddef>
 #include "jacobi.hpp"
 #include "node_tile.hpp"
 
@@ -718,10 +720,11 @@ struct BsRegs {
 // the host's peephole makes of the same sequences, without copies between
 // aliased buffers.  Returns whether the node takes a derivative in the
 // tick; pre: the tick's one pre op (npre 0 or 1); post: the ops after the
-// derivative (npost); xtab: the node's extrapolation abscissae.  lead:
+// derivative (npost); xtab: the node's extrapolation abscissae (xin: a copy
+// of them as the tick found them, which is what is read).  lead:
 // ticks from this one on in which the node surely takes a derivative.
-__device__ bool bs_advance(BsRegs& b, double* xtab, double e, double eps, const CmfBufs& bufs, VecOp* pre,
-                           int& npre, VecOp* post, int& npost, bool& fail, int& lead) {
+__device__ bool bs_advance(BsRegs& b, const double* xin, double* xtab, double e, double eps, const CmfBufs& bufs,
+                           VecOp* pre, int& npre, VecOp* post, int& npost, bool& fail, int& lead) {
 #pragma clang fp contract(off)
   npre = npost = 0;
   lead = 0;
@@ -794,7 +797,7 @@ __device__ bool bs_advance(BsRegs& b, double* xtab, double e, double eps, const 
       bs_op(f, V_FINAL_EXTR, 0, 0, 0, b.hh);
       f->i = b.i;
       f->mi = (b.i < kBsMaxUse) ? b.i + 1 : kBsMaxUse;
-      for (int k = 1; k < f->mi; ++k) f->fx[k] = xtab[b.i - k] / xest;
+      for (int k = 1; k < f->mi; ++k) f->fx[k] = xin[b.i - k] / xest;
       bs_op(post + npost++, V_ERROR, B_Y0, B_Y0 + 1);
       b.st = BS_AWAIT_ERR;
       lead = 1;
@@ -809,29 +812,31 @@ __device__ bool bs_advance(BsRegs& b, double* xtab, double e, double eps, const 
 
 __global__ void __launch_bounds__(TB)
     bs_tick_kernel(BsNode* __restrict__ st, CmfBufs bufs, const DNode* __restrict__ nodes,
-                   const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot, double* err, uint8_t* mask,
-                   BsInfo* info, int slot, int init, double t0, double t1, double eps) {
+                   const cd2* __restrict__ pool, const int64_t* __restrict__ rhoslot, double* err, uint8_t* rec,
+                   int init, double t0, double t1, double eps) {
   __shared__ double red[TB];
   __shared__ cd2 rhos[16 * 16];
   __shared__ VecOp pre;
   __shared__ int npre;
+  // the node's state up to its post ops, read once for the workgroup (one
+  // round trip, before thread 0 can rewrite it)
+  __shared__ double head[kBsHeadWords];
   const int v = blockIdx.x;
   BsNode* b = st + v;
   const DNode nd = nodes[v];
-  // the post ops of the tick that ran, then (k == np) the decisions and the
-  // next tick's pre op: one call site of the op code.  The count is read
-  // once for the workgroup before thread 0 can rewrite it.
-  __shared__ int snp;
-  if (threadIdx.x == 0) snp = init ? 0 : b->npost;
+  if (threadIdx.x < kBsHeadWords) head[threadIdx.x] = reinterpret_cast<const double*>(b)[threadIdx.x];
   __syncthreads();
-  const int np = snp;
+  const BsNode& hd = *reinterpret_cast<const BsNode*>(head);
+  const int np = init ? 0 : hd.npost;
+  // the post ops of the tick that ran, then (k == np) the decisions and the
+  // next tick's pre op: one call site of the op code
   for (int k = 0; k <= np; ++k) {
     const VecOp* opp = &pre;
     if (k < np) {
       opp = &b->post[k];
     } else {
       if (threadIdx.x == 0) {
-        BsRegs r{b->st, b->i, b->step, b->nsub, b->x, b->xend, b->h, b->hh, b->dt};
+        BsRegs r{hd.st, hd.i, hd.step, hd.nsub, hd.x, hd.xend, hd.h, hd.hh, hd.dt};
         if (init) {
           r.st = BS_NEED_START;
           r.x = t0;
@@ -841,7 +846,7 @@ __global__ void __launch_bounds__(TB)
         const double e = (r.st == BS_AWAIT_ERR) ? err[v] : 0.0;
         int nq = 0, lead = 0, n1 = 0;
         bool fail = false;
-        const bool active = bs_advance(r, b->xtab, e, eps, bufs, &pre, n1, b->post, nq, fail, lead);
+        const bool active = bs_advance(r, hd.xtab, b->xtab, e, eps, bufs, &pre, n1, b->post, nq, fail, lead);
         b->st = r.st;
         b->i = r.i;
         b->step = r.step;
@@ -852,18 +857,12 @@ __global__ void __launch_bounds__(TB)
         b->hh = r.hh;
         b->dt = r.dt;
         b->npost = nq;
+        const bool failed = fail || hd.fail;
+        if (fail) b->fail = 1;
+        if (active) b->nderiv = hd.nderiv + 1;
         npre = n1;
-        mask[v] = active ? 1 : 0;
-        if (active) {
-          atomicAdd(&info->live[slot], 1);
-          atomicMax(&info->lead[slot], lead);
-          atomicAdd(&info->derivs, 1ull);
-        }
-        if (fail) atomicOr(&info->fail, 1);
-        if (v == 0) {
-          info->live[(slot + 1) % kBsRing] = 0;
-          info->lead[(slot + 1) % kBsRing] = 0;
-        }
+        rec[v] = active ? 1 : 0;
+        rec[gridDim.x + v] = failed ? kBsFailed : (uint8_t)(lead < 254 ? lead : 254);
       }
       __syncthreads();
       if (!npre) break;
@@ -874,15 +873,12 @@ __global__ void __launch_bounds__(TB)
 }
 
 int launch_bs_tick(BsNode* st, int64_t count, const CmfBufs& bufs, const DNode* nodes, const cplx<double>* pool,
-                   const int64_t* rhoslot, double* err, uint8_t* mask, BsInfo* info, int slot, int init,
-                   double t0, double t1, double eps, hipStream_t s) {
+                   const int64_t* rhoslot, double* err, uint8_t* rec, int init, double t0, double t1, double eps,
+                   hipStream_t s) {
+  static_assert(offsetof(BsNode, post) == 8 * kBsHeadWords, "BsNode head");
   if (count == 0) return QTG_OK;
-  if (slot < 0 || slot >= kBsRing) {
-    set_error("bs_tick: ring slot out of range");
-    return QTG_EARG;
-  }
-  hipLaunchKernelGGL(bs_tick_kernel, dim3((unsigned)count), dim3(TB), 0, s, st, bufs, nodes, pool, rhoslot, err,
-                     mask, info, slot, init, t0, t1, eps);
+  hipLaunchKernelGGL(bs_tick_kernel, dim3((unsigned)count), dim3(TB), 0, s, st, bufs, nodes, pool, rhoslot, err, rec,
+                     init, t0, t1, eps);
   QTG_LAUNCH_CHECK();
   return QTG_OK;
 }

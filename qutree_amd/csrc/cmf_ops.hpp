@@ -79,36 +79,33 @@ __host__ __device__ inline int bs_seq(int i) {
 enum BsState : int32_t { BS_NEED_START = 0, BS_START_MMID, BS_IN_MMID, BS_AWAIT_ERR, BS_DONE };
 
 // One node's Bulirsch-Stoer integrator on the device: the state machine of
-// the host integrator, its step size, and the post ops its last derivative
-// evaluation left for the next tick boundary.
+// the host integrator, its step size, the post ops its last derivative
+// evaluation left for the next tick boundary, a failure flag (step size
+// underflow; the node then stays done) and its derivative evaluations.
 struct BsNode {
   int32_t st, i, step, nsub;
   double x, xend, h, hh, dt;
   double xtab[kBsMax];
-  int32_t npost, pad;
+  int32_t npost, fail;
+  unsigned long long nderiv;
   VecOp post[2];
 };
-// What the device integrators report per tick, in a ring of four ticks: the
-// nodes that take a derivative in the tick, how many ticks from that one on
-// surely have such a node, a sticky failure flag (step size underflow) and
-// the derivative evaluations since the record was cleared.
+constexpr int kBsHeadWords = 20;  // 8-byte words of BsNode before post
+// The per-tick record of the device integrators, a ring of kBsRing ticks of
+// 2 * nodes bytes: the tick's Apply mask (1: the node takes a derivative),
+// then per node the ticks from that one on in which it surely takes one
+// (kBsFailed: its step size underflowed).
 constexpr int kBsRing = 4;
-struct BsInfo {
-  int32_t live[kBsRing];
-  int32_t lead[kBsRing];
-  int32_t fail, pad;
-  unsigned long long derivs;
-};
+constexpr uint8_t kBsFailed = 255;
 
 // One tick boundary of every node's integrator (one workgroup per node): the
 // post ops of the tick that ran (none when init), the accept test on the
-// node error they computed, the next tick's pre op, the node's Apply mask
-// byte and the node's share of info->live/lead[slot].  init starts every
-// node's interval [t0, t1] (the step size carries over).  Also clears ring
-// entry slot + 1 for the next boundary.
+// node error they computed, the next tick's pre op and the node's bytes of
+// record entry rec (2 * count bytes).  init starts every node's interval
+// [t0, t1] (the step size carries over).
 int launch_bs_tick(BsNode* st, int64_t count, const CmfBufs& bufs, const DNode* nodes, const cplx<double>* pool,
-                   const int64_t* rhoslot, double* err, uint8_t* mask, BsInfo* info, int slot, int init,
-                   double t0, double t1, double eps, hipStream_t s);
+                   const int64_t* rhoslot, double* err, uint8_t* rec, int init, double t0, double t1, double eps,
+                   hipStream_t s);
 
 // mask: the Apply mask that programs with kSelect | kMask write (may be null
 // when no program has kMask)
