@@ -366,8 +366,13 @@ int qtg_cmf_set_output_file(qtg_cmf cmf, const char* filename, int append);
  * launched before those errors reach the host, with both outcomes of each
  * waiting node's accept test staged (the device picks one by the same
  * comparison and sets the node's Apply mask); 0 waits for the errors before
- * staging the next tick.  Results are the same either way for all three
- * options. */
+ * staging the next tick.  "device_bs" (default 1): every node's
+ * Bulirsch-Stoer state machine runs on the device, one launch per tick
+ * boundary (post ops, accept test, next pre op, Apply mask), and the host
+ * reads a small per-tick record only where the ticks known to have a
+ * derivative run out; 0 runs the state machines on the host, which the
+ * three options above then shape.  Results are the same either way for all
+ * four options. */
 int qtg_cmf_set_option(qtg_cmf cmf, const char* key, double value);
 int qtg_cmf_destroy(qtg_cmf cmf);
 int64_t qtg_cmf_integrate(qtg_cmf cmf, double* psi_host, double t0, double tend, double dt, double out,

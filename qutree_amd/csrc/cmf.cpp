@@ -1,14 +1,17 @@
 // Constant-mean-field integrator on the device (CMFIntegrator.cpp:25-199 and
 // BS_integrator.h:31-256 of QuTree).
 //
-// Host: step-size control of CMF and of every node's Bulirsch-Stoer
-// integrator in FP64, exactly as the reference decides them.  Device: all
-// tensor arithmetic.  The nodes' BS integrators run in lockstep "ticks": each
-// tick every node that is still integrating takes exactly one derivative
-// evaluation (LayerDerivative with the matrices of the last build), preceded
-// and followed by its own vector program (midpoint updates, rational
-// extrapolation, error norm).  A tick is four launches (program, Apply,
-// finish, program) and one read of the per-node errors.
+// Host: the CMF step-size control in FP64, exactly as the reference decides
+// it.  Device: all tensor arithmetic and, by default, every node's
+// Bulirsch-Stoer state machine (bs_tick_kernel).  The nodes' BS integrators
+// run in lockstep "ticks": each tick every node that is still integrating
+// takes exactly one derivative evaluation (LayerDerivative with the matrices
+// of the last build), preceded and followed by its own vector ops (midpoint
+// updates, rational extrapolation, error norm).  A tick is the masked Apply
+// and one tick-boundary launch; the host reads a small per-tick record only
+// where the ticks known to have a derivative run out.  The host state
+// machine (advance, cmfstep) stays as an option: per-node programs staged in
+// pinned memory and uploaded per tick.
 #include "cmf.hpp"
 #include "jacobi.hpp"
 
