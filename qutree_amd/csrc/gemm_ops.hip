@@ -1135,6 +1135,10 @@ __global__ void __launch_bounds__(kSolveThreads)
     }
     return;
   }
+  // R is needed only after the solve: its load overlaps the Gram products
+  const bool rpf = it.r >= 0 && nn <= kSolveThreads;
+  cplx<double> rpre = czero<double>();
+  if (rpf && (int)threadIdx.x < nn) rpre = pool[it.r + threadIdx.x];
   if (gram) {
     // S = A^H A (wavefronts 0-3) and X = A^H Y (4-7), each one 16 x 16 tile
     // summed over four K chunks of the rows, partials added in chunk order
@@ -1182,15 +1186,16 @@ __global__ void __launch_bounds__(kSolveThreads)
       const cplx<double> v = lds[o];
       e = fmax(e, fabs((i == j ? 1.0 : 0.0) - v.re) + fabs(v.im));
     }
+    // the maximum over the wavefront, then over the 8 wavefronts (the
+    // partials are read before the barrier that precedes any write of part)
+    for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_xor(e, off, 64));
     double* red = reinterpret_cast<double*>(part);
-    red[threadIdx.x] = e;
+    if (lane == 0) red[wave_id()] = e;
     __syncthreads();
-    for (int h = kSolveThreads / 2; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + h]);
-      __syncthreads();
-    }
-    const double rho = n * red[0];
-    __syncthreads();
+    double em = red[0];
+#pragma unroll
+    for (int w = 1; w < kSolveThreads / 64; ++w) em = fmax(em, red[w]);
+    const double rho = n * em;
     if (rho < 0.5) {
       nsk = 0;
       // rho^(2^k) <= 1e-17
@@ -1200,7 +1205,8 @@ __global__ void __launch_bounds__(kSolveThreads)
   if (nsk >= 0) {
     const cplx<double>* S = lds;
     const cplx<double>* X0 = lds + nn;
-    cplx<double>* Z = lds + 2 * nn;  // the second [S | X] buffer holds Z and S Z
+    // Z alternates between the second [S | X] buffer and the partials; T = S Z
+    cplx<double>* Z = lds + 2 * nn;
     cplx<double>* T = lds + 3 * nn;
     cplx<double>* Zn = part;
     for (int o = threadIdx.x; o < nn; o += kSolveThreads) Z[o] = mk<double>(o % n == o / n ? 1.0 : 0.0, 0.0);
@@ -1221,8 +1227,9 @@ __global__ void __launch_bounds__(kSolveThreads)
         Zn[o] = mk<double>(2.0 * z.re - acc.re, 2.0 * z.im - acc.im);
       }
       __syncthreads();
-      for (int o = threadIdx.x; o < nn; o += kSolveThreads) Z[o] = Zn[o];
-      __syncthreads();
+      cplx<double>* t = Z;
+      Z = Zn;
+      Zn = t;
     }
     // X <- Z X into the second buffer's X half
     for (int o = threadIdx.x; o < nn; o += kSolveThreads) {
@@ -1287,7 +1294,11 @@ __global__ void __launch_bounds__(kSolveThreads)
   // beside it for the derivative tiles
   cplx<double>* Rm = lds + (cur ^ 1) * 2 * nn;
   cplx<double>* Wm = Rm + nn;
-  for (int o = threadIdx.x; o < nn; o += kSolveThreads) Rm[o] = pool[it.r + o];
+  if (rpf) {
+    if ((int)threadIdx.x < nn) Rm[threadIdx.x] = rpre;
+  } else {
+    for (int o = threadIdx.x; o < nn; o += kSolveThreads) Rm[o] = pool[it.r + o];
+  }
   __syncthreads();
   for (int o = threadIdx.x; o < nn; o += kSolveThreads) {
     const int k = o % n, m = o / n;
