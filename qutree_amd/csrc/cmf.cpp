@@ -153,7 +153,7 @@ int Cmf::init() {
   // be checked where the LDS kernel also runs)
   const char* force = std::getenv("QTG_ORTHO_GLOBAL");
   bigOrtho_ = (force && force[0] == '1') ||
-              (size_t)(E.nmax_ + 2 * jacobi_elems(E.nspf_)) * sizeof(cplx<double>) > kLdsBudget;
+              (size_t)(E.nmax_ + 2 * jacobi_elems(E.nspf_) + kOrthoNsElems) * sizeof(cplx<double>) > kLdsBudget;
   swLaunch_.clear();
   if (bigOrtho_) {
     std::vector<GmDesc> all;

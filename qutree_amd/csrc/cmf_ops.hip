@@ -456,6 +456,70 @@ __global__ void __launch_bounds__(TB)
   }
 }
 
+// The transposed square root SW(j,k) = S^(1/2)(k,j) of a near-identity
+// S = Phi^H Phi (n <= 16, rho = n max|S - I| < 1/2: the node tensors after
+// a CMF step are orthonormal up to the integrator's error) by the coupled
+// Newton-Schulz iteration T = (3I - Z Y) / 2, Y <- Y T, Z <- T Z from
+// Y = S, Z = I (Y -> S^(1/2), Z -> S^(-1/2), quadratically: the error is
+// below 1e-17 after the steps chosen).  It is the principal square root the
+// eigendecomposition gives, to rounding.  One wavefront; returns false (and
+// writes nothing) when S is not near enough the identity.
+__device__ bool ortho_sqrt_ns(const cd2* A, int ld, int n, cd2* sw) {
+  __shared__ cd2 nsb[5][256];
+  static_assert(sizeof(nsb) == kOrthoNsElems * sizeof(cd2), "Newton-Schulz buffers");
+  if (n > 16) return false;
+  const int nn = n * n;
+  double e = 0.0;
+  for (int o = threadIdx.x; o < nn; o += EW) {
+    const int i = o % n, j = o / n;
+    const cd2 v = A[i + j * ld];
+    e = fmax(e, fabs((i == j ? 1.0 : 0.0) - v.re) + fabs(v.im));
+  }
+  for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_xor(e, off, EW));
+  const double rho = n * e;
+  if (!(rho < 0.5)) return false;
+  int nsk = 1;  // one step beyond the estimate
+  for (double r = rho; r > 1e-17 && nsk < 9; r *= r) ++nsk;
+  cd2 *Y = nsb[0], *Z = nsb[1], *T = nsb[2], *Yn = nsb[3], *Zn = nsb[4];
+  for (int o = threadIdx.x; o < nn; o += EW) {
+    const int i = o % n, j = o / n;
+    Y[o] = A[i + j * ld];
+    Z[o] = mk<double>(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int k = 0; k < nsk; ++k) {
+    for (int o = threadIdx.x; o < nn; o += EW) {
+      const int i = o % n, j = o / n;
+      cd2 acc = czero<double>();
+      for (int l = 0; l < n; ++l) cfma(acc, Z[i + l * n], Y[l + j * n]);
+      T[o] = mk<double>(0.5 * ((i == j ? 3.0 : 0.0) - acc.re), -0.5 * acc.im);
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < nn; o += EW) {
+      const int i = o % n, j = o / n;
+      cd2 y = czero<double>(), z = czero<double>();
+      for (int l = 0; l < n; ++l) {
+        cfma(y, Y[i + l * n], T[l + j * n]);
+        cfma(z, T[i + l * n], Z[l + j * n]);
+      }
+      Yn[o] = y;
+      Zn[o] = z;
+    }
+    __syncthreads();
+    cd2* t = Y;
+    Y = Yn;
+    Yn = t;
+    t = Z;
+    Z = Zn;
+    Zn = t;
+  }
+  for (int o = threadIdx.x; o < nn; o += EW) {
+    const int j = o % n, k = o / n;
+    sw[j + k * n] = Y[k + j * n];
+  }
+  return true;
+}
+
 // ------------------------------------------------------------- orthogonal()
 // S = Phi^H Phi, SW(j,k) = sum_l U(k,l) sqrt(max(l_l,0)) conj(U(j,l)) (the
 // transposed square root), then iterated Gram-Schmidt of Phi in place
@@ -484,6 +548,7 @@ __global__ void __launch_bounds__(EW)
       A[j + k * ld] = acc;
     }
     __syncthreads();
+    if (ortho_sqrt_ns(A, ld, n, sw + it.sw)) goto gram_schmidt;
     jacobi_eig(A, U, n, ld, js);
     for (int i = threadIdx.x; i < n; i += EW) lam[i] = sqrt(fmax(A[i + i * ld].re, 0.0));
     __syncthreads();
@@ -499,6 +564,7 @@ __global__ void __launch_bounds__(EW)
       sw[it.sw + j + k * n] = acc;
     }
   }
+gram_schmidt:
   // Gram-Schmidt: lane owns rows i = lane + 64 t, so every reduction is a
   // wavefront shuffle sum and no other lane touches its elements.
   for (int k = 0; k < n; ++k) {

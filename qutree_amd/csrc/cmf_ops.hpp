@@ -112,6 +112,9 @@ int launch_bs_tick(BsNode* st, int64_t count, const CmfBufs& bufs, const DNode* 
 int launch_programs(const NodeProg* progs, int64_t count, const VecOp* ops, const CmfBufs& bufs,
                     const DNode* nodes, const cplx<double>* pool, const int64_t* rhoslot, double* err,
                     hipStream_t s, uint8_t* mask = nullptr);
+// complex elements of ortho_kernel's static Newton-Schulz buffers (five
+// 16 x 16 matrices), to be counted with its dynamic LDS against the budget
+constexpr int kOrthoNsElems = 5 * 256;
 int launch_ortho(const OrthoItem* items, int64_t count, const DNode* nodes, cplx<double>* P, cplx<double>* sw,
                  int nmax, int nspf, hipStream_t s);
 // The same for node tensors of any size: Phi in place in global memory.
