@@ -202,3 +202,22 @@ def test_cmf_tick_forms_equal_default(model, opts):
         assert np.array_equal(x["psi"], y["psi"])
         for k in ("steps", "rejected", "builds", "derivs", "ticks"):
             assert x[k] == y[k], k
+
+
+@pytest.mark.parametrize("device_bs", [1, 0])
+def test_cmf_step_size_underflow_is_an_error(device_bs):
+    """A Bulirsch-Stoer tolerance that no node error meets (0) makes every
+    node reject until its step size underflows (dx + h == dx,
+    BS_integrator.h:153-164, where the reference prints the message and
+    exits the process).  The device integrators and the host state machine
+    both end the call with that error; the condition is set from the host
+    and every kernel exits as usual (about 14 rejection cycles of 312
+    ticks)."""
+    text, H = W.cmf8_model()
+    print(f"case cmf step size underflow device_bs={device_bs}", flush=True)
+    eng = Q.Engine(text, H)
+    psi = eng.fill_random(8, delta_lowest=True)
+    cmf = Q.CMF(eng)
+    cmf.set_option("device_bs", device_bs)
+    with pytest.raises(Q.QtgError, match="step size underflow"):
+        cmf.integrate(psi, 0.0, 0.1, 0.1, 0.1, acc_cmf=1e-4, acc_bs=0.0)
